@@ -1,0 +1,172 @@
+/*
+ * tgp.h -- C ABI of the MI355X multilevel k-way graph partitioner.
+ *
+ * The reference (iamrupamroyy/TheGraphPartitioning, src/gpu_part.cu) exposes
+ * no library API: its only interface is the process contract
+ *     gpu_part <dir> <nvtxs> <nedges> <k>     (gpu_part.cu:878-888)
+ * reading <dir>/row.txt, column.txt, weight.txt (gpu_part.cu:905-908, 943-945)
+ * and writing ./output.graph, ./output.graph.part.<k> (via gpmetis) and
+ * ./finalpartition.txt (gpu_part.cu:829-858, 1163, 1296-1321).
+ * The `gpu_part` CLI built from this library keeps that contract; the entry
+ * points below are what the CLI, the tests and any FFI caller bind.  Each one
+ * names the part of gpu_part.cu it replaces.
+ *
+ * Plain C types only: pointers and sizes, no torch or C++ types.  The caller
+ * owns every buffer passed in.  Errors are negative return codes; the library
+ * never calls exit().
+ */
+#ifndef TGP_H
+#define TGP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------- */
+enum {
+  TGP_OK = 0,
+  TGP_EINVAL = -1,      /* bad argument or malformed CSR (offsets not monotone,
+                           columns outside [0,n), xadj[n] != nedges)            */
+  TGP_ELIMIT = -2,      /* a reference int32 limit is exceeded in int32 mode:
+                           n*k >= 2^31 (gpu_part.cu:531), n or m > INT_MAX     */
+  TGP_EEDGES = -3,      /* "Invalid new_coarsed_edge" (gpu_part.cu:1121-1125)  */
+  TGP_EINITPART = -4,   /* initial partitioner failed (gpu_part.cu:857-863)    */
+  TGP_EPARTFILE = -5,   /* part file missing (gpu_part.cu:1163-1168)           */
+  TGP_EHIP = -6,        /* HIP runtime error (replaces CUDA_CHECK, gpu_part.cu:40-47) */
+  TGP_ENOMEM = -7,      /* device or host allocation failed                    */
+  TGP_EIO = -8          /* file I/O error (gpu_part.cu:786-800, 830-833, 1306) */
+};
+
+/* ---- graphs ---------------------------------------------------------------- */
+/* CSR as the reference reads it (row.txt / column.txt / weight.txt,
+ * gpu_part.cu:786-800, 943-955), with 64-bit offsets so one type serves both
+ * the reference's int32 limits and graphs past 2^31 arcs.                    */
+typedef struct tgp_graph {
+  int64_t n;              /* vertices                                           */
+  int64_t m;              /* arcs (the reference's nedges); xadj[n] == m       */
+  const int64_t* xadj;    /* n+1 row offsets                                   */
+  const int32_t* adjncy;  /* m column ids in [0,n)                             */
+  const int32_t* adjwgt;  /* m edge weights                                    */
+} tgp_graph;
+
+/* Initial partitioner of the coarsest graph.  Replaces the fixed seam
+ * convertCSRtoMETISandRun + part-file read (gpu_part.cu:803-864, 1157-1174).
+ * `coarsest` is host memory; `npairs` is the unique undirected pair count the
+ * reference writes in the METIS header; `part` (coarsest->n entries) is
+ * written by the callee.  Out-of-range values are mapped to 0 afterwards, as
+ * gpu_part.cu:1172 does.  Return 0 on success.                               */
+typedef int (*tgp_initpart_fn)(const tgp_graph* coarsest, int64_t npairs, int32_t k,
+                               int32_t* part, void* user);
+
+/* The reference seam itself: writes ./output.graph in the current directory
+ * (byte-identical to gpu_part.cu:829-853), runs
+ * `gpmetis -ptype=rb ./output.graph <k>` through /bin/sh and reads
+ * ./output.graph.part.<k> with the reference's rules (gpu_part.cu:1163-1174).
+ * `user` may point to an int that receives system()'s status.                */
+int tgp_initpart_gpmetis(const tgp_graph* coarsest, int64_t npairs, int32_t k,
+                         int32_t* part, void* user);
+/* Deterministic stand-in for hosts without METIS: part[i] = i*k/n_c.  No
+ * counterpart in the reference (which always calls gpmetis).                 */
+int tgp_initpart_block(const tgp_graph* coarsest, int64_t npairs, int32_t k,
+                       int32_t* part, void* user);
+
+typedef struct tgp_options {
+  int32_t k;               /* parts (the reference's num_partitions)           */
+  int32_t wide;            /* 0: reference int32 semantics (cut wraps in int32,
+                              n*k < 2^31); 1: 64-bit cut, no n*k limit          */
+  int64_t coarsen_floor;   /* 0: max(500, n/500) (gpu_part.cu:897); else the
+                              vertex count below which coarsening stops         */
+  tgp_initpart_fn initpart;/* NULL: tgp_initpart_gpmetis (the reference seam)  */
+  void* initpart_user;
+  int32_t device_input;    /* 1: xadj/adjncy/adjwgt are device pointers         */
+  int32_t profile;         /* 1: time kernels with HIP events into tgp_stats    */
+  int32_t check_level;     /* 1: after every pass copy the partition to host and
+                              hash it into the trace (slow; tests only)         */
+  int32_t repair;          /* 0: the reference's semantics, bugs included.
+                              1: opt-in repaired coarsening (no counterpart in
+                              the reference): coarse row c is built from its own
+                              fine vertices with all duplicates merged, and
+                              coarsening stops once a level keeps more than 95%
+                              of its vertices (20*n_c > 19*n).  Refinement is
+                              unchanged.                                        */
+} tgp_options;
+
+/* trace record kinds */
+enum { TGP_TRACE_LEVEL = 1, TGP_TRACE_INIT = 2, TGP_TRACE_PASS = 3, TGP_TRACE_REFINE = 4,
+       TGP_TRACE_FINAL = 5 };
+
+/* trace records (same layout the oracle emits, used by differential tests)  */
+typedef struct tgp_trace {
+  int32_t kind;            /* 1 LEVEL, 2 INIT, 3 PASS, 4 REFINE, 5 FINAL        */
+  int32_t level, phase, pass;
+  int64_t a, b, c, d;      /* LEVEL: n, m, n_c, m_c    INIT: n_c, npairs, cut0
+                              PASS: moves M, selected S, cut, pads in prefix
+                              REFINE: n, nedges, cut, passes  FINAL: n, m, cut  */
+  uint64_t hash;           /* FNV-1a of the partition (only with check_level)  */
+} tgp_trace;
+
+#define TGP_NKERNELS 16
+typedef struct tgp_stats {
+  int32_t levels;                  /* coarsening levels built (reference `pass`) */
+  int32_t passes;                  /* refinement passes run, all levels          */
+  int64_t coarsest_n, coarsest_npairs;
+  int64_t match_rounds;            /* parallel rounds of the exact matching      */
+  double  seconds_total;           /* wall time of tgp_partition                 */
+  double  seconds_initpart;        /* initial partitioner callback               */
+  /* per-kernel totals when options.profile = 1 (see tgp_kernel_name)         */
+  double  kernel_ms[TGP_NKERNELS];
+  int64_t kernel_launches[TGP_NKERNELS];
+  double  kernel_bytes[TGP_NKERNELS]; /* algorithmic bytes moved, per kernel    */
+  tgp_trace* trace;                /* caller buffer for trace records or NULL    */
+  int64_t trace_cap, trace_len;
+  int64_t fail_level, fail_value;  /* TGP_EEDGES: level and the m_c it rejected  */
+} tgp_stats;
+
+/* Partitions `g` into o->k parts.  Replaces main()'s pipeline
+ * (gpu_part.cu:960-1322): coarsening, initial partition, uncoarsening with
+ * refinement, final edge cut.  Outputs (host memory, caller-owned):
+ *   part[n]          final partition (finalpartition.txt content)
+ *   part_count[k]    the reference's d_partCount at exit, i.e. the counts of
+ *                    the last committed refinement pass (gpu_part.cu:1338-1341)
+ *   edge_cut         final cut, floor(arc cut / 2) (gpu_part.cu:1285, 468)
+ * Default device 0, one HIP stream.                                          */
+int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part,
+                  int32_t* part_count, int64_t* edge_cut, tgp_stats* stats);
+
+const char* tgp_strerror(int code);
+/* Detail of the last failure on the calling thread ("" if none).            */
+const char* tgp_last_error(void);
+const char* tgp_kernel_name(int id);
+/* Returns cached device memory of the calling process to the HIP runtime.   */
+void tgp_release_device_memory(void);
+/* Copies a host CSR into device memory (so tgp_partition can run with
+ * device_input = 1 on a graph already resident in HBM); no counterpart in the
+ * reference, whose main() uploads inline (gpu_part.cu:950-955).             */
+int tgp_graph_upload(const tgp_graph* host, tgp_graph* dev);
+void tgp_graph_device_free(tgp_graph* dev);
+
+/* ---- files of the reference contract ------------------------------------- */
+/* Reads <dir>/row.txt, column.txt, weight.txt holding exactly n+1, m, m
+ * decimal ints (gpu_part.cu:786-800, 905-908, 943-945).  Arrays are malloc'd;
+ * release with tgp_graph_free.                                                */
+int tgp_read_graph_dir(const char* dir, int64_t n, int64_t m, tgp_graph* out);
+/* Writes ./finalpartition.txt bytes (gpu_part.cu:1294-1321) to `path`.       */
+int tgp_write_final_partition(const char* path, const int32_t* part, int64_t n, int32_t k);
+/* METIS text of a CSR (gpu_part.cu:803-853) into a malloc'd buffer.          */
+int tgp_metis_text(const tgp_graph* g, char** text, int64_t* len, int64_t* npairs);
+
+/* ---- synthetic workloads (no counterpart in the reference) ---------------- */
+/* weight_mode 0: unit weights; 1: 1 + hash(min,max) mod 16                   */
+int tgp_gen_rmat(int32_t scale, int32_t edge_factor, uint64_t seed, int32_t weight_mode,
+                 tgp_graph* out);
+int tgp_gen_grid3d(int32_t nx, int32_t ny, int32_t nz, tgp_graph* out);
+int tgp_gen_er(int64_t n, double mean_degree, uint64_t seed, tgp_graph* out);
+void tgp_graph_free(tgp_graph* g);
+void tgp_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+"""Seeded test graphs shared by the CPU and GPU test files."""
+import numpy as np
+
+
+def csr_from_edges(n, edges, sym=True, weights=None, sort_rows=True):
+    """CSR from an edge list; duplicates kept, self loops kept as given."""
+    rows = [[] for _ in range(n)]
+    for idx, (u, v) in enumerate(edges):
+        w = 1 if weights is None else int(weights[idx])
+        rows[u].append((v, w))
+        if sym and u != v:
+            rows[v].append((u, w))
+    xadj = [0]
+    adj, wt = [], []
+    for r in rows:
+        if sort_rows:
+            r = sorted(r)
+        for v, w in r:
+            adj.append(v)
+            wt.append(w)
+        xadj.append(len(adj))
+    return (np.array(xadj, np.int64), np.array(adj, np.int32), np.array(wt, np.int32))
+
+
+def readme_graph():
+    """The 5-vertex CSR of the reference README (README.md:58-82)."""
+    return (np.array([0, 2, 4, 4, 5, 6], np.int64), np.array([1, 2, 0, 3, 4, 2], np.int32),
+            np.ones(6, np.int32))
+
+
+def path_graph(n):
+    return csr_from_edges(n, [(i, i + 1) for i in range(n - 1)])
+
+
+def random_graph(n, m, seed, sym=True, wmax=1, dup=False, selfloops=False):
+    rng = np.random.default_rng(seed)
+    edges = set() if not dup else []
+    tries = 0
+    while len(edges) < m and tries < 20 * m:
+        tries += 1
+        u, v = int(rng.integers(n)), int(rng.integers(n))
+        if u == v and not selfloops:
+            continue
+        if dup:
+            edges.append((u, v))
+        else:
+            edges.add((min(u, v), max(u, v)) if sym else (u, v))
+    edges = list(edges) if not dup else edges
+    w = rng.integers(1, wmax + 1, size=len(edges)) if wmax > 1 else None
+    xadj, adj, wt = csr_from_edges(n, edges, sym=sym, weights=w, sort_rows=not dup)
+    return xadj, adj, wt

@@ -1,0 +1,126 @@
+"""GPU path vs the CPU oracle, bit for bit.
+
+Every case runs the full pipeline on the GPU through the C ABI (libtgp.so)
+and the oracle on the same CSR with the same initial partitioner, then
+compares the final partition, the reference's part counts, the edge cut and
+the whole per-level / per-pass trace (move counts, selections, cuts and a
+hash of the partition after every pass).
+"""
+import numpy as np
+import pytest
+
+import graphs
+import thegraphpartitioning_amd as tgp
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(oracle, xadj, adj, w, k, repair=False, floor=0, wide=False, init_part=None):
+    """Runs GPU and oracle; a failure is returned as ("error", code, trace)."""
+    g = tgp.Graph(xadj, adj, w)
+    kw = dict(wide=wide, repair=repair, coarsen_floor=floor, check_level=1, trace_cap=1 << 16)
+    okw = dict(wide=int(wide), coarsen_floor=floor, repair=int(repair))
+    if init_part is not None:
+        kw["init_part"] = init_part
+        okw.update(init_mode=oracle.INIT_ARRAY, init_part=init_part)
+    else:
+        kw["init"] = tgp.INIT_BLOCK
+        okw.update(init_mode=oracle.INIT_BLOCK)
+    try:
+        gp = tgp.partition(g, k, **kw)
+    except tgp.TGPError as e:
+        gp = ("error", e.code, e.trace)
+    try:
+        op = oracle.partition(g.xadj, g.adjncy, g.adjwgt, k, **okw)
+    except oracle.OracleError as e:
+        op = ("error", e.code, e.trace)
+    return gp, op
+
+
+def assert_same(gp, op, label):
+    if gp[0] == "error" or op[0] == "error":
+        assert gp[0] == op[0] == "error", f"{label}: gpu {gp[:2]} oracle {op[:2]}"
+        assert gp[1] == op[1], f"{label}: error code gpu {gp[1]} oracle {op[1]}"
+        assert [tuple(t) for t in gp[2]] == [tuple(t) for t in op[2]], f"{label}: traces"
+        return
+    gpart, gpc, gcut, gst = gp
+    opart, opc, ocut, otr = op
+    gtr = gst.trace
+    assert len(gtr) == len(otr), f"{label}: trace length {len(gtr)} vs oracle {len(otr)}"
+    for i, (a, b) in enumerate(zip(gtr, otr)):
+        assert tuple(a) == tuple(b), f"{label}: trace record {i}: gpu {a} oracle {b}"
+    assert gcut == ocut, label
+    np.testing.assert_array_equal(gpart, opart, err_msg=label)
+    np.testing.assert_array_equal(gpc, opc, err_msg=label)
+
+
+CASES = []
+for seed in range(3):
+    CASES.append(("er300", lambda s=seed: graphs.random_graph(300, 900, s), 4))
+    CASES.append(("er2000w", lambda s=seed: graphs.random_graph(2000, 9000, 10 + s, wmax=9), 8))
+    CASES.append(("asym800", lambda s=seed: graphs.random_graph(800, 3000, 20 + s, sym=False), 5))
+    CASES.append(("dup600", lambda s=seed: graphs.random_graph(600, 2400, 30 + s, dup=True,
+                                                                selfloops=True, wmax=3), 3))
+
+
+@pytest.mark.parametrize("repair", [False, True])
+@pytest.mark.parametrize("floor", [0, 40])
+@pytest.mark.parametrize("name,make,k", CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(CASES)])
+def test_random_graphs(oracle, name, make, k, floor, repair):
+    xadj, adj, w = make()
+    gp, op = run_both(oracle, xadj, adj, w, k, repair=repair, floor=floor)
+    assert_same(gp, op, f"{name} k={k} floor={floor} repair={repair}")
+
+
+@pytest.mark.parametrize("repair", [False, True])
+@pytest.mark.parametrize("spec", [("rmat", 12, 8, 0, 16), ("rmat", 13, 4, 1, 64),
+                                  ("rmat", 12, 16, 1, 7), ("grid", 12, 0, 0, 8),
+                                  ("grid", 16, 0, 0, 128), ("er", 5000, 10, 0, 32)])
+def test_generated_graphs(oracle, spec, repair):
+    kind, a, b, wmode, k = spec
+    if kind == "rmat":
+        g = tgp.gen_rmat(a, b, 0x7C0DE5EED + a, "hash16" if wmode else "unit")
+    elif kind == "grid":
+        g = tgp.gen_grid3d(a)
+    else:
+        g = tgp.gen_er(a, b, 99)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, k, repair=repair, floor=100)
+    assert_same(gp, op, f"{spec} repair={repair}")
+
+
+def test_readme_graph_and_paths(oracle):
+    for xadj, adj, w in [graphs.readme_graph(), graphs.path_graph(4), graphs.path_graph(9)]:
+        for k in (1, 2, 3):
+            gp, op = run_both(oracle, xadj, adj, w, k, floor=2)
+            assert_same(gp, op, f"small n={len(xadj) - 1} k={k}")
+
+
+def test_kat_r1_fixture_initial_partition(oracle):
+    xadj, adj, w = graphs.path_graph(4)
+    gp, op = run_both(oracle, xadj, adj, w, 2, init_part=[0, 1, 0, 1])
+    assert_same(gp, op, "KAT-R1")
+    tr = gp[3].trace
+    assert tr[0][:8] == (2, 0, 0, 0, 4, 3, 2, 0)       # INIT: n_c=4, |pairs|=3, cut 2 (B5)
+    assert tr[1][4:7] == (3, 3, 1)                      # pass 0: M=3, S=3, cut=1
+    assert tr[2][4:7] == (3, 3, 0)                      # pass 1: M=3, S=3, cut=0
+
+
+def test_k_larger_than_n(oracle):
+    xadj, adj, w = graphs.random_graph(50, 120, 5)
+    gp, op = run_both(oracle, xadj, adj, w, 64)
+    assert_same(gp, op, "k > n")
+
+
+def test_wide_mode(oracle):
+    g = tgp.gen_rmat(12, 8, 5, "hash16")
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 16, wide=True, floor=64)
+    assert_same(gp, op, "wide")
+
+
+def test_deterministic_repeat():
+    g = tgp.gen_rmat(14, 8, 11)
+    a = tgp.partition(g, 16, init=tgp.INIT_BLOCK, repair=True)
+    b = tgp.partition(g, 16, init=tgp.INIT_BLOCK, repair=True)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2] == b[2]

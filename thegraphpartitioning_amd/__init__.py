@@ -1,0 +1,326 @@
+"""MI355X multilevel k-way graph partitioner (ctypes binding of libtgp.so).
+
+A drop-in for the partitioning path of iamrupamroyy/TheGraphPartitioning
+(src/gpu_part.cu): heavy-edge matching, contraction, an initial partition of
+the coarsest graph, then projection and deterministic refinement, all on one
+GPU through hand-written gfx950 kernels.  The `gpu_part` command built next to
+this file keeps the reference's command line and files; this module exposes
+the same path to Python callers.
+
+Every call goes to the native library.  There is no CPU fallback: a missing
+library raises ImportError and a GPU failure raises TGPError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["partition", "gen_rmat", "gen_grid3d", "gen_er", "Graph", "TGPError", "Stats",
+           "read_graph_dir", "write_final_partition", "metis_text", "lib", "library_path",
+           "KERNEL_NAMES", "INIT_GPMETIS", "INIT_BLOCK"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtgp.so")
+CLI_PATH = os.path.join(_HERE, "gpu_part")
+
+INIT_GPMETIS = "gpmetis"
+INIT_BLOCK = "block"
+
+TGP_OK, TGP_EINVAL, TGP_ELIMIT, TGP_EEDGES = 0, -1, -2, -3
+TGP_EINITPART, TGP_EPARTFILE, TGP_EHIP, TGP_ENOMEM, TGP_EIO = -4, -5, -6, -7, -8
+NKERNELS = 16
+
+
+class CGraph(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("xadj", C.c_void_p),
+                ("adjncy", C.c_void_p), ("adjwgt", C.c_void_p)]
+
+
+INITPART_FN = C.CFUNCTYPE(C.c_int, C.POINTER(CGraph), C.c_int64, C.c_int32,
+                          C.POINTER(C.c_int32), C.c_void_p)
+
+
+class COptions(C.Structure):
+    _fields_ = [("k", C.c_int32), ("wide", C.c_int32), ("coarsen_floor", C.c_int64),
+                ("initpart", C.c_void_p), ("initpart_user", C.c_void_p),
+                ("device_input", C.c_int32), ("profile", C.c_int32),
+                ("check_level", C.c_int32), ("repair", C.c_int32)]
+
+
+class CTrace(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("level", C.c_int32), ("phase", C.c_int32),
+                ("pass_", C.c_int32), ("a", C.c_int64), ("b", C.c_int64), ("c", C.c_int64),
+                ("d", C.c_int64), ("hash", C.c_uint64)]
+
+
+class CStats(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("passes", C.c_int32), ("coarsest_n", C.c_int64),
+                ("coarsest_npairs", C.c_int64), ("match_rounds", C.c_int64),
+                ("seconds_total", C.c_double), ("seconds_initpart", C.c_double),
+                ("kernel_ms", C.c_double * NKERNELS),
+                ("kernel_launches", C.c_int64 * NKERNELS),
+                ("kernel_bytes", C.c_double * NKERNELS),
+                ("trace", C.POINTER(CTrace)), ("trace_cap", C.c_int64),
+                ("trace_len", C.c_int64), ("fail_level", C.c_int64),
+                ("fail_value", C.c_int64)]
+
+
+_lib = None
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def lib():
+    """Loads libtgp.so (built in-tree by thegraphpartitioning_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError("libtgp.so is not built; run `python -m thegraphpartitioning_amd.build`")
+    L = C.CDLL(_LIB_PATH)
+    P = C.c_void_p
+    L.tgp_partition.argtypes = [C.POINTER(CGraph), C.POINTER(COptions), P, P, P,
+                                C.POINTER(CStats)]
+    L.tgp_partition.restype = C.c_int
+    L.tgp_strerror.argtypes = [C.c_int]
+    L.tgp_strerror.restype = C.c_char_p
+    L.tgp_last_error.restype = C.c_char_p
+    L.tgp_kernel_name.argtypes = [C.c_int]
+    L.tgp_kernel_name.restype = C.c_char_p
+    for name in ("tgp_gen_rmat", "tgp_gen_grid3d", "tgp_gen_er", "tgp_read_graph_dir"):
+        getattr(L, name).restype = C.c_int
+    L.tgp_gen_rmat.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_int32, C.POINTER(CGraph)]
+    L.tgp_gen_grid3d.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(CGraph)]
+    L.tgp_gen_er.argtypes = [C.c_int64, C.c_double, C.c_uint64, C.POINTER(CGraph)]
+    L.tgp_read_graph_dir.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(CGraph)]
+    L.tgp_graph_free.argtypes = [C.POINTER(CGraph)]
+    L.tgp_graph_free.restype = None
+    L.tgp_write_final_partition.argtypes = [C.c_char_p, P, C.c_int64, C.c_int32]
+    L.tgp_write_final_partition.restype = C.c_int
+    L.tgp_metis_text.argtypes = [C.POINTER(CGraph), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.tgp_metis_text.restype = C.c_int
+    L.tgp_free.argtypes = [P]
+    L.tgp_free.restype = None
+    L.tgp_initpart_block.argtypes = [C.POINTER(CGraph), C.c_int64, C.c_int32,
+                                     C.POINTER(C.c_int32), P]
+    L.tgp_initpart_gpmetis.argtypes = L.tgp_initpart_block.argtypes
+    L.tgp_release_device_memory.restype = None
+    L.tgp_graph_upload.argtypes = [C.POINTER(CGraph), C.POINTER(CGraph)]
+    L.tgp_graph_upload.restype = C.c_int
+    L.tgp_graph_device_free.argtypes = [C.POINTER(CGraph)]
+    L.tgp_graph_device_free.restype = None
+    _lib = L
+    return L
+
+
+KERNEL_NAMES = ["match", "relabel", "contract", "pairs", "project", "gain", "token", "compact",
+                "sort", "select", "commit", "cut", "scan", "misc"]
+
+
+class TGPError(RuntimeError):
+    def __init__(self, code, detail=""):
+        msg = lib().tgp_strerror(code).decode()
+        super().__init__(f"{msg} ({code})" + (f": {detail}" if detail else ""))
+        self.code = code
+        self.detail = detail
+
+
+class Graph:
+    """A CSR graph held in numpy arrays (int64 offsets, int32 ids/weights)."""
+
+    def __init__(self, xadj, adjncy, adjwgt=None):
+        self.xadj = np.ascontiguousarray(xadj, dtype=np.int64)
+        self.adjncy = np.ascontiguousarray(adjncy, dtype=np.int32)
+        if adjwgt is None:
+            adjwgt = np.ones(len(self.adjncy), np.int32)
+        self.adjwgt = np.ascontiguousarray(adjwgt, dtype=np.int32)
+        self.n = len(self.xadj) - 1
+        self.m = len(self.adjncy)
+
+    def cgraph(self):
+        return CGraph(self.n, self.m, self.xadj.ctypes.data, self.adjncy.ctypes.data,
+                      self.adjwgt.ctypes.data)
+
+
+def _take_cgraph(cg):
+    n, m = cg.n, cg.m
+    g = Graph(np.ctypeslib.as_array(C.cast(cg.xadj, C.POINTER(C.c_int64)), (n + 1,)).copy(),
+              np.ctypeslib.as_array(C.cast(cg.adjncy, C.POINTER(C.c_int32)), (m,)).copy(),
+              np.ctypeslib.as_array(C.cast(cg.adjwgt, C.POINTER(C.c_int32)), (m,)).copy())
+    lib().tgp_graph_free(C.byref(cg))
+    return g
+
+
+def gen_rmat(scale, edge_factor, seed, weights="unit"):
+    cg = CGraph()
+    rc = lib().tgp_gen_rmat(scale, edge_factor, seed, 1 if weights == "hash16" else 0,
+                            C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc)
+    return _take_cgraph(cg)
+
+
+def gen_grid3d(nx, ny=None, nz=None):
+    cg = CGraph()
+    rc = lib().tgp_gen_grid3d(nx, ny or nx, nz or nx, C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc)
+    return _take_cgraph(cg)
+
+
+def gen_er(n, mean_degree, seed):
+    cg = CGraph()
+    rc = lib().tgp_gen_er(n, mean_degree, seed, C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc)
+    return _take_cgraph(cg)
+
+
+def read_graph_dir(path, n, m):
+    cg = CGraph()
+    rc = lib().tgp_read_graph_dir(path.encode(), n, m, C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc, lib().tgp_last_error().decode())
+    return _take_cgraph(cg)
+
+
+def write_final_partition(path, part, k):
+    part = np.ascontiguousarray(part, dtype=np.int32)
+    rc = lib().tgp_write_final_partition(path.encode(), part.ctypes.data, len(part), k)
+    if rc != 0:
+        raise TGPError(rc)
+
+
+def metis_text(g):
+    cg = g.cgraph()
+    buf = C.c_void_p()
+    ln = C.c_int64()
+    npairs = C.c_int64()
+    rc = lib().tgp_metis_text(C.byref(cg), C.byref(buf), C.byref(ln), C.byref(npairs))
+    if rc != 0:
+        raise TGPError(rc)
+    data = C.string_at(buf, ln.value)
+    lib().tgp_free(buf)
+    return data, npairs.value
+
+
+class DeviceGraph:
+    """A CSR copied once into device memory (HBM); free() returns it."""
+
+    def __init__(self, g):
+        self.n, self.m = g.n, g.m
+        self._cg = CGraph()
+        rc = lib().tgp_graph_upload(C.byref(g.cgraph()), C.byref(self._cg))
+        if rc != 0:
+            raise TGPError(rc)
+
+    def pointers(self):
+        return (self._cg.xadj, self._cg.adjncy, self._cg.adjwgt)
+
+    def free(self):
+        if self._cg.xadj:
+            lib().tgp_graph_device_free(C.byref(self._cg))
+
+
+class Stats:
+    def __init__(self, cs, trace):
+        self.levels = cs.levels
+        self.passes = cs.passes
+        self.coarsest_n = cs.coarsest_n
+        self.coarsest_npairs = cs.coarsest_npairs
+        self.match_rounds = cs.match_rounds
+        self.seconds_total = cs.seconds_total
+        self.seconds_initpart = cs.seconds_initpart
+        self.fail_level = cs.fail_level
+        self.fail_value = cs.fail_value
+        self.kernel_ms = {KERNEL_NAMES[i]: cs.kernel_ms[i] for i in range(len(KERNEL_NAMES))}
+        self.kernel_launches = {KERNEL_NAMES[i]: cs.kernel_launches[i]
+                                for i in range(len(KERNEL_NAMES))}
+        self.kernel_bytes = {KERNEL_NAMES[i]: cs.kernel_bytes[i]
+                             for i in range(len(KERNEL_NAMES))}
+        self.trace = trace
+
+
+def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=False,
+              coarsen_floor=0, profile=False, check_level=0, trace_cap=0, device_graph=None):
+    """Partitions `g` (a Graph) into k parts on the GPU.
+
+    Returns (part[n] int32, part_count[k] int32, edge_cut, Stats).  `init`
+    chooses the initial partitioner of the coarsest graph: "gpmetis" (the
+    reference seam, needs gpmetis on PATH), "block" (i*k/n_c), or a Python
+    callable f(xadj, adjncy, adjwgt, npairs, k) -> part array.  `init_part`
+    supplies fixed values instead (like a fixture part file).
+    `device_graph`, if given, is a (xadj, adjncy, adjwgt) triple of device
+    pointers holding the same CSR already resident in HBM.
+    """
+    L = lib()
+    o = COptions()
+    o.k = k
+    o.wide = int(bool(wide))
+    o.coarsen_floor = int(coarsen_floor)
+    o.profile = int(bool(profile))
+    o.check_level = int(check_level)
+    o.repair = int(bool(repair))
+    keep = []
+    if init_part is not None:
+        fixed = np.ascontiguousarray(init_part, dtype=np.int32)
+
+        def init_cb(cgp, npairs, kk, out, user):
+            nc = cgp.contents.n
+            for i in range(nc):
+                out[i] = int(fixed[i]) if i < len(fixed) else 0
+            return 0
+        cb = INITPART_FN(init_cb)
+        keep.append(cb)
+        o.initpart = C.cast(cb, C.c_void_p).value
+    elif callable(init):
+        def init_cb(cgp, npairs, kk, out, user):
+            cgv = cgp.contents
+            xa = np.ctypeslib.as_array(C.cast(cgv.xadj, C.POINTER(C.c_int64)), (cgv.n + 1,))
+            ad = np.ctypeslib.as_array(C.cast(cgv.adjncy, C.POINTER(C.c_int32)), (max(cgv.m, 1),))
+            wt = np.ctypeslib.as_array(C.cast(cgv.adjwgt, C.POINTER(C.c_int32)), (max(cgv.m, 1),))
+            p = np.asarray(init(xa.copy(), ad[:cgv.m].copy(), wt[:cgv.m].copy(), npairs, kk),
+                           dtype=np.int32)
+            for i in range(cgv.n):
+                out[i] = int(p[i])
+            return 0
+        cb = INITPART_FN(init_cb)
+        keep.append(cb)
+        o.initpart = C.cast(cb, C.c_void_p).value
+    elif init == INIT_BLOCK:
+        o.initpart = C.cast(L.tgp_initpart_block, C.c_void_p).value
+    elif init == INIT_GPMETIS:
+        o.initpart = C.cast(L.tgp_initpart_gpmetis, C.c_void_p).value
+    else:
+        raise ValueError(f"unknown initial partitioner {init!r}")
+    if device_graph is not None:
+        o.device_input = 1
+        cg = CGraph(g.n, g.m, device_graph[0], device_graph[1], device_graph[2])
+    else:
+        cg = g.cgraph()
+    part = np.zeros(g.n, np.int32)
+    pc = np.zeros(k, np.int32)
+    cut = C.c_int64()
+    st = CStats()
+    tbuf = None
+    if trace_cap:
+        tbuf = (CTrace * trace_cap)()
+        st.trace = C.cast(tbuf, C.POINTER(CTrace))
+        st.trace_cap = trace_cap
+    rc = L.tgp_partition(C.byref(cg), C.byref(o), part.ctypes.data, pc.ctypes.data,
+                         C.addressof(cut), C.byref(st))
+    trace = []
+    if tbuf is not None:
+        for i in range(min(st.trace_len, trace_cap)):
+            t = tbuf[i]
+            trace.append((t.kind, t.level, t.phase, t.pass_, t.a, t.b, t.c, t.d, t.hash))
+    if rc != 0:
+        err = TGPError(rc, L.tgp_last_error().decode())
+        err.fail_level = st.fail_level
+        err.fail_value = st.fail_value
+        err.trace = trace
+        raise err
+    return part, pc, cut.value, Stats(st, trace)

@@ -1,0 +1,865 @@
+// tgp_driver.cpp -- host orchestration of the multilevel k-way partitioner on
+// one GPU: the coarsening level loop, the initial-partition seam, projection
+// and the two-phase refinement driver.  Control flow and every host-side fp32
+// expression follow src/gpu_part.cu (iamrupamroyy/TheGraphPartitioning); all
+// graph work runs in the gfx950 kernels of tgp_kernels.hip on one HIP stream.
+//
+// Built with -ffp-contract=off so the balance score keeps separately rounded
+// multiply and add, as the reference's host code does.
+#include "../../include/tgp.h"
+#include "tgp_internal.h"
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace tgp {
+
+static void set_error(const std::string& s) { last_error() = s; }
+
+#define HIP_TRY(call)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + __FILE__ + \
+                ":" + std::to_string(__LINE__));                                    \
+      throw Fail(e_ == hipErrorOutOfMemory ? TGP_ENOMEM : TGP_EHIP);                \
+    }                                                                               \
+  } while (0)
+
+struct Fail {
+  int code;
+  explicit Fail(int c) : code(c) {}
+};
+
+// ---------------------------------------------------------------- device pool --
+// Grow-only cache of device blocks keyed by rounded size; blocks return to the
+// cache when released and are freed when the context is destroyed.
+class DevicePool {
+ public:
+  ~DevicePool() {
+    for (auto& kv : free_)
+      for (void* p : kv.second) (void)hipFree(p);
+  }
+  static size_t round_size(size_t b) {
+    if (b < 256) return 256;
+    size_t p = 1;
+    while (p < b) p <<= 1;
+    const size_t step = p >= 2048 ? p / 8 : p;
+    return ((b + step - 1) / step) * step;
+  }
+  void* get(size_t bytes) {
+    const size_t r = round_size(bytes);
+    auto it = free_.find(r);
+    if (it != free_.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, r);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation of " + std::to_string(r) + " bytes failed");
+      throw Fail(TGP_ENOMEM);
+    }
+    return p;
+  }
+  void put(void* p, size_t bytes) {
+    if (p) free_[round_size(bytes)].push_back(p);
+  }
+  void trim() {
+    for (auto& kv : free_) {
+      for (void* p : kv.second) (void)hipFree(p);
+      kv.second.clear();
+    }
+  }
+
+ private:
+  std::map<size_t, std::vector<void*>> free_;
+};
+
+template <typename T>
+struct DBuf {
+  DevicePool* pool = nullptr;
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(DevicePool* pl, size_t cnt) : pool(pl), p((T*)pl->get(cnt * sizeof(T) + 16)), n(cnt) {}
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    release();
+    pool = o.pool;
+    p = o.p;
+    n = o.n;
+    o.p = nullptr;
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p && pool) pool->put(p, n * sizeof(T) + 16);
+    p = nullptr;
+  }
+  void ensure(DevicePool* pl, size_t cnt) {
+    if (p && n >= cnt) return;
+    release();
+    pool = pl;
+    p = (T*)pl->get(cnt * sizeof(T) + 16);
+    n = cnt;
+  }
+};
+
+// ------------------------------------------------------------------ profiling --
+struct Prof {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> pool;
+  struct Rec {
+    int kid;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  double ms[KID_COUNT] = {};
+  i64 launches[KID_COUNT] = {};
+  double bytes[KID_COUNT] = {};
+  int cur = -1;
+  hipEvent_t cur_ev = nullptr;
+
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    return e;
+  }
+  void begin(int kid) {
+    launches[kid]++;
+    if (!on) return;
+    cur = kid;
+    cur_ev = ev();
+    HIP_TRY(hipEventRecord(cur_ev, s));
+  }
+  void end(int kid, double algo_bytes) {
+    bytes[kid] += algo_bytes;
+    if (!on) return;
+    hipEvent_t b = ev();
+    HIP_TRY(hipEventRecord(b, s));
+    pending.push_back(Rec{kid, cur_ev, b});
+    cur = -1;
+  }
+  void flush() {
+    if (!on || pending.empty()) return;
+    for (const Rec& r : pending) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.kid] += t;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  ~Prof() {
+    for (auto& r : pending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// ---------------------------------------------------------------------- context --
+struct Level {
+  i64 n = 0, m = 0;
+  DBuf<i64> xadj;
+  DBuf<i32> adj, w;
+  DBuf<i32> tb;  // 2n: coarse vertex -> its fine vertices on the level above
+  const i64* xadj_p = nullptr;
+  const i32* adj_p = nullptr;
+  const i32* w_p = nullptr;
+  DGraph dg() const { return DGraph{n, m, xadj_p, adj_p, w_p}; }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t s = nullptr;
+  DevicePool pool;
+  i64* pinned = nullptr;  // small host staging area
+  static constexpr size_t PINNED_ELEMS = 8192 + 64;
+  Prof prof;
+
+  Context() {
+    HIP_TRY(hipGetDevice(&device));
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc((void**)&pinned, PINNED_ELEMS * sizeof(i64), hipHostMallocDefault));
+    prof.s = s;
+  }
+  ~Context() {
+    if (pinned) (void)hipHostFree(pinned);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void sync() {
+    HIP_TRY(hipStreamSynchronize(s));
+    prof.flush();
+  }
+};
+
+// ------------------------------------------------------------------------ run --
+struct Tracer {
+  tgp_trace* buf = nullptr;
+  i64 cap = 0, len = 0;
+  i64* len_out = nullptr;  // mirrors len into tgp_stats so failures keep their trace
+  void push(int kind, int level, int phase, int pass, i64 a, i64 b, i64 c, i64 d, u64 h) {
+    if (buf && len < cap) {
+      tgp_trace& t = buf[len];
+      t.kind = kind;
+      t.level = level;
+      t.phase = phase;
+      t.pass = pass;
+      t.a = a;
+      t.b = b;
+      t.c = c;
+      t.d = d;
+      t.hash = h;
+    }
+    len++;
+    if (len_out) *len_out = len;
+  }
+};
+
+static u64 fnv_host(const i32* v, i64 n) {
+  u64 h = 1469598103934665603ull;
+  for (i64 i = 0; i < n; i++) {
+    const u32 x = (u32)v[i];
+    for (int b = 0; b < 4; b++) {
+      h ^= (x >> (8 * b)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  }
+  return h;
+}
+
+class Run {
+ public:
+  Run(Context& c, const tgp_options& o, tgp_stats* st) : C(c), O(o), S(st) {
+    k = o.k;
+    wide = o.wide != 0;
+    repair = o.repair != 0;
+    if (st) {
+      tr.buf = st->trace;
+      tr.cap = st->trace_cap;
+      tr.len_out = &st->trace_len;
+    }
+  }
+
+  int go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out);
+
+ private:
+  Context& C;
+  const tgp_options& O;
+  tgp_stats* S;
+  Tracer tr;
+  i32 k = 0;
+  bool wide = false, repair = false;
+  int passes_total = 0;
+  i64 match_rounds = 0;
+  double initpart_s = 0;
+
+  std::vector<std::unique_ptr<Level>> L;
+  std::vector<i64> ecount;
+
+  // shared scratch sized for the finest level
+  DBuf<i64> scan_sc;
+  DBuf<i64> small;  // device scalars: [0] total, [1] cut acc, [2..] misc
+  DBuf<i32> small32;
+
+  hipStream_t st() { return C.s; }
+  ScanScratch scratch() { return ScanScratch{scan_sc.p, (i64)scan_sc.n}; }
+
+  u64 hash_dev(const i32* d, i64 n) {
+    std::vector<i32> h((size_t)n);
+    HIP_TRY(hipMemcpyAsync(h.data(), d, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    return fnv_host(h.data(), n);
+  }
+  i64 read_i64(const i64* d) {
+    HIP_TRY(hipMemcpyAsync(C.pinned, d, sizeof(i64), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    return C.pinned[0];
+  }
+
+  i64 wrap_cut(u64 acc) const {
+    if (!wide) {
+      const i32 c = (i32)(u32)acc;
+      return (i64)(c / 2);
+    }
+    return (i64)acc / 2;
+  }
+
+  void coarsen();
+  void match_level(const Level& lv, DBuf<i32>& mate);
+  int initial_partition(DBuf<i32>& part, i64* npairs);
+  i64 compute_cut(const Level& lv, i64 nedges, const i32* part, const i32* heavy, i64 nheavy,
+                  i64 heavy_deg);
+  void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut);
+};
+
+static const i64 HEAVY_DEG = 64;
+
+void Run::match_level(const Level& lv, DBuf<i32>& mate) {
+  const i64 n = lv.n;
+  DBuf<u8> dec(&C.pool, n);
+  DBuf<i32> pend(&C.pool, n), prop(&C.pool, n), la(&C.pool, n), lb(&C.pool, n);
+  DBuf<u64> R(&C.pool, n);
+  DBuf<i32> counters(&C.pool, 2);
+  C.prof.begin(KID_MATCH);
+  launch_match_init(n, mate.p, dec.p, pend.p, R.p, la.p, counters.p, st());
+  u32 round = 0;
+  const DGraph g = lv.dg();
+  // Rounds are issued in batches; the pending count is read back between them.
+  int batch = 4;
+  while (true) {
+    for (int b = 0; b < batch; b++) {
+      round++;
+      i32* lin = (round & 1) ? la.p : lb.p;
+      i32* lout = (round & 1) ? lb.p : la.p;
+      launch_match_round(g, mate.p, dec.p, pend.p, R.p, prop.p, lin, lout, counters.p, round,
+                         st());
+    }
+    HIP_TRY(hipMemcpyAsync(C.pinned, counters.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+    HIP_TRY(hipStreamSynchronize(st()));
+    const i32* c32 = (const i32*)C.pinned;
+    // counters[round&1] = pending count at the start of the last round
+    const i32 pending_last = c32[round & 1];
+    if (pending_last == 0) break;
+    // Each round decides at least the smallest pending vertex, so more rounds
+    // than vertices means a broken invariant: stop instead of spinning.
+    if ((i64)round > n + 2 * 64) {
+      set_error("matching made no progress");
+      throw Fail(TGP_EHIP);
+    }
+    if (batch < 64) batch *= 2;
+  }
+  C.prof.end(KID_MATCH, (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n);
+  match_rounds += round;
+}
+
+void Run::coarsen() {
+  const i64 n0 = L[0]->n;
+  const i64 target = O.coarsen_floor > 0 ? O.coarsen_floor : std::max<i64>(500, n0 / 500);
+  i64 ncur = n0;
+  DBuf<i32> mate(&C.pool, n0), new_ids(&C.pool, n0);
+  DBuf<i64> flags(&C.pool, n0), pos(&C.pool, n0);
+  while (ncur > target) {  // gpu_part.cu:1003
+    Level& lv = *L.back();
+    match_level(lv, mate);
+    auto tb = DBuf<i32>(&C.pool, 2 * ncur);
+    C.prof.begin(KID_RELABEL);
+    launch_relabel(mate.p, ncur, flags.p, pos.p, small.p, new_ids.p, tb.p, scratch(), st());
+    C.prof.end(KID_RELABEL, 16.0 * ncur + 8.0 * ncur);
+    const i64 n_c = read_i64(small.p);
+    if (n_c <= 0 || n_c >= ncur) break;                        // gpu_part.cu:1069
+    if (repair && 20 * n_c > 19 * ncur) break;                 // repaired mode: stall stop
+    std::unique_ptr<Level> c(new Level());
+    c->n = n_c;
+    c->xadj = DBuf<i64>(&C.pool, n_c + 1);
+    DBuf<i64> cnt(&C.pool, n_c + 1);
+    const DGraph g = lv.dg();
+    i64 m_c = 0;
+    C.prof.begin(KID_CONTRACT);
+    if (!repair) {
+      DBuf<i32> sadj(&C.pool, n_c * 64), sw(&C.pool, n_c * 64);
+      launch_contract_ref(g, mate.p, new_ids.p, n_c, sadj.p, sw.p, cnt.p, st());
+      launch_scan_excl(cnt.p, c->xadj.p, n_c, c->xadj.p + n_c, scratch(), st());
+      m_c = read_i64(c->xadj.p + n_c);
+      c->adj = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
+      c->w = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
+      if (m_c > 0) launch_compact_rows(sadj.p, sw.p, c->xadj.p, n_c, c->adj.p, c->w.p, st());
+    } else {
+      DBuf<i64> dsz(&C.pool, n_c + 1), csz(&C.pool, n_c + 1), eoff(&C.pool, n_c + 1),
+          toff(&C.pool, n_c + 1);
+      launch_rep_sizes(lv.xadj_p, tb.p, n_c, dsz.p, csz.p, st());
+      launch_scan_excl(dsz.p, eoff.p, n_c, eoff.p + n_c, scratch(), st());
+      launch_scan_excl(csz.p, toff.p, n_c, toff.p + n_c, scratch(), st());
+      HIP_TRY(hipMemcpyAsync(C.pinned, eoff.p + n_c, sizeof(i64), hipMemcpyDeviceToHost, st()));
+      HIP_TRY(hipMemcpyAsync(C.pinned + 1, toff.p + n_c, sizeof(i64), hipMemcpyDeviceToHost,
+                             st()));
+      C.sync();
+      const i64 etot = C.pinned[0], ttot = C.pinned[1];
+      DBuf<i32> tkey(&C.pool, std::max<i64>(ttot, 1)), tpos(&C.pool, std::max<i64>(ttot, 1)),
+          tw(&C.pool, std::max<i64>(ttot, 1)), eslot(&C.pool, std::max<i64>(etot, 1));
+      launch_fill_i32(tkey.p, ttot, -1, st());
+      launch_fill_i32(tpos.p, ttot, INT_MAX, st());
+      launch_fill_i32(tw.p, ttot, 0, st());
+      launch_rep_insert(g, tb.p, new_ids.p, n_c, toff.p, csz.p, tkey.p, tpos.p, tw.p, st());
+      launch_rep_mark(g, tb.p, new_ids.p, n_c, toff.p, csz.p, eoff.p, tkey.p, tpos.p, eslot.p,
+                      cnt.p, st());
+      launch_scan_excl(cnt.p, c->xadj.p, n_c, c->xadj.p + n_c, scratch(), st());
+      m_c = read_i64(c->xadj.p + n_c);
+      c->adj = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
+      c->w = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
+      if (m_c > 0)
+        launch_rep_emit(eoff.p, dsz.p, eslot.p, toff.p, tkey.p, tw.p, c->xadj.p, n_c,
+                        c->adj.p, c->w.p, st());
+    }
+    C.prof.end(KID_CONTRACT, 24.0 * n_c + 12.0 * lv.m + 8.0 * m_c);
+    i64 max_edges = 2 * ecount.back();                         // gpu_part.cu:1076-1077
+    if (!wide && max_edges > INT_MAX) max_edges = INT_MAX;
+    tr.push(TGP_TRACE_LEVEL, (int)L.size() - 1, 0, 0, ncur, lv.m, n_c, m_c, 0);
+    if (m_c <= 0 || m_c > max_edges) {                         // gpu_part.cu:1121-1125
+      if (S) {
+        S->fail_level = (i64)L.size() - 1;
+        S->fail_value = m_c;
+      }
+      set_error("Invalid new_coarsed_edge: " + std::to_string(m_c));
+      throw Fail(TGP_EEDGES);
+    }
+    c->m = m_c;
+    c->tb = std::move(tb);
+    c->xadj_p = c->xadj.p;
+    c->adj_p = c->adj.p;
+    c->w_p = c->w.p;
+    L.push_back(std::move(c));
+    ecount.push_back(m_c);
+    ncur = n_c;
+  }
+}
+
+i64 Run::compute_cut(const Level& lv, i64 nedges, const i32* part, const i32* heavy,
+                     i64 nheavy, i64 heavy_deg) {
+  C.prof.begin(KID_CUT);
+  HIP_TRY(hipMemsetAsync(small.p + 1, 0, sizeof(i64), st()));
+  launch_cut(lv.dg(), nedges, part, k, (u64*)(small.p + 1), heavy, nheavy, heavy_deg, st());
+  C.prof.end(KID_CUT, 12.0 * lv.m + 12.0 * lv.n);
+  return wrap_cut((u64)read_i64(small.p + 1));
+}
+
+int Run::initial_partition(DBuf<i32>& part, i64* npairs) {
+  Level& cl = *L.back();
+  const i64 n = cl.n;
+  // unique undirected pairs (gpu_part.cu:810-827), counted on the device
+  i64 cap = 1;
+  while (cap < 2 * cl.m + 2) cap <<= 1;
+  {
+    DBuf<u64> table(&C.pool, cap);
+    C.prof.begin(KID_PAIRS);
+    launch_fill_u64(table.p, cap, ~0ull, st());
+    HIP_TRY(hipMemsetAsync(small.p + 2, 0, sizeof(i64), st()));
+    launch_pairs(cl.dg(), table.p, cap, (u64*)(small.p + 2), st());
+    C.prof.end(KID_PAIRS, 12.0 * cl.m + 8.0 * n);
+    *npairs = read_i64(small.p + 2);
+  }
+  part.ensure(&C.pool, n);
+  const auto t0 = std::chrono::steady_clock::now();
+  if (O.initpart == tgp_initpart_block) {
+    launch_block_init(part.p, n, k, st());
+  } else {
+    // host seam: the coarsest CSR goes to the callback in host memory
+    std::vector<i64> hx((size_t)n + 1);
+    std::vector<i32> ha((size_t)std::max<i64>(cl.m, 1)), hw((size_t)std::max<i64>(cl.m, 1));
+    HIP_TRY(hipMemcpyAsync(hx.data(), cl.xadj_p, (n + 1) * sizeof(i64), hipMemcpyDeviceToHost,
+                           st()));
+    if (cl.m > 0) {
+      HIP_TRY(hipMemcpyAsync(ha.data(), cl.adj_p, cl.m * sizeof(i32), hipMemcpyDeviceToHost,
+                             st()));
+      HIP_TRY(hipMemcpyAsync(hw.data(), cl.w_p, cl.m * sizeof(i32), hipMemcpyDeviceToHost, st()));
+    }
+    C.sync();
+    tgp_graph hg{n, cl.m, hx.data(), ha.data(), hw.data()};
+    std::vector<i32> hp((size_t)n, 0);
+    tgp_initpart_fn fn = O.initpart ? O.initpart : tgp_initpart_gpmetis;
+    const int rc = fn(&hg, *npairs, k, hp.data(), O.initpart_user);
+    if (rc != 0) {
+      if (last_error().empty()) set_error("initial partitioner failed");
+      throw Fail(rc < 0 ? rc : TGP_EINITPART);
+    }
+    for (auto& p : hp)
+      if (p < 0 || p >= k) p = 0;  // gpu_part.cu:1172
+    HIP_TRY(hipMemcpyAsync(part.p, hp.data(), n * sizeof(i32), hipMemcpyHostToDevice, st()));
+    C.sync();
+  }
+  initpart_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut) {
+  Level& lv = *L[level];
+  const i64 n = lv.n;
+  const DGraph g = lv.dg();
+  const i64 T = n / k;
+  const float Tf = (float)T;
+  const float maxImbalance = 0.05f;
+
+  DBuf<i32> heavy(&C.pool, std::max<i64>(n, 1));
+  DBuf<i32> cnt32(&C.pool, 8);
+  HIP_TRY(hipMemsetAsync(cnt32.p, 0, 8 * sizeof(i32), st()));
+  launch_level_census(g, nedges, HEAVY_DEG, heavy.p, cnt32.p, cnt32.p + 1, st());
+  HIP_TRY(hipMemcpyAsync(C.pinned, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  const i64 nheavy = ((const i32*)C.pinned)[0];
+  const int unit = ((const i32*)C.pinned)[1] == 0;
+
+  i64 bestCut = *cut;
+  float bestScore = 1e9f;
+  DBuf<i32> best(&C.pool, n);
+  HIP_TRY(hipMemcpyAsync(best.p, part.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
+  int passes = 0;
+
+  if (T == 0) {
+    // Every gain is -inf or NaN (division by T = 0), so no vertex becomes a
+    // candidate and both phases stop at their first pass with no moves.
+    passes = 2;
+  } else {
+    DBuf<float> gain(&C.pool, n), mv_gain(&C.pool, n), sk_gain(&C.pool, n);
+    DBuf<i32> target(&C.pool, n), mv_v(&C.pool, n), mv_src(&C.pool, n), mv_dst(&C.pool, n),
+        sk_idx(&C.pool, n), sdst(&C.pool, n);
+    DBuf<u8> cand(&C.pool, n), sel(&C.pool, n), src_big(&C.pool, k);
+    DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
+    DBuf<float> bg(&C.pool, 2 * (size_t)k);
+    DBuf<i32> outc(&C.pool, k), ctr(&C.pool, 4);
+    DBuf<i32> H(&C.pool, select_scratch_elems(n, k));
+    DBuf<i32> gscr(&C.pool, std::max<i64>(gain_gscratch_elems(k), 1));
+    std::vector<i32> hpc((size_t)k);
+    const double arc_b = unit ? 8.0 : 12.0;
+
+    for (int phase = 0; phase < 2; phase++) {
+      const int maxPass = phase ? 40 : 30;               // gpu_part.cu:545, 647
+      bool balanced = false;
+      for (int pass = 0; pass < maxPass && !(phase && balanced); pass++) {
+        HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
+        HIP_TRY(hipMemsetAsync(ctr.p, 0, 4 * sizeof(i32), st()));
+        C.prof.begin(KID_GAIN);
+        launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
+        launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p, phase,
+                    unit, gscr.p, st());
+        C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
+        C.prof.begin(KID_TOKEN);
+        launch_token(g, nedges, gain.p, cand.p, part.p, target.p, k, flag.p, heavy.p, nheavy,
+                     HEAVY_DEG, st());
+        C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
+        C.prof.begin(KID_COMPACT);
+        launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
+        launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
+                             mv_dst.p, mv_gain.p, outc.p, st());
+        C.prof.end(KID_COMPACT, 24.0 * n);
+        i64 M = read_i64(small.p);
+        M = std::min(M, n);                              // gpu_part.cu:585
+        passes++;
+        if (M <= 0) break;                               // gpu_part.cu:638-640, 742-744
+        C.prof.begin(KID_MISC);
+        launch_capacity(pc.p, outc.p, k, n, phase, avail.p, st());
+        C.prof.end(KID_MISC, 0);
+        i64 len = M;                                     // gpu_part.cu:487-497
+        i64 padded = 1;
+        while (padded < len) padded <<= 1;
+        if (padded > n) padded = n;
+        if (padded > len) len = padded;
+        C.prof.begin(KID_SORT);
+        launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st());
+        C.prof.end(KID_SORT, 16.0 * (double)len);
+        C.prof.begin(KID_SELECT);
+        launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr.p, ctr.p + 1,
+                      st());
+        C.prof.end(KID_SELECT, 13.0 * M);
+        C.prof.begin(KID_COMMIT);
+        launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, pc.p, k, st());
+        C.prof.end(KID_COMMIT, 17.0 * M);
+        C.prof.begin(KID_CUT);
+        HIP_TRY(hipMemsetAsync(small.p + 1, 0, sizeof(i64), st()));
+        launch_cut(g, nedges, part.p, k, (u64*)(small.p + 1), heavy.p, nheavy, HEAVY_DEG, st());
+        C.prof.end(KID_CUT, arc_b * lv.m + 12.0 * n);
+        HIP_TRY(hipMemcpyAsync(C.pinned, small.p + 1, sizeof(i64), hipMemcpyDeviceToHost, st()));
+        HIP_TRY(hipMemcpyAsync(C.pinned + 1, ctr.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+        HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
+        C.sync();
+        const i64 cur = wrap_cut((u64)C.pinned[0]);
+        const i32 Ssel = ((const i32*)(C.pinned + 1))[0];
+        const i32 pads = ((const i32*)(C.pinned + 1))[1];
+        float score = 0.0f;                              // gpu_part.cu:621-626, 723-731
+        balanced = true;
+        for (i32 q = 0; q < k; q++) {
+          const float diff = (float)hpc[q] - Tf;
+          const float sq = diff * diff;
+          score = score + sq;
+          if (std::fabs(diff) > Tf * maxImbalance) balanced = false;
+        }
+        score = std::sqrt(score / (float)k);
+        if (cur < bestCut || ((float)cur <= (float)bestCut * 1.01f && score < bestScore)) {
+          bestCut = cur;
+          bestScore = score;
+          HIP_TRY(hipMemcpyAsync(best.p, part.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
+        }
+        *cut = cur;
+        tr.push(TGP_TRACE_PASS, level, phase, pass, M, Ssel, cur, pads,
+                O.check_level ? hash_dev(part.p, n) : 0);
+        const bool quiet = (M < n / 200) && (Ssel < n / 400);  // gpu_part.cu:634, 738
+        if (quiet && (phase == 0 || balanced)) break;
+      }
+      HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
+      if (phase == 0) *cut = bestCut;
+    }
+  }
+  *cut = compute_cut(lv, nedges, part.p, heavy.p, nheavy, HEAVY_DEG);  // gpu_part.cu:749
+  passes_total += passes;
+  tr.push(TGP_TRACE_REFINE, level, 0, passes, n, nedges, *cut, 0,
+          O.check_level ? hash_dev(part.p, n) : 0);
+}
+
+int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
+  const auto t_start = std::chrono::steady_clock::now();
+  const i64 n = g->n, m = g->m;
+  scan_sc = DBuf<i64>(&C.pool, scan_scratch_elems(std::max<i64>(n, 2) * 2 + 64));
+  small = DBuf<i64>(&C.pool, 16);
+
+  // level 0
+  std::unique_ptr<Level> l0(new Level());
+  l0->n = n;
+  l0->m = m;
+  if (O.device_input) {
+    l0->xadj_p = g->xadj;
+    l0->adj_p = g->adjncy;
+    l0->w_p = g->adjwgt;
+  } else {
+    l0->xadj = DBuf<i64>(&C.pool, n + 1);
+    l0->adj = DBuf<i32>(&C.pool, m);
+    l0->w = DBuf<i32>(&C.pool, m);
+    HIP_TRY(hipMemcpyAsync(l0->xadj.p, g->xadj, (n + 1) * sizeof(i64), hipMemcpyHostToDevice,
+                           st()));
+    HIP_TRY(hipMemcpyAsync(l0->adj.p, g->adjncy, m * sizeof(i32), hipMemcpyHostToDevice, st()));
+    HIP_TRY(hipMemcpyAsync(l0->w.p, g->adjwgt, m * sizeof(i32), hipMemcpyHostToDevice, st()));
+    l0->xadj_p = l0->xadj.p;
+    l0->adj_p = l0->adj.p;
+    l0->w_p = l0->w.p;
+  }
+  L.push_back(std::move(l0));
+  ecount.push_back(m);
+
+  coarsen();
+  const int pass = (int)L.size() - 1;
+
+  DBuf<i32> part, pc(&C.pool, k);
+  i64 npairs = 0;
+  initial_partition(part, &npairs);
+  HIP_TRY(hipMemsetAsync(pc.p, 0, k * sizeof(i32), st()));
+  C.prof.begin(KID_PROJECT);
+  launch_sanitize_hist(part.p, L[pass]->n, k, pc.p, st());
+  C.prof.end(KID_PROJECT, 8.0 * L[pass]->n);
+  i64 cut = 0;
+  {
+    DBuf<i32> none(&C.pool, 1);
+    cut = compute_cut(*L[pass], npairs, part.p, none.p, 0, LLONG_MAX);  // gpu_part.cu:1191
+  }
+  tr.push(TGP_TRACE_INIT, pass, 0, 0, L[pass]->n, npairs, cut, 0,
+          O.check_level ? hash_dev(part.p, L[pass]->n) : 0);
+  refine(pass, npairs, part, pc, &cut);                   // gpu_part.cu:1200 (B5)
+
+  for (int level = pass - 1; level >= 0; level--) {       // gpu_part.cu:1205-1274
+    const i64 nf = L[level]->n, ncs = L[level + 1]->n;
+    DBuf<i32> pf(&C.pool, nf);
+    C.prof.begin(KID_PROJECT);
+    launch_fill_i32(pf.p, nf, -1, st());
+    launch_project(L[level + 1]->tb.p, part.p, ncs, pf.p, nf, k, st());
+    HIP_TRY(hipMemsetAsync(pc.p, 0, k * sizeof(i32), st()));
+    launch_sanitize_hist(pf.p, nf, k, pc.p, st());
+    C.prof.end(KID_PROJECT, 12.0 * ncs + 12.0 * nf);
+    part = std::move(pf);
+    {
+      DBuf<i32> none(&C.pool, 1);
+      cut = compute_cut(*L[level], ecount[level], part.p, none.p, 0, LLONG_MAX);
+    }
+    refine(level, ecount[level], part, pc, &cut);
+  }
+  i64 fcut;
+  {
+    DBuf<i32> none(&C.pool, 1);
+    fcut = compute_cut(*L[0], m, part.p, none.p, 0, LLONG_MAX);  // gpu_part.cu:1285
+  }
+  HIP_TRY(hipMemcpyAsync(part_out, part.p, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(pc_out, pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  tr.push(TGP_TRACE_FINAL, 0, 0, 0, n, m, fcut, pass, O.check_level ? fnv_host(part_out, n) : 0);
+  *cut_out = fcut;
+  if (S) {
+    S->levels = pass;
+    S->passes = passes_total;
+    S->coarsest_n = L[pass]->n;
+    S->coarsest_npairs = npairs;
+    S->match_rounds = match_rounds;
+    S->seconds_initpart = initpart_s;
+    S->seconds_total =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    S->trace_len = tr.len;
+  }
+  return TGP_OK;
+}
+
+}  // namespace tgp
+
+// ====================================================================== C ABI ==
+using namespace tgp;
+
+namespace {
+
+int validate_host_graph(const tgp_graph* g) {
+  if (g->n <= 0 || g->m <= 0 || !g->xadj || !g->adjncy || !g->adjwgt) return TGP_EINVAL;
+  if (g->xadj[0] != 0 || g->xadj[g->n] != g->m) return TGP_EINVAL;
+  for (i64 i = 0; i < g->n; i++)
+    if (g->xadj[i + 1] < g->xadj[i]) return TGP_EINVAL;
+  for (i64 i = 0; i < g->m; i++)
+    if (g->adjncy[i] < 0 || g->adjncy[i] >= g->n) return TGP_EINVAL;
+  return TGP_OK;
+}
+
+// One context per process, created on first use and intentionally never
+// destroyed (its teardown would race the HIP runtime's own at exit).  Calls
+// are serialised by a mutex: one partition at a time per process.
+std::mutex g_ctx_mu;
+Context* default_context() {
+  static Context* ctx = new Context();
+  return ctx;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tgp_last_error(void) { return last_error().c_str(); }
+
+int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
+                  int64_t* edge_cut, tgp_stats* stats) {
+  last_error().clear();
+  if (!g || !o || !part || !part_count || !edge_cut) {
+    set_error("null argument");
+    return TGP_EINVAL;
+  }
+  if (o->k <= 0 || o->k > 4096) {
+    set_error("k must be in [1, 4096]");
+    return TGP_EINVAL;
+  }
+  if (!o->device_input) {
+    const int rc = validate_host_graph(g);
+    if (rc != TGP_OK) {
+      set_error("malformed CSR: need xadj[0]=0, non-decreasing offsets, xadj[n]=m>0, "
+                "columns in [0,n)");
+      return rc;
+    }
+  } else if (g->n <= 0 || g->m <= 0) {
+    set_error("empty graph");
+    return TGP_EINVAL;
+  }
+  if (!o->wide) {  // reference int32 limits (gpu_part.cu:531, 886-888)
+    if (g->n > INT_MAX || g->m > INT_MAX || g->n * (i64)o->k > (i64)INT_MAX) {
+      set_error("n, m or n*k exceeds the reference's int32 limits; use wide mode");
+      return TGP_ELIMIT;
+    }
+  }
+  if (stats) {
+    tgp_trace* tb = stats->trace;
+    const i64 tc = stats->trace_cap;
+    std::memset(stats, 0, sizeof(*stats));
+    stats->trace = tb;
+    stats->trace_cap = tc;
+  }
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  try {
+    Context* C = default_context();
+    C->prof.on = o->profile != 0;
+    for (int i = 0; i < KID_COUNT; i++) {
+      C->prof.ms[i] = 0;
+      C->prof.launches[i] = 0;
+      C->prof.bytes[i] = 0;
+    }
+    int rc;
+    {
+      Run r(*C, *o, stats);
+      rc = r.go(g, part, part_count, edge_cut);
+    }
+    C->sync();
+    if (stats) {
+      for (int i = 0; i < KID_COUNT && i < TGP_NKERNELS; i++) {
+        stats->kernel_ms[i] = C->prof.ms[i];
+        stats->kernel_launches[i] = C->prof.launches[i];
+        stats->kernel_bytes[i] = C->prof.bytes[i];
+      }
+    }
+    return rc;
+  } catch (const Fail& f) {
+    (void)hipStreamSynchronize(default_context()->s);
+    return f.code;
+  }
+}
+
+const char* tgp_kernel_name(int id) {
+  static const char* names[KID_COUNT] = {"match",  "relabel", "contract", "pairs", "project",
+                                         "gain",   "token",   "compact",  "sort",  "select",
+                                         "commit", "cut",     "scan",     "misc"};
+  return (id >= 0 && id < KID_COUNT) ? names[id] : "";
+}
+
+int tgp_graph_upload(const tgp_graph* host, tgp_graph* dev) {
+  if (!host || !dev || host->n <= 0 || host->m <= 0) return TGP_EINVAL;
+  const int rc = validate_host_graph(host);
+  if (rc != TGP_OK) return rc;
+  void *x = nullptr, *a = nullptr, *w = nullptr;
+  if (hipMalloc(&x, (host->n + 1) * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc(&a, host->m * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&w, host->m * sizeof(int32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(x);
+    (void)hipFree(a);
+    (void)hipFree(w);
+    return TGP_ENOMEM;
+  }
+  if (hipMemcpy(x, host->xadj, (host->n + 1) * sizeof(int64_t), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(a, host->adjncy, host->m * sizeof(int32_t), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(w, host->adjwgt, host->m * sizeof(int32_t), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    (void)hipFree(x);
+    (void)hipFree(a);
+    (void)hipFree(w);
+    return TGP_EHIP;
+  }
+  dev->n = host->n;
+  dev->m = host->m;
+  dev->xadj = (const int64_t*)x;
+  dev->adjncy = (const int32_t*)a;
+  dev->adjwgt = (const int32_t*)w;
+  return TGP_OK;
+}
+
+void tgp_graph_device_free(tgp_graph* dev) {
+  if (!dev) return;
+  (void)hipFree((void*)dev->xadj);
+  (void)hipFree((void*)dev->adjncy);
+  (void)hipFree((void*)dev->adjwgt);
+  dev->xadj = nullptr;
+  dev->adjncy = nullptr;
+  dev->adjwgt = nullptr;
+}
+
+void tgp_release_device_memory(void) {
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  Context* C = default_context();
+  C->sync();
+  C->pool.trim();
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// tgp_internal.h -- internal interface between the host driver and the gfx950
+// kernels of the partitioner.  Not part of the public C ABI (include/tgp.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace tgp {
+
+// Detail of the last failure on the calling thread (tgp_last_error).
+inline std::string& last_error() {
+  static thread_local std::string s;
+  return s;
+}
+
+typedef int64_t i64;
+typedef int32_t i32;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef uint8_t u8;
+
+// A CSR level resident in HBM: 64-bit row offsets, 32-bit ids and weights.
+struct DGraph {
+  i64 n;          // vertices
+  i64 m;          // arcs = xadj[n]
+  const i64* xadj;
+  const i32* adj;
+  const i32* w;
+};
+
+// Kernel families, for per-kernel timing (tgp_kernel_name).
+enum KernelId {
+  KID_MATCH = 0,   // exact heavy-edge matching rounds
+  KID_RELABEL,     // root scan + coarse ids + trace-back
+  KID_CONTRACT,    // coarse CSR construction
+  KID_PAIRS,       // unique undirected pair count of the coarsest graph
+  KID_PROJECT,     // projection + part-count histogram
+  KID_GAIN,        // gains + targets + candidates
+  KID_TOKEN,       // token passing + move flags
+  KID_COMPACT,     // move buffer compaction (scan + scatter)
+  KID_SORT,        // bitonic network
+  KID_SELECT,      // per-destination rank selection
+  KID_COMMIT,      // commit of selected moves
+  KID_CUT,         // edge cut
+  KID_SCAN,        // generic scans
+  KID_MISC,        // fills, tables, capacities
+  KID_COUNT
+};
+
+// Scratch provider for multi-level scans (device memory owned by the caller).
+struct ScanScratch {
+  i64* buf;       // at least scan_scratch_elems(n) elements
+  i64 cap;
+};
+i64 scan_scratch_elems(i64 n);
+
+// ---- generic -----------------------------------------------------------------
+void launch_fill_i32(i32* p, i64 n, i32 v, hipStream_t s);
+void launch_fill_i64(i64* p, i64 n, i64 v, hipStream_t s);
+void launch_fill_u8(u8* p, i64 n, u8 v, hipStream_t s);
+void launch_fill_u64(u64* p, i64 n, u64 v, hipStream_t s);
+// out[i] = sum(in[0..i)), *total = sum(in); out may alias nothing.
+void launch_scan_excl(const i64* in, i64* out, i64 n, i64* total, ScanScratch sc,
+                      hipStream_t s);
+// Row-degree census of a level: unit-weight flag, heavy list (range length >
+// heavy_deg, ranges end at `nedges` for the last vertex).
+void launch_level_census(DGraph g, i64 nedges, i64 heavy_deg, i32* heavy_list,
+                         i32* heavy_count, i32* nonunit_flag, hipStream_t s);
+
+// ---- coarsening ---------------------------------------------------------------
+void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i32* list,
+                       i32* counters, hipStream_t s);
+// One round: reserve/propose over the pending list, then decide/apply.
+void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* prop,
+                        i32* list_in, i32* list_out, i32* counters, u32 round,
+                        hipStream_t s);
+void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
+                    i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);
+// Reference contraction (gpu_part.cu:145-242 semantics): padded rows of 64.
+void launch_contract_ref(DGraph g, const i32* mate, const i32* new_ids, i64 n_c,
+                         i32* sadj, i32* sw, i64* cnt, hipStream_t s);
+void launch_compact_rows(const i32* sadj, const i32* sw, const i64* ox, i64 n_c,
+                         i32* oa, i32* ow, hipStream_t s);
+// Repaired contraction.
+void launch_rep_sizes(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* csz,
+                      hipStream_t s);
+void launch_rep_insert(DGraph g, const i32* tb, const i32* new_ids, i64 n_c,
+                       const i64* toff, const i64* csz, i32* tkey, i32* tpos, i32* tw,
+                       hipStream_t s);
+void launch_rep_mark(DGraph g, const i32* tb, const i32* new_ids, i64 n_c,
+                     const i64* toff, const i64* csz, const i64* eoff, const i32* tkey,
+                     const i32* tpos, i32* eslot, i64* cnt, hipStream_t s);
+void launch_rep_emit(const i64* eoff, const i64* dsz, const i32* eslot, const i64* toff,
+                     const i32* tkey, const i32* tw, const i64* ox, i64 n_c, i32* oa,
+                     i32* ow, hipStream_t s);
+// Unique undirected pairs (min,max), u != v, 0 <= u < n, true row ends.
+void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s);
+
+// ---- projection / counts ------------------------------------------------------
+void launch_project(const i32* tb, const i32* part_c, i64 n_c, i32* part_f, i64 n_f,
+                    i32 k, hipStream_t s);
+void launch_sanitize_hist(i32* part, i64 n, i32 k, i32* pc, hipStream_t s);
+void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s);
+
+// ---- refinement ---------------------------------------------------------------
+void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
+                     hipStream_t s);
+void launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+                 const u8* src_big, float* gain, i32* target, u8* cand, int phase,
+                 int unit, i32* gscratch, hipStream_t s);
+i64 gain_gscratch_elems(i32 k);
+void launch_token(DGraph g, i64 nedges, const float* gain, const u8* cand,
+                  const i32* part, const i32* target, i32 k, i64* flag,
+                  const i32* heavy_list, i64 heavy_count, i64 heavy_deg, hipStream_t s);
+void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part,
+                          const i32* target, const float* gain, i64 n, i32 k, i32* mv_v,
+                          i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
+                          hipStream_t s);
+void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase,
+                     i64* avail, hipStream_t s);
+void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32* sk_idx,
+                       hipStream_t s);
+i64 select_scratch_elems(i64 M, i32 k);
+void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64* avail,
+                   i32* sdst, i32* H, u8* sel, i32* S, i32* pads, hipStream_t s);
+void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v,
+                   const i32* mv_src, const i32* mv_dst, i32* part, i32* pc, i32 k,
+                   hipStream_t s);
+void launch_cut(DGraph g, i64 nedges, const i32* part, i32 k, u64* acc,
+                const i32* heavy_list, i64 heavy_count, i64 heavy_deg, hipStream_t s);
+
+}  // namespace tgp
