@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-scale", type=int, default=17)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run one profiled step and print per-kernel times")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -65,6 +67,16 @@ def main():
           f"(generated in {gen_s:.1f}s)", file=sys.stderr, flush=True)
     dg = tgp.DeviceGraph(g)
     kw = dict(init=tgp.INIT_BLOCK, repair=True, device_graph=dg.pointers())
+    if args.profile_only:
+        rp = tgp.partition(g, args.k, profile=True, **kw)
+        st = rp[3]
+        print(json.dumps({"n": g.n, "m": g.m, "seconds": st.seconds_total, "levels": st.levels,
+                          "passes": st.passes, "match_rounds": st.match_rounds,
+                          "cut": rp[2], "coarsest_n": st.coarsest_n,
+                          "kernel_ms": {a: round(b, 2) for a, b in st.kernel_ms.items()},
+                          "launches": st.kernel_launches}), flush=True)
+        dg.free()
+        return
 
     for i in range(args.warmup):
         r = tgp.partition(g, args.k, **kw)
@@ -73,6 +85,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         r = tgp.partition(g, args.k, **kw)  # synchronous: returns after D2H + stream sync
+        print(f"# step {i}: {time.perf_counter() - t0:.3f}s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     elapsed = t1 - t0
     part, pc, cut, st = r

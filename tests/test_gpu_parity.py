@@ -37,9 +37,13 @@ def run_both(oracle, xadj, adj, w, k, repair=False, floor=0, wide=False, init_pa
     return gp, op
 
 
+def is_error(r):
+    return isinstance(r[0], str) and r[0] == "error"
+
+
 def assert_same(gp, op, label):
-    if gp[0] == "error" or op[0] == "error":
-        assert gp[0] == op[0] == "error", f"{label}: gpu {gp[:2]} oracle {op[:2]}"
+    if is_error(gp) or is_error(op):
+        assert is_error(gp) and is_error(op), f"{label}: gpu {gp[:3]} oracle {op[:3]}"
         assert gp[1] == op[1], f"{label}: error code gpu {gp[1]} oracle {op[1]}"
         assert [tuple(t) for t in gp[2]] == [tuple(t) for t in op[2]], f"{label}: traces"
         return

@@ -234,6 +234,74 @@ def round_based_matching(xadj, adj, w):
     return mate
 
 
+def cached_min_matching(xadj, adj, w):
+    """The GPU's symmetric-level variant: cached smallest undecided neighbour
+    mp[x] and cached proposal prop[t], refreshed only when they have decided /
+    been matched; decisions use the round-start state."""
+    n = len(xadj) - 1
+    UNKNOWN, NONE = -1, 1 << 31
+    mate = [-1] * n
+    dec = [False] * n
+    mp = [UNKNOWN] * n
+    prop = [UNKNOWN] * n
+
+    def min_pending(x):
+        c = [u for u in adj[xadj[x]:xadj[x + 1]] if u < n and u != x and not dec[u]]
+        return min(c) if c else NONE
+
+    def best_free(t):
+        best = None
+        for p in range(xadj[t], xadj[t + 1]):
+            u = adj[p]
+            if u < n and u != t and mate[u] == -1 and w[p] >= 0:
+                key = (int(w[p]), -(p - xadj[t]))
+                if best is None or key > best[0]:
+                    best = (key, u)
+        return best[1] if best else -1
+
+    pending = list(range(n))
+    while pending:
+        pending = [t for t in pending if not dec[t]]
+        decision = {}
+        for t in pending:
+            d = "wait"
+            a = mp[t]
+            if a == UNKNOWN or (a != NONE and dec[a]):
+                a = mp[t] = min_pending(t)
+            if a == NONE or a > t:
+                u = prop[t]
+                if u == UNKNOWN or (u >= 0 and mate[u] != -1):
+                    u = prop[t] = best_free(t)
+                if u < 0:
+                    d = "exhaust"
+                elif u > t or dec[u]:
+                    b = mp[u]
+                    if b == UNKNOWN or (b != NONE and dec[b]):
+                        b = mp[u] = min_pending(u)
+                    if b == NONE or b >= t:
+                        d = u
+            decision[t] = d
+        for t, d in decision.items():
+            if d == "exhaust":
+                dec[t] = True
+            elif d != "wait":
+                mate[t], mate[d] = d, t
+                dec[t] = dec[d] = True
+    return mate
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_symmetric_cached_matching_equals_oracle(oracle, seed):
+    x, a, w = graphs.random_graph(200 + 20 * seed, 700, 100 + seed, sym=True,
+                                  wmax=1 if seed < 4 else 5, dup=seed in (3, 7),
+                                  selfloops=seed == 7)
+    if seed == 6:
+        w = w.copy()
+        w[::5] = -1
+    # duplicated-edge graphs are symmetric as arc sets only when built symmetric
+    assert cached_min_matching(x.tolist(), a.tolist(), w.tolist()) == oracle.match(x, a, w).tolist()
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_matching_equals_independent_formulations(oracle, seed):
     sym = seed % 2 == 0

@@ -187,6 +187,7 @@ struct Prof {
 // ---------------------------------------------------------------------- context --
 struct Level {
   i64 n = 0, m = 0;
+  bool symmetric = false;  // every arc s->x (s != x) has its reverse
   DBuf<i64> xadj;
   DBuf<i32> adj, w;
   DBuf<i32> tb;  // 2n: coarse vertex -> its fine vertices on the level above
@@ -314,6 +315,8 @@ class Run {
 
   void coarsen();
   void match_level(const Level& lv, DBuf<i32>& mate);
+  bool check_symmetric(const Level& lv);
+  void census(const Level& lv, i64 nedges, DBuf<i32>& heavy, i64* nheavy, int* unit);
   int initial_partition(DBuf<i32>& part, i64* npairs);
   i64 compute_cut(const Level& lv, i64 nedges, const i32* part, const i32* heavy, i64 nheavy,
                   i64 heavy_deg);
@@ -326,12 +329,18 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const i64 n = lv.n;
   DBuf<u8> dec(&C.pool, n);
   DBuf<i32> pend(&C.pool, n), prop(&C.pool, n), la(&C.pool, n), lb(&C.pool, n);
-  DBuf<u64> R(&C.pool, n);
   DBuf<i32> counters(&C.pool, 2);
+  DBuf<u64> R;
+  if (!lv.symmetric) R = DBuf<u64>(&C.pool, n);
   C.prof.begin(KID_MATCH);
-  launch_match_init(n, mate.p, dec.p, pend.p, R.p, la.p, counters.p, st());
+  if (lv.symmetric)
+    launch_match_sym_init(n, mate.p, dec.p, pend.p, prop.p, la.p, counters.p, st());
+  else
+    launch_match_init(n, mate.p, dec.p, pend.p, R.p, la.p, counters.p, st());
   u32 round = 0;
   const DGraph g = lv.dg();
+  DBuf<i32> decision;
+  if (lv.symmetric) decision = DBuf<i32>(&C.pool, n);
   // Rounds are issued in batches; the pending count is read back between them.
   int batch = 4;
   while (true) {
@@ -339,8 +348,12 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
       round++;
       i32* lin = (round & 1) ? la.p : lb.p;
       i32* lout = (round & 1) ? lb.p : la.p;
-      launch_match_round(g, mate.p, dec.p, pend.p, R.p, prop.p, lin, lout, counters.p, round,
-                         st());
+      if (lv.symmetric)
+        launch_match_sym_round(g, mate.p, dec.p, pend.p, prop.p, decision.p, lin, lout,
+                               counters.p, round, st());
+      else
+        launch_match_round(g, mate.p, dec.p, pend.p, R.p, prop.p, lin, lout, counters.p, round,
+                           st());
     }
     HIP_TRY(hipMemcpyAsync(C.pinned, counters.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
     HIP_TRY(hipStreamSynchronize(st()));
@@ -360,10 +373,25 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   match_rounds += round;
 }
 
+bool Run::check_symmetric(const Level& lv) {
+  i64 cap = 1;
+  while (cap < 2 * lv.m + 2) cap <<= 1;
+  DBuf<u64> table(&C.pool, cap);
+  DBuf<i32> miss(&C.pool, 1);
+  HIP_TRY(hipMemsetAsync(miss.p, 0, sizeof(i32), st()));
+  C.prof.begin(KID_MISC);
+  launch_sym_check(lv.dg(), table.p, cap, miss.p, st());
+  C.prof.end(KID_MISC, 24.0 * lv.m);
+  HIP_TRY(hipMemcpyAsync(C.pinned, miss.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  return ((const i32*)C.pinned)[0] == 0;
+}
+
 void Run::coarsen() {
   const i64 n0 = L[0]->n;
   const i64 target = O.coarsen_floor > 0 ? O.coarsen_floor : std::max<i64>(500, n0 / 500);
   i64 ncur = n0;
+  if (ncur > target) L[0]->symmetric = check_symmetric(*L[0]);
   DBuf<i32> mate(&C.pool, n0), new_ids(&C.pool, n0);
   DBuf<i64> flags(&C.pool, n0), pos(&C.pool, n0);
   while (ncur > target) {  // gpu_part.cu:1003
@@ -431,6 +459,9 @@ void Run::coarsen() {
       throw Fail(TGP_EEDGES);
     }
     c->m = m_c;
+    // the repaired contraction of a symmetric level is symmetric; the
+    // reference's contraction (row r from fine vertex r) is not
+    c->symmetric = repair && lv.symmetric;
     c->tb = std::move(tb);
     c->xadj_p = c->xadj.p;
     c->adj_p = c->adj.p;
@@ -439,6 +470,17 @@ void Run::coarsen() {
     ecount.push_back(m_c);
     ncur = n_c;
   }
+}
+
+void Run::census(const Level& lv, i64 nedges, DBuf<i32>& heavy, i64* nheavy, int* unit) {
+  heavy.ensure(&C.pool, std::max<i64>(lv.n, 1));
+  DBuf<i32> cnt32(&C.pool, 2);
+  HIP_TRY(hipMemsetAsync(cnt32.p, 0, 2 * sizeof(i32), st()));
+  launch_level_census(lv.dg(), nedges, HEAVY_DEG, heavy.p, cnt32.p, cnt32.p + 1, st());
+  HIP_TRY(hipMemcpyAsync(C.pinned, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  *nheavy = ((const i32*)C.pinned)[0];
+  if (unit) *unit = ((const i32*)C.pinned)[1] == 0;
 }
 
 i64 Run::compute_cut(const Level& lv, i64 nedges, const i32* part, const i32* heavy,
@@ -506,14 +548,10 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
   const float Tf = (float)T;
   const float maxImbalance = 0.05f;
 
-  DBuf<i32> heavy(&C.pool, std::max<i64>(n, 1));
-  DBuf<i32> cnt32(&C.pool, 8);
-  HIP_TRY(hipMemsetAsync(cnt32.p, 0, 8 * sizeof(i32), st()));
-  launch_level_census(g, nedges, HEAVY_DEG, heavy.p, cnt32.p, cnt32.p + 1, st());
-  HIP_TRY(hipMemcpyAsync(C.pinned, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
-  C.sync();
-  const i64 nheavy = ((const i32*)C.pinned)[0];
-  const int unit = ((const i32*)C.pinned)[1] == 0;
+  DBuf<i32> heavy;
+  i64 nheavy = 0;
+  int unit = 0;
+  census(lv, nedges, heavy, &nheavy, &unit);
 
   i64 bestCut = *cut;
   float bestScore = 1e9f;
@@ -662,8 +700,10 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
   C.prof.end(KID_PROJECT, 8.0 * L[pass]->n);
   i64 cut = 0;
   {
-    DBuf<i32> none(&C.pool, 1);
-    cut = compute_cut(*L[pass], npairs, part.p, none.p, 0, LLONG_MAX);  // gpu_part.cu:1191
+    DBuf<i32> hv;
+    i64 nh = 0;
+    census(*L[pass], npairs, hv, &nh, nullptr);
+    cut = compute_cut(*L[pass], npairs, part.p, hv.p, nh, HEAVY_DEG);  // gpu_part.cu:1191
   }
   tr.push(TGP_TRACE_INIT, pass, 0, 0, L[pass]->n, npairs, cut, 0,
           O.check_level ? hash_dev(part.p, L[pass]->n) : 0);
@@ -680,15 +720,19 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     C.prof.end(KID_PROJECT, 12.0 * ncs + 12.0 * nf);
     part = std::move(pf);
     {
-      DBuf<i32> none(&C.pool, 1);
-      cut = compute_cut(*L[level], ecount[level], part.p, none.p, 0, LLONG_MAX);
+      DBuf<i32> hv;
+      i64 nh = 0;
+      census(*L[level], ecount[level], hv, &nh, nullptr);
+      cut = compute_cut(*L[level], ecount[level], part.p, hv.p, nh, HEAVY_DEG);
     }
     refine(level, ecount[level], part, pc, &cut);
   }
   i64 fcut;
   {
-    DBuf<i32> none(&C.pool, 1);
-    fcut = compute_cut(*L[0], m, part.p, none.p, 0, LLONG_MAX);  // gpu_part.cu:1285
+    DBuf<i32> hv;
+    i64 nh = 0;
+    census(*L[0], m, hv, &nh, nullptr);
+    fcut = compute_cut(*L[0], m, part.p, hv.p, nh, HEAVY_DEG);  // gpu_part.cu:1285
   }
   HIP_TRY(hipMemcpyAsync(part_out, part.p, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
   HIP_TRY(hipMemcpyAsync(pc_out, pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));

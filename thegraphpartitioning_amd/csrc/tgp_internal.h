@@ -76,6 +76,14 @@ void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i32* list,
 void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* prop,
                         i32* list_in, i32* list_out, i32* counters, u32 round,
                         hipStream_t s);
+// Same matching for levels whose arcs are symmetric (cached minima, no atomics).
+void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* mp, i32* prop, i32* list,
+                           i32* counters, hipStream_t s);
+void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* mp, i32* prop, i32* decision,
+                            i32* list_in, i32* list_out, i32* counters, u32 round,
+                            hipStream_t s);
+// *missing |= 1 when some arc s->x (s != x) lacks its reverse x->s.
+void launch_sym_check(DGraph g, u64* table, i64 cap, i32* missing, hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);
 // Reference contraction (gpu_part.cu:145-242 semantics): padded rows of 64.

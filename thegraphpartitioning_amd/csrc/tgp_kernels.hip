@@ -166,30 +166,42 @@ __global__ __launch_bounds__(MATCH_BS) void k_match_reserve(DGraph g, const i32*
   const int lane = lane_id();
   const i32 cnt = *cnt_in;
   const u64 mykey_hi = (u64)round << 32;
-  for (i64 e = gwave(); e < cnt; e += nwaves()) {
-    const i32 t = list_in[e];
-    if (dec[t]) continue;
-    if (lane == 0) {
-      const i32 p = atomicAdd(cnt_out, 1);
-      list_out[p] = t;
-      pend[t] = (i32)round;
+  for (i64 base = gwave() * WAVE; base < cnt; base += nwaves() * WAVE) {
+    const i64 e = base + lane;
+    const i32 t0 = e < cnt ? list_in[e] : -1;
+    const bool alive = t0 >= 0 && !dec[t0];
+    u64 am = __ballot(alive);
+    if (!am) continue;
+    i32 obase = 0;
+    if (lane == 0) obase = atomicAdd(cnt_out, __popcll(am));
+    obase = __shfl(obase, 0);
+    if (alive) {
+      list_out[obase + __popcll(am & lanes_below(lane))] = t0;
+      pend[t0] = (i32)round;
     }
-    const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
-    const u64 key = mykey_hi | (u64)(0xFFFFFFFFu - (u32)t);
-    u64 best = 0;
-    for (i64 i = s0 + lane; i < s1; i += WAVE) {
-      const i32 u = g.adj[i];
-      if (u < g.n && u != t && mate[u] == -1) {
-        atomicMax((unsigned long long*)&R[u], (unsigned long long)key);
-        const i32 w = g.w[i];
-        if (w >= 0) {  // gpu_part.cu:62,65: weight > maxWeight with maxWeight = -1
-          const u64 c = ((u64)(u32)w << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
-          best = c > best ? c : best;
+    // every surviving vertex reserves its free neighbours and proposes its
+    // heaviest free neighbour; rows are scanned by the whole wave
+    while (am) {
+      const int l = __ffsll((unsigned long long)am) - 1;
+      const i32 t = __shfl(t0, l);
+      const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
+      const u64 key = mykey_hi | (u64)(0xFFFFFFFFu - (u32)t);
+      u64 best = 0;
+      for (i64 i = s0 + lane; i < s1; i += WAVE) {
+        const i32 u = g.adj[i];
+        if (u < g.n && u != t && mate[u] == -1) {
+          atomicMax((unsigned long long*)&R[u], (unsigned long long)key);
+          const i32 w = g.w[i];
+          if (w >= 0) {  // gpu_part.cu:62,65: weight > maxWeight with maxWeight = -1
+            const u64 c = ((u64)(u32)w << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+            best = c > best ? c : best;
+          }
         }
       }
+      best = wave_max_u64(best);
+      if (lane == l) prop[t] = best ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)best)] : -1;
+      am &= am - 1;
     }
-    best = wave_max_u64(best);
-    if (lane == 0) prop[t] = best ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)best)] : -1;
   }
 }
 
@@ -232,6 +244,202 @@ __global__ void k_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i32* 
 __global__ void k_set_counters(i32* c, i32 a, i32 b) {
   c[0] = a;
   c[1] = b;
+}
+
+// Symmetric levels (every arc s->x has its reverse x->s): the vertices that
+// reserve x in a round are exactly x's undecided neighbours, so the round rule
+// above needs only mp[x] = smallest undecided neighbour of x.  mp[x] is cached
+// and stays exact while that neighbour is undecided (decisions are permanent,
+// the undecided set only shrinks); it is recomputed when it has decided.  The
+// proposal prop[t] (heaviest free neighbour, first position) is cached the same
+// way: it stays exact while it is free.  A blocked vertex therefore costs one
+// or two loads per round instead of a row scan with atomics.
+constexpr i32 MP_UNKNOWN = -1;
+constexpr i32 MP_NONE = INT_MAX;
+
+__device__ __forceinline__ i32 wave_min_pending(const DGraph& g, i32 x, const u8* dec, int lane) {
+  const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
+  i32 best = MP_NONE;
+  for (i64 i = s0 + lane; i < s1; i += WAVE) {
+    const i32 u = g.adj[i];
+    if (u < g.n && u != x && !dec[u]) best = u < best ? u : best;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const i32 t = __shfl_xor(best, o);
+    best = t < best ? t : best;
+  }
+  return best;
+}
+
+__device__ __forceinline__ i32 wave_best_free(const DGraph& g, i32 t, const i32* mate, int lane) {
+  const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
+  u64 best = 0;
+  for (i64 i = s0 + lane; i < s1; i += WAVE) {
+    const i32 u = g.adj[i];
+    if (u < g.n && u != t && mate[u] == -1) {
+      const i32 w = g.w[i];
+      if (w >= 0) {
+        const u64 c = ((u64)(u32)w << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+        best = c > best ? c : best;
+      }
+    }
+  }
+  best = wave_max_u64(best);
+  return best ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)best)] : -1;
+}
+
+// decision codes
+constexpr i32 DEC_WAIT = -2;
+constexpr i32 DEC_EXHAUST = -1;
+
+// One lane per pending vertex; survivors are appended with one atomic per wave.
+// Lanes whose cached minimum or proposal must be refreshed are served one at
+// a time by a cooperative scan of the whole wave.
+__global__ __launch_bounds__(MATCH_BS) void k_match_sym_decide(DGraph g, const i32* mate,
+                                                               const u8* dec, i32* mp,
+                                                               i32* prop, i32* decision,
+                                                               const i32* list_in,
+                                                               const i32* cnt_in, i32* list_out,
+                                                               i32* cnt_out) {
+  const int lane = lane_id();
+  const i32 cnt = *cnt_in;
+  for (i64 base = gwave() * WAVE; base < cnt; base += nwaves() * WAVE) {
+    const i64 e = base + lane;
+    const i32 t = e < cnt ? list_in[e] : -1;
+    const bool alive = t >= 0 && !dec[t];
+    const u64 am = __ballot(alive);
+    if (!am) continue;
+    i32 obase = 0;
+    if (lane == 0) obase = atomicAdd(cnt_out, __popcll(am));
+    obase = __shfl(obase, 0);
+    if (alive) list_out[obase + __popcll(am & lanes_below(lane))] = t;
+
+    // condition A: no undecided neighbour below t
+    i32 a = alive ? mp[t] : MP_NONE;
+    u64 need = __ballot(alive && (a == MP_UNKNOWN || (a != MP_NONE && dec[a])));
+    while (need) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(t, l);
+      const i32 r = wave_min_pending(g, x, dec, lane);
+      if (lane == l) {
+        a = r;
+        mp[x] = r;
+      }
+      need &= need - 1;
+    }
+    const bool okA = alive && (a == MP_NONE || a > t);
+    // proposal: heaviest free neighbour (cached while it stays free)
+    i32 u = okA ? prop[t] : -1;
+    need = __ballot(okA && (u == MP_UNKNOWN || (u >= 0 && mate[u] != -1)));
+    while (need) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(t, l);
+      const i32 r = wave_best_free(g, x, mate, lane);
+      if (lane == l) {
+        u = r;
+        prop[x] = r;
+      }
+      need &= need - 1;
+    }
+    // condition B (u above t or already decided) and C (no undecided
+    // neighbour of u below t)
+    const bool tryC = okA && u >= 0 && (u > t || dec[u]);
+    i32 b = tryC ? mp[u] : MP_NONE;
+    need = __ballot(tryC && (b == MP_UNKNOWN || (b != MP_NONE && dec[b])));
+    while (need) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(u, l);
+      const i32 r = wave_min_pending(g, x, dec, lane);
+      if (lane == l) {
+        b = r;
+        mp[x] = r;
+      }
+      need &= need - 1;
+    }
+    if (alive) {
+      i32 d = DEC_WAIT;
+      if (okA) {
+        if (u < 0) d = DEC_EXHAUST;
+        else if (tryC && (b == MP_NONE || b >= t)) d = u;
+      }
+      decision[t] = d;
+    }
+  }
+}
+
+__global__ __launch_bounds__(MATCH_BS) void k_match_sym_apply(const i32* list, const i32* cnt,
+                                                              const i32* decision, i32* mate,
+                                                              u8* dec, i32* cnt_reset) {
+  if (gtid() == 0) *cnt_reset = 0;
+  const i32 c = *cnt;
+  for (i64 e = gtid(); e < c; e += gstride()) {
+    const i32 t = list[e];
+    const i32 d = decision[t];
+    if (d == DEC_EXHAUST) {
+      dec[t] = 1;
+    } else if (d >= 0) {
+      mate[t] = d;
+      mate[d] = t;
+      dec[t] = 1;
+      dec[d] = 1;
+    }
+  }
+}
+
+__global__ void k_match_sym_init(i64 n, i32* mate, u8* dec, i32* mp, i32* prop, i32* list) {
+  for (i64 i = gtid(); i < n; i += gstride()) {
+    mate[i] = -1;
+    dec[i] = 0;
+    mp[i] = MP_UNKNOWN;
+    prop[i] = MP_UNKNOWN;
+    list[i] = (i32)i;
+  }
+}
+
+// Symmetry test of a level: every arc s->x (s != x, x < n) must have x->s.
+__global__ __launch_bounds__(256) void k_sym_insert(DGraph g, u64* table, i64 cap) {
+  const u64 mask = (u64)cap - 1;
+  const int lane = lane_id();
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      if (u < 0 || u >= g.n || u == v) continue;
+      const u64 key = ((u64)v << 32) | (u64)(u32)u;
+      u64 h = hash64(key) & mask;
+      while (true) {
+        const u64 old = atomicCAS((unsigned long long*)&table[h], ~0ull, (unsigned long long)key);
+        if (old == ~0ull || old == key) break;
+        h = (h + 1) & mask;
+      }
+    }
+  }
+}
+__global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, i64 cap,
+                                                   i32* missing) {
+  const u64 mask = (u64)cap - 1;
+  const int lane = lane_id();
+  bool miss = false;
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      if (u < 0 || u >= g.n || u == v) continue;
+      const u64 key = ((u64)(u32)u << 32) | (u64)v;
+      u64 h = hash64(key) & mask;
+      while (true) {
+        const u64 cur = table[h];
+        if (cur == key) break;
+        if (cur == ~0ull) {
+          miss = true;
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+    }
+  }
+  if (__ballot(miss) && lane == 0) atomicOr(missing, 1);
 }
 
 // ------------------------------------------------------------------ relabel --
@@ -1068,12 +1276,35 @@ void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* pr
   // counters[(round-1)&1] holds the input list length, counters[round&1] the output.
   i32* cin = counters + ((round - 1) & 1);
   i32* cout = counters + (round & 1);
-  const unsigned gr = grid_for(g.n, MATCH_BS / WAVE, 4096);
+  const unsigned gr = grid_for(g.n, MATCH_BS, 4096);
   k_match_reserve<<<gr, MATCH_BS, 0, s>>>(g, mate, dec, pend, R, prop, list_in, cin, list_out,
                                           cout, round);
   k_match_decide<<<grid_for(g.n, MATCH_BS, 2048), MATCH_BS, 0, s>>>(list_out, cout, R, prop,
                                                                      pend, mate, dec, cin,
                                                                      round);
+}
+
+void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* mp, i32* prop, i32* list,
+                           i32* counters, hipStream_t s) {
+  k_match_sym_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, mp, prop, list);
+  k_set_counters<<<1, 1, 0, s>>>(counters, (i32)n, 0);
+}
+
+void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* mp, i32* prop, i32* decision,
+                            i32* list_in, i32* list_out, i32* counters, u32 round,
+                            hipStream_t s) {
+  i32* cin = counters + ((round - 1) & 1);
+  i32* cout = counters + (round & 1);
+  k_match_sym_decide<<<grid_for(g.n, MATCH_BS, 4096), MATCH_BS, 0, s>>>(
+      g, mate, dec, mp, prop, decision, list_in, cin, list_out, cout);
+  k_match_sym_apply<<<grid_for(g.n, MATCH_BS, 4096), MATCH_BS, 0, s>>>(list_out, cout, decision,
+                                                                      mate, dec, cin);
+}
+
+void launch_sym_check(DGraph g, u64* table, i64 cap, i32* missing, hipStream_t s) {
+  k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
+  k_sym_insert<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, cap);
+  k_sym_check<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, cap, missing);
 }
 
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total, i32* new_ids,
