@@ -191,7 +191,7 @@ def greedy_over_sorted_arcs(xadj, adj, w):
     return mate
 
 
-def round_based_matching(xadj, adj, w):
+def round_based_matching(xadj, adj, w, window=None):
     """The GPU's round algorithm (deterministic reservations), sequentially."""
     n = len(xadj) - 1
     mate = [-1] * n
@@ -203,7 +203,13 @@ def round_based_matching(xadj, adj, w):
         rnd += 1
         R = {}
         prop = {}
-        for t in pending:
+        if window is not None:
+            pending = [t for t in pending if t < pending[0] + window] + \
+                      [t for t in pending if t >= pending[0] + window]
+            evaluate = [t for t in pending if t < pending[0] + window]
+        else:
+            evaluate = pending
+        for t in evaluate:
             pend[t] = rnd
             best = None
             for p in range(xadj[t], xadj[t + 1]):
@@ -216,7 +222,7 @@ def round_based_matching(xadj, adj, w):
                             best = (key, u)
             prop[t] = best[1] if best else -1
         commits, exh = [], []
-        for t in pending:
+        for t in evaluate:
             a_ok = R.get(t, n) > t
             u = prop[t]
             if u < 0:
@@ -234,20 +240,22 @@ def round_based_matching(xadj, adj, w):
     return mate
 
 
-def cached_min_matching(xadj, adj, w):
-    """The GPU's symmetric-level variant: cached smallest undecided neighbour
-    mp[x] and cached proposal prop[t], refreshed only when they have decided /
-    been matched; decisions use the round-start state."""
+def blocker_matching(xadj, adj, w, window=None):
+    """The GPU's symmetric-level variant: a waiting vertex keeps one blocker
+    (the largest undecided vertex violating its condition) and a cached
+    proposal, refreshed only after the blocker decided / the proposal was
+    matched; decisions use the round-start state."""
     n = len(xadj) - 1
-    UNKNOWN, NONE = -1, 1 << 31
+    UNKNOWN, NONE = -1, -2
     mate = [-1] * n
     dec = [False] * n
-    mp = [UNKNOWN] * n
+    blk_a = [UNKNOWN] * n
+    blk_c = [UNKNOWN] * n
     prop = [UNKNOWN] * n
 
-    def min_pending(x):
-        c = [u for u in adj[xadj[x]:xadj[x + 1]] if u < n and u != x and not dec[u]]
-        return min(c) if c else NONE
+    def max_pending_below(x, lim):
+        c = [u for u in adj[xadj[x]:xadj[x + 1]] if u < lim and u != x and not dec[u]]
+        return max(c) if c else NONE
 
     def best_free(t):
         best = None
@@ -259,26 +267,33 @@ def cached_min_matching(xadj, adj, w):
                     best = (key, u)
         return best[1] if best else -1
 
+    def stale(b):
+        return b == UNKNOWN or (b >= 0 and dec[b])
+
     pending = list(range(n))
     while pending:
         pending = [t for t in pending if not dec[t]]
+        if window is not None and pending:
+            lo = pending[0]
+            pending_eval = [t for t in pending if t < lo + window]
+        else:
+            pending_eval = pending
         decision = {}
-        for t in pending:
+        for t in pending_eval:
             d = "wait"
-            a = mp[t]
-            if a == UNKNOWN or (a != NONE and dec[a]):
-                a = mp[t] = min_pending(t)
-            if a == NONE or a > t:
+            if stale(blk_a[t]):
+                blk_a[t] = max_pending_below(t, t)
+            if blk_a[t] == NONE:
                 u = prop[t]
                 if u == UNKNOWN or (u >= 0 and mate[u] != -1):
                     u = prop[t] = best_free(t)
+                    blk_c[t] = UNKNOWN
                 if u < 0:
                     d = "exhaust"
                 elif u > t or dec[u]:
-                    b = mp[u]
-                    if b == UNKNOWN or (b != NONE and dec[b]):
-                        b = mp[u] = min_pending(u)
-                    if b == NONE or b >= t:
+                    if stale(blk_c[t]):
+                        blk_c[t] = max_pending_below(u, t)
+                    if blk_c[t] == NONE:
                         d = u
             decision[t] = d
         for t, d in decision.items():
@@ -291,7 +306,7 @@ def cached_min_matching(xadj, adj, w):
 
 
 @pytest.mark.parametrize("seed", range(8))
-def test_symmetric_cached_matching_equals_oracle(oracle, seed):
+def test_symmetric_blocker_matching_equals_oracle(oracle, seed):
     x, a, w = graphs.random_graph(200 + 20 * seed, 700, 100 + seed, sym=True,
                                   wmax=1 if seed < 4 else 5, dup=seed in (3, 7),
                                   selfloops=seed == 7)
@@ -299,7 +314,9 @@ def test_symmetric_cached_matching_equals_oracle(oracle, seed):
         w = w.copy()
         w[::5] = -1
     # duplicated-edge graphs are symmetric as arc sets only when built symmetric
-    assert cached_min_matching(x.tolist(), a.tolist(), w.tolist()) == oracle.match(x, a, w).tolist()
+    ref = oracle.match(x, a, w).tolist()
+    assert blocker_matching(x.tolist(), a.tolist(), w.tolist()) == ref
+    assert blocker_matching(x.tolist(), a.tolist(), w.tolist(), window=5) == ref
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -313,6 +330,7 @@ def test_matching_equals_independent_formulations(oracle, seed):
     m = oracle.match(x, a, w).tolist()
     assert m == greedy_over_sorted_arcs(x.tolist(), a.tolist(), w.tolist())
     assert m == round_based_matching(x.tolist(), a.tolist(), w.tolist())
+    assert m == round_based_matching(x.tolist(), a.tolist(), w.tolist(), window=7)
     for v, u in enumerate(m):
         if u >= 0:
             assert m[u] == v

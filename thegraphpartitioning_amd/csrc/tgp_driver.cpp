@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -185,9 +186,24 @@ struct Prof {
 };
 
 // ---------------------------------------------------------------------- context --
+// Per-level data derived once for the refinement kernels (valid for one value
+// of the caller's nedges, which only differs from m on the coarsest level).
+struct LevelAux {
+  bool valid = false;
+  i64 nedges = -1;
+  i64 e_end = 0;      // arcs [0, e_end) = the reference's row ranges
+  i64 nheavy = 0;     // rows longer than HEAVY_DEG
+  int unit = 0;       // all weights are 1
+  DBuf<i32> asrc;     // arc -> source vertex
+  DBuf<i32> heavy;    // heavy rows
+  DBuf<i32> medium, huge;  // heavy rows split at 4096 arcs
+  i64 nmedium = 0, nhuge = 0;
+};
+
 struct Level {
   i64 n = 0, m = 0;
   bool symmetric = false;  // every arc s->x (s != x) has its reverse
+  LevelAux aux;
   DBuf<i64> xadj;
   DBuf<i32> adj, w;
   DBuf<i32> tb;  // 2n: coarse vertex -> its fine vertices on the level above
@@ -316,53 +332,61 @@ class Run {
   void coarsen();
   void match_level(const Level& lv, DBuf<i32>& mate);
   bool check_symmetric(const Level& lv);
-  void census(const Level& lv, i64 nedges, DBuf<i32>& heavy, i64* nheavy, int* unit);
   int initial_partition(DBuf<i32>& part, i64* npairs);
-  i64 compute_cut(const Level& lv, i64 nedges, const i32* part, const i32* heavy, i64 nheavy,
-                  i64 heavy_deg);
+  i64 compute_cut(Level& lv, i64 nedges, const i32* part);
+  LevelAux& aux(Level& lv, i64 nedges);
   void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut);
 };
 
 static const i64 HEAVY_DEG = 64;
 
+static i64 env_i64(const char* name, i64 dflt) {
+  const char* v = std::getenv(name);
+  return (v && *v) ? std::atoll(v) : dflt;
+}
+
 void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const i64 n = lv.n;
+  static const i64 MATCH_WINDOW = std::max<i64>(64, env_i64("TGP_MATCH_WINDOW", 16384));
+  static const bool trace_match = env_i64("TGP_MATCH_TRACE", 0) != 0;
+  const auto t_start = std::chrono::steady_clock::now();
   DBuf<u8> dec(&C.pool, n);
-  DBuf<i32> pend(&C.pool, n), prop(&C.pool, n), la(&C.pool, n), lb(&C.pool, n);
-  DBuf<i32> counters(&C.pool, 2);
+  DBuf<i32> s1(&C.pool, n), prop(&C.pool, n);
+  DBuf<i64> lo(&C.pool, 3);  // [0] window low end, [1..2] scan starts by round parity
   DBuf<u64> R;
-  if (!lv.symmetric) R = DBuf<u64>(&C.pool, n);
+  DBuf<i32> s2;
+  if (lv.symmetric) s2 = DBuf<i32>(&C.pool, n);
+  else R = DBuf<u64>(&C.pool, n);
+  DBuf<i32> decision, req, nreq;
+  if (lv.symmetric) {
+    decision = DBuf<i32>(&C.pool, n);
+    req = DBuf<i32>(&C.pool, 2 * 3 * std::min<i64>(MATCH_WINDOW, n) + 2);
+    nreq = DBuf<i32>(&C.pool, 2);
+    HIP_TRY(hipMemsetAsync(nreq.p, 0, 2 * sizeof(i32), st()));
+  }
   C.prof.begin(KID_MATCH);
   if (lv.symmetric)
-    launch_match_sym_init(n, mate.p, dec.p, pend.p, prop.p, la.p, counters.p, st());
+    launch_match_sym_init(n, mate.p, dec.p, s1.p, s2.p, prop.p, lo.p, st());
   else
-    launch_match_init(n, mate.p, dec.p, pend.p, R.p, la.p, counters.p, st());
+    launch_match_init(n, mate.p, dec.p, s1.p, R.p, lo.p, st());
   u32 round = 0;
   const DGraph g = lv.dg();
-  DBuf<i32> decision;
-  if (lv.symmetric) decision = DBuf<i32>(&C.pool, n);
-  // Rounds are issued in batches; the pending count is read back between them.
+  // Rounds are issued in batches; the window's low end is read back between them.
   int batch = 4;
   while (true) {
     for (int b = 0; b < batch; b++) {
       round++;
-      i32* lin = (round & 1) ? la.p : lb.p;
-      i32* lout = (round & 1) ? lb.p : la.p;
       if (lv.symmetric)
-        launch_match_sym_round(g, mate.p, dec.p, pend.p, prop.p, decision.p, lin, lout,
-                               counters.p, round, st());
+        launch_match_sym_round(g, mate.p, dec.p, s1.p, s2.p, prop.p, decision.p, lo.p,
+                               MATCH_WINDOW, req.p, nreq.p, round, st());
       else
-        launch_match_round(g, mate.p, dec.p, pend.p, R.p, prop.p, lin, lout, counters.p, round,
-                           st());
+        launch_match_round(g, mate.p, dec.p, s1.p, R.p, prop.p, lo.p, MATCH_WINDOW, round, st());
     }
-    HIP_TRY(hipMemcpyAsync(C.pinned, counters.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+    HIP_TRY(hipMemcpyAsync(C.pinned, lo.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
     HIP_TRY(hipStreamSynchronize(st()));
-    const i32* c32 = (const i32*)C.pinned;
-    // counters[round&1] = pending count at the start of the last round
-    const i32 pending_last = c32[round & 1];
-    if (pending_last == 0) break;
-    // Each round decides at least the smallest pending vertex, so more rounds
-    // than vertices means a broken invariant: stop instead of spinning.
+    if (C.pinned[0] >= n) break;
+    // Each round decides at least the smallest undecided vertex, so more
+    // rounds than vertices means a broken invariant: stop instead of spinning.
     if ((i64)round > n + 2 * 64) {
       set_error("matching made no progress");
       throw Fail(TGP_EHIP);
@@ -371,6 +395,11 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   }
   C.prof.end(KID_MATCH, (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n);
   match_rounds += round;
+  if (trace_match)
+    std::fprintf(stderr, "# match n=%lld m=%lld symmetric=%d rounds=%u ms=%.2f\n", (long long)n,
+                 (long long)lv.m, (int)lv.symmetric, round,
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                           t_start).count());
 }
 
 bool Run::check_symmetric(const Level& lv) {
@@ -472,23 +501,44 @@ void Run::coarsen() {
   }
 }
 
-void Run::census(const Level& lv, i64 nedges, DBuf<i32>& heavy, i64* nheavy, int* unit) {
-  heavy.ensure(&C.pool, std::max<i64>(lv.n, 1));
+LevelAux& Run::aux(Level& lv, i64 nedges) {
+  LevelAux& a = lv.aux;
+  if (a.valid && a.nedges == nedges) return a;
+  a.heavy.ensure(&C.pool, std::max<i64>(lv.n, 1));
+  a.asrc.ensure(&C.pool, std::max<i64>(lv.m, 1));
   DBuf<i32> cnt32(&C.pool, 2);
   HIP_TRY(hipMemsetAsync(cnt32.p, 0, 2 * sizeof(i32), st()));
-  launch_level_census(lv.dg(), nedges, HEAVY_DEG, heavy.p, cnt32.p, cnt32.p + 1, st());
+  C.prof.begin(KID_MISC);
+  launch_level_census(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, cnt32.p, cnt32.p + 1, st());
+  launch_arc_source(lv.dg(), a.asrc.p, st());
+  C.prof.end(KID_MISC, 12.0 * lv.m);
   HIP_TRY(hipMemcpyAsync(C.pinned, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned + 1, lv.xadj_p + (lv.n - 1), sizeof(i64),
+                         hipMemcpyDeviceToHost, st()));
   C.sync();
-  *nheavy = ((const i32*)C.pinned)[0];
-  if (unit) *unit = ((const i32*)C.pinned)[1] == 0;
+  a.nheavy = ((const i32*)C.pinned)[0];
+  a.unit = ((const i32*)C.pinned)[1] == 0;
+  a.medium.ensure(&C.pool, std::max<i64>(a.nheavy, 1));
+  a.huge.ensure(&C.pool, std::max<i64>(a.nheavy, 1));
+  HIP_TRY(hipMemsetAsync(cnt32.p, 0, 2 * sizeof(i32), st()));
+  launch_split_heavy(lv.dg(), nedges, a.heavy.p, a.nheavy, a.medium.p, cnt32.p, a.huge.p,
+                     cnt32.p + 1, st());
+  HIP_TRY(hipMemcpyAsync(C.pinned + 2, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  a.nmedium = ((const i32*)(C.pinned + 2))[0];
+  a.nhuge = ((const i32*)(C.pinned + 2))[1];
+  a.e_end = std::max<i64>(C.pinned[1], nedges);  // rows 0..n-2 whole, last ends at nedges
+  a.nedges = nedges;
+  a.valid = true;
+  return a;
 }
 
-i64 Run::compute_cut(const Level& lv, i64 nedges, const i32* part, const i32* heavy,
-                     i64 nheavy, i64 heavy_deg) {
+i64 Run::compute_cut(Level& lv, i64 nedges, const i32* part) {
+  LevelAux& a = aux(lv, nedges);
   C.prof.begin(KID_CUT);
   HIP_TRY(hipMemsetAsync(small.p + 1, 0, sizeof(i64), st()));
-  launch_cut(lv.dg(), nedges, part, k, (u64*)(small.p + 1), heavy, nheavy, heavy_deg, st());
-  C.prof.end(KID_CUT, 12.0 * lv.m + 12.0 * lv.n);
+  launch_cut_arcs(lv.dg(), a.e_end, a.asrc.p, part, k, a.unit, (u64*)(small.p + 1), st());
+  C.prof.end(KID_CUT, (a.unit ? 12.0 : 16.0) * a.e_end);
   return wrap_cut((u64)read_i64(small.p + 1));
 }
 
@@ -548,10 +598,10 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
   const float Tf = (float)T;
   const float maxImbalance = 0.05f;
 
-  DBuf<i32> heavy;
-  i64 nheavy = 0;
-  int unit = 0;
-  census(lv, nedges, heavy, &nheavy, &unit);
+  LevelAux& ax = aux(lv, nedges);
+  const i32* heavy = ax.heavy.p;
+  const i64 nheavy = ax.nheavy;
+  const int unit = ax.unit;
 
   i64 bestCut = *cut;
   float bestScore = 1e9f;
@@ -582,13 +632,15 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       for (int pass = 0; pass < maxPass && !(phase && balanced); pass++) {
         HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
         HIP_TRY(hipMemsetAsync(ctr.p, 0, 4 * sizeof(i32), st()));
-        C.prof.begin(KID_GAIN);
+        C.prof.begin(KID_MISC);
         launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
-        launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p, phase,
-                    unit, gscr.p, st());
+        C.prof.end(KID_MISC, 0);
+        C.prof.begin(KID_GAIN);
+        launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p, phase, unit,
+                    gscr.p, ax.medium.p, ax.nmedium, ax.huge.p, ax.nhuge, st());
         C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
         C.prof.begin(KID_TOKEN);
-        launch_token(g, nedges, gain.p, cand.p, part.p, target.p, k, flag.p, heavy.p, nheavy,
+        launch_token(g, nedges, gain.p, cand.p, part.p, target.p, k, flag.p, heavy, nheavy,
                      HEAVY_DEG, st());
         C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
         C.prof.begin(KID_COMPACT);
@@ -620,8 +672,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         C.prof.end(KID_COMMIT, 17.0 * M);
         C.prof.begin(KID_CUT);
         HIP_TRY(hipMemsetAsync(small.p + 1, 0, sizeof(i64), st()));
-        launch_cut(g, nedges, part.p, k, (u64*)(small.p + 1), heavy.p, nheavy, HEAVY_DEG, st());
-        C.prof.end(KID_CUT, arc_b * lv.m + 12.0 * n);
+        launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, (u64*)(small.p + 1), st());
+        C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
         HIP_TRY(hipMemcpyAsync(C.pinned, small.p + 1, sizeof(i64), hipMemcpyDeviceToHost, st()));
         HIP_TRY(hipMemcpyAsync(C.pinned + 1, ctr.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
         HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
@@ -653,7 +705,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       if (phase == 0) *cut = bestCut;
     }
   }
-  *cut = compute_cut(lv, nedges, part.p, heavy.p, nheavy, HEAVY_DEG);  // gpu_part.cu:749
+  *cut = compute_cut(lv, nedges, part.p);  // gpu_part.cu:749
   passes_total += passes;
   tr.push(TGP_TRACE_REFINE, level, 0, passes, n, nedges, *cut, 0,
           O.check_level ? hash_dev(part.p, n) : 0);
@@ -698,13 +750,7 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
   C.prof.begin(KID_PROJECT);
   launch_sanitize_hist(part.p, L[pass]->n, k, pc.p, st());
   C.prof.end(KID_PROJECT, 8.0 * L[pass]->n);
-  i64 cut = 0;
-  {
-    DBuf<i32> hv;
-    i64 nh = 0;
-    census(*L[pass], npairs, hv, &nh, nullptr);
-    cut = compute_cut(*L[pass], npairs, part.p, hv.p, nh, HEAVY_DEG);  // gpu_part.cu:1191
-  }
+  i64 cut = compute_cut(*L[pass], npairs, part.p);  // gpu_part.cu:1191
   tr.push(TGP_TRACE_INIT, pass, 0, 0, L[pass]->n, npairs, cut, 0,
           O.check_level ? hash_dev(part.p, L[pass]->n) : 0);
   refine(pass, npairs, part, pc, &cut);                   // gpu_part.cu:1200 (B5)
@@ -719,21 +765,10 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     launch_sanitize_hist(pf.p, nf, k, pc.p, st());
     C.prof.end(KID_PROJECT, 12.0 * ncs + 12.0 * nf);
     part = std::move(pf);
-    {
-      DBuf<i32> hv;
-      i64 nh = 0;
-      census(*L[level], ecount[level], hv, &nh, nullptr);
-      cut = compute_cut(*L[level], ecount[level], part.p, hv.p, nh, HEAVY_DEG);
-    }
+    cut = compute_cut(*L[level], ecount[level], part.p);
     refine(level, ecount[level], part, pc, &cut);
   }
-  i64 fcut;
-  {
-    DBuf<i32> hv;
-    i64 nh = 0;
-    census(*L[0], m, hv, &nh, nullptr);
-    fcut = compute_cut(*L[0], m, part.p, hv.p, nh, HEAVY_DEG);  // gpu_part.cu:1285
-  }
+  const i64 fcut = compute_cut(*L[0], m, part.p);  // gpu_part.cu:1285
   HIP_TRY(hipMemcpyAsync(part_out, part.p, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
   HIP_TRY(hipMemcpyAsync(pc_out, pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();

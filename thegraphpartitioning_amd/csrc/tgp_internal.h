@@ -70,17 +70,18 @@ void launch_level_census(DGraph g, i64 nedges, i64 heavy_deg, i32* heavy_list,
                          i32* heavy_count, i32* nonunit_flag, hipStream_t s);
 
 // ---- coarsening ---------------------------------------------------------------
-void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i32* list,
-                       i32* counters, hipStream_t s);
-// One round: reserve/propose over the pending list, then decide/apply.
-void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* prop,
-                        i32* list_in, i32* list_out, i32* counters, u32 round,
-                        hipStream_t s);
-// Same matching for levels whose arcs are symmetric (cached minima, no atomics).
-void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* mp, i32* prop, i32* list,
-                           i32* counters, hipStream_t s);
-void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* mp, i32* prop, i32* decision,
-                            i32* list_in, i32* list_out, i32* counters, u32 round,
+// Exact matching in rounds over the id window [lo, lo + W) (lo = smallest
+// undecided id, kept in *lo on the device; the matching is done when *lo == n).
+void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i64* lo, hipStream_t s);
+void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* prop, i64* lo,
+                        i64 W, u32 round, hipStream_t s);
+// Same matching for levels whose arcs are symmetric (cached blockers, no atomics).
+void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop, i64* lo,
+                           hipStream_t s);
+// req/nreq: queue of deferred long-row scans (W pairs) and its two counters,
+// both zeroed by the caller before round 1.
+void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
+                            i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s);
 // *missing |= 1 when some arc s->x (s != x) lacks its reverse x->s.
 void launch_sym_check(DGraph g, u64* table, i64 cap, i32* missing, hipStream_t s);
@@ -115,9 +116,17 @@ void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s);
 // ---- refinement ---------------------------------------------------------------
 void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
                      hipStream_t s);
+// Per-pass order of parts by (balance term desc, part asc) and run ends.
+void launch_bg_order(const float* bg, i32 k, i32* ord, i32* run_end, hipStream_t s);
+// Heavy rows (> 64 arcs) split into medium (<= 4096) and huge lists.
+void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
+                        i32* nmedium, i32* huge, i32* nhuge, hipStream_t s);
+// Gains: 8 lanes per light row, a wave per medium row, a block per huge row
+// (k <= 512); one wave per vertex for larger k.
 void launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
-                 const u8* src_big, float* gain, i32* target, u8* cand, int phase,
-                 int unit, i32* gscratch, hipStream_t s);
+                 const u8* src_big, float* gain, i32* target, u8* cand, int phase, int unit,
+                 i32* gscratch, const i32* medium, i64 nmedium, const i32* huge, i64 nhuge,
+                 hipStream_t s);
 i64 gain_gscratch_elems(i32 k);
 void launch_token(DGraph g, i64 nedges, const float* gain, const u8* cand,
                   const i32* part, const i32* target, i32 k, i64* flag,
@@ -136,6 +145,12 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v,
                    const i32* mv_src, const i32* mv_dst, i32* part, i32* pc, i32 k,
                    hipStream_t s);
+// asrc[i] = source vertex of arc i (built once per level)
+void launch_arc_source(DGraph g, i32* asrc, hipStream_t s);
+// Edge cut over arcs [0, e_end) with e_end = max(xadj[n-1], nedges): exactly
+// the arcs of the reference's row ranges (gpu_part.cu:404-417).
+void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 k, int unit,
+                     u64* acc, hipStream_t s);
 void launch_cut(DGraph g, i64 nedges, const i32* part, i32 k, u64* acc,
                 const i32* heavy_list, i64 heavy_count, i64 heavy_deg, hipStream_t s);
 

@@ -158,32 +158,46 @@ __global__ __launch_bounds__(256) void k_level_census(DGraph g, i64 nedges, i64 
 // could touch t or u has decided.
 constexpr int MATCH_BS = 256;
 
+// Only undecided vertices below t can block t, so each round evaluates the
+// window [lo, lo + W) of vertex ids, lo = smallest undecided id: vertices
+// above the window decide in later rounds with the same outcome (conditions
+// only delay a decision, never change it) and the smallest undecided vertex
+// always decides.  Every block recomputes lo from the stored value (the
+// decided flags do not change inside the kernel) and block 0 stores it.
+__device__ __forceinline__ i64 block_first_undecided(const u8* dec, i64 lo, i64 n) {
+  __shared__ i64 s_found;
+  for (i64 base = lo; base < n; base += blockDim.x) {
+    if (threadIdx.x == 0) s_found = LLONG_MAX;
+    __syncthreads();
+    const i64 i = base + threadIdx.x;
+    if (i < n && !dec[i]) atomicMin((unsigned long long*)&s_found, (unsigned long long)i);
+    __syncthreads();
+    const i64 f = s_found;
+    __syncthreads();
+    if (f != LLONG_MAX) return f;
+  }
+  return n;
+}
+
 __global__ __launch_bounds__(MATCH_BS) void k_match_reserve(DGraph g, const i32* mate,
                                                             const u8* dec, i32* pend, u64* R,
-                                                            i32* prop, const i32* list_in,
-                                                            const i32* cnt_in, i32* list_out,
-                                                            i32* cnt_out, u32 round) {
+                                                            i32* prop, i64* lo_io, i64 W,
+                                                            u32 round) {
+  const i64 lo = block_first_undecided(dec, *lo_io, g.n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *lo_io = lo;
   const int lane = lane_id();
-  const i32 cnt = *cnt_in;
   const u64 mykey_hi = (u64)round << 32;
-  for (i64 base = gwave() * WAVE; base < cnt; base += nwaves() * WAVE) {
-    const i64 e = base + lane;
-    const i32 t0 = e < cnt ? list_in[e] : -1;
-    const bool alive = t0 >= 0 && !dec[t0];
+  const i64 hi = lo + W < g.n ? lo + W : g.n;
+  for (i64 base = lo + gwave() * WAVE; base < hi; base += nwaves() * WAVE) {
+    const i64 t0 = base + lane;
+    const bool alive = t0 < hi && !dec[t0];
     u64 am = __ballot(alive);
-    if (!am) continue;
-    i32 obase = 0;
-    if (lane == 0) obase = atomicAdd(cnt_out, __popcll(am));
-    obase = __shfl(obase, 0);
-    if (alive) {
-      list_out[obase + __popcll(am & lanes_below(lane))] = t0;
-      pend[t0] = (i32)round;
-    }
-    // every surviving vertex reserves its free neighbours and proposes its
-    // heaviest free neighbour; rows are scanned by the whole wave
+    if (alive) pend[t0] = (i32)round;
+    // every undecided window vertex reserves its free neighbours and proposes
+    // its heaviest free neighbour; rows are scanned by the whole wave
     while (am) {
       const int l = __ffsll((unsigned long long)am) - 1;
-      const i32 t = __shfl(t0, l);
+      const i32 t = (i32)(base + l);
       const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
       const u64 key = mykey_hi | (u64)(0xFFFFFFFFu - (u32)t);
       u64 best = 0;
@@ -205,69 +219,86 @@ __global__ __launch_bounds__(MATCH_BS) void k_match_reserve(DGraph g, const i32*
   }
 }
 
-__global__ __launch_bounds__(MATCH_BS) void k_match_decide(const i32* list, const i32* cnt,
-                                                           const u64* R, const i32* prop,
+// Decide and apply: reads only R, prop and pend, which this kernel does not
+// write, so the decisions see the state at the start of the round.
+__global__ __launch_bounds__(MATCH_BS) void k_match_decide(const u64* R, const i32* prop,
                                                            const i32* pend, i32* mate, u8* dec,
-                                                           i32* cnt_reset, u32 round) {
-  if (gtid() == 0) *cnt_reset = 0;
-  const i32 c = *cnt;
-  for (i64 e = gtid(); e < c; e += gstride()) {
-    const i32 t = list[e];
+                                                           const i64* lo_io, i64 W, i64 n,
+                                                           u32 round) {
+  const i64 lo = *lo_io;
+  const i64 hi = lo + W < n ? lo + W : n;
+  for (i64 t = lo + gtid(); t < hi; t += gstride()) {
+    if (pend[t] != (i32)round) continue;
     const u64 rt = R[t];
-    const bool A = !((u32)(rt >> 32) == round && (i64)(0xFFFFFFFFu - (u32)rt) < (i64)t);
+    const bool A = !((u32)(rt >> 32) == round && (i64)(0xFFFFFFFFu - (u32)rt) < t);
     const i32 u = prop[t];
     if (u < 0) {
       if (A) dec[t] = 1;
       continue;
     }
     const u64 ru = R[u];
-    const bool C = !((u32)(ru >> 32) == round && (i64)(0xFFFFFFFFu - (u32)ru) < (i64)t);
+    const bool C = !((u32)(ru >> 32) == round && (i64)(0xFFFFFFFFu - (u32)ru) < t);
+    // u outside the window lies above t; u below lo is decided
     const bool B = (u > t) || (pend[u] != (i32)round);
     if (A && B && C) {
       mate[t] = u;
-      mate[u] = t;
+      mate[u] = (i32)t;
       dec[t] = 1;
       dec[u] = 1;
     }
   }
 }
 
-__global__ void k_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i32* list) {
+__global__ void k_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R) {
   for (i64 i = gtid(); i < n; i += gstride()) {
     mate[i] = -1;
     dec[i] = 0;
     pend[i] = 0;
     R[i] = 0;
-    list[i] = (i32)i;
   }
-}
-__global__ void k_set_counters(i32* c, i32 a, i32 b) {
-  c[0] = a;
-  c[1] = b;
 }
 
 // Symmetric levels (every arc s->x has its reverse x->s): the vertices that
 // reserve x in a round are exactly x's undecided neighbours, so the round rule
-// above needs only mp[x] = smallest undecided neighbour of x.  mp[x] is cached
-// and stays exact while that neighbour is undecided (decisions are permanent,
-// the undecided set only shrinks); it is recomputed when it has decided.  The
-// proposal prop[t] (heaviest free neighbour, first position) is cached the same
-// way: it stays exact while it is free.  A blocked vertex therefore costs one
-// or two loads per round instead of a row scan with atomics.
-constexpr i32 MP_UNKNOWN = -1;
-constexpr i32 MP_NONE = INT_MAX;
+// above reads: t may decide once no undecided neighbour lies below t
+// (condition A), and may take its proposal u once no undecided neighbour of u
+// lies below t (condition C).  Instead of recomputing these every round, a
+// waiting vertex remembers one blocker, the LARGEST undecided vertex that
+// violates the condition, and is re-examined only after that blocker has
+// decided (decisions are permanent and the undecided set only shrinks, so a
+// condition found true stays true).  Lower neighbours tend to decide in
+// ascending order, so one rescan usually clears the condition.  The proposal
+// (heaviest free neighbour, first position) is cached while it stays free.
+constexpr i32 BLK_UNKNOWN = -1;
+constexpr i32 BLK_NONE = -2;
 
-__device__ __forceinline__ i32 wave_min_pending(const DGraph& g, i32 x, const u8* dec, int lane) {
+// Row scans issue SCAN_U independent loads per lane before using any of them,
+// so a long row costs about len/(64*SCAN_U) memory round trips.
+constexpr int SCAN_U = 8;
+
+// largest undecided neighbour u of x with u < lim, u != x (BLK_NONE if none)
+__device__ __forceinline__ i32 wave_max_pending_below(const DGraph& g, i32 x, i32 lim,
+                                                      const u8* dec, int lane) {
   const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
-  i32 best = MP_NONE;
-  for (i64 i = s0 + lane; i < s1; i += WAVE) {
-    const i32 u = g.adj[i];
-    if (u < g.n && u != x && !dec[u]) best = u < best ? u : best;
+  i32 best = BLK_NONE;
+  for (i64 base = s0; base < s1; base += WAVE * SCAN_U) {
+    i32 u[SCAN_U];
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      const i64 i = base + j * WAVE + lane;
+      u[j] = i < s1 ? g.adj[i] : INT_MAX;
+    }
+    u8 d[SCAN_U];
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) d[j] = (u[j] < lim && u[j] != x) ? dec[u[j]] : (u8)1;
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++)
+      if (!d[j]) best = u[j] > best ? u[j] : best;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const i32 t = __shfl_xor(best, o);
-    best = t < best ? t : best;
+    best = t > best ? t : best;
   }
   return best;
 }
@@ -275,12 +306,22 @@ __device__ __forceinline__ i32 wave_min_pending(const DGraph& g, i32 x, const u8
 __device__ __forceinline__ i32 wave_best_free(const DGraph& g, i32 t, const i32* mate, int lane) {
   const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
   u64 best = 0;
-  for (i64 i = s0 + lane; i < s1; i += WAVE) {
-    const i32 u = g.adj[i];
-    if (u < g.n && u != t && mate[u] == -1) {
-      const i32 w = g.w[i];
-      if (w >= 0) {
-        const u64 c = ((u64)(u32)w << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+  for (i64 base = s0; base < s1; base += WAVE * SCAN_U) {
+    i32 u[SCAN_U], w[SCAN_U], mt[SCAN_U];
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      const i64 i = base + j * WAVE + lane;
+      u[j] = i < s1 ? g.adj[i] : -1;
+      w[j] = i < s1 ? g.w[i] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++)
+      mt[j] = (u[j] >= 0 && u[j] < g.n && u[j] != t && w[j] >= 0) ? mate[u[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      if (mt[j] == -1) {
+        const i64 i = base + j * WAVE + lane;
+        const u64 c = ((u64)(u32)w[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
         best = c > best ? c : best;
       }
     }
@@ -293,45 +334,86 @@ __device__ __forceinline__ i32 wave_best_free(const DGraph& g, i32 t, const i32*
 constexpr i32 DEC_WAIT = -2;
 constexpr i32 DEC_EXHAUST = -1;
 
-// One lane per pending vertex; survivors are appended with one atomic per wave.
-// Lanes whose cached minimum or proposal must be refreshed are served one at
-// a time by a cooperative scan of the whole wave.
-__global__ __launch_bounds__(MATCH_BS) void k_match_sym_decide(DGraph g, const i32* mate,
-                                                               const u8* dec, i32* mp,
-                                                               i32* prop, i32* decision,
-                                                               const i32* list_in,
-                                                               const i32* cnt_in, i32* list_out,
-                                                               i32* cnt_out) {
-  const int lane = lane_id();
-  const i32 cnt = *cnt_in;
-  for (i64 base = gwave() * WAVE; base < cnt; base += nwaves() * WAVE) {
-    const i64 e = base + lane;
-    const i32 t = e < cnt ? list_in[e] : -1;
-    const bool alive = t >= 0 && !dec[t];
-    const u64 am = __ballot(alive);
-    if (!am) continue;
-    i32 obase = 0;
-    if (lane == 0) obase = atomicAdd(cnt_out, __popcll(am));
-    obase = __shfl(obase, 0);
-    if (alive) list_out[obase + __popcll(am & lanes_below(lane))] = t;
+__device__ __forceinline__ bool blocker_stale(i32 b, const u8* dec) {
+  return b == BLK_UNKNOWN || (b >= 0 && dec[b]);
+}
 
-    // condition A: no undecided neighbour below t
-    i32 a = alive ? mp[t] : MP_NONE;
-    u64 need = __ballot(alive && (a == MP_UNKNOWN || (a != MP_NONE && dec[a])));
+// One lane per window vertex; lanes whose blocker or proposal must be
+// refreshed are served one at a time by a cooperative scan of the whole wave.
+// Rows longer than MATCH_HUGE arcs are not scanned here: the lane waits this
+// round and queues a request that k_match_huge serves with a whole block
+// after the round (its results are valid in the later state too).  Every
+// cache is private to its vertex, and all reads of mate/dec see the state at
+// the start of the round.
+constexpr i64 MATCH_HUGE = 4096;
+constexpr int SYM_VPW = 16;  // window vertices per wave (the rest of the lanes help scans)
+enum { REQ_A = 0, REQ_P = 1, REQ_C = 2 };
+
+__device__ __forceinline__ void push_request(int need_huge_lane, int lane, i32 kind, i32 t,
+                                             i32* req, i32* nreq) {
+  const u64 hm = __ballot(need_huge_lane);
+  if (!hm) return;
+  i32 ob = 0;
+  if (lane == 0) ob = atomicAdd(nreq, (i32)__popcll(hm));
+  ob = __shfl(ob, 0);
+  if (need_huge_lane) {
+    const i32 o = ob + __popcll(hm & lanes_below(lane));
+    req[2 * o] = kind;
+    req[2 * o + 1] = t;
+  }
+}
+
+__device__ __forceinline__ i64 row_len(const DGraph& g, i32 x) {
+  return g.xadj[x + 1] - g.xadj[x];
+}
+
+__global__ __launch_bounds__(MATCH_BS) void k_match_sym_decide(DGraph g, const i32* mate,
+                                                               const u8* dec, i32* blkA,
+                                                               i32* blkC, i32* prop,
+                                                               i32* decision, i64* lo_io,
+                                                               i64 W, i32* req, i32* nreq,
+                                                               i32* nreq_next, int par) {
+  // lo_io[0]: window low end of this round; lo_io[1 + parity]: where the scan
+  // for the smallest undecided id starts.  This round reads slot `par` and
+  // initialises the other one, which the apply kernel lowers to the smallest
+  // id it leaves undecided and the next round reads.
+  const i64 lo = block_first_undecided(dec, lo_io[1 + par], g.n);
+  const i64 hi = lo + W < g.n ? lo + W : g.n;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    lo_io[0] = lo;
+    lo_io[2 - par] = hi;
+    *nreq_next = 0;  // the counter the next round appends to (read last round, done)
+  }
+  const int lane = lane_id();
+  for (i64 base = lo + gwave() * SYM_VPW; base < hi; base += nwaves() * SYM_VPW) {
+    const i64 e = base + lane;
+    const bool alive = lane < SYM_VPW && e < hi && !dec[e];
+    const i32 t = alive ? (i32)e : -1;
+    if (!__ballot(alive)) continue;
+
+    // condition A
+    i32 a = alive ? blkA[t] : BLK_NONE;
+    bool stale = alive && blocker_stale(a, dec);
+    bool big = stale && row_len(g, t) > MATCH_HUGE;
+    push_request(big, lane, REQ_A, t, req, nreq);
+    u64 need = __ballot(stale && !big);
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
-      const i32 r = wave_min_pending(g, x, dec, lane);
+      const i32 r = wave_max_pending_below(g, x, x, dec, lane);
       if (lane == l) {
         a = r;
-        mp[x] = r;
+        blkA[x] = r;
       }
       need &= need - 1;
     }
-    const bool okA = alive && (a == MP_NONE || a > t);
-    // proposal: heaviest free neighbour (cached while it stays free)
+    const bool okA = alive && !big && a == BLK_NONE;
+    // proposal
     i32 u = okA ? prop[t] : -1;
-    need = __ballot(okA && (u == MP_UNKNOWN || (u >= 0 && mate[u] != -1)));
+    stale = okA && (u == BLK_UNKNOWN || (u >= 0 && mate[u] != -1));
+    big = stale && row_len(g, t) > MATCH_HUGE;
+    push_request(big, lane, REQ_P, t, req, nreq);
+    need = __ballot(stale && !big);
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
@@ -339,61 +421,131 @@ __global__ __launch_bounds__(MATCH_BS) void k_match_sym_decide(DGraph g, const i
       if (lane == l) {
         u = r;
         prop[x] = r;
+        blkC[x] = BLK_UNKNOWN;
       }
       need &= need - 1;
     }
-    // condition B (u above t or already decided) and C (no undecided
-    // neighbour of u below t)
-    const bool tryC = okA && u >= 0 && (u > t || dec[u]);
-    i32 b = tryC ? mp[u] : MP_NONE;
-    need = __ballot(tryC && (b == MP_UNKNOWN || (b != MP_NONE && dec[b])));
+    const bool okP = okA && !big;
+    // condition B (u above t or already decided), then C
+    const bool tryC = okP && u >= 0 && (u > t || dec[u]);
+    i32 b = tryC ? blkC[t] : BLK_NONE;
+    stale = tryC && blocker_stale(b, dec);
+    big = stale && row_len(g, u) > MATCH_HUGE;
+    push_request(big, lane, REQ_C, t, req, nreq);
+    need = __ballot(stale && !big);
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(u, l);
-      const i32 r = wave_min_pending(g, x, dec, lane);
+      const i32 lim = __shfl(t, l);
+      const i32 r = wave_max_pending_below(g, x, lim, dec, lane);
       if (lane == l) {
         b = r;
-        mp[x] = r;
+        blkC[lim] = r;
       }
       need &= need - 1;
     }
     if (alive) {
       i32 d = DEC_WAIT;
-      if (okA) {
+      if (okP) {
         if (u < 0) d = DEC_EXHAUST;
-        else if (tryC && (b == MP_NONE || b >= t)) d = u;
+        else if (tryC && !big && b == BLK_NONE) d = u;
       }
       decision[t] = d;
     }
   }
 }
 
-__global__ __launch_bounds__(MATCH_BS) void k_match_sym_apply(const i32* list, const i32* cnt,
-                                                              const i32* decision, i32* mate,
-                                                              u8* dec, i32* cnt_reset) {
-  if (gtid() == 0) *cnt_reset = 0;
-  const i32 c = *cnt;
-  for (i64 e = gtid(); e < c; e += gstride()) {
-    const i32 t = list[e];
+// Serves queued scans of long rows with one block per request; runs after
+// the round's apply, so it sees a later (still consistent) state.
+__global__ __launch_bounds__(1024) void k_match_huge(DGraph g, const i32* mate, const u8* dec,
+                                                     i32* blkA, i32* blkC, i32* prop,
+                                                     const i32* req, const i32* nreq) {
+  __shared__ u64 s_red[1024 / WAVE];
+  const i32 cnt = *nreq;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i32 r = blockIdx.x; r < cnt; r += gridDim.x) {
+    const i32 kind = req[2 * r], t = req[2 * r + 1];
+    if (dec[t]) continue;
+    i32 x = t, lim = t;
+    if (kind == REQ_C) {
+      x = prop[t];
+      if (x < 0 || mate[x] != -1) continue;  // proposal gone: re-proposed next round
+    }
+    const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
+    u64 best = 0;  // REQ_A/C: 1 + max pending below lim; REQ_P: (w, ~pos) key
+    for (i64 base = s0; base < s1; base += 1024 * 4) {
+      i32 u[4], w[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const i64 i = base + j * 1024 + threadIdx.x;
+        u[j] = i < s1 ? g.adj[i] : -1;
+        w[j] = (kind == REQ_P && i < s1) ? g.w[i] : -1;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const i64 i = base + j * 1024 + threadIdx.x;
+        if (kind == REQ_P) {
+          if (u[j] >= 0 && u[j] < g.n && u[j] != x && w[j] >= 0 && mate[u[j]] == -1) {
+            const u64 c = ((u64)(u32)w[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+            best = c > best ? c : best;
+          }
+        } else if (u[j] >= 0 && u[j] < lim && u[j] != x && !dec[u[j]]) {
+          const u64 c = (u64)u[j] + 1;
+          best = c > best ? c : best;
+        }
+      }
+    }
+    best = wave_max_u64(best);
+    if (lane == 0) s_red[wv] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w2 = 1; w2 < 1024 / WAVE; w2++) best = s_red[w2] > best ? s_red[w2] : best;
+      if (kind == REQ_A) {
+        blkA[t] = best ? (i32)(best - 1) : BLK_NONE;
+      } else if (kind == REQ_C) {
+        blkC[t] = best ? (i32)(best - 1) : BLK_NONE;
+      } else {
+        prop[t] = best ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)best)] : -1;
+        blkC[t] = BLK_UNKNOWN;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Applies the decisions of the window's vertices that were undecided when the
+// round began (a vertex matched as a partner in this round had decided WAIT).
+__global__ __launch_bounds__(MATCH_BS) void k_match_sym_apply(const i32* decision, i32* mate,
+                                                              u8* dec, i64* lo_io, i64 W, i64 n,
+                                                              int par) {
+  const i64 lo = lo_io[0];
+  const i64 hi = lo + W < n ? lo + W : n;
+  i64 first = LLONG_MAX;
+  for (i64 t = lo + gtid(); t < hi; t += gstride()) {
+    if (dec[t]) continue;
     const i32 d = decision[t];
     if (d == DEC_EXHAUST) {
       dec[t] = 1;
     } else if (d >= 0) {
       mate[t] = d;
-      mate[d] = t;
+      mate[d] = (i32)t;
       dec[t] = 1;
       dec[d] = 1;
+    } else if (t < first) {
+      first = t;  // may since have been matched as a partner: the next scan skips it
     }
   }
+  if (first != LLONG_MAX)
+    atomicMin((unsigned long long*)&lo_io[2 - par], (unsigned long long)first);
 }
 
-__global__ void k_match_sym_init(i64 n, i32* mate, u8* dec, i32* mp, i32* prop, i32* list) {
+__global__ void k_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop) {
   for (i64 i = gtid(); i < n; i += gstride()) {
     mate[i] = -1;
     dec[i] = 0;
-    mp[i] = MP_UNKNOWN;
-    prop[i] = MP_UNKNOWN;
-    list[i] = (i32)i;
+    blkA[i] = BLK_UNKNOWN;
+    blkC[i] = BLK_UNKNOWN;
+    prop[i] = BLK_UNKNOWN;
   }
 }
 
@@ -751,7 +903,8 @@ template <bool UNIT, bool LDS>
 __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* part, i32 k,
                                                const float* bg_g, const u8* src_big,
                                                float* gain, i32* target, u8* cand, int phase,
-                                               i32* gscratch) {
+                                               i32* gscratch, const i32* list_a, i64 cnt_a,
+                                               const i32* list_b, const i32* cnt_b) {
   extern __shared__ i32 lds[];
   const int lane = threadIdx.x;
   i32* tally;
@@ -767,7 +920,9 @@ __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* 
   }
   __syncthreads();
   float* ftally = (float*)tally;
-  for (i64 v = blockIdx.x; v < g.n; v += gridDim.x) {
+  const i64 total = cnt_a + (list_b ? (i64)*cnt_b : 0);
+  for (i64 e = blockIdx.x; e < total; e += gridDim.x) {
+    const i64 v = e < cnt_a ? (list_a ? list_a[e] : e) : list_b[e - cnt_a];
     const i32 cur = part[v];
     if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
     for (int q = lane; q < k; q += WAVE) tally[q] = 0;
@@ -846,6 +1001,461 @@ __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* 
       cand[v] = phase ? (best > -50.0f) : (best > -10.0f);  // gpu_part.cu:308-312
     }
     __syncthreads();
+  }
+}
+
+// Per-pass order of the parts by (balance term desc, part asc) for both
+// source-size variants, and for every position the end of its run of equal
+// balance terms.  For a vertex whose tally of part q is zero the gain is
+// (0 - c_cur) + b_q, which is non-decreasing in b_q, so its best such part is
+// found by walking runs from the top (first allowed part of a run = lowest q).
+__global__ __launch_bounds__(1024) void k_bg_order(const float* bg, i32 k, i32* ord,
+                                                   i32* run_end) {
+  __shared__ u64 key[MAXK_LDS];
+  __shared__ i32 rend[MAXK_LDS];
+  i32 P = 1;
+  while (P < k) P <<= 1;
+  for (int s = 0; s < 2; s++) {
+    const float* b = bg + s * k;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+      if (i < k) {
+        const u32 bits = __float_as_uint(b[i] + 0.0f);  // -0 -> +0
+        const u32 ordk = (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);  // ascending
+        key[i] = ((u64)(~ordk) << 32) | (u32)i;  // descending balance term, then ascending q
+      } else {
+        key[i] = ~0ull;
+      }
+    }
+    __syncthreads();
+    for (i32 kk = 2; kk <= P; kk <<= 1) {
+      for (i32 j = kk >> 1; j > 0; j >>= 1) {
+        for (i32 t = threadIdx.x; t < P / 2; t += blockDim.x) {
+          const i32 lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+          const i32 hi = lo + j;
+          const bool asc = (lo & kk) == 0;
+          const u64 a = key[lo], c = key[hi];
+          if ((a > c) == asc) {
+            key[lo] = c;
+            key[hi] = a;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+      const i32 q = (i32)(key[i] & 0xffffffffu);
+      ord[s * k + i] = q;
+      const bool last = (i == k - 1) || (b[(i32)(key[i + 1] & 0xffffffffu)] != b[q]);
+      rend[i] = last ? i + 1 : INT_MAX;
+    }
+    __syncthreads();
+    for (i32 d = 1; d < k; d <<= 1) {
+      i32 v[4];
+      int c = 0;
+      for (int i = threadIdx.x; i < k; i += blockDim.x, c++) v[c] = (i + d < k) ? rend[i + d] : INT_MAX;
+      __syncthreads();
+      c = 0;
+      for (int i = threadIdx.x; i < k; i += blockDim.x, c++) rend[i] = v[c] < rend[i] ? v[c] : rend[i];
+      __syncthreads();
+    }
+    for (int i = threadIdx.x; i < k; i += blockDim.x) run_end[s * k + i] = rend[i];
+    __syncthreads();
+  }
+}
+
+// One thread per vertex whose range has at most GAIN_LIGHT arcs: the tallies
+// of up to GAIN_SLOTS distinct parts live in registers and are accumulated
+// in row order (exactly the reference's fp32 sums).  Vertices with more
+// distinct parts are deferred to the wave kernel.
+constexpr int GAIN_SLOTS = 16;
+constexpr int GAIN_LIGHT = 64;
+
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_gain_light(DGraph g, i64 nedges, const i32* part, i32 k,
+                                                    const float* bg, const u8* src_big,
+                                                    const i32* ord, const i32* run_end,
+                                                    float* gain, i32* target, u8* cand,
+                                                    int phase, i32* defer, i32* ndefer) {
+  const int lane = lane_id();
+  for (i64 base = gtid() - lane; base < g.n; base += gstride()) {
+    const i64 v = base + lane;
+    bool mine = false, over = false;
+    i32 cur = -1;
+    i32 keys[GAIN_SLOTS];
+    float vals[GAIN_SLOTS];
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < GAIN_SLOTS; j++) {
+      keys[j] = -1;
+      vals[j] = 0.0f;
+    }
+    if (v < g.n) {
+      const i64 s0 = g.xadj[v];
+      const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+      cur = part[v];
+      mine = (s1 - s0 <= GAIN_LIGHT) && cur >= 0 && cur < k;
+      if (mine) {
+        for (i64 i = s0; i < s1; i++) {
+          const i32 u = g.adj[i];
+          if (u >= g.n) continue;
+          const i32 p = part[u];
+          if (p < 0 || p >= k) continue;
+          const float wt = UNIT ? 1.0f : (float)g.w[i];
+          bool found = false;
+#pragma unroll
+          for (int j = 0; j < GAIN_SLOTS; j++) {
+            if (keys[j] == p) {
+              vals[j] += wt;
+              found = true;
+            }
+          }
+          if (!found) {
+            if (cnt < GAIN_SLOTS) {
+#pragma unroll
+              for (int j = 0; j < GAIN_SLOTS; j++) {
+                if (j == cnt) {
+                  keys[j] = p;
+                  vals[j] = wt;
+                }
+              }
+              cnt++;
+            } else {
+              over = true;
+              break;
+            }
+          }
+        }
+      }
+    }
+    // defer vertices with too many distinct parts (one atomic per wave)
+    const u64 om = __ballot(mine && over);
+    if (om) {
+      i32 ob = 0;
+      if (lane == 0) ob = atomicAdd(ndefer, (i32)__popcll(om));
+      ob = __shfl(ob, 0);
+      if (mine && over) defer[ob + __popcll(om & lanes_below(lane))] = (i32)v;
+    }
+    if (!mine || over) continue;
+    float ccur = 0.0f;
+#pragma unroll
+    for (int j = 0; j < GAIN_SLOTS; j++)
+      if (keys[j] == cur) ccur = vals[j];
+    const int sv = src_big[cur] ? 1 : 0;
+    const float* bgs = bg + sv * k;
+    // best over the parts present in the row
+    float bgv = -INFINITY;
+    i32 bq = INT_MAX;
+#pragma unroll
+    for (int j = 0; j < GAIN_SLOTS; j++) {
+      const i32 q = keys[j];
+      if (q >= 0 && q != cur) {
+        const float eg = vals[j] - ccur;
+        const float gg = eg + bgs[q];
+        if (gg > bgv || (gg == bgv && q < bq)) {
+          bgv = gg;
+          bq = q;
+        }
+      }
+    }
+    // best over the parts absent from the row (tally 0.0f)
+    const float e0 = 0.0f - ccur;
+    const i32* o = ord + sv * k;
+    const i32* re = run_end + sv * k;
+    bool found = false;
+    float g1 = 0.0f;
+    i32 q1 = INT_MAX;
+    for (i32 i = 0; i < k;) {
+      const i32 e = re[i];
+      i32 j = i;
+      i32 q = -1;
+      for (; j < e; j++) {
+        q = o[j];
+        bool tch = (q == cur);
+#pragma unroll
+        for (int t = 0; t < GAIN_SLOTS; t++) tch |= (keys[t] == q);
+        if (!tch) break;
+      }
+      if (j < e) {
+        const float gg = e0 + bgs[q];
+        if (!found) {
+          found = true;
+          g1 = gg;
+          q1 = q;
+        } else if (gg == g1) {
+          q1 = q < q1 ? q : q1;
+        } else {
+          break;
+        }
+      }
+      i = e;
+    }
+    if (found && (g1 > bgv || (g1 == bgv && q1 < bq))) {
+      bgv = g1;
+      bq = q1;
+    }
+    float best;
+    i32 tgt;
+    if (bq == INT_MAX || bgv < -1e9f) {
+      best = -1e9f;
+      tgt = cur;
+    } else if (bgv == -1e9f) {
+      best = -1e9f;
+      tgt = bq < cur ? bq : cur;
+    } else {
+      best = bgv;
+      tgt = bq;
+    }
+    gain[v] = best;
+    target[v] = tgt;
+    cand[v] = phase ? (best > -50.0f) : (best > -10.0f);
+  }
+}
+
+// gpu_part.cu:258-312 with G lanes per vertex: each group keeps a dense LDS
+// tally of k ints (exact while the row's sum of |w| <= 2^24, else one lane
+// redoes the row in fp32 in row order), then every lane scores k/G parts and
+// the group reduces by (gain desc, part asc); a zero tally scores exactly like
+// an absent part, so the dense scan is the reference's loop over q.  G = 8
+// for rows of at most 64 arcs (eight vertices per wave in flight), G = 64 for
+// the heavy list.
+template <int G, bool UNIT>
+__global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const i32* part, i32 k,
+                                                    const float* bg_g, const u8* src_big,
+                                                    float* gain, i32* target, u8* cand,
+                                                    int phase, const i32* list, i64 cnt,
+                                                    i64 light_max) {
+  extern __shared__ i32 lds[];
+  constexpr int GPB = 256 / G;  // groups per block
+  float* sbg = (float*)lds;
+  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
+  __syncthreads();
+  const int gl = threadIdx.x & (G - 1);
+  const int grp = threadIdx.x / G;
+  i32* tally = lds + 2 * k + grp * k;
+  float* ftally = (float*)tally;
+  const i64 total = list ? cnt : g.n;
+  const i64 ngroups = (i64)gridDim.x * GPB;
+  for (i64 e = (i64)blockIdx.x * GPB + grp; e < total; e += ngroups) {
+    const i64 v = list ? list[e] : e;
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    const i32 cur = part[v];
+    if (!list && s1 - s0 > light_max) continue;   // heavy rows: the G = 64 launch
+    if (cur >= k || cur < 0) continue;            // gpu_part.cu:262
+    for (int q = gl; q < k; q += G) tally[q] = 0;
+    __builtin_amdgcn_wave_barrier();
+    i64 absw = 0;
+    for (i64 i = s0 + gl; i < s1; i += G) {
+      const i32 u = g.adj[i];
+      if (u < g.n) {
+        const i32 p = part[u];
+        if (p < k && p >= 0) {
+          const i32 w = UNIT ? 1 : g.w[i];
+          atomicAdd(&tally[p], w);
+          if (!UNIT) absw += w < 0 ? -(i64)w : (i64)w;
+        }
+      }
+    }
+    if (!UNIT) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) absw += __shfl_xor(absw, o, G);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const bool exact_int = UNIT ? (s1 - s0) <= (i64)(1 << 24) : absw <= (i64)(1 << 24);
+    if (!exact_int) {
+      for (int q = gl; q < k; q += G) ftally[q] = 0.0f;
+      __builtin_amdgcn_wave_barrier();
+      if (gl == 0) {
+        for (i64 i = s0; i < s1; i++) {
+          const i32 u = g.adj[i];
+          if (u < g.n) {
+            const i32 p = part[u];
+            if (p < k && p >= 0) ftally[p] += (float)g.w[i];
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    const float ccur = exact_int ? (float)tally[cur] : ftally[cur];
+    const float* bgr = sbg + (src_big[cur] ? k : 0);
+    float bgv = -INFINITY;
+    i32 bq = INT_MAX;
+    for (int q = gl; q < k; q += G) {
+      if (q == cur) continue;
+      const float cq = exact_int ? (float)tally[q] : ftally[q];
+      const float eg = cq - ccur;
+      const float gg = eg + bgr[q];
+      if (gg > bgv || (gg == bgv && q < bq)) {
+        bgv = gg;
+        bq = q;
+      }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      const float og = __shfl_xor(bgv, o, G);
+      const i32 oq = __shfl_xor(bq, o, G);
+      if (og > bgv || (og == bgv && oq < bq)) {
+        bgv = og;
+        bq = oq;
+      }
+    }
+    if (gl == 0) {
+      float best;
+      i32 tgt;
+      if (bq == INT_MAX || bgv < -1e9f) {
+        best = -1e9f;
+        tgt = cur;
+      } else if (bgv == -1e9f) {
+        best = -1e9f;
+        tgt = bq < cur ? bq : cur;
+      } else {
+        best = bgv;
+        tgt = bq;
+      }
+      gain[v] = best;
+      target[v] = tgt;
+      cand[v] = phase ? (best > -50.0f) : (best > -10.0f);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Rows longer than GAIN_HUGE arcs: one block of 1024 threads per vertex with
+// four independent loads per thread in flight; block-wide LDS tally and pick.
+constexpr int GAIN_HUGE = 4096;
+constexpr int HUGE_BS = 1024;
+
+template <bool UNIT>
+__global__ __launch_bounds__(HUGE_BS) void k_gain_huge(DGraph g, i64 nedges, const i32* part,
+                                                       i32 k, const float* bg_g,
+                                                       const u8* src_big, float* gain,
+                                                       i32* target, u8* cand, int phase,
+                                                       const i32* list, i64 cnt) {
+  extern __shared__ i32 lds[];
+  __shared__ i64 s_abs[HUGE_BS / WAVE];
+  __shared__ float s_g[HUGE_BS / WAVE];
+  __shared__ i32 s_q[HUGE_BS / WAVE];
+  float* sbg = (float*)lds;
+  i32* tally = lds + 2 * k;
+  float* ftally = (float*)tally;
+  for (int q = threadIdx.x; q < 2 * k; q += HUGE_BS) sbg[q] = bg_g[q];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 e = blockIdx.x; e < cnt; e += gridDim.x) {
+    const i64 v = list[e];
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    const i32 cur = part[v];
+    if (cur >= k || cur < 0) continue;
+    for (int q = threadIdx.x; q < k; q += HUGE_BS) tally[q] = 0;
+    __syncthreads();
+    i64 absw = 0;
+    for (i64 base = s0; base < s1; base += 4 * HUGE_BS) {
+      i32 u[4], w[4], p[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const i64 i = base + j * HUGE_BS + threadIdx.x;
+        u[j] = i < s1 ? g.adj[i] : INT_MAX;
+        w[j] = (UNIT || i >= s1) ? 1 : g.w[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) p[j] = u[j] < g.n ? part[u[j]] : -1;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (p[j] >= 0 && p[j] < k) {
+          atomicAdd(&tally[p[j]], w[j]);
+          if (!UNIT) absw += w[j] < 0 ? -(i64)w[j] : (i64)w[j];
+        }
+      }
+    }
+    if (!UNIT) {
+      absw = wave_sum(absw);
+      if (lane == 0) s_abs[wv] = absw;
+    }
+    __syncthreads();
+    i64 tot = 0;
+    if (!UNIT)
+      for (int w2 = 0; w2 < HUGE_BS / WAVE; w2++) tot += s_abs[w2];
+    const bool exact_int = UNIT ? (s1 - s0) <= (i64)(1 << 24) : tot <= (i64)(1 << 24);
+    if (!exact_int) {
+      __syncthreads();
+      for (int q = threadIdx.x; q < k; q += HUGE_BS) ftally[q] = 0.0f;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (i64 i = s0; i < s1; i++) {
+          const i32 uu = g.adj[i];
+          if (uu < g.n) {
+            const i32 pp = part[uu];
+            if (pp < k && pp >= 0) ftally[pp] += (float)g.w[i];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const float ccur = exact_int ? (float)tally[cur] : ftally[cur];
+    const float* bgr = sbg + (src_big[cur] ? k : 0);
+    float bgv = -INFINITY;
+    i32 bq = INT_MAX;
+    for (int q = threadIdx.x; q < k; q += HUGE_BS) {
+      if (q == cur) continue;
+      const float cq = exact_int ? (float)tally[q] : ftally[q];
+      const float eg = cq - ccur;
+      const float gg = eg + bgr[q];
+      if (gg > bgv || (gg == bgv && q < bq)) {
+        bgv = gg;
+        bq = q;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float og = __shfl_xor(bgv, o);
+      const i32 oq = __shfl_xor(bq, o);
+      if (og > bgv || (og == bgv && oq < bq)) {
+        bgv = og;
+        bq = oq;
+      }
+    }
+    if (lane == 0) {
+      s_g[wv] = bgv;
+      s_q[wv] = bq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w2 = 1; w2 < HUGE_BS / WAVE; w2++) {
+        if (s_g[w2] > bgv || (s_g[w2] == bgv && s_q[w2] < bq)) {
+          bgv = s_g[w2];
+          bq = s_q[w2];
+        }
+      }
+      float best;
+      i32 tgt;
+      if (bq == INT_MAX || bgv < -1e9f) {
+        best = -1e9f;
+        tgt = cur;
+      } else if (bgv == -1e9f) {
+        best = -1e9f;
+        tgt = bq < cur ? bq : cur;
+      } else {
+        best = bgv;
+        tgt = bq;
+      }
+      gain[v] = best;
+      target[v] = tgt;
+      cand[v] = phase ? (best > -50.0f) : (best > -10.0f);
+    }
+    __syncthreads();
+  }
+}
+
+// Heavy rows split into medium (64 < len <= 4096, one wave) and huge (> 4096,
+// one block): entries of `heavy` are partitioned in place by the census.
+__global__ void k_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
+                              i32* nmedium, i32* huge, i32* nhuge) {
+  for (i64 e = gtid(); e < nheavy; e += gstride()) {
+    const i32 v = heavy[e];
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    if (s1 - s0 > GAIN_HUGE) huge[atomicAdd(nhuge, 1)] = v;
+    else medium[atomicAdd(nmedium, 1)] = v;
   }
 }
 
@@ -1214,6 +1824,36 @@ __global__ __launch_bounds__(256) void k_cut_heavy(DGraph g, i64 nedges, const i
   block_add_u64(s, acc);
 }
 
+// Arc -> source vertex, built once per level for the arc-parallel kernels.
+// Only arcs below the effective end of the last row (its range ends at the
+// caller's nedges, gpu_part.cu:408) are sources of work; arcs past it keep -1.
+__global__ __launch_bounds__(256) void k_arc_source(DGraph g, i32* asrc) {
+  const int lane = lane_id();
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) asrc[i] = (i32)v;
+  }
+}
+
+// gpu_part.cu:400-422 as an arc-parallel stream over [0, e_end): every arc of
+// a row in the reference's range is visited once; sum in uint64 (its low 32
+// bits are the reference's int32 wrap).
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_cut_arcs(const i32* asrc, const i32* adj, const i32* w,
+                                                  i64 n, i64 e_end, const i32* part, i32 k,
+                                                  u64* acc) {
+  u64 s = 0;
+  for (i64 i = gtid(); i < e_end; i += gstride()) {
+    const i32 v = asrc[i];
+    const i32 u = adj[i];
+    if (u < n) {
+      const i32 pv = part[v], pu = part[u];
+      if (pv != pu && pv >= 0 && pv < k && pu >= 0 && pu < k) s += UNIT ? 1ull : (u64)(i64)w[i];
+    }
+  }
+  block_add_u64(s, acc);
+}
+
 }  // namespace
 
 // ================================================================ launchers ==
@@ -1264,41 +1904,37 @@ void launch_level_census(DGraph g, i64 nedges, i64 heavy_deg, i32* heavy_list,
       g, nedges, heavy_deg, heavy_list, heavy_count, nonunit);
 }
 
-void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i32* list,
-                       i32* counters, hipStream_t s) {
-  k_match_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, pend, R, list);
-  k_set_counters<<<1, 1, 0, s>>>(counters, (i32)n, 0);
+void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i64* lo,
+                       hipStream_t s) {
+  k_match_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, pend, R);
+  k_fill<i64><<<1, 1, 0, s>>>(lo, 1, 0);
 }
 
-void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* prop,
-                        i32* list_in, i32* list_out, i32* counters, u32 round,
-                        hipStream_t s) {
-  // counters[(round-1)&1] holds the input list length, counters[round&1] the output.
-  i32* cin = counters + ((round - 1) & 1);
-  i32* cout = counters + (round & 1);
-  const unsigned gr = grid_for(g.n, MATCH_BS, 4096);
-  k_match_reserve<<<gr, MATCH_BS, 0, s>>>(g, mate, dec, pend, R, prop, list_in, cin, list_out,
-                                          cout, round);
-  k_match_decide<<<grid_for(g.n, MATCH_BS, 2048), MATCH_BS, 0, s>>>(list_out, cout, R, prop,
-                                                                     pend, mate, dec, cin,
-                                                                     round);
+void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* prop, i64* lo,
+                        i64 W, u32 round, hipStream_t s) {
+  k_match_reserve<<<grid_for(W, MATCH_BS, 4096), MATCH_BS, 0, s>>>(g, mate, dec, pend, R, prop,
+                                                                   lo, W, round);
+  k_match_decide<<<grid_for(W, MATCH_BS, 4096), MATCH_BS, 0, s>>>(R, prop, pend, mate, dec, lo,
+                                                                  W, g.n, round);
 }
 
-void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* mp, i32* prop, i32* list,
-                           i32* counters, hipStream_t s) {
-  k_match_sym_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, mp, prop, list);
-  k_set_counters<<<1, 1, 0, s>>>(counters, (i32)n, 0);
+void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop, i64* lo,
+                           hipStream_t s) {
+  k_match_sym_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, blkA, blkC, prop);
+  k_fill<i64><<<1, 3, 0, s>>>(lo, 3, 0);
 }
 
-void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* mp, i32* prop, i32* decision,
-                            i32* list_in, i32* list_out, i32* counters, u32 round,
+void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
+                            i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s) {
-  i32* cin = counters + ((round - 1) & 1);
-  i32* cout = counters + (round & 1);
-  k_match_sym_decide<<<grid_for(g.n, MATCH_BS, 4096), MATCH_BS, 0, s>>>(
-      g, mate, dec, mp, prop, decision, list_in, cin, list_out, cout);
-  k_match_sym_apply<<<grid_for(g.n, MATCH_BS, 4096), MATCH_BS, 0, s>>>(list_out, cout, decision,
-                                                                      mate, dec, cin);
+  i32* cur = nreq + (round & 1);
+  i32* next = nreq + ((round + 1) & 1);
+  const int par = (int)(round & 1);
+  k_match_sym_decide<<<grid_for(W * (WAVE / SYM_VPW), MATCH_BS, 4096), MATCH_BS, 0, s>>>(
+      g, mate, dec, blkA, blkC, prop, decision, lo, W, req, cur, next, par);
+  k_match_sym_apply<<<grid_for(W, MATCH_BS, 4096), MATCH_BS, 0, s>>>(decision, mate, dec, lo, W,
+                                                                     g.n, par);
+  k_match_huge<<<256, 1024, 0, s>>>(g, mate, dec, blkA, blkC, prop, req, cur);
 }
 
 void launch_sym_check(DGraph g, u64* table, i64 cap, i32* missing, hipStream_t s) {
@@ -1367,26 +2003,75 @@ void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_
 
 i64 gain_gscratch_elems(i32 k) { return k <= MAXK_LDS ? 0 : (i64)65536 * k; }
 
+void launch_bg_order(const float* bg, i32 k, i32* ord, i32* run_end, hipStream_t s) {
+  k_bg_order<<<1, 1024, 0, s>>>(bg, k, ord, run_end);
+}
+
+void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
+                        i32* nmedium, i32* huge, i32* nhuge, hipStream_t s) {
+  if (nheavy > 0)
+    k_split_heavy<<<grid_for(nheavy, 256, 4096), 256, 0, s>>>(g, nedges, heavy, nheavy, medium,
+                                                              nmedium, huge, nhuge);
+}
+
 void launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, u8* cand, int phase, int unit,
-                 i32* gscratch, hipStream_t s) {
+                 i32* gscratch, const i32* medium, i64 nmedium, const i32* huge, i64 nhuge,
+                 hipStream_t s) {
+  if (k <= 512) {
+    // light rows: 8 lanes per vertex, 32 tallies of k ints per block
+    const size_t shm8 = (size_t)(2 * k + 32 * k) * 4;
+    const unsigned gr8 = grid_for(g.n, 32, 1u << 16);
+    if (unit)
+      k_gain_group<8, true><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain, target,
+                                                    cand, phase, nullptr, 0, GAIN_LIGHT);
+    else
+      k_gain_group<8, false><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                     target, cand, phase, nullptr, 0, GAIN_LIGHT);
+    if (nmedium > 0) {
+      const size_t shm64 = (size_t)(2 * k + 4 * k) * 4;
+      const unsigned gr64 = grid_for(nmedium, 4, 1u << 16);
+      if (unit)
+        k_gain_group<64, true><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                         target, cand, phase, medium, nmedium, 0);
+      else
+        k_gain_group<64, false><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                          target, cand, phase, medium, nmedium, 0);
+    }
+    if (nhuge > 0) {
+      const size_t shmh = (size_t)(3 * k) * 4;
+      const unsigned grh = grid_for(nhuge, 1, 8192);
+      if (unit)
+        k_gain_huge<true><<<grh, HUGE_BS, shmh, s>>>(g, nedges, part, k, bg, src_big, gain, target,
+                                                     cand, phase, huge, nhuge);
+      else
+        k_gain_huge<false><<<grh, HUGE_BS, shmh, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                      target, cand, phase, huge, nhuge);
+    }
+    return;
+  }
+  // large k: one wave per vertex for every vertex
   const bool lds = k <= MAXK_LDS;
   const size_t shm = lds ? (size_t)k * 12 : 0;
-  const unsigned gr = grid_for(g.n, 1, 65536);
+  const unsigned gr = 65536;
   if (lds) {
     if (unit)
       k_gain<true, true><<<gr, WAVE, shm, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                               cand, phase, gscratch);
+                                               cand, phase, gscratch, nullptr, g.n, nullptr,
+                                               nullptr);
     else
       k_gain<false, true><<<gr, WAVE, shm, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                                cand, phase, gscratch);
+                                                cand, phase, gscratch, nullptr, g.n, nullptr,
+                                                nullptr);
   } else {
     if (unit)
       k_gain<true, false><<<gr, WAVE, 0, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                              cand, phase, gscratch);
+                                              cand, phase, gscratch, nullptr, g.n, nullptr,
+                                              nullptr);
     else
       k_gain<false, false><<<gr, WAVE, 0, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                               cand, phase, gscratch);
+                                               cand, phase, gscratch, nullptr, g.n, nullptr,
+                                               nullptr);
   }
 }
 
@@ -1445,6 +2130,19 @@ void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, con
                    const i32* mv_dst, i32* part, i32* pc, i32 k, hipStream_t s) {
   k_commit<<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src, mv_dst,
                                                       part, pc, k);
+}
+
+void launch_arc_source(DGraph g, i32* asrc, hipStream_t s) {
+  k_arc_source<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, asrc);
+}
+
+void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 k, int unit,
+                     u64* acc, hipStream_t s) {
+  const unsigned gr = grid_for(e_end, 256 * 8, 8192);
+  if (unit)
+    k_cut_arcs<true><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
+  else
+    k_cut_arcs<false><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
 }
 
 void launch_cut(DGraph g, i64 nedges, const i32* part, i32 k, u64* acc, const i32* heavy_list,
