@@ -220,10 +220,14 @@ struct Context {
   i64* pinned = nullptr;  // small host staging area
   static constexpr size_t PINNED_ELEMS = 8192 + 64;
   Prof prof;
+  i64 chain_budget = 0;  // wall-clock ticks one chained-matching launch may run
 
   Context() {
     HIP_TRY(hipGetDevice(&device));
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    chain_budget = (i64)std::max(khz, 1000) * 100;  // 100 ms
     HIP_TRY(hipHostMalloc((void**)&pinned, PINNED_ELEMS * sizeof(i64), hipHostMallocDefault));
     prof.s = s;
   }
@@ -303,7 +307,8 @@ class Run {
 
   // shared scratch sized for the finest level
   DBuf<i64> scan_sc;
-  DBuf<i64> small;  // device scalars: [0] total, [1] cut acc, [2..] misc
+  DBuf<i64> small;  // device scalars: [0] total, [1] cut, [2..] misc
+  DBuf<u64> accs;   // accumulator slots (ACC_ELEMS), zero between uses
   DBuf<i32> small32;
 
   hipStream_t st() { return C.s; }
@@ -349,6 +354,9 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const i64 n = lv.n;
   static const i64 MATCH_WINDOW = std::max<i64>(64, env_i64("TGP_MATCH_WINDOW", 16384));
   static const bool trace_match = env_i64("TGP_MATCH_TRACE", 0) != 0;
+  // TGP_MATCH_CHAIN=0 selects the round-based kernels on symmetric levels
+  static const bool MATCH_CHAIN = env_i64("TGP_MATCH_CHAIN", 1) != 0;
+  static const int CHAIN_BLOCKS = (int)std::max<i64>(1, env_i64("TGP_CHAIN_BLOCKS", 512));
   const auto t_start = std::chrono::steady_clock::now();
   DBuf<u8> dec(&C.pool, n);
   DBuf<i32> s1(&C.pool, n), prop(&C.pool, n);
@@ -371,6 +379,40 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     launch_match_init(n, mate.p, dec.p, s1.p, R.p, lo.p, st());
   u32 round = 0;
   const DGraph g = lv.dg();
+  if (lv.symmetric && MATCH_CHAIN) {
+    // Chained matching; each launch is bounded by a time budget.  Should
+    // launches stop advancing the first undecided vertex, the round-based
+    // kernels below finish the level from the same (consistent) state.
+    DBuf<u32> ticket(&C.pool, 1);
+    DBuf<i32> stv(&C.pool, n);
+    launch_fill_i32(stv.p, n, -1, st());
+    i64 prev = -1;
+    int stuck = 0;
+    while (stuck < 3) {
+      HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
+      launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, C.chain_budget,
+                         CHAIN_BLOCKS, st());
+      round++;
+      HIP_TRY(hipMemcpyAsync(C.pinned, lo.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
+      C.sync();
+      if (C.pinned[0] >= n) break;
+      stuck = C.pinned[0] > prev ? 0 : stuck + 1;
+      prev = C.pinned[0];
+    }
+    launch_chain_export(stv.p, n, mate.p, dec.p, st());
+    if (C.pinned[0] >= n) {
+      C.prof.end(KID_MATCH, (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n);
+      match_rounds += round;
+      if (trace_match)
+        std::fprintf(stderr, "# match n=%lld m=%lld chained launches=%u ms=%.2f\n", (long long)n,
+                     (long long)lv.m, round,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                               t_start).count());
+      return;
+    }
+    // scan starts of the round kernels: both parity slots at the first undecided vertex
+    launch_fill_i64(lo.p + 1, 2, C.pinned[0], st());
+  }
   // Rounds are issued in batches; the window's low end is read back between them.
   int batch = 4;
   while (true) {
@@ -536,8 +578,8 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
 i64 Run::compute_cut(Level& lv, i64 nedges, const i32* part) {
   LevelAux& a = aux(lv, nedges);
   C.prof.begin(KID_CUT);
-  HIP_TRY(hipMemsetAsync(small.p + 1, 0, sizeof(i64), st()));
-  launch_cut_arcs(lv.dg(), a.e_end, a.asrc.p, part, k, a.unit, (u64*)(small.p + 1), st());
+  launch_cut_arcs(lv.dg(), a.e_end, a.asrc.p, part, k, a.unit, accs.p, st());
+  launch_acc_finish(accs.p, (u64*)(small.p + 1), st());
   C.prof.end(KID_CUT, (a.unit ? 12.0 : 16.0) * a.e_end);
   return wrap_cut((u64)read_i64(small.p + 1));
 }
@@ -626,36 +668,101 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     std::vector<i32> hpc((size_t)k);
     const double arc_b = unit ? 8.0 : 12.0;
 
+    DBuf<u64> cutacc(&C.pool, 1);
     for (int phase = 0; phase < 2; phase++) {
       const int maxPass = phase ? 40 : 30;               // gpu_part.cu:545, 647
       bool balanced = false;
-      for (int pass = 0; pass < maxPass && !(phase && balanced); pass++) {
-        HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
-        HIP_TRY(hipMemsetAsync(ctr.p, 0, 4 * sizeof(i32), st()));
-        C.prof.begin(KID_MISC);
-        launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
-        C.prof.end(KID_MISC, 0);
-        C.prof.begin(KID_GAIN);
-        launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p, phase, unit,
-                    gscr.p, ax.medium.p, ax.nmedium, ax.huge.p, ax.nhuge, st());
-        C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
-        C.prof.begin(KID_TOKEN);
-        launch_token(g, nedges, gain.p, cand.p, part.p, target.p, k, flag.p, heavy, nheavy,
-                     HEAVY_DEG, st());
-        C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
-        C.prof.begin(KID_COMPACT);
-        launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
-        launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
-                             mv_dst.p, mv_gain.p, outc.p, st());
-        C.prof.end(KID_COMPACT, 24.0 * n);
-        i64 M = read_i64(small.p);
-        M = std::min(M, n);                              // gpu_part.cu:585
+      // A pass's edge cut is the cut of the partition the next pass's gain
+      // kernel reads, so that kernel tallies it and each pass costs one host
+      // sync.  The next pass is therefore started before the previous pass's
+      // stop test (gpu_part.cu:632-640, 736-744) is known; until its commit it
+      // changes nothing, and it is dropped when that test ends the phase.
+      bool pending = false;  // the previous pass has committed; its cut, S and pc are in flight
+      i64 prevM = 0;
+      int prevPass = -1;
+      for (int pass = 0;; pass++) {
+        const bool run = pass < maxPass;
+        i32* ctr_cur = ctr.p + 2 * (pass & 1);
+        i32* ctr_prev = ctr.p + 2 * ((pass + 1) & 1);
+        if (run) {
+          HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
+          HIP_TRY(hipMemsetAsync(ctr_cur, 0, 2 * sizeof(i32), st()));
+          C.prof.begin(KID_MISC);
+          launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
+          C.prof.end(KID_MISC, 0);
+          C.prof.begin(KID_GAIN);
+          const bool fused = launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p,
+                                         cand.p, phase, unit, gscr.p, ax.medium.p, ax.nmedium,
+                                         ax.huge.p, ax.nhuge, pending ? accs.p : nullptr, st());
+          C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
+          if (pending && !fused) {
+            C.prof.begin(KID_CUT);
+            launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, accs.p, st());
+            C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
+          }
+          if (pending) launch_acc_finish(accs.p, cutacc.p, st());
+          C.prof.begin(KID_TOKEN);
+          launch_token(g, nedges, gain.p, cand.p, part.p, target.p, k, flag.p, heavy, nheavy,
+                       HEAVY_DEG, st());
+          C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
+          C.prof.begin(KID_COMPACT);
+          launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
+          launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
+                               mv_dst.p, mv_gain.p, outc.p, st());
+          C.prof.end(KID_COMPACT, 24.0 * n);
+          HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
+        } else if (pending) {
+          C.prof.begin(KID_CUT);
+          launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, accs.p, st());
+          launch_acc_finish(accs.p, cutacc.p, st());
+          C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
+        } else {
+          break;
+        }
+        if (pending) {
+          HIP_TRY(hipMemcpyAsync(C.pinned + 1, cutacc.p, sizeof(u64), hipMemcpyDeviceToHost,
+                                 st()));
+          HIP_TRY(hipMemcpyAsync(C.pinned + 2, ctr_prev, 2 * sizeof(i32), hipMemcpyDeviceToHost,
+                                 st()));
+          HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
+        }
+        C.sync();
+        if (pending) {
+          // the previous pass's bookkeeping, gpu_part.cu:605-640 / 707-744
+          pending = false;
+          const i64 cur = wrap_cut((u64)C.pinned[1]);
+          const i32 Ssel = ((const i32*)(C.pinned + 2))[0];
+          const i32 pads = ((const i32*)(C.pinned + 2))[1];
+          float score = 0.0f;                              // gpu_part.cu:621-626, 723-731
+          balanced = true;
+          for (i32 q = 0; q < k; q++) {
+            const float diff = (float)hpc[q] - Tf;
+            const float sq = diff * diff;
+            score = score + sq;
+            if (std::fabs(diff) > Tf * maxImbalance) balanced = false;
+          }
+          score = std::sqrt(score / (float)k);
+          if (cur < bestCut || ((float)cur <= (float)bestCut * 1.01f && score < bestScore)) {
+            bestCut = cur;
+            bestScore = score;
+            HIP_TRY(hipMemcpyAsync(best.p, part.p, n * sizeof(i32), hipMemcpyDeviceToDevice,
+                                   st()));
+          }
+          *cut = cur;
+          tr.push(TGP_TRACE_PASS, level, phase, prevPass, prevM, Ssel, cur, pads,
+                  O.check_level ? hash_dev(part.p, n) : 0);
+          const bool quiet = (prevM < n / 200) && (Ssel < n / 400);  // gpu_part.cu:634, 738
+          if (quiet && (phase == 0 || balanced)) break;
+          if (phase && balanced) break;                    // loop test, gpu_part.cu:647
+        }
+        if (!run) break;
+        i64 M = std::min<i64>(C.pinned[0], n);             // gpu_part.cu:585
         passes++;
-        if (M <= 0) break;                               // gpu_part.cu:638-640, 742-744
+        if (M <= 0) break;                                 // gpu_part.cu:638-640, 742-744
         C.prof.begin(KID_MISC);
         launch_capacity(pc.p, outc.p, k, n, phase, avail.p, st());
         C.prof.end(KID_MISC, 0);
-        i64 len = M;                                     // gpu_part.cu:487-497
+        i64 len = M;                                       // gpu_part.cu:487-497
         i64 padded = 1;
         while (padded < len) padded <<= 1;
         if (padded > n) padded = n;
@@ -664,42 +771,15 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st());
         C.prof.end(KID_SORT, 16.0 * (double)len);
         C.prof.begin(KID_SELECT);
-        launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr.p, ctr.p + 1,
-                      st());
+        launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr_cur,
+                      ctr_cur + 1, st());
         C.prof.end(KID_SELECT, 13.0 * M);
         C.prof.begin(KID_COMMIT);
         launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, pc.p, k, st());
         C.prof.end(KID_COMMIT, 17.0 * M);
-        C.prof.begin(KID_CUT);
-        HIP_TRY(hipMemsetAsync(small.p + 1, 0, sizeof(i64), st()));
-        launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, (u64*)(small.p + 1), st());
-        C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
-        HIP_TRY(hipMemcpyAsync(C.pinned, small.p + 1, sizeof(i64), hipMemcpyDeviceToHost, st()));
-        HIP_TRY(hipMemcpyAsync(C.pinned + 1, ctr.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
-        HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
-        C.sync();
-        const i64 cur = wrap_cut((u64)C.pinned[0]);
-        const i32 Ssel = ((const i32*)(C.pinned + 1))[0];
-        const i32 pads = ((const i32*)(C.pinned + 1))[1];
-        float score = 0.0f;                              // gpu_part.cu:621-626, 723-731
-        balanced = true;
-        for (i32 q = 0; q < k; q++) {
-          const float diff = (float)hpc[q] - Tf;
-          const float sq = diff * diff;
-          score = score + sq;
-          if (std::fabs(diff) > Tf * maxImbalance) balanced = false;
-        }
-        score = std::sqrt(score / (float)k);
-        if (cur < bestCut || ((float)cur <= (float)bestCut * 1.01f && score < bestScore)) {
-          bestCut = cur;
-          bestScore = score;
-          HIP_TRY(hipMemcpyAsync(best.p, part.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
-        }
-        *cut = cur;
-        tr.push(TGP_TRACE_PASS, level, phase, pass, M, Ssel, cur, pads,
-                O.check_level ? hash_dev(part.p, n) : 0);
-        const bool quiet = (M < n / 200) && (Ssel < n / 400);  // gpu_part.cu:634, 738
-        if (quiet && (phase == 0 || balanced)) break;
+        pending = true;
+        prevM = M;
+        prevPass = pass;
       }
       HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
       if (phase == 0) *cut = bestCut;
@@ -716,6 +796,8 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
   const i64 n = g->n, m = g->m;
   scan_sc = DBuf<i64>(&C.pool, scan_scratch_elems(std::max<i64>(n, 2) * 2 + 64));
   small = DBuf<i64>(&C.pool, 16);
+  accs = DBuf<u64>(&C.pool, ACC_ELEMS);
+  HIP_TRY(hipMemsetAsync(accs.p, 0, ACC_ELEMS * sizeof(u64), st()));
 
   // level 0
   std::unique_ptr<Level> l0(new Level());

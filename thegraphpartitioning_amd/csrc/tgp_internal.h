@@ -80,6 +80,14 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
                            hipStream_t s);
 // req/nreq: queue of deferred long-row scans (W pairs) and its two counters,
 // both zeroed by the caller before round 1.
+// Chained matching (symmetric levels) on the state st (-1 undecided, -2
+// decided unmatched, else the partner): one launch decides vertices from
+// lo[0] upward until all are decided or `budget` wall-clock ticks pass, then
+// sets lo[0] to the first undecided vertex (n when done).  *ticket must be 0.
+void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, i64* lo,
+                        u32* ticket, i64 budget, int blocks, hipStream_t s);
+// mate = st where st >= 0 else -1; dec = (st != -1)
+void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s);
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
                             i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s);
@@ -117,16 +125,18 @@ void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s);
 void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
                      hipStream_t s);
 // Per-pass order of parts by (balance term desc, part asc) and run ends.
-void launch_bg_order(const float* bg, i32 k, i32* ord, i32* run_end, hipStream_t s);
 // Heavy rows (> 64 arcs) split into medium (<= 4096) and huge lists.
 void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
                         i32* nmedium, i32* huge, i32* nhuge, hipStream_t s);
 // Gains: 8 lanes per light row, a wave per medium row, a block per huge row
 // (k <= 512); one wave per vertex for larger k.
-void launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+// When cut_acc (accumulator slots) is given and k <= 512, also adds the edge
+// cut of `part` (the value launch_cut_arcs would add) to it and returns true; otherwise
+// returns false and the caller computes the cut separately.
+bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, u8* cand, int phase, int unit,
                  i32* gscratch, const i32* medium, i64 nmedium, const i32* huge, i64 nhuge,
-                 hipStream_t s);
+                 u64* cut_acc, hipStream_t s);
 i64 gain_gscratch_elems(i32 k);
 void launch_token(DGraph g, i64 nedges, const float* gain, const u8* cand,
                   const i32* part, const i32* target, i32 k, i64* flag,
@@ -145,13 +155,19 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v,
                    const i32* mv_src, const i32* mv_dst, i32* part, i32* pc, i32 k,
                    hipStream_t s);
+// Accumulators for block sums: ACC_SLOTS slots, one per 128-byte line, all
+// zero between uses.  Kernels add into them; launch_acc_finish writes the
+// total to *out and clears the slots.
+constexpr int ACC_SLOTS = 256;
+constexpr int ACC_STRIDE = 16;
+constexpr i64 ACC_ELEMS = (i64)ACC_SLOTS * ACC_STRIDE;
+void launch_acc_finish(u64* acc, u64* out, hipStream_t s);
 // asrc[i] = source vertex of arc i (built once per level)
 void launch_arc_source(DGraph g, i32* asrc, hipStream_t s);
 // Edge cut over arcs [0, e_end) with e_end = max(xadj[n-1], nedges): exactly
 // the arcs of the reference's row ranges (gpu_part.cu:404-417).
+// (acc: accumulator slots)
 void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 k, int unit,
                      u64* acc, hipStream_t s);
-void launch_cut(DGraph g, i64 nedges, const i32* part, i32 k, u64* acc,
-                const i32* heavy_list, i64 heavy_count, i64 heavy_deg, hipStream_t s);
 
 }  // namespace tgp

@@ -39,6 +39,26 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v) {
   }
   return v;
 }
+
+// Block-wide sum of one u64 per thread, added by thread 0 to one of the
+// ACC_SLOTS slots of `acc` (chosen by block index, one 128-byte line each) so
+// that thousands of blocks do not serialise on a single address; k_acc_finish
+// folds the slots.  Every thread of the block must call it.
+__device__ __forceinline__ void block_add_u64(u64 v, u64* acc) {
+  __shared__ u64 sw[16];
+  v = wave_sum(v);
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  if (lane == 0) sw[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sw[w];
+    if (s)
+      atomicAdd((unsigned long long*)&acc[(blockIdx.x & (ACC_SLOTS - 1)) * ACC_STRIDE],
+                (unsigned long long)s);
+  }
+}
+
 __device__ __forceinline__ u64 wave_or_u64(u64 v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
@@ -276,7 +296,32 @@ constexpr i32 BLK_NONE = -2;
 // so a long row costs about len/(64*SCAN_U) memory round trips.
 constexpr int SCAN_U = 8;
 
+// Device-scope (coherent) accesses to the matching state, for kernels in
+// which other waves update it while they run.
+__device__ __forceinline__ u8 ld_dec(const u8* dec, i64 i) {
+  return __hip_atomic_load(const_cast<u8*>(dec) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ i32 ld_mate(const i32* mate, i64 i) {
+  return __hip_atomic_load(const_cast<i32*>(mate) + i, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dec(u8* dec, i64 i) {
+  __hip_atomic_store(dec + i, (u8)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_mate(i32* mate, i64 i, i32 v) {
+  __hip_atomic_store(mate + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool COH>
+__device__ __forceinline__ u8 get_dec(const u8* dec, i64 i) {
+  return COH ? ld_dec(dec, i) : dec[i];
+}
+template <bool COH>
+__device__ __forceinline__ i32 get_mate(const i32* mate, i64 i) {
+  return COH ? ld_mate(mate, i) : mate[i];
+}
+
 // largest undecided neighbour u of x with u < lim, u != x (BLK_NONE if none)
+template <bool COH = false>
 __device__ __forceinline__ i32 wave_max_pending_below(const DGraph& g, i32 x, i32 lim,
                                                       const u8* dec, int lane) {
   const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
@@ -290,7 +335,8 @@ __device__ __forceinline__ i32 wave_max_pending_below(const DGraph& g, i32 x, i3
     }
     u8 d[SCAN_U];
 #pragma unroll
-    for (int j = 0; j < SCAN_U; j++) d[j] = (u[j] < lim && u[j] != x) ? dec[u[j]] : (u8)1;
+    for (int j = 0; j < SCAN_U; j++)
+      d[j] = (u[j] < lim && u[j] != x) ? get_dec<COH>(dec, u[j]) : (u8)1;
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++)
       if (!d[j]) best = u[j] > best ? u[j] : best;
@@ -303,6 +349,7 @@ __device__ __forceinline__ i32 wave_max_pending_below(const DGraph& g, i32 x, i3
   return best;
 }
 
+template <bool COH = false>
 __device__ __forceinline__ i32 wave_best_free(const DGraph& g, i32 t, const i32* mate, int lane) {
   const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
   u64 best = 0;
@@ -316,7 +363,7 @@ __device__ __forceinline__ i32 wave_best_free(const DGraph& g, i32 t, const i32*
     }
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++)
-      mt[j] = (u[j] >= 0 && u[j] < g.n && u[j] != t && w[j] >= 0) ? mate[u[j]] : 0;
+      mt[j] = (u[j] >= 0 && u[j] < g.n && u[j] != t && w[j] >= 0) ? get_mate<COH>(mate, u[j]) : 0;
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++) {
       if (mt[j] == -1) {
@@ -547,6 +594,374 @@ __global__ void k_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC
     blkC[i] = BLK_UNKNOWN;
     prop[i] = BLK_UNKNOWN;
   }
+}
+
+// Chained matching for symmetric levels: the decision rule of the rounds
+// above (conditions A, B, C with cached blockers and proposals) without
+// rounds.  The state is one word per vertex, st[x] = -1 (undecided), -2
+// (decided, unmatched) or the partner, and changes once.  Waves take groups
+// of CH_VPW consecutive vertices in ascending order from a ticket counter,
+// keep each vertex's blocker and proposal in registers, and re-examine the
+// group until every vertex is decided, publishing a commit t-u at once as
+// st[t] = u, st[u] = t.  No ordering between the two stores is needed,
+// because whoever must know that u is taken observes it in a word it has to
+// read anyway: a vertex that waits for its lower neighbours (A) sees a
+// neighbour y with st[y] == itself when y took it, and a vertex that waits for
+// the lower neighbours of its proposal u (C) sees st[y] == u when y took u.
+// Every condition is monotone in the decided set, so a decision taken on a
+// later view than another wave's is still the sequential one; the only
+// requirement on the memory system is per-word coherence (device-scope
+// relaxed atomics).  Two vertices of one wave never commit on each other: the
+// larger waits for the smaller by A or C.  The minimum undecided vertex always
+// passes A-C and groups are handed out in ascending order, so the group that
+// holds it makes progress.  Every wave also stops when the launch's time
+// budget is spent; the host then launches again from the first undecided
+// vertex, with the caches written back.
+constexpr i32 ST_UNDEC = -1;
+constexpr i32 ST_EXHAUST = -2;
+constexpr i32 BLK_TAKEN = -3;  // scan result: the scanned word names the asker
+constexpr i32 PROP_NONE = -5;  // proposal: no free neighbour (final)
+
+__device__ __forceinline__ i32 ld_st(const i32* st, i64 i) { return ld_mate(st, i); }
+
+// Among neighbours v of x with v < lim, v != x: BLK_TAKEN if some decided v
+// has st[v] == me, else the largest undecided v, else BLK_NONE.
+__device__ __forceinline__ i32 wave_scan_pending(const DGraph& g, i32 x, i32 lim, i32 me,
+                                                 const i32* st, int lane) {
+  const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
+  i32 best = BLK_NONE;
+  bool taken = false;
+  for (i64 base = s0; base < s1; base += WAVE * SCAN_U) {
+    i32 v[SCAN_U];
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      const i64 i = base + j * WAVE + lane;
+      v[j] = i < s1 ? g.adj[i] : INT_MAX;
+    }
+    i32 sv[SCAN_U];
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) sv[j] = (v[j] < lim && v[j] != x) ? ld_st(st, v[j]) : -4;
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      if (sv[j] == ST_UNDEC) best = v[j] > best ? v[j] : best;
+      taken |= sv[j] == me;
+    }
+  }
+  if (__ballot(taken)) return BLK_TAKEN;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const i32 t2 = __shfl_xor(best, o);
+    best = t2 > best ? t2 : best;
+  }
+  return best;
+}
+
+// heaviest free (st < 0) neighbour of t by (weight, first position); -1 if none
+__device__ __forceinline__ i32 wave_best_free_st(const DGraph& g, i32 t, const i32* st,
+                                                 int lane) {
+  const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
+  u64 best = 0;
+  for (i64 base = s0; base < s1; base += WAVE * SCAN_U) {
+    i32 v[SCAN_U], w[SCAN_U], sv[SCAN_U];
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      const i64 i = base + j * WAVE + lane;
+      v[j] = i < s1 ? g.adj[i] : -1;
+      w[j] = i < s1 ? g.w[i] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++)
+      sv[j] = (v[j] >= 0 && v[j] < g.n && v[j] != t && w[j] >= 0) ? ld_st(st, v[j]) : 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_U; j++) {
+      if (sv[j] < 0) {
+        const i64 i = base + j * WAVE + lane;
+        const u64 c = ((u64)(u32)w[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+        best = c > best ? c : best;
+      }
+    }
+  }
+  best = wave_max_u64(best);
+  return best ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)best)] : -1;
+}
+
+// Sub-group layout: CH_SG lanes per vertex, CH_VPW vertices per wave.  A row
+// of at most CH_ROW arcs is held in registers (CH_ROW / CH_SG entries per
+// lane), so a rescan of it is one batch of state loads and all sub-groups of
+// a wave rescan at once.  Longer rows are scanned by the whole wave, one row
+// at a time.
+constexpr int CH_SG = 8;
+constexpr int CH_VPW = WAVE / CH_SG;
+constexpr int CH_PER = 8;
+constexpr int CH_ROW = CH_SG * CH_PER;
+
+__device__ __forceinline__ i32 sg_max_i32(i32 v) {
+#pragma unroll
+  for (int o = CH_SG / 2; o > 0; o >>= 1) {
+    const i32 t = __shfl_xor(v, o, CH_SG);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+__device__ __forceinline__ bool sg_any(bool f) {
+  i32 v = f;
+#pragma unroll
+  for (int o = CH_SG / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, CH_SG);
+  return v != 0;
+}
+
+__global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC,
+                                                     i32* prop, const i64* lo_io, u32* ticket,
+                                                     i64 budget) {
+  const i64 n = g.n;
+  const i64 lo = lo_io[0];
+  const int lane = lane_id();
+  const int sl = lane & (CH_SG - 1);
+  const int sgbase = lane & ~(CH_SG - 1);
+  const i64 t_start = wall_clock64();
+  i64 base = -1;
+  // per-vertex state, uniform within a sub-group
+  bool own = false, tshort = false, ushort_ = false, uknown = false;
+  i32 t = -1, a = BLK_NONE, u = -1, b = BLK_NONE;
+  i64 ts0 = 0;
+  i32 tv[CH_PER], tw[CH_PER], uv[CH_PER];
+  for (u32 iter = 0; iter < (1u << 26); iter++) {
+    if (base < 0) {
+      u32 tk = 0;
+      if (lane == 0) tk = atomicAdd(ticket, 1u);
+      tk = __shfl(tk, 0);
+      base = lo + (i64)tk * CH_VPW;
+      if (base >= n) break;
+      const i64 e = base + (lane / CH_SG);
+      own = e < n;
+      t = own ? (i32)e : -1;
+      a = own ? blkA[t] : BLK_NONE;
+      u = own ? prop[t] : -1;
+      b = own ? blkC[t] : BLK_NONE;
+      uknown = false;
+      ts0 = own ? g.xadj[t] : 0;
+      const i64 ts1 = own ? g.xadj[t + 1] : 0;
+      tshort = own && ts1 - ts0 <= CH_ROW;
+#pragma unroll
+      for (int j = 0; j < CH_PER; j++) {
+        const i64 i = ts0 + j * CH_SG + sl;
+        tv[j] = (tshort && i < ts1) ? g.adj[i] : -1;
+        tw[j] = (tshort && i < ts1) ? g.w[i] : -1;
+      }
+    }
+    // one batch of coherent loads: lanes 0-3 of the sub-group read st of t, a, u, b
+    i32 mine = ST_UNDEC;
+    if (own) {
+      const i32 x = sl == 0 ? t : sl == 1 ? a : sl == 2 ? u : sl == 3 ? b : -1;
+      if (sl < 4 && x >= 0) mine = ld_st(st, x);
+    }
+    const i32 s_t = __shfl(mine, 0, CH_SG), s_a = __shfl(mine, 1, CH_SG);
+    i32 s_u = __shfl(mine, 2, CH_SG);
+    const i32 s_b = __shfl(mine, 3, CH_SG);
+    own = own && s_t == ST_UNDEC;
+    if (!__ballot(own)) {
+      base = -1;
+      continue;
+    }
+    if (wall_clock64() - t_start > budget) break;
+
+    if (own && a >= 0 && s_a != ST_UNDEC) {
+      if (s_a == t) own = false;  // a took t
+      else a = BLK_UNKNOWN;
+    }
+    if (own && u >= 0 && s_u >= 0) u = BLK_UNKNOWN;  // proposal taken
+    // A and proposal over a row held in registers: one batch of loads
+    const bool needAP = own && (a == BLK_UNKNOWN || u == BLK_UNKNOWN);
+    if (__ballot(needAP && tshort)) {
+      i32 sv[CH_PER];
+#pragma unroll
+      for (int j = 0; j < CH_PER; j++)
+        sv[j] = (needAP && tshort && tv[j] >= 0 && tv[j] < n && tv[j] != t) ? ld_st(st, tv[j])
+                                                                          : -4;
+      i32 bestA = BLK_NONE;
+      bool tk = false;
+      u64 key = 0;
+      i32 kv = -1, ks = 0;
+#pragma unroll
+      for (int j = 0; j < CH_PER; j++) {
+        if (sv[j] == -4) continue;
+        if (tv[j] < t) {
+          if (sv[j] == ST_UNDEC) bestA = tv[j] > bestA ? tv[j] : bestA;
+          tk |= sv[j] == t;
+        }
+        if (sv[j] < 0 && tw[j] >= 0) {
+          const u64 c = ((u64)(u32)tw[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(j * CH_SG + sl));
+          if (c > key) {
+            key = c;
+            kv = tv[j];
+            ks = sv[j];
+          }
+        }
+      }
+      bestA = sg_max_i32(bestA);
+      tk = sg_any(tk);
+#pragma unroll
+      for (int o = CH_SG / 2; o > 0; o >>= 1) {
+        const u64 ok = __shfl_xor(key, o, CH_SG);
+        const i32 ov = __shfl_xor(kv, o, CH_SG), os = __shfl_xor(ks, o, CH_SG);
+        if (ok > key) {
+          key = ok;
+          kv = ov;
+          ks = os;
+        }
+      }
+      if (needAP && tshort) {
+        if (a == BLK_UNKNOWN) {
+          if (tk) own = false;
+          else a = bestA;
+        }
+        if (u == BLK_UNKNOWN) {
+          u = key ? kv : PROP_NONE;
+          s_u = key ? ks : ST_UNDEC;
+          b = BLK_UNKNOWN;
+          uknown = false;
+        }
+      }
+    }
+    // long rows: whole-wave scans, one sub-group at a time
+    u64 need = __ballot(own && !tshort && sl == 0 && a == BLK_UNKNOWN);
+    while (need) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(t, l);
+      const i32 r = wave_scan_pending(g, x, x, x, st, lane);
+      if ((lane & ~(CH_SG - 1)) == l) {
+        if (r == BLK_TAKEN) own = false;
+        else a = r;
+      }
+      need &= need - 1;
+    }
+    need = __ballot(own && !tshort && sl == 0 && u == BLK_UNKNOWN);
+    while (need) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(t, l);
+      const i32 r = wave_best_free_st(g, x, st, lane);
+      const i32 sr = r >= 0 ? ld_st(st, r) : ST_UNDEC;
+      if ((lane & ~(CH_SG - 1)) == l) {
+        u = r >= 0 ? r : PROP_NONE;
+        s_u = sr;
+        b = BLK_UNKNOWN;
+        uknown = false;
+      }
+      need &= need - 1;
+    }
+    // condition B: u above t, or u below t and decided unmatched
+    bool okB = own && u >= 0 && (u > t || s_u == ST_EXHAUST);
+    // condition C: no undecided neighbour of u below t (and none of them took u)
+    if (okB && b >= 0 && s_b != ST_UNDEC) {
+      if (s_b == u) {
+        u = BLK_UNKNOWN;
+        okB = false;
+      } else {
+        b = BLK_UNKNOWN;
+      }
+    }
+    const bool needC = okB && b == BLK_UNKNOWN;
+    const bool fetch = needC && !uknown;
+    if (__ballot(fetch)) {  // the proposal's row, once per proposal
+      i64 us0 = 0, us1 = 0;
+      if (fetch) {
+        us0 = g.xadj[u];
+        us1 = g.xadj[u + 1];
+        ushort_ = us1 - us0 <= CH_ROW;
+        uknown = true;
+      }
+#pragma unroll
+      for (int j = 0; j < CH_PER; j++) {
+        const i64 i = us0 + j * CH_SG + sl;
+        if (fetch && ushort_) uv[j] = i < us1 ? g.adj[i] : -1;
+      }
+    }
+    if (__ballot(needC && ushort_)) {
+      i32 sv[CH_PER];
+#pragma unroll
+      for (int j = 0; j < CH_PER; j++)
+        sv[j] = (needC && ushort_ && uv[j] >= 0 && uv[j] < t && uv[j] != u) ? ld_st(st, uv[j])
+                                                                          : -4;
+      i32 bestC = BLK_NONE;
+      bool tk = false;
+#pragma unroll
+      for (int j = 0; j < CH_PER; j++) {
+        if (sv[j] == ST_UNDEC) bestC = uv[j] > bestC ? uv[j] : bestC;
+        tk |= sv[j] == u;
+      }
+      bestC = sg_max_i32(bestC);
+      tk = sg_any(tk);
+      if (needC && ushort_) {
+        if (tk) {
+          u = BLK_UNKNOWN;
+          okB = false;
+        } else {
+          b = bestC;
+        }
+      }
+    }
+    need = __ballot(needC && !ushort_ && sl == 0);
+    while (need) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(u, l);
+      const i32 lim = __shfl(t, l);
+      const i32 r = wave_scan_pending(g, x, lim, x, st, lane);
+      if ((lane & ~(CH_SG - 1)) == l) {
+        if (r == BLK_TAKEN) {
+          u = BLK_UNKNOWN;
+          okB = false;
+        } else {
+          b = r;
+        }
+      }
+      need &= need - 1;
+    }
+    const bool ready = own && a == BLK_NONE && (u == PROP_NONE || (okB && b == BLK_NONE));
+    if (ready && sl == 0) {
+      if (u >= 0) {
+        st_mate(st, t, u);
+        st_mate(st, u, t);
+      } else {
+        st_mate(st, t, ST_EXHAUST);
+      }
+    }
+    if (ready) own = false;
+    if (!__ballot(ready)) __builtin_amdgcn_s_sleep(2);
+  }
+  if (own && sl == 0) {  // out of time: keep the caches for the next launch
+    blkA[t] = a;
+    prop[t] = u;
+    blkC[t] = b;
+  }
+}
+
+// st -> mate (and dec for the round kernels)
+__global__ void k_chain_export(const i32* st, i64 n, i32* mate, u8* dec) {
+  for (i64 i = gtid(); i < n; i += gstride()) {
+    const i32 v = st[i];
+    mate[i] = v >= 0 ? v : -1;
+    dec[i] = v != ST_UNDEC;
+  }
+}
+
+// lo_io[0] = first vertex at or above lo_io[0] with st == -1 (one block)
+__global__ __launch_bounds__(1024) void k_first_undecided_st(const i32* st, i64 n, i64* lo_io) {
+  __shared__ i64 s_found;
+  i64 res = n;
+  for (i64 base0 = lo_io[0]; base0 < n; base0 += blockDim.x) {
+    if (threadIdx.x == 0) s_found = LLONG_MAX;
+    __syncthreads();
+    const i64 i = base0 + threadIdx.x;
+    if (i < n && st[i] == ST_UNDEC) atomicMin((unsigned long long*)&s_found, (unsigned long long)i);
+    __syncthreads();
+    const i64 f = s_found;
+    __syncthreads();
+    if (f != LLONG_MAX) {
+      res = f;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) lo_io[0] = res;
 }
 
 // Symmetry test of a level: every arc s->x (s != x, x < n) must have x->s.
@@ -1004,212 +1419,12 @@ __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* 
   }
 }
 
-// Per-pass order of the parts by (balance term desc, part asc) for both
-// source-size variants, and for every position the end of its run of equal
-// balance terms.  For a vertex whose tally of part q is zero the gain is
-// (0 - c_cur) + b_q, which is non-decreasing in b_q, so its best such part is
-// found by walking runs from the top (first allowed part of a run = lowest q).
-__global__ __launch_bounds__(1024) void k_bg_order(const float* bg, i32 k, i32* ord,
-                                                   i32* run_end) {
-  __shared__ u64 key[MAXK_LDS];
-  __shared__ i32 rend[MAXK_LDS];
-  i32 P = 1;
-  while (P < k) P <<= 1;
-  for (int s = 0; s < 2; s++) {
-    const float* b = bg + s * k;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
-      if (i < k) {
-        const u32 bits = __float_as_uint(b[i] + 0.0f);  // -0 -> +0
-        const u32 ordk = (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);  // ascending
-        key[i] = ((u64)(~ordk) << 32) | (u32)i;  // descending balance term, then ascending q
-      } else {
-        key[i] = ~0ull;
-      }
-    }
-    __syncthreads();
-    for (i32 kk = 2; kk <= P; kk <<= 1) {
-      for (i32 j = kk >> 1; j > 0; j >>= 1) {
-        for (i32 t = threadIdx.x; t < P / 2; t += blockDim.x) {
-          const i32 lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-          const i32 hi = lo + j;
-          const bool asc = (lo & kk) == 0;
-          const u64 a = key[lo], c = key[hi];
-          if ((a > c) == asc) {
-            key[lo] = c;
-            key[hi] = a;
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
-      const i32 q = (i32)(key[i] & 0xffffffffu);
-      ord[s * k + i] = q;
-      const bool last = (i == k - 1) || (b[(i32)(key[i + 1] & 0xffffffffu)] != b[q]);
-      rend[i] = last ? i + 1 : INT_MAX;
-    }
-    __syncthreads();
-    for (i32 d = 1; d < k; d <<= 1) {
-      i32 v[4];
-      int c = 0;
-      for (int i = threadIdx.x; i < k; i += blockDim.x, c++) v[c] = (i + d < k) ? rend[i + d] : INT_MAX;
-      __syncthreads();
-      c = 0;
-      for (int i = threadIdx.x; i < k; i += blockDim.x, c++) rend[i] = v[c] < rend[i] ? v[c] : rend[i];
-      __syncthreads();
-    }
-    for (int i = threadIdx.x; i < k; i += blockDim.x) run_end[s * k + i] = rend[i];
-    __syncthreads();
-  }
-}
-
 // One thread per vertex whose range has at most GAIN_LIGHT arcs: the tallies
 // of up to GAIN_SLOTS distinct parts live in registers and are accumulated
 // in row order (exactly the reference's fp32 sums).  Vertices with more
 // distinct parts are deferred to the wave kernel.
 constexpr int GAIN_SLOTS = 16;
 constexpr int GAIN_LIGHT = 64;
-
-template <bool UNIT>
-__global__ __launch_bounds__(256) void k_gain_light(DGraph g, i64 nedges, const i32* part, i32 k,
-                                                    const float* bg, const u8* src_big,
-                                                    const i32* ord, const i32* run_end,
-                                                    float* gain, i32* target, u8* cand,
-                                                    int phase, i32* defer, i32* ndefer) {
-  const int lane = lane_id();
-  for (i64 base = gtid() - lane; base < g.n; base += gstride()) {
-    const i64 v = base + lane;
-    bool mine = false, over = false;
-    i32 cur = -1;
-    i32 keys[GAIN_SLOTS];
-    float vals[GAIN_SLOTS];
-    int cnt = 0;
-#pragma unroll
-    for (int j = 0; j < GAIN_SLOTS; j++) {
-      keys[j] = -1;
-      vals[j] = 0.0f;
-    }
-    if (v < g.n) {
-      const i64 s0 = g.xadj[v];
-      const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-      cur = part[v];
-      mine = (s1 - s0 <= GAIN_LIGHT) && cur >= 0 && cur < k;
-      if (mine) {
-        for (i64 i = s0; i < s1; i++) {
-          const i32 u = g.adj[i];
-          if (u >= g.n) continue;
-          const i32 p = part[u];
-          if (p < 0 || p >= k) continue;
-          const float wt = UNIT ? 1.0f : (float)g.w[i];
-          bool found = false;
-#pragma unroll
-          for (int j = 0; j < GAIN_SLOTS; j++) {
-            if (keys[j] == p) {
-              vals[j] += wt;
-              found = true;
-            }
-          }
-          if (!found) {
-            if (cnt < GAIN_SLOTS) {
-#pragma unroll
-              for (int j = 0; j < GAIN_SLOTS; j++) {
-                if (j == cnt) {
-                  keys[j] = p;
-                  vals[j] = wt;
-                }
-              }
-              cnt++;
-            } else {
-              over = true;
-              break;
-            }
-          }
-        }
-      }
-    }
-    // defer vertices with too many distinct parts (one atomic per wave)
-    const u64 om = __ballot(mine && over);
-    if (om) {
-      i32 ob = 0;
-      if (lane == 0) ob = atomicAdd(ndefer, (i32)__popcll(om));
-      ob = __shfl(ob, 0);
-      if (mine && over) defer[ob + __popcll(om & lanes_below(lane))] = (i32)v;
-    }
-    if (!mine || over) continue;
-    float ccur = 0.0f;
-#pragma unroll
-    for (int j = 0; j < GAIN_SLOTS; j++)
-      if (keys[j] == cur) ccur = vals[j];
-    const int sv = src_big[cur] ? 1 : 0;
-    const float* bgs = bg + sv * k;
-    // best over the parts present in the row
-    float bgv = -INFINITY;
-    i32 bq = INT_MAX;
-#pragma unroll
-    for (int j = 0; j < GAIN_SLOTS; j++) {
-      const i32 q = keys[j];
-      if (q >= 0 && q != cur) {
-        const float eg = vals[j] - ccur;
-        const float gg = eg + bgs[q];
-        if (gg > bgv || (gg == bgv && q < bq)) {
-          bgv = gg;
-          bq = q;
-        }
-      }
-    }
-    // best over the parts absent from the row (tally 0.0f)
-    const float e0 = 0.0f - ccur;
-    const i32* o = ord + sv * k;
-    const i32* re = run_end + sv * k;
-    bool found = false;
-    float g1 = 0.0f;
-    i32 q1 = INT_MAX;
-    for (i32 i = 0; i < k;) {
-      const i32 e = re[i];
-      i32 j = i;
-      i32 q = -1;
-      for (; j < e; j++) {
-        q = o[j];
-        bool tch = (q == cur);
-#pragma unroll
-        for (int t = 0; t < GAIN_SLOTS; t++) tch |= (keys[t] == q);
-        if (!tch) break;
-      }
-      if (j < e) {
-        const float gg = e0 + bgs[q];
-        if (!found) {
-          found = true;
-          g1 = gg;
-          q1 = q;
-        } else if (gg == g1) {
-          q1 = q < q1 ? q : q1;
-        } else {
-          break;
-        }
-      }
-      i = e;
-    }
-    if (found && (g1 > bgv || (g1 == bgv && q1 < bq))) {
-      bgv = g1;
-      bq = q1;
-    }
-    float best;
-    i32 tgt;
-    if (bq == INT_MAX || bgv < -1e9f) {
-      best = -1e9f;
-      tgt = cur;
-    } else if (bgv == -1e9f) {
-      best = -1e9f;
-      tgt = bq < cur ? bq : cur;
-    } else {
-      best = bgv;
-      tgt = bq;
-    }
-    gain[v] = best;
-    target[v] = tgt;
-    cand[v] = phase ? (best > -50.0f) : (best > -10.0f);
-  }
-}
 
 // gpu_part.cu:258-312 with G lanes per vertex: each group keeps a dense LDS
 // tally of k ints (exact while the row's sum of |w| <= 2^24, else one lane
@@ -1223,9 +1438,10 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
                                                     const float* bg_g, const u8* src_big,
                                                     float* gain, i32* target, u8* cand,
                                                     int phase, const i32* list, i64 cnt,
-                                                    i64 light_max) {
+                                                    i64 light_max, u64* cut_acc) {
   extern __shared__ i32 lds[];
   constexpr int GPB = 256 / G;  // groups per block
+  u64 cutv = 0;  // weight of this thread's arcs whose ends lie in different parts
   float* sbg = (float*)lds;
   for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
   __syncthreads();
@@ -1253,6 +1469,7 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
           const i32 w = UNIT ? 1 : g.w[i];
           atomicAdd(&tally[p], w);
           if (!UNIT) absw += w < 0 ? -(i64)w : (i64)w;
+          if (p != cur) cutv += (u64)(i64)w;
         }
       }
     }
@@ -1318,6 +1535,7 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
     }
     __builtin_amdgcn_wave_barrier();
   }
+  if (cut_acc) block_add_u64(cutv, cut_acc);
 }
 
 // Rows longer than GAIN_HUGE arcs: one block of 1024 threads per vertex with
@@ -1330,7 +1548,8 @@ __global__ __launch_bounds__(HUGE_BS) void k_gain_huge(DGraph g, i64 nedges, con
                                                        i32 k, const float* bg_g,
                                                        const u8* src_big, float* gain,
                                                        i32* target, u8* cand, int phase,
-                                                       const i32* list, i64 cnt) {
+                                                       const i32* list, i64 cnt, u64* cut_acc) {
+  u64 cutv = 0;
   extern __shared__ i32 lds[];
   __shared__ i64 s_abs[HUGE_BS / WAVE];
   __shared__ float s_g[HUGE_BS / WAVE];
@@ -1364,6 +1583,7 @@ __global__ __launch_bounds__(HUGE_BS) void k_gain_huge(DGraph g, i64 nedges, con
         if (p[j] >= 0 && p[j] < k) {
           atomicAdd(&tally[p[j]], w[j]);
           if (!UNIT) absw += w[j] < 0 ? -(i64)w[j] : (i64)w[j];
+          if (p[j] != cur) cutv += (u64)(i64)w[j];
         }
       }
     }
@@ -1444,6 +1664,7 @@ __global__ __launch_bounds__(HUGE_BS) void k_gain_huge(DGraph g, i64 nedges, con
     }
     __syncthreads();
   }
+  if (cut_acc) block_add_u64(cutv, cut_acc);
 }
 
 // Heavy rows split into medium (64 < len <= 4096, one wave) and huge (> 4096,
@@ -1770,59 +1991,6 @@ __global__ void k_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_
 // -------------------------------------------------------------------- cut --
 // gpu_part.cu:400-422: sum of w over arcs whose ends lie in different parts.
 // Summed in uint64 (the int32 wrap of the reference is its low 32 bits).
-__device__ __forceinline__ void block_add_u64(u64 v, u64* acc) {
-  __shared__ u64 sw[16];
-  v = wave_sum(v);
-  const int lane = lane_id(), wv = threadIdx.x >> 6;
-  if (lane == 0) sw[wv] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    u64 s = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sw[w];
-    if (s) atomicAdd((unsigned long long*)acc, (unsigned long long)s);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_cut_thread(DGraph g, i64 nedges, const i32* part,
-                                                    i32 k, u64* acc, i64 heavy_deg) {
-  u64 s = 0;
-  for (i64 v = gtid(); v < g.n; v += gstride()) {
-    const i64 s0 = g.xadj[v];
-    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    if (s1 - s0 > heavy_deg) continue;
-    const i32 pv = part[v];
-    if (pv < 0 || pv >= k) continue;
-    for (i64 i = s0; i < s1; i++) {
-      const i32 u = g.adj[i];
-      if (u < g.n) {
-        const i32 pu = part[u];
-        if (pu != pv && pu >= 0 && pu < k) s += (u64)(i64)g.w[i];
-      }
-    }
-  }
-  block_add_u64(s, acc);
-}
-
-__global__ __launch_bounds__(256) void k_cut_heavy(DGraph g, i64 nedges, const i32* part,
-                                                   i32 k, u64* acc, const i32* list, i64 cnt) {
-  u64 s = 0;
-  const int lane = lane_id();
-  for (i64 e = gwave(); e < cnt; e += nwaves()) {
-    const i32 v = list[e];
-    const i32 pv = part[v];
-    if (pv < 0 || pv >= k) continue;
-    const i64 s0 = g.xadj[v];
-    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    for (i64 i = s0 + lane; i < s1; i += WAVE) {
-      const i32 u = g.adj[i];
-      if (u < g.n) {
-        const i32 pu = part[u];
-        if (pu != pv && pu >= 0 && pu < k) s += (u64)(i64)g.w[i];
-      }
-    }
-  }
-  block_add_u64(s, acc);
-}
 
 // Arc -> source vertex, built once per level for the arc-parallel kernels.
 // Only arcs below the effective end of the last row (its range ends at the
@@ -1852,6 +2020,22 @@ __global__ __launch_bounds__(256) void k_cut_arcs(const i32* asrc, const i32* ad
     }
   }
   block_add_u64(s, acc);
+}
+
+// *out = sum of the accumulator slots; the slots are cleared for the next use.
+__global__ __launch_bounds__(ACC_SLOTS) void k_acc_finish(u64* acc, u64* out) {
+  u64* slot = &acc[threadIdx.x * ACC_STRIDE];
+  u64 v = *slot;
+  *slot = 0;
+  __shared__ u64 sw[ACC_SLOTS / WAVE];
+  v = wave_sum(v);
+  if (lane_id() == 0) sw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t = 0;
+    for (int w = 0; w < ACC_SLOTS / WAVE; w++) t += sw[w];
+    *out = t;
+  }
 }
 
 }  // namespace
@@ -1922,6 +2106,16 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
                            hipStream_t s) {
   k_match_sym_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, blkA, blkC, prop);
   k_fill<i64><<<1, 3, 0, s>>>(lo, 3, 0);
+}
+
+void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, i64* lo,
+                        u32* ticket, i64 budget, int blocks, hipStream_t s) {
+  k_match_chain<<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget);
+  k_first_undecided_st<<<1, 1024, 0, s>>>(st, g.n, lo);
+}
+
+void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s) {
+  k_chain_export<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(st, n, mate, dec);
 }
 
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
@@ -2003,10 +2197,6 @@ void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_
 
 i64 gain_gscratch_elems(i32 k) { return k <= MAXK_LDS ? 0 : (i64)65536 * k; }
 
-void launch_bg_order(const float* bg, i32 k, i32* ord, i32* run_end, hipStream_t s) {
-  k_bg_order<<<1, 1024, 0, s>>>(bg, k, ord, run_end);
-}
-
 void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
                         i32* nmedium, i32* huge, i32* nhuge, hipStream_t s) {
   if (nheavy > 0)
@@ -2014,41 +2204,41 @@ void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32*
                                                               nmedium, huge, nhuge);
 }
 
-void launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, u8* cand, int phase, int unit,
                  i32* gscratch, const i32* medium, i64 nmedium, const i32* huge, i64 nhuge,
-                 hipStream_t s) {
+                 u64* cut_acc, hipStream_t s) {
   if (k <= 512) {
     // light rows: 8 lanes per vertex, 32 tallies of k ints per block
     const size_t shm8 = (size_t)(2 * k + 32 * k) * 4;
     const unsigned gr8 = grid_for(g.n, 32, 1u << 16);
     if (unit)
       k_gain_group<8, true><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                                    cand, phase, nullptr, 0, GAIN_LIGHT);
+                                                    cand, phase, nullptr, 0, GAIN_LIGHT, cut_acc);
     else
       k_gain_group<8, false><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                     target, cand, phase, nullptr, 0, GAIN_LIGHT);
+                                                     target, cand, phase, nullptr, 0, GAIN_LIGHT, cut_acc);
     if (nmedium > 0) {
       const size_t shm64 = (size_t)(2 * k + 4 * k) * 4;
       const unsigned gr64 = grid_for(nmedium, 4, 1u << 16);
       if (unit)
         k_gain_group<64, true><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                         target, cand, phase, medium, nmedium, 0);
+                                                         target, cand, phase, medium, nmedium, 0, cut_acc);
       else
         k_gain_group<64, false><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                          target, cand, phase, medium, nmedium, 0);
+                                                          target, cand, phase, medium, nmedium, 0, cut_acc);
     }
     if (nhuge > 0) {
       const size_t shmh = (size_t)(3 * k) * 4;
       const unsigned grh = grid_for(nhuge, 1, 8192);
       if (unit)
         k_gain_huge<true><<<grh, HUGE_BS, shmh, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                                     cand, phase, huge, nhuge);
+                                                     cand, phase, huge, nhuge, cut_acc);
       else
         k_gain_huge<false><<<grh, HUGE_BS, shmh, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                      target, cand, phase, huge, nhuge);
+                                                      target, cand, phase, huge, nhuge, cut_acc);
     }
-    return;
+    return cut_acc != nullptr;
   }
   // large k: one wave per vertex for every vertex
   const bool lds = k <= MAXK_LDS;
@@ -2073,6 +2263,7 @@ void launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
                                                cand, phase, gscratch, nullptr, g.n, nullptr,
                                                nullptr);
   }
+  return false;  // the large-k kernels do not tally the cut
 }
 
 void launch_token(DGraph g, i64 nedges, const float* gain, const u8* cand, const i32* part,
@@ -2136,6 +2327,10 @@ void launch_arc_source(DGraph g, i32* asrc, hipStream_t s) {
   k_arc_source<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, asrc);
 }
 
+void launch_acc_finish(u64* acc, u64* out, hipStream_t s) {
+  k_acc_finish<<<1, ACC_SLOTS, 0, s>>>(acc, out);
+}
+
 void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 k, int unit,
                      u64* acc, hipStream_t s) {
   const unsigned gr = grid_for(e_end, 256 * 8, 8192);
@@ -2143,14 +2338,6 @@ void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 
     k_cut_arcs<true><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
   else
     k_cut_arcs<false><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
-}
-
-void launch_cut(DGraph g, i64 nedges, const i32* part, i32 k, u64* acc, const i32* heavy_list,
-                i64 heavy_count, i64 heavy_deg, hipStream_t s) {
-  k_cut_thread<<<grid_for(g.n, 256, 8192), 256, 0, s>>>(g, nedges, part, k, acc, heavy_deg);
-  if (heavy_count > 0)
-    k_cut_heavy<<<grid_for(heavy_count, 4, 8192), 256, 0, s>>>(g, nedges, part, k, acc,
-                                                               heavy_list, heavy_count);
 }
 
 }  // namespace tgp
