@@ -49,3 +49,23 @@ def random_graph(n, m, seed, sym=True, wmax=1, dup=False, selfloops=False):
     w = rng.integers(1, wmax + 1, size=len(edges)) if wmax > 1 else None
     xadj, adj, wt = csr_from_edges(n, edges, sym=sym, weights=w, sort_rows=not dup)
     return xadj, adj, wt
+
+
+def hub_graph(n, hubs, extra, seed, wmax=1, wbig=0):
+    """Vertices 0..hubs-1 are hubs joined to every vertex (rows of n-1 arcs);
+    `extra` random edges among the others; weights 1..wmax, or wbig on hub
+    arcs when given (so a hub row's sum of |w| can pass 2**24)."""
+    rng = np.random.default_rng(seed)
+    edges = set()
+    for h in range(hubs):
+        for v in range(h + 1, n):
+            edges.add((h, v))
+    while len(edges) < extra + sum(n - 1 - h for h in range(hubs)):
+        u, v = int(rng.integers(hubs, n)), int(rng.integers(hubs, n))
+        if u != v:
+            edges.add((min(u, v), max(u, v)))
+    edges = sorted(edges)
+    w = rng.integers(1, wmax + 1, size=len(edges))
+    if wbig:
+        w = np.where(np.array([e[0] < hubs for e in edges]), wbig, w)
+    return csr_from_edges(n, edges, weights=w)

@@ -128,3 +128,39 @@ def test_deterministic_repeat():
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
     assert a[2] == b[2]
+
+
+@pytest.mark.parametrize("repair", [False, True])
+@pytest.mark.parametrize("spec", [(20000, 1, 30000, 1, 0, 8), (40000, 3, 50000, 5, 0, 16),
+                                  (40000, 2, 40000, 3, 1000, 4)])
+def test_long_rows(oracle, spec, repair):
+    """Rows past 4096 and 16384 arcs (several chunks per row) and a row whose
+    sum of |w| passes 2**24 (the fp32 fallback)."""
+    n, hubs, extra, wmax, wbig, k = spec
+    xadj, adj, w = graphs.hub_graph(n, hubs, extra, 7, wmax=wmax, wbig=wbig)
+    gp, op = run_both(oracle, xadj, adj, w, k, repair=repair, floor=200)
+    assert_same(gp, op, f"hubs {spec} repair={repair}")
+
+
+def test_rmat16_repaired(oracle):
+    g = tgp.gen_rmat(16, 16, 0x7C0DE5EED + 16)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 64, repair=True)
+    assert_same(gp, op, "rmat16 k=64 repair")
+
+
+@pytest.mark.parametrize("budget_us", ["1", "40"])
+def test_chained_matching_resume_and_fallback(oracle, monkeypatch, budget_us):
+    """A tiny time budget makes every chained launch stop early: 40 us resumes
+    over many launches from the written-back caches, 1 us makes no progress and
+    hands the level to the round kernels."""
+    monkeypatch.setenv("TGP_CHAIN_BUDGET_US", budget_us)
+    g = tgp.gen_rmat(13, 8, 0x7C0DE5EED + 3)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 16, repair=True, floor=100)
+    assert_same(gp, op, f"budget {budget_us} us")
+
+
+def test_round_matching_on_symmetric_levels(oracle, monkeypatch):
+    monkeypatch.setenv("TGP_MATCH_CHAIN", "0")
+    g = tgp.gen_rmat(13, 8, 0x7C0DE5EED + 5)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 16, repair=True, floor=100)
+    assert_same(gp, op, "round-based matching")

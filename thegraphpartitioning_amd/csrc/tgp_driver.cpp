@@ -198,6 +198,25 @@ struct LevelAux {
   DBuf<i32> heavy;    // heavy rows
   DBuf<i32> medium, huge;  // heavy rows split at 4096 arcs
   i64 nmedium = 0, nhuge = 0;
+  DBuf<i32> ch_row;          // chunks of the long rows: row index in `huge`, arc range
+  DBuf<i64> ch_beg, ch_end;
+  i64 nch = 0;
+  DBuf<i32> gtally;          // nhuge * k tallies and nhuge |w| sums, zero between passes
+  DBuf<i64> gabs;
+  GainLists lists() const {
+    GainLists L;
+    L.medium = medium.p;
+    L.nmedium = nmedium;
+    L.huge = huge.p;
+    L.nhuge = nhuge;
+    L.ch_row = ch_row.p;
+    L.ch_beg = ch_beg.p;
+    L.ch_end = ch_end.p;
+    L.nch = nch;
+    L.gtally = gtally.p;
+    L.gabs = gabs.p;
+    return L;
+  }
 };
 
 struct Level {
@@ -220,14 +239,14 @@ struct Context {
   i64* pinned = nullptr;  // small host staging area
   static constexpr size_t PINNED_ELEMS = 8192 + 64;
   Prof prof;
-  i64 chain_budget = 0;  // wall-clock ticks one chained-matching launch may run
+  i64 clock_khz = 0;  // device wall-clock rate
 
   Context() {
     HIP_TRY(hipGetDevice(&device));
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     int khz = 0;
     HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
-    chain_budget = (i64)std::max(khz, 1000) * 100;  // 100 ms
+    clock_khz = std::max(khz, 1000);
     HIP_TRY(hipHostMalloc((void**)&pinned, PINNED_ELEMS * sizeof(i64), hipHostMallocDefault));
     prof.s = s;
   }
@@ -354,9 +373,13 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const i64 n = lv.n;
   static const i64 MATCH_WINDOW = std::max<i64>(64, env_i64("TGP_MATCH_WINDOW", 16384));
   static const bool trace_match = env_i64("TGP_MATCH_TRACE", 0) != 0;
-  // TGP_MATCH_CHAIN=0 selects the round-based kernels on symmetric levels
-  static const bool MATCH_CHAIN = env_i64("TGP_MATCH_CHAIN", 1) != 0;
-  static const int CHAIN_BLOCKS = (int)std::max<i64>(1, env_i64("TGP_CHAIN_BLOCKS", 512));
+  // Tuning and test knobs, read at every call: TGP_MATCH_CHAIN=0 selects the
+  // round-based kernels on symmetric levels; TGP_CHAIN_BLOCKS sets the chained
+  // kernel's grid; TGP_CHAIN_BUDGET_US bounds one chained launch (100 ms).
+  const bool MATCH_CHAIN = env_i64("TGP_MATCH_CHAIN", 1) != 0;
+  const int CHAIN_BLOCKS = (int)std::max<i64>(1, env_i64("TGP_CHAIN_BLOCKS", 512));
+  const i64 chain_budget =
+      std::max<i64>(1, C.clock_khz * env_i64("TGP_CHAIN_BUDGET_US", 100000) / 1000);
   const auto t_start = std::chrono::steady_clock::now();
   DBuf<u8> dec(&C.pool, n);
   DBuf<i32> s1(&C.pool, n), prop(&C.pool, n);
@@ -390,7 +413,7 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     int stuck = 0;
     while (stuck < 3) {
       HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
-      launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, C.chain_budget,
+      launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, chain_budget,
                          CHAIN_BLOCKS, st());
       round++;
       HIP_TRY(hipMemcpyAsync(C.pinned, lo.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
@@ -569,6 +592,38 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   C.sync();
   a.nmedium = ((const i32*)(C.pinned + 2))[0];
   a.nhuge = ((const i32*)(C.pinned + 2))[1];
+  // chunks of the long rows, built on the host (nhuge <= m / 4096)
+  a.nch = 0;
+  if (a.nhuge > 0) {
+    DBuf<i64> hb(&C.pool, 2 * a.nhuge);
+    launch_huge_bounds(lv.dg(), nedges, a.huge.p, a.nhuge, hb.p, st());
+    std::vector<i64> h((size_t)(2 * a.nhuge));
+    HIP_TRY(hipMemcpyAsync(h.data(), hb.p, h.size() * sizeof(i64), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    std::vector<i32> row;
+    std::vector<i64> beg, end;
+    for (i64 r = 0; r < a.nhuge; r++)
+      for (i64 e = h[2 * r]; e < h[2 * r + 1]; e += GAIN_CHUNK_ARCS) {
+        row.push_back((i32)r);
+        beg.push_back(e);
+        end.push_back(std::min(e + GAIN_CHUNK_ARCS, h[2 * r + 1]));
+      }
+    a.nch = (i64)row.size();
+    a.ch_row.ensure(&C.pool, a.nch);
+    a.ch_beg.ensure(&C.pool, a.nch);
+    a.ch_end.ensure(&C.pool, a.nch);
+    HIP_TRY(hipMemcpyAsync(a.ch_row.p, row.data(), a.nch * sizeof(i32), hipMemcpyHostToDevice,
+                           st()));
+    HIP_TRY(hipMemcpyAsync(a.ch_beg.p, beg.data(), a.nch * sizeof(i64), hipMemcpyHostToDevice,
+                           st()));
+    HIP_TRY(hipMemcpyAsync(a.ch_end.p, end.data(), a.nch * sizeof(i64), hipMemcpyHostToDevice,
+                           st()));
+    a.gtally.ensure(&C.pool, a.nhuge * k);
+    a.gabs.ensure(&C.pool, a.nhuge);
+    HIP_TRY(hipMemsetAsync(a.gtally.p, 0, a.nhuge * k * sizeof(i32), st()));
+    HIP_TRY(hipMemsetAsync(a.gabs.p, 0, a.nhuge * sizeof(i64), st()));
+    C.sync();  // the host vectors are freed on return
+  }
   a.e_end = std::max<i64>(C.pinned[1], nedges);  // rows 0..n-2 whole, last ends at nedges
   a.nedges = nedges;
   a.valid = true;
@@ -659,7 +714,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     DBuf<float> gain(&C.pool, n), mv_gain(&C.pool, n), sk_gain(&C.pool, n);
     DBuf<i32> target(&C.pool, n), mv_v(&C.pool, n), mv_src(&C.pool, n), mv_dst(&C.pool, n),
         sk_idx(&C.pool, n), sdst(&C.pool, n);
-    DBuf<u8> cand(&C.pool, n), sel(&C.pool, n), src_big(&C.pool, k);
+    DBuf<float> cand(&C.pool, n);
+    DBuf<u8> sel(&C.pool, n), src_big(&C.pool, k);
     DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
     DBuf<float> bg(&C.pool, 2 * (size_t)k);
     DBuf<i32> outc(&C.pool, k), ctr(&C.pool, 4);
@@ -692,8 +748,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           C.prof.end(KID_MISC, 0);
           C.prof.begin(KID_GAIN);
           const bool fused = launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p,
-                                         cand.p, phase, unit, gscr.p, ax.medium.p, ax.nmedium,
-                                         ax.huge.p, ax.nhuge, pending ? accs.p : nullptr, st());
+                                         cand.p, phase, unit, gscr.p, ax.lists(),
+                                         pending ? accs.p : nullptr, st());
           C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
           if (pending && !fused) {
             C.prof.begin(KID_CUT);
@@ -702,8 +758,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           }
           if (pending) launch_acc_finish(accs.p, cutacc.p, st());
           C.prof.begin(KID_TOKEN);
-          launch_token(g, nedges, gain.p, cand.p, part.p, target.p, k, flag.p, heavy, nheavy,
-                       HEAVY_DEG, st());
+          launch_token(g, nedges, cand.p, part.p, target.p, k, flag.p, heavy, nheavy, HEAVY_DEG,
+                       st());
           C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
           C.prof.begin(KID_COMPACT);
           launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());

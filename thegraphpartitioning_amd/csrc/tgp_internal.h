@@ -133,14 +133,35 @@ void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32*
 // When cut_acc (accumulator slots) is given and k <= 512, also adds the edge
 // cut of `part` (the value launch_cut_arcs would add) to it and returns true; otherwise
 // returns false and the caller computes the cut separately.
+// Row classes of a level for the gain kernels: medium rows (64 < len <= 4096),
+// long rows (> 4096) and the long rows' chunks of at most GAIN_CHUNK_ARCS arcs
+// with per-row accumulators (nhuge * k tallies, nhuge |w| sums, all zero
+// between passes).
+constexpr i64 GAIN_CHUNK_ARCS = 16384;
+struct GainLists {
+  const i32* medium = nullptr;
+  i64 nmedium = 0;
+  const i32* huge = nullptr;
+  i64 nhuge = 0;
+  const i32* ch_row = nullptr;
+  const i64* ch_beg = nullptr;
+  const i64* ch_end = nullptr;
+  i64 nch = 0;
+  i32* gtally = nullptr;
+  i64* gabs = nullptr;
+};
 bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
-                 const u8* src_big, float* gain, i32* target, u8* cand, int phase, int unit,
-                 i32* gscratch, const i32* medium, i64 nmedium, const i32* huge, i64 nhuge,
-                 u64* cut_acc, hipStream_t s);
+                 const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
+                 i32* gscratch, const GainLists& L, u64* cut_acc, hipStream_t s);
+// hb[2h], hb[2h+1] = arc range of long row h (the last row ends at nedges)
+void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* hb,
+                        hipStream_t s);
 i64 gain_gscratch_elems(i32 k);
-void launch_token(DGraph g, i64 nedges, const float* gain, const u8* cand,
-                  const i32* part, const i32* target, i32 k, i64* flag,
-                  const i32* heavy_list, i64 heavy_count, i64 heavy_deg, hipStream_t s);
+// cand[v] = gain of v when v is a move candidate, NaN otherwise (the gain
+// kernels write it)
+void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
+                  i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
+                  hipStream_t s);
 void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part,
                           const i32* target, const float* gain, i64 n, i32 k, i32* mv_v,
                           i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,

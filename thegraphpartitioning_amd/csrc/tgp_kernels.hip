@@ -1267,6 +1267,8 @@ __global__ void k_project(const i32* tb, const i32* part_c, i64 n_c, i32* part_f
 }
 
 constexpr int MAXK_LDS = 4096;
+// cand[v] of a vertex that is not a move candidate (gains of candidates are finite)
+constexpr float CAND_NO = __builtin_nanf("");
 
 __global__ __launch_bounds__(256) void k_sanitize_hist(i32* part, i64 n, i32 k, i32* pc) {
   __shared__ i32 h[MAXK_LDS];
@@ -1317,7 +1319,7 @@ __global__ void k_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8
 template <bool UNIT, bool LDS>
 __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* part, i32 k,
                                                const float* bg_g, const u8* src_big,
-                                               float* gain, i32* target, u8* cand, int phase,
+                                               float* gain, i32* target, float* cand, int phase,
                                                i32* gscratch, const i32* list_a, i64 cnt_a,
                                                const i32* list_b, const i32* cnt_b) {
   extern __shared__ i32 lds[];
@@ -1413,7 +1415,7 @@ __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* 
       }
       gain[v] = best;
       target[v] = tgt;
-      cand[v] = phase ? (best > -50.0f) : (best > -10.0f);  // gpu_part.cu:308-312
+      cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;  // gpu_part.cu:308-312
     }
     __syncthreads();
   }
@@ -1436,7 +1438,7 @@ constexpr int GAIN_LIGHT = 64;
 template <int G, bool UNIT>
 __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const i32* part, i32 k,
                                                     const float* bg_g, const u8* src_big,
-                                                    float* gain, i32* target, u8* cand,
+                                                    float* gain, i32* target, float* cand,
                                                     int phase, const i32* list, i64 cnt,
                                                     i64 light_max, u64* cut_acc) {
   extern __shared__ i32 lds[];
@@ -1461,15 +1463,24 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
     for (int q = gl; q < k; q += G) tally[q] = 0;
     __builtin_amdgcn_wave_barrier();
     i64 absw = 0;
-    for (i64 i = s0 + gl; i < s1; i += G) {
-      const i32 u = g.adj[i];
-      if (u < g.n) {
-        const i32 p = part[u];
-        if (p < k && p >= 0) {
-          const i32 w = UNIT ? 1 : g.w[i];
-          atomicAdd(&tally[p], w);
-          if (!UNIT) absw += w < 0 ? -(i64)w : (i64)w;
-          if (p != cur) cutv += (u64)(i64)w;
+    // GU independent loads per lane in flight, then their gathers
+    constexpr int GU = G == 8 ? 8 : 4;
+    for (i64 base = s0; base < s1; base += G * GU) {
+      i32 uu[GU], ww[GU], pp[GU];
+#pragma unroll
+      for (int j = 0; j < GU; j++) {
+        const i64 i = base + j * G + gl;
+        uu[j] = i < s1 ? g.adj[i] : INT_MAX;
+        ww[j] = (UNIT || i >= s1) ? 1 : g.w[i];
+      }
+#pragma unroll
+      for (int j = 0; j < GU; j++) pp[j] = uu[j] < g.n ? part[uu[j]] : -1;
+#pragma unroll
+      for (int j = 0; j < GU; j++) {
+        if (pp[j] < k && pp[j] >= 0) {
+          atomicAdd(&tally[pp[j]], ww[j]);
+          if (!UNIT) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
+          if (pp[j] != cur) cutv += (u64)(i64)ww[j];
         }
       }
     }
@@ -1531,7 +1542,7 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
       }
       gain[v] = best;
       target[v] = tgt;
-      cand[v] = phase ? (best > -50.0f) : (best > -10.0f);
+      cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -1543,128 +1554,174 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
 constexpr int GAIN_HUGE = 4096;
 constexpr int HUGE_BS = 1024;
 
+// Rows longer than GAIN_HUGE arcs are cut into chunks of GAIN_CHUNK_ARCS arcs with
+// one block per chunk, so that a long row is spread over many CUs: a block
+// tallies its chunk in LDS and adds the tally, the chunk's sum of |w| and its
+// share of the cut to per-row accumulators in global memory.  Integer tallies
+// are independent of the order of the additions.
 template <bool UNIT>
-__global__ __launch_bounds__(HUGE_BS) void k_gain_huge(DGraph g, i64 nedges, const i32* part,
-                                                       i32 k, const float* bg_g,
-                                                       const u8* src_big, float* gain,
-                                                       i32* target, u8* cand, int phase,
-                                                       const i32* list, i64 cnt, u64* cut_acc) {
-  u64 cutv = 0;
+__global__ __launch_bounds__(HUGE_BS) void k_gain_chunk(DGraph g, const i32* part, i32 k,
+                                                        const i32* huge, const i32* ch_row,
+                                                        const i64* ch_beg, const i64* ch_end,
+                                                        i64 nch, i32* gtally, i64* gabs,
+                                                        u64* cut_acc) {
   extern __shared__ i32 lds[];
   __shared__ i64 s_abs[HUGE_BS / WAVE];
-  __shared__ float s_g[HUGE_BS / WAVE];
-  __shared__ i32 s_q[HUGE_BS / WAVE];
-  float* sbg = (float*)lds;
-  i32* tally = lds + 2 * k;
-  float* ftally = (float*)tally;
-  for (int q = threadIdx.x; q < 2 * k; q += HUGE_BS) sbg[q] = bg_g[q];
+  u64 cutv = 0;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  for (i64 e = blockIdx.x; e < cnt; e += gridDim.x) {
-    const i64 v = list[e];
-    const i64 s0 = g.xadj[v];
-    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    const i32 cur = part[v];
-    if (cur >= k || cur < 0) continue;
-    for (int q = threadIdx.x; q < k; q += HUGE_BS) tally[q] = 0;
+  for (i64 c = blockIdx.x; c < nch; c += gridDim.x) {
+    const i32 h = ch_row[c];
+    const i32 cur = part[huge[h]];
+    if (cur >= k || cur < 0) continue;  // gpu_part.cu:262 (same for the whole block)
+    for (int q = threadIdx.x; q < k; q += HUGE_BS) lds[q] = 0;
     __syncthreads();
+    const i64 e0 = ch_beg[c], e1 = ch_end[c];
     i64 absw = 0;
-    for (i64 base = s0; base < s1; base += 4 * HUGE_BS) {
-      i32 u[4], w[4], p[4];
+    for (i64 base = e0; base < e1; base += 4 * HUGE_BS) {
+      i32 uu[4], ww[4], pp[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const i64 i = base + j * HUGE_BS + threadIdx.x;
-        u[j] = i < s1 ? g.adj[i] : INT_MAX;
-        w[j] = (UNIT || i >= s1) ? 1 : g.w[i];
+        uu[j] = i < e1 ? g.adj[i] : INT_MAX;
+        ww[j] = (UNIT || i >= e1) ? 1 : g.w[i];
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) p[j] = u[j] < g.n ? part[u[j]] : -1;
+      for (int j = 0; j < 4; j++) pp[j] = uu[j] < g.n ? part[uu[j]] : -1;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        if (p[j] >= 0 && p[j] < k) {
-          atomicAdd(&tally[p[j]], w[j]);
-          if (!UNIT) absw += w[j] < 0 ? -(i64)w[j] : (i64)w[j];
-          if (p[j] != cur) cutv += (u64)(i64)w[j];
+        if (pp[j] >= 0 && pp[j] < k) {
+          atomicAdd(&lds[pp[j]], ww[j]);
+          if (!UNIT) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
+          if (pp[j] != cur) cutv += (u64)(i64)ww[j];
         }
       }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < k; q += HUGE_BS) {
+      const i32 tq = lds[q];
+      if (tq) atomicAdd(&gtally[(i64)h * k + q], tq);
     }
     if (!UNIT) {
       absw = wave_sum(absw);
       if (lane == 0) s_abs[wv] = absw;
-    }
-    __syncthreads();
-    i64 tot = 0;
-    if (!UNIT)
-      for (int w2 = 0; w2 < HUGE_BS / WAVE; w2++) tot += s_abs[w2];
-    const bool exact_int = UNIT ? (s1 - s0) <= (i64)(1 << 24) : tot <= (i64)(1 << 24);
-    if (!exact_int) {
-      __syncthreads();
-      for (int q = threadIdx.x; q < k; q += HUGE_BS) ftally[q] = 0.0f;
       __syncthreads();
       if (threadIdx.x == 0) {
-        for (i64 i = s0; i < s1; i++) {
-          const i32 uu = g.adj[i];
-          if (uu < g.n) {
-            const i32 pp = part[uu];
-            if (pp < k && pp >= 0) ftally[pp] += (float)g.w[i];
-          }
-        }
+        i64 tot = 0;
+        for (int w2 = 0; w2 < HUGE_BS / WAVE; w2++) tot += s_abs[w2];
+        if (tot) atomicAdd((unsigned long long*)&gabs[h], (unsigned long long)tot);
       }
-      __syncthreads();
-    }
-    const float ccur = exact_int ? (float)tally[cur] : ftally[cur];
-    const float* bgr = sbg + (src_big[cur] ? k : 0);
-    float bgv = -INFINITY;
-    i32 bq = INT_MAX;
-    for (int q = threadIdx.x; q < k; q += HUGE_BS) {
-      if (q == cur) continue;
-      const float cq = exact_int ? (float)tally[q] : ftally[q];
-      const float eg = cq - ccur;
-      const float gg = eg + bgr[q];
-      if (gg > bgv || (gg == bgv && q < bq)) {
-        bgv = gg;
-        bq = q;
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const float og = __shfl_xor(bgv, o);
-      const i32 oq = __shfl_xor(bq, o);
-      if (og > bgv || (og == bgv && oq < bq)) {
-        bgv = og;
-        bq = oq;
-      }
-    }
-    if (lane == 0) {
-      s_g[wv] = bgv;
-      s_q[wv] = bq;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int w2 = 1; w2 < HUGE_BS / WAVE; w2++) {
-        if (s_g[w2] > bgv || (s_g[w2] == bgv && s_q[w2] < bq)) {
-          bgv = s_g[w2];
-          bq = s_q[w2];
-        }
-      }
-      float best;
-      i32 tgt;
-      if (bq == INT_MAX || bgv < -1e9f) {
-        best = -1e9f;
-        tgt = cur;
-      } else if (bgv == -1e9f) {
-        best = -1e9f;
-        tgt = bq < cur ? bq : cur;
-      } else {
-        best = bgv;
-        tgt = bq;
-      }
-      gain[v] = best;
-      target[v] = tgt;
-      cand[v] = phase ? (best > -50.0f) : (best > -10.0f);
     }
     __syncthreads();
   }
   if (cut_acc) block_add_u64(cutv, cut_acc);
+}
+
+// Target and gain of every long row from its accumulated tally (a row whose
+// sum of |w| exceeds 2^24 is re-tallied in fp32 in row order by one thread, as
+// in the other kernels); the accumulators are cleared for the next pass.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_gain_huge_final(DGraph g, i64 nedges, const i32* part,
+                                                         i32 k, const float* bg_g,
+                                                         const u8* src_big, float* gain,
+                                                         i32* target, float* cand, int phase,
+                                                         const i32* huge, i64 nhuge,
+                                                         i32* gtally, i64* gabs) {
+  extern __shared__ i32 lds[];
+  __shared__ float s_g[256 / WAVE];
+  __shared__ i32 s_q[256 / WAVE];
+  float* sbg = (float*)lds;
+  float* ftally = (float*)(lds + 2 * k);
+  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
+  __syncthreads();
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 h = blockIdx.x; h < nhuge; h += gridDim.x) {
+    const i64 v = huge[h];
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    const i32 cur = part[v];
+    i32* gt = gtally + h * k;
+    if (cur < k && cur >= 0) {
+      const bool exact_int = UNIT ? (s1 - s0) <= (i64)(1 << 24) : gabs[h] <= (i64)(1 << 24);
+      if (!exact_int) {
+        for (int q = threadIdx.x; q < k; q += blockDim.x) ftally[q] = 0.0f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          for (i64 i = s0; i < s1; i++) {
+            const i32 uu = g.adj[i];
+            if (uu < g.n) {
+              const i32 pp = part[uu];
+              if (pp < k && pp >= 0) ftally[pp] += (float)g.w[i];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      const float ccur = exact_int ? (float)gt[cur] : ftally[cur];
+      const float* bgr = sbg + (src_big[cur] ? k : 0);
+      float bgv = -INFINITY;
+      i32 bq = INT_MAX;
+      for (int q = threadIdx.x; q < k; q += blockDim.x) {
+        if (q == cur) continue;
+        const float cq = exact_int ? (float)gt[q] : ftally[q];
+        const float eg = cq - ccur;
+        const float gg = eg + bgr[q];
+        if (gg > bgv || (gg == bgv && q < bq)) {
+          bgv = gg;
+          bq = q;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const float og = __shfl_xor(bgv, o);
+        const i32 oq = __shfl_xor(bq, o);
+        if (og > bgv || (og == bgv && oq < bq)) {
+          bgv = og;
+          bq = oq;
+        }
+      }
+      if (lane == 0) {
+        s_g[wv] = bgv;
+        s_q[wv] = bq;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int w2 = 1; w2 < (int)(blockDim.x / WAVE); w2++) {
+          if (s_g[w2] > bgv || (s_g[w2] == bgv && s_q[w2] < bq)) {
+            bgv = s_g[w2];
+            bq = s_q[w2];
+          }
+        }
+        float best;
+        i32 tgt;
+        if (bq == INT_MAX || bgv < -1e9f) {
+          best = -1e9f;
+          tgt = cur;
+        } else if (bgv == -1e9f) {
+          best = -1e9f;
+          tgt = bq < cur ? bq : cur;
+        } else {
+          best = bgv;
+          tgt = bq;
+        }
+        gain[v] = best;
+        target[v] = tgt;
+        cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < k; q += blockDim.x) gt[q] = 0;
+    if (threadIdx.x == 0) gabs[h] = 0;
+    __syncthreads();
+  }
+}
+
+// (s0, s1) of every long row, with the last row ending at nedges
+__global__ void k_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* hb) {
+  for (i64 h = gtid(); h < nhuge; h += gstride()) {
+    const i64 v = huge[h];
+    hb[2 * h] = g.xadj[v];
+    hb[2 * h + 1] = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+  }
 }
 
 // Heavy rows split into medium (64 < len <= 4096, one wave) and huge (> 4096,
@@ -1685,65 +1742,73 @@ __device__ __forceinline__ bool move_ok(i32 src, i32 dst, i32 k) {
   return src < k && dst < k && src != dst && src >= 0 && dst >= 0;
 }
 
-__global__ __launch_bounds__(256) void k_token_thread(DGraph g, i64 nedges, const float* gain,
-                                                      const u8* cand, const i32* part,
-                                                      const i32* target, i32 k, i64* flag,
-                                                      i64 heavy_deg) {
+// cand[u] is u's gain when u is a candidate and NaN otherwise, so one load
+// answers both questions and every comparison with a non-candidate is false.
+// Light rows (<= heavy_deg arcs): one thread per vertex (most vertices are not
+// candidates and stop at once), eight loads in flight per step.
+__global__ __launch_bounds__(256) void k_token_thread(DGraph g, i64 nedges, const float* cand,
+                                                      const i32* part, const i32* target, i32 k,
+                                                      i64* flag, i64 heavy_deg) {
+  constexpr int TU = 8;
   for (i64 v = gtid(); v < g.n; v += gstride()) {
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
     if (s1 - s0 > heavy_deg) continue;
-    if (!cand[v]) {
+    const float gv = cand[v];
+    if (!(gv == gv)) {
       flag[v] = 0;
       continue;
     }
-    const float gv = gain[v];
     const float gv10 = gv + 10.0f;
-    bool win = true;
-    for (i64 i = s0; i < s1; i++) {
-      const i32 u = g.adj[i];
-      if (u < g.n && cand[u]) {
-        const float gu = gain[u];
-        if (gu > gv10 || (gu == gv && u < v)) {
-          win = false;
-          break;
+    bool lose = false;
+    for (i64 base = s0; base < s1 && !lose; base += TU) {
+      i32 uu[TU];
+#pragma unroll
+      for (int j = 0; j < TU; j++) uu[j] = base + j < s1 ? g.adj[base + j] : INT_MAX;
+#pragma unroll
+      for (int j = 0; j < TU; j++) {
+        if (uu[j] < g.n) {
+          const float gu = cand[uu[j]];
+          lose |= gu > gv10 || (gu == gv && uu[j] < v);
         }
       }
     }
-    flag[v] = (win && move_ok(part[v], target[v], k)) ? 1 : 0;
+    flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
   }
 }
 
-__global__ __launch_bounds__(256) void k_token_heavy(DGraph g, i64 nedges, const float* gain,
-                                                     const u8* cand, const i32* part,
-                                                     const i32* target, i32 k, i64* flag,
-                                                     const i32* list, i64 cnt) {
+// Rows longer than heavy_deg: one wave per vertex, four loads per lane in flight.
+__global__ __launch_bounds__(256) void k_token_heavy(DGraph g, i64 nedges, const float* cand,
+                                                     const i32* part, const i32* target, i32 k,
+                                                     i64* flag, const i32* list, i64 cnt) {
   const int lane = lane_id();
   for (i64 e = gwave(); e < cnt; e += nwaves()) {
     const i32 v = list[e];
-    if (!cand[v]) {
+    const float gv = cand[v];
+    if (!(gv == gv)) {
       if (lane == 0) flag[v] = 0;
       continue;
     }
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    const float gv = gain[v];
     const float gv10 = gv + 10.0f;
     bool lose = false;
-    for (i64 base = s0; base < s1; base += WAVE) {
-      const i64 i = base + lane;
+    for (i64 base = s0; base < s1 && !lose; base += 4 * WAVE) {
+      i32 uu[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const i64 i = base + j * WAVE + lane;
+        uu[j] = i < s1 ? g.adj[i] : INT_MAX;
+      }
       bool l = false;
-      if (i < s1) {
-        const i32 u = g.adj[i];
-        if (u < g.n && cand[u]) {
-          const float gu = gain[u];
-          l = (gu > gv10 || (gu == gv && u < v));
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (uu[j] < g.n) {
+          const float gu = cand[uu[j]];
+          l |= gu > gv10 || (gu == gv && uu[j] < v);
         }
       }
-      if (__ballot(l)) {
-        lose = true;
-        break;
-      }
+      lose = __ballot(l) != 0;
     }
     if (lane == 0) flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
   }
@@ -1798,7 +1863,10 @@ __global__ void k_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int pha
 // pairs.  The element at the higher index of each pair compares itself with
 // the lower one: swap = (gain[hi] < gain[lo]) ^ ((hi & kk) == 0); pairs with
 // hi >= len are skipped.  Stages with stride < 2048 run inside LDS tiles.
-constexpr int BT_ELEMS = 2048;
+// Tiles of BT_TILE elements are sorted and merged in LDS (64 KiB: keys and
+// slots); strides of BT_TILE and more run as global passes that apply up to
+// three consecutive strides at once.
+constexpr int BT_TILE = 8192;
 constexpr int BT_THREADS = 1024;
 
 __global__ void k_sort_init(const float* mv_gain, i64 M, i64 len, float* key, i32* idx) {
@@ -1813,30 +1881,34 @@ __global__ void k_sort_init(const float* mv_gain, i64 M, i64 len, float* key, i3
   }
 }
 
+// One compare-exchange step with stride j (< BT_TILE) over a tile in LDS: the
+// pair (lo, lo + j) is exchanged when (key[hi] < key[lo]) differs from `up`,
+// and skipped when hi lies past len (gpu_part.cu:472-483).
 __device__ __forceinline__ void bt_stage_lds(float* sk, i32* si, i64 base, i64 len, i64 kk,
                                              int j) {
-  const int t = threadIdx.x;
-  const int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-  const int hi = lo + j;
-  if (base + hi < len) {
-    const bool up = ((base + hi) & kk) == 0;
-    const float a = sk[hi], b = sk[lo];
-    if ((a < b) ^ up) {
-      sk[hi] = b;
-      sk[lo] = a;
-      const i32 tt = si[hi];
-      si[hi] = si[lo];
-      si[lo] = tt;
+  for (int t = threadIdx.x; t < BT_TILE / 2; t += BT_THREADS) {
+    const int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+    const int hi = lo + j;
+    if (base + hi < len) {
+      const bool up = ((base + hi) & kk) == 0;
+      const float a = sk[hi], b = sk[lo];
+      if ((a < b) ^ up) {
+        sk[hi] = b;
+        sk[lo] = a;
+        const i32 tt = si[hi];
+        si[hi] = si[lo];
+        si[lo] = tt;
+      }
     }
   }
 }
 
 __global__ __launch_bounds__(BT_THREADS) void k_bt_local(float* key, i32* idx, i64 len,
                                                          i64 kk_fixed) {
-  __shared__ float sk[BT_ELEMS];
-  __shared__ i32 si[BT_ELEMS];
-  const i64 base = (i64)blockIdx.x * BT_ELEMS;
-  for (int e = threadIdx.x; e < BT_ELEMS; e += BT_THREADS) {
+  __shared__ float sk[BT_TILE];
+  __shared__ i32 si[BT_TILE];
+  const i64 base = (i64)blockIdx.x * BT_TILE;
+  for (int e = threadIdx.x; e < BT_TILE; e += BT_THREADS) {
     const i64 gi = base + e;
     if (gi < len) {
       sk[e] = key[gi];
@@ -1845,19 +1917,19 @@ __global__ __launch_bounds__(BT_THREADS) void k_bt_local(float* key, i32* idx, i
   }
   __syncthreads();
   if (kk_fixed == 0) {
-    for (i64 kk = 2; kk <= BT_ELEMS && kk <= len; kk <<= 1) {
+    for (i64 kk = 2; kk <= BT_TILE && kk <= len; kk <<= 1) {
       for (int j = (int)(kk >> 1); j > 0; j >>= 1) {
         bt_stage_lds(sk, si, base, len, kk, j);
         __syncthreads();
       }
     }
   } else {
-    for (int j = BT_ELEMS >> 1; j > 0; j >>= 1) {
+    for (int j = BT_TILE >> 1; j > 0; j >>= 1) {
       bt_stage_lds(sk, si, base, len, kk_fixed, j);
       __syncthreads();
     }
   }
-  for (int e = threadIdx.x; e < BT_ELEMS; e += BT_THREADS) {
+  for (int e = threadIdx.x; e < BT_TILE; e += BT_THREADS) {
     const i64 gi = base + e;
     if (gi < len) {
       key[gi] = sk[e];
@@ -1866,20 +1938,54 @@ __global__ __launch_bounds__(BT_THREADS) void k_bt_local(float* key, i32* idx, i
   }
 }
 
-__global__ __launch_bounds__(256) void k_bt_global(float* key, i32* idx, i64 len, i64 npairs,
-                                                   i64 kk, i64 j) {
-  for (i64 t = gtid(); t < npairs; t += gstride()) {
-    const i64 lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-    const i64 hi = lo + j;
-    if (hi >= len) continue;
-    const bool up = (hi & kk) == 0;
-    const float a = key[hi], b = key[lo];
-    if ((a < b) ^ up) {
-      key[hi] = b;
-      key[lo] = a;
-      const i32 tt = idx[hi];
-      idx[hi] = idx[lo];
-      idx[lo] = tt;
+// S consecutive strides j, j/2, ..., j >> (S-1) of one merge kk in a single
+// pass: every thread owns the 2^S elements whose positions differ only in
+// those stride bits (a set closed under the S steps) and applies the steps in
+// order in registers, which equals applying each step to the whole array.
+template <int S>
+__global__ __launch_bounds__(256) void k_bt_global(float* key, i32* idx, i64 len, i64 nsets,
+                                                   i64 kk, int blo) {
+  constexpr int R = 1 << S;
+  const i64 lowmask = ((i64)1 << blo) - 1;
+  for (i64 t = gtid(); t < nsets; t += gstride()) {
+    const i64 base = ((t >> blo) << (blo + S)) | (t & lowmask);
+    if (base >= len) continue;
+    float a[R];
+    i32 x[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const i64 p = base + ((i64)r << blo);
+      if (p < len) {
+        a[r] = key[p];
+        x[r] = idx[p];
+      }
+    }
+    const bool up = (base & kk) == 0;  // kk is above every stride bit
+#pragma unroll
+    for (int b = S - 1; b >= 0; b--) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (r & (1 << b)) continue;
+        const int rh = r | (1 << b);
+        if (base + ((i64)rh << blo) < len) {
+          if ((a[rh] < a[r]) ^ up) {
+            const float tf = a[rh];
+            a[rh] = a[r];
+            a[r] = tf;
+            const i32 ti = x[rh];
+            x[rh] = x[r];
+            x[r] = ti;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const i64 p = base + ((i64)r << blo);
+      if (p < len) {
+        key[p] = a[r];
+        idx[p] = x[r];
+      }
     }
   }
 }
@@ -1907,17 +2013,36 @@ __global__ __launch_bounds__(SEL_BLK) void k_sel_hist(const i32* sdst, i64 M, i3
     if (d >= 0 && d < k) atomicAdd(&h[d], 1);
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < k; q += SEL_BLK) H[(i64)blockIdx.x * k + q] = h[q];
+  for (int q = threadIdx.x; q < k; q += SEL_BLK) H[(i64)q * gridDim.x + blockIdx.x] = h[q];
 }
 
-__global__ void k_sel_scan(i32* H, i64 nb, i32 k) {
-  for (i64 q = gtid(); q < k; q += gstride()) {
-    i32 run = 0;
-    for (i64 b = 0; b < nb; b++) {
-      const i32 v = H[b * k + q];
-      H[b * k + q] = run;
-      run += v;
+// H is [k][nb]: one block per destination turns its row of per-block counts
+// into exclusive prefix sums (the rank of each block's first move to q).
+constexpr int SELSCAN_BS = 1024;
+__global__ __launch_bounds__(SELSCAN_BS) void k_sel_scan(i32* H, i64 nb) {
+  __shared__ i32 s_w[SELSCAN_BS / WAVE];
+  i32* row = H + (i64)blockIdx.x * nb;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  i32 carry = 0;
+  for (i64 base = 0; base < nb; base += SELSCAN_BS) {
+    const i64 b = base + threadIdx.x;
+    const i32 v = b < nb ? row[b] : 0;
+    i32 x = v;  // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const i32 y = __shfl_up(x, o);
+      if (lane >= o) x += y;
     }
+    if (lane == WAVE - 1) s_w[wv] = x;
+    __syncthreads();
+    i32 before = carry, total = carry;
+    for (int w2 = 0; w2 < SELSCAN_BS / WAVE; w2++) {
+      if (w2 < wv) before += s_w[w2];
+      total += s_w[w2];
+    }
+    if (b < nb) row[b] = before + x - v;
+    carry = total;
+    __syncthreads();
   }
 }
 
@@ -1926,7 +2051,7 @@ __global__ __launch_bounds__(SEL_BLK) void k_sel_mark(const i32* sdst, i64 M, i3
                                                       i32* S, i32* pads) {
   __shared__ i32 cntd[MAXK_LDS];
   __shared__ i32 ssel[SEL_BLK / WAVE];
-  for (int q = threadIdx.x; q < k; q += SEL_BLK) cntd[q] = H[(i64)blockIdx.x * k + q];
+  for (int q = threadIdx.x; q < k; q += SEL_BLK) cntd[q] = H[(i64)q * gridDim.x + blockIdx.x];
   __syncthreads();
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const i64 i = (i64)blockIdx.x * SEL_BLK + threadIdx.x;
@@ -1966,9 +2091,18 @@ __global__ __launch_bounds__(SEL_BLK) void k_sel_mark(const i32* sdst, i64 M, i3
   }
 }
 
-// gpu_part.cu:383-397.
-__global__ void k_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v,
-                         const i32* mv_src, const i32* mv_dst, i32* part, i32* pc, i32 k) {
+// gpu_part.cu:383-397.  Every vertex has at most one move, so the commits are
+// independent; the count changes are summed per block in LDS (k <= MAXK_LDS)
+// and added to pc once per block and part.
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_commit(const i32* sk_idx, const u8* sel, i64 M,
+                                                const i32* mv_v, const i32* mv_src,
+                                                const i32* mv_dst, i32* part, i32* pc, i32 k) {
+  __shared__ i32 dpc[LDS ? MAXK_LDS : 1];
+  if (LDS) {
+    for (int q = threadIdx.x; q < k; q += blockDim.x) dpc[q] = 0;
+    __syncthreads();
+  }
   for (i64 i = gtid(); i < M; i += gstride()) {
     if (!sel[i]) continue;
     const i32 id = sk_idx[i];
@@ -1977,14 +2111,17 @@ __global__ void k_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_
     if (src < k && dst < k && src >= 0 && dst >= 0) {
       const i32 v = mv_v[id];
       const i32 old = part[v];
-      part[v] = dst;
       if (old == src) {
-        atomicAdd(&pc[dst], 1);
-        atomicAdd(&pc[src], -1);
-      } else {
-        part[v] = old;
+        part[v] = dst;
+        atomicAdd(LDS ? &dpc[dst] : &pc[dst], 1);
+        atomicAdd(LDS ? &dpc[src] : &pc[src], -1);
       }
     }
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < k; q += blockDim.x)
+      if (dpc[q]) atomicAdd(&pc[q], dpc[q]);
   }
 }
 
@@ -2204,10 +2341,17 @@ void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32*
                                                               nmedium, huge, nhuge);
 }
 
+void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* hb,
+                        hipStream_t s) {
+  if (nhuge > 0)
+    k_huge_bounds<<<grid_for(nhuge, 256, 1024), 256, 0, s>>>(g, nedges, huge, nhuge, hb);
+}
+
 bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
-                 const u8* src_big, float* gain, i32* target, u8* cand, int phase, int unit,
-                 i32* gscratch, const i32* medium, i64 nmedium, const i32* huge, i64 nhuge,
-                 u64* cut_acc, hipStream_t s) {
+                 const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
+                 i32* gscratch, const GainLists& L, u64* cut_acc, hipStream_t s) {
+  const i32* medium = L.medium;
+  const i64 nmedium = L.nmedium;
   if (k <= 512) {
     // light rows: 8 lanes per vertex, 32 tallies of k ints per block
     const size_t shm8 = (size_t)(2 * k + 32 * k) * 4;
@@ -2228,15 +2372,25 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
         k_gain_group<64, false><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
                                                           target, cand, phase, medium, nmedium, 0, cut_acc);
     }
-    if (nhuge > 0) {
-      const size_t shmh = (size_t)(3 * k) * 4;
-      const unsigned grh = grid_for(nhuge, 1, 8192);
+    if (L.nhuge > 0) {
+      const size_t shmc = (size_t)k * 4;
+      const unsigned grc = grid_for(L.nch, 1, 1u << 16);
       if (unit)
-        k_gain_huge<true><<<grh, HUGE_BS, shmh, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                                     cand, phase, huge, nhuge, cut_acc);
+        k_gain_chunk<true><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
+                                                      L.ch_end, L.nch, L.gtally, L.gabs, cut_acc);
       else
-        k_gain_huge<false><<<grh, HUGE_BS, shmh, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                      target, cand, phase, huge, nhuge, cut_acc);
+        k_gain_chunk<false><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
+                                                       L.ch_end, L.nch, L.gtally, L.gabs, cut_acc);
+      const size_t shmf = (size_t)(3 * k) * 4;
+      const unsigned grf = grid_for(L.nhuge, 1, 8192);
+      if (unit)
+        k_gain_huge_final<true><<<grf, 256, shmf, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                       target, cand, phase, L.huge, L.nhuge,
+                                                       L.gtally, L.gabs);
+      else
+        k_gain_huge_final<false><<<grf, 256, shmf, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                        target, cand, phase, L.huge, L.nhuge,
+                                                        L.gtally, L.gabs);
     }
     return cut_acc != nullptr;
   }
@@ -2266,14 +2420,14 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
   return false;  // the large-k kernels do not tally the cut
 }
 
-void launch_token(DGraph g, i64 nedges, const float* gain, const u8* cand, const i32* part,
-                  const i32* target, i32 k, i64* flag, const i32* heavy_list, i64 heavy_count,
-                  i64 heavy_deg, hipStream_t s) {
-  k_token_thread<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, gain, cand, part,
-                                                              target, k, flag, heavy_deg);
+void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
+                  i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
+                  hipStream_t s) {
+  k_token_thread<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, cand, part, target, k,
+                                                              flag, heavy_deg);
   if (heavy_count > 0)
-    k_token_heavy<<<grid_for(heavy_count, 4, 8192), 256, 0, s>>>(g, nedges, gain, cand, part,
-                                                                 target, k, flag, heavy_list,
+    k_token_heavy<<<grid_for(heavy_count, 4, 8192), 256, 0, s>>>(g, nedges, cand, part, target,
+                                                                 k, flag, heavy_list,
                                                                  heavy_count);
 }
 
@@ -2294,14 +2448,25 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
                        hipStream_t s) {
   k_sort_init<<<grid_for(len, 256, 1u << 16), 256, 0, s>>>(mv_gain, M, len, key, idx);
   if (len < 2) return;
-  const unsigned tiles = (unsigned)((len + BT_ELEMS - 1) / BT_ELEMS);
+  const unsigned tiles = (unsigned)((len + BT_TILE - 1) / BT_TILE);
   k_bt_local<<<tiles, BT_THREADS, 0, s>>>(key, idx, len, 0);
   i64 p2 = 1;
   while (p2 < len) p2 <<= 1;
-  const i64 npairs = p2 >> 1;
-  for (i64 kk = 2 * BT_ELEMS; kk <= len; kk <<= 1) {
-    for (i64 j = kk >> 1; j >= BT_ELEMS; j >>= 1)
-      k_bt_global<<<grid_for(npairs, 256, 1u << 16), 256, 0, s>>>(key, idx, len, npairs, kk, j);
+  for (i64 kk = 2 * BT_TILE; kk <= len; kk <<= 1) {
+    int bhi = 0;  // log2 of the largest stride of this merge
+    while (((i64)1 << (bhi + 1)) < kk) bhi++;
+    int blo_min = 0;
+    while (((i64)1 << blo_min) < BT_TILE) blo_min++;
+    for (int b = bhi; b >= blo_min;) {
+      const int S = std::min(3, b - blo_min + 1);
+      const int blo = b - S + 1;
+      const i64 nsets = p2 >> S;
+      const unsigned gr = grid_for(nsets, 256, 1u << 16);
+      if (S == 3) k_bt_global<3><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      else if (S == 2) k_bt_global<2><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      else k_bt_global<1><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      b -= S;
+    }
     k_bt_local<<<tiles, BT_THREADS, 0, s>>>(key, idx, len, kk);
   }
 }
@@ -2313,14 +2478,18 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
   const i64 nb = (M + SEL_BLK - 1) / SEL_BLK;
   k_sel_dst<<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, mv_dst, M, sdst);
   k_sel_hist<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H);
-  k_sel_scan<<<grid_for(k, 64, 1024), 64, 0, s>>>(H, nb, k);
+  k_sel_scan<<<(unsigned)k, SELSCAN_BS, 0, s>>>(H, nb);
   k_sel_mark<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H, avail, sel, S, pads);
 }
 
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
                    const i32* mv_dst, i32* part, i32* pc, i32 k, hipStream_t s) {
-  k_commit<<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src, mv_dst,
-                                                      part, pc, k);
+  if (k <= MAXK_LDS)
+    k_commit<true><<<grid_for(M, 256 * 16, 2048), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
+                                                               mv_dst, part, pc, k);
+  else
+    k_commit<false><<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
+                                                               mv_dst, part, pc, k);
 }
 
 void launch_arc_source(DGraph g, i32* asrc, hipStream_t s) {
