@@ -196,15 +196,18 @@ struct LevelAux {
   int unit = 0;       // all weights are 1
   DBuf<i32> asrc;     // arc -> source vertex
   DBuf<i32> heavy;    // heavy rows
-  DBuf<i32> medium, huge;  // heavy rows split at 4096 arcs
-  i64 nmedium = 0, nhuge = 0;
+  DBuf<i32> mid, medium, huge;  // heavy rows split at 512 and 4096 arcs
+  i64 nmid = 0, nmedium = 0, nhuge = 0;
   DBuf<i32> ch_row;          // chunks of the long rows: row index in `huge`, arc range
   DBuf<i64> ch_beg, ch_end;
   i64 nch = 0;
   DBuf<i32> gtally;          // nhuge * k tallies and nhuge |w| sums, zero between passes
   DBuf<i64> gabs;
+  int all_exact = 0;         // every row's sum of |w| is at most 2^24
   GainLists lists() const {
     GainLists L;
+    L.mid = mid.p;
+    L.nmid = nmid;
     L.medium = medium.p;
     L.nmedium = nmedium;
     L.huge = huge.p;
@@ -215,6 +218,7 @@ struct LevelAux {
     L.nch = nch;
     L.gtally = gtally.p;
     L.gabs = gabs.p;
+    L.all_exact = all_exact;
     return L;
   }
 };
@@ -408,13 +412,18 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     // kernels below finish the level from the same (consistent) state.
     DBuf<u32> ticket(&C.pool, 1);
     DBuf<i32> stv(&C.pool, n);
+    DBuf<u64> cstats;
+    if (trace_match) {
+      cstats = DBuf<u64>(&C.pool, 10);
+      HIP_TRY(hipMemsetAsync(cstats.p, 0, 10 * sizeof(u64), st()));
+    }
     launch_fill_i32(stv.p, n, -1, st());
     i64 prev = -1;
     int stuck = 0;
     while (stuck < 3) {
       HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
       launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, chain_budget,
-                         CHAIN_BLOCKS, st());
+                         CHAIN_BLOCKS, cstats.p, st());
       round++;
       HIP_TRY(hipMemcpyAsync(C.pinned, lo.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
       C.sync();
@@ -426,11 +435,21 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     if (C.pinned[0] >= n) {
       C.prof.end(KID_MATCH, (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n);
       match_rounds += round;
-      if (trace_match)
-        std::fprintf(stderr, "# match n=%lld m=%lld chained launches=%u ms=%.2f\n", (long long)n,
-                     (long long)lv.m, round,
+      if (trace_match) {
+        u64 cs[10];
+        HIP_TRY(hipMemcpy(cs, cstats.p, sizeof(cs), hipMemcpyDeviceToHost));
+        std::fprintf(stderr,
+                     "# match n=%lld m=%lld chained launches=%u ms=%.2f iter=%llu idle=%llu "
+                     "apS=%llu aL=%llu pL=%llu fetch=%llu cS=%llu cL=%llu dec=%llu arcsL=%llu\n",
+                     (long long)n, (long long)lv.m, round,
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
-                                                               t_start).count());
+                                                               t_start).count(),
+                     (unsigned long long)cs[0], (unsigned long long)cs[1],
+                     (unsigned long long)cs[2], (unsigned long long)cs[3],
+                     (unsigned long long)cs[4], (unsigned long long)cs[5],
+                     (unsigned long long)cs[6], (unsigned long long)cs[7],
+                     (unsigned long long)cs[8], (unsigned long long)cs[9]);
+      }
       return;
     }
     // scan starts of the round kernels: both parity slots at the first undecided vertex
@@ -583,15 +602,27 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   C.sync();
   a.nheavy = ((const i32*)C.pinned)[0];
   a.unit = ((const i32*)C.pinned)[1] == 0;
+  a.mid.ensure(&C.pool, std::max<i64>(a.nheavy, 1));
   a.medium.ensure(&C.pool, std::max<i64>(a.nheavy, 1));
   a.huge.ensure(&C.pool, std::max<i64>(a.nheavy, 1));
-  HIP_TRY(hipMemsetAsync(cnt32.p, 0, 2 * sizeof(i32), st()));
-  launch_split_heavy(lv.dg(), nedges, a.heavy.p, a.nheavy, a.medium.p, cnt32.p, a.huge.p,
-                     cnt32.p + 1, st());
-  HIP_TRY(hipMemcpyAsync(C.pinned + 2, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  DBuf<i32> cnt3(&C.pool, 3);
+  HIP_TRY(hipMemsetAsync(cnt3.p, 0, 3 * sizeof(i32), st()));
+  launch_split_heavy(lv.dg(), nedges, a.heavy.p, a.nheavy, a.mid.p, cnt3.p, a.medium.p,
+                     cnt3.p + 1, a.huge.p, cnt3.p + 2, st());
+  HIP_TRY(hipMemcpyAsync(C.pinned + 2, cnt3.p, 3 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
-  a.nmedium = ((const i32*)(C.pinned + 2))[0];
-  a.nhuge = ((const i32*)(C.pinned + 2))[1];
+  a.nmid = ((const i32*)(C.pinned + 2))[0];
+  a.nmedium = ((const i32*)(C.pinned + 2))[1];
+  a.nhuge = ((const i32*)(C.pinned + 2))[2];
+  // integer tallies are exact for every row when no row's sum of |w| passes 2^24
+  a.all_exact = 1;
+  if (!a.unit) {
+    HIP_TRY(hipMemsetAsync(cnt3.p, 0, sizeof(i32), st()));
+    launch_row_abs(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, a.nheavy, cnt3.p, st());
+    HIP_TRY(hipMemcpyAsync(C.pinned + 3, cnt3.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    a.all_exact = ((const i32*)(C.pinned + 3))[0] == 0;
+  }
   // chunks of the long rows, built on the host (nhuge <= m / 4096)
   a.nch = 0;
   if (a.nhuge > 0) {
@@ -715,6 +746,13 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     DBuf<i32> target(&C.pool, n), mv_v(&C.pool, n), mv_src(&C.pool, n), mv_dst(&C.pool, n),
         sk_idx(&C.pool, n), sdst(&C.pool, n);
     DBuf<float> cand(&C.pool, n);
+    // byte copy of part for the gathers of the gain kernels (1 byte per vertex
+    // keeps the partition resident in L2); rebuilt after every bulk change of part
+    DBuf<u8> part8;
+    if (k <= PART8_MAXK) {
+      part8 = DBuf<u8>(&C.pool, n);
+      launch_part8(part.p, n, k, part8.p, st());
+    }
     DBuf<u8> sel(&C.pool, n), src_big(&C.pool, k);
     DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
     DBuf<float> bg(&C.pool, 2 * (size_t)k);
@@ -747,7 +785,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
           C.prof.end(KID_MISC, 0);
           C.prof.begin(KID_GAIN);
-          const bool fused = launch_gain(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p,
+          const bool fused = launch_gain(g, nedges, part.p, part8.p, k, bg.p, src_big.p, gain.p,
+                                         target.p,
                                          cand.p, phase, unit, gscr.p, ax.lists(),
                                          pending ? accs.p : nullptr, st());
           C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
@@ -831,13 +870,15 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
                       ctr_cur + 1, st());
         C.prof.end(KID_SELECT, 13.0 * M);
         C.prof.begin(KID_COMMIT);
-        launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, pc.p, k, st());
+        launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, part8.p, pc.p, k,
+                      st());
         C.prof.end(KID_COMMIT, 17.0 * M);
         pending = true;
         prevM = M;
         prevPass = pass;
       }
       HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
+      if (part8.p) launch_part8(part.p, n, k, part8.p, st());
       if (phase == 0) *cut = bestCut;
     }
   }

@@ -84,8 +84,11 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 // decided unmatched, else the partner): one launch decides vertices from
 // lo[0] upward until all are decided or `budget` wall-clock ticks pass, then
 // sets lo[0] to the first undecided vertex (n when done).  *ticket must be 0.
+// stats (optional, 10 zeroed u64): wave iterations, idle iterations, short
+// A/proposal scans, long A scans, long proposal scans, proposal-row fetches,
+// short C scans, long C scans, decisions, arcs of the long scans.
 void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, i64* lo,
-                        u32* ticket, i64 budget, int blocks, hipStream_t s);
+                        u32* ticket, i64 budget, int blocks, u64* stats, hipStream_t s);
 // mate = st where st >= 0 else -1; dec = (st != -1)
 void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s);
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
@@ -126,19 +129,23 @@ void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_
                      hipStream_t s);
 // Per-pass order of parts by (balance term desc, part asc) and run ends.
 // Heavy rows (> 64 arcs) split into medium (<= 4096) and huge lists.
-void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
-                        i32* nmedium, i32* huge, i32* nhuge, hipStream_t s);
+void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* mid,
+                        i32* nmid, i32* medium, i32* nmedium, i32* huge, i32* nhuge,
+                        hipStream_t s);
 // Gains: 8 lanes per light row, a wave per medium row, a block per huge row
 // (k <= 512); one wave per vertex for larger k.
 // When cut_acc (accumulator slots) is given and k <= 512, also adds the edge
 // cut of `part` (the value launch_cut_arcs would add) to it and returns true; otherwise
 // returns false and the caller computes the cut separately.
-// Row classes of a level for the gain kernels: medium rows (64 < len <= 4096),
+// Row classes of a level for the gain kernels: mid rows (64 < len <= 512),
+// medium rows (512 < len <= 4096),
 // long rows (> 4096) and the long rows' chunks of at most GAIN_CHUNK_ARCS arcs
 // with per-row accumulators (nhuge * k tallies, nhuge |w| sums, all zero
 // between passes).
 constexpr i64 GAIN_CHUNK_ARCS = 16384;
 struct GainLists {
+  const i32* mid = nullptr;
+  i64 nmid = 0;
   const i32* medium = nullptr;
   i64 nmedium = 0;
   const i32* huge = nullptr;
@@ -149,10 +156,19 @@ struct GainLists {
   i64 nch = 0;
   i32* gtally = nullptr;
   i64* gabs = nullptr;
+  int all_exact = 0;  // every row's sum of |w| is at most 2^24 (integer tallies are exact)
 };
-bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+// *over |= (some row's sum of |w| > 2^24); heavy = rows longer than heavy_deg
+void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
+                    i32* over, hipStream_t s);
+// part8 (optional, k <= PART8_MAXK): byte copy of part that the neighbour
+// gathers read instead of part.
+constexpr i32 PART8_MAXK = 255;
+bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
                  i32* gscratch, const GainLists& L, u64* cut_acc, hipStream_t s);
+// part8[v] = part[v] if 0 <= part[v] < k, else 255
+void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s);
 // hb[2h], hb[2h+1] = arc range of long row h (the last row ends at nedges)
 void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* hb,
                         hipStream_t s);
@@ -173,9 +189,9 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32
 i64 select_scratch_elems(i64 M, i32 k);
 void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64* avail,
                    i32* sdst, i32* H, u8* sel, i32* S, i32* pads, hipStream_t s);
-void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v,
-                   const i32* mv_src, const i32* mv_dst, i32* part, i32* pc, i32 k,
-                   hipStream_t s);
+// part8 (optional) is kept equal to part's byte copy
+void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
+                   const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, hipStream_t s);
 // Accumulators for block sums: ACC_SLOTS slots, one per 128-byte line, all
 // zero between uses.  Kernels add into them; launch_acc_finish writes the
 // total to *out and clears the slots.

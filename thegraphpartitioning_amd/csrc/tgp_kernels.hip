@@ -712,7 +712,10 @@ __device__ __forceinline__ bool sg_any(bool f) {
 
 __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC,
                                                      i32* prop, const i64* lo_io, u32* ticket,
-                                                     i64 budget) {
+                                                     i64 budget, u64* stats) {
+  // stats (optional): per-kind event counts, summed over the grid
+  u64 c_iter = 0, c_sleep = 0, c_apS = 0, c_aL = 0, c_pL = 0, c_fetch = 0, c_cS = 0, c_cL = 0,
+      c_commit = 0, c_arcsL = 0;
   const i64 n = g.n;
   const i64 lo = lo_io[0];
   const int lane = lane_id();
@@ -764,6 +767,7 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       continue;
     }
     if (wall_clock64() - t_start > budget) break;
+    c_iter++;
 
     if (own && a >= 0 && s_a != ST_UNDEC) {
       if (s_a == t) own = false;  // a took t
@@ -773,6 +777,7 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
     // A and proposal over a row held in registers: one batch of loads
     const bool needAP = own && (a == BLK_UNKNOWN || u == BLK_UNKNOWN);
     if (__ballot(needAP && tshort)) {
+      c_apS++;
       i32 sv[CH_PER];
 #pragma unroll
       for (int j = 0; j < CH_PER; j++)
@@ -828,6 +833,8 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
+      c_aL++;
+      c_arcsL += g.xadj[x + 1] - g.xadj[x];
       const i32 r = wave_scan_pending(g, x, x, x, st, lane);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) own = false;
@@ -839,6 +846,8 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
+      c_pL++;
+      c_arcsL += g.xadj[x + 1] - g.xadj[x];
       const i32 r = wave_best_free_st(g, x, st, lane);
       const i32 sr = r >= 0 ? ld_st(st, r) : ST_UNDEC;
       if ((lane & ~(CH_SG - 1)) == l) {
@@ -863,6 +872,7 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
     const bool needC = okB && b == BLK_UNKNOWN;
     const bool fetch = needC && !uknown;
     if (__ballot(fetch)) {  // the proposal's row, once per proposal
+      c_fetch++;
       i64 us0 = 0, us1 = 0;
       if (fetch) {
         us0 = g.xadj[u];
@@ -877,6 +887,7 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       }
     }
     if (__ballot(needC && ushort_)) {
+      c_cS++;
       i32 sv[CH_PER];
 #pragma unroll
       for (int j = 0; j < CH_PER; j++)
@@ -905,6 +916,8 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(u, l);
       const i32 lim = __shfl(t, l);
+      c_cL++;
+      c_arcsL += g.xadj[x + 1] - g.xadj[x];
       const i32 r = wave_scan_pending(g, x, lim, x, st, lane);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) {
@@ -926,12 +939,21 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       }
     }
     if (ready) own = false;
-    if (!__ballot(ready)) __builtin_amdgcn_s_sleep(2);
+    c_commit += __popcll(__ballot(ready && sl == 0));
+    if (!__ballot(ready)) {
+      c_sleep++;
+      __builtin_amdgcn_s_sleep(2);
+    }
   }
   if (own && sl == 0) {  // out of time: keep the caches for the next launch
     blkA[t] = a;
     prop[t] = u;
     blkC[t] = b;
+  }
+  if (stats && lane == 0) {
+    const u64 c[10] = {c_iter, c_sleep, c_apS, c_aL, c_pL, c_fetch, c_cS, c_cL, c_commit,
+                       c_arcsL};
+    for (int q = 0; q < 10; q++) atomicAdd((unsigned long long*)&stats[q], (unsigned long long)c[q]);
   }
 }
 
@@ -1435,22 +1457,25 @@ constexpr int GAIN_LIGHT = 64;
 // an absent part, so the dense scan is the reference's loop over q.  G = 8
 // for rows of at most 64 arcs (eight vertices per wave in flight), G = 64 for
 // the heavy list.
-template <int G, bool UNIT>
-__global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const i32* part, i32 k,
+template <int G, bool UNIT, typename PT>
+__global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const PT* part, i32 k,
                                                     const float* bg_g, const u8* src_big,
                                                     float* gain, i32* target, float* cand,
                                                     int phase, const i32* list, i64 cnt,
-                                                    i64 light_max, u64* cut_acc) {
+                                                    i64 light_max, int all_exact, u64* cut_acc) {
   extern __shared__ i32 lds[];
   constexpr int GPB = 256 / G;  // groups per block
   u64 cutv = 0;  // weight of this thread's arcs whose ends lie in different parts
   float* sbg = (float*)lds;
   for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
-  __syncthreads();
   const int gl = threadIdx.x & (G - 1);
   const int grp = threadIdx.x / G;
-  i32* tally = lds + 2 * k + grp * k;
+  // one tally of k ints per group at an odd stride, so that the groups of a
+  // wave fall in different LDS banks; the pick below leaves it zeroed
+  i32* tally = lds + 2 * k + grp * (k | 1);
   float* ftally = (float*)tally;
+  for (int q = gl; q < k; q += G) tally[q] = 0;
+  __syncthreads();
   const i64 total = list ? cnt : g.n;
   const i64 ngroups = (i64)gridDim.x * GPB;
   for (i64 e = (i64)blockIdx.x * GPB + grp; e < total; e += ngroups) {
@@ -1460,11 +1485,9 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
     const i32 cur = part[v];
     if (!list && s1 - s0 > light_max) continue;   // heavy rows: the G = 64 launch
     if (cur >= k || cur < 0) continue;            // gpu_part.cu:262
-    for (int q = gl; q < k; q += G) tally[q] = 0;
-    __builtin_amdgcn_wave_barrier();
     i64 absw = 0;
     // GU independent loads per lane in flight, then their gathers
-    constexpr int GU = G == 8 ? 8 : 4;
+    constexpr int GU = G == 64 ? 4 : 8;
     for (i64 base = s0; base < s1; base += G * GU) {
       i32 uu[GU], ww[GU], pp[GU];
 #pragma unroll
@@ -1479,17 +1502,18 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
       for (int j = 0; j < GU; j++) {
         if (pp[j] < k && pp[j] >= 0) {
           atomicAdd(&tally[pp[j]], ww[j]);
-          if (!UNIT) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
+          if (!UNIT && !all_exact) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
           if (pp[j] != cur) cutv += (u64)(i64)ww[j];
         }
       }
     }
-    if (!UNIT) {
+    if (!UNIT && !all_exact) {
 #pragma unroll
       for (int o = G / 2; o > 0; o >>= 1) absw += __shfl_xor(absw, o, G);
     }
     __builtin_amdgcn_wave_barrier();
-    const bool exact_int = UNIT ? (s1 - s0) <= (i64)(1 << 24) : absw <= (i64)(1 << 24);
+    const bool exact_int =
+        UNIT ? (s1 - s0) <= (i64)(1 << 24) : (all_exact || absw <= (i64)(1 << 24));
     if (!exact_int) {
       for (int q = gl; q < k; q += G) ftally[q] = 0.0f;
       __builtin_amdgcn_wave_barrier();
@@ -1505,12 +1529,14 @@ __global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const 
       __builtin_amdgcn_wave_barrier();
     }
     const float ccur = exact_int ? (float)tally[cur] : ftally[cur];
+    __builtin_amdgcn_wave_barrier();
     const float* bgr = sbg + (src_big[cur] ? k : 0);
     float bgv = -INFINITY;
     i32 bq = INT_MAX;
     for (int q = gl; q < k; q += G) {
-      if (q == cur) continue;
       const float cq = exact_int ? (float)tally[q] : ftally[q];
+      tally[q] = 0;
+      if (q == cur) continue;
       const float eg = cq - ccur;
       const float gg = eg + bgr[q];
       if (gg > bgv || (gg == bgv && q < bq)) {
@@ -1559,8 +1585,8 @@ constexpr int HUGE_BS = 1024;
 // tallies its chunk in LDS and adds the tally, the chunk's sum of |w| and its
 // share of the cut to per-row accumulators in global memory.  Integer tallies
 // are independent of the order of the additions.
-template <bool UNIT>
-__global__ __launch_bounds__(HUGE_BS) void k_gain_chunk(DGraph g, const i32* part, i32 k,
+template <bool UNIT, typename PT>
+__global__ __launch_bounds__(HUGE_BS) void k_gain_chunk(DGraph g, const PT* part, i32 k,
                                                         const i32* huge, const i32* ch_row,
                                                         const i64* ch_beg, const i64* ch_end,
                                                         i64 nch, i32* gtally, i64* gabs,
@@ -1726,14 +1752,43 @@ __global__ void k_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, 
 
 // Heavy rows split into medium (64 < len <= 4096, one wave) and huge (> 4096,
 // one block): entries of `heavy` are partitioned in place by the census.
-__global__ void k_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
-                              i32* nmedium, i32* huge, i32* nhuge) {
+// *over = 1 when some row's sum of |w| (over [s0, s1), the last row ending at
+// nedges) exceeds 2^24; rows longer than heavy_deg are summed by one wave
+// each from the heavy list.
+__global__ void k_row_abs_light(DGraph g, i64 nedges, i64 heavy_deg, i32* over) {
+  for (i64 v = gtid(); v < g.n; v += gstride()) {
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    if (s1 - s0 > heavy_deg) continue;
+    i64 a = 0;
+    for (i64 i = s0; i < s1; i++) a += g.w[i] < 0 ? -(i64)g.w[i] : (i64)g.w[i];
+    if (a > (i64)(1 << 24)) atomicOr(over, 1);
+  }
+}
+__global__ __launch_bounds__(256) void k_row_abs_heavy(DGraph g, i64 nedges, const i32* list,
+                                                       i64 cnt, i32* over) {
+  const int lane = lane_id();
+  for (i64 e = gwave(); e < cnt; e += nwaves()) {
+    const i64 v = list[e];
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    i64 a = 0;
+    for (i64 i = s0 + lane; i < s1; i += WAVE) a += g.w[i] < 0 ? -(i64)g.w[i] : (i64)g.w[i];
+    a = wave_sum(a);
+    if (lane == 0 && a > (i64)(1 << 24)) atomicOr(over, 1);
+  }
+}
+
+constexpr int GAIN_MID = 512;
+__global__ void k_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* mid,
+                              i32* nmid, i32* medium, i32* nmedium, i32* huge, i32* nhuge) {
   for (i64 e = gtid(); e < nheavy; e += gstride()) {
     const i32 v = heavy[e];
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
     if (s1 - s0 > GAIN_HUGE) huge[atomicAdd(nhuge, 1)] = v;
-    else medium[atomicAdd(nmedium, 1)] = v;
+    else if (s1 - s0 > GAIN_MID) medium[atomicAdd(nmedium, 1)] = v;
+    else mid[atomicAdd(nmid, 1)] = v;
   }
 }
 
@@ -1777,7 +1832,8 @@ __global__ __launch_bounds__(256) void k_token_thread(DGraph g, i64 nedges, cons
   }
 }
 
-// Rows longer than heavy_deg: one wave per vertex, four loads per lane in flight.
+// Rows longer than heavy_deg: one wave per vertex, 64 arcs per step (the first
+// losing neighbour usually comes early, so wider steps only add traffic).
 __global__ __launch_bounds__(256) void k_token_heavy(DGraph g, i64 nedges, const float* cand,
                                                      const i32* part, const i32* target, i32 k,
                                                      i64* flag, const i32* list, i64 cnt) {
@@ -1793,22 +1849,20 @@ __global__ __launch_bounds__(256) void k_token_heavy(DGraph g, i64 nedges, const
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
     const float gv10 = gv + 10.0f;
     bool lose = false;
-    for (i64 base = s0; base < s1 && !lose; base += 4 * WAVE) {
-      i32 uu[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const i64 i = base + j * WAVE + lane;
-        uu[j] = i < s1 ? g.adj[i] : INT_MAX;
-      }
+    for (i64 base = s0; base < s1; base += WAVE) {
+      const i64 i = base + lane;
       bool l = false;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (uu[j] < g.n) {
-          const float gu = cand[uu[j]];
-          l |= gu > gv10 || (gu == gv && uu[j] < v);
+      if (i < s1) {
+        const i32 u = g.adj[i];
+        if (u < g.n) {
+          const float gu = cand[u];
+          l = gu > gv10 || (gu == gv && u < v);
         }
       }
-      lose = __ballot(l) != 0;
+      if (__ballot(l)) {
+        lose = true;
+        break;
+      }
     }
     if (lane == 0) flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
   }
@@ -1903,37 +1957,127 @@ __device__ __forceinline__ void bt_stage_lds(float* sk, i32* si, i64 base, i64 l
   }
 }
 
+// In-tile stages keep 8 consecutive elements per thread in registers:
+// strides 1, 2 and 4 are exchanges between a thread's own registers, strides 8
+// to 256 are exchanges with the lane j/8 away (both lanes compute the same
+// decision and each keeps its side), and strides of 512 and more go through
+// LDS.  Every step is the same compare-exchange as bt_stage_lds.
+constexpr int BT_PER = BT_TILE / BT_THREADS;  // 8
+
+template <int J>
+__device__ __forceinline__ void bt_reg_stage(float (&e)[BT_PER], i32 (&x)[BT_PER], i64 p0, i64 len,
+                                             i64 kk) {
+#pragma unroll
+  for (int r = 0; r < BT_PER; r++) {
+    if (r & J) continue;
+    const int rh = r | J;
+    if (p0 + rh < len) {
+      const bool up = ((p0 + rh) & kk) == 0;
+      if ((e[rh] < e[r]) ^ up) {
+        const float tf = e[r];
+        e[r] = e[rh];
+        e[rh] = tf;
+        const i32 ti = x[r];
+        x[r] = x[rh];
+        x[rh] = ti;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void bt_shfl_stage(float (&e)[BT_PER], i32 (&x)[BT_PER], i64 p0,
+                                              int lane, int d, i64 len, i64 kk) {
+  const i64 j = (i64)d * BT_PER;
+  const bool lo = (lane & d) == 0;
+#pragma unroll
+  for (int r = 0; r < BT_PER; r++) {
+    const float o = __shfl_xor(e[r], d);
+    const i32 oi = __shfl_xor(x[r], d);
+    const i64 ph = lo ? p0 + r + j : p0 + r;
+    if (ph < len) {
+      const bool up = (ph & kk) == 0;
+      const float a = lo ? o : e[r];  // key at the higher position
+      const float b = lo ? e[r] : o;  // key at the lower position
+      if ((a < b) ^ up) {
+        e[r] = o;
+        x[r] = oi;
+      }
+    }
+  }
+}
+
+// strides jtop, jtop/2, ..., 1 (jtop <= 256) on the registers
+__device__ __forceinline__ void bt_regs_down(float (&e)[BT_PER], i32 (&x)[BT_PER], i64 p0,
+                                             int lane, i64 len, i64 kk, int jtop) {
+  for (int jj = jtop; jj >= BT_PER; jj >>= 1) bt_shfl_stage(e, x, p0, lane, jj / BT_PER, len, kk);
+  if (jtop >= 4) bt_reg_stage<4>(e, x, p0, len, kk);
+  if (jtop >= 2) bt_reg_stage<2>(e, x, p0, len, kk);
+  bt_reg_stage<1>(e, x, p0, len, kk);
+}
+
 __global__ __launch_bounds__(BT_THREADS) void k_bt_local(float* key, i32* idx, i64 len,
                                                          i64 kk_fixed) {
-  __shared__ float sk[BT_TILE];
-  __shared__ i32 si[BT_TILE];
+  __shared__ __align__(16) float sk[BT_TILE];
+  __shared__ __align__(16) i32 si[BT_TILE];
   const i64 base = (i64)blockIdx.x * BT_TILE;
-  for (int e = threadIdx.x; e < BT_TILE; e += BT_THREADS) {
-    const i64 gi = base + e;
+  for (int q = threadIdx.x; q < BT_TILE; q += BT_THREADS) {
+    const i64 gi = base + q;
     if (gi < len) {
-      sk[e] = key[gi];
-      si[e] = idx[gi];
+      sk[q] = key[gi];
+      si[q] = idx[gi];
     }
   }
   __syncthreads();
-  if (kk_fixed == 0) {
-    for (i64 kk = 2; kk <= BT_TILE && kk <= len; kk <<= 1) {
-      for (int j = (int)(kk >> 1); j > 0; j >>= 1) {
-        bt_stage_lds(sk, si, base, len, kk, j);
+  const int lane = lane_id();
+  const int o = threadIdx.x * BT_PER;
+  const i64 p0 = base + o;
+  float e[BT_PER];
+  i32 x[BT_PER];
+#pragma unroll
+  for (int r = 0; r < BT_PER; r++) {
+    e[r] = sk[o + r];
+    x[r] = si[o + r];
+  }
+  // strides jtop .. 512 of merge kk through LDS, then the rest in registers
+  auto merge = [&](i64 kk, int jtop) {
+    if (jtop >= 2 * 256) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < BT_PER; r++) {
+        sk[o + r] = e[r];
+        si[o + r] = x[r];
+      }
+      __syncthreads();
+      for (int jj = jtop; jj >= 2 * 256; jj >>= 1) {
+        bt_stage_lds(sk, si, base, len, kk, jj);
         __syncthreads();
       }
+#pragma unroll
+      for (int r = 0; r < BT_PER; r++) {
+        e[r] = sk[o + r];
+        x[r] = si[o + r];
+      }
+      jtop = 256;
     }
+    bt_regs_down(e, x, p0, lane, len, kk, jtop);
+  };
+  if (kk_fixed == 0) {
+    for (i64 kk = 2; kk <= BT_TILE && kk <= len; kk <<= 1) merge(kk, (int)(kk >> 1));
   } else {
-    for (int j = BT_TILE >> 1; j > 0; j >>= 1) {
-      bt_stage_lds(sk, si, base, len, kk_fixed, j);
-      __syncthreads();
-    }
+    merge(kk_fixed, BT_TILE >> 1);
   }
-  for (int e = threadIdx.x; e < BT_TILE; e += BT_THREADS) {
-    const i64 gi = base + e;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < BT_PER; r++) {
+    sk[o + r] = e[r];
+    si[o + r] = x[r];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < BT_TILE; q += BT_THREADS) {
+    const i64 gi = base + q;
     if (gi < len) {
-      key[gi] = sk[e];
-      idx[gi] = si[e];
+      key[gi] = sk[q];
+      idx[gi] = si[q];
     }
   }
 }
@@ -2091,13 +2235,22 @@ __global__ __launch_bounds__(SEL_BLK) void k_sel_mark(const i32* sdst, i64 M, i3
   }
 }
 
+// part8[v] = part[v] when it names a part, 255 otherwise (k <= PART8_MAXK)
+__global__ void k_part8(const i32* part, i64 n, i32 k, u8* part8) {
+  for (i64 i = gtid(); i < n; i += gstride()) {
+    const i32 p = part[i];
+    part8[i] = (p >= 0 && p < k) ? (u8)p : (u8)255;
+  }
+}
+
 // gpu_part.cu:383-397.  Every vertex has at most one move, so the commits are
 // independent; the count changes are summed per block in LDS (k <= MAXK_LDS)
 // and added to pc once per block and part.
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_commit(const i32* sk_idx, const u8* sel, i64 M,
                                                 const i32* mv_v, const i32* mv_src,
-                                                const i32* mv_dst, i32* part, i32* pc, i32 k) {
+                                                const i32* mv_dst, i32* part, u8* part8,
+                                                i32* pc, i32 k) {
   __shared__ i32 dpc[LDS ? MAXK_LDS : 1];
   if (LDS) {
     for (int q = threadIdx.x; q < k; q += blockDim.x) dpc[q] = 0;
@@ -2113,6 +2266,7 @@ __global__ __launch_bounds__(256) void k_commit(const i32* sk_idx, const u8* sel
       const i32 old = part[v];
       if (old == src) {
         part[v] = dst;
+        if (part8) part8[v] = (u8)dst;
         atomicAdd(LDS ? &dpc[dst] : &pc[dst], 1);
         atomicAdd(LDS ? &dpc[src] : &pc[src], -1);
       }
@@ -2246,8 +2400,8 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 }
 
 void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, i64* lo,
-                        u32* ticket, i64 budget, int blocks, hipStream_t s) {
-  k_match_chain<<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget);
+                        u32* ticket, i64 budget, int blocks, u64* stats, hipStream_t s) {
+  k_match_chain<<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
   k_first_undecided_st<<<1, 1024, 0, s>>>(st, g.n, lo);
 }
 
@@ -2334,11 +2488,12 @@ void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_
 
 i64 gain_gscratch_elems(i32 k) { return k <= MAXK_LDS ? 0 : (i64)65536 * k; }
 
-void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* medium,
-                        i32* nmedium, i32* huge, i32* nhuge, hipStream_t s) {
+void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* mid,
+                        i32* nmid, i32* medium, i32* nmedium, i32* huge, i32* nhuge,
+                        hipStream_t s) {
   if (nheavy > 0)
-    k_split_heavy<<<grid_for(nheavy, 256, 4096), 256, 0, s>>>(g, nedges, heavy, nheavy, medium,
-                                                              nmedium, huge, nhuge);
+    k_split_heavy<<<grid_for(nheavy, 256, 4096), 256, 0, s>>>(g, nedges, heavy, nheavy, mid,
+                                                              nmid, medium, nmedium, huge, nhuge);
 }
 
 void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* hb,
@@ -2347,51 +2502,87 @@ void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* h
     k_huge_bounds<<<grid_for(nhuge, 256, 1024), 256, 0, s>>>(g, nedges, huge, nhuge, hb);
 }
 
-bool launch_gain(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+template <typename PT>
+static void launch_gain_small_k(DGraph g, i64 nedges, const PT* part, const i32* part32, i32 k,
+                                const float* bg, const u8* src_big, float* gain, i32* target,
+                                float* cand, int phase, int unit, const GainLists& L,
+                                u64* cut_acc, hipStream_t s) {
+  // light rows: 8 lanes per vertex, 32 tallies of k ints per block
+  const size_t shm8 = (size_t)(2 * k + 32 * (k | 1)) * 4;
+  const unsigned gr8 = grid_for(g.n, 32, 1u << 16);
+  if (unit)
+    k_gain_group<8, true, PT><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                     target, cand, phase, nullptr, 0, GAIN_LIGHT, L.all_exact,
+                                                     cut_acc);
+  else
+    k_gain_group<8, false, PT><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                      target, cand, phase, nullptr, 0,
+                                                      GAIN_LIGHT, L.all_exact, cut_acc);
+  if (L.nmid > 0) {
+    const size_t shm16 = (size_t)(2 * k + 16 * (k | 1)) * 4;
+    const unsigned gr16 = grid_for(L.nmid, 16, 1u << 16);
+    if (unit)
+      k_gain_group<16, true, PT><<<gr16, 256, shm16, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                          target, cand, phase, L.mid, L.nmid, 0, L.all_exact,
+                                                          cut_acc);
+    else
+      k_gain_group<16, false, PT><<<gr16, 256, shm16, s>>>(g, nedges, part, k, bg, src_big,
+                                                           gain, target, cand, phase, L.mid,
+                                                           L.nmid, 0, L.all_exact, cut_acc);
+  }
+  if (L.nmedium > 0) {
+    const size_t shm64 = (size_t)(2 * k + 4 * (k | 1)) * 4;
+    const unsigned gr64 = grid_for(L.nmedium, 4, 1u << 16);
+    if (unit)
+      k_gain_group<64, true, PT><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
+                                                          target, cand, phase, L.medium,
+                                                          L.nmedium, 0, L.all_exact, cut_acc);
+    else
+      k_gain_group<64, false, PT><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big,
+                                                           gain, target, cand, phase, L.medium,
+                                                           L.nmedium, 0, L.all_exact, cut_acc);
+  }
+  if (L.nhuge > 0) {
+    const size_t shmc = (size_t)k * 4;
+    const unsigned grc = grid_for(L.nch, 1, 1u << 16);
+    if (unit)
+      k_gain_chunk<true, PT><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
+                                                        L.ch_end, L.nch, L.gtally, L.gabs,
+                                                        cut_acc);
+    else
+      k_gain_chunk<false, PT><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
+                                                         L.ch_end, L.nch, L.gtally, L.gabs,
+                                                         cut_acc);
+    const size_t shmf = (size_t)(3 * k) * 4;
+    const unsigned grf = grid_for(L.nhuge, 1, 8192);
+    if (unit)
+      k_gain_huge_final<true><<<grf, 256, shmf, s>>>(g, nedges, part32, k, bg, src_big, gain,
+                                                     target, cand, phase, L.huge, L.nhuge,
+                                                     L.gtally, L.gabs);
+    else
+      k_gain_huge_final<false><<<grf, 256, shmf, s>>>(g, nedges, part32, k, bg, src_big, gain,
+                                                      target, cand, phase, L.huge, L.nhuge,
+                                                      L.gtally, L.gabs);
+  }
+}
+
+void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
+                    i32* over, hipStream_t s) {
+  k_row_abs_light<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, heavy_deg, over);
+  if (nheavy > 0)
+    k_row_abs_heavy<<<grid_for(nheavy, 4, 8192), 256, 0, s>>>(g, nedges, heavy, nheavy, over);
+}
+
+bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
                  i32* gscratch, const GainLists& L, u64* cut_acc, hipStream_t s) {
-  const i32* medium = L.medium;
-  const i64 nmedium = L.nmedium;
   if (k <= 512) {
-    // light rows: 8 lanes per vertex, 32 tallies of k ints per block
-    const size_t shm8 = (size_t)(2 * k + 32 * k) * 4;
-    const unsigned gr8 = grid_for(g.n, 32, 1u << 16);
-    if (unit)
-      k_gain_group<8, true><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain, target,
-                                                    cand, phase, nullptr, 0, GAIN_LIGHT, cut_acc);
+    if (part8 && k <= PART8_MAXK)
+      launch_gain_small_k<u8>(g, nedges, part8, part, k, bg, src_big, gain, target, cand, phase,
+                              unit, L, cut_acc, s);
     else
-      k_gain_group<8, false><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                     target, cand, phase, nullptr, 0, GAIN_LIGHT, cut_acc);
-    if (nmedium > 0) {
-      const size_t shm64 = (size_t)(2 * k + 4 * k) * 4;
-      const unsigned gr64 = grid_for(nmedium, 4, 1u << 16);
-      if (unit)
-        k_gain_group<64, true><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                         target, cand, phase, medium, nmedium, 0, cut_acc);
-      else
-        k_gain_group<64, false><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                          target, cand, phase, medium, nmedium, 0, cut_acc);
-    }
-    if (L.nhuge > 0) {
-      const size_t shmc = (size_t)k * 4;
-      const unsigned grc = grid_for(L.nch, 1, 1u << 16);
-      if (unit)
-        k_gain_chunk<true><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
-                                                      L.ch_end, L.nch, L.gtally, L.gabs, cut_acc);
-      else
-        k_gain_chunk<false><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
-                                                       L.ch_end, L.nch, L.gtally, L.gabs, cut_acc);
-      const size_t shmf = (size_t)(3 * k) * 4;
-      const unsigned grf = grid_for(L.nhuge, 1, 8192);
-      if (unit)
-        k_gain_huge_final<true><<<grf, 256, shmf, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                       target, cand, phase, L.huge, L.nhuge,
-                                                       L.gtally, L.gabs);
-      else
-        k_gain_huge_final<false><<<grf, 256, shmf, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                        target, cand, phase, L.huge, L.nhuge,
-                                                        L.gtally, L.gabs);
-    }
+      launch_gain_small_k<i32>(g, nedges, part, part, k, bg, src_big, gain, target, cand, phase,
+                               unit, L, cut_acc, s);
     return cut_acc != nullptr;
   }
   // large k: one wave per vertex for every vertex
@@ -2483,13 +2674,17 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
 }
 
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
-                   const i32* mv_dst, i32* part, i32* pc, i32 k, hipStream_t s) {
+                   const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, hipStream_t s) {
   if (k <= MAXK_LDS)
     k_commit<true><<<grid_for(M, 256 * 16, 2048), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
-                                                               mv_dst, part, pc, k);
+                                                               mv_dst, part, part8, pc, k);
   else
     k_commit<false><<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
-                                                               mv_dst, part, pc, k);
+                                                               mv_dst, part, part8, pc, k);
+}
+
+void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s) {
+  if (n > 0) k_part8<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(part, n, k, part8);
 }
 
 void launch_arc_source(DGraph g, i32* asrc, hipStream_t s) {
