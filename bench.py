@@ -11,8 +11,12 @@ reference's.  The initial partition of the coarsest graph is the
 deterministic block rule (no METIS on the machine).
 
 Prints one JSON line: M edges/s (level-0 arcs x steps / seconds), the roofline
-of the dominant kernel family from HIP-event timing, and the oracle's CPU
-throughput on a bounded sample of the same workload family.
+of the dominant kernel (the chained matching kernel, k_match_chain) from
+HIP-event timing of that kernel alone, and the oracle's CPU throughput on a
+bounded sample of the same workload family.  `roofline.traffic` (HBM bytes per
+launch of that kernel) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+committed under profiles/, scaled by the same counters' reading for a plain
+copy of known size (`--copy-probe`).
 """
 import argparse
 import json
@@ -25,6 +29,34 @@ sys.path.insert(0, ROOT)
 
 SEED = 0x7C0DE5EED + 1
 HBM_PEAK_GBS = 8000.0
+ROOFLINE_FAMILY = "match_chain"          # HIP-event bracket around k_match_chain only
+ROOFLINE_KERNEL = "k_match_chain"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_match_chain_rmat22_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, for calibrating FETCH_SIZE and
+    WRITE_SIZE (profile this run with --pmc and compare with mib * 1024 KiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "copies": 3, "bytes_read_per_copy": n * 4,
+                      "bytes_written_per_copy": n * 4}), flush=True)
 
 
 def cpu_baseline(scale, ef, k):
@@ -53,7 +85,12 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=17)
     ap.add_argument("--profile-only", action="store_true",
                     help="run one profiled step and print per-kernel times")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(every GPU box here has one GPU)", file=sys.stderr)
@@ -92,29 +129,42 @@ def main():
     value = g.m * args.steps / elapsed / 1e6
     print(f"# timed {args.steps} steps: {elapsed:.3f}s cut={cut}", file=sys.stderr, flush=True)
 
+    # one untimed step from host buffers: the rate including the CSR upload over PCIe
+    kw_host = dict(init=tgp.INIT_BLOCK, repair=True)
+    t2 = time.perf_counter()
+    tgp.partition(g, args.k, **kw_host)
+    pcie_value = g.m / (time.perf_counter() - t2) / 1e6
+
     # one extra step with per-kernel HIP-event timing (not part of the timed region)
     rp = tgp.partition(g, args.k, profile=True, **kw)
     stp = rp[3]
-    fam = max(stp.kernel_ms, key=lambda n: stp.kernel_ms[n])
-    ms, byts, launches = stp.kernel_ms[fam], stp.kernel_bytes[fam], stp.kernel_launches[fam]
+    ms = stp.kernel_ms[ROOFLINE_FAMILY]
+    byts = stp.kernel_bytes[ROOFLINE_FAMILY]
+    launches = stp.kernel_launches[ROOFLINE_FAMILY]
     achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     total_kernel_ms = sum(stp.kernel_ms.values())
+    import numpy as np
+    sizes = np.bincount(part[part >= 0], minlength=args.k)
     detail = {
         "levels": st.levels, "passes": st.passes, "coarsest_n": st.coarsest_n,
         "edge_cut": int(cut), "n": g.n, "m": g.m,
-        "imbalance": round(float(max(pc)) / ((g.n + args.k - 1) // args.k) - 1.0, 4),
+        "imbalance": round(float(sizes.max()) / ((g.n + args.k - 1) // args.k) - 1.0, 4),
         "kernel_ms": {k_: round(v, 3) for k_, v in stp.kernel_ms.items() if v > 0},
         "kernel_launches": {k_: v for k_, v in stp.kernel_launches.items() if v > 0},
         "kernel_gbs": {k_: round(stp.kernel_bytes[k_] / (v / 1e3) / 1e9, 1)
                        for k_, v in stp.kernel_ms.items() if v > 0},
         "kernel_ms_total": round(total_kernel_ms, 3),
         "profiled_step_s": round(stp.seconds_total, 4),
-        "match_rounds": stp.match_rounds,
+        "match_launches": stp.match_rounds,
+        "pcie_inclusive_value": round(pcie_value, 3),
     }
-    roofline = {"bound": "hbm", "kernel": fam, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_ms": round(ms / max(launches, 1), 4),
-                "bytes_per_launch": round(byts / max(launches, 1))}
+    traffic = measured_traffic()
+    roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic, "launches": launches,
+                "avg_launch_ms": round(ms / max(launches, 1), 4),
+                "bytes_per_launch": round(byts / max(launches, 1)),
+                "kernel_share_of_step": round(ms / max(total_kernel_ms, 1e-9), 3)}
     dg.free()
 
     cpu = None

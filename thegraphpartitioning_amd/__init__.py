@@ -117,7 +117,7 @@ def lib():
 
 
 KERNEL_NAMES = ["match", "relabel", "contract", "pairs", "project", "gain", "token", "compact",
-                "sort", "select", "commit", "cut", "scan", "misc"]
+                "sort", "select", "commit", "cut", "scan", "misc", "match_chain"]
 
 
 class TGPError(RuntimeError):

@@ -399,6 +399,8 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     nreq = DBuf<i32>(&C.pool, 2);
     HIP_TRY(hipMemsetAsync(nreq.p, 0, 2 * sizeof(i32), st()));
   }
+  // algorithmic bytes of one level's matching: offsets, arcs and weights, state
+  const double match_bytes = (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n;
   C.prof.begin(KID_MATCH);
   if (lv.symmetric)
     launch_match_sym_init(n, mate.p, dec.p, s1.p, s2.p, prop.p, lo.p, st());
@@ -418,22 +420,28 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
       HIP_TRY(hipMemsetAsync(cstats.p, 0, 10 * sizeof(u64), st()));
     }
     launch_fill_i32(stv.p, n, -1, st());
+    HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
+    C.prof.end(KID_MATCH, 0);
     i64 prev = -1;
     int stuck = 0;
     while (stuck < 3) {
-      HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
+      C.prof.begin(KID_CHAIN);
       launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, chain_budget,
                          CHAIN_BLOCKS, cstats.p, st());
+      C.prof.end(KID_CHAIN, round == 0 ? match_bytes : 0.0);
       round++;
+      launch_first_undecided(stv.p, n, lo.p, st());
+      HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
       HIP_TRY(hipMemcpyAsync(C.pinned, lo.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
       C.sync();
       if (C.pinned[0] >= n) break;
       stuck = C.pinned[0] > prev ? 0 : stuck + 1;
       prev = C.pinned[0];
     }
+    C.prof.begin(KID_MATCH);
     launch_chain_export(stv.p, n, mate.p, dec.p, st());
     if (C.pinned[0] >= n) {
-      C.prof.end(KID_MATCH, (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n);
+      C.prof.end(KID_MATCH, 0);
       match_rounds += round;
       if (trace_match) {
         u64 cs[10];
@@ -477,7 +485,7 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     }
     if (batch < 64) batch *= 2;
   }
-  C.prof.end(KID_MATCH, (double)(lv.n + 1) * 8 + 12.0 * lv.m + 4.0 * lv.n);
+  C.prof.end(KID_MATCH, match_bytes);
   match_rounds += round;
   if (trace_match)
     std::fprintf(stderr, "# match n=%lld m=%lld symmetric=%d rounds=%u ms=%.2f\n", (long long)n,
@@ -1066,7 +1074,8 @@ int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32
 const char* tgp_kernel_name(int id) {
   static const char* names[KID_COUNT] = {"match",  "relabel", "contract", "pairs", "project",
                                          "gain",   "token",   "compact",  "sort",  "select",
-                                         "commit", "cut",     "scan",     "misc"};
+                                         "commit", "cut",     "scan",     "misc",
+                                         "match_chain"};
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 

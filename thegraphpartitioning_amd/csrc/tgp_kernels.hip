@@ -2399,10 +2399,13 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
   k_fill<i64><<<1, 3, 0, s>>>(lo, 3, 0);
 }
 
-void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, i64* lo,
+void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
                         u32* ticket, i64 budget, int blocks, u64* stats, hipStream_t s) {
   k_match_chain<<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
-  k_first_undecided_st<<<1, 1024, 0, s>>>(st, g.n, lo);
+}
+
+void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s) {
+  k_first_undecided_st<<<1, 1024, 0, s>>>(st, n, lo);
 }
 
 void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s) {
