@@ -541,31 +541,70 @@ void Run::coarsen() {
       c->w = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
       if (m_c > 0) launch_compact_rows(sadj.p, sw.p, c->xadj.p, n_c, c->adj.p, c->w.p, st());
     } else {
-      DBuf<i64> dsz(&C.pool, n_c + 1), csz(&C.pool, n_c + 1), eoff(&C.pool, n_c + 1),
-          toff(&C.pool, n_c + 1);
-      launch_rep_sizes(lv.xadj_p, tb.p, n_c, dsz.p, csz.p, st());
-      launch_scan_excl(dsz.p, eoff.p, n_c, eoff.p + n_c, scratch(), st());
+      DBuf<i64> dsz(&C.pool, n_c + 1), dlong(&C.pool, n_c + 1), csz(&C.pool, n_c + 1),
+          eoff(&C.pool, n_c + 1), toff(&C.pool, n_c + 1), nchk(&C.pool, n_c + 1),
+          cbase(&C.pool, n_c + 1);
+      DBuf<i32> med(&C.pool, n_c), lng(&C.pool, n_c), ncls(&C.pool, 2);
+      HIP_TRY(hipMemsetAsync(ncls.p, 0, 2 * sizeof(i32), st()));
+      launch_rep_classify(lv.xadj_p, tb.p, n_c, dsz.p, dlong.p, csz.p, med.p, ncls.p, lng.p,
+                          ncls.p + 1, nchk.p, st());
+      launch_scan_excl(dlong.p, eoff.p, n_c, eoff.p + n_c, scratch(), st());
       launch_scan_excl(csz.p, toff.p, n_c, toff.p + n_c, scratch(), st());
       HIP_TRY(hipMemcpyAsync(C.pinned, eoff.p + n_c, sizeof(i64), hipMemcpyDeviceToHost, st()));
       HIP_TRY(hipMemcpyAsync(C.pinned + 1, toff.p + n_c, sizeof(i64), hipMemcpyDeviceToHost,
                              st()));
+      HIP_TRY(hipMemcpyAsync(C.pinned + 2, ncls.p, 2 * sizeof(i32), hipMemcpyDeviceToHost,
+                             st()));
       C.sync();
       const i64 etot = C.pinned[0], ttot = C.pinned[1];
-      DBuf<i32> tkey(&C.pool, std::max<i64>(ttot, 1)), tpos(&C.pool, std::max<i64>(ttot, 1)),
-          tw(&C.pool, std::max<i64>(ttot, 1)), eslot(&C.pool, std::max<i64>(etot, 1));
-      launch_fill_i32(tkey.p, ttot, -1, st());
-      launch_fill_i32(tpos.p, ttot, INT_MAX, st());
-      launch_fill_i32(tw.p, ttot, 0, st());
-      launch_rep_insert(g, tb.p, new_ids.p, n_c, toff.p, csz.p, tkey.p, tpos.p, tw.p, st());
-      launch_rep_mark(g, tb.p, new_ids.p, n_c, toff.p, csz.p, eoff.p, tkey.p, tpos.p, eslot.p,
-                      cnt.p, st());
+      const i64 nmed = ((const i32*)(C.pinned + 2))[0], nlng = ((const i32*)(C.pinned + 2))[1];
+      RepLong RL;
+      DBuf<u64> T;
+      DBuf<i32> W, E, ch_row;
+      DBuf<i64> ch_beg, chunk_cnt, choff;
+      if (nlng > 0) {
+        launch_scan_excl(nchk.p, cbase.p, nlng, cbase.p + nlng, scratch(), st());
+        const i64 nch = read_i64(cbase.p + nlng);
+        T = DBuf<u64>(&C.pool, ttot);
+        W = DBuf<i32>(&C.pool, ttot);
+        E = DBuf<i32>(&C.pool, etot);
+        ch_row = DBuf<i32>(&C.pool, nch);
+        ch_beg = DBuf<i64>(&C.pool, nch);
+        chunk_cnt = DBuf<i64>(&C.pool, nch + 1);
+        choff = DBuf<i64>(&C.pool, nch + 1);
+        launch_fill_u64(T.p, ttot, ~0ull, st());
+        launch_fill_i32(W.p, ttot, 0, st());
+        launch_rep_chunks(lng.p, nlng, dsz.p, cbase.p, ch_row.p, ch_beg.p, st());
+        RL.lng = lng.p;
+        RL.nl = nlng;
+        RL.dsz = dsz.p;
+        RL.cbase = cbase.p;
+        RL.ch_row = ch_row.p;
+        RL.ch_beg = ch_beg.p;
+        RL.nch = nch;
+        RL.toff = toff.p;
+        RL.csz = csz.p;
+        RL.T = T.p;
+        RL.W = W.p;
+        RL.eoff = eoff.p;
+        RL.E = E.p;
+        RL.chunk_cnt = chunk_cnt.p;
+        RL.choff = choff.p;
+        launch_rep_long(g, tb.p, new_ids.p, RL, 0, nullptr, nullptr, nullptr, st());
+        launch_scan_excl(chunk_cnt.p, choff.p, nch, choff.p + nch, scratch(), st());
+        launch_rep_rowcnt(RL, cnt.p, st());
+      }
+      launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, 0, cnt.p, nullptr, nullptr,
+                      nullptr, st());
       launch_scan_excl(cnt.p, c->xadj.p, n_c, c->xadj.p + n_c, scratch(), st());
       m_c = read_i64(c->xadj.p + n_c);
       c->adj = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
       c->w = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
-      if (m_c > 0)
-        launch_rep_emit(eoff.p, dsz.p, eslot.p, toff.p, tkey.p, tw.p, c->xadj.p, n_c,
+      if (m_c > 0) {
+        launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, 1, nullptr, c->xadj.p,
                         c->adj.p, c->w.p, st());
+        launch_rep_long(g, tb.p, new_ids.p, RL, 1, c->xadj.p, c->adj.p, c->w.p, st());
+      }
     }
     C.prof.end(KID_CONTRACT, 24.0 * n_c + 12.0 * lv.m + 8.0 * m_c);
     i64 max_edges = 2 * ecount.back();                         // gpu_part.cu:1076-1077

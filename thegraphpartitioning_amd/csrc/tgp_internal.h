@@ -106,18 +106,48 @@ void launch_contract_ref(DGraph g, const i32* mate, const i32* new_ids, i64 n_c,
                          i32* sadj, i32* sw, i64* cnt, hipStream_t s);
 void launch_compact_rows(const i32* sadj, const i32* sw, const i64* ox, i64 n_c,
                          i32* oa, i32* ow, hipStream_t s);
-// Repaired contraction.
-void launch_rep_sizes(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* csz,
-                      hipStream_t s);
-void launch_rep_insert(DGraph g, const i32* tb, const i32* new_ids, i64 n_c,
-                       const i64* toff, const i64* csz, i32* tkey, i32* tpos, i32* tw,
-                       hipStream_t s);
-void launch_rep_mark(DGraph g, const i32* tb, const i32* new_ids, i64 n_c,
-                     const i64* toff, const i64* csz, const i64* eoff, const i32* tkey,
-                     const i32* tpos, i32* eslot, i64* cnt, hipStream_t s);
-void launch_rep_emit(const i64* eoff, const i64* dsz, const i32* eslot, const i64* toff,
-                     const i32* tkey, const i32* tw, const i64* ox, i64 n_c, i32* oa,
+constexpr i64 REP_CHUNK_ENTRIES = 4096;
+// Repaired contraction.  Coarse row c enumerates fine rows tb[2c] then
+// tb[2c+1] (d entries); classify writes d, and for rows past REP_MED entries
+// (long rows, listed in lng) their size and global table capacity.
+// (nchk[e] = chunks of long row e)
+void launch_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* dlong,
+                         i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk,
+                         hipStream_t s);
+// rows of at most REP_MED entries (registers, or an LDS table for the listed
+// medium rows): pass 0 writes cnt[c], pass 1 writes row c at ox[c]
+void launch_rep_rows(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* dsz,
+                     const i32* med, i64 nmed, int pass, i64* cnt, const i64* ox, i32* oa,
                      i32* ow, hipStream_t s);
+// Long rows: listed in lng (nl rows), cut into chunks of REP_CHUNK_ENTRIES
+// (cbase[e] = first chunk of row e, exclusive scan; ch_row/ch_beg describe
+// each chunk); per-row global tables T (64-bit slots) and W at toff with
+// capacity csz; first-occurrence slots E at eoff; per-chunk counts and their
+// exclusive scan choff.
+struct RepLong {
+  const i32* lng = nullptr;
+  i64 nl = 0;
+  const i64* dsz = nullptr;
+  const i64* cbase = nullptr;
+  const i32* ch_row = nullptr;
+  const i64* ch_beg = nullptr;
+  i64 nch = 0;
+  const i64* toff = nullptr;
+  const i64* csz = nullptr;
+  u64* T = nullptr;
+  i32* W = nullptr;
+  const i64* eoff = nullptr;
+  i32* E = nullptr;
+  i64* chunk_cnt = nullptr;
+  const i64* choff = nullptr;
+};
+void launch_rep_chunks(const i32* lng, i64 nl, const i64* dsz, const i64* cbase, i32* ch_row,
+                       i64* ch_beg, hipStream_t s);
+// pass 0: insert + mark (chunk counts); pass 1: emit at ox
+void launch_rep_long(DGraph g, const i32* tb, const i32* new_ids, const RepLong& L, int pass,
+                     const i64* ox, i32* oa, i32* ow, hipStream_t s);
+// cnt[c] of the long rows from the scanned chunk counts
+void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s);
 // Unique undirected pairs (min,max), u != v, 0 <= u < n, true row ends.
 void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s);
 

@@ -1140,40 +1140,263 @@ __device__ __forceinline__ i64 next_pow2_i64(i64 x) {
   return p;
 }
 
-__global__ void k_rep_sizes(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* csz) {
+// Row classes by enumerated size d (fine row + mate row): d <= REP_SMALL rows
+// are deduplicated in registers, d <= REP_MED rows in an LDS table, longer rows
+// in global tables (dlong = d and csz = table capacity only for those).
+constexpr i64 REP_SMALL = 64;
+constexpr i64 REP_MED = 2048;
+
+__global__ void k_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* dlong,
+                               i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk) {
   for (i64 c = gtid(); c < n_c; c += gstride()) {
     const i32 r = tb[2 * c], mt = tb[2 * c + 1];
     i64 d = xadj[r + 1] - xadj[r];
     if (mt >= 0) d += xadj[mt + 1] - xadj[mt];
     dsz[c] = d;
-    csz[c] = d == 0 ? 0 : next_pow2_i64(2 * d);
+    const bool is_long = d > REP_MED;
+    dlong[c] = is_long ? d : 0;
+    csz[c] = is_long ? next_pow2_i64(2 * d) : 0;
+    if (is_long) {
+      const i32 e = atomicAdd(nlng, 1);
+      lng[e] = (i32)c;
+      nchk[e] = (d + REP_CHUNK_ENTRIES - 1) / REP_CHUNK_ENTRIES;
+    }
+    else if (d > REP_SMALL) med[atomicAdd(nmed, 1)] = (i32)c;
   }
 }
 
-__device__ __forceinline__ void rep_entry(const DGraph& g, const i32* tb, i64 c, i64 p, i64 la,
-                                          i64 a0, i64 b0, i32* col, i32* wt) {
-  const i64 i = p < la ? a0 + p : b0 + (p - la);
-  *col = g.adj[i];
-  *wt = g.w[i];
+// Long rows (d > REP_MED): chunks of REP_CHUNK entries, one block per chunk,
+// sharing the row's global table.  A table slot is one 64-bit word
+// (x << 32 | first position), so an insert is one compare-and-swap, or one
+// 64-bit atomicMin when x is already there; weights go to W by atomicAdd.
+constexpr i64 REP_CHUNK = REP_CHUNK_ENTRIES;
+constexpr u64 REP_EMPTY = ~0ull;
+
+// chunk descriptors of the long rows: row index in the list, first entry
+__global__ void k_rep_chunks(const i32* lng, i64 nl, const i64* dsz, const i64* cbase,
+                             i32* ch_row, i64* ch_beg) {
+  for (i64 e = gtid(); e < nl; e += gstride()) {
+    const i64 d = dsz[lng[e]];
+    i64 o = cbase[e];
+    for (i64 b = 0; b < d; b += REP_CHUNK, o++) {
+      ch_row[o] = (i32)e;
+      ch_beg[o] = b;
+    }
+  }
+}
+
+__device__ __forceinline__ void rep_fetch(const DGraph& g, const i32* tb, const i32* new_ids,
+                                          i64 c, i64 p, i32* x, i32* w) {
+  const i32 r = tb[2 * c], mt = tb[2 * c + 1];
+  const i64 a0 = g.xadj[r], la = g.xadj[r + 1] - a0;
+  const i64 i = p < la ? a0 + p : g.xadj[mt] + (p - la);
+  *x = new_ids[g.adj[i]];
+  *w = g.w[i];
 }
 
 __global__ __launch_bounds__(256) void k_rep_insert(DGraph g, const i32* tb, const i32* new_ids,
-                                                    i64 n_c, const i64* toff, const i64* csz,
-                                                    i32* tkey, i32* tpos, i32* tw) {
+                                                    const i32* lng, const i32* ch_row,
+                                                    const i64* ch_beg, i64 nch, const i64* dsz,
+                                                    const i64* toff, const i64* csz, u64* T,
+                                                    i32* W) {
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i64 c = lng[ch_row[k2]];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 e = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const u64 mask = (u64)(csz[c] - 1);
+    u64* Tc = T + toff[c];
+    i32* Wc = W + toff[c];
+    for (i64 p = b + threadIdx.x; p < e; p += blockDim.x) {
+      i32 x, w;
+      rep_fetch(g, tb, new_ids, c, p, &x, &w);
+      if (x == (i32)c || x == -1) continue;
+      const u64 word = ((u64)(u32)x << 32) | (u64)(u32)p;
+      u64 h = (u64)hash32((u32)x) & mask;
+      while (true) {
+        const u64 old = atomicCAS((unsigned long long*)&Tc[h], (unsigned long long)REP_EMPTY,
+                                  (unsigned long long)word);
+        if (old == REP_EMPTY) break;
+        if ((old >> 32) == (u64)(u32)x) {
+          if ((u32)old > (u32)p) atomicMin((unsigned long long*)&Tc[h], (unsigned long long)word);
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+      atomicAdd(&Wc[h], w);
+    }
+  }
+}
+
+// E[p] = table slot of entry p when it is its x's first occurrence, else -1;
+// chunk_cnt = number of first occurrences in the chunk.
+__global__ __launch_bounds__(256) void k_rep_mark(DGraph g, const i32* tb, const i32* new_ids,
+                                                  const i32* lng, const i32* ch_row,
+                                                  const i64* ch_beg, i64 nch, const i64* dsz,
+                                                  const i64* toff, const i64* csz,
+                                                  const i64* eoff, const u64* T, i32* E,
+                                                  i64* chunk_cnt) {
+  __shared__ i64 s_c[256 / WAVE];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i64 c = lng[ch_row[k2]];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 e = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const u64 mask = (u64)(csz[c] - 1);
+    const u64* Tc = T + toff[c];
+    i32* Ec = E + eoff[c];
+    i64 reps = 0;
+    for (i64 p = b + threadIdx.x; p < e; p += blockDim.x) {
+      i32 x, w;
+      rep_fetch(g, tb, new_ids, c, p, &x, &w);
+      i32 sl = -1;
+      if (!(x == (i32)c || x == -1)) {
+        u64 h = (u64)hash32((u32)x) & mask;
+        u64 t;
+        while (((t = Tc[h]) >> 32) != (u64)(u32)x) h = (h + 1) & mask;
+        if ((u32)t == (u32)p) sl = (i32)h;
+      }
+      Ec[p] = sl;
+      reps += sl >= 0;
+    }
+    reps = wave_sum(reps);
+    if (lane == 0) s_c[wv] = reps;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      i64 t2 = 0;
+      for (int w2 = 0; w2 < 256 / WAVE; w2++) t2 += s_c[w2];
+      chunk_cnt[k2] = t2;
+    }
+    __syncthreads();
+  }
+}
+
+// cnt[c] of every long row = sum of its chunks' counts (chunks are consecutive)
+__global__ void k_rep_rowcnt(const i32* lng, i64 nl, const i64* cbase, const i64* choff,
+                             i64* cnt) {
+  for (i64 e = gtid(); e < nl; e += gstride()) cnt[lng[e]] = choff[cbase[e + 1]] - choff[cbase[e]];
+}
+
+// writes the first occurrences of a chunk at ox[c] + (its earlier chunks' counts)
+__global__ __launch_bounds__(256) void k_rep_emit(const i32* lng, const i32* ch_row,
+                                                  const i64* ch_beg, i64 nch, const i64* dsz,
+                                                  const i64* cbase, const i64* choff,
+                                                  const i64* toff, const i64* eoff, const u64* T,
+                                                  const i32* W, const i32* E, const i64* ox,
+                                                  i32* oa, i32* ow) {
+  __shared__ i32 s_w[256 / WAVE];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i32 e0 = ch_row[k2];
+    const i64 c = lng[e0];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 e = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const u64* Tc = T + toff[c];
+    const i32* Wc = W + toff[c];
+    const i32* Ec = E + eoff[c];
+    i64 run = ox[c] + (choff[k2] - choff[cbase[e0]]);
+    for (i64 base = b; base < e; base += blockDim.x) {
+      const i64 p = base + threadIdx.x;
+      const i32 sl = p < e ? Ec[p] : -1;
+      const u64 bm = __ballot(sl >= 0);
+      if (lane == 0) s_w[wv] = __popcll(bm);
+      __syncthreads();
+      i64 before = 0, tot = 0;
+      for (int w2 = 0; w2 < 256 / WAVE; w2++) {
+        if (w2 < wv) before += s_w[w2];
+        tot += s_w[w2];
+      }
+      if (sl >= 0) {
+        const i64 o = run + before + __popcll(bm & lanes_below(lane));
+        oa[o] = (i32)(Tc[sl] >> 32);
+        ow[o] = Wc[sl];
+      }
+      run += tot;
+      __syncthreads();
+    }
+  }
+}
+
+// Rows with d <= SG entries: SG lanes per row, one entry per lane.  An entry
+// is the first occurrence of its coarse neighbour x when no earlier lane holds
+// x; its weight is the sum over all lanes holding x.  PASS 0 writes the row's
+// count, PASS 1 writes the first occurrences, in position order, at ox[c].
+template <int SG, int PASS>
+__global__ __launch_bounds__(256) void k_rep_small(DGraph g, const i32* tb, const i32* new_ids,
+                                                   i64 n_c, const i64* dsz, i64 dmin, i64* cnt,
+                                                   const i64* ox, i32* oa, i32* ow) {
   const int lane = lane_id();
-  for (i64 c = gwave(); c < n_c; c += nwaves()) {
+  const int sl = lane & (SG - 1);
+  const int sgb = lane & ~(SG - 1);
+  const i64 nsg = (i64)gridDim.x * (blockDim.x / SG);
+  for (i64 c = (i64)blockIdx.x * (blockDim.x / SG) + threadIdx.x / SG; c < n_c; c += nsg) {
+    const i64 d = dsz[c];
+    if (d < dmin || d > SG) continue;
     const i32 r = tb[2 * c], mt = tb[2 * c + 1];
     const i64 a0 = g.xadj[r], la = g.xadj[r + 1] - a0;
-    const i64 b0 = mt >= 0 ? g.xadj[mt] : 0, lb = mt >= 0 ? g.xadj[mt + 1] - b0 : 0;
-    const i64 tot = la + lb;
-    const u32 mask = (u32)(csz[c] - 1);
-    i32* K = tkey + toff[c];
-    i32* P = tpos + toff[c];
-    i32* W = tw + toff[c];
-    for (i64 p = lane; p < tot; p += WAVE) {
-      i32 col, w;
-      rep_entry(g, tb, c, p, la, a0, b0, &col, &w);
-      const i32 x = new_ids[col];
+    const i64 b0 = mt >= 0 ? g.xadj[mt] : 0;
+    bool valid = sl < d;
+    i32 x = -1, w = 0;
+    if (valid) {
+      const i64 i = sl < la ? a0 + sl : b0 + (sl - la);
+      x = new_ids[g.adj[i]];
+      w = g.w[i];
+      valid = !(x == (i32)c || x == -1);
+    }
+    bool first = valid;
+    u32 wsum = 0;
+    const int dq = (int)d;
+    for (int q = 0; q < (SG == WAVE ? dq : SG); q++) {
+      const i32 xq = __shfl(x, q, SG);
+      const i32 wq = __shfl(w, q, SG);
+      const int vq = __shfl((int)valid, q, SG);
+      if (vq && valid && xq == x) {
+        if (q < sl) first = false;
+        wsum += (u32)wq;
+      }
+    }
+    const u64 fm = (__ballot(first) >> sgb) & (SG == WAVE ? ~0ull : ((1ull << SG) - 1));
+    if (PASS == 0) {
+      if (sl == 0) cnt[c] = __popcll(fm);
+    } else if (first) {
+      const i64 o = ox[c] + __popcll(fm & ((1ull << sl) - 1));
+      oa[o] = x;
+      ow[o] = (i32)wsum;
+    }
+  }
+}
+
+// Rows with REP_SMALL < d <= REP_MED: one block per row, an LDS table of
+// capacity pow2 >= 2d recording each x's first position (atomicMin) and
+// weight sum (atomicAdd); PASS 0 counts the first occurrences, PASS 1 writes
+// them in position order (block-wide scan over 256-entry chunks).
+constexpr int REP_TCAP = 4096;
+template <int PASS>
+__global__ __launch_bounds__(256) void k_rep_med(DGraph g, const i32* tb, const i32* new_ids,
+                                                 const i32* lst, i64 nl, const i64* dsz,
+                                                 i64* cnt, const i64* ox, i32* oa, i32* ow) {
+  __shared__ i32 K[REP_TCAP], P[REP_TCAP], W[REP_TCAP];
+  __shared__ i32 s_w[256 / WAVE];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 e = blockIdx.x; e < nl; e += gridDim.x) {
+    const i64 c = lst[e];
+    const i64 d = dsz[c];
+    const i32 r = tb[2 * c], mt = tb[2 * c + 1];
+    const i64 a0 = g.xadj[r], la = g.xadj[r + 1] - a0;
+    const i64 b0 = mt >= 0 ? g.xadj[mt] : 0;
+    const i32 cap = (i32)next_pow2_i64(2 * d);
+    const u32 mask = (u32)(cap - 1);
+    for (int h = threadIdx.x; h < cap; h += blockDim.x) {
+      K[h] = -1;
+      P[h] = INT_MAX;
+      W[h] = 0;
+    }
+    __syncthreads();
+    for (i64 p = threadIdx.x; p < d; p += blockDim.x) {
+      const i64 i = p < la ? a0 + p : b0 + (p - la);
+      const i32 x = new_ids[g.adj[i]];
       if (x == (i32)c || x == -1) continue;
       u32 h = hash32((u32)x) & mask;
       while (true) {
@@ -1182,66 +1405,42 @@ __global__ __launch_bounds__(256) void k_rep_insert(DGraph g, const i32* tb, con
         h = (h + 1) & mask;
       }
       atomicMin(&P[h], (i32)p);
-      atomicAdd(&W[h], w);
+      atomicAdd(&W[h], g.w[i]);
     }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_rep_mark(DGraph g, const i32* tb, const i32* new_ids,
-                                                  i64 n_c, const i64* toff, const i64* csz,
-                                                  const i64* eoff, const i32* tkey,
-                                                  const i32* tpos, i32* eslot, i64* cnt) {
-  const int lane = lane_id();
-  for (i64 c = gwave(); c < n_c; c += nwaves()) {
-    const i32 r = tb[2 * c], mt = tb[2 * c + 1];
-    const i64 a0 = g.xadj[r], la = g.xadj[r + 1] - a0;
-    const i64 b0 = mt >= 0 ? g.xadj[mt] : 0, lb = mt >= 0 ? g.xadj[mt + 1] - b0 : 0;
-    const i64 tot = la + lb;
-    const u32 mask = (u32)(csz[c] - 1);
-    const i32* K = tkey + toff[c];
-    const i32* P = tpos + toff[c];
-    i32* E = eslot + eoff[c];
-    i64 reps = 0;
-    for (i64 p = lane; p < tot; p += WAVE) {
-      i32 col, w;
-      rep_entry(g, tb, c, p, la, a0, b0, &col, &w);
-      const i32 x = new_ids[col];
-      i32 s = -1;
-      if (!(x == (i32)c || x == -1)) {
-        u32 h = hash32((u32)x) & mask;
-        while (K[h] != x) h = (h + 1) & mask;
-        if (P[h] == (i32)p) s = (i32)h;
+    __syncthreads();
+    i64 run = PASS ? ox[c] : 0;
+    for (i64 base = 0; base < d; base += blockDim.x) {
+      const i64 p = base + threadIdx.x;
+      bool first = false;
+      i32 x = -1;
+      u32 h = 0;
+      if (p < d) {
+        const i64 i = p < la ? a0 + p : b0 + (p - la);
+        x = new_ids[g.adj[i]];
+        if (!(x == (i32)c || x == -1)) {
+          h = hash32((u32)x) & mask;
+          while (K[h] != x) h = (h + 1) & mask;
+          first = P[h] == (i32)p;
+        }
       }
-      E[p] = s;
-      reps += (s >= 0);
-    }
-    reps = wave_sum(reps);
-    if (lane == 0) cnt[c] = reps;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_rep_emit(const i64* eoff, const i64* dsz,
-                                                  const i32* eslot, const i64* toff,
-                                                  const i32* tkey, const i32* tw, const i64* ox,
-                                                  i64 n_c, i32* oa, i32* ow) {
-  const int lane = lane_id();
-  for (i64 c = gwave(); c < n_c; c += nwaves()) {
-    const i64 tot = dsz[c];
-    const i32* E = eslot + eoff[c];
-    const i32* K = tkey + toff[c];
-    const i32* W = tw + toff[c];
-    i64 run = ox[c];
-    for (i64 base = 0; base < tot; base += WAVE) {
-      const i64 p = base + lane;
-      const i32 s = p < tot ? E[p] : -1;
-      const u64 bm = __ballot(s >= 0);
-      if (s >= 0) {
-        const i64 o = run + __popcll(bm & lanes_below(lane));
-        oa[o] = K[s];
-        ow[o] = W[s];
+      const u64 bm = __ballot(first);
+      if (lane == 0) s_w[wv] = __popcll(bm);
+      __syncthreads();
+      i64 before = 0, tot = 0;
+      for (int w2 = 0; w2 < (int)(blockDim.x / WAVE); w2++) {
+        if (w2 < wv) before += s_w[w2];
+        tot += s_w[w2];
       }
-      run += __popcll(bm);
+      if (PASS == 1 && first) {
+        const i64 o = run + before + __popcll(bm & lanes_below(lane));
+        oa[o] = x;
+        ow[o] = W[h];
+      }
+      run += tot;
+      __syncthreads();
     }
+    if (PASS == 0 && threadIdx.x == 0) cnt[c] = run;
+    __syncthreads();
   }
 }
 
@@ -2448,26 +2647,49 @@ void launch_compact_rows(const i32* sadj, const i32* sw, const i64* ox, i64 n_c,
   k_compact_rows<<<grid_for(n_c, 4, 1u << 16), 256, 0, s>>>(sadj, sw, ox, n_c, oa, ow);
 }
 
-void launch_rep_sizes(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* csz,
-                      hipStream_t s) {
-  k_rep_sizes<<<grid_for(n_c, 256, 1u << 16), 256, 0, s>>>(xadj, tb, n_c, dsz, csz);
+void launch_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* dlong,
+                         i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk,
+                         hipStream_t s) {
+  k_rep_classify<<<grid_for(n_c, 256, 1u << 16), 256, 0, s>>>(xadj, tb, n_c, dsz, dlong, csz,
+                                                              med, nmed, lng, nlng, nchk);
 }
-void launch_rep_insert(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* toff,
-                       const i64* csz, i32* tkey, i32* tpos, i32* tw, hipStream_t s) {
-  k_rep_insert<<<grid_for(n_c, 4, 1u << 16), 256, 0, s>>>(g, tb, new_ids, n_c, toff, csz, tkey,
-                                                          tpos, tw);
+void launch_rep_chunks(const i32* lng, i64 nl, const i64* dsz, const i64* cbase, i32* ch_row,
+                       i64* ch_beg, hipStream_t s) {
+  if (nl > 0)
+    k_rep_chunks<<<grid_for(nl, 256, 4096), 256, 0, s>>>(lng, nl, dsz, cbase, ch_row, ch_beg);
 }
-void launch_rep_mark(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* toff,
-                     const i64* csz, const i64* eoff, const i32* tkey, const i32* tpos,
-                     i32* eslot, i64* cnt, hipStream_t s) {
-  k_rep_mark<<<grid_for(n_c, 4, 1u << 16), 256, 0, s>>>(g, tb, new_ids, n_c, toff, csz, eoff,
-                                                        tkey, tpos, eslot, cnt);
+void launch_rep_long(DGraph g, const i32* tb, const i32* new_ids, const RepLong& L, int pass,
+                     const i64* ox, i32* oa, i32* ow, hipStream_t s) {
+  if (L.nch <= 0) return;
+  const unsigned gr = grid_for(L.nch, 1, 1u << 16);
+  if (pass == 0) {
+    k_rep_insert<<<gr, 256, 0, s>>>(g, tb, new_ids, L.lng, L.ch_row, L.ch_beg, L.nch, L.dsz,
+                                    L.toff, L.csz, L.T, L.W);
+    k_rep_mark<<<gr, 256, 0, s>>>(g, tb, new_ids, L.lng, L.ch_row, L.ch_beg, L.nch, L.dsz,
+                                  L.toff, L.csz, L.eoff, L.T, L.E, L.chunk_cnt);
+  } else {
+    k_rep_emit<<<gr, 256, 0, s>>>(L.lng, L.ch_row, L.ch_beg, L.nch, L.dsz, L.cbase, L.choff,
+                                  L.toff, L.eoff, L.T, L.W, L.E, ox, oa, ow);
+  }
 }
-void launch_rep_emit(const i64* eoff, const i64* dsz, const i32* eslot, const i64* toff,
-                     const i32* tkey, const i32* tw, const i64* ox, i64 n_c, i32* oa, i32* ow,
-                     hipStream_t s) {
-  k_rep_emit<<<grid_for(n_c, 4, 1u << 16), 256, 0, s>>>(eoff, dsz, eslot, toff, tkey, tw, ox,
-                                                        n_c, oa, ow);
+void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s) {
+  if (L.nl > 0)
+    k_rep_rowcnt<<<grid_for(L.nl, 256, 4096), 256, 0, s>>>(L.lng, L.nl, L.cbase, L.choff, cnt);
+}
+void launch_rep_rows(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* dsz,
+                     const i32* med, i64 nmed, int pass, i64* cnt, const i64* ox, i32* oa,
+                     i32* ow, hipStream_t s) {
+  const unsigned g8 = grid_for(n_c, 256 / 8, 1u << 16), g64 = grid_for(n_c, 4, 1u << 16);
+  const unsigned gm = grid_for(nmed, 1, 1u << 16);
+  if (pass == 0) {
+    k_rep_small<8, 0><<<g8, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 0, cnt, ox, oa, ow);
+    k_rep_small<64, 0><<<g64, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 9, cnt, ox, oa, ow);
+    if (nmed > 0) k_rep_med<0><<<gm, 256, 0, s>>>(g, tb, new_ids, med, nmed, dsz, cnt, ox, oa, ow);
+  } else {
+    k_rep_small<8, 1><<<g8, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 0, cnt, ox, oa, ow);
+    k_rep_small<64, 1><<<g64, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 9, cnt, ox, oa, ow);
+    if (nmed > 0) k_rep_med<1><<<gm, 256, 0, s>>>(g, tb, new_ids, med, nmed, dsz, cnt, ox, oa, ow);
+  }
 }
 void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s) {
   k_pairs<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, cap, count);
