@@ -192,6 +192,7 @@ struct LevelAux {
   bool valid = false;
   i64 nedges = -1;
   i64 e_end = 0;      // arcs [0, e_end) = the reference's row ranges
+  i64 last_begin = 0; // xadj[n-1]
   i64 nheavy = 0;     // rows longer than HEAVY_DEG
   int unit = 0;       // all weights are 1
   DBuf<i32> asrc;     // arc -> source vertex
@@ -204,6 +205,7 @@ struct LevelAux {
   DBuf<i32> gtally;          // nhuge * k tallies and nhuge |w| sums, zero between passes
   DBuf<i64> gabs;
   int all_exact = 0;         // every row's sum of |w| is at most 2^24
+  DBuf<u32> ctr;             // work-item counter of the fused gain kernel
   GainLists lists() const {
     GainLists L;
     L.mid = mid.p;
@@ -219,6 +221,7 @@ struct LevelAux {
     L.gtally = gtally.p;
     L.gabs = gabs.p;
     L.all_exact = all_exact;
+    L.ctr = ctr.p;
     return L;
   }
 };
@@ -226,6 +229,7 @@ struct LevelAux {
 struct Level {
   i64 n = 0, m = 0;
   bool symmetric = false;  // every arc s->x (s != x) has its reverse
+  bool simple_ws = false;  // and every row is simple and each reverse has the same weight
   LevelAux aux;
   DBuf<i64> xadj;
   DBuf<i32> adj, w;
@@ -359,7 +363,7 @@ class Run {
 
   void coarsen();
   void match_level(const Level& lv, DBuf<i32>& mate);
-  bool check_symmetric(const Level& lv);
+  void check_symmetric(Level& lv);
   int initial_partition(DBuf<i32>& part, i64* npairs);
   i64 compute_cut(Level& lv, i64 nedges, const i32* part);
   LevelAux& aux(Level& lv, i64 nedges);
@@ -494,25 +498,27 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
                                                            t_start).count());
 }
 
-bool Run::check_symmetric(const Level& lv) {
+void Run::check_symmetric(Level& lv) {
   i64 cap = 1;
   while (cap < 2 * lv.m + 2) cap <<= 1;
   DBuf<u64> table(&C.pool, cap);
-  DBuf<i32> miss(&C.pool, 1);
-  HIP_TRY(hipMemsetAsync(miss.p, 0, sizeof(i32), st()));
+  DBuf<i32> wt(&C.pool, cap), flags(&C.pool, 3);
+  HIP_TRY(hipMemsetAsync(flags.p, 0, 3 * sizeof(i32), st()));
   C.prof.begin(KID_MISC);
-  launch_sym_check(lv.dg(), table.p, cap, miss.p, st());
-  C.prof.end(KID_MISC, 24.0 * lv.m);
-  HIP_TRY(hipMemcpyAsync(C.pinned, miss.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+  launch_sym_check(lv.dg(), table.p, wt.p, cap, flags.p, st());
+  C.prof.end(KID_MISC, 28.0 * lv.m);
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 3 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
-  return ((const i32*)C.pinned)[0] == 0;
+  const i32* f = (const i32*)C.pinned;
+  lv.symmetric = f[0] == 0;
+  lv.simple_ws = f[0] == 0 && f[1] == 0 && f[2] == 0;
 }
 
 void Run::coarsen() {
   const i64 n0 = L[0]->n;
   const i64 target = O.coarsen_floor > 0 ? O.coarsen_floor : std::max<i64>(500, n0 / 500);
   i64 ncur = n0;
-  if (ncur > target) L[0]->symmetric = check_symmetric(*L[0]);
+  if (ncur > target) check_symmetric(*L[0]);
   DBuf<i32> mate(&C.pool, n0), new_ids(&C.pool, n0);
   DBuf<i64> flags(&C.pool, n0), pos(&C.pool, n0);
   while (ncur > target) {  // gpu_part.cu:1003
@@ -622,6 +628,9 @@ void Run::coarsen() {
     // the repaired contraction of a symmetric level is symmetric; the
     // reference's contraction (row r from fine vertex r) is not
     c->symmetric = repair && lv.symmetric;
+    // a repaired contraction sums both directions of a weight-symmetric level
+    // alike and keeps one entry per neighbour
+    c->simple_ws = repair && lv.simple_ws;
     c->tb = std::move(tb);
     c->xadj_p = c->xadj.p;
     c->adj_p = c->adj.p;
@@ -661,6 +670,7 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   a.nmid = ((const i32*)(C.pinned + 2))[0];
   a.nmedium = ((const i32*)(C.pinned + 2))[1];
   a.nhuge = ((const i32*)(C.pinned + 2))[2];
+  a.ctr.ensure(&C.pool, 8 * 32);  // GAIN_CTRS counters, 128 bytes apart
   // integer tallies are exact for every row when no row's sum of |w| passes 2^24
   a.all_exact = 1;
   if (!a.unit) {
@@ -703,6 +713,7 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
     C.sync();  // the host vectors are freed on return
   }
   a.e_end = std::max<i64>(C.pinned[1], nedges);  // rows 0..n-2 whole, last ends at nedges
+  a.last_begin = C.pinned[1];
   a.nedges = nedges;
   a.valid = true;
   return a;
@@ -800,6 +811,20 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       part8 = DBuf<u8>(&C.pool, n);
       launch_part8(part.p, n, k, part8.p, st());
     }
+    // Kept tallies (n x k ints) on simple weight-symmetric levels with exact
+    // integer tallies: built by the first gain launch after any bulk change of
+    // part, then updated from the applied moves of each commit.
+    const bool tmode = k <= 512 && ax.all_exact && lv.simple_ws &&
+                       (double)n * k * 4 <= 8e9 && env_i64("TGP_KEEP_TALLIES", 1) != 0;
+    DBuf<i32> T, app, napp;
+    bool T_valid = false;
+    const bool last_full = nedges == lv.m;
+    const bool last_partial = !last_full && nedges > ax.last_begin;
+    if (tmode) {
+      T = DBuf<i32>(&C.pool, n * k);
+      app = DBuf<i32>(&C.pool, n);
+      napp = DBuf<i32>(&C.pool, 1);
+    }
     DBuf<u8> sel(&C.pool, n), src_big(&C.pool, k);
     DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
     DBuf<float> bg(&C.pool, 2 * (size_t)k);
@@ -832,11 +857,20 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
           C.prof.end(KID_MISC, 0);
           C.prof.begin(KID_GAIN);
-          const bool fused = launch_gain(g, nedges, part.p, part8.p, k, bg.p, src_big.p, gain.p,
-                                         target.p,
-                                         cand.p, phase, unit, gscr.p, ax.lists(),
-                                         pending ? accs.p : nullptr, st());
-          C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
+          bool fused;
+          if (tmode && T_valid) {
+            if (last_partial) launch_tally_last(g, nedges, part.p, T.p, k, unit, st());
+            launch_gain_tallies(n, part.p, T.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
+                                phase, pending ? accs.p : nullptr, st());
+            fused = true;
+            C.prof.end(KID_GAIN, 4.0 * k * n + 17.0 * n);
+          } else {
+            fused = launch_gain(g, nedges, part.p, part8.p, k, bg.p, src_big.p, gain.p, target.p,
+                                cand.p, phase, unit, gscr.p, ax.lists(),
+                                pending ? accs.p : nullptr, tmode ? T.p : nullptr, st());
+            T_valid = tmode;
+            C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
+          }
           if (pending && !fused) {
             C.prof.begin(KID_CUT);
             launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, accs.p, st());
@@ -918,7 +952,10 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         C.prof.end(KID_SELECT, 13.0 * M);
         C.prof.begin(KID_COMMIT);
         launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, part8.p, pc.p, k,
-                      st());
+                      tmode ? app.p : nullptr, napp.p, st());
+        if (tmode && T_valid)
+          launch_tally_update(g, T.p, k, app.p, napp.p, M, mv_v.p, mv_src.p, mv_dst.p, unit,
+                              last_full ? 0 : 1, st());
         C.prof.end(KID_COMMIT, 17.0 * M);
         pending = true;
         prevM = M;
@@ -926,6 +963,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       }
       HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
       if (part8.p) launch_part8(part.p, n, k, part8.p, st());
+      T_valid = false;
       if (phase == 0) *cut = bestCut;
     }
   }

@@ -97,8 +97,10 @@ void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
                             i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s);
-// *missing |= 1 when some arc s->x (s != x) lacks its reverse x->s.
-void launch_sym_check(DGraph g, u64* table, i64 cap, i32* missing, hipStream_t s);
+// Arc checks of a level (table: cap >= 2m slots, pow2; wt: cap ints):
+// flags[0] |= some arc s->x (s != x, x < n) has no reverse; flags[1] |= some
+// (s, x) appears twice in s's row; flags[2] |= some reverse has another weight.
+void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);
 // Reference contraction (gpu_part.cu:145-242 semantics): padded rows of 64.
@@ -175,7 +177,7 @@ void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32*
 // long rows (> 4096) and the long rows' chunks of at most GAIN_CHUNK_ARCS arcs
 // with per-row accumulators (nhuge * k tallies, nhuge |w| sums, all zero
 // between passes).
-constexpr i64 GAIN_CHUNK_ARCS = 16384;
+constexpr i64 GAIN_CHUNK_ARCS = 4096;
 struct GainLists {
   const i32* mid = nullptr;
   i64 nmid = 0;
@@ -190,6 +192,7 @@ struct GainLists {
   i32* gtally = nullptr;
   i64* gabs = nullptr;
   int all_exact = 0;  // every row's sum of |w| is at most 2^24 (integer tallies are exact)
+  u32* ctr = nullptr;  // 8 work counters (256 u32) of the fused gain kernel, reset by it
 };
 // *over |= (some row's sum of |w| > 2^24); heavy = rows longer than heavy_deg
 void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
@@ -197,9 +200,22 @@ void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 n
 // part8 (optional, k <= PART8_MAXK): byte copy of part that the neighbour
 // gathers read instead of part.
 constexpr i32 PART8_MAXK = 255;
+// Tout (optional, n * k ints): also writes each vertex's integer tally.
 bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
-                 i32* gscratch, const GainLists& L, u64* cut_acc, hipStream_t s);
+                 i32* gscratch, const GainLists& L, u64* cut_acc, i32* Tout, hipStream_t s);
+// Gains (and the cut into cut_acc slots) from kept tallies T (n * k ints).
+void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
+                         const u8* src_big, float* gain, i32* target, float* cand, int phase,
+                         u64* cut_acc, hipStream_t s);
+// T update for the applied moves app[0..*napp) (at most max_app) of a commit;
+// skip_last leaves T[n-1] alone.
+void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
+                         const i32* mv_v, const i32* mv_src, const i32* mv_dst, int unit,
+                         int skip_last, hipStream_t s);
+// T[n-1] from its range [xadj[n-1], nedges)
+void launch_tally_last(DGraph g, i64 nedges, const i32* part, i32* T, i32 k, int unit,
+                       hipStream_t s);
 // part8[v] = part[v] if 0 <= part[v] < k, else 255
 void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s);
 // hb[2h], hb[2h+1] = arc range of long row h (the last row ends at nedges)
@@ -222,9 +238,11 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32
 i64 select_scratch_elems(i64 M, i32 k);
 void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64* avail,
                    i32* sdst, i32* H, u8* sel, i32* S, i32* pads, hipStream_t s);
-// part8 (optional) is kept equal to part's byte copy
+// part8 (optional) is kept equal to part's byte copy; app (optional) receives
+// the ids of the applied moves, *napp their number
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
-                   const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, hipStream_t s);
+                   const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, i32* app,
+                   i32* napp, hipStream_t s);
 // Accumulators for block sums: ACC_SLOTS slots, one per 128-byte line, all
 // zero between uses.  Kernels add into them; launch_acc_finish writes the
 // total to *out and clears the slots.

@@ -987,9 +987,14 @@ __global__ __launch_bounds__(1024) void k_first_undecided_st(const i32* st, i64 
 }
 
 // Symmetry test of a level: every arc s->x (s != x, x < n) must have x->s.
-__global__ __launch_bounds__(256) void k_sym_insert(DGraph g, u64* table, i64 cap) {
+// Also records each arc's weight and flags a repeated (v, u) as a duplicate,
+// so the check also tells whether every row is simple and every arc's reverse
+// has the same weight (flags[1], flags[2]).
+__global__ __launch_bounds__(256) void k_sym_insert(DGraph g, u64* table, i32* wt, i64 cap,
+                                                    i32* flags) {
   const u64 mask = (u64)cap - 1;
   const int lane = lane_id();
+  bool dup = false;
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
     for (i64 i = s0 + lane; i < s1; i += WAVE) {
@@ -999,17 +1004,25 @@ __global__ __launch_bounds__(256) void k_sym_insert(DGraph g, u64* table, i64 ca
       u64 h = hash64(key) & mask;
       while (true) {
         const u64 old = atomicCAS((unsigned long long*)&table[h], ~0ull, (unsigned long long)key);
-        if (old == ~0ull || old == key) break;
+        if (old == ~0ull) {
+          wt[h] = g.w[i];
+          break;
+        }
+        if (old == key) {
+          dup = true;
+          break;
+        }
         h = (h + 1) & mask;
       }
     }
   }
+  if (__ballot(dup) && lane == 0) atomicOr(&flags[1], 1);
 }
-__global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, i64 cap,
-                                                   i32* missing) {
+__global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, const i32* wt,
+                                                   i64 cap, i32* flags) {
   const u64 mask = (u64)cap - 1;
   const int lane = lane_id();
-  bool miss = false;
+  bool miss = false, wmis = false;
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
     for (i64 i = s0 + lane; i < s1; i += WAVE) {
@@ -1019,7 +1032,10 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, i
       u64 h = hash64(key) & mask;
       while (true) {
         const u64 cur = table[h];
-        if (cur == key) break;
+        if (cur == key) {
+          wmis |= wt[h] != g.w[i];
+          break;
+        }
         if (cur == ~0ull) {
           miss = true;
           break;
@@ -1028,7 +1044,8 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, i
       }
     }
   }
-  if (__ballot(miss) && lane == 0) atomicOr(missing, 1);
+  if (__ballot(miss) && lane == 0) atomicOr(&flags[0], 1);
+  if (__ballot(wmis) && lane == 0) atomicOr(&flags[2], 1);
 }
 
 // ------------------------------------------------------------------ relabel --
@@ -1649,197 +1666,12 @@ __global__ __launch_bounds__(WAVE) void k_gain(DGraph g, i64 nedges, const i32* 
 constexpr int GAIN_SLOTS = 16;
 constexpr int GAIN_LIGHT = 64;
 
-// gpu_part.cu:258-312 with G lanes per vertex: each group keeps a dense LDS
-// tally of k ints (exact while the row's sum of |w| <= 2^24, else one lane
-// redoes the row in fp32 in row order), then every lane scores k/G parts and
-// the group reduces by (gain desc, part asc); a zero tally scores exactly like
-// an absent part, so the dense scan is the reference's loop over q.  G = 8
-// for rows of at most 64 arcs (eight vertices per wave in flight), G = 64 for
-// the heavy list.
-template <int G, bool UNIT, typename PT>
-__global__ __launch_bounds__(256) void k_gain_group(DGraph g, i64 nedges, const PT* part, i32 k,
-                                                    const float* bg_g, const u8* src_big,
-                                                    float* gain, i32* target, float* cand,
-                                                    int phase, const i32* list, i64 cnt,
-                                                    i64 light_max, int all_exact, u64* cut_acc) {
-  extern __shared__ i32 lds[];
-  constexpr int GPB = 256 / G;  // groups per block
-  u64 cutv = 0;  // weight of this thread's arcs whose ends lie in different parts
-  float* sbg = (float*)lds;
-  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
-  const int gl = threadIdx.x & (G - 1);
-  const int grp = threadIdx.x / G;
-  // one tally of k ints per group at an odd stride, so that the groups of a
-  // wave fall in different LDS banks; the pick below leaves it zeroed
-  i32* tally = lds + 2 * k + grp * (k | 1);
-  float* ftally = (float*)tally;
-  for (int q = gl; q < k; q += G) tally[q] = 0;
-  __syncthreads();
-  const i64 total = list ? cnt : g.n;
-  const i64 ngroups = (i64)gridDim.x * GPB;
-  for (i64 e = (i64)blockIdx.x * GPB + grp; e < total; e += ngroups) {
-    const i64 v = list ? list[e] : e;
-    const i64 s0 = g.xadj[v];
-    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    const i32 cur = part[v];
-    if (!list && s1 - s0 > light_max) continue;   // heavy rows: the G = 64 launch
-    if (cur >= k || cur < 0) continue;            // gpu_part.cu:262
-    i64 absw = 0;
-    // GU independent loads per lane in flight, then their gathers
-    constexpr int GU = G == 64 ? 4 : 8;
-    for (i64 base = s0; base < s1; base += G * GU) {
-      i32 uu[GU], ww[GU], pp[GU];
-#pragma unroll
-      for (int j = 0; j < GU; j++) {
-        const i64 i = base + j * G + gl;
-        uu[j] = i < s1 ? g.adj[i] : INT_MAX;
-        ww[j] = (UNIT || i >= s1) ? 1 : g.w[i];
-      }
-#pragma unroll
-      for (int j = 0; j < GU; j++) pp[j] = uu[j] < g.n ? part[uu[j]] : -1;
-#pragma unroll
-      for (int j = 0; j < GU; j++) {
-        if (pp[j] < k && pp[j] >= 0) {
-          atomicAdd(&tally[pp[j]], ww[j]);
-          if (!UNIT && !all_exact) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
-          if (pp[j] != cur) cutv += (u64)(i64)ww[j];
-        }
-      }
-    }
-    if (!UNIT && !all_exact) {
-#pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) absw += __shfl_xor(absw, o, G);
-    }
-    __builtin_amdgcn_wave_barrier();
-    const bool exact_int =
-        UNIT ? (s1 - s0) <= (i64)(1 << 24) : (all_exact || absw <= (i64)(1 << 24));
-    if (!exact_int) {
-      for (int q = gl; q < k; q += G) ftally[q] = 0.0f;
-      __builtin_amdgcn_wave_barrier();
-      if (gl == 0) {
-        for (i64 i = s0; i < s1; i++) {
-          const i32 u = g.adj[i];
-          if (u < g.n) {
-            const i32 p = part[u];
-            if (p < k && p >= 0) ftally[p] += (float)g.w[i];
-          }
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
-    const float ccur = exact_int ? (float)tally[cur] : ftally[cur];
-    __builtin_amdgcn_wave_barrier();
-    const float* bgr = sbg + (src_big[cur] ? k : 0);
-    float bgv = -INFINITY;
-    i32 bq = INT_MAX;
-    for (int q = gl; q < k; q += G) {
-      const float cq = exact_int ? (float)tally[q] : ftally[q];
-      tally[q] = 0;
-      if (q == cur) continue;
-      const float eg = cq - ccur;
-      const float gg = eg + bgr[q];
-      if (gg > bgv || (gg == bgv && q < bq)) {
-        bgv = gg;
-        bq = q;
-      }
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-      const float og = __shfl_xor(bgv, o, G);
-      const i32 oq = __shfl_xor(bq, o, G);
-      if (og > bgv || (og == bgv && oq < bq)) {
-        bgv = og;
-        bq = oq;
-      }
-    }
-    if (gl == 0) {
-      float best;
-      i32 tgt;
-      if (bq == INT_MAX || bgv < -1e9f) {
-        best = -1e9f;
-        tgt = cur;
-      } else if (bgv == -1e9f) {
-        best = -1e9f;
-        tgt = bq < cur ? bq : cur;
-      } else {
-        best = bgv;
-        tgt = bq;
-      }
-      gain[v] = best;
-      target[v] = tgt;
-      cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (cut_acc) block_add_u64(cutv, cut_acc);
-}
 
 // Rows longer than GAIN_HUGE arcs: one block of 1024 threads per vertex with
 // four independent loads per thread in flight; block-wide LDS tally and pick.
 constexpr int GAIN_HUGE = 4096;
 constexpr int HUGE_BS = 1024;
 
-// Rows longer than GAIN_HUGE arcs are cut into chunks of GAIN_CHUNK_ARCS arcs with
-// one block per chunk, so that a long row is spread over many CUs: a block
-// tallies its chunk in LDS and adds the tally, the chunk's sum of |w| and its
-// share of the cut to per-row accumulators in global memory.  Integer tallies
-// are independent of the order of the additions.
-template <bool UNIT, typename PT>
-__global__ __launch_bounds__(HUGE_BS) void k_gain_chunk(DGraph g, const PT* part, i32 k,
-                                                        const i32* huge, const i32* ch_row,
-                                                        const i64* ch_beg, const i64* ch_end,
-                                                        i64 nch, i32* gtally, i64* gabs,
-                                                        u64* cut_acc) {
-  extern __shared__ i32 lds[];
-  __shared__ i64 s_abs[HUGE_BS / WAVE];
-  u64 cutv = 0;
-  const int lane = lane_id(), wv = threadIdx.x >> 6;
-  for (i64 c = blockIdx.x; c < nch; c += gridDim.x) {
-    const i32 h = ch_row[c];
-    const i32 cur = part[huge[h]];
-    if (cur >= k || cur < 0) continue;  // gpu_part.cu:262 (same for the whole block)
-    for (int q = threadIdx.x; q < k; q += HUGE_BS) lds[q] = 0;
-    __syncthreads();
-    const i64 e0 = ch_beg[c], e1 = ch_end[c];
-    i64 absw = 0;
-    for (i64 base = e0; base < e1; base += 4 * HUGE_BS) {
-      i32 uu[4], ww[4], pp[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const i64 i = base + j * HUGE_BS + threadIdx.x;
-        uu[j] = i < e1 ? g.adj[i] : INT_MAX;
-        ww[j] = (UNIT || i >= e1) ? 1 : g.w[i];
-      }
-#pragma unroll
-      for (int j = 0; j < 4; j++) pp[j] = uu[j] < g.n ? part[uu[j]] : -1;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (pp[j] >= 0 && pp[j] < k) {
-          atomicAdd(&lds[pp[j]], ww[j]);
-          if (!UNIT) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
-          if (pp[j] != cur) cutv += (u64)(i64)ww[j];
-        }
-      }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < k; q += HUGE_BS) {
-      const i32 tq = lds[q];
-      if (tq) atomicAdd(&gtally[(i64)h * k + q], tq);
-    }
-    if (!UNIT) {
-      absw = wave_sum(absw);
-      if (lane == 0) s_abs[wv] = absw;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        i64 tot = 0;
-        for (int w2 = 0; w2 < HUGE_BS / WAVE; w2++) tot += s_abs[w2];
-        if (tot) atomicAdd((unsigned long long*)&gabs[h], (unsigned long long)tot);
-      }
-    }
-    __syncthreads();
-  }
-  if (cut_acc) block_add_u64(cutv, cut_acc);
-}
 
 // Target and gain of every long row from its accumulated tally (a row whose
 // sum of |w| exceeds 2^24 is re-tallied in fp32 in row order by one thread, as
@@ -1850,7 +1682,7 @@ __global__ __launch_bounds__(256) void k_gain_huge_final(DGraph g, i64 nedges, c
                                                          const u8* src_big, float* gain,
                                                          i32* target, float* cand, int phase,
                                                          const i32* huge, i64 nhuge,
-                                                         i32* gtally, i64* gabs) {
+                                                         i32* gtally, i64* gabs, i32* Tout) {
   extern __shared__ i32 lds[];
   __shared__ float s_g[256 / WAVE];
   __shared__ i32 s_q[256 / WAVE];
@@ -1934,7 +1766,10 @@ __global__ __launch_bounds__(256) void k_gain_huge_final(DGraph g, i64 nedges, c
       }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < k; q += blockDim.x) gt[q] = 0;
+    for (int q = threadIdx.x; q < k; q += blockDim.x) {
+      if (Tout) Tout[v * k + q] = gt[q];
+      gt[q] = 0;
+    }
     if (threadIdx.x == 0) gabs[h] = 0;
     __syncthreads();
   }
@@ -1946,6 +1781,359 @@ __global__ void k_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, 
     const i64 v = huge[h];
     hb[2 * h] = g.xadj[v];
     hb[2 * h + 1] = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+  }
+}
+
+// ---- fused gain pass ---------------------------------------------------------
+constexpr unsigned GAIN_FUSED_BLOCKS = 2048;  // 8 blocks of 4 waves per CU
+constexpr int GAIN_CTRS = 8;                  // work counters, 128 bytes apart
+// One persistent launch serves every row class.  Waves take work items from
+// one counter, largest first: chunks of long rows (one wave per chunk), then
+// medium rows (one wave per vertex), mid rows (4 vertices per wave, 16 lanes
+// each), light rows (8 vertices per wave, 8 lanes each).  All classes run at
+// once and the dynamic order balances the power-law row lengths; the next
+// item's counter increment is issued before the current item is processed.
+// Each wave owns 8 tallies of k ints in LDS (odd stride), zero between items.
+
+// One vertex per G-lane group: tally, exactness, pick (the body of the
+// per-class kernels).  `tally` is the group's LDS tally, zero on entry and exit.
+template <int G, bool UNIT, typename PT>
+__device__ __forceinline__ void gain_vertex(const DGraph& g, i64 nedges, const PT* part, i32 k,
+                                            const float* sbg, const u8* src_big, float* gain,
+                                            i32* target, float* cand, int phase, int all_exact,
+                                            i64 v, i32* tally, u64& cutv, int gl, i32* Tout) {
+  float* ftally = (float*)tally;
+  const i64 s0 = g.xadj[v];
+  const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+  const i32 cur = part[v];
+  if (cur >= k || cur < 0) return;  // gpu_part.cu:262
+  i64 absw = 0;
+  i32 own = 0;  // arcs into cur are summed in a register (most arcs; one contended LDS word)
+  constexpr int GU = 8;
+  // software pipeline: the next step's arcs are loaded before this step's
+  // gathers and tallies, so two steps of loads are in flight
+  i32 uu[GU], ww[GU];
+#pragma unroll
+  for (int j = 0; j < GU; j++) {
+    const i64 i = s0 + j * G + gl;
+    uu[j] = i < s1 ? g.adj[i] : INT_MAX;
+    ww[j] = (UNIT || i >= s1) ? 1 : g.w[i];
+  }
+  for (i64 base = s0; base < s1; base += G * GU) {
+    i32 pp[GU];
+#pragma unroll
+    for (int j = 0; j < GU; j++) pp[j] = uu[j] < g.n ? part[uu[j]] : -1;
+    const i32 cw[GU] = {ww[0], ww[1], ww[2], ww[3], ww[4], ww[5], ww[6], ww[7]};
+    const i64 nb = base + G * GU;
+    if (nb < s1) {
+#pragma unroll
+      for (int j = 0; j < GU; j++) {
+        const i64 i = nb + j * G + gl;
+        uu[j] = i < s1 ? g.adj[i] : INT_MAX;
+        ww[j] = (UNIT || i >= s1) ? 1 : g.w[i];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < GU; j++) {
+      if (pp[j] < k && pp[j] >= 0) {
+        if (!UNIT && !all_exact) absw += cw[j] < 0 ? -(i64)cw[j] : (i64)cw[j];
+        if (pp[j] == cur) {
+          own = (i32)((u32)own + (u32)cw[j]);
+        } else {
+          atomicAdd(&tally[pp[j]], cw[j]);
+          cutv += (u64)(i64)cw[j];
+        }
+      }
+    }
+  }
+  if (!UNIT && !all_exact) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) absw += __shfl_xor(absw, o, G);
+  }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) own = (i32)((u32)own + (u32)__shfl_xor(own, o, G));
+  __builtin_amdgcn_wave_barrier();
+  const bool exact_int =
+      UNIT ? (s1 - s0) <= (i64)(1 << 24) : (all_exact || absw <= (i64)(1 << 24));
+  if (!exact_int) {
+    for (int q = gl; q < k; q += G) ftally[q] = 0.0f;
+    __builtin_amdgcn_wave_barrier();
+    if (gl == 0) {
+      for (i64 i = s0; i < s1; i++) {
+        const i32 u = g.adj[i];
+        if (u < g.n) {
+          const i32 p = part[u];
+          if (p < k && p >= 0) ftally[p] += (float)g.w[i];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const float ccur = exact_int ? (float)own : ftally[cur];
+  __builtin_amdgcn_wave_barrier();
+  const float* bgr = sbg + (src_big[cur] ? k : 0);
+  float bgv = -INFINITY;
+  i32 bq = INT_MAX;
+  for (int q = gl; q < k; q += G) {
+    const i32 tq = tally[q];
+    const float cq = exact_int ? (float)tq : ftally[q];
+    tally[q] = 0;
+    if (Tout) Tout[v * k + q] = q == cur ? own : tq;  // (integer tallies: exact_int holds)
+    if (q == cur) continue;
+    const float eg = cq - ccur;
+    const float gg = eg + bgr[q];
+    if (gg > bgv || (gg == bgv && q < bq)) {
+      bgv = gg;
+      bq = q;
+    }
+  }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    const float og = __shfl_xor(bgv, o, G);
+    const i32 oq = __shfl_xor(bq, o, G);
+    if (og > bgv || (og == bgv && oq < bq)) {
+      bgv = og;
+      bq = oq;
+    }
+  }
+  if (gl == 0) {
+    float best;
+    i32 tgt;
+    if (bq == INT_MAX || bgv < -1e9f) {
+      best = -1e9f;
+      tgt = cur;
+    } else if (bgv == -1e9f) {
+      best = -1e9f;
+      tgt = bq < cur ? bq : cur;
+    } else {
+      best = bgv;
+      tgt = bq;
+    }
+    gain[v] = best;
+    target[v] = tgt;
+    cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One chunk of a long row by one wave: tally in the wave's LDS, merged into the
+// row's global tally with integer atomics (order-free); the tally is re-zeroed.
+template <bool UNIT, typename PT>
+__device__ __forceinline__ void gain_chunk_wave(const DGraph& g, const PT* part, i32 k,
+                                                const GainLists& L, i64 c, i32* tally,
+                                                u64& cutv, int lane, int all_exact) {
+  const i32 h = L.ch_row[c];
+  const i32 cur = part[L.huge[h]];
+  if (cur >= k || cur < 0) return;  // gpu_part.cu:262
+  const i64 e0 = L.ch_beg[c], e1 = L.ch_end[c];
+  i64 absw = 0;
+  i32 own = 0;
+  for (i64 base = e0; base < e1; base += 8 * WAVE) {
+    i32 uu[8], ww[8], pp[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const i64 i = base + j * WAVE + lane;
+      uu[j] = i < e1 ? g.adj[i] : INT_MAX;
+      ww[j] = (UNIT || i >= e1) ? 1 : g.w[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) pp[j] = uu[j] < g.n ? part[uu[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (pp[j] >= 0 && pp[j] < k) {
+        if (!UNIT && !all_exact) absw += ww[j] < 0 ? -(i64)ww[j] : (i64)ww[j];
+        if (pp[j] == cur) {
+          own = (i32)((u32)own + (u32)ww[j]);
+        } else {
+          atomicAdd(&tally[pp[j]], ww[j]);
+          cutv += (u64)(i64)ww[j];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) own = (i32)((u32)own + (u32)__shfl_xor(own, o));
+  if (lane == 0 && own) atomicAdd(&L.gtally[(i64)h * k + cur], own);
+  __builtin_amdgcn_wave_barrier();
+  for (int q = lane; q < k; q += WAVE) {
+    const i32 tq = tally[q];
+    tally[q] = 0;
+    if (tq) atomicAdd(&L.gtally[(i64)h * k + q], tq);
+  }
+  if (!UNIT && !all_exact) {
+    absw = wave_sum(absw);
+    if (lane == 0 && absw) atomicAdd((unsigned long long*)&L.gabs[h], (unsigned long long)absw);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <bool UNIT, typename PT>
+__global__ __launch_bounds__(256) void k_gain_fused(DGraph g, i64 nedges, const PT* part, i32 k,
+                                                    const float* bg_g, const u8* src_big,
+                                                    float* gain, i32* target, float* cand,
+                                                    int phase, GainLists L, u32* ctr,
+                                                    u64* cut_acc, i32* Tout) {
+  extern __shared__ i32 lds[];
+  float* sbg = (float*)lds;
+  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const int ks = k | 1;
+  i32* wt = lds + 2 * k + wv * 8 * ks;
+  for (int q = lane; q < 8 * ks; q += WAVE) wt[q] = 0;
+  __syncthreads();
+  const int all_exact = L.all_exact;
+  u64 cutv = 0;
+  // items: long-row chunks, medium rows, 16 mid rows, 64 vertices (light rows)
+  const i64 e1 = L.nch, e2 = e1 + L.nmedium, e3 = e2 + (L.nmid + 15) / 16,
+            e4 = e3 + (g.n + 63) / 64;
+  // GAIN_CTRS counters, one per 128-byte line, hand out interleaved items
+  const int cq = (blockIdx.x * 4 + wv) % GAIN_CTRS;
+  u32* myctr = ctr + cq * 32;
+  u32 it = 0;
+  if (lane == 0) it = atomicAdd(myctr, 1u);
+  i64 item = (i64)__shfl(it, 0) * GAIN_CTRS + cq;
+  while (item < e4) {
+    u32 nxt = 0;
+    if (lane == 0) nxt = atomicAdd(myctr, 1u);
+    if (item < e1) {
+      gain_chunk_wave<UNIT, PT>(g, part, k, L, item, wt, cutv, lane, all_exact);
+    } else if (item < e2) {
+      gain_vertex<64, UNIT, PT>(g, nedges, part, k, sbg, src_big, gain, target, cand, phase,
+                                all_exact, L.medium[item - e1], wt, cutv, lane, Tout);
+    } else if (item < e3) {
+      for (int r = 0; r < 4; r++) {
+        const i64 idx = (item - e2) * 16 + r * 4 + (lane >> 4);
+        if (idx < L.nmid)
+          gain_vertex<16, UNIT, PT>(g, nedges, part, k, sbg, src_big, gain, target, cand, phase,
+                                    all_exact, L.mid[idx], wt + (lane >> 4) * ks, cutv,
+                                    lane & 15, Tout);
+      }
+    } else {
+      for (int r = 0; r < 8; r++) {
+        const i64 v = (item - e3) * 64 + r * 8 + (lane >> 3);
+        bool light = v < g.n;
+        if (light) {
+          const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+          light = s1 - g.xadj[v] <= GAIN_LIGHT;
+        }
+        if (light)
+          gain_vertex<8, UNIT, PT>(g, nedges, part, k, sbg, src_big, gain, target, cand, phase,
+                                   all_exact, v, wt + (lane >> 3) * ks, cutv, lane & 7, Tout);
+      }
+    }
+    item = (i64)__shfl(nxt, 0) * GAIN_CTRS + cq;
+  }
+  if (cut_acc) block_add_u64(cutv, cut_acc);
+}
+
+// ---- refinement from kept tallies --------------------------------------------
+// On a level whose rows are simple and weight-symmetric, T[v][q] (the weight of
+// v's arcs into part q, the tally the gain kernels gather) is kept across
+// passes: only the neighbours of the moved vertices change, by -w on the old
+// part and +w on the new one, and a pass reads T instead of gathering part[u]
+// over every arc.  Integer tallies make the two ways of computing T equal.
+
+// The pick of gain_vertex from T, 16 lanes per vertex; also the vertex's
+// share of the edge cut (its arcs into other parts).
+__global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, const i32* T,
+                                                      i32 k, const float* bg_g,
+                                                      const u8* src_big, float* gain,
+                                                      i32* target, float* cand, int phase,
+                                                      u64* cut_acc) {
+  extern __shared__ float sbgt[];
+  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbgt[q] = bg_g[q];
+  __syncthreads();
+  constexpr int G = 16;
+  const int gl = threadIdx.x & (G - 1);
+  const i64 ngroups = (i64)gridDim.x * (blockDim.x / G);
+  u64 cutv = 0;
+  for (i64 v = (i64)blockIdx.x * (blockDim.x / G) + threadIdx.x / G; v < n; v += ngroups) {
+    const i32 cur = part[v];
+    if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
+    const i32* Tv = T + v * k;
+    const float ccur = (float)Tv[cur];
+    const float* bgr = sbgt + (src_big[cur] ? k : 0);
+    float bgv = -INFINITY;
+    i32 bq = INT_MAX;
+    i64 other = 0;
+    for (int q = gl; q < k; q += G) {
+      const i32 tq = Tv[q];
+      if (q == cur) continue;
+      other += tq;
+      const float eg = (float)tq - ccur;
+      const float gg = eg + bgr[q];
+      if (gg > bgv || (gg == bgv && q < bq)) {
+        bgv = gg;
+        bq = q;
+      }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      const float og = __shfl_xor(bgv, o, G);
+      const i32 oq = __shfl_xor(bq, o, G);
+      if (og > bgv || (og == bgv && oq < bq)) {
+        bgv = og;
+        bq = oq;
+      }
+    }
+    cutv += (u64)other;
+    if (gl == 0) {
+      float best;
+      i32 tgt;
+      if (bq == INT_MAX || bgv < -1e9f) {
+        best = -1e9f;
+        tgt = cur;
+      } else if (bgv == -1e9f) {
+        best = -1e9f;
+        tgt = bq < cur ? bq : cur;
+      } else {
+        best = bgv;
+        tgt = bq;
+      }
+      gain[v] = best;
+      target[v] = tgt;
+      cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
+    }
+  }
+  if (cut_acc) block_add_u64(cutv, cut_acc);
+}
+
+// After a commit: every applied move (v: src -> dst) shifts the weight of each
+// of v's arcs (v, u) from T[u][src] to T[u][dst]; the reverse arc (u, v) has
+// the same weight on these levels.  T[n-1] is skipped when the last row's
+// range is not the whole row (it is rebuilt by k_tally_last then).
+__global__ __launch_bounds__(256) void k_tally_update(DGraph g, i32* T, i32 k, const i32* app,
+                                                      const i32* napp, const i32* mv_v,
+                                                      const i32* mv_src, const i32* mv_dst,
+                                                      int unit, int skip_last) {
+  const int lane = lane_id();
+  const i64 cnt = *napp;
+  for (i64 e = gwave(); e < cnt; e += nwaves()) {
+    const i32 id = app[e];
+    const i32 v = mv_v[id], src = mv_src[id], dst = mv_dst[id];
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      if (u < 0 || u >= g.n || (skip_last && u == g.n - 1)) continue;
+      const i32 w = unit ? 1 : g.w[i];
+      atomicAdd(&T[(i64)u * k + src], -w);
+      atomicAdd(&T[(i64)u * k + dst], w);
+    }
+  }
+}
+
+// T[n-1] from its range [xadj[n-1], nedges) (one block)
+__global__ __launch_bounds__(256) void k_tally_last(DGraph g, i64 nedges, const i32* part,
+                                                    i32* T, i32 k, int unit) {
+  const i64 v = g.n - 1;
+  i32* Tv = T + v * k;
+  for (int q = threadIdx.x; q < k; q += blockDim.x) Tv[q] = 0;
+  __syncthreads();
+  for (i64 i = g.xadj[v] + threadIdx.x; i < nedges; i += blockDim.x) {
+    const i32 u = g.adj[i];
+    if (u >= g.n) continue;
+    const i32 p = part[u];
+    if (p >= 0 && p < k) atomicAdd(&Tv[p], unit ? 1 : g.w[i]);
   }
 }
 
@@ -2449,26 +2637,40 @@ template <bool LDS>
 __global__ __launch_bounds__(256) void k_commit(const i32* sk_idx, const u8* sel, i64 M,
                                                 const i32* mv_v, const i32* mv_src,
                                                 const i32* mv_dst, i32* part, u8* part8,
-                                                i32* pc, i32 k) {
+                                                i32* pc, i32 k, i32* app, i32* napp) {
   __shared__ i32 dpc[LDS ? MAXK_LDS : 1];
   if (LDS) {
     for (int q = threadIdx.x; q < k; q += blockDim.x) dpc[q] = 0;
     __syncthreads();
   }
-  for (i64 i = gtid(); i < M; i += gstride()) {
-    if (!sel[i]) continue;
-    const i32 id = sk_idx[i];
-    if (id < 0) continue;
-    const i32 src = mv_src[id], dst = mv_dst[id];
-    if (src < k && dst < k && src >= 0 && dst >= 0) {
-      const i32 v = mv_v[id];
-      const i32 old = part[v];
-      if (old == src) {
-        part[v] = dst;
-        if (part8) part8[v] = (u8)dst;
-        atomicAdd(LDS ? &dpc[dst] : &pc[dst], 1);
-        atomicAdd(LDS ? &dpc[src] : &pc[src], -1);
+  const int lane = lane_id();
+  for (i64 base = (i64)blockIdx.x * blockDim.x; base < M; base += gstride()) {
+    const i64 i = base + threadIdx.x;
+    bool applied = false;
+    i32 id = -1;
+    if (i < M && sel[i]) {
+      id = sk_idx[i];
+      if (id >= 0) {
+        const i32 src = mv_src[id], dst = mv_dst[id];
+        if (src < k && dst < k && src >= 0 && dst >= 0) {
+          const i32 v = mv_v[id];
+          const i32 old = part[v];
+          if (old == src) {
+            part[v] = dst;
+            if (part8) part8[v] = (u8)dst;
+            atomicAdd(LDS ? &dpc[dst] : &pc[dst], 1);
+            atomicAdd(LDS ? &dpc[src] : &pc[src], -1);
+            applied = true;
+          }
+        }
       }
+    }
+    if (app) {  // the applied moves, for the tally update (any order)
+      const u64 am = __ballot(applied);
+      i32 ob = 0;
+      if (lane == 0 && am) ob = atomicAdd(napp, (i32)__popcll(am));
+      ob = __shfl(ob, 0);
+      if (applied) app[ob + __popcll(am & lanes_below(lane))] = id;
     }
   }
   if (LDS) {
@@ -2624,10 +2826,10 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
   k_match_huge<<<256, 1024, 0, s>>>(g, mate, dec, blkA, blkC, prop, req, cur);
 }
 
-void launch_sym_check(DGraph g, u64* table, i64 cap, i32* missing, hipStream_t s) {
+void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s) {
   k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
-  k_sym_insert<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, cap);
-  k_sym_check<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, cap, missing);
+  k_sym_insert<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, flags);
+  k_sym_check<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, flags);
 }
 
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total, i32* new_ids,
@@ -2727,70 +2929,6 @@ void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* h
     k_huge_bounds<<<grid_for(nhuge, 256, 1024), 256, 0, s>>>(g, nedges, huge, nhuge, hb);
 }
 
-template <typename PT>
-static void launch_gain_small_k(DGraph g, i64 nedges, const PT* part, const i32* part32, i32 k,
-                                const float* bg, const u8* src_big, float* gain, i32* target,
-                                float* cand, int phase, int unit, const GainLists& L,
-                                u64* cut_acc, hipStream_t s) {
-  // light rows: 8 lanes per vertex, 32 tallies of k ints per block
-  const size_t shm8 = (size_t)(2 * k + 32 * (k | 1)) * 4;
-  const unsigned gr8 = grid_for(g.n, 32, 1u << 16);
-  if (unit)
-    k_gain_group<8, true, PT><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                     target, cand, phase, nullptr, 0, GAIN_LIGHT, L.all_exact,
-                                                     cut_acc);
-  else
-    k_gain_group<8, false, PT><<<gr8, 256, shm8, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                      target, cand, phase, nullptr, 0,
-                                                      GAIN_LIGHT, L.all_exact, cut_acc);
-  if (L.nmid > 0) {
-    const size_t shm16 = (size_t)(2 * k + 16 * (k | 1)) * 4;
-    const unsigned gr16 = grid_for(L.nmid, 16, 1u << 16);
-    if (unit)
-      k_gain_group<16, true, PT><<<gr16, 256, shm16, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                          target, cand, phase, L.mid, L.nmid, 0, L.all_exact,
-                                                          cut_acc);
-    else
-      k_gain_group<16, false, PT><<<gr16, 256, shm16, s>>>(g, nedges, part, k, bg, src_big,
-                                                           gain, target, cand, phase, L.mid,
-                                                           L.nmid, 0, L.all_exact, cut_acc);
-  }
-  if (L.nmedium > 0) {
-    const size_t shm64 = (size_t)(2 * k + 4 * (k | 1)) * 4;
-    const unsigned gr64 = grid_for(L.nmedium, 4, 1u << 16);
-    if (unit)
-      k_gain_group<64, true, PT><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big, gain,
-                                                          target, cand, phase, L.medium,
-                                                          L.nmedium, 0, L.all_exact, cut_acc);
-    else
-      k_gain_group<64, false, PT><<<gr64, 256, shm64, s>>>(g, nedges, part, k, bg, src_big,
-                                                           gain, target, cand, phase, L.medium,
-                                                           L.nmedium, 0, L.all_exact, cut_acc);
-  }
-  if (L.nhuge > 0) {
-    const size_t shmc = (size_t)k * 4;
-    const unsigned grc = grid_for(L.nch, 1, 1u << 16);
-    if (unit)
-      k_gain_chunk<true, PT><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
-                                                        L.ch_end, L.nch, L.gtally, L.gabs,
-                                                        cut_acc);
-    else
-      k_gain_chunk<false, PT><<<grc, HUGE_BS, shmc, s>>>(g, part, k, L.huge, L.ch_row, L.ch_beg,
-                                                         L.ch_end, L.nch, L.gtally, L.gabs,
-                                                         cut_acc);
-    const size_t shmf = (size_t)(3 * k) * 4;
-    const unsigned grf = grid_for(L.nhuge, 1, 8192);
-    if (unit)
-      k_gain_huge_final<true><<<grf, 256, shmf, s>>>(g, nedges, part32, k, bg, src_big, gain,
-                                                     target, cand, phase, L.huge, L.nhuge,
-                                                     L.gtally, L.gabs);
-    else
-      k_gain_huge_final<false><<<grf, 256, shmf, s>>>(g, nedges, part32, k, bg, src_big, gain,
-                                                      target, cand, phase, L.huge, L.nhuge,
-                                                      L.gtally, L.gabs);
-  }
-}
-
 void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
                     i32* over, hipStream_t s) {
   k_row_abs_light<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, heavy_deg, over);
@@ -2798,16 +2936,45 @@ void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 n
     k_row_abs_heavy<<<grid_for(nheavy, 4, 8192), 256, 0, s>>>(g, nedges, heavy, nheavy, over);
 }
 
+template <typename PT>
+static void launch_gain_small_k(DGraph g, i64 nedges, const PT* part, const i32* part32, i32 k,
+                                const float* bg, const u8* src_big, float* gain, i32* target,
+                                float* cand, int phase, int unit, const GainLists& L,
+                                u32* ctr, u64* cut_acc, i32* Tout, hipStream_t s) {
+  const size_t shm = (size_t)(2 * k + 4 * 8 * (k | 1)) * 4;
+  k_fill<u32><<<1, 256, 0, s>>>(ctr, GAIN_CTRS * 32, 0u);
+  if (unit)
+    k_gain_fused<true, PT><<<GAIN_FUSED_BLOCKS, 256, shm, s>>>(g, nedges, part, k, bg, src_big,
+                                                               gain, target, cand, phase, L,
+                                                               ctr, cut_acc, Tout);
+  else
+    k_gain_fused<false, PT><<<GAIN_FUSED_BLOCKS, 256, shm, s>>>(g, nedges, part, k, bg, src_big,
+                                                                gain, target, cand, phase, L,
+                                                                ctr, cut_acc, Tout);
+  if (L.nhuge > 0) {
+    const size_t shmf = (size_t)(3 * k) * 4;
+    const unsigned grf = grid_for(L.nhuge, 1, 8192);
+    if (unit)
+      k_gain_huge_final<true><<<grf, 256, shmf, s>>>(g, nedges, part32, k, bg, src_big, gain,
+                                                     target, cand, phase, L.huge, L.nhuge,
+                                                     L.gtally, L.gabs, Tout);
+    else
+      k_gain_huge_final<false><<<grf, 256, shmf, s>>>(g, nedges, part32, k, bg, src_big, gain,
+                                                      target, cand, phase, L.huge, L.nhuge,
+                                                      L.gtally, L.gabs, Tout);
+  }
+}
+
 bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
-                 i32* gscratch, const GainLists& L, u64* cut_acc, hipStream_t s) {
+                 i32* gscratch, const GainLists& L, u64* cut_acc, i32* Tout, hipStream_t s) {
   if (k <= 512) {
     if (part8 && k <= PART8_MAXK)
       launch_gain_small_k<u8>(g, nedges, part8, part, k, bg, src_big, gain, target, cand, phase,
-                              unit, L, cut_acc, s);
+                              unit, L, L.ctr, cut_acc, Tout, s);
     else
       launch_gain_small_k<i32>(g, nedges, part, part, k, bg, src_big, gain, target, cand, phase,
-                               unit, L, cut_acc, s);
+                               unit, L, L.ctr, cut_acc, Tout, s);
     return cut_acc != nullptr;
   }
   // large k: one wave per vertex for every vertex
@@ -2899,13 +3066,34 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
 }
 
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
-                   const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, hipStream_t s) {
+                   const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, i32* app,
+                   i32* napp, hipStream_t s) {
+  if (app) k_fill<i32><<<1, 1, 0, s>>>(napp, 1, 0);
   if (k <= MAXK_LDS)
     k_commit<true><<<grid_for(M, 256 * 16, 2048), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
-                                                               mv_dst, part, part8, pc, k);
+                                                               mv_dst, part, part8, pc, k, app,
+                                                               napp);
   else
     k_commit<false><<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
-                                                               mv_dst, part, part8, pc, k);
+                                                               mv_dst, part, part8, pc, k, app,
+                                                               napp);
+}
+
+void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
+                         const u8* src_big, float* gain, i32* target, float* cand, int phase,
+                         u64* cut_acc, hipStream_t s) {
+  k_gain_tallies<<<grid_for(n, 16, 1u << 16), 256, (size_t)2 * k * 4, s>>>(
+      n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+}
+void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
+                         const i32* mv_v, const i32* mv_src, const i32* mv_dst, int unit,
+                         int skip_last, hipStream_t s) {
+  k_tally_update<<<grid_for(max_app, 4, 4096), 256, 0, s>>>(g, T, k, app, napp, mv_v, mv_src,
+                                                            mv_dst, unit, skip_last);
+}
+void launch_tally_last(DGraph g, i64 nedges, const i32* part, i32* T, i32 k, int unit,
+                       hipStream_t s) {
+  k_tally_last<<<1, 256, 0, s>>>(g, nedges, part, T, k, unit);
 }
 
 void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s) {
