@@ -294,7 +294,7 @@ constexpr i32 BLK_NONE = -2;
 
 // Row scans issue SCAN_U independent loads per lane before using any of them,
 // so a long row costs about len/(64*SCAN_U) memory round trips.
-constexpr int SCAN_U = 8;
+constexpr int SCAN_U = 16;
 
 // Device-scope (coherent) accesses to the matching state, for kernels in
 // which other waves update it while they run.
@@ -656,33 +656,52 @@ __device__ __forceinline__ i32 wave_scan_pending(const DGraph& g, i32 x, i32 lim
   return best;
 }
 
-// heaviest free (st < 0) neighbour of t by (weight, first position); -1 if none
-__device__ __forceinline__ i32 wave_best_free_st(const DGraph& g, i32 t, const i32* st,
-                                                 int lane) {
+// The 4 best free (st < 0) neighbours of t by (weight, first position), as
+// keys (w << 32 | ~position), best first, 0 where there are fewer.  Every other
+// free neighbour ranks below all four, and a taken vertex stays taken, so the
+// best free neighbour at any later time is the first still-free key here
+// (when one is left).
+constexpr int TU = 8;  // loads per lane per step (fewer registers than SCAN_U)
+__device__ __forceinline__ void wave_best_free_top4(const DGraph& g, i32 t, const i32* st,
+                                                    int lane, u64 (&out)[4]) {
   const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
-  u64 best = 0;
-  for (i64 base = s0; base < s1; base += WAVE * SCAN_U) {
-    i32 v[SCAN_U], w[SCAN_U], sv[SCAN_U];
+  u64 k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+  for (i64 base = s0; base < s1; base += WAVE * TU) {
+    i32 v[TU], w[TU], sv[TU];
 #pragma unroll
-    for (int j = 0; j < SCAN_U; j++) {
+    for (int j = 0; j < TU; j++) {
       const i64 i = base + j * WAVE + lane;
       v[j] = i < s1 ? g.adj[i] : -1;
       w[j] = i < s1 ? g.w[i] : -1;
     }
 #pragma unroll
-    for (int j = 0; j < SCAN_U; j++)
+    for (int j = 0; j < TU; j++)
       sv[j] = (v[j] >= 0 && v[j] < g.n && v[j] != t && w[j] >= 0) ? ld_st(st, v[j]) : 0;
 #pragma unroll
-    for (int j = 0; j < SCAN_U; j++) {
+    for (int j = 0; j < TU; j++) {
       if (sv[j] < 0) {
         const i64 i = base + j * WAVE + lane;
-        const u64 c = ((u64)(u32)w[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
-        best = c > best ? c : best;
+        u64 c = ((u64)(u32)w[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+        if (c > k3) {
+          k3 = c;
+          if (k3 > k2) { c = k2; k2 = k3; k3 = c; }
+          if (k2 > k1) { c = k1; k1 = k2; k2 = c; }
+          if (k1 > k0) { c = k0; k0 = k1; k1 = c; }
+        }
       }
     }
   }
-  best = wave_max_u64(best);
-  return best ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)best)] : -1;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const u64 m = wave_max_u64(k0);
+    out[r] = m;
+    if (m != 0 && k0 == m) {  // positions are distinct: one lane holds m
+      k0 = k1;
+      k1 = k2;
+      k2 = k3;
+      k3 = 0;
+    }
+  }
 }
 
 // Sub-group layout: CH_SG lanes per vertex, CH_VPW vertices per wave.  A row
@@ -726,6 +745,10 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
   // per-vertex state, uniform within a sub-group
   bool own = false, tshort = false, ushort_ = false, uknown = false;
   i32 t = -1, a = BLK_NONE, u = -1, b = BLK_NONE;
+  // long rows: lanes 0-3 of the sub-group hold the 4 best free candidates of
+  // the last full proposal scan (pci), ph = the entry now proposed (4: none)
+  i32 pci = -1;
+  int ph = 4;
   i64 ts0 = 0;
   i32 tv[CH_PER], tw[CH_PER], uv[CH_PER];
   for (u32 iter = 0; iter < (1u << 26); iter++) {
@@ -742,6 +765,8 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       u = own ? prop[t] : -1;
       b = own ? blkC[t] : BLK_NONE;
       uknown = false;
+      pci = -1;
+      ph = 4;
       ts0 = own ? g.xadj[t] : 0;
       const i64 ts1 = own ? g.xadj[t + 1] : 0;
       tshort = own && ts1 - ts0 <= CH_ROW;
@@ -842,16 +867,43 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       }
       need &= need - 1;
     }
+    // long rows: a taken proposal is replaced by the first still-free cached
+    // candidate; only when none is left is the row scanned again
+    {
+      const bool tryc = own && !tshort && u == BLK_UNKNOWN && ph < 3;
+      const bool mine = tryc && sl > ph && sl < 4 && pci >= 0;
+      const i32 sv = mine ? ld_st(st, pci) : 0;
+      const u64 fm = (__ballot(mine && sv < 0) >> (lane & ~(CH_SG - 1))) & ((1ull << CH_SG) - 1);
+      if (tryc) {
+        if (fm) {
+          const int r = __ffsll((unsigned long long)fm) - 1;
+          u = __shfl(pci, r, CH_SG);
+          s_u = __shfl(sv, r, CH_SG);
+          b = BLK_UNKNOWN;
+          uknown = false;
+          ph = r;
+        } else {
+          ph = 4;
+        }
+      }
+    }
     need = __ballot(own && !tshort && sl == 0 && u == BLK_UNKNOWN);
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
       c_pL++;
       c_arcsL += g.xadj[x + 1] - g.xadj[x];
-      const i32 r = wave_best_free_st(g, x, st, lane);
-      const i32 sr = r >= 0 ? ld_st(st, r) : ST_UNDEC;
+      u64 keys[4];
+      wave_best_free_top4(g, x, st, lane, keys);
+      const i64 xs0 = g.xadj[x];
+      const i32 r0 = keys[0] ? g.adj[xs0 + (i64)(0xFFFFFFFFu - (u32)keys[0])] : -1;
+      const i32 sr = r0 >= 0 ? ld_st(st, r0) : ST_UNDEC;
       if ((lane & ~(CH_SG - 1)) == l) {
-        u = r >= 0 ? r : PROP_NONE;
+        const int r = lane & (CH_SG - 1);
+        const u64 kk = r == 0 ? keys[0] : r == 1 ? keys[1] : r == 2 ? keys[2] : r == 3 ? keys[3] : 0;
+        pci = kk ? g.adj[xs0 + (i64)(0xFFFFFFFFu - (u32)kk)] : -1;
+        ph = 0;
+        u = r0 >= 0 ? r0 : PROP_NONE;
         s_u = sr;
         b = BLK_UNKNOWN;
         uknown = false;
