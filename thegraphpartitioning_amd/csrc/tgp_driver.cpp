@@ -499,19 +499,33 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
 }
 
 void Run::check_symmetric(Level& lv) {
-  i64 cap = 1;
-  while (cap < 2 * lv.m + 2) cap <<= 1;
-  DBuf<u64> table(&C.pool, cap);
-  DBuf<i32> wt(&C.pool, cap), flags(&C.pool, 3);
-  HIP_TRY(hipMemsetAsync(flags.p, 0, 3 * sizeof(i32), st()));
+  DBuf<i32> flags(&C.pool, 4);
+  DBuf<u64> cnt(&C.pool, 2);
+  HIP_TRY(hipMemsetAsync(flags.p, 0, 4 * sizeof(i32), st()));
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, 2 * sizeof(u64), st()));
   C.prof.begin(KID_MISC);
-  launch_sym_check(lv.dg(), table.p, wt.p, cap, flags.p, st());
-  C.prof.end(KID_MISC, 28.0 * lv.m);
-  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 3 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  launch_rows_strict(lv.dg(), flags.p, st());
+  C.prof.end(KID_MISC, 4.0 * lv.m);
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  const bool strict = ((const i32*)C.pinned)[3] == 0;
+  // strictly ascending rows hold no duplicates: half the arcs go into the
+  // table and the other half look their reverse up
+  i64 cap = 1;
+  while (cap < (strict ? lv.m + 2 : 2 * lv.m + 2)) cap <<= 1;
+  DBuf<u64> table(&C.pool, cap);
+  DBuf<i32> wt(&C.pool, cap);
+  C.prof.begin(KID_MISC);
+  if (strict) launch_sym_check_sorted(lv.dg(), table.p, wt.p, cap, flags.p, cnt.p, st());
+  else launch_sym_check(lv.dg(), table.p, wt.p, cap, flags.p, st());
+  C.prof.end(KID_MISC, (strict ? 14.0 : 28.0) * lv.m);
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned + 2, cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, st()));
   C.sync();
   const i32* f = (const i32*)C.pinned;
-  lv.symmetric = f[0] == 0;
-  lv.simple_ws = f[0] == 0 && f[1] == 0 && f[2] == 0;
+  const bool sym = f[0] == 0 && (!strict || C.pinned[2] == C.pinned[3]);
+  lv.symmetric = sym;
+  lv.simple_ws = sym && f[1] == 0 && f[2] == 0;
 }
 
 void Run::coarsen() {
@@ -815,13 +829,14 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     // integer tallies: built by the first gain launch after any bulk change of
     // part, then updated from the applied moves of each commit.
     const bool tmode = k <= 512 && ax.all_exact && lv.simple_ws &&
-                       (double)n * k * 4 <= 8e9 && env_i64("TGP_KEEP_TALLIES", 1) != 0;
+                       (double)n * tally_row_stride(k) * 4 <= 8e9 &&
+                       env_i64("TGP_KEEP_TALLIES", 1) != 0;
     DBuf<i32> T, app, napp;
     bool T_valid = false;
     const bool last_full = nedges == lv.m;
     const bool last_partial = !last_full && nedges > ax.last_begin;
     if (tmode) {
-      T = DBuf<i32>(&C.pool, n * k);
+      T = DBuf<i32>(&C.pool, n * tally_row_stride(k));
       app = DBuf<i32>(&C.pool, n);
       napp = DBuf<i32>(&C.pool, 1);
     }

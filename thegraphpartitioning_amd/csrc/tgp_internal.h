@@ -101,6 +101,13 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 // flags[0] |= some arc s->x (s != x, x < n) has no reverse; flags[1] |= some
 // (s, x) appears twice in s's row; flags[2] |= some reverse has another weight.
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s);
+// flags[3] |= some row is not strictly ascending
+void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
+// the same checks for strictly ascending rows with half the table (cap >= m,
+// pow2): flags[0], flags[2] as above; cnt[0] = arcs up, cnt[1] = arcs down
+// whose reverse was found (symmetric iff flags[0] == 0 and they agree)
+void launch_sym_check_sorted(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, u64* cnt,
+                             hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);
 // Reference contraction (gpu_part.cu:145-242 semantics): padded rows of 64.
@@ -204,7 +211,10 @@ constexpr i32 PART8_MAXK = 255;
 bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
                  i32* gscratch, const GainLists& L, u64* cut_acc, i32* Tout, hipStream_t s);
-// Gains (and the cut into cut_acc slots) from kept tallies T (n * k ints).
+// Kept tallies: row v of T holds k ints at v * tally_stride(k) (k rounded up
+// to 4, so rows are 16-byte aligned); Tout of launch_gain uses the same layout.
+inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
+// Gains (and the cut into cut_acc slots) from kept tallies T.
 void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
                          const u8* src_big, float* gain, i32* target, float* cand, int phase,
                          u64* cut_acc, hipStream_t s);

@@ -23,6 +23,8 @@ namespace {
 constexpr int WAVE = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+// row stride of a kept tally array: k rounded up to 4 ints (16-byte rows)
+__host__ __device__ __forceinline__ i64 tally_stride(i32 k) { return (i64)((k + 3) & ~3); }
 __device__ __forceinline__ u64 lanes_below(int lane) { return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane)); }
 
 template <typename T>
@@ -57,6 +59,19 @@ __device__ __forceinline__ void block_add_u64(u64 v, u64* acc) {
       atomicAdd((unsigned long long*)&acc[(blockIdx.x & (ACC_SLOTS - 1)) * ACC_STRIDE],
                 (unsigned long long)s);
   }
+}
+
+// list[(*cnt)++] = x for every active lane with pred, in lane order within the
+// wave, with one atomic per wave (callable from divergent code).
+__device__ __forceinline__ void wave_append(bool pred, i32 x, i32* list, i32* cnt) {
+  const u64 m = __ballot(pred);
+  if (!m) return;
+  const int lane = lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  i32 base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (i32)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1))] = x;
 }
 
 __device__ __forceinline__ u64 wave_or_u64(u64 v) {
@@ -160,7 +175,7 @@ __global__ __launch_bounds__(256) void k_level_census(DGraph g, i64 nedges, i64 
   for (i64 v = gtid(); v < g.n; v += gstride()) {
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    if (s1 - s0 > heavy_deg) heavy_list[atomicAdd(heavy_count, 1)] = (i32)v;
+    wave_append(s1 - s0 > heavy_deg, (i32)v, heavy_list, heavy_count);
   }
   bool nu = false;
   for (i64 i = gtid(); i < g.m; i += gstride()) nu |= (g.w[i] != 1);
@@ -1100,6 +1115,82 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, c
   if (__ballot(wmis) && lane == 0) atomicOr(&flags[2], 1);
 }
 
+// flags[3] |= some row is not strictly ascending (so it may hold duplicates)
+__global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
+  bool bad = false;
+  const int lane = lane_id();
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + 1 + lane; i < s1; i += WAVE) bad |= g.adj[i] <= g.adj[i - 1];
+  }
+  if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
+}
+
+// Strictly ascending rows: each arc v->u with u > v is inserted once with its
+// weight (counted in cnt[0]); each arc v->u with u < v looks up (u, v) and
+// compares weights (counted in cnt[1] when found).  Symmetric iff nothing is
+// missing and both counts agree, since no key repeats.
+__global__ __launch_bounds__(256) void k_sym_insert_up(DGraph g, u64* table, i32* wt, i64 cap,
+                                                       u64* cnt) {
+  const u64 mask = (u64)cap - 1;
+  const int lane = lane_id();
+  u64 c = 0;
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      if (u <= v || u >= g.n) continue;
+      const u64 key = ((u64)v << 32) | (u64)(u32)u;
+      u64 h = hash64(key) & mask;
+      while (true) {
+        const u64 old = atomicCAS((unsigned long long*)&table[h], ~0ull, (unsigned long long)key);
+        if (old == ~0ull) {
+          wt[h] = g.w[i];
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+      c++;
+    }
+  }
+  c = wave_sum(c);
+  if (lane == 0 && c) atomicAdd((unsigned long long*)&cnt[0], (unsigned long long)c);
+}
+__global__ __launch_bounds__(256) void k_sym_check_down(DGraph g, const u64* table,
+                                                        const i32* wt, i64 cap, i32* flags,
+                                                        u64* cnt) {
+  const u64 mask = (u64)cap - 1;
+  const int lane = lane_id();
+  bool miss = false, wmis = false;
+  u64 c = 0;
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      if (u < 0 || u >= v) continue;
+      const u64 key = ((u64)(u32)u << 32) | (u64)v;
+      u64 h = hash64(key) & mask;
+      while (true) {
+        const u64 cur = table[h];
+        if (cur == key) {
+          wmis |= wt[h] != g.w[i];
+          c++;
+          break;
+        }
+        if (cur == ~0ull) {
+          miss = true;
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+    }
+  }
+  c = wave_sum(c);
+  if (lane == 0 && c) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)c);
+  if (__ballot(miss) && lane == 0) atomicOr(&flags[0], 1);
+  if (__ballot(wmis) && lane == 0) atomicOr(&flags[2], 1);
+}
+
 // ------------------------------------------------------------------ relabel --
 // gpu_part.cu:109-142 under SEQ: roots (unmatched, or the smaller end of a
 // pair) get consecutive coarse ids in ascending order.
@@ -1230,7 +1321,7 @@ __global__ void k_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz
       lng[e] = (i32)c;
       nchk[e] = (d + REP_CHUNK_ENTRIES - 1) / REP_CHUNK_ENTRIES;
     }
-    else if (d > REP_SMALL) med[atomicAdd(nmed, 1)] = (i32)c;
+    wave_append(!is_long && d > REP_SMALL, (i32)c, med, nmed);
   }
 }
 
@@ -1819,7 +1910,7 @@ __global__ __launch_bounds__(256) void k_gain_huge_final(DGraph g, i64 nedges, c
     }
     __syncthreads();
     for (int q = threadIdx.x; q < k; q += blockDim.x) {
-      if (Tout) Tout[v * k + q] = gt[q];
+      if (Tout) Tout[v * tally_stride(k) + q] = gt[q];
       gt[q] = 0;
     }
     if (threadIdx.x == 0) gabs[h] = 0;
@@ -1930,7 +2021,7 @@ __device__ __forceinline__ void gain_vertex(const DGraph& g, i64 nedges, const P
     const i32 tq = tally[q];
     const float cq = exact_int ? (float)tq : ftally[q];
     tally[q] = 0;
-    if (Tout) Tout[v * k + q] = q == cur ? own : tq;  // (integer tallies: exact_int holds)
+    if (Tout) Tout[v * tally_stride(k) + q] = q == cur ? own : tq;  // (exact_int holds)
     if (q == cur) continue;
     const float eg = cq - ccur;
     const float gg = eg + bgr[q];
@@ -2098,25 +2189,33 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, co
   constexpr int G = 16;
   const int gl = threadIdx.x & (G - 1);
   const i64 ngroups = (i64)gridDim.x * (blockDim.x / G);
+  const i64 ts = tally_stride(k);
   u64 cutv = 0;
   for (i64 v = (i64)blockIdx.x * (blockDim.x / G) + threadIdx.x / G; v < n; v += ngroups) {
     const i32 cur = part[v];
     if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
-    const i32* Tv = T + v * k;
+    const i32* Tv = T + v * ts;
+    const int4* T4 = (const int4*)Tv;  // rows are 16-byte aligned (stride multiple of 4)
     const float ccur = (float)Tv[cur];
     const float* bgr = sbgt + (src_big[cur] ? k : 0);
     float bgv = -INFINITY;
     i32 bq = INT_MAX;
     i64 other = 0;
-    for (int q = gl; q < k; q += G) {
-      const i32 tq = Tv[q];
-      if (q == cur) continue;
-      other += tq;
-      const float eg = (float)tq - ccur;
-      const float gg = eg + bgr[q];
-      if (gg > bgv || (gg == bgv && q < bq)) {
-        bgv = gg;
-        bq = q;
+    for (int c = gl; c < (int)(ts >> 2); c += G) {
+      const int4 t4 = T4[c];
+      const i32 tv4[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const i32 q = 4 * c + j;
+        if (q >= k || q == cur) continue;
+        const i32 tq = tv4[j];
+        other += tq;
+        const float eg = (float)tq - ccur;
+        const float gg = eg + bgr[q];
+        if (gg > bgv || (gg == bgv && q < bq)) {
+          bgv = gg;
+          bq = q;
+        }
       }
     }
 #pragma unroll
@@ -2160,6 +2259,7 @@ __global__ __launch_bounds__(256) void k_tally_update(DGraph g, i32* T, i32 k, c
                                                       int unit, int skip_last) {
   const int lane = lane_id();
   const i64 cnt = *napp;
+  const i64 ts = tally_stride(k);
   for (i64 e = gwave(); e < cnt; e += nwaves()) {
     const i32 id = app[e];
     const i32 v = mv_v[id], src = mv_src[id], dst = mv_dst[id];
@@ -2168,8 +2268,8 @@ __global__ __launch_bounds__(256) void k_tally_update(DGraph g, i32* T, i32 k, c
       const i32 u = g.adj[i];
       if (u < 0 || u >= g.n || (skip_last && u == g.n - 1)) continue;
       const i32 w = unit ? 1 : g.w[i];
-      atomicAdd(&T[(i64)u * k + src], -w);
-      atomicAdd(&T[(i64)u * k + dst], w);
+      atomicAdd(&T[(i64)u * ts + src], -w);
+      atomicAdd(&T[(i64)u * ts + dst], w);
     }
   }
 }
@@ -2178,7 +2278,7 @@ __global__ __launch_bounds__(256) void k_tally_update(DGraph g, i32* T, i32 k, c
 __global__ __launch_bounds__(256) void k_tally_last(DGraph g, i64 nedges, const i32* part,
                                                     i32* T, i32 k, int unit) {
   const i64 v = g.n - 1;
-  i32* Tv = T + v * k;
+  i32* Tv = T + v * tally_stride(k);
   for (int q = threadIdx.x; q < k; q += blockDim.x) Tv[q] = 0;
   __syncthreads();
   for (i64 i = g.xadj[v] + threadIdx.x; i < nedges; i += blockDim.x) {
@@ -2225,9 +2325,10 @@ __global__ void k_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy
     const i32 v = heavy[e];
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    if (s1 - s0 > GAIN_HUGE) huge[atomicAdd(nhuge, 1)] = v;
-    else if (s1 - s0 > GAIN_MID) medium[atomicAdd(nmedium, 1)] = v;
-    else mid[atomicAdd(nmid, 1)] = v;
+    const i64 d = s1 - s0;
+    wave_append(d > GAIN_HUGE, v, huge, nhuge);
+    wave_append(d > GAIN_MID && d <= GAIN_HUGE, v, medium, nmedium);
+    wave_append(d <= GAIN_MID, v, mid, nmid);
   }
 }
 
@@ -2238,46 +2339,50 @@ __device__ __forceinline__ bool move_ok(i32 src, i32 dst, i32 k) {
 
 // cand[u] is u's gain when u is a candidate and NaN otherwise, so one load
 // answers both questions and every comparison with a non-candidate is false.
-// Light rows (<= heavy_deg arcs): one thread per vertex (most vertices are not
-// candidates and stop at once), eight loads in flight per step.
-__global__ __launch_bounds__(256) void k_token_thread(DGraph g, i64 nedges, const float* cand,
-                                                      const i32* part, const i32* target, i32 k,
-                                                      i64* flag, i64 heavy_deg) {
-  constexpr int TU = 8;
-  for (i64 v = gtid(); v < g.n; v += gstride()) {
-    const i64 s0 = g.xadj[v];
-    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    if (s1 - s0 > heavy_deg) continue;
-    const float gv = cand[v];
-    if (!(gv == gv)) {
-      flag[v] = 0;
-      continue;
-    }
-    const float gv10 = gv + 10.0f;
-    bool lose = false;
-    for (i64 base = s0; base < s1 && !lose; base += TU) {
-      i32 uu[TU];
+// One launch serves both row classes at once: the first `bh` blocks (issued
+// first, they run longest) take the heavy rows (one wave per vertex, 64
+// neighbours per step: the first losing neighbour usually comes early, so
+// wider steps only add traffic), the others the light rows (<= heavy_deg arcs,
+// one thread per vertex, 8 neighbours per step; most vertices are not
+// candidates and stop at once).
+__global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float* cand,
+                                               const i32* part, const i32* target, i32 k,
+                                               i64* flag, i64 heavy_deg, const i32* list,
+                                               i64 cnt, unsigned bh) {
+  if (blockIdx.x >= bh) {
+    constexpr int TU = 8;
+    const i64 stride = (i64)(gridDim.x - bh) * blockDim.x;
+    for (i64 v = (i64)(blockIdx.x - bh) * blockDim.x + threadIdx.x; v < g.n; v += stride) {
+      const i64 s0 = g.xadj[v];
+      const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+      if (s1 - s0 > heavy_deg) continue;
+      const float gv = cand[v];
+      if (!(gv == gv)) {
+        flag[v] = 0;
+        continue;
+      }
+      const float gv10 = gv + 10.0f;
+      bool lose = false;
+      for (i64 base = s0; base < s1 && !lose; base += TU) {
+        i32 uu[TU];
 #pragma unroll
-      for (int j = 0; j < TU; j++) uu[j] = base + j < s1 ? g.adj[base + j] : INT_MAX;
+        for (int j = 0; j < TU; j++) uu[j] = base + j < s1 ? g.adj[base + j] : INT_MAX;
 #pragma unroll
-      for (int j = 0; j < TU; j++) {
-        if (uu[j] < g.n) {
-          const float gu = cand[uu[j]];
-          lose |= gu > gv10 || (gu == gv && uu[j] < v);
+        for (int j = 0; j < TU; j++) {
+          if (uu[j] < g.n) {
+            const float gu = cand[uu[j]];
+            lose |= gu > gv10 || (gu == gv && uu[j] < v);
+          }
         }
       }
+      flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
     }
-    flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
+    return;
   }
-}
-
-// Rows longer than heavy_deg: one wave per vertex, 64 arcs per step (the first
-// losing neighbour usually comes early, so wider steps only add traffic).
-__global__ __launch_bounds__(256) void k_token_heavy(DGraph g, i64 nedges, const float* cand,
-                                                     const i32* part, const i32* target, i32 k,
-                                                     i64* flag, const i32* list, i64 cnt) {
   const int lane = lane_id();
-  for (i64 e = gwave(); e < cnt; e += nwaves()) {
+  const i64 w0 = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const i64 nw = ((i64)bh * blockDim.x) / WAVE;
+  for (i64 e = w0; e < cnt; e += nw) {
     const i32 v = list[e];
     const float gv = cand[v];
     if (!(gv == gv)) {
@@ -2878,6 +2983,15 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
   k_match_huge<<<256, 1024, 0, s>>>(g, mate, dec, blkA, blkC, prop, req, cur);
 }
 
+void launch_rows_strict(DGraph g, i32* flags, hipStream_t s) {
+  k_rows_strict<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags);
+}
+void launch_sym_check_sorted(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, u64* cnt,
+                             hipStream_t s) {
+  k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
+  k_sym_insert_up<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, cnt);
+  k_sym_check_down<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, flags, cnt);
+}
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s) {
   k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
   k_sym_insert<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, flags);
@@ -3058,12 +3172,10 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, 
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
                   hipStream_t s) {
-  k_token_thread<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, cand, part, target, k,
-                                                              flag, heavy_deg);
-  if (heavy_count > 0)
-    k_token_heavy<<<grid_for(heavy_count, 4, 8192), 256, 0, s>>>(g, nedges, cand, part, target,
-                                                                 k, flag, heavy_list,
-                                                                 heavy_count);
+  const unsigned bl = grid_for(g.n, 256, 1u << 16);
+  const unsigned bh = heavy_count > 0 ? grid_for(heavy_count, 4, 8192) : 0;
+  k_token<<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg, heavy_list,
+                                  heavy_count, bh);
 }
 
 void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part, const i32* target,
@@ -3134,7 +3246,7 @@ void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, con
 void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
                          const u8* src_big, float* gain, i32* target, float* cand, int phase,
                          u64* cut_acc, hipStream_t s) {
-  k_gain_tallies<<<grid_for(n, 16, 1u << 16), 256, (size_t)2 * k * 4, s>>>(
+  k_gain_tallies<<<grid_for(n, 16, 4096), 256, (size_t)2 * k * 4, s>>>(
       n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
 }
 void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
