@@ -166,6 +166,26 @@ def test_round_matching_on_symmetric_levels(oracle, monkeypatch):
     assert_same(gp, op, "round-based matching")
 
 
+@pytest.mark.parametrize("shape", [("8", "8"), ("16", "8"), ("32", "8")])
+def test_chained_matching_subgroup_widths(oracle, monkeypatch, shape):
+    """Every compiled lanes-per-vertex width of the chained kernel: rows of up
+    to 64, 128 or 256 arcs held in registers, longer rows scanned by the wave."""
+    monkeypatch.setenv("TGP_CHAIN_SG", shape[0])
+    monkeypatch.setenv("TGP_CHAIN_PER", shape[1])
+    g = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 6)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 32, repair=True, floor=100)
+    assert_same(gp, op, f"chained matching sg={shape}")
+
+
+def test_gather_gains_without_kept_tallies(oracle, monkeypatch):
+    """TGP_KEEP_TALLIES=0: gains re-gathered from every arc on each pass, on
+    levels that would otherwise keep per-vertex part tallies."""
+    monkeypatch.setenv("TGP_KEEP_TALLIES", "0")
+    g = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 7)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 64, repair=True, floor=100)
+    assert_same(gp, op, "gather gains")
+
+
 def cut_from_partition(g, part):
     """Edge cut recounted on the host from the final partition (every row whole)."""
     src = np.repeat(np.arange(g.n, dtype=np.int32), np.diff(g.xadj))

@@ -383,9 +383,13 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   static const bool trace_match = env_i64("TGP_MATCH_TRACE", 0) != 0;
   // Tuning and test knobs, read at every call: TGP_MATCH_CHAIN=0 selects the
   // round-based kernels on symmetric levels; TGP_CHAIN_BLOCKS sets the chained
-  // kernel's grid; TGP_CHAIN_BUDGET_US bounds one chained launch (100 ms).
+  // kernel's grid, TGP_CHAIN_SG and TGP_CHAIN_PER its lanes per vertex and
+  // row entries per lane;
+  // TGP_CHAIN_BUDGET_US bounds one chained launch (100 ms).
   const bool MATCH_CHAIN = env_i64("TGP_MATCH_CHAIN", 1) != 0;
   const int CHAIN_BLOCKS = (int)std::max<i64>(1, env_i64("TGP_CHAIN_BLOCKS", 512));
+  const int CHAIN_SG = (int)env_i64("TGP_CHAIN_SG", 32);
+  const int CHAIN_PER = (int)env_i64("TGP_CHAIN_PER", 8);
   const i64 chain_budget =
       std::max<i64>(1, C.clock_khz * env_i64("TGP_CHAIN_BUDGET_US", 100000) / 1000);
   const auto t_start = std::chrono::steady_clock::now();
@@ -431,7 +435,7 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     while (stuck < 3) {
       C.prof.begin(KID_CHAIN);
       launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, chain_budget,
-                         CHAIN_BLOCKS, cstats.p, st());
+                         CHAIN_BLOCKS, CHAIN_SG, CHAIN_PER, cstats.p, st());
       C.prof.end(KID_CHAIN, round == 0 ? match_bytes : 0.0);
       round++;
       launch_first_undecided(stv.p, n, lo.p, st());

@@ -724,11 +724,7 @@ __device__ __forceinline__ void wave_best_free_top4(const DGraph& g, i32 t, cons
 // lane), so a rescan of it is one batch of state loads and all sub-groups of
 // a wave rescan at once.  Longer rows are scanned by the whole wave, one row
 // at a time.
-constexpr int CH_SG = 8;
-constexpr int CH_VPW = WAVE / CH_SG;
-constexpr int CH_PER = 8;
-constexpr int CH_ROW = CH_SG * CH_PER;
-
+template <int CH_SG>
 __device__ __forceinline__ i32 sg_max_i32(i32 v) {
 #pragma unroll
   for (int o = CH_SG / 2; o > 0; o >>= 1) {
@@ -737,6 +733,7 @@ __device__ __forceinline__ i32 sg_max_i32(i32 v) {
   }
   return v;
 }
+template <int CH_SG>
 __device__ __forceinline__ bool sg_any(bool f) {
   i32 v = f;
 #pragma unroll
@@ -744,9 +741,13 @@ __device__ __forceinline__ bool sg_any(bool f) {
   return v != 0;
 }
 
+template <int CH_SG, int CH_PER>
 __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC,
                                                      i32* prop, const i64* lo_io, u32* ticket,
                                                      i64 budget, u64* stats) {
+  constexpr int CH_VPW = WAVE / CH_SG;
+  constexpr int CH_ROW = CH_SG * CH_PER;
+  constexpr u64 SG_MASK = CH_SG == 64 ? ~0ull : (1ull << (CH_SG % 64)) - 1;
   // stats (optional): per-kind event counts, summed over the grid
   u64 c_iter = 0, c_sleep = 0, c_apS = 0, c_aL = 0, c_pL = 0, c_fetch = 0, c_cS = 0, c_cL = 0,
       c_commit = 0, c_arcsL = 0;
@@ -843,8 +844,8 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
           }
         }
       }
-      bestA = sg_max_i32(bestA);
-      tk = sg_any(tk);
+      bestA = sg_max_i32<CH_SG>(bestA);
+      tk = sg_any<CH_SG>(tk);
 #pragma unroll
       for (int o = CH_SG / 2; o > 0; o >>= 1) {
         const u64 ok = __shfl_xor(key, o, CH_SG);
@@ -888,7 +889,7 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
       const bool tryc = own && !tshort && u == BLK_UNKNOWN && ph < 3;
       const bool mine = tryc && sl > ph && sl < 4 && pci >= 0;
       const i32 sv = mine ? ld_st(st, pci) : 0;
-      const u64 fm = (__ballot(mine && sv < 0) >> (lane & ~(CH_SG - 1))) & ((1ull << CH_SG) - 1);
+      const u64 fm = (__ballot(mine && sv < 0) >> (lane & ~(CH_SG - 1))) & SG_MASK;
       if (tryc) {
         if (fm) {
           const int r = __ffsll((unsigned long long)fm) - 1;
@@ -967,8 +968,8 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
         if (sv[j] == ST_UNDEC) bestC = uv[j] > bestC ? uv[j] : bestC;
         tk |= sv[j] == u;
       }
-      bestC = sg_max_i32(bestC);
-      tk = sg_any(tk);
+      bestC = sg_max_i32<CH_SG>(bestC);
+      tk = sg_any<CH_SG>(tk);
       if (needC && ushort_) {
         if (tk) {
           u = BLK_UNKNOWN;
@@ -2958,8 +2959,20 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 }
 
 void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
-                        u32* ticket, i64 budget, int blocks, u64* stats, hipStream_t s) {
-  k_match_chain<<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
+                        u32* ticket, i64 budget, int blocks, int sg, int per, u64* stats,
+                        hipStream_t s) {
+  // sub-group width (lanes per vertex) and register entries per lane: rows
+  // of up to sg * per arcs are held in registers
+#define TGP_CHAIN_CASE(SG, PER)                                                               \
+  if (sg == SG && per == PER) {                                                               \
+    k_match_chain<SG, PER><<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, \
+                                                  stats);                                     \
+    return;                                                                                   \
+  }
+  TGP_CHAIN_CASE(16, 8)
+  TGP_CHAIN_CASE(32, 8)
+#undef TGP_CHAIN_CASE
+  k_match_chain<8, 8><<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
 }
 
 void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s) {
