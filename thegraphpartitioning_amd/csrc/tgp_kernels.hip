@@ -2461,12 +2461,11 @@ __global__ void k_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int pha
 // gpu_part.cu:472-506: the exact compare-exchange sequence on (gain, slot)
 // pairs.  The element at the higher index of each pair compares itself with
 // the lower one: swap = (gain[hi] < gain[lo]) ^ ((hi & kk) == 0); pairs with
-// hi >= len are skipped.  Stages with stride < 2048 run inside LDS tiles.
-// Tiles of BT_TILE elements are sorted and merged in LDS (64 KiB: keys and
-// slots); strides of BT_TILE and more run as global passes that apply up to
-// three consecutive strides at once.
-constexpr int BT_TILE = 8192;
-constexpr int BT_THREADS = 1024;
+// hi >= len are skipped.  Strides below the tile size (4096) run inside
+// tiles held in registers (k_bt_tile); larger strides run as global passes
+// that apply up to four consecutive strides at once (k_bt_global).
+constexpr int BT_TB = 12;
+constexpr int BT_SMAX = 4;
 
 __global__ void k_sort_init(const float* mv_gain, i64 M, i64 len, float* key, i32* idx) {
   for (i64 i = gtid(); i < len; i += gstride()) {
@@ -2480,151 +2479,190 @@ __global__ void k_sort_init(const float* mv_gain, i64 M, i64 len, float* key, i3
   }
 }
 
-// One compare-exchange step with stride j (< BT_TILE) over a tile in LDS: the
-// pair (lo, lo + j) is exchanged when (key[hi] < key[lo]) differs from `up`,
-// and skipped when hi lies past len (gpu_part.cu:472-483).
-__device__ __forceinline__ void bt_stage_lds(float* sk, i32* si, i64 base, i64 len, i64 kk,
-                                             int j) {
-  for (int t = threadIdx.x; t < BT_TILE / 2; t += BT_THREADS) {
-    const int lo = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-    const int hi = lo + j;
-    if (base + hi < len) {
-      const bool up = ((base + hi) & kk) == 0;
-      const float a = sk[hi], b = sk[lo];
-      if ((a < b) ^ up) {
-        sk[hi] = b;
-        sk[lo] = a;
-        const i32 tt = si[hi];
-        si[hi] = si[lo];
-        si[lo] = tt;
-      }
-    }
-  }
+// value of the neighbouring lane (lane ^ 1), through DPP quad_perm [1,0,3,2]
+__device__ __forceinline__ i32 lane_xor1_i(i32 v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_xor1_f(float v) {
+  return __int_as_float(lane_xor1_i(__float_as_int(v)));
 }
 
-// In-tile stages keep 8 consecutive elements per thread in registers:
-// strides 1, 2 and 4 are exchanges between a thread's own registers, strides 8
-// to 256 are exchanges with the lane j/8 away (both lanes compute the same
-// decision and each keeps its side), and strides of 512 and more go through
-// LDS.  Every step is the same compare-exchange as bt_stage_lds.
-constexpr int BT_PER = BT_TILE / BT_THREADS;  // 8
+// In-tile stages of the same network with 16 elements per thread held in
+// registers under three layouts.  Layout R: register r of thread t holds tile
+// position (t & (2^R - 1)) | r << R | (t >> R) << (R + 4), so the strides
+// 2^R .. 2^(R+3) are exchanges between a thread's own registers.  A tile of
+// 2^TB positions uses R = TB - 4 (A), R = TB - 8 (B) and R = 0 (C); with
+// TB = 13 the one stride left (16) is an exchange between neighbouring lanes.
+// Layouts change through LDS at swizzled addresses (p ^ ((p >> 4) & 63)),
+// which keeps every layout's 32-bit accesses free of bank conflicts.
+// Everything about a merge is a template argument: KB = log2 of the merge
+// width when it is narrower than the tile (the direction then follows bit KB
+// of the position), KB = 0 for a merge at least as wide as the tile (one
+// direction for the whole tile, from its base); FULL = the tile lies wholly
+// below len, so no pair is skipped.
+template <int R>
+__device__ __forceinline__ int bt_pos(int t, int r) {
+  return (t & ((1 << R) - 1)) | (r << R) | ((t >> R) << (R + 4));
+}
+__device__ __forceinline__ int bt_swz(int p) { return p ^ ((p >> 4) & 63); }
 
-template <int J>
-__device__ __forceinline__ void bt_reg_stage(float (&e)[BT_PER], i32 (&x)[BT_PER], i64 p0, i64 len,
-                                             i64 kk) {
+template <int R, int JB, int KB, bool FULL, int UPC>
+__device__ __forceinline__ void bt_tile_stage(float (&e)[16], i32 (&x)[16], int t, int lim,
+                                              bool tile_up) {
+  constexpr int J = 1 << JB;
 #pragma unroll
-  for (int r = 0; r < BT_PER; r++) {
+  for (int r = 0; r < 16; r++) {
     if (r & J) continue;
     const int rh = r | J;
-    if (p0 + rh < len) {
-      const bool up = ((p0 + rh) & kk) == 0;
-      if ((e[rh] < e[r]) ^ up) {
-        const float tf = e[r];
-        e[r] = e[rh];
-        e[rh] = tf;
-        const i32 ti = x[r];
-        x[r] = x[rh];
-        x[rh] = ti;
-      }
-    }
+    const int hi = bt_pos<R>(t, rh);
+    const bool up = UPC >= 0 ? UPC != 0 : KB == 0 ? tile_up : ((hi >> KB) & 1) == 0;
+    const float a = e[r], b = e[rh];
+    const i32 ia = x[r], ib = x[rh];
+    // branch-free exchange (selects, not a divergent branch)
+    const bool sw = (FULL || hi < lim) && ((b < a) != up);
+    e[r] = sw ? b : a;
+    e[rh] = sw ? a : b;
+    x[r] = sw ? ib : ia;
+    x[rh] = sw ? ia : ib;
   }
 }
 
-__device__ __forceinline__ void bt_shfl_stage(float (&e)[BT_PER], i32 (&x)[BT_PER], i64 p0,
-                                              int lane, int d, i64 len, i64 kk) {
-  const i64 j = (i64)d * BT_PER;
-  const bool lo = (lane & d) == 0;
+// strides 2^BTOP down to 2^BBOT, all within R .. R + 3, in layout R
+template <int R, int BTOP, int BBOT, int KB, bool FULL, int UPC>
+__device__ __forceinline__ void bt_tile_stages(float (&e)[16], i32 (&x)[16], int t, int lim,
+                                               bool tile_up) {
+  if constexpr (BTOP >= BBOT) {
+    bt_tile_stage<R, BTOP - R, KB, FULL, UPC>(e, x, t, lim, tile_up);
+    bt_tile_stages<R, BTOP - 1, BBOT, KB, FULL, UPC>(e, x, t, lim, tile_up);
+  }
+}
+
+template <int R, int RN>
+__device__ __forceinline__ void bt_tile_relayout(float (&e)[16], i32 (&x)[16], float* sk, i32* si,
+                                                 int t) {
+  __syncthreads();
 #pragma unroll
-  for (int r = 0; r < BT_PER; r++) {
-    const float o = __shfl_xor(e[r], d);
-    const i32 oi = __shfl_xor(x[r], d);
-    const i64 ph = lo ? p0 + r + j : p0 + r;
-    if (ph < len) {
-      const bool up = (ph & kk) == 0;
-      const float a = lo ? o : e[r];  // key at the higher position
-      const float b = lo ? e[r] : o;  // key at the lower position
-      if ((a < b) ^ up) {
-        e[r] = o;
-        x[r] = oi;
-      }
-    }
-  }
-}
-
-// strides jtop, jtop/2, ..., 1 (jtop <= 256) on the registers
-__device__ __forceinline__ void bt_regs_down(float (&e)[BT_PER], i32 (&x)[BT_PER], i64 p0,
-                                             int lane, i64 len, i64 kk, int jtop) {
-  for (int jj = jtop; jj >= BT_PER; jj >>= 1) bt_shfl_stage(e, x, p0, lane, jj / BT_PER, len, kk);
-  if (jtop >= 4) bt_reg_stage<4>(e, x, p0, len, kk);
-  if (jtop >= 2) bt_reg_stage<2>(e, x, p0, len, kk);
-  bt_reg_stage<1>(e, x, p0, len, kk);
-}
-
-__global__ __launch_bounds__(BT_THREADS) void k_bt_local(float* key, i32* idx, i64 len,
-                                                         i64 kk_fixed) {
-  __shared__ __align__(16) float sk[BT_TILE];
-  __shared__ __align__(16) i32 si[BT_TILE];
-  const i64 base = (i64)blockIdx.x * BT_TILE;
-  for (int q = threadIdx.x; q < BT_TILE; q += BT_THREADS) {
-    const i64 gi = base + q;
-    if (gi < len) {
-      sk[q] = key[gi];
-      si[q] = idx[gi];
-    }
+  for (int r = 0; r < 16; r++) {
+    const int q = bt_swz(bt_pos<R>(t, r));
+    sk[q] = e[r];
+    si[q] = x[r];
   }
   __syncthreads();
-  const int lane = lane_id();
-  const int o = threadIdx.x * BT_PER;
-  const i64 p0 = base + o;
-  float e[BT_PER];
-  i32 x[BT_PER];
 #pragma unroll
-  for (int r = 0; r < BT_PER; r++) {
-    e[r] = sk[o + r];
-    x[r] = si[o + r];
+  for (int r = 0; r < 16; r++) {
+    const int q = bt_swz(bt_pos<RN>(t, r));
+    e[r] = sk[q];
+    x[r] = si[q];
   }
-  // strides jtop .. 512 of merge kk through LDS, then the rest in registers
-  auto merge = [&](i64 kk, int jtop) {
-    if (jtop >= 2 * 256) {
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < BT_PER; r++) {
-        sk[o + r] = e[r];
-        si[o + r] = x[r];
-      }
-      __syncthreads();
-      for (int jj = jtop; jj >= 2 * 256; jj >>= 1) {
-        bt_stage_lds(sk, si, base, len, kk, jj);
-        __syncthreads();
-      }
-#pragma unroll
-      for (int r = 0; r < BT_PER; r++) {
-        e[r] = sk[o + r];
-        x[r] = si[o + r];
-      }
-      jtop = 256;
+}
+
+// One merge over the tile, entered in layout CUR (0: A, 2: C), left in C.
+// UPC = 0 or 1: the direction of a merge wider than the tile, as a constant.
+template <int TB, int KB, int CUR, bool FULL, int UPC = -1>
+__device__ __forceinline__ void bt_tile_merge(float (&e)[16], i32 (&x)[16], float* sk, i32* si,
+                                              int t, int lim, bool tile_up) {
+  constexpr int RA = TB - 4, RB = TB - 8;
+  constexpr int TOP = KB == 0 ? TB - 1 : KB - 1;  // highest stride bit of the merge
+  if constexpr (TOP >= RA) {
+    if constexpr (CUR == 2) bt_tile_relayout<0, RA>(e, x, sk, si, t);
+    bt_tile_stages<RA, TOP, RA, KB, FULL, UPC>(e, x, t, lim, tile_up);
+  }
+  if constexpr (TOP >= RB) {
+    if constexpr (TOP >= RA || CUR == 0) {
+      bt_tile_relayout<RA, RB>(e, x, sk, si, t);
+    } else {
+      bt_tile_relayout<0, RB>(e, x, sk, si, t);
     }
-    bt_regs_down(e, x, p0, lane, len, kk, jtop);
-  };
-  if (kk_fixed == 0) {
-    for (i64 kk = 2; kk <= BT_TILE && kk <= len; kk <<= 1) merge(kk, (int)(kk >> 1));
+    constexpr int BT = TOP < RA - 1 ? TOP : RA - 1;
+    bt_tile_stages<RB, BT, RB, KB, FULL, UPC>(e, x, t, lim, tile_up);
+    bt_tile_relayout<RB, 0>(e, x, sk, si, t);
+  } else if constexpr (CUR == 0) {
+    bt_tile_relayout<RA, 0>(e, x, sk, si, t);
+  }
+  if constexpr (TB == 13 && TOP >= 4) {  // stride 16: lanes t and t ^ 1
+    const bool lo = (t & 1) == 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float o = lane_xor1_f(e[r]);
+      const i32 oi = lane_xor1_i(x[r]);
+      const int p = bt_pos<0>(t, r);
+      const int hi = lo ? p + 16 : p;
+      const bool up = UPC >= 0 ? UPC != 0 : KB == 0 ? tile_up : ((hi >> KB) & 1) == 0;
+      const float a = lo ? o : e[r];
+      const float bb = lo ? e[r] : o;
+      const bool sw = (FULL || hi < lim) && ((a < bb) != up);
+      e[r] = sw ? o : e[r];
+      x[r] = sw ? oi : x[r];
+    }
+  }
+  bt_tile_stages<0, (TOP < 3 ? TOP : 3), 0, KB, FULL, UPC>(e, x, t, lim, tile_up);
+}
+
+// merges of width 2^KB, 2^(KB+1), ... up to the tile (and not above len)
+template <int TB, int KB, bool FULL>
+__device__ __forceinline__ void bt_tile_sort(float (&e)[16], i32 (&x)[16], float* sk, i32* si,
+                                             int t, int lim, i64 base, i64 len) {
+  if constexpr (KB <= TB) {
+    if (((i64)1 << KB) > len) return;
+    const bool tile_up = (base & ((i64)1 << KB)) == 0;
+    bt_tile_merge<TB, (KB == TB ? 0 : KB), 2, FULL>(e, x, sk, si, t, lim, tile_up);
+    bt_tile_sort<TB, KB + 1, FULL>(e, x, sk, si, t, lim, base, len);
+  }
+}
+
+// FIRST: every merge up to the tile width (and not above len); otherwise the
+// strides below the tile of the merge of width kk (wider than the tile).
+// A partial last tile takes the FULL = false form, so the full tiles' code
+// carries no bounds tests.
+template <int TB, bool FULL, bool FIRST>
+__device__ __forceinline__ void bt_tile_body(float* key, i32* idx, i64 len, i64 kk, float* sk,
+                                             i32* si) {
+  constexpr int TILE = 1 << TB;
+  constexpr int RA = TB - 4;
+  const i64 base = (i64)blockIdx.x * TILE;
+  const int lim = FULL ? TILE : (int)(len - base);
+  const int t = threadIdx.x;
+  const float* kb = key + base;
+  const i32* ib = idx + base;
+  float e[16];
+  i32 x[16];
+  constexpr int RL = FIRST ? 0 : RA;  // load layout: C for a whole sort, A for a merge
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int p = bt_pos<RL>(t, r);
+    e[r] = (FULL || p < lim) ? kb[p] : 0.f;
+    x[r] = (FULL || p < lim) ? ib[p] : -1;
+  }
+  if constexpr (FIRST) bt_tile_sort<TB, 1, FULL>(e, x, sk, si, t, lim, base, len);
+  else if ((base & kk) == 0) bt_tile_merge<TB, 0, 0, FULL, 1>(e, x, sk, si, t, lim, true);
+  else bt_tile_merge<TB, 0, 0, FULL, 0>(e, x, sk, si, t, lim, false);
+  const int o = bt_pos<0>(t, 0);
+  float* ko = key + base;
+  i32* io = idx + base;
+  if constexpr (FULL) {
+#pragma unroll
+    for (int r = 0; r < 16; r += 4) {
+      *reinterpret_cast<float4*>(ko + o + r) = make_float4(e[r], e[r + 1], e[r + 2], e[r + 3]);
+      *reinterpret_cast<int4*>(io + o + r) = make_int4(x[r], x[r + 1], x[r + 2], x[r + 3]);
+    }
   } else {
-    merge(kk_fixed, BT_TILE >> 1);
-  }
-  __syncthreads();
 #pragma unroll
-  for (int r = 0; r < BT_PER; r++) {
-    sk[o + r] = e[r];
-    si[o + r] = x[r];
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < BT_TILE; q += BT_THREADS) {
-    const i64 gi = base + q;
-    if (gi < len) {
-      key[gi] = sk[q];
-      idx[gi] = si[q];
+    for (int r = 0; r < 16; r++) {
+      if (o + r < lim) {
+        ko[o + r] = e[r];
+        io[o + r] = x[r];
+      }
     }
   }
+}
+
+template <int TB, bool FIRST>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_bt_tile(float* key, i32* idx, i64 len,
+                                                           i64 kk) {
+  __shared__ float sk[1 << TB];
+  __shared__ i32 si[1 << TB];
+  if ((i64)(blockIdx.x + 1) << TB <= len) bt_tile_body<TB, true, FIRST>(key, idx, len, kk, sk, si);
+  else bt_tile_body<TB, false, FIRST>(key, idx, len, kk, sk, si);
 }
 
 // S consecutive strides j, j/2, ..., j >> (S-1) of one merge kk in a single
@@ -3206,28 +3244,29 @@ void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i6
 
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* idx,
                        hipStream_t s) {
+  constexpr i64 TILE = (i64)1 << BT_TB;
   k_sort_init<<<grid_for(len, 256, 1u << 16), 256, 0, s>>>(mv_gain, M, len, key, idx);
   if (len < 2) return;
-  const unsigned tiles = (unsigned)((len + BT_TILE - 1) / BT_TILE);
-  k_bt_local<<<tiles, BT_THREADS, 0, s>>>(key, idx, len, 0);
+  const unsigned tiles = (unsigned)((len + TILE - 1) / TILE);
+  constexpr int NT = 1 << (BT_TB - 4);
+  k_bt_tile<BT_TB, true><<<tiles, NT, 0, s>>>(key, idx, len, 0);
   i64 p2 = 1;
   while (p2 < len) p2 <<= 1;
-  for (i64 kk = 2 * BT_TILE; kk <= len; kk <<= 1) {
+  for (i64 kk = 2 * TILE; kk <= len; kk <<= 1) {
     int bhi = 0;  // log2 of the largest stride of this merge
     while (((i64)1 << (bhi + 1)) < kk) bhi++;
-    int blo_min = 0;
-    while (((i64)1 << blo_min) < BT_TILE) blo_min++;
-    for (int b = bhi; b >= blo_min;) {
-      const int S = std::min(3, b - blo_min + 1);
+    for (int b = bhi; b >= BT_TB;) {
+      const int S = std::min(BT_SMAX, b - BT_TB + 1);
       const int blo = b - S + 1;
       const i64 nsets = p2 >> S;
       const unsigned gr = grid_for(nsets, 256, 1u << 16);
-      if (S == 3) k_bt_global<3><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      if (S == 4) k_bt_global<4><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      else if (S == 3) k_bt_global<3><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       else if (S == 2) k_bt_global<2><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       else k_bt_global<1><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       b -= S;
     }
-    k_bt_local<<<tiles, BT_THREADS, 0, s>>>(key, idx, len, kk);
+    k_bt_tile<BT_TB, false><<<tiles, NT, 0, s>>>(key, idx, len, kk);
   }
 }
 
