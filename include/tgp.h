@@ -135,6 +135,28 @@ typedef struct tgp_stats {
 int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part,
                   int32_t* part_count, int64_t* edge_cut, tgp_stats* stats);
 
+/* ---- batched form (no counterpart in the reference, which partitions one
+ * graph per process, gpu_part.cu:878-1344) ---------------------------------
+ * Partitions `count` independent graphs.  Each item is partitioned exactly as
+ * tgp_partition(&it->g, &it->o, it->part, it->part_count, &it->edge_cut,
+ * it->stats) would do alone, so every result is bit-identical to its solo
+ * run; up to `workers` items (1..64) run at once, each on its own HIP stream
+ * and device pool.  Initial-partition callbacks are serialised (the gpmetis
+ * seam uses fixed file names in the working directory).  Returns TGP_OK when
+ * every item succeeded, else the first failing item's code; each item's own
+ * code is in it->rc.                                                         */
+typedef struct tgp_batch_item {
+  tgp_graph g;             /* host CSR, or device CSR when o.device_input = 1  */
+  tgp_options o;
+  int32_t* part;           /* out: part[g.n]                                    */
+  int32_t* part_count;     /* out: part_count[o.k]                              */
+  tgp_stats* stats;        /* optional                                          */
+  int64_t edge_cut;        /* out                                               */
+  int32_t rc;              /* out: TGP_OK or the item's error code              */
+  int32_t reserved;
+} tgp_batch_item;
+int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers);
+
 const char* tgp_strerror(int code);
 /* Detail of the last failure on the calling thread ("" if none).            */
 const char* tgp_last_error(void);
@@ -163,6 +185,9 @@ int tgp_gen_rmat(int32_t scale, int32_t edge_factor, uint64_t seed, int32_t weig
                  tgp_graph* out);
 int tgp_gen_grid3d(int32_t nx, int32_t ny, int32_t nz, tgp_graph* out);
 int tgp_gen_er(int64_t n, double mean_degree, uint64_t seed, tgp_graph* out);
+/* R-MAT on n vertices (any n >= 2): samples of the scale ceil(log2 n) model
+ * whose two endpoints are both below n, edge_factor * n of them.            */
+int tgp_gen_rmat_n(int64_t n, int32_t edge_factor, uint64_t seed, tgp_graph* out);
 void tgp_graph_free(tgp_graph* g);
 void tgp_free(void* p);
 

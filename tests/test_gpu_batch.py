@@ -1,0 +1,89 @@
+"""Batched form (tgp_partition_batch): every graph's result equals its solo run.
+
+The batch mixes R-MAT and Erdos-Renyi graphs of uneven size with per-graph k,
+as the batched workload does, and runs them several at a time on separate
+streams; each partition, part count and cut must equal what tgp_partition
+returns for that graph alone, and a few are also checked against the oracle.
+"""
+import numpy as np
+import pytest
+
+import thegraphpartitioning_amd as tgp
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_graphs(count, seed, nmax):
+    rng = np.random.default_rng(seed)
+    gs, ks = [], []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1000), np.log(nmax)))))
+        g = tgp.gen_rmat_n(n, 8, seed * 1000 + i) if i % 2 == 0 else \
+            tgp.gen_er(n, 10.0, seed * 1000 + i)
+        gs.append(g)
+        ks.append(int(rng.choice([2, 4, 8, 16])))
+    return gs, ks
+
+
+@pytest.mark.parametrize("workers", [1, 6])
+def test_batch_equals_solo(workers):
+    gs, ks = batch_graphs(20, 11, 30000)
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=workers)
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
+        bp, bpc, bcut, _ = out[i]
+        assert bcut == cut, f"graph {i} (n={g.n}, k={k})"
+        np.testing.assert_array_equal(bp, part, err_msg=f"graph {i}")
+        np.testing.assert_array_equal(bpc, pc, err_msg=f"graph {i}")
+
+
+def test_batch_matches_oracle(oracle):
+    gs, ks = batch_graphs(6, 12, 8000)
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=3)
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        opart, opc, ocut, _ = oracle.partition(g.xadj, g.adjncy, g.adjwgt, k, repair=1)
+        assert out[i][2] == ocut, f"graph {i}"
+        np.testing.assert_array_equal(out[i][0], opart)
+        np.testing.assert_array_equal(out[i][1], opc)
+
+
+def test_batch_reference_semantics_and_errors():
+    """Reference mode (bugs included): graphs whose coarsening the reference
+    rejects fail in the batch with the same code as alone, the others match."""
+    gs, ks = batch_graphs(10, 13, 20000)
+    items = []
+    for g, k in zip(gs, ks):
+        try:
+            items.append(tgp.partition(g, k, init=tgp.INIT_BLOCK))
+        except tgp.TGPError as e:
+            items.append(e.code)
+    for i in range(len(gs)):
+        try:
+            r = tgp.partition_batch([gs[i]], [ks[i]], init=tgp.INIT_BLOCK, workers=1)[0]
+        except tgp.TGPError as e:
+            assert items[i] == e.code, f"graph {i}"
+            continue
+        assert not isinstance(items[i], int), f"graph {i} failed alone, not in the batch"
+        assert r[2] == items[i][2]
+        np.testing.assert_array_equal(r[0], items[i][0])
+    ok = [i for i in range(len(gs)) if not isinstance(items[i], int)]
+    out = tgp.partition_batch([gs[i] for i in ok], [ks[i] for i in ok], init=tgp.INIT_BLOCK,
+                              workers=4)
+    for j, i in enumerate(ok):
+        assert out[j][2] == items[i][2]
+        np.testing.assert_array_equal(out[j][0], items[i][0])
+
+
+def test_batch_device_resident():
+    gs, ks = batch_graphs(8, 14, 20000)
+    dgs = [tgp.DeviceGraph(g) for g in gs]
+    try:
+        out = tgp.partition_batch(gs, ks, repair=True, workers=4,
+                                  device_graphs=[d.pointers() for d in dgs])
+    finally:
+        for d in dgs:
+            d.free()
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
+        assert out[i][2] == cut
+        np.testing.assert_array_equal(out[i][0], part)

@@ -47,6 +47,10 @@ class COptions(C.Structure):
                 ("check_level", C.c_int32), ("repair", C.c_int32)]
 
 
+class CBatchItem(C.Structure):
+    pass
+
+
 class CTrace(C.Structure):
     _fields_ = [("kind", C.c_int32), ("level", C.c_int32), ("phase", C.c_int32),
                 ("pass_", C.c_int32), ("a", C.c_int64), ("b", C.c_int64), ("c", C.c_int64),
@@ -66,6 +70,11 @@ class CStats(C.Structure):
 
 
 _lib = None
+
+
+CBatchItem._fields_ = [("g", CGraph), ("o", COptions), ("part", C.c_void_p),
+                       ("part_count", C.c_void_p), ("stats", C.POINTER(CStats)),
+                       ("edge_cut", C.c_int64), ("rc", C.c_int32), ("reserved", C.c_int32)]
 
 
 def library_path():
@@ -94,6 +103,9 @@ def lib():
     L.tgp_gen_rmat.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_int32, C.POINTER(CGraph)]
     L.tgp_gen_grid3d.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(CGraph)]
     L.tgp_gen_er.argtypes = [C.c_int64, C.c_double, C.c_uint64, C.POINTER(CGraph)]
+    L.tgp_gen_rmat_n.argtypes = [C.c_int64, C.c_int32, C.c_uint64, C.POINTER(CGraph)]
+    L.tgp_partition_batch.argtypes = [C.POINTER(CBatchItem), C.c_int64, C.c_int32]
+    L.tgp_partition_batch.restype = C.c_int
     L.tgp_read_graph_dir.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(CGraph)]
     L.tgp_graph_free.argtypes = [C.POINTER(CGraph)]
     L.tgp_graph_free.restype = None
@@ -158,6 +170,16 @@ def gen_rmat(scale, edge_factor, seed, weights="unit"):
     cg = CGraph()
     rc = lib().tgp_gen_rmat(scale, edge_factor, seed, 1 if weights == "hash16" else 0,
                             C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc)
+    return _take_cgraph(cg)
+
+
+def gen_rmat_n(n, edge_factor, seed):
+    """R-MAT (0.57, 0.19, 0.19, 0.05) on n vertices: the first edge_factor*n
+    samples of the scale ceil(log2 n) model with both endpoints below n."""
+    cg = CGraph()
+    rc = lib().tgp_gen_rmat_n(n, edge_factor, seed, C.byref(cg))
     if rc != 0:
         raise TGPError(rc)
     return _take_cgraph(cg)
@@ -324,3 +346,54 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
         err.trace = trace
         raise err
     return part, pc, cut.value, Stats(st, trace)
+
+
+def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, coarsen_floor=0,
+                    workers=8, device_graphs=None, profile=False):
+    """Partitions many independent graphs through tgp_partition_batch.
+
+    Each result equals what partition() returns for that graph alone (bit for
+    bit); up to `workers` graphs run at once, each on its own HIP stream.
+    `init` is "block" or "gpmetis"; `device_graphs`, if given, holds one
+    (xadj, adjncy, adjwgt) triple of device pointers per graph.  Returns a list
+    of (part, part_count, edge_cut, Stats); raises TGPError naming the first
+    failing graph.
+    """
+    L = lib()
+    cnt = len(graphs)
+    if len(ks) != cnt:
+        raise ValueError("one k per graph")
+    if init == INIT_BLOCK:
+        fn = C.cast(L.tgp_initpart_block, C.c_void_p).value
+    elif init == INIT_GPMETIS:
+        fn = C.cast(L.tgp_initpart_gpmetis, C.c_void_p).value
+    else:
+        raise ValueError(f"batched form takes init 'block' or 'gpmetis', not {init!r}")
+    items = (CBatchItem * max(cnt, 1))()
+    outs = []
+    for i, g in enumerate(graphs):
+        it = items[i]
+        if device_graphs is not None:
+            d = device_graphs[i]
+            it.g = CGraph(g.n, g.m, d[0], d[1], d[2])
+            it.o.device_input = 1
+        else:
+            it.g = g.cgraph()
+        it.o.k = int(ks[i])
+        it.o.wide = int(bool(wide))
+        it.o.repair = int(bool(repair))
+        it.o.coarsen_floor = int(coarsen_floor)
+        it.o.profile = int(bool(profile))
+        it.o.initpart = fn
+        part = np.zeros(g.n, np.int32)
+        pc = np.zeros(int(ks[i]), np.int32)
+        st = CStats()
+        it.part = part.ctypes.data
+        it.part_count = pc.ctypes.data
+        it.stats = C.pointer(st)
+        outs.append((part, pc, st))
+    rc = L.tgp_partition_batch(items, cnt, int(workers))
+    if rc != 0:
+        bad = next(i for i in range(cnt) if items[i].rc != 0)
+        raise TGPError(rc, f"graph {bad}: {L.tgp_strerror(items[bad].rc).decode()}")
+    return [(part, pc, items[i].edge_cut, Stats(st, [])) for i, (part, pc, st) in enumerate(outs)]

@@ -18,6 +18,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -780,7 +782,14 @@ int Run::initial_partition(DBuf<i32>& part, i64* npairs) {
     tgp_graph hg{n, cl.m, hx.data(), ha.data(), hw.data()};
     std::vector<i32> hp((size_t)n, 0);
     tgp_initpart_fn fn = O.initpart ? O.initpart : tgp_initpart_gpmetis;
-    const int rc = fn(&hg, *npairs, k, hp.data(), O.initpart_user);
+    int rc;
+    {
+      // one callback at a time: the gpmetis seam uses fixed file names in the
+      // working directory (gpu_part.cu:829, 1163)
+      static std::mutex initpart_mu;
+      std::lock_guard<std::mutex> lock(initpart_mu);
+      rc = fn(&hg, *npairs, k, hp.data(), O.initpart_user);
+    }
     if (rc != 0) {
       if (last_error().empty()) set_error("initial partitioner failed");
       throw Fail(rc < 0 ? rc : TGP_EINITPART);
@@ -1097,14 +1106,18 @@ Context* default_context() {
   return ctx;
 }
 
-}  // namespace
+// Contexts of the batched form's workers, created on first use and, like the
+// default context, never destroyed.
+Context* worker_context(int w) {
+  static std::vector<Context*> pool;
+  while ((int)pool.size() <= w) pool.push_back(new Context());
+  return pool[w];
+}
 
-extern "C" {
-
-const char* tgp_last_error(void) { return last_error().c_str(); }
-
-int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
-                  int64_t* edge_cut, tgp_stats* stats) {
+// One partition on context C (its stream, pool and profile); the caller
+// holds the lock that serialises use of C.
+int partition_on(Context& Cx, const tgp_graph* g, const tgp_options* o, int32_t* part,
+                 int32_t* part_count, int64_t* edge_cut, tgp_stats* stats) {
   last_error().clear();
   if (!g || !o || !part || !part_count || !edge_cut) {
     set_error("null argument");
@@ -1138,9 +1151,8 @@ int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32
     stats->trace = tb;
     stats->trace_cap = tc;
   }
-  std::lock_guard<std::mutex> lock(g_ctx_mu);
   try {
-    Context* C = default_context();
+    Context* C = &Cx;
     C->prof.on = o->profile != 0;
     for (int i = 0; i < KID_COUNT; i++) {
       C->prof.ms[i] = 0;
@@ -1162,9 +1174,54 @@ int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32
     }
     return rc;
   } catch (const Fail& f) {
-    (void)hipStreamSynchronize(default_context()->s);
+    (void)hipStreamSynchronize(Cx.s);
     return f.code;
   }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tgp_last_error(void) { return last_error().c_str(); }
+
+int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
+                  int64_t* edge_cut, tgp_stats* stats) {
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  return partition_on(*default_context(), g, o, part, part_count, edge_cut, stats);
+}
+
+int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
+  last_error().clear();
+  if (count < 0 || (count > 0 && !items) || workers < 1 || workers > 64) {
+    set_error("batch: need items, count >= 0 and workers in [1, 64]");
+    return TGP_EINVAL;
+  }
+  if (count == 0) return TGP_OK;
+  const int W = (int)std::min<i64>(workers, count);
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  std::vector<Context*> ctx;
+  try {
+    for (int w = 0; w < W; w++) ctx.push_back(worker_context(w));
+  } catch (const Fail& f) {
+    return f.code;
+  }
+  std::atomic<i64> next{0};
+  auto work = [&](Context* C) {
+    for (;;) {
+      const i64 i = next.fetch_add(1);
+      if (i >= count) break;
+      tgp_batch_item& it = items[i];
+      it.rc = partition_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < W; w++) th.emplace_back(work, ctx[w]);
+  work(ctx[0]);
+  for (auto& t : th) t.join();
+  for (i64 i = 0; i < count; i++)
+    if (items[i].rc != TGP_OK) return items[i].rc;
+  return TGP_OK;
 }
 
 const char* tgp_kernel_name(int id) {

@@ -6,7 +6,8 @@
 // rows listing their neighbours in ascending order.
 //
 //   R-MAT(scale s, edge factor ef, a,b,c,d = 0.57,0.19,0.19,0.05): ef*2^s
-//     samples, no noise, no vertex scrambling.
+//     samples, no noise, no vertex scrambling.  R-MAT on n vertices keeps the
+//     first ef*n samples of the scale ceil(log2 n) model inside [0, n)^2.
 //   3-D 7-point grid nx*ny*nz: id = x + nx*y + nx*ny*z.
 //   Erdos-Renyi G(n, p) by geometric skip sampling (Batagelj-Brandes).
 //
@@ -162,6 +163,34 @@ int tgp_gen_rmat(int32_t scale, int32_t edge_factor, uint64_t seed, int32_t weig
   std::vector<int32_t>().swap(sv);
   std::vector<int64_t>().swap(keep);
   return build_csr(nv, src, dst, weight_mode, seed, out);
+}
+
+int tgp_gen_rmat_n(int64_t n, int32_t edge_factor, uint64_t seed, tgp_graph* out) {
+  if (!out || n < 2 || n > INT32_MAX || edge_factor < 1) return TGP_EINVAL;
+  int scale = 1;
+  while (((int64_t)1 << scale) < n) scale++;
+  const int64_t want = n * edge_factor;
+  const double a = 0.57, b = 0.19, c = 0.19;
+  std::vector<int32_t> src, dst;
+  src.reserve((size_t)want * 2);
+  dst.reserve((size_t)want * 2);
+  int64_t kept = 0;
+  for (uint64_t i = 0; kept < want; i++) {
+    int64_t u = 0, v = 0;
+    for (int l = 0; l < scale; l++) {
+      const double r = unit(draw(seed, (uint64_t)l + 1, i));
+      const int bu = r >= a + b;
+      const int bv = (r >= a && r < a + b) || r >= a + b + c;
+      u = (u << 1) | bu;
+      v = (v << 1) | bv;
+    }
+    if (u >= n || v >= n) continue;
+    kept++;
+    if (u == v) continue;
+    src.push_back((int32_t)u); dst.push_back((int32_t)v);
+    src.push_back((int32_t)v); dst.push_back((int32_t)u);
+  }
+  return build_csr(n, src, dst, 0, seed, out);
 }
 
 int tgp_gen_grid3d(int32_t nx, int32_t ny, int32_t nz, tgp_graph* out) {
