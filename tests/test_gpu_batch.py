@@ -25,8 +25,11 @@ def batch_graphs(count, seed, nmax):
     return gs, ks
 
 
+@pytest.mark.parametrize("engine", ["1", "0"])
 @pytest.mark.parametrize("workers", [1, 6])
-def test_batch_equals_solo(workers):
+def test_batch_equals_solo(workers, engine, monkeypatch):
+    """engine "1": the persistent engine takes every graph; "0": per-stream workers."""
+    monkeypatch.setenv("TGP_BATCH_ENGINE", engine)
     gs, ks = batch_graphs(20, 11, 30000)
     out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=workers)
     for i, (g, k) in enumerate(zip(gs, ks)):
@@ -87,3 +90,25 @@ def test_batch_device_resident():
         part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
         assert out[i][2] == cut
         np.testing.assert_array_equal(out[i][0], part)
+
+
+def test_batch_mixed_scope():
+    """Graphs the engine does not take -- k above 16, a graph with a missing
+    reverse arc, and one with a self loop -- go to the per-stream workers in
+    the same call; every result still equals its solo run."""
+    gs, ks = batch_graphs(6, 15, 6000)
+    ks[1] = 32
+    g = gs[2]
+    adj = g.adjncy.copy()
+    adj[g.xadj[10]] = (adj[g.xadj[10]] + 1) % g.n if g.xadj[11] > g.xadj[10] else adj[0]
+    gs[2] = tgp.Graph(g.xadj.copy(), adj, g.adjwgt.copy())
+    g = gs[3]
+    adj = g.adjncy.copy()
+    adj[g.xadj[5]] = 5
+    gs[3] = tgp.Graph(g.xadj.copy(), adj, g.adjwgt.copy())
+    out = tgp.partition_batch(gs, ks, repair=True, workers=3)
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
+        assert out[i][2] == cut, f"graph {i}"
+        np.testing.assert_array_equal(out[i][0], part)
+        np.testing.assert_array_equal(out[i][1], pc)

@@ -1106,6 +1106,165 @@ Context* default_context() {
   return ctx;
 }
 
+// The batched engine over the items it can take (repaired coarsening, block
+// initial partition, k <= BATCH_MAXK, no profiling or trace); the others,
+// and every graph the engine hands back, are appended to `rest`.
+void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>& rest) {
+  std::vector<i64> take;
+  for (i64 i = 0; i < count; i++) {
+    const tgp_batch_item& it = items[i];
+    const bool ok = it.o.repair && it.o.initpart == tgp_initpart_block && !it.o.profile &&
+                    !it.o.check_level && (!it.stats || !it.stats->trace) && it.o.k >= 1 &&
+                    it.o.k <= BATCH_MAXK && it.g.n >= 2 && it.g.n <= INT_MAX && it.g.m > 0 &&
+                    it.g.m <= INT_MAX && it.part && it.part_count &&
+                    (it.o.device_input || validate_host_graph(&it.g) == TGP_OK);
+    if (ok) take.push_back(i);
+    else rest.push_back(i);
+  }
+  if (take.empty()) return;
+  const i64 T = (i64)take.size();
+  i64 maxn = 0, maxm = 0, sum_n = 0, up_bytes = 0;
+  for (i64 i : take) {
+    const tgp_graph& g = items[i].g;
+    maxn = std::max<i64>(maxn, g.n);
+    maxm = std::max<i64>(maxm, g.m);
+    sum_n += g.n;
+    if (!items[i].o.device_input) up_bytes += 8 * (g.n + 1) + 8 * g.m + 64;
+  }
+  // inputs held in host memory go up once, into one buffer
+  DBuf<u8> up;
+  std::vector<BatchGraph> hg((size_t)T);
+  if (up_bytes) up = DBuf<u8>(&C.pool, (size_t)up_bytes);
+  i64 uo = 0;
+  DBuf<i32> parts(&C.pool, (size_t)sum_n), pcs(&C.pool, (size_t)T * BATCH_MAXK),
+      status(&C.pool, (size_t)T), order(&C.pool, (size_t)T);
+  DBuf<i64> cuts(&C.pool, (size_t)T);
+  DBuf<BatchGraph> dg(&C.pool, (size_t)T);
+  DBuf<u32> next(&C.pool, 1);
+  std::vector<i64> part_off((size_t)T);
+  i64 po = 0;
+  for (i64 j = 0; j < T; j++) {
+    const tgp_batch_item& it = items[take[j]];
+    BatchGraph& b = hg[j];
+    b.n = it.g.n;
+    b.m = it.g.m;
+    if (it.o.device_input) {
+      b.xadj = it.g.xadj;
+      b.adj = it.g.adjncy;
+      b.w = it.g.adjwgt;
+    } else {
+      u8* base = up.p + uo;
+      HIP_TRY(hipMemcpyAsync(base, it.g.xadj, 8 * (b.n + 1), hipMemcpyHostToDevice, C.s));
+      HIP_TRY(hipMemcpyAsync(base + 8 * (b.n + 1), it.g.adjncy, 4 * b.m, hipMemcpyHostToDevice,
+                             C.s));
+      HIP_TRY(hipMemcpyAsync(base + 8 * (b.n + 1) + 4 * b.m, it.g.adjwgt, 4 * b.m,
+                             hipMemcpyHostToDevice, C.s));
+      b.xadj = (const i64*)base;
+      b.adj = (const i32*)(base + 8 * (b.n + 1));
+      b.w = (const i32*)(base + 8 * (b.n + 1) + 4 * b.m);
+      uo += 8 * (b.n + 1) + 8 * b.m + 64;
+      uo = (uo + 255) & ~(i64)255;
+    }
+    b.k = it.o.k;
+    b.wide = it.o.wide;
+    b.target = it.o.coarsen_floor > 0 ? it.o.coarsen_floor : std::max<i64>(500, b.n / 500);
+    part_off[j] = po;
+    b.part_out = parts.p + po;
+    po += b.n;
+    b.pc_out = pcs.p + j * BATCH_MAXK;
+    b.cut_out = cuts.p + j;
+    b.status = status.p + j;
+  }
+  // largest graphs first, so the long ones start early
+  std::vector<i32> ord((size_t)T);
+  for (i64 j = 0; j < T; j++) ord[j] = (i32)j;
+  std::stable_sort(ord.begin(), ord.end(), [&](i32 a, i32 b) { return hg[a].m > hg[b].m; });
+  HIP_TRY(hipMemcpyAsync(dg.p, hg.data(), sizeof(BatchGraph) * T, hipMemcpyHostToDevice, C.s));
+  HIP_TRY(hipMemcpyAsync(order.p, ord.data(), sizeof(i32) * T, hipMemcpyHostToDevice, C.s));
+  HIP_TRY(hipMemsetAsync(next.p, 0, sizeof(u32), C.s));
+  launch_fill_i32(status.p, T, 1, C.s);
+  // one workgroup per CU, each with scratch for the largest graph
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+  const i64 per_wg = batch_scratch_bytes(maxn, maxm);
+  int blocks = (int)std::min<i64>(T, std::max(cus, 1));
+  void* scratch = nullptr;
+  while (blocks > 0) {
+    if (hipMalloc(&scratch, (size_t)(per_wg * blocks)) == hipSuccess) break;
+    (void)hipGetLastError();
+    scratch = nullptr;
+    blocks /= 2;
+  }
+  if (!scratch) {
+    for (i64 i : take) rest.push_back(i);
+    return;
+  }
+  const i64 budget = C.clock_khz * 2000;  // 2 s per level's matching
+  static const bool trace = env_i64("TGP_BATCH_TRACE", 0) != 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DBuf<u64> bst;
+  if (trace) {
+    bst = DBuf<u64>(&C.pool, 12);
+    HIP_TRY(hipMemsetAsync(bst.p, 0, 12 * sizeof(u64), C.s));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, C.s));
+  }
+  launch_batch_engine(dg.p, order.p, (i32)T, next.p, (u8*)scratch, maxn, maxm, blocks, budget,
+                      bst.p, C.s);
+  if (trace) HIP_TRY(hipEventRecord(e1, C.s));
+  std::vector<i32> hstat((size_t)T), hpc((size_t)T * BATCH_MAXK);
+  std::vector<i64> hcut((size_t)T);
+  HIP_TRY(hipMemcpyAsync(hstat.data(), status.p, sizeof(i32) * T, hipMemcpyDeviceToHost, C.s));
+  HIP_TRY(hipMemcpyAsync(hpc.data(), pcs.p, sizeof(i32) * T * BATCH_MAXK, hipMemcpyDeviceToHost,
+                         C.s));
+  HIP_TRY(hipMemcpyAsync(hcut.data(), cuts.p, sizeof(i64) * T, hipMemcpyDeviceToHost, C.s));
+  for (i64 j = 0; j < T; j++) {
+    tgp_batch_item& it = items[take[j]];
+    HIP_TRY(hipMemcpyAsync(it.part, parts.p + part_off[j], sizeof(i32) * it.g.n,
+                           hipMemcpyDeviceToHost, C.s));
+  }
+  C.sync();
+  (void)hipFree(scratch);
+  if (trace) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    i64 back = 0;
+    for (i64 j = 0; j < T; j++) back += hstat[j] != 0;
+    std::fprintf(stderr,
+                 "# batch engine: %lld graphs, %lld handed back, %d workgroups x %.1f MB, "
+                 "%.2f ms\n",
+                 (long long)T, (long long)back, blocks, per_wg / 1048576.0, ms);
+    u64 hb[12];
+    HIP_TRY(hipMemcpy(hb, bst.p, sizeof(hb), hipMemcpyDeviceToHost));
+    const char* nm[10] = {"check", "match", "contract", "gain", "token",
+                          "moves", "sort",  "select",   "cut",  "misc"};
+    std::fprintf(stderr, "# batch engine phases (workgroup-ms):");
+    for (int q = 0; q < 10; q++) std::fprintf(stderr, " %s=%.1f", nm[q], hb[q] / (double)C.clock_khz);
+    std::fprintf(stderr, " passes=%llu levels=%llu\n", (unsigned long long)hb[10],
+                 (unsigned long long)hb[11]);
+  }
+  for (i64 j = 0; j < T; j++) {
+    tgp_batch_item& it = items[take[j]];
+    if (hstat[j] != 0) {
+      rest.push_back(take[j]);
+      continue;
+    }
+    std::memcpy(it.part_count, hpc.data() + j * BATCH_MAXK, sizeof(i32) * it.o.k);
+    it.edge_cut = hcut[j];
+    it.rc = TGP_OK;
+    if (it.stats) {
+      tgp_trace* tb = it.stats->trace;
+      const i64 tc = it.stats->trace_cap;
+      std::memset(it.stats, 0, sizeof(*it.stats));
+      it.stats->trace = tb;
+      it.stats->trace_cap = tc;
+    }
+  }
+}
+
 // Contexts of the batched form's workers, created on first use and, like the
 // default context, never destroyed.
 Context* worker_context(int w) {
@@ -1198,8 +1357,22 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
     return TGP_EINVAL;
   }
   if (count == 0) return TGP_OK;
-  const int W = (int)std::min<i64>(workers, count);
   std::lock_guard<std::mutex> lock(g_ctx_mu);
+  // graphs within the engine's scope run in one persistent launch
+  // (TGP_BATCH_ENGINE=0 sends every graph to the per-stream workers)
+  std::vector<i64> rest;
+  if (std::getenv("TGP_BATCH_ENGINE") && std::atoi(std::getenv("TGP_BATCH_ENGINE")) == 0) {
+    for (i64 i = 0; i < count; i++) rest.push_back(i);
+  } else {
+    try {
+      engine_batch(*default_context(), items, count, rest);
+    } catch (const Fail& f) {
+      (void)hipStreamSynchronize(default_context()->s);
+      return f.code;
+    }
+  }
+  const i64 nrest = (i64)rest.size();
+  const int W = (int)std::min<i64>(workers, std::max<i64>(nrest, 1));
   std::vector<Context*> ctx;
   try {
     for (int w = 0; w < W; w++) ctx.push_back(worker_context(w));
@@ -1209,9 +1382,9 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
   std::atomic<i64> next{0};
   auto work = [&](Context* C) {
     for (;;) {
-      const i64 i = next.fetch_add(1);
-      if (i >= count) break;
-      tgp_batch_item& it = items[i];
+      const i64 r = next.fetch_add(1);
+      if (r >= nrest) break;
+      tgp_batch_item& it = items[rest[r]];
       it.rc = partition_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
     }
   };

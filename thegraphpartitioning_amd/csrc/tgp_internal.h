@@ -95,6 +95,35 @@ void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, cons
 void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s);
 // mate = st where st >= 0 else -1; dec = (st != -1)
 void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s);
+
+// ---- batched engine: one workgroup per graph, the whole partition on device
+// One graph of a batch (device pointers).  status on return: 0 done, 1 the
+// graph needs the multi-kernel path (outside the engine's limits).
+struct BatchGraph {
+  i64 n, m;
+  const i64* xadj;
+  const i32* adj;
+  const i32* w;
+  i32 k, wide;
+  i64 target;  // coarsening stops at or below this many vertices
+  i32* part_out;
+  i32* pc_out;
+  i64* cut_out;
+  i32* status;
+};
+constexpr int BATCH_MAXK = 16;
+// Scratch bytes one workgroup needs for graphs of at most maxn vertices and
+// maxm arcs (the level arena is sized from maxm; a graph whose coarse levels
+// outgrow it falls back).
+i64 batch_scratch_bytes(i64 maxn, i64 maxm);
+// One persistent launch over `count` graphs taken in the order of `order`;
+// `blocks` workgroups, each with batch_scratch_bytes(maxn, maxm) of scratch.
+// stats (optional, 12 u64): wall-clock ticks per phase summed over
+// workgroups (check, match, contract, gain, token, moves, sort, select, cut,
+// misc), then refinement passes and levels built.
+void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count, u32* next,
+                         u8* scratch, i64 maxn, i64 maxm, int blocks, i64 budget,
+                         u64* stats, hipStream_t s);
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
                             i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s);

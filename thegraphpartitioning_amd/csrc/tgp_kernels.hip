@@ -741,10 +741,12 @@ __device__ __forceinline__ bool sg_any(bool f) {
   return v != 0;
 }
 
+// The chained matching of one wave: groups from `ticket` starting at vertex
+// lo, until the groups run out or the wall-clock budget is spent.
 template <int CH_SG, int CH_PER>
-__global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC,
-                                                     i32* prop, const i64* lo_io, u32* ticket,
-                                                     i64 budget, u64* stats) {
+__device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, i32* blkC,
+                                           i32* prop, const i64 lo, u32* ticket, i64 budget,
+                                           u64* stats) {
   constexpr int CH_VPW = WAVE / CH_SG;
   constexpr int CH_ROW = CH_SG * CH_PER;
   constexpr u64 SG_MASK = CH_SG == 64 ? ~0ull : (1ull << (CH_SG % 64)) - 1;
@@ -752,7 +754,6 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
   u64 c_iter = 0, c_sleep = 0, c_apS = 0, c_aL = 0, c_pL = 0, c_fetch = 0, c_cS = 0, c_cL = 0,
       c_commit = 0, c_arcsL = 0;
   const i64 n = g.n;
-  const i64 lo = lo_io[0];
   const int lane = lane_id();
   const int sl = lane & (CH_SG - 1);
   const int sgbase = lane & ~(CH_SG - 1);
@@ -1023,6 +1024,13 @@ __global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blk
                        c_arcsL};
     for (int q = 0; q < 10; q++) atomicAdd((unsigned long long*)&stats[q], (unsigned long long)c[q]);
   }
+}
+
+template <int CH_SG, int CH_PER>
+__global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC,
+                                                     i32* prop, const i64* lo_io, u32* ticket,
+                                                     i64 budget, u64* stats) {
+  chain_wave<CH_SG, CH_PER>(g, st, blkA, blkC, prop, lo_io[0], ticket, budget, stats);
 }
 
 // st -> mate (and dec for the round kernels)
@@ -2926,6 +2934,8 @@ __global__ __launch_bounds__(ACC_SLOTS) void k_acc_finish(u64* acc, u64* out) {
   }
 }
 
+#include "tgp_batch_engine.inc"
+
 }  // namespace
 
 // ================================================================ launchers ==
@@ -3331,6 +3341,18 @@ void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 
     k_cut_arcs<true><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
   else
     k_cut_arcs<false><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
+}
+
+i64 batch_scratch_bytes(i64 maxn, i64 maxm) {
+  BeScratch X;
+  return be_layout(maxn, &X, nullptr) + be_arena_bytes(maxn, maxm);
+}
+
+void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count, u32* next,
+                         u8* scratch, i64 maxn, i64 maxm, int blocks, i64 budget,
+                         u64* stats, hipStream_t s) {
+  k_batch_engine<<<blocks, BE_T, 0, s>>>(graphs, order, count, next, scratch, maxn,
+                                          batch_scratch_bytes(maxn, maxm), budget, stats);
 }
 
 }  // namespace tgp
