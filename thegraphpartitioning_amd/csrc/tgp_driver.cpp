@@ -1123,11 +1123,10 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
   }
   if (take.empty()) return;
   const i64 T = (i64)take.size();
-  i64 maxn = 0, maxm = 0, sum_n = 0, up_bytes = 0;
+  i64 maxn = 0, sum_n = 0, up_bytes = 0;
   for (i64 i : take) {
     const tgp_graph& g = items[i].g;
     maxn = std::max<i64>(maxn, g.n);
-    maxm = std::max<i64>(maxm, g.m);
     sum_n += g.n;
     if (!items[i].o.device_input) up_bytes += 8 * (g.n + 1) + 8 * g.m + 64;
   }
@@ -1140,7 +1139,6 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
       status(&C.pool, (size_t)T), order(&C.pool, (size_t)T);
   DBuf<i64> cuts(&C.pool, (size_t)T);
   DBuf<BatchGraph> dg(&C.pool, (size_t)T);
-  DBuf<u32> next(&C.pool, 1);
   std::vector<i64> part_off((size_t)T);
   i64 po = 0;
   for (i64 j = 0; j < T; j++) {
@@ -1181,24 +1179,38 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
   std::stable_sort(ord.begin(), ord.end(), [&](i32 a, i32 b) { return hg[a].m > hg[b].m; });
   HIP_TRY(hipMemcpyAsync(dg.p, hg.data(), sizeof(BatchGraph) * T, hipMemcpyHostToDevice, C.s));
   HIP_TRY(hipMemcpyAsync(order.p, ord.data(), sizeof(i32) * T, hipMemcpyHostToDevice, C.s));
-  HIP_TRY(hipMemsetAsync(next.p, 0, sizeof(u32), C.s));
   launch_fill_i32(status.p, T, 1, C.s);
-  // one workgroup per CU, each with scratch for the largest graph
+  // Coarsening: one workgroup per CU; refinement: two per CU; each with
+  // scratch for the largest graph.  The coarse levels of the graphs share a
+  // pool (up to 70% of the free memory); graphs that find it full wait for
+  // the next round.
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
-  const i64 per_wg = batch_scratch_bytes(maxn, maxm);
-  int blocks = (int)std::min<i64>(T, std::max(cus, 1));
-  void* scratch = nullptr;
-  while (blocks > 0) {
-    if (hipMalloc(&scratch, (size_t)(per_wg * blocks)) == hipSuccess) break;
+  cus = std::max(cus, 1);
+  BatchBuffers B{};
+  B.maxn = maxn;
+  B.blocks_c = (int)std::min<i64>(T, cus);
+  B.blocks_r = (int)std::min<i64>(T, 2 * (i64)cus);
+  const i64 bc = batch_coarsen_bytes(maxn), br = batch_refine_bytes(maxn);
+  i64 want = 1 << 24;
+  for (i64 i : take) want += 6 * (8 * items[i].g.m + 16 * (items[i].g.n + 1));
+  size_t freeb = 0, totb = 0;
+  HIP_TRY(hipMemGetInfo(&freeb, &totb));
+  const i64 fixed = bc * B.blocks_c + br * B.blocks_r + batch_record_bytes() * T + 4096;
+  B.pool_cap = std::min<i64>(want, (i64)((double)freeb * 0.7) - fixed);
+  void* mem = nullptr;
+  if (B.pool_cap <= 0 || hipMalloc(&mem, (size_t)(fixed + B.pool_cap)) != hipSuccess) {
     (void)hipGetLastError();
-    scratch = nullptr;
-    blocks /= 2;
-  }
-  if (!scratch) {
     for (i64 i : take) rest.push_back(i);
     return;
   }
+  u8* m8 = (u8*)mem;
+  B.next = (u32*)m8;
+  B.pool_used = (u64*)(m8 + 64);
+  B.recs = m8 + 4096;
+  B.scratch_c = B.recs + ((batch_record_bytes() * T + 255) & ~(i64)255);
+  B.scratch_r = B.scratch_c + bc * B.blocks_c;
+  B.pool = m8 + fixed;
   const i64 budget = C.clock_khz * 2000;  // 2 s per level's matching
   static const bool trace = env_i64("TGP_BATCH_TRACE", 0) != 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1210,12 +1222,33 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, C.s));
   }
-  launch_batch_engine(dg.p, order.p, (i32)T, next.p, (u8*)scratch, maxn, maxm, blocks, budget,
-                      bst.p, C.s);
-  if (trace) HIP_TRY(hipEventRecord(e1, C.s));
   std::vector<i32> hstat((size_t)T), hpc((size_t)T * BATCH_MAXK);
+  std::vector<i32> queue = ord;
+  int rounds = 0;
+  while (!queue.empty()) {
+    rounds++;
+    HIP_TRY(hipMemsetAsync(m8, 0, 4096, C.s));  // queue counters, pool offset
+    if (rounds > 1)
+      HIP_TRY(hipMemcpyAsync(order.p, queue.data(), sizeof(i32) * queue.size(),
+                             hipMemcpyHostToDevice, C.s));
+    launch_batch_engine(dg.p, order.p, (i32)queue.size(), B, budget, bst.p, C.s);
+    HIP_TRY(hipMemcpyAsync(hstat.data(), status.p, sizeof(i32) * T, hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    // graphs that found the pool full go again, with the pool to themselves
+    std::vector<i32> again;
+    for (i32 j : queue)
+      if (hstat[j] == 3) again.push_back(j);
+    for (i32 j : again) hstat[j] = 1;
+    if (again.size() == queue.size()) break;  // no progress: the pool is too small
+    queue.swap(again);
+    if (!queue.empty()) {
+      HIP_TRY(hipMemcpyAsync(status.p, hstat.data(), sizeof(i32) * T, hipMemcpyHostToDevice,
+                             C.s));
+      C.sync();
+    }
+  }
+  if (trace) HIP_TRY(hipEventRecord(e1, C.s));
   std::vector<i64> hcut((size_t)T);
-  HIP_TRY(hipMemcpyAsync(hstat.data(), status.p, sizeof(i32) * T, hipMemcpyDeviceToHost, C.s));
   HIP_TRY(hipMemcpyAsync(hpc.data(), pcs.p, sizeof(i32) * T * BATCH_MAXK, hipMemcpyDeviceToHost,
                          C.s));
   HIP_TRY(hipMemcpyAsync(hcut.data(), cuts.p, sizeof(i64) * T, hipMemcpyDeviceToHost, C.s));
@@ -1225,7 +1258,7 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
                            hipMemcpyDeviceToHost, C.s));
   }
   C.sync();
-  (void)hipFree(scratch);
+  (void)hipFree(mem);
   if (trace) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
@@ -1234,9 +1267,10 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
     i64 back = 0;
     for (i64 j = 0; j < T; j++) back += hstat[j] != 0;
     std::fprintf(stderr,
-                 "# batch engine: %lld graphs, %lld handed back, %d workgroups x %.1f MB, "
-                 "%.2f ms\n",
-                 (long long)T, (long long)back, blocks, per_wg / 1048576.0, ms);
+                 "# batch engine: %lld graphs, %lld handed back, %d + %d workgroups, "
+                 "level pool %.1f GB, %d rounds, %.2f ms\n",
+                 (long long)T, (long long)back, B.blocks_c, B.blocks_r, B.pool_cap / 1e9, rounds,
+                 ms);
     u64 hb[12];
     HIP_TRY(hipMemcpy(hb, bst.p, sizeof(hb), hipMemcpyDeviceToHost));
     const char* nm[10] = {"check", "match", "contract", "gain", "token",

@@ -98,7 +98,8 @@ void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s
 
 // ---- batched engine: one workgroup per graph, the whole partition on device
 // One graph of a batch (device pointers).  status on return: 0 done, 1 the
-// graph needs the multi-kernel path (outside the engine's limits).
+// graph needs the multi-kernel path (outside the engine's limits); 2 marks a
+// coarsened graph between the two launches.
 struct BatchGraph {
   i64 n, m;
   const i64* xadj;
@@ -112,18 +113,29 @@ struct BatchGraph {
   i32* status;
 };
 constexpr int BATCH_MAXK = 16;
-// Scratch bytes one workgroup needs for graphs of at most maxn vertices and
-// maxm arcs (the level arena is sized from maxm; a graph whose coarse levels
-// outgrow it falls back).
-i64 batch_scratch_bytes(i64 maxn, i64 maxm);
-// One persistent launch over `count` graphs taken in the order of `order`;
-// `blocks` workgroups, each with batch_scratch_bytes(maxn, maxm) of scratch.
+// Scratch of one coarsening / refinement workgroup for graphs of at most
+// maxn vertices, and the size of one graph's level record.
+i64 batch_coarsen_bytes(i64 maxn);
+i64 batch_refine_bytes(i64 maxn);
+i64 batch_record_bytes();
+struct BatchBuffers {
+  i64 maxn;
+  u8* scratch_c;  // blocks_c x batch_coarsen_bytes(maxn)
+  int blocks_c;
+  u8* scratch_r;  // blocks_r x batch_refine_bytes(maxn)
+  int blocks_r;
+  u8* pool;       // coarse levels of every graph
+  i64 pool_cap;
+  u64* pool_used; // zero before the launch
+  u8* recs;       // count x batch_record_bytes()
+  u32* next;      // two queue counters, zero before the launch
+};
+// The coarsening launch over the queue, then the refinement launch.
 // stats (optional, 12 u64): wall-clock ticks per phase summed over
 // workgroups (check, match, contract, gain, token, moves, sort, select, cut,
 // misc), then refinement passes and levels built.
-void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count, u32* next,
-                         u8* scratch, i64 maxn, i64 maxm, int blocks, i64 budget,
-                         u64* stats, hipStream_t s);
+void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
+                         const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s);
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
                             i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s);

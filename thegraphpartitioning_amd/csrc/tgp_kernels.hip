@@ -3343,16 +3343,25 @@ void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 
     k_cut_arcs<false><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
 }
 
-i64 batch_scratch_bytes(i64 maxn, i64 maxm) {
+i64 batch_coarsen_bytes(i64 maxn) {
   BeScratch X;
-  return be_layout(maxn, &X, nullptr) + be_arena_bytes(maxn, maxm);
+  return be_layout(maxn, true, &X, nullptr);
 }
+i64 batch_refine_bytes(i64 maxn) {
+  BeScratch X;
+  return be_layout(maxn, false, &X, nullptr);
+}
+i64 batch_record_bytes() { return (i64)sizeof(BeLevRec); }
 
-void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count, u32* next,
-                         u8* scratch, i64 maxn, i64 maxm, int blocks, i64 budget,
-                         u64* stats, hipStream_t s) {
-  k_batch_engine<<<blocks, BE_T, 0, s>>>(graphs, order, count, next, scratch, maxn,
-                                          batch_scratch_bytes(maxn, maxm), budget, stats);
+void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
+                         const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s) {
+  const BePool pool{B.pool, B.pool_cap, B.pool_used};
+  k_batch_coarsen<<<B.blocks_c, BE_T, 0, s>>>(graphs, order, count, B.next, B.scratch_c,
+                                               B.maxn, batch_coarsen_bytes(B.maxn), pool,
+                                               (BeLevRec*)B.recs, budget, stats);
+  k_batch_refine<<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 1, B.scratch_r,
+                                              B.maxn, batch_refine_bytes(B.maxn),
+                                              (const BeLevRec*)B.recs, stats);
 }
 
 }  // namespace tgp
