@@ -147,6 +147,57 @@ def bench_batched(args):
                    "bit_identical_to_solo": f"{same}/{len(sample)}"}}), flush=True)
 
 
+def bench_hbm_scale(args):
+    """The HBM-scale configuration: R-MAT scale 26, edge factor 16, symmetrized
+    (int64 offsets, ~2.1G arcs), int32 weights in [1, 16], k = 256, wide mode,
+    repaired coarsening.  Too large for the oracle, so it reports throughput,
+    per-family algorithmic GB/s, and the size-independent checks: a repeat run
+    is identical, the reported cut equals a host recount, the part counts sum
+    to n."""
+    import numpy as np
+    import thegraphpartitioning_amd as tgp
+    t0 = time.perf_counter()
+    g = tgp.gen_rmat(args.scale, args.edge_factor, SEED + 1, "hash16")
+    print(f"# R-MAT {args.scale}/{args.edge_factor} hash16: n={g.n} m={g.m} "
+          f"(generated in {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    dg = tgp.DeviceGraph(g)
+    kw = dict(init=tgp.INIT_BLOCK, repair=True, wide=True, device_graph=dg.pointers())
+    a = tgp.partition(g, args.k, profile=True, **kw)
+    st = a[3]
+    fam = {name: {"ms": round(ms, 1), "GB/s": round(st.kernel_bytes[name] / ms / 1e6, 1)}
+           for name, ms in st.kernel_ms.items() if ms > 1}
+    print(f"# profiled partition: {st.seconds_total:.2f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = tgp.partition(g, args.k, **kw)
+    secs = time.perf_counter() - t0
+    part = b[0]
+    recount = 0
+    step = 1 << 22
+    for v0 in range(0, g.n, step):
+        v1 = min(g.n, v0 + step)
+        e0, e1 = int(g.xadj[v0]), int(g.xadj[v1])
+        src = np.repeat(np.arange(v0, v1, dtype=np.int64), np.diff(g.xadj[v0:v1 + 1]))
+        diff = part[src] != part[g.adjncy[e0:e1]]
+        recount += int(g.adjwgt[e0:e1][diff].astype(np.int64).sum())
+    dg.free()
+    checks = {"repeat_identical": bool(np.array_equal(a[0], b[0]) and a[2] == b[2] and
+                                       np.array_equal(a[1], b[1])),
+              "cut_equals_recount": b[2] == recount // 2,
+              "counts_sum_to_n": int(b[1].sum()) == g.n,
+              "parts_in_range": bool(part.min() >= 0 and part.max() < args.k)}
+    print(json.dumps({
+        "metric": "M edges/s end-to-end at k=256 (HBM-scale)", "value": round(g.m / secs / 1e6, 3),
+        "unit": "M edges/s", "n_gpus": 1, "seconds": round(secs, 3), "dtype": "int32+fp32",
+        "data": "synthetic",
+        "config": {"workload": f"rmat{args.scale}-ef{args.edge_factor}-hash16-k{args.k}",
+                   "vertices": g.n, "arcs": g.m, "k": args.k, "wide": True,
+                   "coarsening": "repaired", "initial_partition": "block i*k/n_c"},
+        "edge_cut": b[2], "levels": b[3].levels, "passes": b[3].passes, "checks": checks,
+        "kernel_families": fam}), flush=True)
+    if not all(checks.values()):
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +216,8 @@ def main():
                     help="measure the batched form on GRAPHS independent graphs instead")
     ap.add_argument("--workers", type=int, default=16,
                     help="graphs partitioned at once by the batched form's stream workers")
+    ap.add_argument("--hbm-scale", action="store_true",
+                    help="run the HBM-scale configuration (use with --scale 26 --k 256)")
     ap.add_argument("--plain-sample", type=int, default=0, metavar="GRAPHS",
                     help="time the plain path on an evenly spaced sample of this many graphs "
                          "(0: all of them)")
@@ -178,6 +231,9 @@ def main():
         sys.exit(2)
     if args.batched:
         bench_batched(args)
+        return
+    if args.hbm_scale:
+        bench_hbm_scale(args)
         return
 
     import thegraphpartitioning_amd as tgp

@@ -211,3 +211,14 @@ def test_full_size_properties(spec):
     part = a[0]
     assert part.min() >= 0 and part.max() < k
     assert a[2] == cut_from_partition(g, part)
+
+
+@pytest.mark.parametrize("k,wide", [(256, False), (300, True), (700, False), (2000, True)])
+def test_large_k(oracle, k, wide):
+    """k above the byte-partition limit (255), above the kept-tally and
+    small-k gain limit (512), and far above it; int32 edge weights in [1, 16]
+    as in the HBM-scale configuration."""
+    scale = 14 if k <= 512 else 12
+    g = tgp.gen_rmat(scale, 8, 0x7C0DE5EED + k, "hash16")
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, k, repair=True, floor=100, wide=wide)
+    assert_same(gp, op, f"rmat{scale} k={k} wide={wide}")

@@ -840,9 +840,15 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     }
     // Kept tallies (n x k ints) on simple weight-symmetric levels with exact
     // integer tallies: built by the first gain launch after any bulk change of
-    // part, then updated from the applied moves of each commit.
+    // part, then updated from the applied moves of each commit.  The table
+    // may take up to 40% of the free device memory.
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      (void)hipGetLastError();
+      free_b = 0;
+    }
     const bool tmode = k <= 512 && ax.all_exact && lv.simple_ws &&
-                       (double)n * tally_row_stride(k) * 4 <= 8e9 &&
+                       (double)n * tally_row_stride(k) * 4 <= 0.4 * (double)free_b &&
                        env_i64("TGP_KEEP_TALLIES", 1) != 0;
     DBuf<i32> T, app, napp;
     bool T_valid = false;
