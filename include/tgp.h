@@ -179,6 +179,27 @@ int tgp_write_final_partition(const char* path, const int32_t* part, int64_t n, 
 /* METIS text of a CSR (gpu_part.cu:803-853) into a malloc'd buffer.          */
 int tgp_metis_text(const tgp_graph* g, char** text, int64_t* len, int64_t* npairs);
 
+/* ---- input and output formats -------------------------------------------- */
+/* The README form's inputs (README.md:133-141): row.txt and column.txt only,
+ * n = count(row) - 1, m = count(column), unit weights.                      */
+int tgp_read_csr_files(const char* row_path, const char* col_path, tgp_graph* out);
+/* Binary CSR file (no counterpart in the reference): "TGPCSR01", int64 n,
+ * int64 m, int64 xadj[n+1], int32 adjncy[m], int32 adjwgt[m], little-endian. */
+int tgp_write_bcsr(const char* path, const tgp_graph* g);
+int tgp_read_bcsr(const char* path, tgp_graph* out);
+/* METIS-style partition file: one line per vertex with its part (no
+ * counterpart in the reference's outputs, which are finalpartition.txt).     */
+int tgp_write_part_file(const char* path, const int32_t* part, int64_t n);
+/* preprocess/conversion.py:30-110.  tgp_convert_edges reads an edge list by
+ * extension (.mtx/.mmio Matrix Market, arcs column -> row; .tsv; .txt with 4
+ * header lines; .tsv_1 with 1), drops self loops, makes every edge
+ * bidirectional once, removes isolated vertices (order-preserving relabel),
+ * unit weights, rows ascending (DGL's own column order is not specified).
+ * tgp_write_conversion_files writes row.txt, column.txt, weight.txt,
+ * output_output.csr and temp.graph into `dir` in the script's text forms.  */
+int tgp_convert_edges(const char* path, tgp_graph* out);
+int tgp_write_conversion_files(const tgp_graph* g, const char* dir);
+
 /* ---- synthetic workloads (no counterpart in the reference) ---------------- */
 /* weight_mode 0: unit weights; 1: 1 + hash(min,max) mod 16                   */
 int tgp_gen_rmat(int32_t scale, int32_t edge_factor, uint64_t seed, int32_t weight_mode,

@@ -130,3 +130,19 @@ def test_rmat_fixture_digest():
     h = hashlib.sha256(g.xadj.tobytes() + g.adjncy.tobytes() + g.adjwgt.tobytes()).hexdigest()
     golden = open(os.path.join(ROOT, "tests", "golden", "rmat10_ef4.sha256")).read().split()[0]
     assert h == golden
+
+
+def test_cli_other_input_forms_errors(tmp_path):
+    """README form (row file, column file, k) and binary form (graph.bcsr, k):
+    argument and file errors are reported before any device work."""
+    row = tmp_path / "row.txt"
+    row.write_text("0\n1\n2\n")
+    r = run_cli([str(row), str(tmp_path / "nope.txt"), "2"], str(tmp_path))
+    assert r.returncode == 1 and "Error opening file" in r.stderr
+    r = run_cli([str(row), str(row), "0"], str(tmp_path))
+    assert r.returncode == 1
+    assert r.stderr.strip() == "Invalid input: nvtxs, nedges, and numPartitions must be positive."
+    r = run_cli([str(tmp_path / "g.bcsr"), "2"], str(tmp_path))
+    assert r.returncode == 1 and "Error opening file" in r.stderr
+    r = run_cli([str(row), str(row), "2", "--bogus"], str(tmp_path))
+    assert r.returncode == 1 and "Unknown option: --bogus" in r.stderr

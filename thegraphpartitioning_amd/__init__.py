@@ -104,6 +104,18 @@ def lib():
     L.tgp_gen_grid3d.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(CGraph)]
     L.tgp_gen_er.argtypes = [C.c_int64, C.c_double, C.c_uint64, C.POINTER(CGraph)]
     L.tgp_gen_rmat_n.argtypes = [C.c_int64, C.c_int32, C.c_uint64, C.POINTER(CGraph)]
+    L.tgp_read_csr_files.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(CGraph)]
+    L.tgp_read_csr_files.restype = C.c_int
+    L.tgp_write_bcsr.argtypes = [C.c_char_p, C.POINTER(CGraph)]
+    L.tgp_write_bcsr.restype = C.c_int
+    L.tgp_read_bcsr.argtypes = [C.c_char_p, C.POINTER(CGraph)]
+    L.tgp_read_bcsr.restype = C.c_int
+    L.tgp_write_part_file.argtypes = [C.c_char_p, P, C.c_int64]
+    L.tgp_write_part_file.restype = C.c_int
+    L.tgp_convert_edges.argtypes = [C.c_char_p, C.POINTER(CGraph)]
+    L.tgp_convert_edges.restype = C.c_int
+    L.tgp_write_conversion_files.argtypes = [C.POINTER(CGraph), C.c_char_p]
+    L.tgp_write_conversion_files.restype = C.c_int
     L.tgp_partition_batch.argtypes = [C.POINTER(CBatchItem), C.c_int64, C.c_int32]
     L.tgp_partition_batch.restype = C.c_int
     L.tgp_read_graph_dir.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(CGraph)]
@@ -207,6 +219,55 @@ def read_graph_dir(path, n, m):
     if rc != 0:
         raise TGPError(rc, lib().tgp_last_error().decode())
     return _take_cgraph(cg)
+
+
+def read_csr_files(row_path, col_path):
+    """The README form's inputs: row.txt and column.txt, unit weights."""
+    cg = CGraph()
+    rc = lib().tgp_read_csr_files(os.fsencode(row_path), os.fsencode(col_path), C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc, lib().tgp_last_error().decode())
+    return _take_cgraph(cg)
+
+
+def write_bcsr(path, g):
+    """Binary CSR file ("TGPCSR01", n, m, xadj, adjncy, adjwgt)."""
+    rc = lib().tgp_write_bcsr(os.fsencode(path), C.byref(g.cgraph()))
+    if rc != 0:
+        raise TGPError(rc, lib().tgp_last_error().decode())
+
+
+def read_bcsr(path):
+    cg = CGraph()
+    rc = lib().tgp_read_bcsr(os.fsencode(path), C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc, lib().tgp_last_error().decode())
+    return _take_cgraph(cg)
+
+
+def write_part_file(path, part):
+    """METIS-style partition file: one part id per line, vertex order."""
+    p = np.ascontiguousarray(part, dtype=np.int32)
+    rc = lib().tgp_write_part_file(os.fsencode(path), p.ctypes.data, len(p))
+    if rc != 0:
+        raise TGPError(rc, lib().tgp_last_error().decode())
+
+
+def convert_edges(path):
+    """An edge-list file as conversion.py reads it (.mtx/.mmio, .tsv, .txt,
+    .tsv_1), made simple, bidirected and free of isolated vertices."""
+    cg = CGraph()
+    rc = lib().tgp_convert_edges(os.fsencode(path), C.byref(cg))
+    if rc != 0:
+        raise TGPError(rc, f"cannot convert {path}")
+    return _take_cgraph(cg)
+
+
+def write_conversion_files(g, out_dir="."):
+    """row.txt, column.txt, weight.txt, output_output.csr and temp.graph."""
+    rc = lib().tgp_write_conversion_files(C.byref(g.cgraph()), os.fsencode(out_dir))
+    if rc != 0:
+        raise TGPError(rc, f"cannot write into {out_dir}")
 
 
 def write_final_partition(path, part, k):

@@ -8,13 +8,21 @@
 // reference's seam), then ./finalpartition.txt and the same report lines.
 // Errors print the reference's messages and exit with status 1.
 //
-// Options after the four positional arguments (none exist in the reference):
-//   --init=block   deterministic i*k/n_c initial partition instead of gpmetis
-//   --repair       opt-in repaired coarsening (tgp_options.repair)
-//   --wide         64-bit cut and no n*k < 2^31 limit (tgp_options.wide)
+// Two more input forms (no counterpart in the reference's code):
+//   gpu_part <row.txt> <column.txt> <num_partitions> [options]   the README
+//       form (README.md:133-141): sizes from the files, unit weights
+//   gpu_part <graph.bcsr> <num_partitions> [options]             binary CSR
+//
+// Options after the positional arguments (none exist in the reference):
+//   --init=block        deterministic i*k/n_c initial partition instead of gpmetis
+//   --repair            opt-in repaired coarsening (tgp_options.repair)
+//   --wide              64-bit cut and no n*k < 2^31 limit (tgp_options.wide)
+//   --part-out=<path>   also write a METIS-style per-vertex part file
+//   --write-bcsr=<path> also save the input graph as a binary CSR file
 #include "../../include/tgp.h"
 
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <chrono>
@@ -26,39 +34,75 @@
 #include <string>
 #include <vector>
 
+static bool ends_with(const std::string& s, const std::string& t) {
+  return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
+}
+
 int main(int argc, char* argv[]) {
-  if (argc < 5) {  // gpu_part.cu:880-883
+  // which input form: binary CSR file, README row/column pair, or directory
+  int form = 0;  // 0 directory, 1 README pair, 2 binary
+  if (argc >= 3 && ends_with(argv[1], ".bcsr")) {
+    form = 2;
+  } else if (argc >= 4) {
+    struct stat sb;
+    if (stat(argv[1], &sb) == 0 && S_ISREG(sb.st_mode)) form = 1;
+  }
+  const int npos = form == 0 ? 4 : form == 1 ? 3 : 2;
+  if (form == 0 && argc < 5) {  // gpu_part.cu:880-883
     std::cerr << "Usage: " << argv[0] << " <directory> <nvtxs> <nedges> <num_partitions>\n";
     return 1;
   }
   bool wide = false, repair = false, block = false;
-  for (int i = 5; i < argc; i++) {
+  std::string part_out, bcsr_out;
+  for (int i = npos + 1; i < argc; i++) {
     if (!std::strcmp(argv[i], "--wide")) wide = true;
     else if (!std::strcmp(argv[i], "--repair")) repair = true;
     else if (!std::strcmp(argv[i], "--init=block")) block = true;
     else if (!std::strcmp(argv[i], "--init=gpmetis")) block = false;
+    else if (!std::strncmp(argv[i], "--part-out=", 11)) part_out = argv[i] + 11;
+    else if (!std::strncmp(argv[i], "--write-bcsr=", 13)) bcsr_out = argv[i] + 13;
     else {
       std::cerr << "Unknown option: " << argv[i] << "\n";
       return 1;
     }
   }
-  const char* dir = argv[1];
-  const long long nv = wide ? std::atoll(argv[2]) : (long long)std::atoi(argv[2]);
-  const long long ne = wide ? std::atoll(argv[3]) : (long long)std::atoi(argv[3]);
-  const int k = std::atoi(argv[4]);
-  if (nv <= 0 || ne <= 0 || k <= 0) {  // gpu_part.cu:891-894
-    std::cerr << "Invalid input: nvtxs, nedges, and numPartitions must be positive.\n";
-    return 1;
-  }
-  const long long target_vertices = std::max<long long>(500, nv / 500);  // gpu_part.cu:897
-
+  long long nv = 0, ne = 0;
+  int k = 0;
   auto t0 = std::chrono::steady_clock::now();
   tgp_graph g;
-  int rc = tgp_read_graph_dir(dir, nv, ne, &g);
+  int rc;
+  if (form == 0) {
+    nv = wide ? std::atoll(argv[2]) : (long long)std::atoi(argv[2]);
+    ne = wide ? std::atoll(argv[3]) : (long long)std::atoi(argv[3]);
+    k = std::atoi(argv[4]);
+    if (nv <= 0 || ne <= 0 || k <= 0) {  // gpu_part.cu:891-894
+      std::cerr << "Invalid input: nvtxs, nedges, and numPartitions must be positive.\n";
+      return 1;
+    }
+    rc = tgp_read_graph_dir(argv[1], nv, ne, &g);
+  } else {
+    k = std::atoi(argv[npos]);
+    if (k <= 0) {
+      std::cerr << "Invalid input: nvtxs, nedges, and numPartitions must be positive.\n";
+      return 1;
+    }
+    rc = form == 1 ? tgp_read_csr_files(argv[1], argv[2], &g) : tgp_read_bcsr(argv[1], &g);
+    if (rc == TGP_OK) {
+      nv = g.n;
+      ne = g.m;
+    }
+  }
   if (rc != TGP_OK) {
     std::cerr << tgp_last_error() << "\n";
     return 1;
   }
+  if (!bcsr_out.empty() && tgp_write_bcsr(bcsr_out.c_str(), &g) != TGP_OK) {
+    std::cerr << tgp_last_error() << "\n";
+    tgp_graph_free(&g);
+    return 1;
+  }
+  const long long target_vertices = std::max<long long>(500, nv / 500);  // gpu_part.cu:897
+
   const double t_read = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
   tgp_options o;
@@ -83,6 +127,11 @@ int main(int argc, char* argv[]) {
   rc = tgp_write_final_partition("./finalpartition.txt", part.data(), nv, k);
   if (rc != TGP_OK) {
     std::cerr << "Error: Cannot open finalpartition.txt for writing\n";
+    tgp_graph_free(&g);
+    return 1;
+  }
+  if (!part_out.empty() && tgp_write_part_file(part_out.c_str(), part.data(), nv) != TGP_OK) {
+    std::cerr << tgp_last_error() << "\n";
     tgp_graph_free(&g);
     return 1;
   }

@@ -22,7 +22,9 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -247,6 +249,182 @@ int tgp_gen_er(int64_t n, double mean_degree, uint64_t seed, tgp_graph* out) {
     }
   }
   return build_csr(n, src, dst, 0, seed, out);
+}
+
+// ---- edge-list files in the forms preprocess/conversion.py reads (30-58):
+// .mtx/.mmio (Matrix Market coordinate, 1-based; the graph's arcs run from
+// column to row, conversion.py:33-37), .tsv (source, dest, data), .txt (four
+// header lines skipped), .tsv_1 (one header line skipped).  Then 66-69:
+// self loops dropped, every edge in both directions once, isolated vertices
+// removed with an order-preserving relabel; unit weights.  Rows list their
+// neighbours in ascending order (DGL's own order is not specified).
+int tgp_convert_edges(const char* path, tgp_graph* out) {
+  if (!path || !out) return TGP_EINVAL;
+  const std::string p(path);
+  const size_t dot = p.find_last_of('.');
+  const std::string ext = dot == std::string::npos ? "" : p.substr(dot);
+  int skip = 0;
+  bool mtx = false;
+  if (ext == ".mtx" || ext == ".mmio") mtx = true;
+  else if (ext == ".tsv") skip = 0;
+  else if (ext == ".txt") skip = 4;
+  else if (ext == ".tsv_1") skip = 1;
+  else return TGP_EINVAL;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return TGP_EIO;
+  std::vector<char> text;
+  {
+    std::vector<char> buf(1 << 22);
+    size_t len;
+    while ((len = std::fread(buf.data(), 1, buf.size(), f)) > 0)
+      text.insert(text.end(), buf.begin(), buf.begin() + (long)len);
+  }
+  std::fclose(f);
+  text.push_back('\n');
+  std::vector<int64_t> es, ed;  // edges source -> destination
+  size_t pos = 0;
+  const size_t end = text.size();
+  auto line_end = [&](size_t q) {
+    while (q < end && text[q] != '\n') q++;
+    return q;
+  };
+  auto parse_int = [&](size_t& q, size_t le, int64_t& v) {
+    while (q < le && (text[q] == ' ' || text[q] == '\t' || text[q] == '\r' || text[q] == ',')) q++;
+    if (q >= le || text[q] < '0' || text[q] > '9') return false;
+    v = 0;
+    while (q < le && text[q] >= '0' && text[q] <= '9') v = v * 10 + (text[q++] - '0');
+    return true;
+  };
+  for (int s = 0; s < skip && pos < end; s++) pos = line_end(pos) + 1;
+  bool size_line = mtx;  // Matrix Market: the first non-comment line is "rows cols nnz"
+  while (pos < end) {
+    const size_t le = line_end(pos);
+    size_t q = pos;
+    while (q < le && (text[q] == ' ' || text[q] == '\t' || text[q] == '\r')) q++;
+    const bool blank = q >= le;
+    const bool comment = mtx && q < le && text[q] == '%';
+    if (!blank && !comment) {
+      if (size_line) {
+        size_line = false;
+      } else {
+        int64_t a, b;
+        if (!parse_int(q, le, a) || !parse_int(q, le, b)) return TGP_EINVAL;
+        if (mtx) {
+          if (a < 1 || b < 1) return TGP_EINVAL;
+          es.push_back(b - 1);  // column -> row
+          ed.push_back(a - 1);
+        } else {
+          es.push_back(a);
+          ed.push_back(b);
+        }
+      }
+    }
+    pos = le + 1;
+  }
+  std::vector<char>().swap(text);
+  // vertices that keep an edge once self loops are gone, in id order
+  std::vector<int64_t> ids;
+  ids.reserve(2 * es.size());
+  for (size_t i = 0; i < es.size(); i++)
+    if (es[i] != ed[i]) {
+      ids.push_back(es[i]);
+      ids.push_back(ed[i]);
+    }
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  if (ids.empty() || (int64_t)ids.size() > INT32_MAX) return TGP_EINVAL;
+  auto rank = [&](int64_t x) {
+    return (int32_t)(std::lower_bound(ids.begin(), ids.end(), x) - ids.begin());
+  };
+  std::vector<int32_t> src, dst;
+  src.reserve(2 * es.size());
+  dst.reserve(2 * es.size());
+  for (size_t i = 0; i < es.size(); i++) {
+    if (es[i] == ed[i]) continue;
+    const int32_t u = rank(es[i]), v = rank(ed[i]);
+    src.push_back(u);
+    dst.push_back(v);
+    src.push_back(v);
+    dst.push_back(u);
+  }
+  std::vector<int64_t>().swap(es);
+  std::vector<int64_t>().swap(ed);
+  return build_csr((int64_t)ids.size(), src, dst, 0, 0, out);
+}
+
+// The files conversion.py writes (71-110): row.txt, column.txt, weight.txt
+// (all ones), output_output.csr and temp.graph (METIS text, edges counted
+// once, unit weights), into directory `dir`.
+int tgp_write_conversion_files(const tgp_graph* g, const char* dir) {
+  if (!g || !dir || g->n <= 0) return TGP_EINVAL;
+  const std::string d(dir);
+  const int64_t n = g->n, m = g->m;
+  auto emit = [&](const std::string& name, auto body) {
+    FILE* f = std::fopen((d + "/" + name).c_str(), "wb");
+    if (!f) return false;
+    std::string buf;
+    auto flush = [&](bool force) {
+      if (force || buf.size() > (1 << 22)) {
+        std::fwrite(buf.data(), 1, buf.size(), f);
+        buf.clear();
+      }
+    };
+    body(buf, flush);
+    flush(true);
+    return std::fclose(f) == 0;
+  };
+  char num[32];
+  auto put = [&](std::string& buf, long long v, const char* sep) {
+    const int len = std::snprintf(num, sizeof(num), "%lld%s", v, sep);
+    buf.append(num, (size_t)len);
+  };
+  bool ok = emit("row.txt", [&](std::string& b, auto& fl) {
+    for (int64_t i = 0; i <= n; i++) {
+      put(b, g->xadj[i], "\n");
+      fl(false);
+    }
+  });
+  ok = ok && emit("column.txt", [&](std::string& b, auto& fl) {
+    for (int64_t i = 0; i < m; i++) {
+      put(b, g->adjncy[i], "\n");
+      fl(false);
+    }
+  });
+  ok = ok && emit("weight.txt", [&](std::string& b, auto& fl) {
+    for (int64_t i = 0; i < m; i++) {
+      b += "1\n";
+      fl(false);
+    }
+  });
+  ok = ok && emit("output_output.csr", [&](std::string& b, auto& fl) {
+    put(b, n, " ");
+    put(b, m, " ");
+    put(b, n + 1, " ");
+    put(b, m, "\n");
+    for (int64_t i = 0; i <= n; i++) {
+      put(b, g->xadj[i], " ");
+      fl(false);
+    }
+    b += "\n";
+    for (int64_t i = 0; i < m; i++) {
+      put(b, g->adjncy[i], " ");
+      fl(false);
+    }
+  });
+  ok = ok && emit("temp.graph", [&](std::string& b, auto& fl) {
+    put(b, n, " ");
+    put(b, m / 2, " ");
+    b += "1\n";
+    for (int64_t i = 0; i < n; i++) {
+      for (int64_t j = g->xadj[i]; j < g->xadj[i + 1]; j++) {
+        put(b, (long long)g->adjncy[j] + 1, " 1");
+        if (j + 1 < g->xadj[i + 1]) b += ' ';
+      }
+      b += '\n';
+      fl(false);
+    }
+  });
+  return ok ? TGP_OK : TGP_EIO;
 }
 
 void tgp_graph_free(tgp_graph* g) {

@@ -1,5 +1,8 @@
 // tgp_host.cpp -- host side of the reference's file contract:
 //   row.txt / column.txt / weight.txt readers (gpu_part.cu:786-800, 943-945),
+//   the README form's row/column pair with unit weights (README.md:133-141),
+//   a binary CSR file and a per-vertex part file (no counterpart in the
+//   reference),
 //   the METIS text of the coarsest graph and the gpmetis seam
 //   (gpu_part.cu:803-864, 1157-1174), finalpartition.txt (gpu_part.cu:1294-1321).
 #include "../../include/tgp.h"
@@ -129,6 +132,159 @@ int tgp_read_graph_dir(const char* dir, int64_t n, int64_t m, tgp_graph* out) {
   out->adjncy = adj;
   out->adjwgt = w;
   return TGP_OK;
+}
+
+// Counts the whitespace-separated decimal ints of a file (stops at the first
+// token that is not one, as the reference's fscanf loop would).
+static int count_ints(const std::string& path, i64* count) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) {
+    tgp::last_error() = "Error opening file: " + path + ": " + std::strerror(errno);
+    return TGP_EIO;
+  }
+  std::vector<char> buf(1 << 20);
+  i64 c = 0;
+  bool in = false, ok = true;
+  size_t len;
+  while (ok && (len = std::fread(buf.data(), 1, buf.size(), f)) > 0) {
+    for (size_t i = 0; i < len; i++) {
+      const char ch = buf[i];
+      const bool digit = (ch >= '0' && ch <= '9') || ch == '-' || ch == '+';
+      const bool space = ch == ' ' || ch == '\n' || ch == '\t' || ch == '\r' || ch == '\v' ||
+                         ch == '\f';
+      if (digit) {
+        if (!in) c++;
+        in = true;
+      } else if (space) {
+        in = false;
+      } else {
+        ok = false;
+        break;
+      }
+    }
+  }
+  std::fclose(f);
+  *count = c;
+  return TGP_OK;
+}
+
+int tgp_read_csr_files(const char* row_path, const char* col_path, tgp_graph* out) {
+  if (!row_path || !col_path || !out) return TGP_EINVAL;
+  i64 nr = 0, m = 0;
+  int rc = count_ints(row_path, &nr);
+  if (rc == TGP_OK) rc = count_ints(col_path, &m);
+  if (rc != TGP_OK) return rc;
+  const i64 n = nr - 1;
+  if (n <= 0 || m <= 0) {
+    tgp::last_error() = "empty row or column file";
+    return TGP_EINVAL;
+  }
+  std::vector<i32> row((size_t)nr);
+  int64_t* xadj = (int64_t*)std::malloc(sizeof(int64_t) * (n + 1));
+  int32_t* adj = (int32_t*)std::malloc(sizeof(int32_t) * m);
+  int32_t* w = (int32_t*)std::malloc(sizeof(int32_t) * m);
+  if (!xadj || !adj || !w) {
+    std::free(xadj);
+    std::free(adj);
+    std::free(w);
+    return TGP_ENOMEM;
+  }
+  rc = read_file(row_path, row.data(), nr);
+  if (rc == TGP_OK) rc = read_file(col_path, adj, m);
+  if (rc != TGP_OK) {
+    std::free(xadj);
+    std::free(adj);
+    std::free(w);
+    return rc;
+  }
+  for (i64 i = 0; i <= n; i++) xadj[i] = row[i];
+  for (i64 i = 0; i < m; i++) w[i] = 1;
+  out->n = n;
+  out->m = m;
+  out->xadj = xadj;
+  out->adjncy = adj;
+  out->adjwgt = w;
+  return TGP_OK;
+}
+
+// Binary CSR: "TGPCSR01", int64 n, int64 m, int64 xadj[n+1], int32 adj[m],
+// int32 w[m], little-endian.
+static const char BCSR_MAGIC[8] = {'T', 'G', 'P', 'C', 'S', 'R', '0', '1'};
+
+int tgp_write_bcsr(const char* path, const tgp_graph* g) {
+  if (!path || !g || g->n <= 0 || g->m < 0) return TGP_EINVAL;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    tgp::last_error() = std::string("Error opening file: ") + path + ": " + std::strerror(errno);
+    return TGP_EIO;
+  }
+  const int64_t hdr[2] = {g->n, g->m};
+  bool ok = std::fwrite(BCSR_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr, 8, 2, f) == 2 &&
+            std::fwrite(g->xadj, 8, (size_t)g->n + 1, f) == (size_t)g->n + 1 &&
+            std::fwrite(g->adjncy, 4, (size_t)g->m, f) == (size_t)g->m &&
+            std::fwrite(g->adjwgt, 4, (size_t)g->m, f) == (size_t)g->m;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? TGP_OK : TGP_EIO;
+}
+
+int tgp_read_bcsr(const char* path, tgp_graph* out) {
+  if (!path || !out) return TGP_EINVAL;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) {
+    tgp::last_error() = std::string("Error opening file: ") + path + ": " + std::strerror(errno);
+    return TGP_EIO;
+  }
+  char magic[8];
+  int64_t hdr[2] = {0, 0};
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, BCSR_MAGIC, 8) != 0 ||
+      std::fread(hdr, 8, 2, f) != 2 || hdr[0] <= 0 || hdr[1] < 0) {
+    std::fclose(f);
+    tgp::last_error() = std::string("not a binary CSR file: ") + path;
+    return TGP_EIO;
+  }
+  const int64_t n = hdr[0], m = hdr[1];
+  int64_t* xadj = (int64_t*)std::malloc(sizeof(int64_t) * (n + 1));
+  int32_t* adj = (int32_t*)std::malloc(sizeof(int32_t) * std::max<int64_t>(m, 1));
+  int32_t* w = (int32_t*)std::malloc(sizeof(int32_t) * std::max<int64_t>(m, 1));
+  const bool ok = xadj && adj && w &&
+                  std::fread(xadj, 8, (size_t)n + 1, f) == (size_t)n + 1 &&
+                  std::fread(adj, 4, (size_t)m, f) == (size_t)m &&
+                  std::fread(w, 4, (size_t)m, f) == (size_t)m;
+  std::fclose(f);
+  if (!ok) {
+    std::free(xadj);
+    std::free(adj);
+    std::free(w);
+    tgp::last_error() = std::string("truncated binary CSR file: ") + path;
+    return TGP_EIO;
+  }
+  out->n = n;
+  out->m = m;
+  out->xadj = xadj;
+  out->adjncy = adj;
+  out->adjwgt = w;
+  return TGP_OK;
+}
+
+int tgp_write_part_file(const char* path, const int32_t* part, int64_t n) {
+  if (!path || !part || n <= 0) return TGP_EINVAL;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    tgp::last_error() = std::string("Error opening file: ") + path + ": " + std::strerror(errno);
+    return TGP_EIO;
+  }
+  std::string buf;
+  char num[16];
+  for (i64 i = 0; i < n; i++) {
+    const int len = std::snprintf(num, sizeof(num), "%d\n", part[i]);
+    buf.append(num, (size_t)len);
+    if (buf.size() > (1 << 20)) {
+      std::fwrite(buf.data(), 1, buf.size(), f);
+      buf.clear();
+    }
+  }
+  std::fwrite(buf.data(), 1, buf.size(), f);
+  return std::fclose(f) == 0 ? TGP_OK : TGP_EIO;
 }
 
 int tgp_write_final_partition(const char* path, const int32_t* part, int64_t n, int32_t k) {
