@@ -91,7 +91,38 @@ typedef struct tgp_options {
                               coarsening stops once a level keeps more than 95%
                               of its vertices (20*n_c > 19*n).  Refinement is
                               unchanged.                                        */
+  /* Optional refinement trace (no counterpart in the reference): called on the
+   * host with the partition of a level at the start of each phase (pass = -1)
+   * and after each committed pass; used to record the training data of the
+   * learned move proposer (tgp_learn.h).  Slow: a copy per pass.            */
+  void (*pass_dump)(int32_t level, int32_t phase, int32_t pass, int64_t n,
+                    const int32_t* part, void* user);
+  void* pass_dump_user;
+  /* Optional learned move proposer (tgp_learned_model; no counterpart in the
+   * reference): when set, the finest level runs learned passes before its
+   * refinement (see tgp_learned_model).                                      */
+  const struct tgp_learned_model* learned;
 } tgp_options;
+
+/* The learned move proposer's weights (device memory) for the GNN of
+ * tgp_learn.h.  Each learned pass at the finest level: features -> GNN ->
+ * for each vertex its best-scoring other part q, admitted only when the exact
+ * integer gain T_v[q] - T_v[part_v] is positive; the reference's token rule
+ * on those gains, move buffer, capacity (gpu_part.cu:316-369), sort, select
+ * and commit (472-397).  Learned passes stop when fewer than min_admit * n
+ * moves are admitted (or after max_passes); the level's ordinary refinement
+ * follows (the fallback).                                                   */
+typedef struct tgp_learned_model {
+  int32_t k, features, hidden, max_passes;
+  const uint16_t* W1a;  /* features x hidden, BF16 */
+  const uint16_t* W1b;  /* features x hidden */
+  const uint16_t* W2a;  /* hidden x k */
+  const uint16_t* W2b;  /* hidden x k */
+  const float* b1;      /* hidden, FP32 */
+  const float* b2;      /* k */
+  float min_admit;
+  int32_t reserved;
+} tgp_learned_model;
 
 /* trace record kinds */
 enum { TGP_TRACE_LEVEL = 1, TGP_TRACE_INIT = 2, TGP_TRACE_PASS = 3, TGP_TRACE_REFINE = 4,

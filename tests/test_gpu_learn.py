@@ -1,0 +1,202 @@
+"""Learned move proposer: its kernels against a plain PyTorch FP32 reference of
+the same operations (on BF16-rounded inputs, since the kernels take BF16), the
+proposal rule against a NumPy restatement, and the learned refinement's output
+against the size-independent checks (parts in range, counts summing to n, the
+reported cut equal to a recount, repeat runs identical).
+
+Tolerances: products of BF16 values are exact in FP32, so GEMM results differ
+from the reference only by FP32 summation order (rtol 1e-4 of the row scale);
+outputs written back as BF16 carry BF16 rounding (rtol 2^-8).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import thegraphpartitioning_amd as tgp
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+learn = pytest.importorskip("thegraphpartitioning_amd.learn")
+
+
+def bf16_round(x):
+    return torch.as_tensor(x, dtype=torch.float32).to(torch.bfloat16).to(torch.float32)
+
+
+def stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("shape", [(100, 70, 45, 0, 0), (64, 64, 64, 1, 0), (37, 130, 200, 0, 1),
+                                   (300, 16, 96, 1, 1)])
+def test_gemm_matches_fp32(shape):
+    M, N, K, ta, tb = shape
+    g = torch.Generator().manual_seed(M * N + K)
+    A = bf16_round(torch.randn(K, M, generator=g) if ta else torch.randn(M, K, generator=g))
+    B = bf16_round(torch.randn(N, K, generator=g) if tb else torch.randn(K, N, generator=g))
+    Cin = torch.randn(M, N, generator=g)
+    bias = torch.randn(N, generator=g)
+    ref = (A.t() if ta else A) @ (B.t() if tb else B)
+    ref = torch.relu(0.5 * ref + 2.0 * Cin + bias)
+    dA = A.to(torch.bfloat16).cuda()
+    dB = B.to(torch.bfloat16).cuda()
+    dC = Cin.clone().cuda()
+    dCb = torch.empty(M, N, dtype=torch.bfloat16, device="cuda")
+    L = learn._lib()
+    rc = L.tgpl_gemm(learn._p(dA), dA.shape[1], ta, learn._p(dB), dB.shape[1], tb, learn._p(dC), N,
+                     learn._p(dCb), M, N, K, 0.5, 2.0, learn._p(bias.cuda()), 1, stream())
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = dC.cpu()
+    scale = ref.abs().max().item() + 1.0
+    assert (got - ref).abs().max().item() <= 1e-4 * scale * K ** 0.5
+    assert torch.allclose(dCb.cpu().float(), got, rtol=2 ** -8, atol=1e-6)
+
+
+def small_graph():
+    return tgp.gen_rmat(9, 8, 5, "hash16")
+
+
+def test_mean_aggregation_and_adjoint():
+    g = small_graph()
+    dg = learn.DevGraph(g)
+    F = 48
+    X = bf16_round(torch.randn(g.n, F, generator=torch.Generator().manual_seed(1)))
+    deg = np.diff(g.xadj)
+    src = np.repeat(np.arange(g.n), deg)
+    A = torch.zeros(g.n, g.n)
+    A[torch.from_numpy(src), torch.from_numpy(g.adjncy.astype(np.int64))] = 1.0
+    D = torch.from_numpy(np.maximum(deg, 1).astype(np.float32))
+    fwd = (A @ X) / D[:, None]
+    adj = A.t() @ (X / D[:, None])
+    L = learn._lib()
+    for transpose, ref in ((0, fwd), (1, adj)):
+        out = torch.empty(g.n, F, dtype=torch.bfloat16, device="cuda")
+        outf = torch.empty(g.n, F, device="cuda")
+        assert L.tgpl_mean_agg(g.n, learn._p(dg.xadj), learn._p(dg.adj),
+                               learn._p(X.to(torch.bfloat16).cuda()), F, transpose,
+                               learn._p(out), learn._p(outf), stream()) == 0
+        torch.cuda.synchronize()
+        assert torch.allclose(outf.cpu(), ref, rtol=1e-5, atol=1e-5)
+
+
+def test_features_and_proposals():
+    g = small_graph()
+    k = 8
+    dg = learn.DevGraph(g)
+    part = (np.arange(g.n) * k // g.n).astype(np.int32)
+    F = learn.feature_count(k)
+    X = torch.empty(g.n, F, dtype=torch.bfloat16, device="cuda")
+    tally = torch.empty(g.n, k, dtype=torch.int32, device="cuda")
+    dpart = torch.from_numpy(part).cuda()
+    L = learn._lib()
+    assert L.tgpl_features(g.n, learn._p(dg.xadj), learn._p(dg.adj), learn._p(dg.w),
+                           learn._p(dpart), k, F, learn._p(X), learn._p(tally), stream()) == 0
+    torch.cuda.synchronize()
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj))
+    T = np.zeros((g.n, k), np.int64)
+    np.add.at(T, (src, part[g.adjncy]), g.adjwgt)
+    np.testing.assert_array_equal(tally.cpu().numpy(), T)
+    deg = T.sum(1)
+    x = X.cpu().float().numpy()
+    np.testing.assert_allclose(x[:, :k], bf16_round(T / np.maximum(deg, 1)[:, None]).numpy())
+    np.testing.assert_array_equal(x[:, k:2 * k], np.eye(k)[part])
+    assert (x[:, 2 * k + 1] == 1).all() and (x[:, 2 * k + 2:] == 0).all()
+    # proposals: best other part by logit (lowest on ties), admitted only with a
+    # positive exact gain and a logit above the own part's
+    Z = torch.randn(g.n, k, generator=torch.Generator().manual_seed(4))
+    Z[:5] = 0.0
+    gain = torch.empty(g.n, device="cuda")
+    tgt = torch.empty(g.n, dtype=torch.int32, device="cuda")
+    assert L.tgpl_propose(g.n, k, learn._p(Z.cuda()), learn._p(dpart), learn._p(tally),
+                          learn._p(gain), learn._p(tgt), stream()) == 0
+    torch.cuda.synchronize()
+    Zn = Z.numpy()
+    for v in range(g.n):
+        cur = part[v]
+        order = [q for q in range(k) if q != cur]
+        best = max(order, key=lambda q: (Zn[v, q], -q))
+        d = T[v, best] - T[v, cur]
+        ok = Zn[v, best] > Zn[v, cur] and d > 0
+        assert tgt[v].item() == best
+        gv = gain[v].item()
+        assert (gv == d) if ok else np.isnan(gv)
+
+
+def reference_forward(model, g, part):
+    """The same GNN in PyTorch FP32 on CPU, BF16 rounding where the kernels
+    store BF16 (features, aggregates, hidden activations, weights)."""
+    k, F = model.k, model.F
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj))
+    T = np.zeros((g.n, k))
+    np.add.at(T, (src, part[g.adjncy]), g.adjwgt)
+    deg = T.sum(1)
+    X = np.zeros((g.n, F), np.float32)
+    X[:, :k] = T / np.maximum(deg, 1)[:, None]
+    X[:, k:2 * k] = np.eye(k)[part]
+    X[:, 2 * k] = np.log2(1 + np.diff(g.xadj)) / 16
+    X[:, 2 * k + 1] = 1
+    X = bf16_round(X)
+    A = torch.zeros(g.n, g.n)
+    A[torch.from_numpy(src), torch.from_numpy(g.adjncy.astype(np.int64))] = 1.0
+    D = torch.from_numpy(np.maximum(np.diff(g.xadj), 1).astype(np.float32))[:, None]
+    W = {n: w.cpu() for n, w in model.W.items()}
+    Wb = {n: bf16_round(w) for n, w in W.items()}
+    M0 = bf16_round((A @ X) / D)
+    H = bf16_round(torch.relu(X @ Wb["W1a"] + M0 @ Wb["W1b"] + W["b1"]))
+    M1 = bf16_round((A @ H) / D)
+    return H @ Wb["W2a"] + M1 @ Wb["W2b"] + W["b2"]
+
+
+def test_forward_matches_reference():
+    g = small_graph()
+    k = 8
+    model = learn.GNN(k, seed=3)
+    part = (np.arange(g.n) * k // g.n).astype(np.int32)
+    Z, _ = model.forward(learn.DevGraph(g), torch.from_numpy(part).cuda())
+    ref = reference_forward(model, g, part)
+    torch.cuda.synchronize()
+    assert torch.allclose(Z.cpu(), ref, rtol=2e-2, atol=2e-2)
+
+
+def test_training_reduces_loss_on_plain_traces():
+    gs = [tgp.gen_rmat(11, 8, 30 + i) for i in range(2)]
+    k = 8
+    data = learn.collect_traces(gs, k, init=tgp.INIT_BLOCK, repair=True, coarsen_floor=100)
+    assert all(len(p) > 0 for _, p in data)
+    model = learn.GNN(k, seed=0)
+    dgs = [learn.DevGraph(g) for g, _ in data]
+    g0, pairs0 = data[0]
+    b0 = torch.from_numpy(pairs0[0][0]).cuda()
+    a0 = torch.from_numpy(pairs0[0][1]).cuda()
+    before = model.evaluate(dgs[0], b0, a0)[0]
+    for _ in range(15):
+        for dg, (g, pairs) in zip(dgs, data):
+            for b, a in pairs[:4]:
+                model.train_step(dg, torch.from_numpy(b).cuda(), torch.from_numpy(a).cuda())
+    after = model.evaluate(dgs[0], b0, a0)[0]
+    assert after < before
+
+
+def test_learned_partition_is_valid_and_deterministic():
+    g = tgp.gen_rmat(12, 8, 41)
+    k = 8
+    data = learn.collect_traces([g], k, init=tgp.INIT_BLOCK, repair=True, coarsen_floor=100)
+    model = learn.GNN(k, seed=1)
+    dg = learn.DevGraph(g)
+    for _ in range(10):
+        for b, a in data[0][1][:6]:
+            model.train_step(dg, torch.from_numpy(b).cuda(), torch.from_numpy(a).cuda())
+    h = model.handle(max_passes=10, min_admit=0.0)
+    kw = dict(init=tgp.INIT_BLOCK, repair=True, coarsen_floor=100, learned=h)
+    r1 = tgp.partition(g, k, **kw)
+    r2 = tgp.partition(g, k, **kw)
+    part, pc, cut, _ = r1
+    np.testing.assert_array_equal(r1[0], r2[0])
+    assert r1[2] == r2[2]
+    assert part.min() >= 0 and part.max() < k and pc.sum() == g.n
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj))
+    recount = int(g.adjwgt[part[src] != part[g.adjncy]].astype(np.int64).sum()) // 2
+    assert cut == recount

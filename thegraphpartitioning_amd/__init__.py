@@ -38,13 +38,17 @@ class CGraph(C.Structure):
 
 INITPART_FN = C.CFUNCTYPE(C.c_int, C.POINTER(CGraph), C.c_int64, C.c_int32,
                           C.POINTER(C.c_int32), C.c_void_p)
+PASS_DUMP_FN = C.CFUNCTYPE(None, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                           C.POINTER(C.c_int32), C.c_void_p)
 
 
 class COptions(C.Structure):
     _fields_ = [("k", C.c_int32), ("wide", C.c_int32), ("coarsen_floor", C.c_int64),
                 ("initpart", C.c_void_p), ("initpart_user", C.c_void_p),
                 ("device_input", C.c_int32), ("profile", C.c_int32),
-                ("check_level", C.c_int32), ("repair", C.c_int32)]
+                ("check_level", C.c_int32), ("repair", C.c_int32),
+                ("pass_dump", C.c_void_p), ("pass_dump_user", C.c_void_p),
+                ("learned", C.c_void_p)]
 
 
 class CBatchItem(C.Structure):
@@ -328,7 +332,8 @@ class Stats:
 
 
 def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=False,
-              coarsen_floor=0, profile=False, check_level=0, trace_cap=0, device_graph=None):
+              coarsen_floor=0, profile=False, check_level=0, trace_cap=0, device_graph=None,
+              pass_dump=None, learned=None):
     """Partitions `g` (a Graph) into k parts on the GPU.
 
     Returns (part[n] int32, part_count[k] int32, edge_cut, Stats).  `init`
@@ -337,7 +342,10 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
     callable f(xadj, adjncy, adjwgt, npairs, k) -> part array.  `init_part`
     supplies fixed values instead (like a fixture part file).
     `device_graph`, if given, is a (xadj, adjncy, adjwgt) triple of device
-    pointers holding the same CSR already resident in HBM.
+    pointers holding the same CSR already resident in HBM.  `pass_dump` is a
+    PASS_DUMP_FN called with each level's partition at every phase start and
+    after every committed pass; `learned` a tgp_learned_model (learn.GNN.handle())
+    whose learned passes run at the finest level before its refinement.
     """
     L = lib()
     o = COptions()
@@ -379,6 +387,12 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
         o.initpart = C.cast(L.tgp_initpart_gpmetis, C.c_void_p).value
     else:
         raise ValueError(f"unknown initial partitioner {init!r}")
+    if pass_dump is not None:
+        o.pass_dump = C.cast(pass_dump, C.c_void_p).value
+        keep.append(pass_dump)
+    if learned is not None:
+        o.learned = C.addressof(learned)
+        keep.append(learned)
     if device_graph is not None:
         o.device_input = 1
         cg = CGraph(g.n, g.m, device_graph[0], device_graph[1], device_graph[2])

@@ -8,6 +8,7 @@
 // multiply and add, as the reference's host code does.
 #include "../../include/tgp.h"
 #include "tgp_internal.h"
+#include "../../include/tgp_learn.h"
 
 #include <chrono>
 #include <climits>
@@ -370,6 +371,14 @@ class Run {
   i64 compute_cut(Level& lv, i64 nedges, const i32* part);
   LevelAux& aux(Level& lv, i64 nedges);
   void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut);
+  void learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc);
+  // the refinement trace callback with a host copy of part
+  void dump_part(int level, int phase, int pass, const i32* part, i64 n) {
+    std::vector<i32> h((size_t)n);
+    HIP_TRY(hipMemcpyAsync(h.data(), part, n * sizeof(i32), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    O.pass_dump(level, phase, pass, n, h.data(), O.pass_dump_user);
+  }
 };
 
 static const i64 HEAVY_DEG = 64;
@@ -872,6 +881,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     for (int phase = 0; phase < 2; phase++) {
       const int maxPass = phase ? 40 : 30;               // gpu_part.cu:545, 647
       bool balanced = false;
+      if (O.pass_dump) dump_part(level, phase, -1, part.p, n);
       // A pass's edge cut is the cut of the partition the next pass's gain
       // kernel reads, so that kernel tallies it and each pass costs one host
       // sync.  The next pass is therefore started before the previous pass's
@@ -961,6 +971,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           *cut = cur;
           tr.push(TGP_TRACE_PASS, level, phase, prevPass, prevM, Ssel, cur, pads,
                   O.check_level ? hash_dev(part.p, n) : 0);
+          if (O.pass_dump) dump_part(level, phase, prevPass, part.p, n);
           const bool quiet = (prevM < n / 200) && (Ssel < n / 400);  // gpu_part.cu:634, 738
           if (quiet && (phase == 0 || balanced)) break;
           if (phase && balanced) break;                    // loop test, gpu_part.cu:647
@@ -1005,6 +1016,74 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
   passes_total += passes;
   tr.push(TGP_TRACE_REFINE, level, 0, passes, n, nedges, *cut, 0,
           O.check_level ? hash_dev(part.p, n) : 0);
+}
+
+// Learned passes at one level (tgp_learned_model): GNN scores -> each
+// vertex's best other part, admitted only with a positive exact integer gain;
+// then the reference's token rule on those gains, move buffer, capacity,
+// sort, selection and commit (gpu_part.cu:316-397, 472-506).  Stops when
+// fewer than min_admit * n moves are admitted.
+void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) {
+  const tgp_learned_model& Mdl = *O.learned;
+  Level& lv = *L[level];
+  const i64 n = lv.n;
+  if (Mdl.k != k || k > 64 || Mdl.hidden <= 0 || Mdl.features < 2 * k + 2 || n < 2) return;
+  LevelAux& ax = aux(lv, nedges);
+  const DGraph g = lv.dg();
+  const int F = Mdl.features, Hd = Mdl.hidden;
+  typedef uint16_t u16;
+  DBuf<u16> X(&C.pool, n * F), M0(&C.pool, n * F), Hb(&C.pool, n * Hd), M1(&C.pool, n * Hd);
+  DBuf<float> Hf(&C.pool, n * Hd), Z(&C.pool, n * k);
+  DBuf<i32> tally(&C.pool, n * k);
+  DBuf<float> gain(&C.pool, n), mv_gain(&C.pool, n), sk_gain(&C.pool, n);
+  DBuf<i32> target(&C.pool, n), mv_v(&C.pool, n), mv_src(&C.pool, n), mv_dst(&C.pool, n),
+      sk_idx(&C.pool, n), sdst(&C.pool, n), outc(&C.pool, k), ctr(&C.pool, 2);
+  DBuf<i32> H(&C.pool, select_scratch_elems(n, k));
+  DBuf<u8> sel(&C.pool, n);
+  DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
+  void* s = (void*)st();
+  auto ck = [&](int rc) {
+    if (rc != 0) {
+      set_error("learned pass: kernel launch failed");
+      throw Fail(TGP_EHIP);
+    }
+  };
+  const i64 min_moves = std::max<i64>(1, (i64)(Mdl.min_admit * (double)n));
+  for (int p = 0; p < Mdl.max_passes; p++) {
+    ck(tgpl_features(n, g.xadj, g.adj, g.w, part.p, k, F, X.p, tally.p, s));
+    ck(tgpl_mean_agg(n, g.xadj, g.adj, X.p, F, 0, M0.p, nullptr, s));
+    ck(tgpl_gemm(X.p, F, 0, Mdl.W1a, Hd, 0, Hf.p, Hd, nullptr, (int)n, Hd, F, 1.0f, 0.0f,
+                 nullptr, 0, s));
+    ck(tgpl_gemm(M0.p, F, 0, Mdl.W1b, Hd, 0, Hf.p, Hd, Hb.p, (int)n, Hd, F, 1.0f, 1.0f, Mdl.b1,
+                 1, s));
+    ck(tgpl_mean_agg(n, g.xadj, g.adj, Hb.p, Hd, 0, M1.p, nullptr, s));
+    ck(tgpl_gemm(Hb.p, Hd, 0, Mdl.W2a, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 0.0f, nullptr,
+                 0, s));
+    ck(tgpl_gemm(M1.p, Hd, 0, Mdl.W2b, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 1.0f, Mdl.b2,
+                 0, s));
+    ck(tgpl_propose(n, k, Z.p, part.p, tally.p, gain.p, target.p, s));
+    HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
+    HIP_TRY(hipMemsetAsync(ctr.p, 0, 2 * sizeof(i32), st()));
+    launch_token(g, nedges, gain.p, part.p, target.p, k, flag.p, ax.heavy.p, ax.nheavy,
+                 HEAVY_DEG, st());
+    launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
+    launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
+                         mv_dst.p, mv_gain.p, outc.p, st());
+    HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    const i64 M = std::min<i64>(C.pinned[0], n);
+    if (M < min_moves) break;
+    launch_capacity(pc.p, outc.p, k, n, 0, avail.p, st());
+    i64 len = M, padded = 1;
+    while (padded < len) padded <<= 1;
+    if (padded > n) padded = n;
+    if (padded > len) len = padded;
+    launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st());
+    launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr.p, ctr.p + 1, st());
+    launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, nullptr, pc.p, k,
+                  nullptr, nullptr, st());
+    passes_total++;
+  }
 }
 
 int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
@@ -1063,6 +1142,7 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     launch_sanitize_hist(pf.p, nf, k, pc.p, st());
     C.prof.end(KID_PROJECT, 12.0 * ncs + 12.0 * nf);
     part = std::move(pf);
+    if (O.learned && level == 0) learned_passes(level, ecount[level], part, pc);
     cut = compute_cut(*L[level], ecount[level], part.p);
     refine(level, ecount[level], part, pc, &cut);
   }
@@ -1120,6 +1200,7 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
   for (i64 i = 0; i < count; i++) {
     const tgp_batch_item& it = items[i];
     const bool ok = it.o.repair && it.o.initpart == tgp_initpart_block && !it.o.profile &&
+                    !it.o.pass_dump && !it.o.learned &&
                     !it.o.check_level && (!it.stats || !it.stats->trace) && it.o.k >= 1 &&
                     it.o.k <= BATCH_MAXK && it.g.n >= 2 && it.g.n <= INT_MAX && it.g.m > 0 &&
                     it.g.m <= INT_MAX && it.part && it.part_count &&
