@@ -20,6 +20,7 @@ copy of known size (`--copy-probe`).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -198,6 +199,74 @@ def bench_hbm_scale(args):
         sys.exit(1)
 
 
+def bench_learned(args):
+    """The learned move proposer against the plain path on the same GPU.
+
+    Training: the plain partitioner's own refinement passes at the finest
+    level on seeded R-MAT graphs (scales 14-16, edge factor 16, k as given),
+    one full-graph step per recorded pass.  Evaluation: R-MAT scale
+    args.scale edge factor 16 (resident in HBM), plain vs learned (learned
+    passes at the finest level, every move admitted by its exact integer gain
+    and the capacity rule, then the ordinary refinement): edges/s, edge cut
+    and imbalance of each."""
+    import numpy as np
+    import torch
+    import thegraphpartitioning_amd as tgp
+    from thegraphpartitioning_amd import learn
+    k = args.k
+    t0 = time.perf_counter()
+    train = [tgp.gen_rmat(s, 16, SEED + 100 + i) for i, s in
+             enumerate([14, 14, 15, 15, 16, 16][:args.learn_graphs])]
+    data = learn.collect_traces(train, k, init=tgp.INIT_BLOCK, repair=True)
+    npairs = sum(len(p) for _, p in data)
+    print(f"# traces: {len(train)} graphs, {npairs} passes ({time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    model = learn.GNN(k, seed=0)
+    dgs = [learn.DevGraph(g) for g, _ in data]
+    items = [(i, j) for i, (_, p) in enumerate(data) for j in range(len(p))]
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    for ep in range(args.learn_epochs):
+        rng.shuffle(items)
+        loss = accu = 0.0
+        for i, j in items:
+            b, a = data[i][1][j]
+            l, acc = model.train_step(dgs[i], torch.from_numpy(b).cuda(), torch.from_numpy(a).cuda())
+            loss += l
+            accu += acc
+        print(f"# epoch {ep}: loss {loss / len(items):.4f} accuracy {accu / len(items):.4f}",
+              file=sys.stderr, flush=True)
+    train_s = time.perf_counter() - t0
+    g = tgp.gen_rmat(args.scale, args.edge_factor, SEED)
+    dg = tgp.DeviceGraph(g)
+    kw = dict(init=tgp.INIT_BLOCK, repair=True, device_graph=dg.pointers())
+    handle = model.handle(max_passes=20, min_admit=0.0005, margin=args.learn_margin)
+
+    def run(learned):
+        extra = dict(learned=handle) if learned else {}
+        tgp.partition(g, k, **kw, **extra)  # warm-up
+        t = time.perf_counter()
+        r = tgp.partition(g, k, **kw, **extra)
+        secs = time.perf_counter() - t
+        part = r[0]
+        imb = float(np.bincount(part, minlength=k).max()) / math.ceil(g.n / k) - 1.0
+        return dict(seconds=round(secs, 4), value=round(g.m / secs / 1e6, 3), edge_cut=r[2],
+                    imbalance=round(imb, 4), passes=r[3].passes)
+    plain = run(False)
+    lrn = run(True)
+    dg.free()
+    print(json.dumps({
+        "metric": "M edges/s end-to-end at k=64 (learned)", "value": lrn["value"],
+        "unit": "M edges/s", "n_gpus": 1, "dtype": "bf16+fp32", "data": "synthetic",
+        "config": {"workload": f"rmat{args.scale}-ef{args.edge_factor}-k{k}", "vertices": g.n,
+                   "arcs": g.m, "train_graphs": len(train), "train_passes": npairs,
+                   "train_epochs": args.learn_epochs, "train_seconds": round(train_s, 2),
+                   "coarsening": "repaired", "initial_partition": "block i*k/n_c"},
+        "plain": plain, "learned": lrn,
+        "cut_vs_plain": round(lrn["edge_cut"] / max(plain["edge_cut"], 1), 4),
+        "speed_vs_plain": round(lrn["value"] / plain["value"], 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +285,12 @@ def main():
                     help="measure the batched form on GRAPHS independent graphs instead")
     ap.add_argument("--workers", type=int, default=16,
                     help="graphs partitioned at once by the batched form's stream workers")
+    ap.add_argument("--learned", action="store_true",
+                    help="train the learned move proposer and compare it with the plain path")
+    ap.add_argument("--learn-graphs", type=int, default=6)
+    ap.add_argument("--learn-epochs", type=int, default=3)
+    ap.add_argument("--learn-margin", type=float, default=3.0,
+                    help="a proposal's logit may trail the own part's by this much")
     ap.add_argument("--hbm-scale", action="store_true",
                     help="run the HBM-scale configuration (use with --scale 26 --k 256)")
     ap.add_argument("--plain-sample", type=int, default=0, metavar="GRAPHS",
@@ -234,6 +309,9 @@ def main():
         return
     if args.hbm_scale:
         bench_hbm_scale(args)
+        return
+    if args.learned:
+        bench_learned(args)
         return
 
     import thegraphpartitioning_amd as tgp

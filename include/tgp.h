@@ -106,8 +106,9 @@ typedef struct tgp_options {
 
 /* The learned move proposer's weights (device memory) for the GNN of
  * tgp_learn.h.  Each learned pass at the finest level: features -> GNN ->
- * for each vertex its best-scoring other part q, admitted only when the exact
- * integer gain T_v[q] - T_v[part_v] is positive; the reference's token rule
+ * for each vertex its best-scoring other part q, proposed when its logit is
+ * within `margin` of the own part's and admitted only when the exact integer
+ * gain T_v[q] - T_v[part_v] is positive; the reference's token rule
  * on those gains, move buffer, capacity (gpu_part.cu:316-369), sort, select
  * and commit (472-397).  Learned passes stop when fewer than min_admit * n
  * moves are admitted (or after max_passes); the level's ordinary refinement
@@ -121,7 +122,7 @@ typedef struct tgp_learned_model {
   const float* b1;      /* hidden, FP32 */
   const float* b2;      /* k */
   float min_admit;
-  int32_t reserved;
+  float margin;         /* a proposal needs logit(q) > logit(part_v) - margin */
 } tgp_learned_model;
 
 /* trace record kinds */

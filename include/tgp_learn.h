@@ -47,11 +47,11 @@ int tgpl_adam(int64_t cnt, float* W, const float* G, float* m, float* v, float l
 int tgpl_to_bf16(int64_t cnt, const float* x, uint16_t* y, void* stream);
 /* y += a x */
 int tgpl_axpy(int64_t cnt, float a, const float* x, float* y, void* stream);
-/* gain[v] = exact integer gain of v's best non-own part when positive (and
- * the logits prefer it to the own part), NaN otherwise; target[v] = that
- * part.  tally as written by tgpl_features.                                  */
+/* gain[v] = exact integer gain of v's best non-own part q when it is
+ * positive and logit(q) > logit(part_v) - margin, NaN otherwise; target[v] =
+ * q.  tally as written by tgpl_features.                                     */
 int tgpl_propose(int64_t n, int k, const float* Z, const int32_t* part, const int32_t* tally,
-                 float* gain, int32_t* target, void* stream);
+                 float margin, float* gain, int32_t* target, void* stream);
 
 #ifdef __cplusplus
 }

@@ -105,12 +105,13 @@ def test_features_and_proposals():
     np.testing.assert_array_equal(x[:, k:2 * k], np.eye(k)[part])
     assert (x[:, 2 * k + 1] == 1).all() and (x[:, 2 * k + 2:] == 0).all()
     # proposals: best other part by logit (lowest on ties), admitted only with a
-    # positive exact gain and a logit above the own part's
+    # positive exact gain and a logit within the margin of the own part's
     Z = torch.randn(g.n, k, generator=torch.Generator().manual_seed(4))
     Z[:5] = 0.0
     gain = torch.empty(g.n, device="cuda")
     tgt = torch.empty(g.n, dtype=torch.int32, device="cuda")
-    assert L.tgpl_propose(g.n, k, learn._p(Z.cuda()), learn._p(dpart), learn._p(tally),
+    margin = 0.5
+    assert L.tgpl_propose(g.n, k, learn._p(Z.cuda()), learn._p(dpart), learn._p(tally), margin,
                           learn._p(gain), learn._p(tgt), stream()) == 0
     torch.cuda.synchronize()
     Zn = Z.numpy()
@@ -119,7 +120,7 @@ def test_features_and_proposals():
         order = [q for q in range(k) if q != cur]
         best = max(order, key=lambda q: (Zn[v, q], -q))
         d = T[v, best] - T[v, cur]
-        ok = Zn[v, best] > Zn[v, cur] and d > 0
+        ok = Zn[v, best] > Zn[v, cur] - np.float32(margin) and d > 0
         assert tgt[v].item() == best
         gv = gain[v].item()
         assert (gv == d) if ok else np.isnan(gv)

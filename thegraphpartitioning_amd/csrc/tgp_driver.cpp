@@ -1061,7 +1061,7 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
                  0, s));
     ck(tgpl_gemm(M1.p, Hd, 0, Mdl.W2b, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 1.0f, Mdl.b2,
                  0, s));
-    ck(tgpl_propose(n, k, Z.p, part.p, tally.p, gain.p, target.p, s));
+    ck(tgpl_propose(n, k, Z.p, part.p, tally.p, Mdl.margin, gain.p, target.p, s));
     HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
     HIP_TRY(hipMemsetAsync(ctr.p, 0, 2 * sizeof(i32), st()));
     launch_token(g, nedges, gain.p, part.p, target.p, k, flag.p, ax.heavy.p, ax.nheavy,

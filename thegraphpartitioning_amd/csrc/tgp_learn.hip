@@ -246,11 +246,12 @@ __global__ void k_to_bf16(i64 cnt, const float* x, bf16* y) {
 
 // ----------------------------------------------------------------- proposals --
 // The learned proposal of each vertex: q* = argmax over parts other than its
-// own (lowest q on ties); admitted when the exact integer gain T_v[q*] -
-// T_v[part_v] is positive.  gain[v] = that gain (as float) for an admitted
+// own (lowest q on ties), proposed when its logit is within `margin` of the
+// own part's; admitted when the exact integer gain T_v[q*] - T_v[part_v] is
+// positive.  gain[v] = that gain (as float) for an admitted
 // proposal, NaN otherwise; target[v] = q*.
 __global__ void k_propose(i64 n, int k, const float* Z, const i32* part, const i32* tally,
-                          float* gain, i32* target) {
+                          float margin, float* gain, i32* target) {
   for (i64 v = (i64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (i64)gridDim.x * blockDim.x) {
     const i32 cur = part[v];
     const float* z = Z + v * k;
@@ -264,7 +265,7 @@ __global__ void k_propose(i64 n, int k, const float* Z, const i32* part, const i
       }
     }
     float g = __int_as_float(0x7fc00000);  // NaN: not a candidate
-    if (best >= 0 && cur >= 0 && cur < k && z[best] > z[cur]) {
+    if (best >= 0 && cur >= 0 && cur < k && z[best] > z[cur] - margin) {
       const i32 d = tally[v * k + best] - tally[v * k + cur];
       if (d > 0) g = (float)d;
     }
@@ -350,8 +351,9 @@ int tgpl_to_bf16(int64_t cnt, const float* x, uint16_t* y, void* stream) {
 }
 
 int tgpl_propose(int64_t n, int k, const float* Z, const int32_t* part, const int32_t* tally,
-                 float* gain, int32_t* target, void* stream) {
-  k_propose<<<grid(n, 256), 256, 0, (hipStream_t)stream>>>(n, k, Z, part, tally, gain, target);
+                 float margin, float* gain, int32_t* target, void* stream) {
+  k_propose<<<grid(n, 256), 256, 0, (hipStream_t)stream>>>(n, k, Z, part, tally, margin, gain,
+                                                            target);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -28,7 +28,7 @@ class CModel(C.Structure):
     _fields_ = [("k", C.c_int32), ("features", C.c_int32), ("hidden", C.c_int32),
                 ("max_passes", C.c_int32), ("W1a", C.c_void_p), ("W1b", C.c_void_p),
                 ("W2a", C.c_void_p), ("W2b", C.c_void_p), ("b1", C.c_void_p),
-                ("b2", C.c_void_p), ("min_admit", C.c_float), ("reserved", C.c_int32)]
+                ("b2", C.c_void_p), ("min_admit", C.c_float), ("margin", C.c_float)]
 
 
 def _lib():
@@ -45,7 +45,7 @@ def _lib():
     L.tgpl_adam.argtypes = [I64, P, P, P, P, F32, F32, F32, F32, I, P, P]
     L.tgpl_to_bf16.argtypes = [I64, P, P, P]
     L.tgpl_axpy.argtypes = [I64, F32, P, P, P]
-    L.tgpl_propose.argtypes = [I64, I, P, P, P, P, P, P]
+    L.tgpl_propose.argtypes = [I64, I, P, P, P, F32, P, P, P]
     for name in ("tgpl_gemm", "tgpl_mean_agg", "tgpl_features", "tgpl_softmax_xent",
                  "tgpl_relu_bwd", "tgpl_colsum", "tgpl_adam", "tgpl_to_bf16", "tgpl_axpy",
                  "tgpl_propose"):
@@ -207,12 +207,12 @@ class GNN:
         return a[0] / max(a[1], 1e-30), a[2] / max(a[3], 1), rec
 
     # ---- use in the partitioner ------------------------------------------------
-    def handle(self, max_passes=20, min_admit=0.0005):
+    def handle(self, max_passes=20, min_admit=0.0005, margin=3.0):
         """A tgp_learned_model for tgp.partition(..., learned=...); keep self alive."""
         self._cm = CModel(self.k, self.F, self.hidden, max_passes,
                           self.Wb["W1a"].data_ptr(), self.Wb["W1b"].data_ptr(),
                           self.Wb["W2a"].data_ptr(), self.Wb["W2b"].data_ptr(),
-                          self.W["b1"].data_ptr(), self.W["b2"].data_ptr(), min_admit, 0)
+                          self.W["b1"].data_ptr(), self.W["b2"].data_ptr(), min_admit, margin)
         return self._cm
 
     def state(self):
