@@ -4,7 +4,7 @@
  * A 2-layer mean-aggregation GNN over per-vertex features, BF16 weights and
  * activations with FP32 accumulation on the matrix cores:
  *   x_v = [T_v[q]/deg_v (q < k), onehot(part_v) (k), log2(1+deg_v)/16, 1, 0 pad] (F columns)
- *   H   = ReLU(X W1a + mean_N(X) W1b + b1)      hidden 64
+ *   H   = ReLU(X W1a + mean_N(X W1b) + b1)      hidden 64
  *   Z   = H W2a + mean_N(H) W2b + b2            k logits
  * trained on the plain refinement's own passes (label: the part a vertex holds
  * after the pass).  Proposals are admitted only through the exact integer

@@ -145,8 +145,8 @@ def reference_forward(model, g, part):
     D = torch.from_numpy(np.maximum(np.diff(g.xadj), 1).astype(np.float32))[:, None]
     W = {n: w.cpu() for n, w in model.W.items()}
     Wb = {n: bf16_round(w) for n, w in W.items()}
-    M0 = bf16_round((A @ X) / D)
-    H = bf16_round(torch.relu(X @ Wb["W1a"] + M0 @ Wb["W1b"] + W["b1"]))
+    P = bf16_round(X @ Wb["W1b"])
+    H = bf16_round(torch.relu(X @ Wb["W1a"] + (A @ P) / D + W["b1"]))
     M1 = bf16_round((A @ H) / D)
     return H @ Wb["W2a"] + M1 @ Wb["W2b"] + W["b2"]
 

@@ -1032,8 +1032,8 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
   const DGraph g = lv.dg();
   const int F = Mdl.features, Hd = Mdl.hidden;
   typedef uint16_t u16;
-  DBuf<u16> X(&C.pool, n * F), M0(&C.pool, n * F), Hb(&C.pool, n * Hd), M1(&C.pool, n * Hd);
-  DBuf<float> Hf(&C.pool, n * Hd), Z(&C.pool, n * k);
+  DBuf<u16> X(&C.pool, n * F), Pb(&C.pool, n * Hd), Hb(&C.pool, n * Hd), M1(&C.pool, n * Hd);
+  DBuf<float> Pf(&C.pool, n * Hd), Hf(&C.pool, n * Hd), Z(&C.pool, n * k);
   DBuf<i32> tally(&C.pool, n * k);
   DBuf<float> gain(&C.pool, n), mv_gain(&C.pool, n), sk_gain(&C.pool, n);
   DBuf<i32> target(&C.pool, n), mv_v(&C.pool, n), mv_src(&C.pool, n), mv_dst(&C.pool, n),
@@ -1051,11 +1051,12 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
   const i64 min_moves = std::max<i64>(1, (i64)(Mdl.min_admit * (double)n));
   for (int p = 0; p < Mdl.max_passes; p++) {
     ck(tgpl_features(n, g.xadj, g.adj, g.w, part.p, k, F, X.p, tally.p, s));
-    ck(tgpl_mean_agg(n, g.xadj, g.adj, X.p, F, 0, M0.p, nullptr, s));
-    ck(tgpl_gemm(X.p, F, 0, Mdl.W1a, Hd, 0, Hf.p, Hd, nullptr, (int)n, Hd, F, 1.0f, 0.0f,
-                 nullptr, 0, s));
-    ck(tgpl_gemm(M0.p, F, 0, Mdl.W1b, Hd, 0, Hf.p, Hd, Hb.p, (int)n, Hd, F, 1.0f, 1.0f, Mdl.b1,
-                 1, s));
+    // H = ReLU(X W1a + mean_N(X W1b) + b1): the mean over the 64 columns of X W1b
+    ck(tgpl_gemm(X.p, F, 0, Mdl.W1b, Hd, 0, Pf.p, Hd, Pb.p, (int)n, Hd, F, 1.0f, 0.0f, nullptr, 0,
+                 s));
+    ck(tgpl_mean_agg(n, g.xadj, g.adj, Pb.p, Hd, 0, M1.p, Hf.p, s));
+    ck(tgpl_gemm(X.p, F, 0, Mdl.W1a, Hd, 0, Hf.p, Hd, Hb.p, (int)n, Hd, F, 1.0f, 1.0f, Mdl.b1, 1,
+                 s));
     ck(tgpl_mean_agg(n, g.xadj, g.adj, Hb.p, Hd, 0, M1.p, nullptr, s));
     ck(tgpl_gemm(Hb.p, Hd, 0, Mdl.W2a, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 0.0f, nullptr,
                  0, s));

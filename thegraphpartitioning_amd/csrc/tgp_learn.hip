@@ -5,7 +5,7 @@
 //
 //   features  x_v = [ T_v[q] / deg_v (q < k), onehot(part_v) (k), log2(1+deg_v)/16, 1, 0... ]
 //             (T_v[q] = weight of v's arcs into part q), row stride F (multiple of 32)
-//   layer 1   H = ReLU(X W1a + mean_N(X) W1b + b1)          (hidden 64)
+//   layer 1   H = ReLU(X W1a + mean_N(X W1b) + b1)          (hidden 64)
 //   layer 2   Z = H W2a + mean_N(H) W2b + b2                 (k logits)
 //   target    the part the vertex holds after the plain pass (softmax cross-entropy)
 #include <hip/hip_runtime.h>

@@ -11,7 +11,9 @@ only holds the device buffers.
     part, pc, cut, st = tgp.partition(g, k, learned=model.handle(), ...)
 
 Inputs per vertex: x_v = [T_v[q]/deg_v (q < k), onehot(part_v), log2(1+deg_v)/16,
-1] (padded to a multiple of 32 columns); hidden 64; output k logits.  The
+1] (padded to a multiple of 32 columns); hidden 64; output k logits.  The first
+layer's mean term is computed as mean_N(X W1b) (64 columns) rather than
+mean_N(X) W1b (F columns); the two are equal in exact arithmetic.  The
 label of a vertex in a pass is the part it holds after the plain pass.
 """
 import ctypes as C
@@ -123,13 +125,16 @@ class GNN:
         tally = torch.empty(n, k, dtype=torch.int32, device=self.device)
         _ok(L.tgpl_features(n, _p(dg.xadj), _p(dg.adj), _p(dg.w), _p(part), k, F, _p(X),
                             _p(tally), s), "features")
-        M0 = torch.empty(n, F, dtype=bf, device=self.device)
-        _ok(L.tgpl_mean_agg(n, _p(dg.xadj), _p(dg.adj), _p(X), F, 0, _p(M0), None, s), "agg")
-        H = torch.empty(n, Hd, device=self.device)
-        Hb = torch.empty(n, Hd, dtype=bf, device=self.device)
-        _ok(L.tgpl_gemm(_p(X), F, 0, _p(self.Wb["W1a"]), Hd, 0, _p(H), Hd, None, n, Hd, F,
+        # mean_N(X) W1b = mean_N(X W1b): aggregate the 64 columns of P = X W1b
+        P = torch.empty(n, Hd, device=self.device)
+        Pb = torch.empty(n, Hd, dtype=bf, device=self.device)
+        _ok(L.tgpl_gemm(_p(X), F, 0, _p(self.Wb["W1b"]), Hd, 0, _p(P), Hd, _p(Pb), n, Hd, F,
                         1.0, 0.0, None, 0, s), "gemm")
-        _ok(L.tgpl_gemm(_p(M0), F, 0, _p(self.Wb["W1b"]), Hd, 0, _p(H), Hd, _p(Hb), n, Hd, F,
+        H = torch.empty(n, Hd, device=self.device)  # starts as mean_N(P)
+        M0b = torch.empty(n, Hd, dtype=bf, device=self.device)
+        _ok(L.tgpl_mean_agg(n, _p(dg.xadj), _p(dg.adj), _p(Pb), Hd, 0, _p(M0b), _p(H), s), "agg")
+        Hb = torch.empty(n, Hd, dtype=bf, device=self.device)
+        _ok(L.tgpl_gemm(_p(X), F, 0, _p(self.Wb["W1a"]), Hd, 0, _p(H), Hd, _p(Hb), n, Hd, F,
                         1.0, 1.0, _p(self.W["b1"]), 1, s), "gemm")
         M1 = torch.empty(n, Hd, dtype=bf, device=self.device)
         _ok(L.tgpl_mean_agg(n, _p(dg.xadj), _p(dg.adj), _p(Hb), Hd, 0, _p(M1), None, s), "agg")
@@ -138,7 +143,7 @@ class GNN:
                         1.0, 0.0, None, 0, s), "gemm")
         _ok(L.tgpl_gemm(_p(M1), Hd, 0, _p(self.Wb["W2b"]), k, 0, _p(Z), k, None, n, k, Hd,
                         1.0, 1.0, _p(self.W["b2"]), 0, s), "gemm")
-        return Z, dict(X=X, M0=M0, H=H, Hb=Hb, M1=M1, tally=tally)
+        return Z, dict(X=X, H=H, Hb=Hb, M1=M1, tally=tally)
 
     # ---- one training step ---------------------------------------------------
     def train_step(self, dg, part, label, lr=3e-3, wmove=8.0):
@@ -182,7 +187,12 @@ class GNN:
         G["W1b"] = torch.empty(F, Hd, device=self.device)
         _ok(L.tgpl_gemm(_p(c["X"]), F, 1, _p(dHb), Hd, 0, _p(G["W1a"]), Hd, None, F, Hd, n,
                         1.0, 0.0, None, 0, s), "gemm")
-        _ok(L.tgpl_gemm(_p(c["M0"]), F, 1, _p(dHb), Hd, 0, _p(G["W1b"]), Hd, None, F, Hd, n,
+        # H's mean term: dP = adjoint mean of dH, dW1b = X^T dP
+        dPf = torch.empty(n, Hd, device=self.device)
+        dPb = torch.empty(n, Hd, dtype=bf, device=self.device)
+        _ok(L.tgpl_mean_agg(n, _p(dg.xadj), _p(dg.adj), _p(dHb), Hd, 1, _p(dPb), _p(dPf), s),
+            "agg")
+        _ok(L.tgpl_gemm(_p(c["X"]), F, 1, _p(dPb), Hd, 0, _p(G["W1b"]), Hd, None, F, Hd, n,
                         1.0, 0.0, None, 0, s), "gemm")
         G["b1"] = torch.empty(Hd, device=self.device)
         _ok(L.tgpl_colsum(n, Hd, _p(dH), _p(G["b1"]), s), "colsum")
