@@ -222,3 +222,43 @@ def test_large_k(oracle, k, wide):
     g = tgp.gen_rmat(scale, 8, 0x7C0DE5EED + k, "hash16")
     gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, k, repair=True, floor=100, wide=wide)
     assert_same(gp, op, f"rmat{scale} k={k} wide={wide}")
+
+
+def _drop_or_reweight(kind, seed):
+    """Sorted, repeat-free rows that are not symmetric in one way each:
+    'weights': every reverse arc exists but some lower arcs carry another
+    weight; 'extra': one upper arc 0 -> n-1 has no reverse (every lower arc
+    still finds its reverse, only the up/down counts differ); 'missing': one
+    lower arc loses its reverse."""
+    xadj, adj, w = graphs.random_graph(2000, 9000, seed, wmax=9)
+    xadj, adj, w = (np.asarray(a).copy() for a in (xadj, adj, w))
+    n = len(xadj) - 1
+    src = np.repeat(np.arange(n), np.diff(xadj))
+    if kind == "weights":
+        lower = np.nonzero(adj < src)[0]
+        w[lower[::7]] += 1
+        return xadj, adj, w
+    rows = [list(zip(adj[xadj[v]:xadj[v + 1]], w[xadj[v]:xadj[v + 1]])) for v in range(n)]
+    if kind == "extra":
+        if all(u != n - 1 for u, _ in rows[0]):
+            rows[0].append((n - 1, 3))
+    else:
+        v = int(src[np.nonzero(adj < src)[0][0]])
+        u = min(u for u, _ in rows[v])
+        rows[u] = [(x, y) for x, y in rows[u] if x != v]
+    xadj = np.zeros(n + 1, np.int64)
+    xadj[1:] = np.cumsum([len(r) for r in rows])
+    adj = np.array([x for r in rows for x, _ in r], np.int32)
+    w = np.array([y for r in rows for _, y in r], np.int32)
+    return xadj, adj, w
+
+
+@pytest.mark.parametrize("repair", [False, True])
+@pytest.mark.parametrize("kind", ["weights", "extra", "missing"])
+def test_nearly_symmetric_sorted_rows(oracle, kind, repair):
+    """The sorted-row symmetry check must reject each of these levels: a
+    wrong verdict sends the level to the chained matching or the kept
+    tallies, and the trace then departs from the oracle's."""
+    xadj, adj, w = _drop_or_reweight(kind, 60)
+    gp, op = run_both(oracle, xadj, adj, w, 8, repair=repair, floor=40)
+    assert_same(gp, op, f"nearly symmetric ({kind}) repair={repair}")

@@ -388,6 +388,11 @@ static i64 env_i64(const char* name, i64 dflt) {
   return (v && *v) ? std::atoll(v) : dflt;
 }
 
+// Lanes per row in the token pass (1, 4 or 8: TGP_TOKEN_LANES) and in the
+// gain pass over kept tallies (2, 4, 8 or 16: TGP_GAIN_LANES).
+static int token_lanes() { return (int)env_i64("TGP_TOKEN_LANES", 1); }
+static int gain_lanes() { return (int)env_i64("TGP_GAIN_LANES", 4); }
+
 void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const i64 n = lv.n;
   static const i64 MATCH_WINDOW = std::max<i64>(64, env_i64("TGP_MATCH_WINDOW", 16384));
@@ -524,15 +529,21 @@ void Run::check_symmetric(Level& lv) {
   HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
   const bool strict = ((const i32*)C.pinned)[3] == 0;
-  // strictly ascending rows hold no duplicates: half the arcs go into the
-  // table and the other half look their reverse up
-  i64 cap = 1;
-  while (cap < (strict ? lv.m + 2 : 2 * lv.m + 2)) cap <<= 1;
-  DBuf<u64> table(&C.pool, cap);
-  DBuf<i32> wt(&C.pool, cap);
+  // strictly ascending rows hold no duplicates: each arc down finds its
+  // reverse by binary search in the sorted row; other rows go through a hash
+  // table of every arc
+  DBuf<u64> table;
+  DBuf<i32> wt;
   C.prof.begin(KID_MISC);
-  if (strict) launch_sym_check_sorted(lv.dg(), table.p, wt.p, cap, flags.p, cnt.p, st());
-  else launch_sym_check(lv.dg(), table.p, wt.p, cap, flags.p, st());
+  if (strict) {
+    launch_sym_check_sorted(lv.dg(), flags.p, cnt.p, st());
+  } else {
+    i64 cap = 1;
+    while (cap < 2 * lv.m + 2) cap <<= 1;
+    table = DBuf<u64>(&C.pool, cap);
+    wt = DBuf<i32>(&C.pool, cap);
+    launch_sym_check(lv.dg(), table.p, wt.p, cap, flags.p, st());
+  }
   C.prof.end(KID_MISC, (strict ? 14.0 : 28.0) * lv.m);
   HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   HIP_TRY(hipMemcpyAsync(C.pinned + 2, cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, st()));
@@ -813,6 +824,7 @@ int Run::initial_partition(DBuf<i32>& part, i64* npairs) {
 }
 
 void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut) {
+  const int TOKEN_LANES = token_lanes(), GAIN_LANES = gain_lanes();
   Level& lv = *L[level];
   const i64 n = lv.n;
   const DGraph g = lv.dg();
@@ -905,7 +917,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           if (tmode && T_valid) {
             if (last_partial) launch_tally_last(g, nedges, part.p, T.p, k, unit, st());
             launch_gain_tallies(n, part.p, T.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
-                                phase, pending ? accs.p : nullptr, st());
+                                phase, pending ? accs.p : nullptr, GAIN_LANES, st());
             fused = true;
             C.prof.end(KID_GAIN, 4.0 * k * n + 17.0 * n);
           } else {
@@ -923,7 +935,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           if (pending) launch_acc_finish(accs.p, cutacc.p, st());
           C.prof.begin(KID_TOKEN);
           launch_token(g, nedges, cand.p, part.p, target.p, k, flag.p, heavy, nheavy, HEAVY_DEG,
-                       st());
+                       TOKEN_LANES, st());
           C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
           C.prof.begin(KID_COMPACT);
           launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
@@ -1024,6 +1036,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
 // sort, selection and commit (gpu_part.cu:316-397, 472-506).  Stops when
 // fewer than min_admit * n moves are admitted.
 void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) {
+  const int TOKEN_LANES = token_lanes();
   const tgp_learned_model& Mdl = *O.learned;
   Level& lv = *L[level];
   const i64 n = lv.n;
@@ -1066,7 +1079,7 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
     HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
     HIP_TRY(hipMemsetAsync(ctr.p, 0, 2 * sizeof(i32), st()));
     launch_token(g, nedges, gain.p, part.p, target.p, k, flag.p, ax.heavy.p, ax.nheavy,
-                 HEAVY_DEG, st());
+                 HEAVY_DEG, TOKEN_LANES, st());
     launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
     launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
                          mv_dst.p, mv_gain.p, outc.p, st());

@@ -145,11 +145,11 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s);
 // flags[3] |= some row is not strictly ascending
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
-// the same checks for strictly ascending rows with half the table (cap >= m,
-// pow2): flags[0], flags[2] as above; cnt[0] = arcs up, cnt[1] = arcs down
-// whose reverse was found (symmetric iff flags[0] == 0 and they agree)
-void launch_sym_check_sorted(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, u64* cnt,
-                             hipStream_t s);
+// the same checks for strictly ascending rows, by binary search of the
+// reverse arc in its (sorted) row: flags[0], flags[2] as above; cnt[0] = arcs
+// up, cnt[1] = arcs down whose reverse was found (symmetric iff flags[0] == 0
+// and they agree)
+void launch_sym_check_sorted(DGraph g, i32* flags, u64* cnt, hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);
 // Reference contraction (gpu_part.cu:145-242 semantics): padded rows of 64.
@@ -256,10 +256,11 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, 
 // Kept tallies: row v of T holds k ints at v * tally_stride(k) (k rounded up
 // to 4, so rows are 16-byte aligned); Tout of launch_gain uses the same layout.
 inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
-// Gains (and the cut into cut_acc slots) from kept tallies T.
+// Gains (and the cut into cut_acc slots) from kept tallies T; lanes (2, 4, 8 or
+// 16) read each row.
 void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
                          const u8* src_big, float* gain, i32* target, float* cand, int phase,
-                         u64* cut_acc, hipStream_t s);
+                         u64* cut_acc, int lanes, hipStream_t s);
 // T update for the applied moves app[0..*napp) (at most max_app) of a commit;
 // skip_last leaves T[n-1] alone.
 void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
@@ -275,10 +276,10 @@ void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* h
                         hipStream_t s);
 i64 gain_gscratch_elems(i32 k);
 // cand[v] = gain of v when v is a move candidate, NaN otherwise (the gain
-// kernels write it)
+// kernels write it); lanes (1, 4 or 8) per row of at most heavy_deg arcs
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
-                  hipStream_t s);
+                  int lanes, hipStream_t s);
 void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part,
                           const i32* target, const float* gain, i64 n, i32 k, i32* mv_v,
                           i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,

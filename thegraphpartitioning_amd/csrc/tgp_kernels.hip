@@ -1135,67 +1135,42 @@ __global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
   if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
 }
 
-// Strictly ascending rows: each arc v->u with u > v is inserted once with its
-// weight (counted in cnt[0]); each arc v->u with u < v looks up (u, v) and
-// compares weights (counted in cnt[1] when found).  Symmetric iff nothing is
-// missing and both counts agree, since no key repeats.
-__global__ __launch_bounds__(256) void k_sym_insert_up(DGraph g, u64* table, i32* wt, i64 cap,
-                                                       u64* cnt) {
-  const u64 mask = (u64)cap - 1;
-  const int lane = lane_id();
-  u64 c = 0;
-  for (i64 v = gwave(); v < g.n; v += nwaves()) {
-    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
-    for (i64 i = s0 + lane; i < s1; i += WAVE) {
-      const i32 u = g.adj[i];
-      if (u <= v || u >= g.n) continue;
-      const u64 key = ((u64)v << 32) | (u64)(u32)u;
-      u64 h = hash64(key) & mask;
-      while (true) {
-        const u64 old = atomicCAS((unsigned long long*)&table[h], ~0ull, (unsigned long long)key);
-        if (old == ~0ull) {
-          wt[h] = g.w[i];
-          break;
-        }
-        h = (h + 1) & mask;
-      }
-      c++;
-    }
-  }
-  c = wave_sum(c);
-  if (lane == 0 && c) atomicAdd((unsigned long long*)&cnt[0], (unsigned long long)c);
-}
-__global__ __launch_bounds__(256) void k_sym_check_down(DGraph g, const u64* table,
-                                                        const i32* wt, i64 cap, i32* flags,
-                                                        u64* cnt) {
-  const u64 mask = (u64)cap - 1;
+// Strictly ascending rows: each arc v->u with u > v is counted (cnt[0]); each
+// arc v->u with u < v binary-searches row u for v and compares weights
+// (counted in cnt[1] when found, flags[0] when missing, flags[2] on a weight
+// mismatch).  Rows hold no repeats, so distinct lower arcs find distinct upper
+// arcs: symmetric iff nothing is missing and both counts agree.
+__global__ __launch_bounds__(256) void k_sym_search_sorted(DGraph g, i32* flags, u64* cnt) {
   const int lane = lane_id();
   bool miss = false, wmis = false;
-  u64 c = 0;
+  u64 cu = 0, cd = 0;
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
     for (i64 i = s0 + lane; i < s1; i += WAVE) {
       const i32 u = g.adj[i];
-      if (u < 0 || u >= v) continue;
-      const u64 key = ((u64)(u32)u << 32) | (u64)v;
-      u64 h = hash64(key) & mask;
-      while (true) {
-        const u64 cur = table[h];
-        if (cur == key) {
-          wmis |= wt[h] != g.w[i];
-          c++;
-          break;
-        }
-        if (cur == ~0ull) {
-          miss = true;
-          break;
-        }
-        h = (h + 1) & mask;
+      if (u >= g.n || u < 0 || u == v) continue;
+      if (u > v) {
+        cu++;
+        continue;
+      }
+      i64 lo = g.xadj[u], hi = g.xadj[u + 1];  // first entry >= v in [lo, hi)
+      while (lo < hi) {
+        const i64 mid = lo + ((hi - lo) >> 1);
+        if (g.adj[mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < g.xadj[u + 1] && g.adj[lo] == v) {
+        wmis |= g.w[lo] != g.w[i];
+        cd++;
+      } else {
+        miss = true;
       }
     }
   }
-  c = wave_sum(c);
-  if (lane == 0 && c) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)c);
+  cu = wave_sum(cu);
+  cd = wave_sum(cd);
+  if (lane == 0 && cu) atomicAdd((unsigned long long*)&cnt[0], (unsigned long long)cu);
+  if (lane == 0 && cd) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)cd);
   if (__ballot(miss) && lane == 0) atomicOr(&flags[0], 1);
   if (__ballot(wmis) && lane == 0) atomicOr(&flags[2], 1);
 }
@@ -2187,6 +2162,7 @@ __global__ __launch_bounds__(256) void k_gain_fused(DGraph g, i64 nedges, const 
 
 // The pick of gain_vertex from T, 16 lanes per vertex; also the vertex's
 // share of the edge cut (its arcs into other parts).
+template <int G, int R>
 __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, const i32* T,
                                                       i32 k, const float* bg_g,
                                                       const u8* src_big, float* gain,
@@ -2195,10 +2171,10 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, co
   extern __shared__ float sbgt[];
   for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbgt[q] = bg_g[q];
   __syncthreads();
-  constexpr int G = 16;
   const int gl = threadIdx.x & (G - 1);
   const i64 ngroups = (i64)gridDim.x * (blockDim.x / G);
   const i64 ts = tally_stride(k);
+  const int nv4 = (int)(ts >> 2);
   u64 cutv = 0;
   for (i64 v = (i64)blockIdx.x * (blockDim.x / G) + threadIdx.x / G; v < n; v += ngroups) {
     const i32 cur = part[v];
@@ -2210,20 +2186,29 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, co
     float bgv = -INFINITY;
     i32 bq = INT_MAX;
     i64 other = 0;
-    for (int c = gl; c < (int)(ts >> 2); c += G) {
-      const int4 t4 = T4[c];
-      const i32 tv4[4] = {t4.x, t4.y, t4.z, t4.w};
+    for (int c0 = 0; c0 < nv4; c0 += G * R) {
+      int4 t4r[R];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const i32 q = 4 * c + j;
-        if (q >= k || q == cur) continue;
-        const i32 tq = tv4[j];
-        other += tq;
-        const float eg = (float)tq - ccur;
-        const float gg = eg + bgr[q];
-        if (gg > bgv || (gg == bgv && q < bq)) {
-          bgv = gg;
-          bq = q;
+      for (int r = 0; r < R; r++) {
+        const int c = c0 + gl + r * G;
+        t4r[r] = c < nv4 ? T4[c] : make_int4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int c = c0 + gl + r * G;
+        const i32 tv4[4] = {t4r[r].x, t4r[r].y, t4r[r].z, t4r[r].w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const i32 q = 4 * c + j;
+          if (q >= k || q == cur) continue;
+          const i32 tq = tv4[j];
+          other += tq;
+          const float eg = (float)tq - ccur;
+          const float gg = eg + bgr[q];
+          if (gg > bgv || (gg == bgv && q < bq)) {
+            bgv = gg;
+            bq = q;
+          }
         }
       }
     }
@@ -2351,13 +2336,65 @@ __device__ __forceinline__ bool move_ok(i32 src, i32 dst, i32 k) {
 // One launch serves both row classes at once: the first `bh` blocks (issued
 // first, they run longest) take the heavy rows (one wave per vertex, 64
 // neighbours per step: the first losing neighbour usually comes early, so
-// wider steps only add traffic), the others the light rows (<= heavy_deg arcs,
-// one thread per vertex, 8 neighbours per step; most vertices are not
-// candidates and stop at once).
+// wider steps only add traffic), the others the light rows (<= heavy_deg
+// arcs).  G = 1: one thread per light vertex, 8 neighbours per step; G > 1:
+// G lanes per light vertex, G * TU neighbours per step (the lanes read
+// consecutive arcs, and the group stops at the first step with a loser).
+template <int G, int TU>
+__device__ __forceinline__ void token_light_groups(const DGraph& g, i64 nedges,
+                                                   const float* cand, const i32* part,
+                                                   const i32* target, i32 k, i64* flag,
+                                                   i64 heavy_deg, unsigned bh) {
+  const int gl = threadIdx.x & (G - 1);
+  const int gbase = lane_id() & ~(G - 1);
+  constexpr u64 GMASK = (G == 64) ? ~0ull : ((1ull << G) - 1);
+  const i64 ngroups = (i64)(gridDim.x - bh) * (blockDim.x / G);
+  for (i64 v = (i64)(blockIdx.x - bh) * (blockDim.x / G) + threadIdx.x / G; v < g.n;
+       v += ngroups) {
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    if (s1 - s0 > heavy_deg) continue;
+    const float gv = cand[v];
+    if (!(gv == gv)) {
+      if (gl == 0) flag[v] = 0;
+      continue;
+    }
+    const float gv10 = gv + 10.0f;
+    bool lose = false;
+    for (i64 base = s0; base < s1; base += G * TU) {
+      i32 uu[TU];
+#pragma unroll
+      for (int j = 0; j < TU; j++) {
+        const i64 i = base + gl + (i64)j * G;
+        uu[j] = i < s1 ? g.adj[i] : INT_MAX;
+      }
+      bool l = false;
+#pragma unroll
+      for (int j = 0; j < TU; j++) {
+        if (uu[j] < g.n) {
+          const float gu = cand[uu[j]];
+          l |= gu > gv10 || (gu == gv && uu[j] < v);
+        }
+      }
+      if ((__ballot(l) >> gbase) & GMASK) {
+        lose = true;
+        break;
+      }
+    }
+    if (gl == 0) flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
+  }
+}
+
+template <int G>
 __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float* cand,
                                                const i32* part, const i32* target, i32 k,
                                                i64* flag, i64 heavy_deg, const i32* list,
                                                i64 cnt, unsigned bh) {
+  if (blockIdx.x >= bh && G > 1) {
+    token_light_groups<(G > 1 ? G : 2), 4>(g, nedges, cand, part, target, k, flag, heavy_deg,
+                                           bh);
+    return;
+  }
   if (blockIdx.x >= bh) {
     constexpr int TU = 8;
     const i64 stride = (i64)(gridDim.x - bh) * blockDim.x;
@@ -3047,11 +3084,8 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s) {
   k_rows_strict<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags);
 }
-void launch_sym_check_sorted(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, u64* cnt,
-                             hipStream_t s) {
-  k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
-  k_sym_insert_up<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, cnt);
-  k_sym_check_down<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, wt, cap, flags, cnt);
+void launch_sym_check_sorted(DGraph g, i32* flags, u64* cnt, hipStream_t s) {
+  k_sym_search_sorted<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags, cnt);
 }
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s) {
   k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
@@ -3232,11 +3266,19 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, 
 
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
-                  hipStream_t s) {
-  const unsigned bl = grid_for(g.n, 256, 1u << 16);
+                  int lanes, hipStream_t s) {
+  const int G = (lanes == 4 || lanes == 8) ? lanes : 1;
+  const unsigned bl = grid_for(g.n, 256 / G, 1u << 16);
   const unsigned bh = heavy_count > 0 ? grid_for(heavy_count, 4, 8192) : 0;
-  k_token<<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg, heavy_list,
-                                  heavy_count, bh);
+  if (G == 8)
+    k_token<8><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
+                                       heavy_list, heavy_count, bh);
+  else if (G == 4)
+    k_token<4><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
+                                       heavy_list, heavy_count, bh);
+  else
+    k_token<1><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
+                                       heavy_list, heavy_count, bh);
 }
 
 void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part, const i32* target,
@@ -3307,9 +3349,20 @@ void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, con
 
 void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
                          const u8* src_big, float* gain, i32* target, float* cand, int phase,
-                         u64* cut_acc, hipStream_t s) {
-  k_gain_tallies<<<grid_for(n, 16, 4096), 256, (size_t)2 * k * 4, s>>>(
-      n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+                         u64* cut_acc, int lanes, hipStream_t s) {
+  const size_t lds = (size_t)2 * k * 4;
+  if (lanes == 2)
+    k_gain_tallies<2, 8><<<grid_for(n, 128, 4096), 256, lds, s>>>(
+        n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+  else if (lanes == 4)
+    k_gain_tallies<4, 4><<<grid_for(n, 64, 4096), 256, lds, s>>>(
+        n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+  else if (lanes == 8)
+    k_gain_tallies<8, 2><<<grid_for(n, 32, 4096), 256, lds, s>>>(
+        n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+  else
+    k_gain_tallies<16, 1><<<grid_for(n, 16, 4096), 256, lds, s>>>(
+        n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
 }
 void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
                          const i32* mv_v, const i32* mv_src, const i32* mv_dst, int unit,
