@@ -67,6 +67,21 @@ typedef int (*tgp_initpart_fn)(const tgp_graph* coarsest, int64_t npairs, int32_
  * `user` may point to an int that receives system()'s status.                */
 int tgp_initpart_gpmetis(const tgp_graph* coarsest, int64_t npairs, int32_t k,
                          int32_t* part, void* user);
+/* The same seam in a working directory of its own, so that several graphs
+ * can call gpmetis at once (the reference's fixed names, gpu_part.cu:829,
+ * 1163, allow one per directory): `user` points to a tgp_gpmetis_dir; writes
+ * <dir>/output.graph (the same bytes), runs
+ * `gpmetis -ptype=rb '<dir>/output.graph' <k>` and reads
+ * <dir>/output.graph.part.<k> with the same rules.  The directory must exist
+ * and its name may not contain a single quote.  No counterpart in the
+ * reference.                                                                 */
+typedef struct tgp_gpmetis_dir {
+  const char* dir;
+  int32_t status;          /* out: system()'s status                           */
+  int32_t reserved;
+} tgp_gpmetis_dir;
+int tgp_initpart_gpmetis_dir(const tgp_graph* coarsest, int64_t npairs, int32_t k,
+                             int32_t* part, void* user);
 /* Deterministic stand-in for hosts without METIS: part[i] = i*k/n_c.  No
  * counterpart in the reference (which always calls gpmetis).                 */
 int tgp_initpart_block(const tgp_graph* coarsest, int64_t npairs, int32_t k,
@@ -174,7 +189,9 @@ int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part,
  * it->stats) would do alone, so every result is bit-identical to its solo
  * run; up to `workers` items (1..64) run at once, each on its own HIP stream
  * and device pool.  Initial-partition callbacks are serialised (the gpmetis
- * seam uses fixed file names in the working directory).  Returns TGP_OK when
+ * seam uses fixed file names in the working directory), except
+ * tgp_initpart_block and tgp_initpart_gpmetis_dir, which run concurrently
+ * (give each item its own directory).  Returns TGP_OK when
  * every item succeeded, else the first failing item's code; each item's own
  * code is in it->rc.                                                         */
 typedef struct tgp_batch_item {

@@ -17,3 +17,30 @@ def oracle():
     import oracle_ffi
     oracle_ffi.lib()
     return oracle_ffi
+
+
+STUB_GPMETIS = """#!/bin/sh
+# stand-in for gpmetis: checks the arguments, writes <graph>.part.<k> with
+# line i holding (3*i + 1) mod k, and logs its call
+[ "$1" = "-ptype=rb" ] || exit 3
+n=$(head -1 "$2" | cut -d' ' -f1)
+i=0
+: > "$2.part.$3"
+while [ $i -lt $n ]; do echo $(( (3 * i + 1) % $3 )) >> "$2.part.$3"; i=$((i + 1)); done
+echo "$2 $3" >> "$STUB_GPMETIS_LOG"
+"""
+
+
+@pytest.fixture
+def stub_gpmetis(tmp_path, monkeypatch):
+    """Puts a stand-in `gpmetis` first on PATH; returns the file that logs
+    each call as "<graph path> <k>"."""
+    bindir = tmp_path / "stub_bin"
+    bindir.mkdir()
+    stub = bindir / "gpmetis"
+    stub.write_text(STUB_GPMETIS)
+    stub.chmod(0o755)
+    monkeypatch.setenv("PATH", f"{bindir}:{os.environ['PATH']}")
+    log = tmp_path / "gpmetis_calls.log"
+    monkeypatch.setenv("STUB_GPMETIS_LOG", str(log))
+    return log

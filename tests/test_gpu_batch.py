@@ -112,3 +112,24 @@ def test_batch_mixed_scope():
         assert out[i][2] == cut, f"graph {i}"
         np.testing.assert_array_equal(out[i][0], part)
         np.testing.assert_array_equal(out[i][1], pc)
+
+
+def test_batch_gpmetis_in_own_directories(tmp_path, monkeypatch, stub_gpmetis):
+    """init "gpmetis" with a working directory: each graph calls gpmetis on its
+    own <workdir>/graph<i>/output.graph (concurrently, from the worker
+    threads); every result equals the solo run through the reference seam in
+    the working directory, with the same stand-in gpmetis."""
+    monkeypatch.chdir(tmp_path)
+    gs, ks = batch_graphs(8, 16, 20000)
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_GPMETIS, repair=True, workers=4,
+                              workdir=str(tmp_path / "work"))
+    calls = stub_gpmetis.read_text().splitlines()
+    assert sorted(calls) == sorted(f"{tmp_path}/work/graph{i}/output.graph {k}"
+                                   for i, k in enumerate(ks))
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_GPMETIS, repair=True)
+        assert (tmp_path / "work" / f"graph{i}" / "output.graph").read_bytes() == \
+            (tmp_path / "output.graph").read_bytes(), f"graph {i}"
+        assert out[i][2] == cut, f"graph {i}"
+        np.testing.assert_array_equal(out[i][0], part)
+        np.testing.assert_array_equal(out[i][1], pc)

@@ -146,3 +146,42 @@ def test_cli_other_input_forms_errors(tmp_path):
     assert r.returncode == 1 and "Error opening file" in r.stderr
     r = run_cli([str(row), str(row), "2", "--bogus"], str(tmp_path))
     assert r.returncode == 1 and "Unknown option: --bogus" in r.stderr
+
+
+
+def test_gpmetis_seam_in_a_directory(tmp_path, monkeypatch, stub_gpmetis):
+    """tgp_initpart_gpmetis_dir writes the same METIS text as the working-
+    directory seam (itself checked against the oracle above), runs gpmetis on
+    the file in its own directory and reads the part file by the same rules."""
+    import ctypes as C
+    log = stub_gpmetis
+    L = tgp.lib()
+    xadj, adj, w = graphs.random_graph(60, 150, 4, wmax=5)
+    g = tgp.Graph(xadj, adj, w)
+    cg = g.cgraph()
+    k = 4
+    cwd = tmp_path / "cwd"
+    cwd.mkdir()
+    monkeypatch.chdir(cwd)
+    status = C.c_int(-1)
+    part_cwd = (C.c_int32 * g.n)()
+    assert L.tgp_initpart_gpmetis(C.byref(cg), 0, k, part_cwd,
+                                  C.cast(C.byref(status), C.c_void_p)) == 0
+    assert status.value == 0
+    own = tmp_path / "graph 0"        # a space in the name: the path is quoted
+    own.mkdir()
+    d = tgp.CGpmetisDir(str(own).encode(), -1, 0)
+    part_dir = (C.c_int32 * g.n)()
+    assert L.tgp_initpart_gpmetis_dir(C.byref(cg), 0, k, part_dir,
+                                      C.cast(C.byref(d), C.c_void_p)) == 0
+    assert d.status == 0
+    assert (cwd / "output.graph").read_bytes() == tgp.metis_text(g)[0]
+    assert (own / "output.graph").read_bytes() == tgp.metis_text(g)[0]
+    expect = [(3 * i + 1) % k for i in range(g.n)]
+    assert list(part_cwd) == expect and list(part_dir) == expect
+    calls = log.read_text().splitlines()
+    assert calls == [f"./output.graph {k}", f"{own}/output.graph {k}"]
+    for name in (str(tmp_path / "it's"), ""):
+        bad = tgp.CGpmetisDir(name.encode(), 0, 0)
+        assert L.tgp_initpart_gpmetis_dir(C.byref(cg), 0, k, part_dir,
+                                          C.cast(C.byref(bad), C.c_void_p)) == -1

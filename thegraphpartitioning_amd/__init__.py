@@ -51,6 +51,11 @@ class COptions(C.Structure):
                 ("learned", C.c_void_p)]
 
 
+class CGpmetisDir(C.Structure):
+    """tgp_gpmetis_dir: the working directory of one gpmetis call."""
+    _fields_ = [("dir", C.c_char_p), ("status", C.c_int32), ("reserved", C.c_int32)]
+
+
 class CBatchItem(C.Structure):
     pass
 
@@ -135,6 +140,7 @@ def lib():
     L.tgp_initpart_block.argtypes = [C.POINTER(CGraph), C.c_int64, C.c_int32,
                                      C.POINTER(C.c_int32), P]
     L.tgp_initpart_gpmetis.argtypes = L.tgp_initpart_block.argtypes
+    L.tgp_initpart_gpmetis_dir.argtypes = L.tgp_initpart_block.argtypes
     L.tgp_release_device_memory.restype = None
     L.tgp_graph_upload.argtypes = [C.POINTER(CGraph), C.POINTER(CGraph)]
     L.tgp_graph_upload.restype = C.c_int
@@ -424,22 +430,33 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
 
 
 def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, coarsen_floor=0,
-                    workers=8, device_graphs=None, profile=False):
+                    workers=8, device_graphs=None, profile=False, workdir=None):
     """Partitions many independent graphs through tgp_partition_batch.
 
     Each result equals what partition() returns for that graph alone (bit for
     bit); up to `workers` graphs run at once, each on its own HIP stream.
-    `init` is "block" or "gpmetis"; `device_graphs`, if given, holds one
-    (xadj, adjncy, adjwgt) triple of device pointers per graph.  Returns a list
-    of (part, part_count, edge_cut, Stats); raises TGPError naming the first
-    failing graph.
+    `init` is "block" or "gpmetis"; with "gpmetis" and a `workdir`, graph i
+    calls gpmetis in its own directory <workdir>/graph<i> (created), so the
+    calls run concurrently on host threads; without one they share the
+    working directory and run one at a time.  `device_graphs`, if given,
+    holds one (xadj, adjncy, adjwgt) triple of device pointers per graph.
+    Returns a list of (part, part_count, edge_cut, Stats); raises TGPError
+    naming the first failing graph.
     """
     L = lib()
     cnt = len(graphs)
     if len(ks) != cnt:
         raise ValueError("one k per graph")
+    dirs = None
     if init == INIT_BLOCK:
         fn = C.cast(L.tgp_initpart_block, C.c_void_p).value
+    elif init == INIT_GPMETIS and workdir is not None:
+        fn = C.cast(L.tgp_initpart_gpmetis_dir, C.c_void_p).value
+        dirs = []
+        for i in range(cnt):
+            d = os.path.join(os.path.abspath(workdir), f"graph{i}")
+            os.makedirs(d, exist_ok=True)
+            dirs.append(CGpmetisDir(d.encode(), 0, 0))
     elif init == INIT_GPMETIS:
         fn = C.cast(L.tgp_initpart_gpmetis, C.c_void_p).value
     else:
@@ -448,6 +465,8 @@ def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, co
     outs = []
     for i, g in enumerate(graphs):
         it = items[i]
+        if dirs is not None:
+            it.o.initpart_user = C.addressof(dirs[i])
         if device_graphs is not None:
             d = device_graphs[i]
             it.g = CGraph(g.n, g.m, d[0], d[1], d[2])

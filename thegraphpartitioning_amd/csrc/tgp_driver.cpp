@@ -805,9 +805,12 @@ int Run::initial_partition(DBuf<i32>& part, i64* npairs) {
     int rc;
     {
       // one callback at a time: the gpmetis seam uses fixed file names in the
-      // working directory (gpu_part.cu:829, 1163)
+      // working directory (gpu_part.cu:829, 1163), and a caller's callback
+      // need not be reentrant; the block partition and the seam in a
+      // directory of the item's own run concurrently
       static std::mutex initpart_mu;
-      std::lock_guard<std::mutex> lock(initpart_mu);
+      std::unique_lock<std::mutex> lock(initpart_mu, std::defer_lock);
+      if (fn != tgp_initpart_block && fn != tgp_initpart_gpmetis_dir) lock.lock();
       rc = fn(&hg, *npairs, k, hp.data(), O.initpart_user);
     }
     if (rc != 0) {

@@ -401,30 +401,36 @@ int tgp_metis_text(const tgp_graph* g, char** text, int64_t* len, int64_t* npair
   return TGP_OK;
 }
 
-int tgp_initpart_gpmetis(const tgp_graph* coarsest, int64_t npairs, int32_t k, int32_t* part,
-                         void* user) {
-  (void)npairs;
+// Writes <dir>output.graph, runs gpmetis on it and reads the part file
+// (gpu_part.cu:829-864, 1157-1174).  An empty `dir` is the working directory
+// with the reference's own names and command; otherwise the command quotes
+// the graph path.
+static int gpmetis_in(const std::string& dir, const tgp_graph* coarsest, int32_t k,
+                      int32_t* part, int* status) {
   char* text = nullptr;
   int64_t len = 0, np = 0;
   int rc = tgp_metis_text(coarsest, &text, &len, &np);
   if (rc != TGP_OK) return rc;
-  FILE* f = std::fopen("output.graph", "wb");
+  const std::string gpath = dir + "output.graph";
+  FILE* f = std::fopen(gpath.c_str(), "wb");
   if (!f) {
     std::free(text);
-    tgp::last_error() = "Error: Cannot create output.graph";
+    tgp::last_error() = "Error: Cannot create " + gpath;
     return TGP_EIO;
   }
   std::fwrite(text, 1, (size_t)len, f);
   std::fclose(f);
   std::free(text);
-  const std::string cmd = "gpmetis -ptype=rb ./output.graph " + std::to_string(k);
+  const std::string cmd = "gpmetis -ptype=rb " +
+                          (dir.empty() ? std::string("./output.graph") : "'" + gpath + "'") +
+                          " " + std::to_string(k);
   const int ret = std::system(cmd.c_str());
-  if (user) *(int*)user = ret;
+  if (status) *status = ret;
   if (ret != 0) {
     tgp::last_error() = "Error: gpmetis failed with return code " + std::to_string(ret);
     return TGP_EINITPART;
   }
-  const std::string pf = "output.graph.part." + std::to_string(k);
+  const std::string pf = gpath + ".part." + std::to_string(k);
   std::ifstream fin(pf);
   if (!fin.is_open()) {
     tgp::last_error() = "Error: Cannot open " + pf;
@@ -437,6 +443,25 @@ int tgp_initpart_gpmetis(const tgp_graph* coarsest, int64_t npairs, int32_t k, i
   }
   for (i64 i = 0; i < coarsest->n; i++) part[i] = h[i];
   return TGP_OK;
+}
+
+int tgp_initpart_gpmetis(const tgp_graph* coarsest, int64_t npairs, int32_t k, int32_t* part,
+                         void* user) {
+  (void)npairs;
+  return gpmetis_in("", coarsest, k, part, (int*)user);
+}
+
+int tgp_initpart_gpmetis_dir(const tgp_graph* coarsest, int64_t npairs, int32_t k,
+                             int32_t* part, void* user) {
+  (void)npairs;
+  tgp_gpmetis_dir* d = (tgp_gpmetis_dir*)user;
+  if (!d || !d->dir || !*d->dir || std::strchr(d->dir, '\'')) {
+    tgp::last_error() = "Error: gpmetis working directory missing or holds a quote";
+    return TGP_EINVAL;
+  }
+  std::string dir(d->dir);
+  if (dir.back() != '/') dir += '/';
+  return gpmetis_in(dir, coarsest, k, part, &d->status);
 }
 
 int tgp_initpart_block(const tgp_graph* coarsest, int64_t npairs, int32_t k, int32_t* part,
