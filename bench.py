@@ -204,8 +204,9 @@ def bench_learned(args):
 
     Training: the plain partitioner's own refinement passes at the finest
     level on seeded R-MAT graphs (scales 14-16, edge factor 16, k as given),
-    one full-graph step per recorded pass.  Evaluation: R-MAT scale
-    args.scale edge factor 16 (resident in HBM), plain vs learned (learned
+    or 3-D grids of side 24-48 with --learn-grid, one full-graph step per
+    recorded pass.  Evaluation: R-MAT scale args.scale edge factor 16, or the
+    --learn-grid grid in wide mode (resident in HBM), plain vs learned (learned
     passes at the finest level, every move admitted by its exact integer gain
     and the capacity rule, then the ordinary refinement): edges/s, edge cut
     and imbalance of each."""
@@ -214,10 +215,14 @@ def bench_learned(args):
     import thegraphpartitioning_amd as tgp
     from thegraphpartitioning_amd import learn
     k = args.k
+    grid = args.learn_grid
     t0 = time.perf_counter()
-    train = [tgp.gen_rmat(s, 16, SEED + 100 + i) for i, s in
-             enumerate([14, 14, 15, 15, 16, 16][:args.learn_graphs])]
-    data = learn.collect_traces(train, k, init=tgp.INIT_BLOCK, repair=True)
+    if grid:
+        train = [tgp.gen_grid3d(s) for s in [24, 28, 32, 36, 40, 48][:args.learn_graphs]]
+    else:
+        train = [tgp.gen_rmat(s, 16, SEED + 100 + i) for i, s in
+                 enumerate([14, 14, 15, 15, 16, 16][:args.learn_graphs])]
+    data = learn.collect_traces(train, k, init=tgp.INIT_BLOCK, repair=True, wide=bool(grid))
     npairs = sum(len(p) for _, p in data)
     print(f"# traces: {len(train)} graphs, {npairs} passes ({time.perf_counter() - t0:.1f}s)",
           file=sys.stderr, flush=True)
@@ -237,9 +242,9 @@ def bench_learned(args):
         print(f"# epoch {ep}: loss {loss / len(items):.4f} accuracy {accu / len(items):.4f}",
               file=sys.stderr, flush=True)
     train_s = time.perf_counter() - t0
-    g = tgp.gen_rmat(args.scale, args.edge_factor, SEED)
+    g = tgp.gen_grid3d(grid) if grid else tgp.gen_rmat(args.scale, args.edge_factor, SEED)
     dg = tgp.DeviceGraph(g)
-    kw = dict(init=tgp.INIT_BLOCK, repair=True, device_graph=dg.pointers())
+    kw = dict(init=tgp.INIT_BLOCK, repair=True, wide=bool(grid), device_graph=dg.pointers())
     handle = model.handle(max_passes=20, min_admit=0.0005, margin=args.learn_margin)
 
     def run(learned):
@@ -256,9 +261,10 @@ def bench_learned(args):
     lrn = run(True)
     dg.free()
     print(json.dumps({
-        "metric": "M edges/s end-to-end at k=64 (learned)", "value": lrn["value"],
+        "metric": f"M edges/s end-to-end at k={k} (learned)", "value": lrn["value"],
         "unit": "M edges/s", "n_gpus": 1, "dtype": "bf16+fp32", "data": "synthetic",
-        "config": {"workload": f"rmat{args.scale}-ef{args.edge_factor}-k{k}", "vertices": g.n,
+        "config": {"workload": f"grid{grid}^3-k{k}" if grid else
+                   f"rmat{args.scale}-ef{args.edge_factor}-k{k}", "vertices": g.n,
                    "arcs": g.m, "train_graphs": len(train), "train_passes": npairs,
                    "train_epochs": args.learn_epochs, "train_seconds": round(train_s, 2),
                    "coarsening": "repaired", "initial_partition": "block i*k/n_c"},
@@ -289,6 +295,9 @@ def main():
                     help="train the learned move proposer and compare it with the plain path")
     ap.add_argument("--learn-graphs", type=int, default=6)
     ap.add_argument("--learn-epochs", type=int, default=3)
+    ap.add_argument("--learn-grid", type=int, default=0, metavar="SIDE",
+                    help="with --learned: train on 3-D grids and evaluate on the SIDE^3 grid "
+                         "(wide mode) instead of R-MAT")
     ap.add_argument("--learn-margin", type=float, default=3.0,
                     help="a proposal's logit may trail the own part's by this much")
     ap.add_argument("--hbm-scale", action="store_true",

@@ -127,7 +127,8 @@ typedef struct tgp_options {
  * on those gains, move buffer, capacity (gpu_part.cu:316-369), sort, select
  * and commit (472-397).  Learned passes stop when fewer than min_admit * n
  * moves are admitted (or after max_passes); the level's ordinary refinement
- * follows (the fallback).                                                   */
+ * follows (the fallback).  Needs k equal to the partition's k, k <= 256,
+ * hidden > 0 and features >= 2k + 2 (else TGP_EINVAL).                      */
 typedef struct tgp_learned_model {
   int32_t k, features, hidden, max_passes;
   const uint16_t* W1a;  /* features x hidden, BF16 */

@@ -32,7 +32,7 @@ int tgpl_gemm(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb
 int tgpl_mean_agg(int64_t n, const int64_t* xadj, const int32_t* adj, const uint16_t* X, int F,
                   int transpose, uint16_t* out, float* outf, void* stream);
 /* X (n x F, BF16) from the partition; tally (optional, n x k) = T_v[q].
- * k <= 64, F >= 2k + 2.                                                      */
+ * k <= 256, F >= 2k + 2.                                                     */
 int tgpl_features(int64_t n, const int64_t* xadj, const int32_t* adj, const int32_t* w,
                   const int32_t* part, int k, int F, uint16_t* X, int32_t* tally, void* stream);
 /* weighted softmax cross-entropy; dZ (optional) = gradient; acc[4] += (loss,

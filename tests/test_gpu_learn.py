@@ -82,9 +82,9 @@ def test_mean_aggregation_and_adjoint():
         assert torch.allclose(outf.cpu(), ref, rtol=1e-5, atol=1e-5)
 
 
-def test_features_and_proposals():
+@pytest.mark.parametrize("k", [8, 130])
+def test_features_and_proposals(k):
     g = small_graph()
-    k = 8
     dg = learn.DevGraph(g)
     part = (np.arange(g.n) * k // g.n).astype(np.int32)
     F = learn.feature_count(k)

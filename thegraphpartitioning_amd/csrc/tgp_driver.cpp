@@ -1043,7 +1043,11 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
   const tgp_learned_model& Mdl = *O.learned;
   Level& lv = *L[level];
   const i64 n = lv.n;
-  if (Mdl.k != k || k > 64 || Mdl.hidden <= 0 || Mdl.features < 2 * k + 2 || n < 2) return;
+  if (Mdl.k != k || k > 256 || Mdl.hidden <= 0 || Mdl.features < 2 * k + 2) {
+    set_error("learned model does not fit: model k, k <= 256, hidden > 0, features >= 2k+2");
+    throw Fail(TGP_EINVAL);
+  }
+  if (n < 2) return;
   LevelAux& ax = aux(lv, nedges);
   const DGraph g = lv.dg();
   const int F = Mdl.features, Hd = Mdl.hidden;

@@ -128,14 +128,15 @@ __global__ __launch_bounds__(256) void k_mean_agg(i64 n, const i64* xadj, const 
 }
 
 // --------------------------------------------------------------- features --
-// one wave per vertex: part tallies in LDS (k <= 64), then the row of X
+// one wave per vertex: part tallies in LDS (k <= FEAT_MAXK), then the row of X
+constexpr int FEAT_MAXK = 256;
 __global__ __launch_bounds__(256) void k_features(i64 n, const i64* xadj, const i32* adj,
                                                   const i32* w, const i32* part, int k, int F,
                                                   bf16* X, i32* tally) {
-  __shared__ float tl[4][64];
+  __shared__ float tl[4][FEAT_MAXK];
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (i64 v = (i64)blockIdx.x * 4 + wv; v < n; v += (i64)gridDim.x * 4) {
-    tl[wv][l] = 0.0f;
+    for (int q = l; q < k; q += WAVE) tl[wv][q] = 0.0f;
     __builtin_amdgcn_wave_barrier();
     const i64 s0 = xadj[v], s1 = xadj[v + 1];
     float deg = 0.0f;
@@ -158,7 +159,8 @@ __global__ __launch_bounds__(256) void k_features(i64 n, const i64* xadj, const 
       else if (f == 2 * k + 1) x = 1.0f;
       row[f] = f2bf(x);
     }
-    if (tally && l < k) tally[v * k + l] = (i32)tl[wv][l];
+    if (tally)
+      for (int q = l; q < k; q += WAVE) tally[v * k + q] = (i32)tl[wv][q];
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -304,7 +306,7 @@ int tgpl_mean_agg(int64_t n, const int64_t* xadj, const int32_t* adj, const uint
 
 int tgpl_features(int64_t n, const int64_t* xadj, const int32_t* adj, const int32_t* w,
                   const int32_t* part, int k, int F, uint16_t* X, int32_t* tally, void* stream) {
-  if (n <= 0 || k < 1 || k > 64 || F < 2 * k + 2) return -1;
+  if (n <= 0 || k < 1 || k > FEAT_MAXK || F < 2 * k + 2) return -1;
   k_features<<<grid(n, 4), 256, 0, (hipStream_t)stream>>>(n, xadj, adj, w, part, k, F, X, tally);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
