@@ -762,7 +762,12 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
 i64 Run::compute_cut(Level& lv, i64 nedges, const i32* part) {
   LevelAux& a = aux(lv, nedges);
   C.prof.begin(KID_CUT);
-  launch_cut_arcs(lv.dg(), a.e_end, a.asrc.p, part, k, a.unit, accs.p, st());
+  DBuf<u8> part8;
+  if (k <= PART8_MAXK) {  // a byte copy keeps the gathered partition in L2
+    part8 = DBuf<u8>(&C.pool, lv.n);
+    launch_part8(part, lv.n, k, part8.p, st());
+  }
+  launch_cut_arcs(lv.dg(), a.e_end, a.asrc.p, part, part8.p, k, a.unit, accs.p, st());
   launch_acc_finish(accs.p, (u64*)(small.p + 1), st());
   C.prof.end(KID_CUT, (a.unit ? 12.0 : 16.0) * a.e_end);
   return wrap_cut((u64)read_i64(small.p + 1));
@@ -932,7 +937,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           }
           if (pending && !fused) {
             C.prof.begin(KID_CUT);
-            launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, accs.p, st());
+            launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
             C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
           }
           if (pending) launch_acc_finish(accs.p, cutacc.p, st());
@@ -948,7 +953,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
         } else if (pending) {
           C.prof.begin(KID_CUT);
-          launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, k, unit, accs.p, st());
+          launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
           launch_acc_finish(accs.p, cutacc.p, st());
           C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
         } else {

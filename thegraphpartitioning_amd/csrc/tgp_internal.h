@@ -307,8 +307,9 @@ void launch_acc_finish(u64* acc, u64* out, hipStream_t s);
 void launch_arc_source(DGraph g, i32* asrc, hipStream_t s);
 // Edge cut over arcs [0, e_end) with e_end = max(xadj[n-1], nedges): exactly
 // the arcs of the reference's row ranges (gpu_part.cu:404-417).
-// (acc: accumulator slots)
-void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 k, int unit,
-                     u64* acc, hipStream_t s);
+// (acc: accumulator slots).  part8 (optional, k <= PART8_MAXK): the byte
+// copy of part, read instead of part.
+void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, const u8* part8,
+                     i32 k, int unit, u64* acc, hipStream_t s);
 
 }  // namespace tgp

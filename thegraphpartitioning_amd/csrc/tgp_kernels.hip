@@ -2939,9 +2939,11 @@ __global__ __launch_bounds__(256) void k_arc_source(DGraph g, i32* asrc) {
 // gpu_part.cu:400-422 as an arc-parallel stream over [0, e_end): every arc of
 // a row in the reference's range is visited once; sum in uint64 (its low 32
 // bits are the reference's int32 wrap).
-template <bool UNIT>
+// PT = u8: the byte copy part8 (k <= 255, 255 for every value outside
+// [0, k)), whose gathers stay in L2; the test is the same.
+template <bool UNIT, typename PT>
 __global__ __launch_bounds__(256) void k_cut_arcs(const i32* asrc, const i32* adj, const i32* w,
-                                                  i64 n, i64 e_end, const i32* part, i32 k,
+                                                  i64 n, i64 e_end, const PT* part, i32 k,
                                                   u64* acc) {
   u64 s = 0;
   for (i64 i = gtid(); i < e_end; i += gstride()) {
@@ -2949,7 +2951,9 @@ __global__ __launch_bounds__(256) void k_cut_arcs(const i32* asrc, const i32* ad
     const i32 u = adj[i];
     if (u < n) {
       const i32 pv = part[v], pu = part[u];
-      if (pv != pu && pv >= 0 && pv < k && pu >= 0 && pu < k) s += UNIT ? 1ull : (u64)(i64)w[i];
+      const bool in = sizeof(PT) == 1 ? (pv != 255 && pu != 255)
+                                      : (pv >= 0 && pv < k && pu >= 0 && pu < k);
+      if (pv != pu && in) s += UNIT ? 1ull : (u64)(i64)w[i];
     }
   }
   block_add_u64(s, acc);
@@ -3387,13 +3391,19 @@ void launch_acc_finish(u64* acc, u64* out, hipStream_t s) {
   k_acc_finish<<<1, ACC_SLOTS, 0, s>>>(acc, out);
 }
 
-void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, i32 k, int unit,
-                     u64* acc, hipStream_t s) {
+void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, const u8* part8,
+                     i32 k, int unit, u64* acc, hipStream_t s) {
   const unsigned gr = grid_for(e_end, 256 * 8, 8192);
-  if (unit)
-    k_cut_arcs<true><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
-  else
-    k_cut_arcs<false><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
+  if (part8 && k <= PART8_MAXK) {
+    if (unit)
+      k_cut_arcs<true, u8><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part8, k, acc);
+    else
+      k_cut_arcs<false, u8><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part8, k, acc);
+  } else if (unit) {
+    k_cut_arcs<true, i32><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
+  } else {
+    k_cut_arcs<false, i32><<<gr, 256, 0, s>>>(asrc, g.adj, g.w, g.n, e_end, part, k, acc);
+  }
 }
 
 i64 batch_coarsen_bytes(i64 maxn) {
