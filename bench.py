@@ -203,9 +203,10 @@ def bench_learned(args):
     """The learned move proposer against the plain path on the same GPU.
 
     Training: the plain partitioner's own refinement passes at the finest
-    level on seeded R-MAT graphs (scales 14-16, edge factor 16, k as given),
-    or 3-D grids of side 24-48 with --learn-grid, one full-graph step per
-    recorded pass.  Evaluation: R-MAT scale args.scale edge factor 16, or the
+    level on seeded R-MAT graphs (scales 14-16, edge factor 16, k as given;
+    with more than 6 graphs, scales 16-20 streamed one graph at a time, a
+    few evenly spaced passes each), or 3-D grids of side 24-48 with
+    --learn-grid, one full-graph step per recorded pass.  Evaluation: R-MAT scale args.scale edge factor 16, or the
     --learn-grid grid in wide mode (resident in HBM), plain vs learned (learned
     passes at the finest level, every move admitted by its exact integer gain
     and the capacity rule, then the ordinary refinement): edges/s, edge cut
@@ -217,30 +218,62 @@ def bench_learned(args):
     k = args.k
     grid = args.learn_grid
     t0 = time.perf_counter()
-    if grid:
-        train = [tgp.gen_grid3d(s) for s in [24, 28, 32, 36, 40, 48][:args.learn_graphs]]
-    else:
-        train = [tgp.gen_rmat(s, 16, SEED + 100 + i) for i, s in
-                 enumerate([14, 14, 15, 15, 16, 16][:args.learn_graphs])]
-    data = learn.collect_traces(train, k, init=tgp.INIT_BLOCK, repair=True, wide=bool(grid))
-    npairs = sum(len(p) for _, p in data)
-    print(f"# traces: {len(train)} graphs, {npairs} passes ({time.perf_counter() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
     model = learn.GNN(k, seed=0)
-    dgs = [learn.DevGraph(g) for g, _ in data]
-    items = [(i, j) for i, (_, p) in enumerate(data) for j in range(len(p))]
-    rng = np.random.default_rng(0)
-    t0 = time.perf_counter()
-    for ep in range(args.learn_epochs):
-        rng.shuffle(items)
+    if args.learn_graphs > 6 and not grid:
+        # streamed: R-MAT scales 16-20 in turn, each graph's passes recorded,
+        # args.learn_pairs of them (evenly spaced) trained on, then dropped
+        npairs = 0
         loss = accu = 0.0
-        for i, j in items:
-            b, a = data[i][1][j]
-            l, acc = model.train_step(dgs[i], torch.from_numpy(b).cuda(), torch.from_numpy(a).cuda())
-            loss += l
-            accu += acc
-        print(f"# epoch {ep}: loss {loss / len(items):.4f} accuracy {accu / len(items):.4f}",
+        steps = 0
+        for i in range(args.learn_graphs):
+            g = tgp.gen_rmat(16 + i % 5, 16, SEED + 100 + i)
+            ((_, pairs),) = learn.collect_traces([g], k, init=tgp.INIT_BLOCK, repair=True)
+            if not pairs:
+                continue
+            sel = pairs[::max(1, len(pairs) // args.learn_pairs)][:args.learn_pairs]
+            dg = learn.DevGraph(g)
+            for _ in range(args.learn_epochs):
+                for b, a in sel:
+                    l, acc = model.train_step(dg, torch.from_numpy(b).cuda(),
+                                              torch.from_numpy(a).cuda())
+                    loss += l
+                    accu += acc
+                    steps += 1
+            npairs += len(sel)
+            del dg, pairs, sel
+            if (i + 1) % 32 == 0:
+                print(f"# {i + 1} graphs, {steps} steps: loss {loss / max(steps, 1):.4f} "
+                      f"accuracy {accu / max(steps, 1):.4f} ({time.perf_counter() - t0:.1f}s)",
+                      file=sys.stderr, flush=True)
+                loss = accu = 0.0
+                steps = 0
+        ntrain = args.learn_graphs
+    else:
+        if grid:
+            train = [tgp.gen_grid3d(s) for s in [24, 28, 32, 36, 40, 48][:args.learn_graphs]]
+        else:
+            train = [tgp.gen_rmat(s, 16, SEED + 100 + i) for i, s in
+                     enumerate([14, 14, 15, 15, 16, 16][:args.learn_graphs])]
+        data = learn.collect_traces(train, k, init=tgp.INIT_BLOCK, repair=True, wide=bool(grid))
+        npairs = sum(len(p) for _, p in data)
+        ntrain = len(train)
+        print(f"# traces: {len(train)} graphs, {npairs} passes ({time.perf_counter() - t0:.1f}s)",
               file=sys.stderr, flush=True)
+        dgs = [learn.DevGraph(g) for g, _ in data]
+        items = [(i, j) for i, (_, p) in enumerate(data) for j in range(len(p))]
+        rng = np.random.default_rng(0)
+        for ep in range(args.learn_epochs):
+            rng.shuffle(items)
+            loss = accu = 0.0
+            for i, j in items:
+                b, a = data[i][1][j]
+                l, acc = model.train_step(dgs[i], torch.from_numpy(b).cuda(),
+                                          torch.from_numpy(a).cuda())
+                loss += l
+                accu += acc
+            print(f"# epoch {ep}: loss {loss / len(items):.4f} accuracy {accu / len(items):.4f}",
+                  file=sys.stderr, flush=True)
+        del dgs, data
     train_s = time.perf_counter() - t0
     g = tgp.gen_grid3d(grid) if grid else tgp.gen_rmat(args.scale, args.edge_factor, SEED)
     dg = tgp.DeviceGraph(g)
@@ -265,7 +298,7 @@ def bench_learned(args):
         "unit": "M edges/s", "n_gpus": 1, "dtype": "bf16+fp32", "data": "synthetic",
         "config": {"workload": f"grid{grid}^3-k{k}" if grid else
                    f"rmat{args.scale}-ef{args.edge_factor}-k{k}", "vertices": g.n,
-                   "arcs": g.m, "train_graphs": len(train), "train_passes": npairs,
+                   "arcs": g.m, "train_graphs": ntrain, "train_passes": npairs,
                    "train_epochs": args.learn_epochs, "train_seconds": round(train_s, 2),
                    "coarsening": "repaired", "initial_partition": "block i*k/n_c"},
         "plain": plain, "learned": lrn,
@@ -293,7 +326,11 @@ def main():
                     help="graphs partitioned at once by the batched form's stream workers")
     ap.add_argument("--learned", action="store_true",
                     help="train the learned move proposer and compare it with the plain path")
-    ap.add_argument("--learn-graphs", type=int, default=6)
+    ap.add_argument("--learn-graphs", type=int, default=6,
+                    help="with --learned: training graphs; above 6, R-MAT scales 16-20 "
+                         "streamed one graph at a time")
+    ap.add_argument("--learn-pairs", type=int, default=4,
+                    help="recorded passes trained on per streamed graph")
     ap.add_argument("--learn-epochs", type=int, default=3)
     ap.add_argument("--learn-grid", type=int, default=0, metavar="SIDE",
                     help="with --learned: train on 3-D grids and evaluate on the SIDE^3 grid "

@@ -27,6 +27,13 @@ extern "C" {
 int tgpl_gemm(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb, int transB,
               float* C, int ldc, uint16_t* Cb, int M, int N, int K, float alpha, float beta,
               const float* bias, int relu, void* stream);
+/* The same product with a workspace of work_elems floats: when the output
+ * has few 64 x 64 tiles and K is long (the weight gradients, K = vertices),
+ * K is split into slices of about 4096 (at most 512 and work_elems / (M N)
+ * of them) whose partial products are added in slice order.              */
+int tgpl_gemm_ws(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb, int transB,
+                 float* C, int ldc, uint16_t* Cb, int M, int N, int K, float alpha, float beta,
+                 const float* bias, int relu, float* work, int64_t work_elems, void* stream);
 /* out = mean over neighbours of X (transpose 0) or its adjoint on a
  * symmetric graph, out[u] = sum_{v in N(u)} X[v]/deg v (transpose 1).        */
 int tgpl_mean_agg(int64_t n, const int64_t* xadj, const int32_t* adj, const uint16_t* X, int F,

@@ -55,36 +55,87 @@ def test_gemm_matches_fp32(shape):
     assert torch.allclose(dCb.cpu().float(), got, rtol=2 ** -8, atol=1e-6)
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 20000, 1, 0), (150, 40, 70001, 1, 0),
+                                   (130, 64, 9000, 1, 1)])
+def test_split_k_gemm(shape):
+    """Weight-gradient shapes (few output tiles, long K) through the split-K
+    path: against float64, each entry within K * 2^-23 * (|A| |B|) of the
+    exact product (FP32 sums of exact BF16 products in any order), and equal
+    on a repeat (the slices are added in a fixed order)."""
+    M, N, K, ta, tb = shape
+    g = torch.Generator().manual_seed(M + N + K)
+    A = bf16_round(torch.randn(K, M, generator=g) if ta else torch.randn(M, K, generator=g))
+    B = bf16_round(torch.randn(N, K, generator=g) if tb else torch.randn(K, N, generator=g))
+    opA = (A.t() if ta else A).double()
+    opB = (B.t() if tb else B).double()
+    ref = (opA @ opB).numpy()
+    bound = K * 2.0 ** -23 * (opA.abs() @ opB.abs()).numpy() + 1e-30
+    dA = A.to(torch.bfloat16).cuda()
+    dB = B.to(torch.bfloat16).cuda()
+    work = torch.empty(4 << 20, device="cuda")
+    L = learn._lib()
+    outs = []
+    for _ in range(2):
+        dC = torch.empty(M, N, device="cuda")
+        assert L.tgpl_gemm_ws(learn._p(dA), dA.shape[1], ta, learn._p(dB), dB.shape[1], tb,
+                              learn._p(dC), N, None, M, N, K, 1.0, 0.0, None, 0,
+                              learn._p(work), work.numel(), stream()) == 0
+        torch.cuda.synchronize()
+        outs.append(dC.cpu().double().numpy())
+    assert (np.abs(outs[0] - ref) <= bound).all(), np.max(np.abs(outs[0] - ref) / bound)
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 def small_graph():
     return tgp.gen_rmat(9, 8, 5, "hash16")
 
 
-def test_mean_aggregation_and_adjoint():
-    g = small_graph()
+def hub_graph():
+    """Two hub rows of 4999 and 4998 arcs: past the 2048-arc limit above which
+    a row is aggregated by a block of 16 waves instead of one wave."""
+    import graphs
+    xadj, adj, w = graphs.hub_graph(5000, 2, 6000, 3, wmax=5)
+    return tgp.Graph(xadj, adj, w)
+
+
+@pytest.mark.parametrize("which", ["rmat9", "hubs"])
+def test_mean_aggregation_and_adjoint(which):
+    """Against float64 sparse products.  Tolerance: an FP32 sum of d terms in
+    any order is within d * 2^-23 * sum|terms| of the exact sum (plus the
+    final division's half ulp), so each entry is held to that bound."""
+    import scipy.sparse as sp
+    g = small_graph() if which == "rmat9" else hub_graph()
     dg = learn.DevGraph(g)
-    F = 48
+    F = 64 if which == "hubs" else 48
     X = bf16_round(torch.randn(g.n, F, generator=torch.Generator().manual_seed(1)))
     deg = np.diff(g.xadj)
     src = np.repeat(np.arange(g.n), deg)
-    A = torch.zeros(g.n, g.n)
-    A[torch.from_numpy(src), torch.from_numpy(g.adjncy.astype(np.int64))] = 1.0
-    D = torch.from_numpy(np.maximum(deg, 1).astype(np.float32))
-    fwd = (A @ X) / D[:, None]
-    adj = A.t() @ (X / D[:, None])
+    A = sp.csr_matrix((np.ones(len(src)), (src, g.adjncy.astype(np.int64))), shape=(g.n, g.n))
+    D = np.maximum(deg, 1).astype(np.float64)
+    Xd = X.double().numpy()
+    fwd = (A @ Xd) / D[:, None]
+    fwd_abs = (A @ np.abs(Xd)) / D[:, None]
+    adj = A.T @ (Xd / D[:, None])
+    adj_abs = A.T @ (np.abs(Xd) / D[:, None])
+    cnt_in = np.asarray(A.sum(0)).ravel()
     L = learn._lib()
-    for transpose, ref in ((0, fwd), (1, adj)):
+    for transpose, ref, mag, terms in ((0, fwd, fwd_abs, deg), (1, adj, adj_abs, cnt_in)):
         out = torch.empty(g.n, F, dtype=torch.bfloat16, device="cuda")
         outf = torch.empty(g.n, F, device="cuda")
         assert L.tgpl_mean_agg(g.n, learn._p(dg.xadj), learn._p(dg.adj),
                                learn._p(X.to(torch.bfloat16).cuda()), F, transpose,
                                learn._p(out), learn._p(outf), stream()) == 0
         torch.cuda.synchronize()
-        assert torch.allclose(outf.cpu(), ref, rtol=1e-5, atol=1e-5)
+        got = outf.cpu().double().numpy()
+        bound = (np.maximum(terms, 1)[:, None] + 2) * 2.0 ** -23 * mag + 1e-30
+        assert (np.abs(got - ref) <= bound).all(), np.max(np.abs(got - ref) / bound)
+        np.testing.assert_array_equal(out.cpu().float().numpy(),
+                                      bf16_round(torch.from_numpy(got).float()).numpy())
 
 
-@pytest.mark.parametrize("k", [8, 130])
-def test_features_and_proposals(k):
-    g = small_graph()
+@pytest.mark.parametrize("k,which", [(8, "rmat9"), (130, "rmat9"), (16, "hubs")])
+def test_features_and_proposals(k, which):
+    g = small_graph() if which == "rmat9" else hub_graph()
     dg = learn.DevGraph(g)
     part = (np.arange(g.n) * k // g.n).astype(np.int32)
     F = learn.feature_count(k)
@@ -115,15 +166,16 @@ def test_features_and_proposals(k):
                           learn._p(gain), learn._p(tgt), stream()) == 0
     torch.cuda.synchronize()
     Zn = Z.numpy()
+    tgt_h = tgt.cpu().numpy()
+    gain_h = gain.cpu().numpy()
     for v in range(g.n):
         cur = part[v]
         order = [q for q in range(k) if q != cur]
         best = max(order, key=lambda q: (Zn[v, q], -q))
         d = T[v, best] - T[v, cur]
         ok = Zn[v, best] > Zn[v, cur] - np.float32(margin) and d > 0
-        assert tgt[v].item() == best
-        gv = gain[v].item()
-        assert (gv == d) if ok else np.isnan(gv)
+        assert tgt_h[v] == best
+        assert (gain_h[v] == d) if ok else np.isnan(gain_h[v])
 
 
 def reference_forward(model, g, part):

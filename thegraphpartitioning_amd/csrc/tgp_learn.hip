@@ -40,29 +40,36 @@ __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t
 // l % 32, in accumulator r.
 constexpr int GT = 64, GK = 32;
 
+// Split K (work != nullptr): block z covers k in [z Kc, (z + 1) Kc) and
+// stores its raw partial tile at work[z][M][N]; k_splitk_reduce adds the
+// slices in order and applies the epilogue.  Tile loads walk along the
+// operand's contiguous dimension.
 __global__ __launch_bounds__(256) void k_gemm_bf16(const bf16* A, int lda, int transA, const bf16* B,
                                                    int ldb, int transB, float* C, int ldc, bf16* Cb,
                                                    int M, int N, int K, float alpha, float beta,
-                                                   const float* bias, int relu) {
+                                                   const float* bias, int relu, float* work,
+                                                   int Kc) {
   __shared__ bf16 As[GT][GK + 4];
   __shared__ bf16 Bs[GK][GT + 4];
   const int t = threadIdx.x, l = t & 63, w = t >> 6;
   const int bm = blockIdx.x * GT, bn = blockIdx.y * GT;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int kbeg = work ? (int)blockIdx.z * Kc : 0;
+  const int kend = work ? (kbeg + Kc < K ? kbeg + Kc : K) : K;
   v16f acc = {0};
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  for (int k0 = kbeg; k0 < kend; k0 += GK) {
     for (int e = t; e < GT * GK; e += 256) {
-      const int i = e / GK, k = e % GK;
+      const int i = transA ? e % GT : e / GK, k = transA ? e / GT : e % GK;
       const int gi = bm + i, gk = k0 + k;
       bf16 v = 0;
-      if (gi < M && gk < K) v = transA ? A[(i64)gk * lda + gi] : A[(i64)gi * lda + gk];
+      if (gi < M && gk < kend) v = transA ? A[(i64)gk * lda + gi] : A[(i64)gi * lda + gk];
       As[i][k] = v;
     }
     for (int e = t; e < GK * GT; e += 256) {
-      const int k = e / GT, j = e % GT;
+      const int k = transB ? e % GK : e / GT, j = transB ? e / GK : e % GT;
       const int gk = k0 + k, gj = bn + j;
       bf16 v = 0;
-      if (gk < K && gj < N) v = transB ? B[(i64)gj * ldb + gk] : B[(i64)gk * ldb + gj];
+      if (gk < kend && gj < N) v = transB ? B[(i64)gj * ldb + gk] : B[(i64)gk * ldb + gj];
       Bs[k][j] = v;
     }
     __syncthreads();
@@ -83,7 +90,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const bf16* A, int lda, int t
   for (int r = 0; r < 16; r++) {
     const int i = bm + wr + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
     const int j = bn + wc + (l & 31);
-    if (i < M && j < N) {
+    if (work) {
+      if (i < M && j < N) work[((i64)blockIdx.z * M + i) * N + j] = acc[r];
+    } else if (i < M && j < N) {
       float v = alpha * acc[r];
       if (beta != 0.0f) v += beta * C[(i64)i * ldc + j];
       if (bias) v += bias[j];
@@ -94,31 +103,87 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const bf16* A, int lda, int t
   }
 }
 
+__global__ void k_splitk_reduce(const float* work, int S, int M, int N, float* C, int ldc,
+                                bf16* Cb, float alpha, float beta, const float* bias, int relu) {
+  const i64 MN = (i64)M * N;
+  for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < MN;
+       e += (i64)gridDim.x * blockDim.x) {
+    const int i = (int)(e / N), j = (int)(e % N);
+    float sum = 0.0f;
+    for (int z = 0; z < S; z++) sum += work[z * MN + e];
+    float v = alpha * sum;
+    if (beta != 0.0f) v += beta * C[(i64)i * ldc + j];
+    if (bias) v += bias[j];
+    if (relu && v < 0.0f) v = 0.0f;
+    C[(i64)i * ldc + j] = v;
+    if (Cb) Cb[(i64)i * ldc + j] = f2bf(v);
+  }
+}
+
 // ------------------------------------------------------------ aggregation --
 // out[v] = (1/deg v) sum_{u in N(v)} X[u]   (transpose = 0), or
 // out[u] = sum_{v in N(u)} X[v] / deg v     (transpose = 1: the adjoint of the
-// mean on a symmetric graph).  One wave per vertex, lanes over features; BF16
-// in, FP32 sums, BF16 and optional FP32 out.
+// mean on a symmetric graph).  BF16 in, FP32 sums, BF16 and optional FP32 out.
+// Rows of more than AGG_HEAVY arcs go to a 1024-thread block each
+// (k_mean_agg_heavy), the others to a wave each.  Lanes hold features; a
+// row's neighbour ids (and, for the adjoint, their degrees) are loaded 64 at a
+// time by the lanes and broadcast with readlane, so the feature loads of a
+// step are independent of one another.
+constexpr i64 AGG_HEAVY = 2048;
+constexpr int AGG_U = 16;
+constexpr int HB_WAVES = 16;  // waves of a heavy-row block
+
+// Sum over the arcs base, base + stride, ... (64 consecutive arcs per step)
+// of row [.., s1): X[u][f], or X[u][f] / deg u for the adjoint.
+__device__ __forceinline__ float agg_range(const i64* xadj, const i32* adj, const bf16* X, int F,
+                                           int f, bool fok, int transpose, i64 b, i64 s1,
+                                           i64 stride, int l) {
+  float s = 0.0f;
+  for (i64 base = b; base < s1; base += stride) {
+    const i64 i = base + l;
+    const i32 myu = i < s1 ? adj[i] : 0;
+    float mydu = 0.0f;
+    if (transpose && i < s1) mydu = (float)(xadj[myu + 1] - xadj[myu]);
+    const int cnt = (int)(s1 - base < WAVE ? s1 - base : WAVE);
+    for (int j0 = 0; j0 < cnt; j0 += AGG_U) {
+      float xs[AGG_U];
+#pragma unroll
+      for (int jj = 0; jj < AGG_U; jj++) {
+        xs[jj] = 0.0f;
+        if (j0 + jj < cnt) {
+          const i32 u = __builtin_amdgcn_readlane(myu, j0 + jj);
+          if (fok) xs[jj] = bf2f(X[(i64)u * F + f]);
+        }
+      }
+#pragma unroll
+      for (int jj = 0; jj < AGG_U; jj++) {
+        if (j0 + jj < cnt) {
+          if (transpose) {
+            const float du =
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mydu), j0 + jj));
+            s += du > 0.0f ? xs[jj] / du : 0.0f;
+          } else {
+            s += xs[jj];
+          }
+        }
+      }
+    }
+  }
+  return s;
+}
+
 __global__ __launch_bounds__(256) void k_mean_agg(i64 n, const i64* xadj, const i32* adj,
                                                   const bf16* X, int F, int transpose, bf16* out,
                                                   float* outf) {
   const int l = threadIdx.x & 63;
   for (i64 v = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); v < n; v += (i64)gridDim.x * 4) {
     const i64 s0 = xadj[v], s1 = xadj[v + 1];
+    if (s1 - s0 > AGG_HEAVY) continue;
     for (int f0 = 0; f0 < F; f0 += WAVE) {
       const int f = f0 + l;
-      float s = 0.0f;
-      if (f < F) {
-        for (i64 i = s0; i < s1; i++) {
-          const i32 u = adj[i];
-          const float x = bf2f(X[(i64)u * F + f]);
-          if (transpose) {
-            const i64 du = xadj[u + 1] - xadj[u];
-            s += du > 0 ? x / (float)du : 0.0f;
-          } else {
-            s += x;
-          }
-        }
+      const bool fok = f < F;
+      float s = agg_range(xadj, adj, X, F, f, fok, transpose, s0, s1, WAVE, l);
+      if (fok) {
         if (!transpose && s1 > s0) s /= (float)(s1 - s0);
         out[(i64)v * F + f] = f2bf(s);
         if (outf) outf[(i64)v * F + f] = s;
@@ -127,41 +192,139 @@ __global__ __launch_bounds__(256) void k_mean_agg(i64 n, const i64* xadj, const 
   }
 }
 
+// Heavy rows: block b takes the vertices b, b + G, b + 2G, ... (G blocks,
+// so neighbouring hub ids land in different blocks), 1024 of them per round;
+// its 16 waves split each row over AGG_HEAVY arcs and the partial sums are
+// added in wave order.
+__global__ __launch_bounds__(1024) void k_mean_agg_heavy(i64 n, const i64* xadj, const i32* adj,
+                                                         const bf16* X, int F, int transpose,
+                                                         bf16* out, float* outf) {
+  __shared__ i32 hv[1024];
+  __shared__ int nh;
+  __shared__ float ps[HB_WAVES][WAVE];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const i64 G = gridDim.x;
+  for (i64 r0 = 0; (i64)blockIdx.x + r0 * G < n; r0 += blockDim.x) {
+    if (threadIdx.x == 0) nh = 0;
+    __syncthreads();
+    {
+      const i64 v = (i64)blockIdx.x + (r0 + threadIdx.x) * G;
+      if (v < n && xadj[v + 1] - xadj[v] > AGG_HEAVY) hv[atomicAdd(&nh, 1)] = (i32)v;
+    }
+    __syncthreads();
+    const int cnt = nh;
+    for (int h = 0; h < cnt; h++) {
+      const i64 v = hv[h];
+      const i64 s0 = xadj[v], s1 = xadj[v + 1];
+      for (int f0 = 0; f0 < F; f0 += WAVE) {
+        const int f = f0 + l;
+        const bool fok = f < F;
+        ps[wv][l] = agg_range(xadj, adj, X, F, f, fok, transpose, s0 + (i64)wv * WAVE, s1,
+                              (i64)HB_WAVES * WAVE, l);
+        __syncthreads();
+        if (wv == 0 && fok) {
+          float t = 0.0f;
+          for (int w2 = 0; w2 < HB_WAVES; w2++) t += ps[w2][l];
+          if (!transpose) t /= (float)(s1 - s0);
+          out[(i64)v * F + f] = f2bf(t);
+          if (outf) outf[(i64)v * F + f] = t;
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
 // --------------------------------------------------------------- features --
-// one wave per vertex: part tallies in LDS (k <= FEAT_MAXK), then the row of X
+// Part tallies T_v (integer LDS atomics, exact) and deg_v = sum of v's arc
+// weights, then the row of X.  Rows of more than FEAT_HEAVY arcs go to a
+// 1024-thread block each (k_features_heavy), the others to a wave each.
 constexpr int FEAT_MAXK = 256;
+constexpr i64 FEAT_HEAVY = 2048;
+
+__device__ __forceinline__ void feature_row(bf16* row, int F, int k, const i32* tl, i64 deg,
+                                            i32 cur, i64 d, i32* tally_row, int l) {
+  const float degf = (float)deg;
+  for (int f = l; f < F; f += WAVE) {
+    float x = 0.0f;
+    if (f < k) x = deg > 0 ? (float)tl[f] / degf : 0.0f;
+    else if (f < 2 * k) x = (f - k) == cur ? 1.0f : 0.0f;
+    else if (f == 2 * k) x = log2f(1.0f + (float)d) / 16.0f;
+    else if (f == 2 * k + 1) x = 1.0f;
+    row[f] = f2bf(x);
+  }
+  if (tally_row)
+    for (int q = l; q < k; q += WAVE) tally_row[q] = tl[q];
+}
+
 __global__ __launch_bounds__(256) void k_features(i64 n, const i64* xadj, const i32* adj,
                                                   const i32* w, const i32* part, int k, int F,
                                                   bf16* X, i32* tally) {
-  __shared__ float tl[4][FEAT_MAXK];
+  __shared__ i32 tl[4][FEAT_MAXK];
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (i64 v = (i64)blockIdx.x * 4 + wv; v < n; v += (i64)gridDim.x * 4) {
-    for (int q = l; q < k; q += WAVE) tl[wv][q] = 0.0f;
-    __builtin_amdgcn_wave_barrier();
     const i64 s0 = xadj[v], s1 = xadj[v + 1];
-    float deg = 0.0f;
+    if (s1 - s0 > FEAT_HEAVY) continue;
+    for (int q = l; q < k; q += WAVE) tl[wv][q] = 0;
+    __builtin_amdgcn_wave_barrier();
+    i64 deg = 0;
     for (i64 i = s0 + l; i < s1; i += WAVE) {
       const i32 p = part[adj[i]];
-      const float wt = (float)w[i];
+      const i32 wt = w[i];
       deg += wt;
       if (p >= 0 && p < k) atomicAdd(&tl[wv][p], wt);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o);
     __builtin_amdgcn_wave_barrier();
-    const i32 cur = part[v];
-    bf16* row = X + (i64)v * F;
-    for (int f = l; f < F; f += WAVE) {
-      float x = 0.0f;
-      if (f < k) x = deg > 0.0f ? tl[wv][f] / deg : 0.0f;
-      else if (f < 2 * k) x = (f - k) == cur ? 1.0f : 0.0f;
-      else if (f == 2 * k) x = log2f(1.0f + (float)(s1 - s0)) / 16.0f;
-      else if (f == 2 * k + 1) x = 1.0f;
-      row[f] = f2bf(x);
-    }
-    if (tally)
-      for (int q = l; q < k; q += WAVE) tally[v * k + q] = (i32)tl[wv][q];
+    feature_row(X + v * F, F, k, tl[wv], deg, part[v], s1 - s0, tally ? tally + v * k : nullptr,
+                l);
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Heavy rows, distributed over blocks as in k_mean_agg_heavy.
+__global__ __launch_bounds__(1024) void k_features_heavy(i64 n, const i64* xadj, const i32* adj,
+                                                         const i32* w, const i32* part, int k,
+                                                         int F, bf16* X, i32* tally) {
+  __shared__ i32 hv[1024];
+  __shared__ int nh;
+  __shared__ i32 tl[FEAT_MAXK];
+  __shared__ i64 dg[HB_WAVES];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const i64 G = gridDim.x;
+  for (i64 r0 = 0; (i64)blockIdx.x + r0 * G < n; r0 += blockDim.x) {
+    if (threadIdx.x == 0) nh = 0;
+    __syncthreads();
+    {
+      const i64 v = (i64)blockIdx.x + (r0 + threadIdx.x) * G;
+      if (v < n && xadj[v + 1] - xadj[v] > FEAT_HEAVY) hv[atomicAdd(&nh, 1)] = (i32)v;
+    }
+    __syncthreads();
+    const int cnt = nh;
+    for (int h = 0; h < cnt; h++) {
+      const i64 v = hv[h];
+      const i64 s0 = xadj[v], s1 = xadj[v + 1];
+      for (int q = threadIdx.x; q < k; q += blockDim.x) tl[q] = 0;
+      __syncthreads();
+      i64 deg = 0;
+      for (i64 i = s0 + threadIdx.x; i < s1; i += blockDim.x) {
+        const i32 p = part[adj[i]];
+        const i32 wt = w[i];
+        deg += wt;
+        if (p >= 0 && p < k) atomicAdd(&tl[p], wt);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o);
+      if (l == 0) dg[wv] = deg;
+      __syncthreads();
+      if (wv == 0) {
+        i64 t = 0;
+        for (int w2 = 0; w2 < HB_WAVES; w2++) t += dg[w2];
+        feature_row(X + v * F, F, k, tl, t, part[v], s1 - s0, tally ? tally + v * k : nullptr, l);
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -286,21 +449,50 @@ unsigned grid(i64 work, int per_block, unsigned cap = 65536) {
 
 extern "C" {
 
-int tgpl_gemm(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb, int transB,
-              float* C, int ldc, uint16_t* Cb, int M, int N, int K, float alpha, float beta,
-              const float* bias, int relu, void* stream) {
+int tgpl_gemm_ws(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb, int transB,
+                 float* C, int ldc, uint16_t* Cb, int M, int N, int K, float alpha, float beta,
+                 const float* bias, int relu, float* work, int64_t work_elems, void* stream) {
   if (M <= 0 || N <= 0 || K <= 0) return -1;
   dim3 g((unsigned)((M + GT - 1) / GT), (unsigned)((N + GT - 1) / GT));
   if (g.y > 65535) return -1;
+  // split K when the output has few tiles and K is long: about 4096 per slice,
+  // at most 512 slices and what the workspace holds
+  const i64 tiles = (i64)g.x * g.y;
+  i64 S = 1;
+  if (work && tiles < 256 && K > 8192) {
+    S = ((i64)K + 4095) / 4096;
+    if (S > 512) S = 512;
+    if (S > work_elems / ((i64)M * N)) S = work_elems / ((i64)M * N);
+  }
+  if (S <= 1) {
+    k_gemm_bf16<<<g, 256, 0, (hipStream_t)stream>>>(A, lda, transA, B, ldb, transB, C, ldc, Cb,
+                                                     M, N, K, alpha, beta, bias, relu, nullptr, 0);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  i64 Kc = ((i64)K + S - 1) / S;
+  Kc = (Kc + GK - 1) / GK * GK;
+  S = ((i64)K + Kc - 1) / Kc;
+  g.z = (unsigned)S;
   k_gemm_bf16<<<g, 256, 0, (hipStream_t)stream>>>(A, lda, transA, B, ldb, transB, C, ldc, Cb, M,
-                                                   N, K, alpha, beta, bias, relu);
+                                                   N, K, alpha, beta, bias, relu, work, (int)Kc);
+  k_splitk_reduce<<<grid((i64)M * N, 256, 4096), 256, 0, (hipStream_t)stream>>>(
+      work, (int)S, M, N, C, ldc, Cb, alpha, beta, bias, relu);
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int tgpl_gemm(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb, int transB,
+              float* C, int ldc, uint16_t* Cb, int M, int N, int K, float alpha, float beta,
+              const float* bias, int relu, void* stream) {
+  return tgpl_gemm_ws(A, lda, transA, B, ldb, transB, C, ldc, Cb, M, N, K, alpha, beta, bias,
+                      relu, nullptr, 0, stream);
 }
 
 int tgpl_mean_agg(int64_t n, const int64_t* xadj, const int32_t* adj, const uint16_t* X, int F,
                   int transpose, uint16_t* out, float* outf, void* stream) {
   if (n <= 0 || F <= 0) return -1;
   k_mean_agg<<<grid(n, 4), 256, 0, (hipStream_t)stream>>>(n, xadj, adj, X, F, transpose, out, outf);
+  k_mean_agg_heavy<<<grid(n, 1024, 2048), 1024, 0, (hipStream_t)stream>>>(n, xadj, adj, X, F,
+                                                                          transpose, out, outf);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -308,6 +500,8 @@ int tgpl_features(int64_t n, const int64_t* xadj, const int32_t* adj, const int3
                   const int32_t* part, int k, int F, uint16_t* X, int32_t* tally, void* stream) {
   if (n <= 0 || k < 1 || k > FEAT_MAXK || F < 2 * k + 2) return -1;
   k_features<<<grid(n, 4), 256, 0, (hipStream_t)stream>>>(n, xadj, adj, w, part, k, F, X, tally);
+  k_features_heavy<<<grid(n, 1024, 2048), 1024, 0, (hipStream_t)stream>>>(n, xadj, adj, w, part,
+                                                                          k, F, X, tally);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -39,6 +39,7 @@ def _lib():
         return L
     P, I, I64, F32 = C.c_void_p, C.c_int, C.c_int64, C.c_float
     L.tgpl_gemm.argtypes = [P, I, I, P, I, I, P, I, P, I, I, I, F32, F32, P, I, P]
+    L.tgpl_gemm_ws.argtypes = L.tgpl_gemm.argtypes[:-1] + [P, I64, P]
     L.tgpl_mean_agg.argtypes = [I64, P, P, P, I, I, P, P, P]
     L.tgpl_features.argtypes = [I64, P, P, P, P, I, I, P, P, P]
     L.tgpl_softmax_xent.argtypes = [I64, I, P, P, P, F32, F32, F32, P, P, P, P]
@@ -101,6 +102,8 @@ class GNN:
         self.Wb = {n: torch.empty(w.shape, dtype=torch.bfloat16, device=device)
                    for n, w in self.W.items() if n[0] == "W"}
         self.m = {n: torch.zeros_like(w) for n, w in self.W.items()}
+        # split-K workspace of the weight-gradient products (K = vertices)
+        self.work = torch.empty(8 << 20, device=device)
         self.v = {n: torch.zeros_like(w) for n, w in self.W.items()}
         self.step = 0
         self._refresh_bf16()
@@ -163,10 +166,11 @@ class GNN:
         G = {}
         G["W2a"] = torch.empty(Hd, k, device=self.device)
         G["W2b"] = torch.empty(Hd, k, device=self.device)
-        _ok(L.tgpl_gemm(_p(c["Hb"]), Hd, 1, _p(dZb), k, 0, _p(G["W2a"]), k, None, Hd, k, n,
-                        1.0, 0.0, None, 0, s), "gemm")
-        _ok(L.tgpl_gemm(_p(c["M1"]), Hd, 1, _p(dZb), k, 0, _p(G["W2b"]), k, None, Hd, k, n,
-                        1.0, 0.0, None, 0, s), "gemm")
+        ws, wn = _p(self.work), self.work.numel()
+        _ok(L.tgpl_gemm_ws(_p(c["Hb"]), Hd, 1, _p(dZb), k, 0, _p(G["W2a"]), k, None, Hd, k, n,
+                           1.0, 0.0, None, 0, ws, wn, s), "gemm")
+        _ok(L.tgpl_gemm_ws(_p(c["M1"]), Hd, 1, _p(dZb), k, 0, _p(G["W2b"]), k, None, Hd, k, n,
+                           1.0, 0.0, None, 0, ws, wn, s), "gemm")
         G["b2"] = torch.empty(k, device=self.device)
         _ok(L.tgpl_colsum(n, k, _p(dZ), _p(G["b2"]), s), "colsum")
         dH = torch.empty(n, Hd, device=self.device)
@@ -185,15 +189,15 @@ class GNN:
         _ok(L.tgpl_relu_bwd(n * Hd, _p(c["H"]), _p(dH), _p(dHb), s), "relu")
         G["W1a"] = torch.empty(F, Hd, device=self.device)
         G["W1b"] = torch.empty(F, Hd, device=self.device)
-        _ok(L.tgpl_gemm(_p(c["X"]), F, 1, _p(dHb), Hd, 0, _p(G["W1a"]), Hd, None, F, Hd, n,
-                        1.0, 0.0, None, 0, s), "gemm")
+        _ok(L.tgpl_gemm_ws(_p(c["X"]), F, 1, _p(dHb), Hd, 0, _p(G["W1a"]), Hd, None, F, Hd, n,
+                           1.0, 0.0, None, 0, ws, wn, s), "gemm")
         # H's mean term: dP = adjoint mean of dH, dW1b = X^T dP
         dPf = torch.empty(n, Hd, device=self.device)
         dPb = torch.empty(n, Hd, dtype=bf, device=self.device)
         _ok(L.tgpl_mean_agg(n, _p(dg.xadj), _p(dg.adj), _p(dHb), Hd, 1, _p(dPb), _p(dPf), s),
             "agg")
-        _ok(L.tgpl_gemm(_p(c["X"]), F, 1, _p(dPb), Hd, 0, _p(G["W1b"]), Hd, None, F, Hd, n,
-                        1.0, 0.0, None, 0, s), "gemm")
+        _ok(L.tgpl_gemm_ws(_p(c["X"]), F, 1, _p(dPb), Hd, 0, _p(G["W1b"]), Hd, None, F, Hd, n,
+                           1.0, 0.0, None, 0, ws, wn, s), "gemm")
         G["b1"] = torch.empty(Hd, device=self.device)
         _ok(L.tgpl_colsum(n, Hd, _p(dH), _p(G["b1"]), s), "colsum")
         self.step += 1
