@@ -262,3 +262,14 @@ def test_nearly_symmetric_sorted_rows(oracle, kind, repair):
     xadj, adj, w = _drop_or_reweight(kind, 60)
     gp, op = run_both(oracle, xadj, adj, w, 8, repair=repair, floor=40)
     assert_same(gp, op, f"nearly symmetric ({kind}) repair={repair}")
+
+
+@pytest.mark.parametrize("token,gain", [("1", "16"), ("4", "8"), ("8", "2")])
+def test_token_and_gain_lane_widths(oracle, monkeypatch, token, gain):
+    """Every lanes-per-row setting of token passing (TGP_TOKEN_LANES) and of
+    the gain pass over kept tallies (TGP_GAIN_LANES) gives the oracle's trace."""
+    monkeypatch.setenv("TGP_TOKEN_LANES", token)
+    monkeypatch.setenv("TGP_GAIN_LANES", gain)
+    g = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 8)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 32, repair=True, floor=100)
+    assert_same(gp, op, f"token lanes {token}, gain lanes {gain}")
