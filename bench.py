@@ -222,11 +222,19 @@ def bench_learned(args):
     if args.learn_graphs > 6 and not grid:
         # streamed: R-MAT scales 16-20 in turn, each graph's passes recorded,
         # args.learn_pairs of them (evenly spaced) trained on, then dropped
+        from concurrent.futures import ThreadPoolExecutor
         npairs = 0
         loss = accu = 0.0
         steps = 0
+        # graphs are generated on 8 host threads, 16 ahead of the GPU work
+        pool = ThreadPoolExecutor(8)
+        gen = lambda i: tgp.gen_rmat(16 + i % 5, 16, SEED + 100 + i)
+        ahead = [pool.submit(gen, i) for i in range(min(16, args.learn_graphs))]
         for i in range(args.learn_graphs):
-            g = tgp.gen_rmat(16 + i % 5, 16, SEED + 100 + i)
+            g = ahead[i].result()
+            ahead[i] = None
+            if i + 16 < args.learn_graphs:
+                ahead.append(pool.submit(gen, i + 16))
             ((_, pairs),) = learn.collect_traces([g], k, init=tgp.INIT_BLOCK, repair=True)
             if not pairs:
                 continue
@@ -247,6 +255,7 @@ def bench_learned(args):
                       file=sys.stderr, flush=True)
                 loss = accu = 0.0
                 steps = 0
+        pool.shutdown()
         ntrain = args.learn_graphs
     else:
         if grid:
