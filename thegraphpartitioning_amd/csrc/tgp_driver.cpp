@@ -776,10 +776,21 @@ i64 Run::compute_cut(Level& lv, i64 nedges, const i32* part) {
 int Run::initial_partition(DBuf<i32>& part, i64* npairs) {
   Level& cl = *L.back();
   const i64 n = cl.n;
-  // unique undirected pairs (gpu_part.cu:810-827), counted on the device
-  i64 cap = 1;
-  while (cap < 2 * cl.m + 2) cap <<= 1;
-  {
+  // unique undirected pairs (gpu_part.cu:810-827), counted on the device: on
+  // a symmetric level whose rows hold no repeats, half the non-loop arcs;
+  // otherwise through a hash table of the pairs
+  *npairs = -1;
+  if (cl.symmetric && cl.simple_ws) {
+    C.prof.begin(KID_PAIRS);
+    HIP_TRY(hipMemsetAsync(small.p + 2, 0, sizeof(i64), st()));
+    launch_count_pair_arcs(cl.dg(), (u64*)(small.p + 2), st());
+    C.prof.end(KID_PAIRS, 4.0 * cl.m + 8.0 * n);
+    const i64 arcs = read_i64(small.p + 2);
+    if ((arcs & 1) == 0) *npairs = arcs / 2;
+  }
+  if (*npairs < 0) {
+    i64 cap = 1;
+    while (cap < 2 * cl.m + 2) cap <<= 1;
     DBuf<u64> table(&C.pool, cap);
     C.prof.begin(KID_PAIRS);
     launch_fill_u64(table.p, cap, ~0ull, st());

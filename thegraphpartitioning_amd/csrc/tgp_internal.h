@@ -201,6 +201,10 @@ void launch_rep_long(DGraph g, const i32* tb, const i32* new_ids, const RepLong&
 void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s);
 // Unique undirected pairs (min,max), u != v, 0 <= u < n, true row ends.
 void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s);
+// count += arcs v -> u with 0 <= u < n, u != v (same row ranges as
+// launch_pairs); half of it is the unique pair count on a symmetric level
+// whose rows hold no repeats
+void launch_count_pair_arcs(DGraph g, u64* count, hipStream_t s);
 
 // ---- projection / counts ------------------------------------------------------
 void launch_project(const i32* tb, const i32* part_c, i64 n_c, i32* part_f, i64 n_f,

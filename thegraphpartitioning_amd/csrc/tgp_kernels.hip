@@ -1618,6 +1618,24 @@ __global__ __launch_bounds__(256) void k_pairs(DGraph g, u64* table, i64 cap, u6
   if (lane == 0 && local) atomicAdd((unsigned long long*)count, (unsigned long long)local);
 }
 
+// Arcs v -> u with 0 <= u < n and u != v over the same row ranges as k_pairs.
+// On a symmetric level whose rows hold no repeats every such pair {u, v}
+// appears exactly twice, so the unique pairs are half this count.
+__global__ __launch_bounds__(256) void k_count_pair_arcs(DGraph g, u64* count) {
+  u64 local = 0;
+  const int lane = lane_id();
+  for (i64 v = gwave(); v < g.n; v += nwaves()) {
+    const i64 s0 = g.xadj[v];
+    const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : g.m;
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      local += (u >= 0 && u < g.n && u != v) ? 1 : 0;
+    }
+  }
+  local = wave_sum(local);
+  if (lane == 0 && local) atomicAdd((unsigned long long*)count, (unsigned long long)local);
+}
+
 // --------------------------------------------------------------- projection --
 // gpu_part.cu:763-772 then the host recount 1244-1252 (-1 -> part 0).
 __global__ void k_project(const i32* tb, const i32* part_c, i64 n_c, i32* part_f, i64 n_f,
@@ -3160,6 +3178,9 @@ void launch_rep_rows(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const
 }
 void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s) {
   k_pairs<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, table, cap, count);
+}
+void launch_count_pair_arcs(DGraph g, u64* count, hipStream_t s) {
+  k_count_pair_arcs<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, count);
 }
 
 void launch_project(const i32* tb, const i32* part_c, i64 n_c, i32* part_f, i64 n_f, i32 k,
