@@ -639,14 +639,18 @@ constexpr i32 PROP_NONE = -5;  // proposal: no free neighbour (final)
 
 __device__ __forceinline__ i32 ld_st(const i32* st, i64 i) { return ld_mate(st, i); }
 
-// Among neighbours v of x with v < lim, v != x: BLK_TAKEN if some decided v
-// has st[v] == me, else the largest undecided v, else BLK_NONE.
+// Among neighbours v of x with v < lim, v != x, at row positions from `from`
+// on, one trip of WAVE * SCAN_U entries at a time: BLK_TAKEN if a decided v of
+// the trip has st[v] == me, else the largest undecided v of the first trip
+// that holds one (`from` is set to that trip's start), else BLK_NONE (`from`
+// set to the row's end).  A decided word never changes, so entries before
+// `from` stay decided and not equal to me: a later scan of the same question
+// (same x, lim, me) resumes there, and every long row is read about once
+// per question instead of once per blocker.
 __device__ __forceinline__ i32 wave_scan_pending(const DGraph& g, i32 x, i32 lim, i32 me,
-                                                 const i32* st, int lane) {
+                                                 const i32* st, int lane, i32& from) {
   const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
-  i32 best = BLK_NONE;
-  bool taken = false;
-  for (i64 base = s0; base < s1; base += WAVE * SCAN_U) {
+  for (i64 base = s0 + from; base < s1; base += WAVE * SCAN_U) {
     i32 v[SCAN_U];
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++) {
@@ -656,19 +660,26 @@ __device__ __forceinline__ i32 wave_scan_pending(const DGraph& g, i32 x, i32 lim
     i32 sv[SCAN_U];
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++) sv[j] = (v[j] < lim && v[j] != x) ? ld_st(st, v[j]) : -4;
+    i32 best = BLK_NONE;
+    bool taken = false;
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++) {
       if (sv[j] == ST_UNDEC) best = v[j] > best ? v[j] : best;
       taken |= sv[j] == me;
     }
-  }
-  if (__ballot(taken)) return BLK_TAKEN;
+    if (__ballot(taken)) return BLK_TAKEN;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const i32 t2 = __shfl_xor(best, o);
-    best = t2 > best ? t2 : best;
+    for (int o = 32; o > 0; o >>= 1) {
+      const i32 t2 = __shfl_xor(best, o);
+      best = t2 > best ? t2 : best;
+    }
+    if (best != BLK_NONE) {
+      from = (i32)(base - s0);
+      return best;
+    }
   }
-  return best;
+  from = (i32)(s1 - s0);
+  return BLK_NONE;
 }
 
 // The 4 best free (st < 0) neighbours of t by (weight, first position), as
@@ -761,6 +772,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
   i64 base = -1;
   // per-vertex state, uniform within a sub-group
   bool own = false, tshort = false, ushort_ = false, uknown = false;
+  // long rows: where the next A scan of t's row and C scan of u's row resume
+  i32 apos = 0, cpos = 0;
   i32 t = -1, a = BLK_NONE, u = -1, b = BLK_NONE;
   // long rows: lanes 0-3 of the sub-group hold the 4 best free candidates of
   // the last full proposal scan (pci), ph = the entry now proposed (4: none)
@@ -782,6 +795,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
       u = own ? prop[t] : -1;
       b = own ? blkC[t] : BLK_NONE;
       uknown = false;
+      cpos = 0;
+      apos = 0;
       pci = -1;
       ph = 4;
       ts0 = own ? g.xadj[t] : 0;
@@ -867,6 +882,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
           s_u = key ? ks : ST_UNDEC;
           b = BLK_UNKNOWN;
           uknown = false;
+          cpos = 0;
         }
       }
     }
@@ -875,12 +891,14 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
+      i32 from = __shfl(apos, l);
       c_aL++;
-      c_arcsL += g.xadj[x + 1] - g.xadj[x];
-      const i32 r = wave_scan_pending(g, x, x, x, st, lane);
+      c_arcsL += g.xadj[x + 1] - g.xadj[x] - from;
+      const i32 r = wave_scan_pending(g, x, x, x, st, lane, from);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) own = false;
         else a = r;
+        apos = from;
       }
       need &= need - 1;
     }
@@ -898,6 +916,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
           s_u = __shfl(sv, r, CH_SG);
           b = BLK_UNKNOWN;
           uknown = false;
+          cpos = 0;
           ph = r;
         } else {
           ph = 4;
@@ -924,6 +943,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
         s_u = sr;
         b = BLK_UNKNOWN;
         uknown = false;
+        cpos = 0;
       }
       need &= need - 1;
     }
@@ -985,9 +1005,10 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(u, l);
       const i32 lim = __shfl(t, l);
+      i32 from = __shfl(cpos, l);
       c_cL++;
-      c_arcsL += g.xadj[x + 1] - g.xadj[x];
-      const i32 r = wave_scan_pending(g, x, lim, x, st, lane);
+      c_arcsL += g.xadj[x + 1] - g.xadj[x] - from;
+      const i32 r = wave_scan_pending(g, x, lim, x, st, lane, from);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) {
           u = BLK_UNKNOWN;
@@ -995,6 +1016,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
         } else {
           b = r;
         }
+        cpos = from;
       }
       need &= need - 1;
     }
