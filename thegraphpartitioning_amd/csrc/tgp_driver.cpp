@@ -906,6 +906,10 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     DBuf<i32> H(&C.pool, select_scratch_elems(n, k));
     DBuf<i32> gscr(&C.pool, std::max<i64>(gain_gscratch_elems(k), 1));
     std::vector<i32> hpc((size_t)k);
+    // a pass's count, cut, selection counters and part counts go to the host
+    // in one copy when they fit the pinned staging area
+    const bool pack = 6 + (i64)k <= 2 * (i64)Context::PINNED_ELEMS;
+    DBuf<i64> pstat(&C.pool, pack ? 3 + ((size_t)k + 1) / 2 : 1);
     const double arc_b = unit ? 8.0 : 12.0;
 
     DBuf<u64> cutacc(&C.pool, 1);
@@ -961,7 +965,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
                                mv_dst.p, mv_gain.p, outc.p, st());
           C.prof.end(KID_COMPACT, 24.0 * n);
-          HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
+          if (!pack)
+            HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
         } else if (pending) {
           C.prof.begin(KID_CUT);
           launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
@@ -970,7 +975,13 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         } else {
           break;
         }
-        if (pending) {
+        if (pack) {
+          launch_pass_status(run ? (const i64*)small.p : nullptr, pending ? cutacc.p : nullptr,
+                             pending ? ctr_prev : nullptr, pending ? pc.p : nullptr, k, pstat.p,
+                             st());
+          const size_t bytes = pending ? 24 + 4 * (size_t)k : 8;
+          HIP_TRY(hipMemcpyAsync(C.pinned, pstat.p, bytes, hipMemcpyDeviceToHost, st()));
+        } else if (pending) {
           HIP_TRY(hipMemcpyAsync(C.pinned + 1, cutacc.p, sizeof(u64), hipMemcpyDeviceToHost,
                                  st()));
           HIP_TRY(hipMemcpyAsync(C.pinned + 2, ctr_prev, 2 * sizeof(i32), hipMemcpyDeviceToHost,
@@ -978,6 +989,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
         }
         C.sync();
+        if (pack && pending) std::memcpy(hpc.data(), (const i32*)C.pinned + 6, k * sizeof(i32));
         if (pending) {
           // the previous pass's bookkeeping, gpu_part.cu:605-640 / 707-744
           pending = false;

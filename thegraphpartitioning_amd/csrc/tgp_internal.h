@@ -307,6 +307,10 @@ constexpr int ACC_SLOTS = 256;
 constexpr int ACC_STRIDE = 16;
 constexpr i64 ACC_ELEMS = (i64)ACC_SLOTS * ACC_STRIDE;
 void launch_acc_finish(u64* acc, u64* out, hipStream_t s);
+// out[0] = *m, out[1] = *cut, ((i32*)out)[4..5] = ctr[0..1], ((i32*)out)[6..6+k)
+// = pc; a null input leaves zeros (out holds 3 + (k + 1) / 2 words)
+void launch_pass_status(const i64* m, const u64* cut, const i32* ctr, const i32* pc, i32 k,
+                        i64* out, hipStream_t s);
 // asrc[i] = source vertex of arc i (built once per level)
 void launch_arc_source(DGraph g, i32* asrc, hipStream_t s);
 // Edge cut over arcs [0, e_end) with e_end = max(xadj[n-1], nedges): exactly

@@ -2999,6 +2999,23 @@ __global__ __launch_bounds__(256) void k_cut_arcs(const i32* asrc, const i32* ad
   block_add_u64(s, acc);
 }
 
+// One word block with a pass's host-side inputs, copied to the host at once:
+// out[0] = *m, out[1] = *cut, out32[4..5] = ctr[0..1], out32[6..6+k) = pc
+// (absent inputs leave zeros).
+__global__ __launch_bounds__(256) void k_pass_status(const i64* m, const u64* cut,
+                                                     const i32* ctr, const i32* pc, i32 k,
+                                                     i64* out) {
+  i32* o32 = (i32*)out;
+  if (threadIdx.x == 0) {
+    out[0] = m ? *m : 0;
+    out[1] = cut ? (i64)*cut : 0;
+    o32[4] = ctr ? ctr[0] : 0;
+    o32[5] = ctr ? ctr[1] : 0;
+  }
+  if (pc)
+    for (int q = threadIdx.x; q < k; q += blockDim.x) o32[6 + q] = pc[q];
+}
+
 // *out = sum of the accumulator slots; the slots are cleared for the next use.
 __global__ __launch_bounds__(ACC_SLOTS) void k_acc_finish(u64* acc, u64* out) {
   u64* slot = &acc[threadIdx.x * ACC_STRIDE];
@@ -3430,6 +3447,10 @@ void launch_arc_source(DGraph g, i32* asrc, hipStream_t s) {
   k_arc_source<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, asrc);
 }
 
+void launch_pass_status(const i64* m, const u64* cut, const i32* ctr, const i32* pc, i32 k,
+                        i64* out, hipStream_t s) {
+  k_pass_status<<<1, 256, 0, s>>>(m, cut, ctr, pc, k, out);
+}
 void launch_acc_finish(u64* acc, u64* out, hipStream_t s) {
   k_acc_finish<<<1, ACC_SLOTS, 0, s>>>(acc, out);
 }
