@@ -930,10 +930,9 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         i32* ctr_cur = ctr.p + 2 * (pass & 1);
         i32* ctr_prev = ctr.p + 2 * ((pass + 1) & 1);
         if (run) {
-          HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
-          HIP_TRY(hipMemsetAsync(ctr_cur, 0, 2 * sizeof(i32), st()));
           C.prof.begin(KID_MISC);
-          launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, st());
+          // also clears outc[0..k) and the pass's two selection counters
+          launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, outc.p, ctr_cur, st());
           C.prof.end(KID_MISC, 0);
           C.prof.begin(KID_GAIN);
           bool fused;

@@ -213,9 +213,10 @@ void launch_sanitize_hist(i32* part, i64 n, i32 k, i32* pc, hipStream_t s);
 void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s);
 
 // ---- refinement ---------------------------------------------------------------
+// per-pass balance table bg[2][k] and src_big[k]; also zeroes zero_k[0..k)
+// and zero_2[0..1] when given
 void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
-                     hipStream_t s);
-// Per-pass order of parts by (balance term desc, part asc) and run ends.
+                     i32* zero_k, i32* zero_2, hipStream_t s);
 // Heavy rows (> 64 arcs) split into medium (<= 4096) and huge lists.
 void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* mid,
                         i32* nmid, i32* medium, i32* nmedium, i32* huge, i32* nhuge,

@@ -1700,10 +1700,13 @@ __global__ void k_block_init(i32* part, i64 n, i32 k) {
 // gpu_part.cu:286-295: the balance term depends only on (q, phase, whether
 // the source part is oversized), so it is tabulated per pass with the
 // reference's operation order: b = (c*(T-d))/T; [+A if src big]; [+B if dst small].
-__global__ void k_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big) {
+__global__ void k_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
+                           i32* zero_k, i32* zero_2) {
   const i64 T = n / k;
   const float Tf = (float)T;
+  if (zero_2 && gtid() < 2) zero_2[gtid()] = 0;
   for (i64 q = gtid(); q < k; q += gstride()) {
+    if (zero_k) zero_k[q] = 0;
     const float d = (float)pc[q];
     const float base = phase ? 500.0f * (Tf - d) / Tf : 200.0f * (Tf - d) / Tf;
     const float A = phase ? 250.0f : 100.0f;
@@ -2701,7 +2704,7 @@ __device__ __forceinline__ void bt_tile_sort(float (&e)[16], i32 (&x)[16], float
 // carries no bounds tests.
 template <int TB, bool FULL, bool FIRST>
 __device__ __forceinline__ void bt_tile_body(float* key, i32* idx, i64 len, i64 kk, float* sk,
-                                             i32* si) {
+                                             i32* si, const float* src, i64 M) {
   constexpr int TILE = 1 << TB;
   constexpr int RA = TB - 4;
   const i64 base = (i64)blockIdx.x * TILE;
@@ -2715,8 +2718,14 @@ __device__ __forceinline__ void bt_tile_body(float* key, i32* idx, i64 len, i64 
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int p = bt_pos<RL>(t, r);
-    e[r] = (FULL || p < lim) ? kb[p] : 0.f;
-    x[r] = (FULL || p < lim) ? ib[p] : -1;
+    if constexpr (FIRST) {  // the sort's input: moves [0, M), then (-1e9, -1) pads
+      const i64 gp = base + p;
+      e[r] = (FULL || p < lim) ? (gp < M ? src[gp] : -1e9f) : 0.f;
+      x[r] = (FULL || p < lim) && gp < M ? (i32)gp : -1;
+    } else {
+      e[r] = (FULL || p < lim) ? kb[p] : 0.f;
+      x[r] = (FULL || p < lim) ? ib[p] : -1;
+    }
   }
   if constexpr (FIRST) bt_tile_sort<TB, 1, FULL>(e, x, sk, si, t, lim, base, len);
   else if ((base & kk) == 0) bt_tile_merge<TB, 0, 0, FULL, 1>(e, x, sk, si, t, lim, true);
@@ -2743,11 +2752,13 @@ __device__ __forceinline__ void bt_tile_body(float* key, i32* idx, i64 len, i64 
 
 template <int TB, bool FIRST>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_bt_tile(float* key, i32* idx, i64 len,
-                                                           i64 kk) {
+                                                           i64 kk, const float* src, i64 M) {
   __shared__ float sk[1 << TB];
   __shared__ i32 si[1 << TB];
-  if ((i64)(blockIdx.x + 1) << TB <= len) bt_tile_body<TB, true, FIRST>(key, idx, len, kk, sk, si);
-  else bt_tile_body<TB, false, FIRST>(key, idx, len, kk, sk, si);
+  if ((i64)(blockIdx.x + 1) << TB <= len)
+    bt_tile_body<TB, true, FIRST>(key, idx, len, kk, sk, si, src, M);
+  else
+    bt_tile_body<TB, false, FIRST>(key, idx, len, kk, sk, si, src, M);
 }
 
 // S consecutive strides j, j/2, ..., j >> (S-1) of one merge kk in a single
@@ -3234,8 +3245,8 @@ void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s) {
 }
 
 void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
-                     hipStream_t s) {
-  k_bg_table<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, k, n, phase, bg, src_big);
+                     i32* zero_k, i32* zero_2, hipStream_t s) {
+  k_bg_table<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, k, n, phase, bg, src_big, zero_k, zero_2);
 }
 
 i64 gain_gscratch_elems(i32 k) { return k <= MAXK_LDS ? 0 : (i64)65536 * k; }
@@ -3361,11 +3372,14 @@ void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i6
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* idx,
                        hipStream_t s) {
   constexpr i64 TILE = (i64)1 << BT_TB;
-  k_sort_init<<<grid_for(len, 256, 1u << 16), 256, 0, s>>>(mv_gain, M, len, key, idx);
-  if (len < 2) return;
+  if (len < 2) {
+    k_sort_init<<<1, 64, 0, s>>>(mv_gain, M, len, key, idx);
+    return;
+  }
   const unsigned tiles = (unsigned)((len + TILE - 1) / TILE);
   constexpr int NT = 1 << (BT_TB - 4);
-  k_bt_tile<BT_TB, true><<<tiles, NT, 0, s>>>(key, idx, len, 0);
+  // the first pass reads the moves itself (pads as k_sort_init writes them)
+  k_bt_tile<BT_TB, true><<<tiles, NT, 0, s>>>(key, idx, len, 0, mv_gain, M);
   i64 p2 = 1;
   while (p2 < len) p2 <<= 1;
   for (i64 kk = 2 * TILE; kk <= len; kk <<= 1) {
@@ -3382,7 +3396,7 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
       else k_bt_global<1><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       b -= S;
     }
-    k_bt_tile<BT_TB, false><<<tiles, NT, 0, s>>>(key, idx, len, kk);
+    k_bt_tile<BT_TB, false><<<tiles, NT, 0, s>>>(key, idx, len, kk, nullptr, 0);
   }
 }
 
