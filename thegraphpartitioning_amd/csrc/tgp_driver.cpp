@@ -900,7 +900,9 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       napp = DBuf<i32>(&C.pool, 1);
     }
     DBuf<u8> sel(&C.pool, n), src_big(&C.pool, k);
-    DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
+    DBuf<u8> flag(&C.pool, n);
+    DBuf<i32> bcnt(&C.pool, compaction_blocks()), boff(&C.pool, compaction_blocks());
+    DBuf<i64> avail(&C.pool, k);
     DBuf<float> bg(&C.pool, 2 * (size_t)k);
     DBuf<i32> outc(&C.pool, k), ctr(&C.pool, 4);
     DBuf<i32> H(&C.pool, select_scratch_elems(n, k));
@@ -960,10 +962,9 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
                        TOKEN_LANES, st());
           C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
           C.prof.begin(KID_COMPACT);
-          launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
-          launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
-                               mv_dst.p, mv_gain.p, outc.p, st());
-          C.prof.end(KID_COMPACT, 24.0 * n);
+          launch_compact_moves(flag.p, n, part.p, target.p, gain.p, k, bcnt.p, boff.p, small.p,
+                               mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, st());
+          C.prof.end(KID_COMPACT, 2.0 * n);
           if (!pack)
             HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
         } else if (pending) {
@@ -1087,7 +1088,9 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
       sk_idx(&C.pool, n), sdst(&C.pool, n), outc(&C.pool, k), ctr(&C.pool, 2);
   DBuf<i32> H(&C.pool, select_scratch_elems(n, k));
   DBuf<u8> sel(&C.pool, n);
-  DBuf<i64> flag(&C.pool, n), pos(&C.pool, n), avail(&C.pool, k);
+  DBuf<u8> flag(&C.pool, n);
+  DBuf<i32> bcnt(&C.pool, compaction_blocks()), boff(&C.pool, compaction_blocks());
+  DBuf<i64> avail(&C.pool, k);
   void* s = (void*)st();
   auto ck = [&](int rc) {
     if (rc != 0) {
@@ -1114,9 +1117,8 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
     HIP_TRY(hipMemsetAsync(ctr.p, 0, 2 * sizeof(i32), st()));
     launch_token(g, nedges, gain.p, part.p, target.p, k, flag.p, ax.heavy.p, ax.nheavy,
                  HEAVY_DEG, TOKEN_LANES, st());
-    launch_scan_excl(flag.p, pos.p, n, small.p, scratch(), st());
-    launch_scatter_moves(flag.p, pos.p, part.p, target.p, gain.p, n, k, mv_v.p, mv_src.p,
-                         mv_dst.p, mv_gain.p, outc.p, st());
+    launch_compact_moves(flag.p, n, part.p, target.p, gain.p, k, bcnt.p, boff.p, small.p,
+                         mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, st());
     HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
     C.sync();
     const i64 M = std::min<i64>(C.pinned[0], n);

@@ -283,12 +283,16 @@ i64 gain_gscratch_elems(i32 k);
 // cand[v] = gain of v when v is a move candidate, NaN otherwise (the gain
 // kernels write it); lanes (1, 4 or 8) per row of at most heavy_deg arcs
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
-                  i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
+                  i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
                   int lanes, hipStream_t s);
-void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part,
-                          const i32* target, const float* gain, i64 n, i32 k, i32* mv_v,
-                          i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
+// Winners (flag[v] = 1) to the move arrays in ascending vertex order; *total
+// = their count, outc[src] += moves out of each part.  bcnt and boff hold
+// compaction_blocks() ints each.
+void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* target,
+                          const float* gain, i32 k, i32* bcnt, i32* boff, i64* total,
+                          i32* mv_v, i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
                           hipStream_t s);
+inline i64 compaction_blocks() { return 4096; }
 void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase,
                      i64* avail, hipStream_t s);
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32* sk_idx,

@@ -2386,7 +2386,7 @@ __device__ __forceinline__ bool move_ok(i32 src, i32 dst, i32 k) {
 template <int G, int TU>
 __device__ __forceinline__ void token_light_groups(const DGraph& g, i64 nedges,
                                                    const float* cand, const i32* part,
-                                                   const i32* target, i32 k, i64* flag,
+                                                   const i32* target, i32 k, u8* flag,
                                                    i64 heavy_deg, unsigned bh) {
   const int gl = threadIdx.x & (G - 1);
   const int gbase = lane_id() & ~(G - 1);
@@ -2431,7 +2431,7 @@ __device__ __forceinline__ void token_light_groups(const DGraph& g, i64 nedges,
 template <int G>
 __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float* cand,
                                                const i32* part, const i32* target, i32 k,
-                                               i64* flag, i64 heavy_deg, const i32* list,
+                                               u8* flag, i64 heavy_deg, const i32* list,
                                                i64 cnt, unsigned bh) {
   if (blockIdx.x >= bh && G > 1) {
     token_light_groups<(G > 1 ? G : 2), 4>(g, nedges, cand, part, target, k, flag, heavy_deg,
@@ -2501,28 +2501,105 @@ __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float
   }
 }
 
-// gpu_part.cu:337-350 under SEQ: winners in ascending vertex order.
-__global__ __launch_bounds__(256) void k_scatter_moves(const i64* flag, const i64* pos,
-                                                       const i32* part, const i32* target,
-                                                       const float* gain, i64 n, i32 k,
-                                                       i32* mv_v, i32* mv_src, i32* mv_dst,
-                                                       float* mv_gain, i32* outc) {
-  __shared__ i32 h[MAXK_LDS];
-  for (int q = threadIdx.x; q < k; q += blockDim.x) h[q] = 0;
+// gpu_part.cu:337-350 under SEQ: winners in ascending vertex order.  Block b
+// of the compaction owns vertices [b per, (b + 1) per): k_flag_count counts
+// its winners, k_flag_offsets scans the counts (one block), and
+// k_scatter_moves writes the block's winners in order from its offset.
+constexpr int CMP_BS = 256;
+constexpr int CMP_MAXB = 4096;
+__device__ __forceinline__ i32 block_sum_i32(i32 v, i32* sw) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sw[wv] = v;
   __syncthreads();
-  for (i64 v = gtid(); v < n; v += gstride()) {
-    if (!flag[v]) continue;
-    const i64 p = pos[v];
-    const i32 src = part[v];
-    mv_v[p] = (i32)v;
-    mv_src[p] = src;
-    mv_dst[p] = target[v];
-    mv_gain[p] = gain[v];
-    atomicAdd(&h[src], 1);
+  i32 t = 0;
+  for (int w2 = 0; w2 < CMP_BS / WAVE; w2++) t += sw[w2];
+  __syncthreads();
+  return t;
+}
+__global__ __launch_bounds__(CMP_BS) void k_flag_count(const u8* flag, i64 n, i64 per, i32* bcnt) {
+  __shared__ i32 sw[CMP_BS / WAVE];
+  const i64 b0 = (i64)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  i32 c = 0;
+  for (i64 v = b0 + threadIdx.x; v < b1; v += CMP_BS) c += flag[v];
+  c = block_sum_i32(c, sw);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = c;
+}
+// one block: boff = exclusive scan of bcnt[0..nb), *total = the sum
+__global__ __launch_bounds__(1024) void k_flag_offsets(const i32* bcnt, i32 nb, i32* boff,
+                                                       i64* total) {
+  __shared__ i32 sw[1024 / WAVE];
+  constexpr int PER = CMP_MAXB / 1024;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  i32 v[PER], s = 0;
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int i = threadIdx.x * PER + r;
+    v[r] = i < nb ? bcnt[i] : 0;
+    s += v[r];
   }
+  i32 x = s;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const i32 y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == WAVE - 1) sw[wv] = x;
   __syncthreads();
-  for (int q = threadIdx.x; q < k; q += blockDim.x)
-    if (h[q]) atomicAdd(&outc[q], h[q]);
+  i32 before = 0, all = 0;
+  for (int w2 = 0; w2 < 1024 / WAVE; w2++) {
+    if (w2 < wv) before += sw[w2];
+    all += sw[w2];
+  }
+  i32 run = before + x - s;
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int i = threadIdx.x * PER + r;
+    if (i < nb) boff[i] = run;
+    run += v[r];
+  }
+  if (threadIdx.x == 0) *total = all;
+}
+__global__ __launch_bounds__(CMP_BS) void k_scatter_moves(const u8* flag, i64 n, i64 per,
+                                                         const i32* boff, const i32* part,
+                                                         const i32* target, const float* gain,
+                                                         i32 k, i32* mv_v, i32* mv_src,
+                                                         i32* mv_dst, float* mv_gain,
+                                                         i32* outc) {
+  __shared__ i32 h[MAXK_LDS];
+  __shared__ i32 sw[CMP_BS / WAVE];
+  for (int q = threadIdx.x; q < k && q < MAXK_LDS; q += CMP_BS) h[q] = 0;
+  __syncthreads();
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const i64 b0 = (i64)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  i64 run = boff[blockIdx.x];
+  for (i64 base = b0; base < b1; base += CMP_BS) {
+    const i64 v = base + threadIdx.x;
+    const bool f = v < b1 && flag[v];
+    const u64 bm = __ballot(f);
+    if (lane == 0) sw[wv] = __popcll(bm);
+    __syncthreads();
+    i32 before = 0, tot = 0;
+    for (int w2 = 0; w2 < CMP_BS / WAVE; w2++) {
+      if (w2 < wv) before += sw[w2];
+      tot += sw[w2];
+    }
+    if (f) {
+      const i64 p = run + before + __popcll(bm & lanes_below(lane));
+      const i32 src = part[v];
+      mv_v[p] = (i32)v;
+      mv_src[p] = src;
+      mv_dst[p] = target[v];
+      mv_gain[p] = gain[v];
+      if (k <= MAXK_LDS) atomicAdd(&h[src], 1);
+      else atomicAdd(&outc[src], 1);
+    }
+    run += tot;
+    __syncthreads();
+  }
+  if (k <= MAXK_LDS)
+    for (int q = threadIdx.x; q < k; q += CMP_BS)
+      if (h[q]) atomicAdd(&outc[q], h[q]);
 }
 
 // gpu_part.cu:357-369.
@@ -2819,20 +2896,18 @@ __global__ __launch_bounds__(256) void k_bt_global(float* key, i32* idx, i64 len
 // avail[dst].  Padding that reached the prefix is not selected.
 constexpr int SEL_BLK = 1024;
 
-__global__ void k_sel_dst(const i32* sk_idx, const i32* mv_dst, i64 M, i32* sdst) {
-  for (i64 i = gtid(); i < M; i += gstride()) {
-    const i32 id = sk_idx[i];
-    sdst[i] = id >= 0 ? mv_dst[id] : -1;
-  }
-}
-
-__global__ __launch_bounds__(SEL_BLK) void k_sel_hist(const i32* sdst, i64 M, i32 k, i32* H) {
+// sdst[i] = destination of the i-th move in sorted order (-1 for a pad), and
+// per-block destination counts H[k][nb].
+__global__ __launch_bounds__(SEL_BLK) void k_sel_hist(const i32* sk_idx, const i32* mv_dst, i64 M,
+                                                      i32 k, i32* H, i32* sdst) {
   __shared__ i32 h[MAXK_LDS];
   for (int q = threadIdx.x; q < k; q += SEL_BLK) h[q] = 0;
   __syncthreads();
   const i64 i = (i64)blockIdx.x * SEL_BLK + threadIdx.x;
   if (i < M) {
-    const i32 d = sdst[i];
+    const i32 id = sk_idx[i];
+    const i32 d = id >= 0 ? mv_dst[id] : -1;
+    sdst[i] = d;
     if (d >= 0 && d < k) atomicAdd(&h[d], 1);
   }
   __syncthreads();
@@ -3340,7 +3415,7 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, 
 }
 
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
-                  i32 k, i64* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
+                  i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
                   int lanes, hipStream_t s) {
   const int G = (lanes == 4 || lanes == 8) ? lanes : 1;
   const unsigned bl = grid_for(g.n, 256 / G, 1u << 16);
@@ -3356,12 +3431,21 @@ void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, cons
                                        heavy_list, heavy_count, bh);
 }
 
-void launch_scatter_moves(const i64* flag, const i64* pos, const i32* part, const i32* target,
-                          const float* gain, i64 n, i32 k, i32* mv_v, i32* mv_src, i32* mv_dst,
-                          float* mv_gain, i32* outc, hipStream_t s) {
-  k_scatter_moves<<<grid_for(n, 256 * 8, 4096), 256, 0, s>>>(flag, pos, part, target, gain, n,
-                                                             k, mv_v, mv_src, mv_dst, mv_gain,
-                                                             outc);
+void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* target,
+                          const float* gain, i32 k, i32* bcnt, i32* boff, i64* total,
+                          i32* mv_v, i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
+                          hipStream_t s) {
+  i64 nb = (n + 4095) / 4096;
+  if (nb < 1) nb = 1;
+  if (nb > CMP_MAXB) nb = CMP_MAXB;
+  i64 per = (n + nb - 1) / nb;
+  per = (per + CMP_BS - 1) / CMP_BS * CMP_BS;
+  nb = (n + per - 1) / per;
+  if (nb < 1) nb = 1;
+  k_flag_count<<<(unsigned)nb, CMP_BS, 0, s>>>(flag, n, per, bcnt);
+  k_flag_offsets<<<1, 1024, 0, s>>>(bcnt, (i32)nb, boff, total);
+  k_scatter_moves<<<(unsigned)nb, CMP_BS, 0, s>>>(flag, n, per, boff, part, target, gain, k,
+                                                   mv_v, mv_src, mv_dst, mv_gain, outc);
 }
 
 void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i64* avail,
@@ -3405,8 +3489,7 @@ i64 select_scratch_elems(i64 M, i32 k) { return ((M + SEL_BLK - 1) / SEL_BLK + 1
 void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64* avail,
                    i32* sdst, i32* H, u8* sel, i32* S, i32* pads, hipStream_t s) {
   const i64 nb = (M + SEL_BLK - 1) / SEL_BLK;
-  k_sel_dst<<<grid_for(M, 256, 1u << 16), 256, 0, s>>>(sk_idx, mv_dst, M, sdst);
-  k_sel_hist<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H);
+  k_sel_hist<<<(unsigned)nb, SEL_BLK, 0, s>>>(sk_idx, mv_dst, M, k, H, sdst);
   k_sel_scan<<<(unsigned)k, SELSCAN_BS, 0, s>>>(H, nb);
   k_sel_mark<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H, avail, sel, S, pads);
 }
