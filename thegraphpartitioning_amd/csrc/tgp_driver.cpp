@@ -1024,7 +1024,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         passes++;
         if (M <= 0) break;                                 // gpu_part.cu:638-640, 742-744
         C.prof.begin(KID_MISC);
-        launch_capacity(pc.p, outc.p, k, n, phase, avail.p, st());
+        launch_capacity(pc.p, outc.p, k, n, phase, avail.p, tmode ? napp.p : nullptr, st());
         C.prof.end(KID_MISC, 0);
         i64 len = M;                                       // gpu_part.cu:487-497
         i64 padded = 1;
@@ -1123,7 +1123,7 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
     C.sync();
     const i64 M = std::min<i64>(C.pinned[0], n);
     if (M < min_moves) break;
-    launch_capacity(pc.p, outc.p, k, n, 0, avail.p, st());
+    launch_capacity(pc.p, outc.p, k, n, 0, avail.p, nullptr, st());
     i64 len = M, padded = 1;
     while (padded < len) padded <<= 1;
     if (padded > n) padded = n;

@@ -293,15 +293,17 @@ void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* tar
                           i32* mv_v, i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
                           hipStream_t s);
 inline i64 compaction_blocks() { return 4096; }
-void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase,
-                     i64* avail, hipStream_t s);
+// avail[q] per part (gpu_part.cu:358-369); also zeroes *zero1 when given (the
+// applied-move count that launch_commit appends to)
+void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i64* avail,
+                     i32* zero1, hipStream_t s);
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32* sk_idx,
                        hipStream_t s);
 i64 select_scratch_elems(i64 M, i32 k);
 void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64* avail,
                    i32* sdst, i32* H, u8* sel, i32* S, i32* pads, hipStream_t s);
 // part8 (optional) is kept equal to part's byte copy; app (optional) receives
-// the ids of the applied moves, *napp their number
+// the ids of the applied moves, *napp their number (*napp must start at 0)
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
                    const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, i32* app,
                    i32* napp, hipStream_t s);

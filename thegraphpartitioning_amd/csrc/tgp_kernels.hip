@@ -2604,7 +2604,8 @@ __global__ __launch_bounds__(CMP_BS) void k_scatter_moves(const u8* flag, i64 n,
 
 // gpu_part.cu:357-369.
 __global__ void k_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase,
-                           i64* avail) {
+                           i64* avail, i32* zero1) {
+  if (zero1 && gtid() == 0) *zero1 = 0;
   const i64 T = n / k;
   const float Tf = (float)T;
   const i64 slack0 = n / ((i64)k * 10);
@@ -3449,8 +3450,8 @@ void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* tar
 }
 
 void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i64* avail,
-                     hipStream_t s) {
-  k_capacity<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, outc, k, n, phase, avail);
+                     i32* zero1, hipStream_t s) {
+  k_capacity<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, outc, k, n, phase, avail, zero1);
 }
 
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* idx,
@@ -3497,7 +3498,6 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
                    const i32* mv_dst, i32* part, u8* part8, i32* pc, i32 k, i32* app,
                    i32* napp, hipStream_t s) {
-  if (app) k_fill<i32><<<1, 1, 0, s>>>(napp, 1, 0);
   if (k <= MAXK_LDS)
     k_commit<true><<<grid_for(M, 256 * 16, 2048), 256, 0, s>>>(sk_idx, sel, M, mv_v, mv_src,
                                                                mv_dst, part, part8, pc, k, app,
