@@ -357,7 +357,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(every GPU box here has one GPU)", file=sys.stderr)
+              "(the multi-GPU design is described, not built)", file=sys.stderr)
         sys.exit(2)
     if args.batched:
         bench_batched(args)

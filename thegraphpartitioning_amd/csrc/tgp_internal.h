@@ -80,7 +80,7 @@ void launch_match_round(DGraph g, i32* mate, u8* dec, i32* pend, u64* R, i32* pr
 void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop, i64* lo,
                            hipStream_t s);
 // req/nreq: queue of deferred long-row scans (W pairs) and its two counters,
-// both zeroed by the caller before round 1.
+// both zeroed by the caller before the first round.
 // Chained matching (symmetric levels) on the state st (-1 undecided, -2
 // decided unmatched, else the partner): one launch decides vertices from
 // lo[0] upward until all are decided or `budget` wall-clock ticks pass, then
