@@ -411,7 +411,9 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const auto t_start = std::chrono::steady_clock::now();
   DBuf<u8> dec(&C.pool, n);
   DBuf<i32> s1(&C.pool, n), prop(&C.pool, n);
-  DBuf<i64> lo(&C.pool, 3);  // [0] window low end, [1..2] scan starts by round parity
+  // [0] window low end, [1..2] scan starts by round parity, [3] first-undecided
+  // accumulator (n between uses)
+  DBuf<i64> lo(&C.pool, 4);
   DBuf<u64> R;
   DBuf<i32> s2;
   if (lv.symmetric) s2 = DBuf<i32>(&C.pool, n);

@@ -91,7 +91,8 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
                         u32* ticket, i64 budget, int blocks, int sg, int per, u64* stats,
                         hipStream_t s);
-// lo[0] = first vertex at or above lo[0] with st == -1 (n if none)
+// lo[0] = first vertex at or above lo[0] with st == -1 (n if none); lo[3]
+// must hold n on entry and holds n again on exit
 void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s);
 // mate = st where st >= 0 else -1; dec = (st != -1)
 void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s);

@@ -1065,23 +1065,27 @@ __global__ void k_chain_export(const i32* st, i64 n, i32* mate, u8* dec) {
 }
 
 // lo_io[0] = first vertex at or above lo_io[0] with st == -1 (one block)
-__global__ __launch_bounds__(1024) void k_first_undecided_st(const i32* st, i64 n, i64* lo_io) {
+// First undecided vertex at or above lo[0]: blocks scan disjoint strips and
+// fold their first hit into lo[3] (which holds n on entry) with atomicMin;
+// k_first_undecided_take moves the result to lo[0] and resets lo[3] to n.
+__global__ __launch_bounds__(256) void k_first_undecided_st(const i32* st, i64 n, i64* lo) {
   __shared__ i64 s_found;
-  i64 res = n;
-  for (i64 base0 = lo_io[0]; base0 < n; base0 += blockDim.x) {
-    if (threadIdx.x == 0) s_found = LLONG_MAX;
-    __syncthreads();
-    const i64 i = base0 + threadIdx.x;
-    if (i < n && st[i] == ST_UNDEC) atomicMin((unsigned long long*)&s_found, (unsigned long long)i);
-    __syncthreads();
-    const i64 f = s_found;
-    __syncthreads();
-    if (f != LLONG_MAX) {
-      res = f;
-      break;
+  if (threadIdx.x == 0) s_found = LLONG_MAX;
+  __syncthreads();
+  const i64 start = lo[0];
+  for (i64 i = start + gtid(); i < n; i += gstride()) {
+    if (st[i] == ST_UNDEC) {
+      atomicMin((unsigned long long*)&s_found, (unsigned long long)i);
+      break;  // later hits of this thread are larger
     }
   }
-  if (threadIdx.x == 0) lo_io[0] = res;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_found != LLONG_MAX)
+    atomicMin((unsigned long long*)&lo[3], (unsigned long long)s_found);
+}
+__global__ void k_first_undecided_take(i64* lo, i64 n) {
+  lo[0] = lo[3];
+  lo[3] = n;
 }
 
 // Symmetry test of a level: every arc s->x (s != x, x < n) must have x->s.
@@ -3189,6 +3193,7 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
                            hipStream_t s) {
   k_match_sym_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, blkA, blkC, prop);
   k_fill<i64><<<1, 3, 0, s>>>(lo, 3, 0);
+  k_fill<i64><<<1, 1, 0, s>>>(lo + 3, 1, n);  // first-undecided accumulator
 }
 
 void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
@@ -3209,7 +3214,8 @@ void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, cons
 }
 
 void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s) {
-  k_first_undecided_st<<<1, 1024, 0, s>>>(st, n, lo);
+  k_first_undecided_st<<<grid_for(n, 256 * 16, 4096), 256, 0, s>>>(st, n, lo);
+  k_first_undecided_take<<<1, 1, 0, s>>>(lo, n);
 }
 
 void launch_chain_export(const i32* st, i64 n, i32* mate, u8* dec, hipStream_t s) {
