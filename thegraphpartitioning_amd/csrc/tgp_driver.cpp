@@ -689,9 +689,12 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   a.heavy.ensure(&C.pool, std::max<i64>(lv.n, 1));
   a.asrc.ensure(&C.pool, std::max<i64>(lv.m, 1));
   DBuf<i32> cnt32(&C.pool, 2);
+  DBuf<u64> maxabs(&C.pool, 1);
   HIP_TRY(hipMemsetAsync(cnt32.p, 0, 2 * sizeof(i32), st()));
+  HIP_TRY(hipMemsetAsync(maxabs.p, 0, sizeof(u64), st()));
   C.prof.begin(KID_MISC);
-  launch_level_census(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, cnt32.p, cnt32.p + 1, st());
+  launch_level_census(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, cnt32.p, cnt32.p + 1, maxabs.p,
+                      st());
   launch_arc_source(lv.dg(), a.asrc.p, st());
   C.prof.end(KID_MISC, 12.0 * lv.m);
   HIP_TRY(hipMemcpyAsync(C.pinned, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
@@ -717,7 +720,7 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   a.all_exact = 1;
   if (!a.unit) {
     HIP_TRY(hipMemsetAsync(cnt3.p, 0, sizeof(i32), st()));
-    launch_row_abs(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, a.nheavy, cnt3.p, st());
+    launch_row_abs(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, a.nheavy, maxabs.p, cnt3.p, st());
     HIP_TRY(hipMemcpyAsync(C.pinned + 3, cnt3.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
     C.sync();
     a.all_exact = ((const i32*)(C.pinned + 3))[0] == 0;

@@ -66,9 +66,10 @@ void launch_fill_u64(u64* p, i64 n, u64 v, hipStream_t s);
 void launch_scan_excl(const i64* in, i64* out, i64 n, i64* total, ScanScratch sc,
                       hipStream_t s);
 // Row-degree census of a level: unit-weight flag, heavy list (range length >
-// heavy_deg, ranges end at `nedges` for the last vertex).
+// heavy_deg, ranges end at `nedges` for the last vertex), and *maxabs =
+// max |w| over the level's arcs (zeroed by the caller).
 void launch_level_census(DGraph g, i64 nedges, i64 heavy_deg, i32* heavy_list,
-                         i32* heavy_count, i32* nonunit_flag, hipStream_t s);
+                         i32* heavy_count, i32* nonunit_flag, u64* maxabs, hipStream_t s);
 
 // ---- coarsening ---------------------------------------------------------------
 // Exact matching in rounds over the id window [lo, lo + W) (lo = smallest
@@ -249,9 +250,10 @@ struct GainLists {
   int all_exact = 0;  // every row's sum of |w| is at most 2^24 (integer tallies are exact)
   u32* ctr = nullptr;  // 8 work counters (256 u32) of the fused gain kernel, reset by it
 };
-// *over |= (some row's sum of |w| > 2^24); heavy = rows longer than heavy_deg
+// *over |= (some row's sum of |w| > 2^24); heavy = rows longer than heavy_deg;
+// rows with degree * *maxabs <= 2^24 (the census's max |w|) are not summed
 void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
-                    i32* over, hipStream_t s);
+                    const u64* maxabs, i32* over, hipStream_t s);
 // part8 (optional, k <= PART8_MAXK): byte copy of part that the neighbour
 // gathers read instead of part.
 constexpr i32 PART8_MAXK = 255;

@@ -171,15 +171,23 @@ __global__ void k_copy1(const i64* src, i64* dst) { *dst = *src; }
 
 __global__ __launch_bounds__(256) void k_level_census(DGraph g, i64 nedges, i64 heavy_deg,
                                                       i32* heavy_list, i32* heavy_count,
-                                                      i32* nonunit) {
+                                                      i32* nonunit, u64* maxabs) {
   for (i64 v = gtid(); v < g.n; v += gstride()) {
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
     wave_append(s1 - s0 > heavy_deg, (i32)v, heavy_list, heavy_count);
   }
   bool nu = false;
-  for (i64 i = gtid(); i < g.m; i += gstride()) nu |= (g.w[i] != 1);
+  u64 mx = 0;
+  for (i64 i = gtid(); i < g.m; i += gstride()) {
+    const i32 w = g.w[i];
+    nu |= w != 1;
+    const u64 a = w < 0 ? (u64)(-(i64)w) : (u64)w;
+    mx = a > mx ? a : mx;
+  }
   if (nu) atomicOr(nonunit, 1);
+  mx = wave_max_u64(mx);
+  if (lane_id() == 0 && mx) atomicMax((unsigned long long*)maxabs, (unsigned long long)mx);
 }
 
 // ----------------------------------------------------------------- matching --
@@ -2335,23 +2343,33 @@ __global__ __launch_bounds__(256) void k_tally_last(DGraph g, i64 nedges, const 
 // *over = 1 when some row's sum of |w| (over [s0, s1), the last row ending at
 // nedges) exceeds 2^24; rows longer than heavy_deg are summed by one wave
 // each from the heavy list.
-__global__ void k_row_abs_light(DGraph g, i64 nedges, i64 heavy_deg, i32* over) {
+// A row whose degree times the level's largest |w| is at most 2^24 (and that
+// lies within the level's arcs, which that maximum covers) cannot pass 2^24
+// and is not summed.
+__device__ __forceinline__ bool row_abs_bounded(const DGraph& g, i64 s1, i64 d, u64 maxabs) {
+  return s1 <= g.m && (u64)d * maxabs <= (u64)(1 << 24);
+}
+__global__ void k_row_abs_light(DGraph g, i64 nedges, i64 heavy_deg, const u64* maxabs_p,
+                                i32* over) {
+  const u64 maxabs = *maxabs_p;
   for (i64 v = gtid(); v < g.n; v += gstride()) {
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
-    if (s1 - s0 > heavy_deg) continue;
+    if (s1 - s0 > heavy_deg || row_abs_bounded(g, s1, s1 - s0, maxabs)) continue;
     i64 a = 0;
     for (i64 i = s0; i < s1; i++) a += g.w[i] < 0 ? -(i64)g.w[i] : (i64)g.w[i];
     if (a > (i64)(1 << 24)) atomicOr(over, 1);
   }
 }
 __global__ __launch_bounds__(256) void k_row_abs_heavy(DGraph g, i64 nedges, const i32* list,
-                                                       i64 cnt, i32* over) {
+                                                       i64 cnt, const u64* maxabs_p, i32* over) {
   const int lane = lane_id();
+  const u64 maxabs = *maxabs_p;
   for (i64 e = gwave(); e < cnt; e += nwaves()) {
     const i64 v = list[e];
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
+    if (row_abs_bounded(g, s1, s1 - s0, maxabs)) continue;
     i64 a = 0;
     for (i64 i = s0 + lane; i < s1; i += WAVE) a += g.w[i] < 0 ? -(i64)g.w[i] : (i64)g.w[i];
     a = wave_sum(a);
@@ -3170,9 +3188,9 @@ void launch_scan_excl(const i64* in, i64* out, i64 n, i64* total, ScanScratch sc
 }
 
 void launch_level_census(DGraph g, i64 nedges, i64 heavy_deg, i32* heavy_list,
-                         i32* heavy_count, i32* nonunit, hipStream_t s) {
+                         i32* heavy_count, i32* nonunit, u64* maxabs, hipStream_t s) {
   k_level_census<<<grid_for(g.n > g.m ? g.n : g.m, 256, 8192), 256, 0, s>>>(
-      g, nedges, heavy_deg, heavy_list, heavy_count, nonunit);
+      g, nedges, heavy_deg, heavy_list, heavy_count, nonunit, maxabs);
 }
 
 void launch_match_init(i64 n, i32* mate, u8* dec, i32* pend, u64* R, i64* lo,
@@ -3348,10 +3366,12 @@ void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* h
 }
 
 void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
-                    i32* over, hipStream_t s) {
-  k_row_abs_light<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, heavy_deg, over);
+                    const u64* maxabs, i32* over, hipStream_t s) {
+  k_row_abs_light<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, heavy_deg, maxabs,
+                                                                over);
   if (nheavy > 0)
-    k_row_abs_heavy<<<grid_for(nheavy, 4, 8192), 256, 0, s>>>(g, nedges, heavy, nheavy, over);
+    k_row_abs_heavy<<<grid_for(nheavy, 4, 8192), 256, 0, s>>>(g, nedges, heavy, nheavy, maxabs,
+                                                               over);
 }
 
 template <typename PT>
