@@ -217,6 +217,16 @@ void tgp_release_device_memory(void);
  * device_input = 1 on a graph already resident in HBM); no counterpart in the
  * reference, whose main() uploads inline (gpu_part.cu:950-955).             */
 int tgp_graph_upload(const tgp_graph* host, tgp_graph* dev);
+
+/* The move sort of one refinement pass, on device buffers: key/idx[0..len)
+ * receive the reference's bitonic network (bitonicSortStep +
+ * customSortMoves, gpu_part.cu:472-506) applied to (gain[i], i) for i < M
+ * and (-1e9, -1) pads for M <= i < len (len <= 2^31), enqueued on `stream`
+ * (a hipStream_t; NULL = the default stream).  The same kernels the
+ * partitioner runs; exported so that the network can be checked on its own
+ * at lengths the partition tests do not reach.                              */
+int tgp_sort_moves(const float* gain, int64_t M, int64_t len, float* key, int32_t* idx,
+                   void* stream);
 void tgp_graph_device_free(tgp_graph* dev);
 
 /* ---- files of the reference contract ------------------------------------- */

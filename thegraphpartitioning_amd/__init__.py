@@ -142,6 +142,7 @@ def lib():
     L.tgp_initpart_gpmetis.argtypes = L.tgp_initpart_block.argtypes
     L.tgp_initpart_gpmetis_dir.argtypes = L.tgp_initpart_block.argtypes
     L.tgp_release_device_memory.restype = None
+    L.tgp_sort_moves.argtypes = [P, C.c_int64, C.c_int64, P, P, P]
     L.tgp_graph_upload.argtypes = [C.POINTER(CGraph), C.POINTER(CGraph)]
     L.tgp_graph_upload.restype = C.c_int
     L.tgp_graph_device_free.argtypes = [C.POINTER(CGraph)]

@@ -1581,6 +1581,14 @@ const char* tgp_kernel_name(int id) {
   return (id >= 0 && id < KID_COUNT) ? names[id] : "";
 }
 
+int tgp_sort_moves(const float* gain, int64_t M, int64_t len, float* key, int32_t* idx,
+                   void* stream) {
+  if (!gain || !key || !idx || M < 0 || len < M || len > ((int64_t)1 << 31)) return TGP_EINVAL;
+  if (len == 0) return TGP_OK;
+  tgp::launch_sort_moves(gain, M, len, key, idx, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? TGP_OK : TGP_EHIP;
+}
+
 int tgp_graph_upload(const tgp_graph* host, tgp_graph* dev) {
   if (!host || !dev || host->n <= 0 || host->m <= 0) return TGP_EINVAL;
   const int rc = validate_host_graph(host);
