@@ -84,6 +84,13 @@ class DevicePool {
   void put(void* p, size_t bytes) {
     if (p) free_[round_size(bytes)].push_back(p);
   }
+  // bytes held in released blocks (hipMemGetInfo counts them as used; get()
+  // returns them to the runtime when an allocation would otherwise fail)
+  size_t cached_bytes() const {
+    size_t t = 0;
+    for (const auto& kv : free_) t += kv.first * kv.second.size();
+    return t;
+  }
   void trim() {
     for (auto& kv : free_) {
       for (void* p : kv.second) (void)hipFree(p);
@@ -208,6 +215,8 @@ struct LevelAux {
   DBuf<i32> gtally;          // nhuge * k tallies and nhuge |w| sums, zero between passes
   DBuf<i64> gabs;
   int all_exact = 0;         // every row's sum of |w| is at most 2^24
+  DBuf<i32> over;            // [0] = count, then the rows whose sum of |w| passes 2^24
+  i64 nover = 0;
   DBuf<u32> ctr;             // work-item counter of the fused gain kernel
   GainLists lists() const {
     GainLists L;
@@ -382,6 +391,9 @@ class Run {
 };
 
 static const i64 HEAVY_DEG = 64;
+// most rows with a sum of |w| over 2^24 that a level may have and still keep
+// part tallies (their gains are gathered again on every pass)
+static const i64 OVER_ROWS_MAX = 65536;
 
 static i64 env_i64(const char* name, i64 dflt) {
   const char* v = std::getenv(name);
@@ -718,12 +730,15 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   a.ctr.ensure(&C.pool, 8 * 32);  // GAIN_CTRS counters, 128 bytes apart
   // integer tallies are exact for every row when no row's sum of |w| passes 2^24
   a.all_exact = 1;
+  a.nover = 0;
   if (!a.unit) {
-    HIP_TRY(hipMemsetAsync(cnt3.p, 0, sizeof(i32), st()));
-    launch_row_abs(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, a.nheavy, maxabs.p, cnt3.p, st());
-    HIP_TRY(hipMemcpyAsync(C.pinned + 3, cnt3.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+    a.over.ensure(&C.pool, lv.n + 1);
+    HIP_TRY(hipMemsetAsync(a.over.p, 0, sizeof(i32), st()));
+    launch_row_abs(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, a.nheavy, maxabs.p, a.over.p, st());
+    HIP_TRY(hipMemcpyAsync(C.pinned + 3, a.over.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
     C.sync();
-    a.all_exact = ((const i32*)(C.pinned + 3))[0] == 0;
+    a.nover = ((const i32*)(C.pinned + 3))[0];
+    a.all_exact = a.nover == 0;
   }
   // chunks of the long rows, built on the host (nhuge <= m / 4096)
   a.nch = 0;
@@ -892,7 +907,11 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       (void)hipGetLastError();
       free_b = 0;
     }
-    const bool tmode = k <= 512 && ax.all_exact && lv.simple_ws &&
+    free_b += C.pool.cached_bytes();
+    // A few rows whose sum of |w| passes 2^24 keep the kept-tally form: their
+    // gains are gathered again in fp32 row order after every tally pass.
+    const bool tmode = k <= 512 && (ax.all_exact || ax.nover <= OVER_ROWS_MAX) &&
+                       lv.simple_ws &&
                        (double)n * tally_row_stride(k) * 4 <= 0.4 * (double)free_b &&
                        env_i64("TGP_KEEP_TALLIES", 1) != 0;
     DBuf<i32> T, app, napp;
@@ -947,6 +966,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
             if (last_partial) launch_tally_last(g, nedges, part.p, T.p, k, unit, st());
             launch_gain_tallies(n, part.p, T.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
                                 phase, pending ? accs.p : nullptr, GAIN_LANES, st());
+            launch_gain_rows(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
+                             phase, unit, ax.over.p + 1, ax.nover, st());
             fused = true;
             C.prof.end(KID_GAIN, 4.0 * k * n + 17.0 * n);
           } else {

@@ -250,8 +250,9 @@ struct GainLists {
   int all_exact = 0;  // every row's sum of |w| is at most 2^24 (integer tallies are exact)
   u32* ctr = nullptr;  // 8 work counters (256 u32) of the fused gain kernel, reset by it
 };
-// *over |= (some row's sum of |w| > 2^24); heavy = rows longer than heavy_deg;
-// rows with degree * *maxabs <= 2^24 (the census's max |w|) are not summed
+// over[0] += rows whose sum of |w| passes 2^24, listed at over + 1 (room for
+// n); heavy = rows longer than heavy_deg; rows with degree * *maxabs <= 2^24
+// (the census's max |w|) are not summed
 void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 nheavy,
                     const u64* maxabs, i32* over, hipStream_t s);
 // part8 (optional, k <= PART8_MAXK): byte copy of part that the neighbour
@@ -264,6 +265,11 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, 
 // Kept tallies: row v of T holds k ints at v * tally_stride(k) (k rounded up
 // to 4, so rows are 16-byte aligned); Tout of launch_gain uses the same layout.
 inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
+// Gains of the listed rows only, gathered from their arcs (k <= 512): rows
+// whose sum of |w| passes 2^24 are tallied in fp32 in row order.
+void launch_gain_rows(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+                      const u8* src_big, float* gain, i32* target, float* cand, int phase,
+                      int unit, const i32* list, i64 cnt, hipStream_t s);
 // Gains (and the cut into cut_acc slots) from kept tallies T; lanes (2, 4, 8 or
 // 16) read each row.
 void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,

@@ -2060,7 +2060,7 @@ __device__ __forceinline__ void gain_vertex(const DGraph& g, i64 nedges, const P
     const i32 tq = tally[q];
     const float cq = exact_int ? (float)tq : ftally[q];
     tally[q] = 0;
-    if (Tout) Tout[v * tally_stride(k) + q] = q == cur ? own : tq;  // (exact_int holds)
+    if (Tout) Tout[v * tally_stride(k) + q] = q == cur ? own : tq;  // integers: exact
     if (q == cur) continue;
     const float eg = cq - ccur;
     const float gg = eg + bgr[q];
@@ -2349,6 +2349,8 @@ __global__ __launch_bounds__(256) void k_tally_last(DGraph g, i64 nedges, const 
 __device__ __forceinline__ bool row_abs_bounded(const DGraph& g, i64 s1, i64 d, u64 maxabs) {
   return s1 <= g.m && (u64)d * maxabs <= (u64)(1 << 24);
 }
+// over[0] counts the rows whose sum of |w| passes 2^24 and lists them in
+// over + 1 (in no particular order).
 __global__ void k_row_abs_light(DGraph g, i64 nedges, i64 heavy_deg, const u64* maxabs_p,
                                 i32* over) {
   const u64 maxabs = *maxabs_p;
@@ -2358,7 +2360,7 @@ __global__ void k_row_abs_light(DGraph g, i64 nedges, i64 heavy_deg, const u64* 
     if (s1 - s0 > heavy_deg || row_abs_bounded(g, s1, s1 - s0, maxabs)) continue;
     i64 a = 0;
     for (i64 i = s0; i < s1; i++) a += g.w[i] < 0 ? -(i64)g.w[i] : (i64)g.w[i];
-    if (a > (i64)(1 << 24)) atomicOr(over, 1);
+    if (a > (i64)(1 << 24)) over[1 + atomicAdd(&over[0], 1)] = (i32)v;
   }
 }
 __global__ __launch_bounds__(256) void k_row_abs_heavy(DGraph g, i64 nedges, const i32* list,
@@ -2373,7 +2375,7 @@ __global__ __launch_bounds__(256) void k_row_abs_heavy(DGraph g, i64 nedges, con
     i64 a = 0;
     for (i64 i = s0 + lane; i < s1; i += WAVE) a += g.w[i] < 0 ? -(i64)g.w[i] : (i64)g.w[i];
     a = wave_sum(a);
-    if (lane == 0 && a > (i64)(1 << 24)) atomicOr(over, 1);
+    if (lane == 0 && a > (i64)(1 << 24)) over[1 + atomicAdd(&over[0], 1)] = (i32)v;
   }
 }
 
@@ -3440,6 +3442,20 @@ bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, 
                                                nullptr);
   }
   return false;  // the large-k kernels do not tally the cut
+}
+
+void launch_gain_rows(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
+                      const u8* src_big, float* gain, i32* target, float* cand, int phase,
+                      int unit, const i32* list, i64 cnt, hipStream_t s) {
+  if (cnt <= 0) return;
+  const size_t shm = (size_t)k * 12;
+  const unsigned gr = grid_for(cnt, 1, 65536);
+  if (unit)
+    k_gain<true, true><<<gr, WAVE, shm, s>>>(g, nedges, part, k, bg, src_big, gain, target, cand,
+                                             phase, nullptr, list, cnt, nullptr, nullptr);
+  else
+    k_gain<false, true><<<gr, WAVE, shm, s>>>(g, nedges, part, k, bg, src_big, gain, target,
+                                              cand, phase, nullptr, list, cnt, nullptr, nullptr);
 }
 
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
