@@ -92,6 +92,8 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
                         u32* ticket, i64 budget, int blocks, int sg, int per, u64* stats,
                         hipStream_t s);
+// *out = the level's longest row (xadj[v+1] - xadj[v]); *out zeroed by the caller
+void launch_max_row(DGraph g, u64* out, hipStream_t s);
 // lo[0] = first vertex at or above lo[0] with st == -1 (n if none); lo[3]
 // must hold n on entry and holds n again on exit
 void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s);

@@ -1073,6 +1073,17 @@ __global__ void k_chain_export(const i32* st, i64 n, i32* mate, u8* dec) {
 }
 
 // lo_io[0] = first vertex at or above lo_io[0] with st == -1 (one block)
+// *out = max over v of the row length xadj[v+1] - xadj[v] (zeroed by the caller)
+__global__ __launch_bounds__(256) void k_max_row(DGraph g, u64* out) {
+  u64 mx = 0;
+  for (i64 v = gtid(); v < g.n; v += gstride()) {
+    const u64 d = (u64)(g.xadj[v + 1] - g.xadj[v]);
+    mx = d > mx ? d : mx;
+  }
+  mx = wave_max_u64(mx);
+  if (lane_id() == 0 && mx) atomicMax((unsigned long long*)out, (unsigned long long)mx);
+}
+
 // First undecided vertex at or above lo[0]: blocks scan disjoint strips and
 // fold their first hit into lo[3] (which holds n on entry) with atomicMin;
 // k_first_undecided_take moves the result to lo[0] and resets lo[3] to n.
@@ -3234,6 +3245,9 @@ void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, cons
   k_match_chain<8, 8><<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
 }
 
+void launch_max_row(DGraph g, u64* out, hipStream_t s) {
+  k_max_row<<<grid_for(g.n, 256, 4096), 256, 0, s>>>(g, out);
+}
 void launch_first_undecided(const i32* st, i64 n, i64* lo, hipStream_t s) {
   k_first_undecided_st<<<grid_for(n, 256 * 16, 4096), 256, 0, s>>>(st, n, lo);
   k_first_undecided_take<<<1, 1, 0, s>>>(lo, n);
