@@ -415,7 +415,7 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   // row entries per lane;
   // TGP_CHAIN_BUDGET_US bounds one chained launch (100 ms).
   const bool MATCH_CHAIN = env_i64("TGP_MATCH_CHAIN", 1) != 0;
-  const int CHAIN_BLOCKS = (int)std::max<i64>(1, env_i64("TGP_CHAIN_BLOCKS", 512));
+  const int CHAIN_BLOCKS = (int)env_i64("TGP_CHAIN_BLOCKS", 0);  // 0: all resident blocks
   const int CHAIN_PER = (int)env_i64("TGP_CHAIN_PER", 8);
   // Lanes per vertex: 8 when every row fits 8 lanes' registers (8 vertices
   // per wave, so 4x as many vertices in flight: on a 3-D grid the decisions

@@ -14,6 +14,7 @@
 // them in row order in fp32.
 #include "tgp_internal.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace tgp {
@@ -3233,16 +3234,30 @@ void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, cons
                         hipStream_t s) {
   // sub-group width (lanes per vertex) and register entries per lane: rows
   // of up to sg * per arcs are held in registers
+  // blocks <= 0: as many as the device holds at once (every wave then takes
+  // part in the chain from the start; more blocks only queue behind them)
+  auto resident = [&](const void* fn) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return 512;
+    }
+    return std::max(1, per_cu * cus);
+  };
 #define TGP_CHAIN_CASE(SG, PER)                                                               \
   if (sg == SG && per == PER) {                                                               \
-    k_match_chain<SG, PER><<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, \
-                                                  stats);                                     \
+    const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<SG, PER>);       \
+    k_match_chain<SG, PER><<<nb, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget,     \
+                                              stats);                                         \
     return;                                                                                   \
   }
   TGP_CHAIN_CASE(16, 8)
   TGP_CHAIN_CASE(32, 8)
 #undef TGP_CHAIN_CASE
-  k_match_chain<8, 8><<<blocks, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
+  const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<8, 8>);
+  k_match_chain<8, 8><<<nb, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
 }
 
 void launch_max_row(DGraph g, u64* out, hipStream_t s) {
