@@ -185,3 +185,19 @@ def test_gpmetis_seam_in_a_directory(tmp_path, monkeypatch, stub_gpmetis):
         bad = tgp.CGpmetisDir(name.encode(), 0, 0)
         assert L.tgp_initpart_gpmetis_dir(C.byref(cg), 0, k, part_dir,
                                           C.cast(C.byref(bad), C.c_void_p)) == -1
+
+
+def test_sort_moves_rejects_bad_arguments():
+    """tgp_sort_moves checks its sizes and pointers before touching the device."""
+    import ctypes as C
+    L = tgp.lib()
+    buf = (C.c_float * 4)()
+    ids = (C.c_int32 * 4)()
+    assert L.tgp_sort_moves(None, 2, 4, C.cast(buf, C.c_void_p), C.cast(ids, C.c_void_p),
+                            None) == -1
+    assert L.tgp_sort_moves(C.cast(buf, C.c_void_p), 5, 4, C.cast(buf, C.c_void_p),
+                            C.cast(ids, C.c_void_p), None) == -1          # M > len
+    assert L.tgp_sort_moves(C.cast(buf, C.c_void_p), -1, 4, C.cast(buf, C.c_void_p),
+                            C.cast(ids, C.c_void_p), None) == -1
+    assert L.tgp_sort_moves(C.cast(buf, C.c_void_p), 0, 0, C.cast(buf, C.c_void_p),
+                            C.cast(ids, C.c_void_p), None) == 0           # nothing to sort
