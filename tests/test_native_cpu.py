@@ -201,3 +201,13 @@ def test_sort_moves_rejects_bad_arguments():
                             C.cast(ids, C.c_void_p), None) == -1
     assert L.tgp_sort_moves(C.cast(buf, C.c_void_p), 0, 0, C.cast(buf, C.c_void_p),
                             C.cast(ids, C.c_void_p), None) == 0           # nothing to sort
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says so,
+    before touching a device."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
