@@ -192,6 +192,73 @@ __global__ __launch_bounds__(256) void k_mean_agg(i64 n, const i64* xadj, const 
   }
 }
 
+// F = 64 (the hidden width): four vertices per wave, 16 lanes each holding 4
+// consecutive features (one 8-byte load per neighbour), neighbour ids shared
+// within the 16 lanes; four neighbours' loads in flight per step.
+constexpr int AGG4_U = 4;
+__global__ __launch_bounds__(256) void k_mean_agg64(i64 n, const i64* xadj, const i32* adj,
+                                                    const bf16* X, int transpose, bf16* out,
+                                                    float* outf) {
+  const int gl = threadIdx.x & 15;
+  const i64 ngroups = (i64)gridDim.x * (blockDim.x / 16);
+  for (i64 v = (i64)blockIdx.x * (blockDim.x / 16) + threadIdx.x / 16; v < n; v += ngroups) {
+    const i64 s0 = xadj[v], s1 = xadj[v + 1];
+    if (s1 - s0 > AGG_HEAVY) continue;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    for (i64 base = s0; base < s1; base += 16) {
+      const i64 i = base + gl;
+      const i32 myu = i < s1 ? adj[i] : 0;
+      float mydu = 0.0f;
+      if (transpose && i < s1) mydu = (float)(xadj[myu + 1] - xadj[myu]);
+      const int cnt = (int)(s1 - base < 16 ? s1 - base : 16);
+      for (int j0 = 0; j0 < cnt; j0 += AGG4_U) {
+        uint2 xs[AGG4_U];
+        float dus[AGG4_U];
+#pragma unroll
+        for (int jj = 0; jj < AGG4_U; jj++) {
+          const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+          const i32 u = __shfl(myu, j, 16);
+          dus[jj] = __shfl(mydu, j, 16);
+          xs[jj] = *reinterpret_cast<const uint2*>(X + (i64)u * 64 + 4 * gl);
+        }
+#pragma unroll
+        for (int jj = 0; jj < AGG4_U; jj++) {
+          if (j0 + jj >= cnt) break;
+          const float x0 = __uint_as_float(xs[jj].x << 16);
+          const float x1 = __uint_as_float(xs[jj].x & 0xffff0000u);
+          const float x2 = __uint_as_float(xs[jj].y << 16);
+          const float x3 = __uint_as_float(xs[jj].y & 0xffff0000u);
+          if (transpose) {
+            const float du = dus[jj];
+            a0 += du > 0.0f ? x0 / du : 0.0f;
+            a1 += du > 0.0f ? x1 / du : 0.0f;
+            a2 += du > 0.0f ? x2 / du : 0.0f;
+            a3 += du > 0.0f ? x3 / du : 0.0f;
+          } else {
+            a0 += x0;
+            a1 += x1;
+            a2 += x2;
+            a3 += x3;
+          }
+        }
+      }
+    }
+    if (!transpose && s1 > s0) {
+      const float d = (float)(s1 - s0);
+      a0 /= d;
+      a1 /= d;
+      a2 /= d;
+      a3 /= d;
+    }
+    const i64 o = v * 64 + 4 * gl;
+    out[o] = f2bf(a0);
+    out[o + 1] = f2bf(a1);
+    out[o + 2] = f2bf(a2);
+    out[o + 3] = f2bf(a3);
+    if (outf) *reinterpret_cast<float4*>(outf + o) = make_float4(a0, a1, a2, a3);
+  }
+}
+
 // Heavy rows: block b takes the vertices b, b + G, b + 2G, ... (G blocks,
 // so neighbouring hub ids land in different blocks), 1024 of them per round;
 // its 16 waves split each row over AGG_HEAVY arcs and the partial sums are
@@ -490,7 +557,12 @@ int tgpl_gemm(const uint16_t* A, int lda, int transA, const uint16_t* B, int ldb
 int tgpl_mean_agg(int64_t n, const int64_t* xadj, const int32_t* adj, const uint16_t* X, int F,
                   int transpose, uint16_t* out, float* outf, void* stream) {
   if (n <= 0 || F <= 0) return -1;
-  k_mean_agg<<<grid(n, 4), 256, 0, (hipStream_t)stream>>>(n, xadj, adj, X, F, transpose, out, outf);
+  if (F == 64)
+    k_mean_agg64<<<grid(n, 16), 256, 0, (hipStream_t)stream>>>(n, xadj, adj, X, transpose, out,
+                                                                outf);
+  else
+    k_mean_agg<<<grid(n, 4), 256, 0, (hipStream_t)stream>>>(n, xadj, adj, X, F, transpose, out,
+                                                             outf);
   k_mean_agg_heavy<<<grid(n, 1024, 2048), 1024, 0, (hipStream_t)stream>>>(n, xadj, adj, X, F,
                                                                           transpose, out, outf);
   return hipGetLastError() == hipSuccess ? 0 : -2;
