@@ -3,11 +3,13 @@
 The reference ships no tests or golden vectors (its only data is the 5-vertex
 CSR of README.md:58-82), so the oracle is pinned by:
   * known answers derived by hand from src/gpu_part.cu under the SEQ schedule
-    (KAT-C1, KAT-C2, KAT-M1, KAT-R1);
+    (KAT-C1, KAT-C2, KAT-M1, KAT-R1, and KAT-R2, where the capacity rule
+    binds and the per-destination rank decides the selection);
   * independent formulations of the same rule (greedy matching as the
     lexicographically-first maximal matching over arcs sorted by
     (source, -weight, position); the parallel round-based matching used on
-    the GPU; brute-force edge cuts);
+    the GPU; brute-force edge cuts; the repaired contraction as the quotient
+    graph P^T A P built with scipy);
   * invariants (sortedness of the bitonic output for power-of-two lengths,
     permutation property, symmetric matchings, partition counts).
 """
@@ -127,6 +129,119 @@ def test_kat_r1_passes(oracle):
     assert passes[1][4:7] == (3, 3, 0)
     assert passes[2][4:7] == (3, 3, 1)
     assert cut == 0 and part.tolist() == [1, 1, 1, 1]
+
+
+def fnv1a_int32(values):
+    """The trace hash: FNV-1a 64 (prime 0x100000001B3) over the little-endian
+    bytes of int32 values, started from 1469598103934665603."""
+    h = 1469598103934665603
+    for b in np.ascontiguousarray(values, dtype="<i4").tobytes():
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def kat_r2_graph():
+    """KAT-R2: 200 disjoint pairs (2i, 2i+1) of weight 200 - i, n = 400 (no
+    coarsening), k = 2, vertices 0..79 in part 1 and the rest in part 0."""
+    n = 400
+    edges = [(2 * i, 2 * i + 1) for i in range(200)]
+    x, a, w = graphs.csr_from_edges(n, edges, weights=[200 - i for i in range(200)])
+    part0 = np.where(np.arange(n) < 80, 1, 0).astype(np.int32)
+    return x, a, w, part0
+
+
+def test_kat_r2_capacity_binds(oracle):
+    """Phase 0, derived by hand from gpu_part.cu:258-380 under SEQ.
+
+    T = 200, slack = max(100, 400/20) = 100, imb = 10, maxSize = 310, minSize = 90,
+    |pairs| = 200 so vertex 399's row range [399, 200) is empty (B5).
+    Pass 0, pc = [320, 80]: a part-0 vertex gets bg = 200*(200-80)/200 + 100 + 50
+    = 270 and gain 270 - w (vertex 399: 270).  Token passing keeps the lower
+    vertex of each part-0 pair (equal gains) plus 398 and 399: M = 161 moves,
+    all 0 -> 1, gains 70 + i and 270, all distinct, so the network (length 256
+    <= n) orders them uniquely.  Capacity of part 1: 200 - 80 + 0 + 100 = 220,
+    within maxSize - 80 + 100 = 330, then pc[1] = 80 <= minSize in phase 0
+    clamps it to slack/2 = 50.  The 50 highest-gain moves are selected: 399
+    and 2i for i = 199..151.  Cut: pairs 151..198 split, sum of 200 - i = 1224.
+    Pass 1, pc = [270, 130]: bg = 70 + 100 + 50 = 220; candidates are the
+    part-0 vertices (gain 20 + i for pairs 40..150, 420 - i for the upper
+    vertices of pairs 151..198); M = 111 + 48 = 159 <= avail[1] = min(200 -
+    130 + 100, 310 - 130 + 100) = 170, so all are selected; cut = sum over
+    pairs 40..150 of 200 - i = 11655."""
+    x, a, w, part0 = kat_r2_graph()
+    n = len(part0)
+    part, pc, cut, tr = oracle.partition(x, a, w, 2, init_mode=oracle.INIT_ARRAY,
+                                         init_part=part0)
+    init = [t for t in tr if t[0] == oracle.TR_INIT][0]
+    assert init[4:7] == (400, 200, 0)
+    passes = [t for t in tr if t[0] == oracle.TR_PASS]
+    exp0 = part0.copy()
+    exp0[399] = 1
+    exp0[[2 * i for i in range(151, 200)]] = 1
+    assert passes[0][2:8] == (0, 0, 161, 50, 1224, 0)
+    assert passes[0][8] == fnv1a_int32(exp0)
+    exp1 = exp0.copy()
+    exp1[[2 * i for i in range(40, 151)]] = 1
+    exp1[[2 * i + 1 for i in range(151, 199)]] = 1
+    assert passes[1][2:8] == (0, 1, 159, 159, 11655, 0)
+    assert passes[1][8] == fnv1a_int32(exp1)
+    assert int((exp1 == 1).sum()) == 289
+    assert n == 400
+
+
+def test_kat_r2_planted_mistakes_fail(oracle):
+    """The KAT-R2 numbers separate the reference's rule from plausible misreadings
+    of it, restated here on the pass-0 move list (ascending vertex order)."""
+    moves = [(2 * i, 70 + i) for i in range(40, 200)] + [(399, 270)]
+    by_gain = [v for v, _ in sorted(moves, key=lambda m: -m[1])]
+    right = set(by_gain[:50])                      # rank after the sort, avail 50
+    assert right == {399} | {2 * i for i in range(151, 200)}
+    no_clamp = set(by_gain[:220])                  # slack/2 clamp dropped: all 161
+    assert len(no_clamp) == 161 != 50
+    pre_sort_rank = {v for v, _ in moves[:50]}     # rank in buffer order
+    assert pre_sort_rank != right
+    full_slack = set(by_gain[:100])                # clamp to slack instead of slack/2
+    assert len(full_slack) == 100 != 50
+
+
+def test_repaired_contraction_equals_quotient_graph(oracle):
+    """The repaired contraction against an independent construction: the
+    weights of coarse row c are the off-diagonal entries of P^T A P (P maps each
+    fine vertex to its coarse id, A sums duplicate arcs) and its order is the
+    order of first occurrence over fine row tb[2c] then fine row tb[2c+1]."""
+    import scipy.sparse as sp
+    for seed in range(6):
+        x, a, w = graphs.random_graph(300, 1500, 40 + seed, dup=seed % 2 == 0,
+                                      selfloops=seed % 3 == 0, wmax=1 if seed == 1 else 9)
+        if seed == 5:
+            w = w.copy()
+            w[::3] = 2 ** 30                     # sums wrap in int32
+        n = len(x) - 1
+        m = oracle.match(x, a, w)
+        new_ids, tb, n_c = oracle.relabel(m)
+        ox, oa, ow = oracle.contract_repaired(x, a, w, tb, new_ids, n_c)
+        src = np.repeat(np.arange(n), np.diff(x))
+        A = sp.coo_matrix((w.astype(np.int64), (src, a)), shape=(n, n)).tocsr()
+        P = sp.coo_matrix((np.ones(n, np.int64), (np.arange(n), new_ids)),
+                          shape=(n, n_c)).tocsr()
+        Q = (P.T @ A @ P).tocoo()
+        expect = {}
+        for r, c, v in zip(Q.row, Q.col, Q.data):
+            if r != c:
+                expect[(int(r), int(c))] = int(np.int64(v).astype(np.int32))
+        got = {}
+        for c in range(n_c):
+            row = oa[ox[c]:ox[c + 1]]
+            seq = np.concatenate([new_ids[a[x[f]:x[f + 1]]] for f in tb[2 * c:2 * c + 2]
+                                  if f >= 0])
+            seq = seq[seq != c]
+            _, first = np.unique(seq, return_index=True)
+            np.testing.assert_array_equal(row, seq[np.sort(first)])
+            for col, wt in zip(row.tolist(), ow[ox[c]:ox[c + 1]].tolist()):
+                got[(c, col)] = wt
+        # a pair whose weights sum to 0 is absent from P^T A P but kept by the rule
+        assert {key: v for key, v in got.items() if v != 0 or key in expect} == expect
+        assert all(v == 0 for key, v in got.items() if key not in expect)
 
 
 def test_gains_balance_phase_constants(oracle):
