@@ -34,6 +34,10 @@
 
 namespace {
 
+// Planted-mistake switch for the tests that show the known-answer tests can
+// fail (tgpo_plant_mistake); 0 everywhere else.
+int g_plant = 0;
+
 struct Graph {
   int64_t n = 0;
   std::vector<int64_t> xadj;  // n+1
@@ -201,7 +205,8 @@ int64_t contract_repaired(const View& g, const int32_t* tb, const int32_t* new_i
           ow[pos] = g.w[i];
           pos++;
         } else {
-          ow[slot_of[x]] = (int32_t)((uint32_t)ow[slot_of[x]] + (uint32_t)g.w[i]);
+          if (g_plant != 3)  // planted mistake 3: keep the first weight only
+            ow[slot_of[x]] = (int32_t)((uint32_t)ow[slot_of[x]] + (uint32_t)g.w[i]);
         }
       }
     }
@@ -413,7 +418,8 @@ void refine_seq(const View& g, int64_t nedges, int32_t k, int wide, int32_t* par
         int64_t a = T - current + outc[q] + slack;
         if (a < 0) a = 0;
         if (a > maxSize - current + slack) a = maxSize - current + slack;
-        if (current <= minSize && !phase) a = std::min(a, slack / 2);
+        if (current <= minSize && !phase)  // planted mistake 1: slack, not slack/2
+          a = std::min(a, g_plant == 1 ? slack : slack / 2);
         avail[q] = a;
       }
 
@@ -425,7 +431,8 @@ void refine_seq(const View& g, int64_t nedges, int32_t k, int wide, int32_t* par
         for (int64_t i = len; i < padded; i++) buf[i] = Move{-1, -1, -1, -1e9f};
         len = padded;
       }
-      bitonic_seq(buf.data(), len, [](const Move& m) { return m.gain; });
+      if (g_plant != 2)  // planted mistake 2: rank moves in buffer order (no sort)
+        bitonic_seq(buf.data(), len, [](const Move& m) { return m.gain; });
 
       std::fill(dc.begin(), dc.end(), 0);                   // selectMoves 373-380
       int64_t pads = 0, S = 0;
@@ -508,6 +515,8 @@ int read_part_file(const std::string& path, int64_t n_c, int32_t k, std::vector<
 }  // namespace
 
 extern "C" {
+
+void tgpo_plant_mistake(int32_t which) { g_plant = which; }
 
 uint64_t tgpo_hash(const int32_t* v, int64_t n) { return fnv(v, n); }
 

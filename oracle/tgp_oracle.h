@@ -113,6 +113,12 @@ int tgpo_refine(int64_t n, const int64_t* xadj, const int32_t* adjncy,
 /* gpu_part.cu:1296-1321: bytes of finalpartition.txt */
 int64_t tgpo_final_text(int64_t n, const int32_t* part, int32_t k, char* buf,
                         int64_t cap);
+/* Plants one deliberate mistake in the restatement so tests can show that
+   the known-answer tests catch it: 0 none (the default), 1 the phase-0
+   capacity clamp uses slack instead of slack/2 (gpu_part.cu:368), 2 moves are
+   ranked in buffer order instead of after the network (gpu_part.cu:596-600),
+   3 the repaired contraction keeps the first weight instead of the sum.     */
+void tgpo_plant_mistake(int32_t which);
 /* FNV-1a 64 over int32 values, the hash used in trace records */
 uint64_t tgpo_hash(const int32_t* v, int64_t n);
 

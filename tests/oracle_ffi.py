@@ -63,6 +63,7 @@ def lib():
                                   C.POINTER(Trace), i64, P]
         L.tgpo_final_text.argtypes = [i64, P, i32, C.c_char_p, i64]
         L.tgpo_final_text.restype = i64
+        L.tgpo_plant_mistake.argtypes = [i32]
         L.tgpo_hash.argtypes = [P, i64]
         L.tgpo_hash.restype = C.c_uint64
         _lib = L
@@ -221,6 +222,18 @@ def final_text(part, k):
     buf = C.create_string_buffer(int(size) + 1)
     lib().tgpo_final_text(len(part), _p(part), k, buf, size)
     return buf.raw[:size]
+
+
+class planted:
+    """Context manager: the oracle runs with planted mistake `which`."""
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().tgpo_plant_mistake(self.which)
+
+    def __exit__(self, *exc):
+        lib().tgpo_plant_mistake(0)
 
 
 def hash_part(part):

@@ -150,6 +150,24 @@ def kat_r2_graph():
     return x, a, w, part0
 
 
+def kat_r2_holds(oracle):
+    """True when the oracle reproduces KAT-R2's first two passes (below)."""
+    x, a, w, part0 = kat_r2_graph()
+    part, pc, cut, tr = oracle.partition(x, a, w, 2, init_mode=oracle.INIT_ARRAY,
+                                         init_part=part0)
+    init = [t for t in tr if t[0] == oracle.TR_INIT][0]
+    passes = [t for t in tr if t[0] == oracle.TR_PASS]
+    exp0 = part0.copy()
+    exp0[399] = 1
+    exp0[[2 * i for i in range(151, 200)]] = 1
+    exp1 = exp0.copy()
+    exp1[[2 * i for i in range(40, 151)]] = 1
+    exp1[[2 * i + 1 for i in range(151, 199)]] = 1
+    return (init[4:7] == (400, 200, 0) and
+            passes[0][2:9] == (0, 0, 161, 50, 1224, 0, fnv1a_int32(exp0)) and
+            passes[1][2:9] == (0, 1, 159, 159, 11655, 0, fnv1a_int32(exp1)))
+
+
 def test_kat_r2_capacity_binds(oracle):
     """Phase 0, derived by hand from gpu_part.cu:258-380 under SEQ.
 
@@ -168,49 +186,26 @@ def test_kat_r2_capacity_binds(oracle):
     vertices of pairs 151..198); M = 111 + 48 = 159 <= avail[1] = min(200 -
     130 + 100, 310 - 130 + 100) = 170, so all are selected; cut = sum over
     pairs 40..150 of 200 - i = 11655."""
-    x, a, w, part0 = kat_r2_graph()
-    n = len(part0)
-    part, pc, cut, tr = oracle.partition(x, a, w, 2, init_mode=oracle.INIT_ARRAY,
-                                         init_part=part0)
-    init = [t for t in tr if t[0] == oracle.TR_INIT][0]
-    assert init[4:7] == (400, 200, 0)
-    passes = [t for t in tr if t[0] == oracle.TR_PASS]
-    exp0 = part0.copy()
-    exp0[399] = 1
-    exp0[[2 * i for i in range(151, 200)]] = 1
-    assert passes[0][2:8] == (0, 0, 161, 50, 1224, 0)
-    assert passes[0][8] == fnv1a_int32(exp0)
-    exp1 = exp0.copy()
-    exp1[[2 * i for i in range(40, 151)]] = 1
-    exp1[[2 * i + 1 for i in range(151, 199)]] = 1
-    assert passes[1][2:8] == (0, 1, 159, 159, 11655, 0)
-    assert passes[1][8] == fnv1a_int32(exp1)
-    assert int((exp1 == 1).sum()) == 289
-    assert n == 400
+    assert kat_r2_holds(oracle)
 
 
-def test_kat_r2_planted_mistakes_fail(oracle):
-    """The KAT-R2 numbers separate the reference's rule from plausible misreadings
-    of it, restated here on the pass-0 move list (ascending vertex order)."""
-    moves = [(2 * i, 70 + i) for i in range(40, 200)] + [(399, 270)]
-    by_gain = [v for v, _ in sorted(moves, key=lambda m: -m[1])]
-    right = set(by_gain[:50])                      # rank after the sort, avail 50
-    assert right == {399} | {2 * i for i in range(151, 200)}
-    no_clamp = set(by_gain[:220])                  # slack/2 clamp dropped: all 161
-    assert len(no_clamp) == 161 != 50
-    pre_sort_rank = {v for v, _ in moves[:50]}     # rank in buffer order
-    assert pre_sort_rank != right
-    full_slack = set(by_gain[:100])                # clamp to slack instead of slack/2
-    assert len(full_slack) == 100 != 50
+@pytest.mark.parametrize("which", [1, 2])
+def test_kat_r2_catches_planted_mistakes(oracle, which):
+    """With the phase-0 clamp at slack instead of slack/2 (100 moves would be
+    selected), or with moves ranked in buffer order (the 50 lowest vertices),
+    KAT-R2 fails."""
+    with oracle.planted(which):
+        assert not kat_r2_holds(oracle)
+    assert kat_r2_holds(oracle)
 
 
-def test_repaired_contraction_equals_quotient_graph(oracle):
+def quotient_graph_holds(oracle, seeds=range(6)):
     """The repaired contraction against an independent construction: the
     weights of coarse row c are the off-diagonal entries of P^T A P (P maps each
     fine vertex to its coarse id, A sums duplicate arcs) and its order is the
     order of first occurrence over fine row tb[2c] then fine row tb[2c+1]."""
     import scipy.sparse as sp
-    for seed in range(6):
+    for seed in seeds:
         x, a, w = graphs.random_graph(300, 1500, 40 + seed, dup=seed % 2 == 0,
                                       selfloops=seed % 3 == 0, wmax=1 if seed == 1 else 9)
         if seed == 5:
@@ -236,12 +231,26 @@ def test_repaired_contraction_equals_quotient_graph(oracle):
                                   if f >= 0])
             seq = seq[seq != c]
             _, first = np.unique(seq, return_index=True)
-            np.testing.assert_array_equal(row, seq[np.sort(first)])
+            if not np.array_equal(row, seq[np.sort(first)]):
+                return False
             for col, wt in zip(row.tolist(), ow[ox[c]:ox[c + 1]].tolist()):
                 got[(c, col)] = wt
         # a pair whose weights sum to 0 is absent from P^T A P but kept by the rule
-        assert {key: v for key, v in got.items() if v != 0 or key in expect} == expect
-        assert all(v == 0 for key, v in got.items() if key not in expect)
+        if {key: v for key, v in got.items() if v != 0 or key in expect} != expect:
+            return False
+        if not all(v == 0 for key, v in got.items() if key not in expect):
+            return False
+    return True
+
+
+def test_repaired_contraction_equals_quotient_graph(oracle):
+    assert quotient_graph_holds(oracle)
+
+
+def test_quotient_graph_catches_planted_mistake(oracle):
+    """Keeping the first weight of a repeated neighbour instead of the sum fails."""
+    with oracle.planted(3):
+        assert not quotient_graph_holds(oracle)
 
 
 def test_gains_balance_phase_constants(oracle):
