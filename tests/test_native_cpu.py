@@ -211,3 +211,113 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
+
+
+STUB_DIR = os.path.join(ROOT, "tests", "golden", "stub_bin")
+
+
+def seam_call(tmp_path, monkeypatch, mode, k=4):
+    """Runs tgp_initpart_gpmetis (the working-directory seam) and the
+    directory seam with tests/golden/stub_bin/gpmetis in TGP_STUB_MODE `mode`;
+    returns (rc, part, status, last_error) of each."""
+    import ctypes as C
+    monkeypatch.setenv("PATH", f"{STUB_DIR}:{os.environ['PATH']}")
+    monkeypatch.setenv("TGP_STUB_MODE", mode)
+    L = tgp.lib()
+    xadj, adj, w = graphs.random_graph(40, 100, 9, wmax=3)
+    g = tgp.Graph(xadj, adj, w)
+    cg = g.cgraph()
+    cwd = tmp_path / f"cwd_{mode}"
+    cwd.mkdir()
+    monkeypatch.chdir(cwd)
+    out = []
+    status = C.c_int(-1)
+    part = (C.c_int32 * g.n)(*([77] * g.n))
+    rc = L.tgp_initpart_gpmetis(C.byref(cg), 0, k, part, C.cast(C.byref(status), C.c_void_p))
+    out.append((rc, list(part), status.value, L.tgp_last_error().decode()))
+    own = tmp_path / f"own_{mode}"
+    own.mkdir()
+    d = tgp.CGpmetisDir(str(own).encode(), -1, 0)
+    part = (C.c_int32 * g.n)(*([77] * g.n))
+    rc = L.tgp_initpart_gpmetis_dir(C.byref(cg), 0, k, part, C.cast(C.byref(d), C.c_void_p))
+    out.append((rc, list(part), d.status, L.tgp_last_error().decode()))
+    return g, cwd, own, out
+
+
+def test_gpmetis_seam_fault_modes(tmp_path, monkeypatch):
+    """The reference's handling of its METIS seam (gpu_part.cu:855-863,
+    1162-1173), with the stand-in's injected faults:
+      ok      parts i*k/n read back;
+      fail    gpmetis exits 3: "Error: gpmetis failed with return code 768"
+              (system()'s status), TGP_EINITPART;
+      nofile  no part file: "Error: Cannot open <graph>.part.<k>", TGP_EPARTFILE;
+      short   half the lines: the missing reads leave 0;
+      range   every value k+5: out-of-range ids become 0."""
+    k = 4
+    g, cwd, own, out = seam_call(tmp_path, monkeypatch, "", k)
+    n = g.n
+    expect = [i * k // n for i in range(n)]
+    for rc, part, status, _ in out:
+        assert rc == 0 and status == 0 and part == expect
+    assert (cwd / "output.graph").read_bytes() == tgp.metis_text(g)[0]
+
+    _, _, own, out = seam_call(tmp_path, monkeypatch, "fail", k)
+    for rc, part, status, err in out:
+        assert rc == -4 and status == 768
+        assert err == "Error: gpmetis failed with return code 768"
+
+    _, _, own, out = seam_call(tmp_path, monkeypatch, "nofile", k)
+    assert out[0][0] == -5 and out[0][3] == f"Error: Cannot open output.graph.part.{k}"
+    assert out[1][0] == -5 and out[1][3] == f"Error: Cannot open {own}/output.graph.part.{k}"
+
+    _, _, _, out = seam_call(tmp_path, monkeypatch, "short", k)
+    half = [i * k // n for i in range(n // 2)] + [0] * (n - n // 2)
+    for rc, part, _, _ in out:
+        assert rc == 0 and part == half
+
+    _, _, _, out = seam_call(tmp_path, monkeypatch, "range", k)
+    for rc, part, _, _ in out:
+        assert rc == 0 and part == [0] * n
+
+
+def test_gpmetis_seam_matches_oracle_part_file_rules(tmp_path, monkeypatch, oracle):
+    """The oracle's gpmetis mode reads the same stand-in outputs the same way:
+    the partition it starts from (visible in its INIT record's cut) equals
+    the library seam's, for the valid and the sanitised part files."""
+    x, a, w = graphs.path_graph(12)
+    monkeypatch.setenv("PATH", f"{STUB_DIR}:{os.environ['PATH']}")
+    for mode in ("", "short", "range"):
+        monkeypatch.setenv("TGP_STUB_MODE", mode)
+        d = tmp_path / f"o_{mode}"
+        d.mkdir()
+        monkeypatch.chdir(d)
+        _, _, _, tr = oracle.partition(x, a, w, 3, init_mode=oracle.INIT_GPMETIS)
+        init = [t for t in tr if t[0] == oracle.TR_INIT][0]
+        lines = open(d / "output.graph.part.3").read().split()
+        vals = [int(v) for v in lines] + [0] * (12 - len(lines))
+        vals = [v if 0 <= v < 3 else 0 for v in vals]
+        assert init[6] == oracle.edge_cut(x, a, w, init[5], vals, 3)
+    monkeypatch.setenv("TGP_STUB_MODE", "nofile")
+    d = tmp_path / "o_nofile"
+    d.mkdir()
+    monkeypatch.chdir(d)
+    with pytest.raises(oracle.OracleError) as ei:
+        oracle.partition(x, a, w, 3, init_mode=oracle.INIT_GPMETIS)
+    assert ei.value.code == -5
+    monkeypatch.setenv("TGP_STUB_MODE", "fail")
+    with pytest.raises(oracle.OracleError) as ei:
+        oracle.partition(x, a, w, 3, init_mode=oracle.INIT_GPMETIS)
+    assert ei.value.code == -4
+
+
+def test_row_offsets_are_read_as_int64(tmp_path):
+    """row.txt values past INT_MAX are kept exactly (int64 offsets), so the
+    text forms can describe more than 2^31 arcs in wide mode instead of
+    wrapping into a different, possibly valid-looking CSR."""
+    (tmp_path / "row.txt").write_text("0\n1\n4294967297\n")
+    (tmp_path / "column.txt").write_text("1\n0\n")
+    (tmp_path / "weight.txt").write_text("1\n1\n")
+    g = tgp.read_csr_files(str(tmp_path / "row.txt"), str(tmp_path / "column.txt"))
+    assert g.xadj.tolist() == [0, 1, 4294967297]
+    g = tgp.read_graph_dir(str(tmp_path), 2, 2)
+    assert g.xadj.tolist() == [0, 1, 4294967297]

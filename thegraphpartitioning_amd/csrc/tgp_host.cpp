@@ -25,8 +25,11 @@ namespace {
 
 // Reads exactly `count` decimal ints the way fscanf("%d") does for
 // well-formed input: skip white space, optional sign, digits.  Returns the
-// index of the first element that could not be read, or count.
-i64 read_ints(FILE* f, i32* out, i64 count) {
+// index of the first element that could not be read, or count.  Row offsets
+// are read into int64 (so row.txt can describe more than 2^31 arcs); ids and
+// weights into int32.
+template <typename T>
+i64 read_ints(FILE* f, T* out, i64 count) {
   std::vector<char> buf(1 << 20);
   size_t len = 0, pos = 0;
   bool eof = false;
@@ -58,16 +61,17 @@ i64 read_ints(FILE* f, i32* out, i64 count) {
     int64_t v = 0;
     while (c != EOF && c >= '0' && c <= '9') {
       v = v * 10 + (c - '0');
-      if (v > (int64_t)1 << 40) v = (int64_t)1 << 40;
+      if (v > (int64_t)1 << 62) v = (int64_t)1 << 62;
       pos++;
       c = peek();
     }
-    out[i] = (i32)(neg ? -v : v);
+    out[i] = (T)(neg ? -v : v);
   }
   return count;
 }
 
-int read_file(const std::string& path, i32* out, i64 count) {
+template <typename T>
+int read_file(const std::string& path, T* out, i64 count) {
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) {
     tgp::last_error() = "Error opening file: " + path + ": " + std::strerror(errno);
@@ -105,7 +109,6 @@ void tgp_free(void* p) { std::free(p); }
 
 int tgp_read_graph_dir(const char* dir, int64_t n, int64_t m, tgp_graph* out) {
   if (!dir || !out || n <= 0 || m <= 0) return TGP_EINVAL;
-  std::vector<i32> row((size_t)n + 1);
   int64_t* xadj = (int64_t*)std::malloc(sizeof(int64_t) * (n + 1));
   int32_t* adj = (int32_t*)std::malloc(sizeof(int32_t) * m);
   int32_t* w = (int32_t*)std::malloc(sizeof(int32_t) * m);
@@ -116,7 +119,7 @@ int tgp_read_graph_dir(const char* dir, int64_t n, int64_t m, tgp_graph* out) {
     return TGP_ENOMEM;
   }
   const std::string d(dir);
-  int rc = read_file(d + "/row.txt", row.data(), n + 1);
+  int rc = read_file(d + "/row.txt", xadj, n + 1);
   if (rc == TGP_OK) rc = read_file(d + "/column.txt", adj, m);
   if (rc == TGP_OK) rc = read_file(d + "/weight.txt", w, m);
   if (rc != TGP_OK) {
@@ -125,7 +128,6 @@ int tgp_read_graph_dir(const char* dir, int64_t n, int64_t m, tgp_graph* out) {
     std::free(w);
     return rc;
   }
-  for (i64 i = 0; i <= n; i++) xadj[i] = row[i];
   out->n = n;
   out->m = m;
   out->xadj = xadj;
@@ -179,7 +181,6 @@ int tgp_read_csr_files(const char* row_path, const char* col_path, tgp_graph* ou
     tgp::last_error() = "empty row or column file";
     return TGP_EINVAL;
   }
-  std::vector<i32> row((size_t)nr);
   int64_t* xadj = (int64_t*)std::malloc(sizeof(int64_t) * (n + 1));
   int32_t* adj = (int32_t*)std::malloc(sizeof(int32_t) * m);
   int32_t* w = (int32_t*)std::malloc(sizeof(int32_t) * m);
@@ -189,7 +190,7 @@ int tgp_read_csr_files(const char* row_path, const char* col_path, tgp_graph* ou
     std::free(w);
     return TGP_ENOMEM;
   }
-  rc = read_file(row_path, row.data(), nr);
+  rc = read_file(row_path, xadj, nr);
   if (rc == TGP_OK) rc = read_file(col_path, adj, m);
   if (rc != TGP_OK) {
     std::free(xadj);
@@ -197,7 +198,6 @@ int tgp_read_csr_files(const char* row_path, const char* col_path, tgp_graph* ou
     std::free(w);
     return rc;
   }
-  for (i64 i = 0; i <= n; i++) xadj[i] = row[i];
   for (i64 i = 0; i < m; i++) w[i] = 1;
   out->n = n;
   out->m = m;
