@@ -128,7 +128,9 @@ typedef struct tgp_options {
  * and commit (472-397).  Learned passes stop when fewer than min_admit * n
  * moves are admitted (or after max_passes); the level's ordinary refinement
  * follows (the fallback).  Needs k equal to the partition's k, k <= 256,
- * hidden > 0 and features >= 2k + 2 (else TGP_EINVAL).                      */
+ * hidden > 0 and features >= 2k + 2 (else TGP_EINVAL).  The library reads
+ * the weights on its own stream: work that writes them must have finished
+ * (or be ordered before the call) when tgp_partition is called.            */
 typedef struct tgp_learned_model {
   int32_t k, features, hidden, max_passes;
   const uint16_t* W1a;  /* features x hidden, BF16 */

@@ -69,3 +69,16 @@ def hub_graph(n, hubs, extra, seed, wmax=1, wbig=0):
     if wbig:
         w = np.where(np.array([e[0] < hubs for e in edges]), wbig, w)
     return csr_from_edges(n, edges, weights=w)
+
+
+def cut_recount(g, part, step=1 << 22):
+    """Edge cut of `part` over all arcs of g (sum of w over arcs whose ends lie
+    in different parts, halved), in vertex chunks so large graphs fit."""
+    total = 0
+    for v0 in range(0, g.n, step):
+        v1 = min(g.n, v0 + step)
+        e0, e1 = int(g.xadj[v0]), int(g.xadj[v1])
+        src = np.repeat(np.arange(v0, v1, dtype=np.int64), np.diff(g.xadj[v0:v1 + 1]))
+        diff = part[src] != part[g.adjncy[e0:e1]]
+        total += int(g.adjwgt[e0:e1][diff].astype(np.int64).sum())
+    return total // 2

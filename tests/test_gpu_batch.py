@@ -5,10 +5,14 @@ as the batched workload does, and runs them several at a time on separate
 streams; each partition, part count and cut must equal what tgp_partition
 returns for that graph alone, and a few are also checked against the oracle.
 """
+import os
+
 import numpy as np
 import pytest
 
 import thegraphpartitioning_amd as tgp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -133,3 +137,50 @@ def test_batch_gpmetis_in_own_directories(tmp_path, monkeypatch, stub_gpmetis):
         assert out[i][2] == cut, f"graph {i}"
         np.testing.assert_array_equal(out[i][0], part)
         np.testing.assert_array_equal(out[i][1], pc)
+
+
+def test_batch_small_level_pool_goes_again(monkeypatch, capfd):
+    """A level pool too small for every graph at once: graphs that find it full
+    are partitioned in later rounds of the persistent engine (or, if one alone
+    does not fit, by the per-stream path); every result still equals its solo run."""
+    monkeypatch.setenv("TGP_BATCH_ENGINE", "1")
+    monkeypatch.setenv("TGP_BATCH_TRACE", "1")
+    monkeypatch.setenv("TGP_BATCH_POOL_BYTES", str(16 << 20))
+    gs, ks = batch_graphs(20, 11, 30000)
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=4)
+    err = capfd.readouterr().err
+    line = [ln for ln in err.splitlines() if ln.startswith("# batch engine:")][-1]
+    print(line, flush=True)
+    rounds = int(line.split(" rounds")[0].split()[-1])
+    assert rounds >= 2, line
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
+        assert out[i][2] == cut, f"graph {i}"
+        np.testing.assert_array_equal(out[i][0], part, err_msg=f"graph {i}")
+        np.testing.assert_array_equal(out[i][1], pc, err_msg=f"graph {i}")
+
+
+def test_batch_4096_graphs_sample_equals_solo():
+    """The batched workload at its full size: 4096 graphs, n log-uniform in
+    [1k, 200k], half R-MAT edge factor 8, half Erdos-Renyi mean degree 10, k in
+    {2, 4, 8, 16}, resident in HBM, in one call; an evenly spaced sample of 64
+    results equals the solo runs bit for bit."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    gs, ks = bench.batched_workload(4096, bench.SEED)
+    print(f"batch 4096: {sum(g.m for g in gs)} arcs", flush=True)
+    dgs = [tgp.DeviceGraph(g) for g in gs]
+    try:
+        ptrs = [d.pointers() for d in dgs]
+        out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=16,
+                                  device_graphs=ptrs)
+        for i in range(0, 4096, 64):
+            part, pc, cut, _ = tgp.partition(gs[i], ks[i], init=tgp.INIT_BLOCK, repair=True,
+                                             device_graph=ptrs[i])
+            assert out[i][2] == cut, f"graph {i}"
+            np.testing.assert_array_equal(out[i][0], part, err_msg=f"graph {i}")
+            np.testing.assert_array_equal(out[i][1], pc, err_msg=f"graph {i}")
+    finally:
+        for d in dgs:
+            d.free()

@@ -1,0 +1,210 @@
+"""The drop-in at the workloads and the process contract a caller uses.
+
+* C1 (seeded R-MAT scale 18, edge factor 4, k = 4) against the oracle: the
+  repaired coarsening with the full per-level / per-pass trace, and the
+  reference's own semantics, which end in its fatal "Invalid
+  new_coarsed_edge" exit (gpu_part.cu:1121-1125) at the same level with the
+  same trace.
+* `gpu_part <dir> <n> <m> <k>` in its default mode (reference semantics, the
+  gpmetis seam with the stand-in tests/golden/stub_bin/gpmetis on PATH)
+  against the oracle's gpmetis mode: ./output.graph, ./finalpartition.txt
+  and the non-timing report lines byte for byte (gpu_part.cu:829-863,
+  1163-1173, 1296-1344), and each injected gpmetis fault.
+* The HBM-scale configuration (R-MAT scale 26, edge factor 16, weights in
+  [1, 16], int64 offsets past 2^31 arcs, k = 256, wide mode) through the
+  size-independent checks: a repeat run is identical, the reported cut equals
+  a host recount (gpu_part.cu:1285, 433-469), the part counts sum to n and
+  every part id is in range (gpu_part.cu:1296-1321).
+"""
+import os
+import subprocess
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import thegraphpartitioning_amd as tgp
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+STUB_DIR = os.path.join(ROOT, "tests", "golden", "stub_bin")
+C1_SEED = 0x7C0DE5EED
+
+
+def in_thread(fn, *a, **kw):
+    """Runs fn in a thread (the oracle's ctypes calls release the GIL, so it
+    overlaps the GPU run); returns a join() that yields fn's result or raises."""
+    box = {}
+
+    def body():
+        try:
+            box["r"] = fn(*a, **kw)
+        except BaseException as e:  # re-raised in join
+            box["e"] = e
+    t = threading.Thread(target=body)
+    t.start()
+
+    def join():
+        t.join()
+        if "e" in box:
+            return ("raised", box["e"])
+        return ("ok", box["r"])
+    return join
+
+
+def trace_tuples(tr):
+    return [tuple(t) for t in tr]
+
+
+def test_c1_repaired_full_trace(oracle):
+    g = tgp.gen_rmat(18, 4, C1_SEED)
+    print(f"C1 repaired: n={g.n} m={g.m} k=4", flush=True)
+    assert g.n == 122107 and g.m == 2016756
+    join = in_thread(oracle.partition, g.xadj, g.adjncy, g.adjwgt, 4, repair=1)
+    part, pc, cut, st = tgp.partition(g, 4, init=tgp.INIT_BLOCK, repair=True, check_level=1,
+                                      trace_cap=1 << 16)
+    kind, r = join()
+    assert kind == "ok", r
+    opart, opc, ocut, otr = r
+    assert trace_tuples(st.trace) == trace_tuples(otr)
+    assert cut == ocut
+    np.testing.assert_array_equal(part, opart)
+    np.testing.assert_array_equal(pc, opc)
+
+
+def test_c1_reference_semantics_fail_like_oracle(oracle):
+    g = tgp.gen_rmat(18, 4, C1_SEED)
+    print(f"C1 reference semantics: n={g.n} m={g.m} k=4", flush=True)
+    join = in_thread(oracle.partition, g.xadj, g.adjncy, g.adjwgt, 4, repair=0,
+                     trace_cap=1 << 15)
+    with pytest.raises(tgp.TGPError) as ei:
+        tgp.partition(g, 4, init=tgp.INIT_BLOCK, trace_cap=1 << 15)
+    kind, r = join()
+    assert kind == "raised" and isinstance(r, oracle.OracleError), r
+    assert ei.value.code == r.code == -3
+    assert "Invalid new_coarsed_edge: 0" in ei.value.detail
+    assert trace_tuples(ei.value.trace) == trace_tuples(r.trace)
+    levels = [t for t in r.trace if t[0] == oracle.TR_LEVEL]
+    assert len(levels) > 1000 and levels[-1][7] == 0          # m_c == 0
+
+
+def write_graph_dir(d, g):
+    d.mkdir()
+    np.savetxt(d / "row.txt", g.xadj, fmt="%d")
+    np.savetxt(d / "column.txt", g.adjncy, fmt="%d")
+    np.savetxt(d / "weight.txt", g.adjwgt, fmt="%d")
+
+
+def run_cli(args, cwd, mode=""):
+    env = dict(os.environ)
+    env["PATH"] = f"{STUB_DIR}:{env['PATH']}"
+    env["TGP_STUB_MODE"] = mode
+    return subprocess.run([tgp.CLI_PATH] + args, cwd=cwd, env=env, capture_output=True,
+                          text=True, timeout=120)
+
+
+def oracle_gpmetis_run(oracle, g, k, cwd, mode, monkeypatch):
+    monkeypatch.setenv("PATH", f"{STUB_DIR}:{os.environ['PATH']}")
+    monkeypatch.setenv("TGP_STUB_MODE", mode)
+    monkeypatch.chdir(cwd)
+    try:
+        return oracle.partition(g.xadj, g.adjncy, g.adjwgt, k, init_mode=oracle.INIT_GPMETIS)
+    finally:
+        monkeypatch.chdir(ROOT)
+
+
+def report_lines(n, m, k, cut, pc):
+    """The reference's non-timing report (gpu_part.cu:1330-1344)."""
+    out = ["Graph Detail ******************************************************",
+           f"{'Number of Vertices:':<35}{n:>10}", f"{'Number of Edges:':<35}{m:>10}",
+           f"{'Number of Partitions:':<35}{k:>10}", "",
+           "K-way GPU with Deterministic Refinement Details *******************",
+           f"{'Edge Cut:':<35}{cut:>10}", "Partition Sizes:"]
+    out += [f"  Partition {q}:{int(c):>10} vertices" for q, c in enumerate(pc)]
+    out += [f"{'Balance Constraint:':<35}{'3.0':>10}%",
+            f"{'Coarsening Threshold:':<35}{max(500, n // 500):>10} vertices",
+            f"{'Output File:':<35}{'./finalpartition.txt':>10}", ""]
+    return out
+
+
+@pytest.mark.parametrize("mode", ["", "short", "range"])
+def test_cli_default_mode_matches_oracle(oracle, tmp_path, monkeypatch, mode):
+    """Reference semantics and the gpmetis seam, end to end, as a caller runs it."""
+    g = tgp.gen_er(6000, 8.0, 17)
+    k = 4
+    d = tmp_path / "g"
+    write_graph_dir(d, g)
+    (tmp_path / "cli").mkdir()
+    (tmp_path / "orc").mkdir()
+    print(f"CLI default mode: stub mode {mode!r} n={g.n} m={g.m} k={k}", flush=True)
+    r = run_cli([str(d), str(g.n), str(g.m), str(k)], str(tmp_path / "cli"), mode)
+    assert r.returncode == 0, r.stderr
+    opart, opc, ocut, _ = oracle_gpmetis_run(oracle, g, k, tmp_path / "orc", mode, monkeypatch)
+    cli, orc = tmp_path / "cli", tmp_path / "orc"
+    assert (cli / "output.graph").read_bytes() == (orc / "output.graph").read_bytes()
+    assert (cli / f"output.graph.part.{k}").read_bytes() == \
+        (orc / f"output.graph.part.{k}").read_bytes()
+    assert (cli / "finalpartition.txt").read_bytes() == oracle.final_text(opart, k)
+    lines = r.stdout.splitlines()
+    expect = report_lines(g.n, g.m, k, ocut, opc)
+    assert lines[:len(expect)] == expect
+
+
+@pytest.mark.parametrize("mode,message", [
+    ("fail", "Error: gpmetis failed with return code 768"),
+    ("nofile", "Error: Cannot open output.graph.part.4")])
+def test_cli_gpmetis_faults(tmp_path, mode, message):
+    """gpu_part.cu:860-863 and 1164-1168: the message on stderr and exit status 1."""
+    g = tgp.gen_er(3000, 8.0, 5)
+    d = tmp_path / "g"
+    write_graph_dir(d, g)
+    (tmp_path / "run").mkdir()
+    r = run_cli([str(d), str(g.n), str(g.m), "4"], str(tmp_path / "run"), mode)
+    assert r.returncode == 1
+    assert r.stderr.strip().splitlines()[-1] == message
+    assert not (tmp_path / "run" / "finalpartition.txt").exists()
+
+
+def test_cli_reference_fatal_exit(tmp_path):
+    """An R-MAT graph ends the reference's coarsening in its fatal exit: the
+    command prints the message and exits with status 1 before calling gpmetis."""
+    g = tgp.gen_rmat(12, 8, C1_SEED + 12)
+    d = tmp_path / "g"
+    write_graph_dir(d, g)
+    (tmp_path / "run").mkdir()
+    r = run_cli([str(d), str(g.n), str(g.m), "16"], str(tmp_path / "run"))
+    assert r.returncode == 1
+    assert r.stderr.strip().splitlines()[-1] == "Invalid new_coarsed_edge: 0"
+    assert not (tmp_path / "run" / "output.graph").exists()
+
+
+def test_hbm_scale_rmat26_properties():
+    t = tgp.gen_rmat(26, 16, C1_SEED + 3, "hash16")
+    print(f"HBM-scale R-MAT 26: n={t.n} m={t.m} k=256", flush=True)
+    assert t.m > 2 ** 31 and int(t.xadj[-1]) == t.m
+    assert t.adjwgt.min() >= 1 and t.adjwgt.max() <= 16
+    dg = tgp.DeviceGraph(t)
+    try:
+        kw = dict(init=tgp.INIT_BLOCK, repair=True, wide=True, device_graph=dg.pointers())
+        a = tgp.partition(t, 256, **kw)
+        print(f"first run: cut={a[2]} levels={a[3].levels}", flush=True)
+        b = tgp.partition(t, 256, **kw)
+    finally:
+        dg.free()
+    assert a[2] == b[2]
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    part = b[0]
+    assert part.min() >= 0 and part.max() < 256
+    assert int(b[1].sum()) == t.n
+    recount = 0
+    step = 1 << 22
+    for v0 in range(0, t.n, step):
+        v1 = min(t.n, v0 + step)
+        e0, e1 = int(t.xadj[v0]), int(t.xadj[v1])
+        src = np.repeat(np.arange(v0, v1, dtype=np.int64), np.diff(t.xadj[v0:v1 + 1]))
+        diff = part[src] != part[t.adjncy[e0:e1]]
+        recount += int(t.adjwgt[e0:e1][diff].astype(np.int64).sum())
+    assert b[2] == recount // 2

@@ -253,3 +253,41 @@ def test_learned_partition_is_valid_and_deterministic():
     src = np.repeat(np.arange(g.n), np.diff(g.xadj))
     recount = int(g.adjwgt[part[src] != part[g.adjncy]].astype(np.int64).sum()) // 2
     assert cut == recount
+
+
+@pytest.mark.parametrize("which", ["rmat24", "grid256"])
+def test_learned_at_evaluation_size(which):
+    """The learned form on its evaluation graphs -- R-MAT scale 24 edge factor
+    16 at k = 64 and the 256^3 grid at k = 128 (wide mode) -- with a model
+    trained briefly on small graphs of the same family: parts in range, counts
+    summing to n, the reported cut equal to a host recount, repeat runs
+    identical."""
+    import graphs
+    if which == "rmat24":
+        k, train, wide = 64, tgp.gen_rmat(14, 16, 7), False
+        g = tgp.gen_rmat(24, 16, 0x7C0DE5EED + 1)
+    else:
+        k, train, wide = 128, tgp.gen_grid3d(24), True
+        g = tgp.gen_grid3d(256)
+    print(f"learned {which}: n={g.n} m={g.m} k={k}", flush=True)
+    data = learn.collect_traces([train], k, init=tgp.INIT_BLOCK, repair=True, wide=wide)
+    model = learn.GNN(k, seed=2)
+    dtrain = learn.DevGraph(train)
+    for b, a in data[0][1][:8]:
+        model.train_step(dtrain, torch.from_numpy(b).cuda(), torch.from_numpy(a).cuda())
+    del dtrain
+    h = model.handle(max_passes=4, min_admit=0.0)
+    dg = tgp.DeviceGraph(g)
+    try:
+        kw = dict(init=tgp.INIT_BLOCK, repair=True, wide=wide, learned=h,
+                  device_graph=dg.pointers())
+        r1 = tgp.partition(g, k, **kw)
+        r2 = tgp.partition(g, k, **kw)
+    finally:
+        dg.free()
+    part, pc, cut, _ = r1
+    np.testing.assert_array_equal(r1[0], r2[0])
+    np.testing.assert_array_equal(r1[1], r2[1])
+    assert r1[2] == r2[2]
+    assert part.min() >= 0 and part.max() < k and int(pc.sum()) == g.n
+    assert cut == graphs.cut_recount(g, part)

@@ -1374,6 +1374,10 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
   HIP_TRY(hipMemGetInfo(&freeb, &totb));
   const i64 fixed = bc * B.blocks_c + br * B.blocks_r + batch_record_bytes() * T + 4096;
   B.pool_cap = std::min<i64>(want, (i64)((double)freeb * 0.7) - fixed);
+  // test knob: a smaller level pool, so graphs find it full and go again in
+  // later rounds
+  const i64 pool_limit = env_i64("TGP_BATCH_POOL_BYTES", 0);
+  if (pool_limit > 0) B.pool_cap = std::min<i64>(B.pool_cap, pool_limit);
   void* mem = nullptr;
   if (B.pool_cap <= 0 || hipMalloc(&mem, (size_t)(fixed + B.pool_cap)) != hipSuccess) {
     (void)hipGetLastError();
@@ -1388,7 +1392,7 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
   B.scratch_r = B.scratch_c + bc * B.blocks_c;
   B.pool = m8 + fixed;
   const i64 budget = C.clock_khz * 2000;  // 2 s per level's matching
-  static const bool trace = env_i64("TGP_BATCH_TRACE", 0) != 0;
+  const bool trace = env_i64("TGP_BATCH_TRACE", 0) != 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DBuf<u64> bst;
   if (trace) {

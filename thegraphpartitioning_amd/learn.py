@@ -222,7 +222,12 @@ class GNN:
 
     # ---- use in the partitioner ------------------------------------------------
     def handle(self, max_passes=20, min_admit=0.0005, margin=3.0):
-        """A tgp_learned_model for tgp.partition(..., learned=...); keep self alive."""
+        """A tgp_learned_model for tgp.partition(..., learned=...); keep self alive.
+
+        The library reads the weights on its own stream, so the work that
+        wrote them on torch's current stream (training, load_state) is
+        finished here first."""
+        self.torch.cuda.current_stream().synchronize()
         self._cm = CModel(self.k, self.F, self.hidden, max_passes,
                           self.Wb["W1a"].data_ptr(), self.Wb["W1b"].data_ptr(),
                           self.Wb["W2a"].data_ptr(), self.Wb["W2b"].data_ptr(),
