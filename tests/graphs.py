@@ -71,14 +71,20 @@ def hub_graph(n, hubs, extra, seed, wmax=1, wbig=0):
     return csr_from_edges(n, edges, weights=w)
 
 
-def cut_recount(g, part, step=1 << 22):
+def cut_recount(g, part, step=1 << 24):
     """Edge cut of `part` over all arcs of g (sum of w over arcs whose ends lie
-    in different parts, halved), in vertex chunks so large graphs fit."""
+    in different parts, halved), with torch on the GPU in vertex chunks (an
+    independent count: no code of the partitioner runs)."""
+    import torch
+    dev = "cuda"
+    p = torch.from_numpy(np.ascontiguousarray(part)).to(dev)
     total = 0
     for v0 in range(0, g.n, step):
         v1 = min(g.n, v0 + step)
         e0, e1 = int(g.xadj[v0]), int(g.xadj[v1])
-        src = np.repeat(np.arange(v0, v1, dtype=np.int64), np.diff(g.xadj[v0:v1 + 1]))
-        diff = part[src] != part[g.adjncy[e0:e1]]
-        total += int(g.adjwgt[e0:e1][diff].astype(np.int64).sum())
+        deg = torch.from_numpy(np.diff(g.xadj[v0:v1 + 1])).to(dev)
+        src = torch.repeat_interleave(torch.arange(v0, v1, device=dev), deg)
+        adj = torch.from_numpy(g.adjncy[e0:e1]).to(dev).long()
+        w = torch.from_numpy(g.adjwgt[e0:e1]).to(dev).long()
+        total += int(torch.where(p[src] != p[adj], w, torch.zeros_like(w)).sum().item())
     return total // 2
