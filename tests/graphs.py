@@ -71,11 +71,13 @@ def hub_graph(n, hubs, extra, seed, wmax=1, wbig=0):
     return csr_from_edges(n, edges, weights=w)
 
 
-def cut_recount(g, part, step=1 << 24):
+def cut_recount(g, part, step=1 << 22):
     """Edge cut of `part` over all arcs of g (sum of w over arcs whose ends lie
     in different parts, halved), with torch on the GPU in vertex chunks (an
     independent count: no code of the partitioner runs)."""
     import torch
+    import thegraphpartitioning_amd as tgp
+    tgp.release_device_memory()          # the library's cached blocks go back first
     dev = "cuda"
     p = torch.from_numpy(np.ascontiguousarray(part)).to(dev)
     total = 0
@@ -87,4 +89,7 @@ def cut_recount(g, part, step=1 << 24):
         adj = torch.from_numpy(g.adjncy[e0:e1]).to(dev).long()
         w = torch.from_numpy(g.adjwgt[e0:e1]).to(dev).long()
         total += int(torch.where(p[src] != p[adj], w, torch.zeros_like(w)).sum().item())
+        del deg, src, adj, w
+    del p
+    torch.cuda.empty_cache()
     return total // 2

@@ -301,6 +301,12 @@ def metis_text(g):
     return data, npairs.value
 
 
+def release_device_memory():
+    """Returns the device blocks the library's pool keeps for reuse to the HIP
+    runtime (tgp_release_device_memory)."""
+    lib().tgp_release_device_memory()
+
+
 class DeviceGraph:
     """A CSR copied once into device memory (HBM); free() returns it."""
 
