@@ -209,6 +209,30 @@ typedef struct tgp_batch_item {
 } tgp_batch_item;
 int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers);
 
+/* ---- partition without allocating (no counterpart in the reference) -------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * partition as tgp_partition (bit for bit, same arguments and outputs), but
+ * every device buffer it needs -- the uploaded CSR when the input is host
+ * memory, the coarse levels, refinement state, tallies -- comes from one
+ * static device array of TGP_NOALLOC_SCRATCH_BYTES per GPU, loaded with the
+ * library and held until the process ends; it calls no hipMalloc, hipFree or
+ * other allocating routine.  Graphs above TGP_NOALLOC_MAX_N vertices,
+ * TGP_NOALLOC_MAX_M arcs or TGP_NOALLOC_MAX_K parts are rejected (TGP_EINVAL,
+ * message in tgp_last_error); a partition that still needs more than the
+ * scratch holds fails with TGP_ENOMEM.  No learned proposer or profiling.
+ * The first call on a device sets up that device's context (its stream and
+ * the scratch address).  Calls share the device's scratch, which keeps the
+ * last call's contents: the library clears whatever must start at zero, and
+ * calls on one device are serialised by a lock (calls that overlapped on one
+ * GPU's scratch would race).                                                 */
+#define TGP_NOALLOC_SCRATCH_BYTES ((int64_t)512 << 20)
+#define TGP_NOALLOC_MAX_N ((int64_t)1 << 20)
+#define TGP_NOALLOC_MAX_M ((int64_t)1 << 23)
+#define TGP_NOALLOC_MAX_K 256
+int tgp_partition_noalloc(const tgp_graph* g, const tgp_options* o, int32_t* part,
+                          int32_t* part_count, int64_t* edge_cut, tgp_stats* stats);
+
 const char* tgp_strerror(int code);
 /* Detail of the last failure on the calling thread ("" if none).            */
 const char* tgp_last_error(void);

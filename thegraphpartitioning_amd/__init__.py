@@ -23,6 +23,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtgp.so")
 CLI_PATH = os.path.join(_HERE, "gpu_part")
 
+# size limits of the entry point that allocates nothing (tgp.h)
+NOALLOC_MAX_N = 1 << 20
+NOALLOC_MAX_M = 1 << 23
+NOALLOC_MAX_K = 256
+
 INIT_GPMETIS = "gpmetis"
 INIT_BLOCK = "block"
 
@@ -102,6 +107,8 @@ def lib():
     L.tgp_partition.argtypes = [C.POINTER(CGraph), C.POINTER(COptions), P, P, P,
                                 C.POINTER(CStats)]
     L.tgp_partition.restype = C.c_int
+    L.tgp_partition_noalloc.argtypes = L.tgp_partition.argtypes
+    L.tgp_partition_noalloc.restype = C.c_int
     L.tgp_strerror.argtypes = [C.c_int]
     L.tgp_strerror.restype = C.c_char_p
     L.tgp_last_error.restype = C.c_char_p
@@ -346,7 +353,7 @@ class Stats:
 
 def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=False,
               coarsen_floor=0, profile=False, check_level=0, trace_cap=0, device_graph=None,
-              pass_dump=None, learned=None):
+              pass_dump=None, learned=None, noalloc=False):
     """Partitions `g` (a Graph) into k parts on the GPU.
 
     Returns (part[n] int32, part_count[k] int32, edge_cut, Stats).  `init`
@@ -359,6 +366,8 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
     PASS_DUMP_FN called with each level's partition at every phase start and
     after every committed pass; `learned` a tgp_learned_model (learn.GNN.handle())
     whose learned passes run at the finest level before its refinement.
+    `noalloc` selects tgp_partition_noalloc (same result; every device buffer
+    from the library's fixed scratch; size limits TGP_NOALLOC_MAX_*).
     """
     L = lib()
     o = COptions()
@@ -420,8 +429,9 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
         tbuf = (CTrace * trace_cap)()
         st.trace = C.cast(tbuf, C.POINTER(CTrace))
         st.trace_cap = trace_cap
-    rc = L.tgp_partition(C.byref(cg), C.byref(o), part.ctypes.data, pc.ctypes.data,
-                         C.addressof(cut), C.byref(st))
+    entry = L.tgp_partition_noalloc if noalloc else L.tgp_partition
+    rc = entry(C.byref(cg), C.byref(o), part.ctypes.data, pc.ctypes.data,
+               C.addressof(cut), C.byref(st))
     trace = []
     if tbuf is not None:
         for i in range(min(st.trace_len, trace_cap)):

@@ -46,11 +46,29 @@ struct Fail {
 // ---------------------------------------------------------------- device pool --
 // Grow-only cache of device blocks keyed by rounded size; blocks return to the
 // cache when released and are freed when the context is destroyed.
+//
+// Arena mode (set_arena): every block comes from one fixed device region the
+// caller owns -- first fit over a list of free ranges, coalesced on release --
+// and the pool never calls hipMalloc or hipFree.  Used by the entry point that
+// allocates nothing (tgp_partition_noalloc).
 class DevicePool {
  public:
   ~DevicePool() {
     for (auto& kv : free_)
       for (void* p : kv.second) (void)hipFree(p);
+  }
+  void set_arena(void* base, size_t cap) {
+    arena_ = (char*)base;
+    arena_cap_ = cap;
+    ranges_.clear();
+    ranges_[0] = cap;
+  }
+  bool arena() const { return arena_ != nullptr; }
+  size_t arena_cap() const { return arena_cap_; }
+  size_t arena_free() const {
+    size_t t = 0;
+    for (const auto& r : ranges_) t += r.second;
+    return t;
   }
   static size_t round_size(size_t b) {
     if (b < 256) return 256;
@@ -60,6 +78,7 @@ class DevicePool {
     return ((b + step - 1) / step) * step;
   }
   void* get(size_t bytes) {
+    if (arena_) return arena_get(bytes);
     const size_t r = round_size(bytes);
     auto it = free_.find(r);
     if (it != free_.end() && !it->second.empty()) {
@@ -82,7 +101,12 @@ class DevicePool {
     return p;
   }
   void put(void* p, size_t bytes) {
-    if (p) free_[round_size(bytes)].push_back(p);
+    if (!p) return;
+    if (arena_) {
+      arena_put(p, bytes);
+      return;
+    }
+    free_[round_size(bytes)].push_back(p);
   }
   // bytes held in released blocks (hipMemGetInfo counts them as used; get()
   // returns them to the runtime when an allocation would otherwise fail)
@@ -92,6 +116,7 @@ class DevicePool {
     return t;
   }
   void trim() {
+    if (arena_) return;
     for (auto& kv : free_) {
       for (void* p : kv.second) (void)hipFree(p);
       kv.second.clear();
@@ -99,7 +124,42 @@ class DevicePool {
   }
 
  private:
+  static size_t arena_round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
+  void* arena_get(size_t bytes) {
+    const size_t r = arena_round(bytes);
+    for (auto it = ranges_.begin(); it != ranges_.end(); ++it) {
+      if (it->second < r) continue;
+      const size_t off = it->first, len = it->second;
+      ranges_.erase(it);
+      if (len > r) ranges_[off + r] = len - r;
+      return arena_ + off;
+    }
+    set_error("the no-allocation entry point's scratch of " + std::to_string(arena_cap_ >> 20) +
+              " MiB is exhausted (" + std::to_string(r) + " more bytes needed, " +
+              std::to_string(arena_free()) + " free in pieces)");
+    throw Fail(TGP_ENOMEM);
+  }
+  void arena_put(void* p, size_t bytes) {
+    size_t off = (size_t)((char*)p - arena_), len = arena_round(bytes);
+    auto next = ranges_.lower_bound(off);
+    if (next != ranges_.end() && off + len == next->first) {
+      len += next->second;
+      next = ranges_.erase(next);
+    }
+    if (next != ranges_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += len;
+        return;
+      }
+    }
+    ranges_[off] = len;
+  }
+
   std::map<size_t, std::vector<void*>> free_;
+  char* arena_ = nullptr;
+  size_t arena_cap_ = 0;
+  std::map<size_t, size_t> ranges_;  // arena mode: free ranges, offset -> length
 };
 
 template <typename T>
@@ -261,19 +321,29 @@ struct Context {
   Prof prof;
   i64 clock_khz = 0;  // device wall-clock rate
 
-  Context() {
+  // arena != nullptr: the pool works inside that device region and the host
+  // staging area is `host_stage` (ordinary memory), so the context allocates
+  // nothing after construction
+  explicit Context(void* arena = nullptr, size_t arena_bytes = 0, i64* host_stage = nullptr) {
     HIP_TRY(hipGetDevice(&device));
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     int khz = 0;
     HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
     clock_khz = std::max(khz, 1000);
-    HIP_TRY(hipHostMalloc((void**)&pinned, PINNED_ELEMS * sizeof(i64), hipHostMallocDefault));
+    if (arena) {
+      pool.set_arena(arena, arena_bytes);
+      pinned = host_stage;
+    } else {
+      HIP_TRY(hipHostMalloc((void**)&pinned, PINNED_ELEMS * sizeof(i64), hipHostMallocDefault));
+      own_pinned = true;
+    }
     prof.s = s;
   }
   ~Context() {
-    if (pinned) (void)hipHostFree(pinned);
+    if (pinned && own_pinned) (void)hipHostFree(pinned);
     if (s) (void)hipStreamDestroy(s);
   }
+  bool own_pinned = false;
   void sync() {
     HIP_TRY(hipStreamSynchronize(s));
     prof.flush();
@@ -916,11 +986,15 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     // part, then updated from the applied moves of each commit.  The table
     // may take up to 40% of the free device memory.
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
-      (void)hipGetLastError();
-      free_b = 0;
+    if (C.pool.arena()) {
+      free_b = C.pool.arena_free();
+    } else {
+      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        free_b = 0;
+      }
+      free_b += C.pool.cached_bytes();
     }
-    free_b += C.pool.cached_bytes();
     // A few rows whose sum of |w| passes 2^24 keep the kept-tally form: their
     // gains are gathered again in fp32 row order after every tally pass.
     const bool tmode = k <= 512 && (ax.all_exact || ax.nover <= OVER_ROWS_MAX) &&
@@ -1271,6 +1345,24 @@ int validate_host_graph(const tgp_graph* g) {
     if (g->adjncy[i] < 0 || g->adjncy[i] >= g->n) return TGP_EINVAL;
   return TGP_OK;
 }
+
+// Device scratch of the entry point that allocates nothing
+// (tgp_partition_noalloc): every block its partitions use comes from here.  One
+// copy per GPU, loaded with the code object and held until the process ends.
+// Host code never passes `scratch` itself (that is a host-side stand-in, not a
+// device address): the address each device holds it at comes from
+// hipGetSymbolAddress at set-up.
+static __device__ __align__(256) unsigned char scratch[TGP_NOALLOC_SCRATCH_BYTES];
+
+// The no-allocation contexts, one per device: built at the first call on the
+// device (with it current) around that device's copy of `scratch`.
+struct NoallocDevice {
+  Context* ctx = nullptr;
+  void* scratch_ptr = nullptr;
+  std::unique_ptr<i64[]> stage;  // host staging area (ordinary memory)
+};
+std::mutex g_noalloc_mu;
+std::map<int, NoallocDevice> g_noalloc;
 
 // One context per process, created on first use and intentionally never
 // destroyed (its teardown would race the HIP runtime's own at exit).  Calls
@@ -1668,6 +1760,53 @@ void tgp_graph_device_free(tgp_graph* dev) {
   dev->xadj = nullptr;
   dev->adjncy = nullptr;
   dev->adjwgt = nullptr;
+}
+
+int tgp_partition_noalloc(const tgp_graph* g, const tgp_options* o, int32_t* part,
+                          int32_t* part_count, int64_t* edge_cut, tgp_stats* stats) {
+  last_error().clear();
+  if (!g || !o) {
+    set_error("null argument");
+    return TGP_EINVAL;
+  }
+  if (g->n > TGP_NOALLOC_MAX_N || g->m > TGP_NOALLOC_MAX_M || o->k > TGP_NOALLOC_MAX_K) {
+    set_error("tgp_partition_noalloc takes n <= " + std::to_string(TGP_NOALLOC_MAX_N) +
+              ", m <= " + std::to_string(TGP_NOALLOC_MAX_M) + " and k <= " +
+              std::to_string(TGP_NOALLOC_MAX_K) + " (its device scratch holds " +
+              std::to_string(TGP_NOALLOC_SCRATCH_BYTES >> 20) +
+              " MiB); use tgp_partition for larger graphs");
+    return TGP_EINVAL;
+  }
+  if (o->learned || o->profile) {
+    set_error("tgp_partition_noalloc runs neither the learned proposer nor profiling");
+    return TGP_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("no HIP device");
+    return TGP_EHIP;
+  }
+  NoallocDevice& nd = g_noalloc[dev];
+  if (!nd.ctx) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      set_error(std::string("hipGetSymbolAddress of the scratch array failed: ") +
+                hipGetErrorString(e));
+      return TGP_EHIP;
+    }
+    nd.scratch_ptr = ptr;
+    nd.stage.reset(new i64[Context::PINNED_ELEMS]);
+    try {
+      nd.ctx = new Context(nd.scratch_ptr, (size_t)TGP_NOALLOC_SCRATCH_BYTES, nd.stage.get());
+    } catch (const Fail& f) {
+      return f.code;
+    }
+  }
+  return partition_on(*nd.ctx, g, o, part, part_count, edge_cut, stats);
 }
 
 void tgp_release_device_memory(void) {
