@@ -382,6 +382,7 @@ def main():
         st = rp[3]
         print(json.dumps({"n": g.n, "m": g.m, "seconds": st.seconds_total, "levels": st.levels,
                           "passes": st.passes, "match_rounds": st.match_rounds,
+                          "sorts_skipped": st.sorts_skipped,
                           "cut": rp[2], "coarsest_n": st.coarsest_n,
                           "kernel_ms": {a: round(b, 2) for a, b in st.kernel_ms.items()},
                           "launches": st.kernel_launches}), flush=True)

@@ -172,6 +172,10 @@ typedef struct tgp_stats {
   tgp_trace* trace;                /* caller buffer for trace records or NULL    */
   int64_t trace_cap, trace_len;
   int64_t fail_level, fail_value;  /* TGP_EEDGES: level and the m_c it rejected  */
+  int64_t sorts_skipped;           /* passes whose move sort was skipped: every
+                                      move fit its destination and the network
+                                      would have sorted them all, so it could not
+                                      change the selection                       */
 } tgp_stats;
 
 /* Partitions `g` into o->k parts.  Replaces main()'s pipeline

@@ -273,3 +273,26 @@ def test_token_and_gain_lane_widths(oracle, monkeypatch, token, gain):
     g = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 8)
     gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 32, repair=True, floor=100)
     assert_same(gp, op, f"token lanes {token}, gain lanes {gain}")
+
+
+@pytest.mark.parametrize("skip", ["1", "0"])
+def test_sort_skip_is_exact(oracle, monkeypatch, skip):
+    """The move sort is skipped on passes where every move fits its destination
+    and the network would sort all of them (TGP_SORT_SKIP=1, the default);
+    results and traces equal the oracle's with and without the skip.  KAT-R2
+    (200 weighted pairs, capacity binds on pass 0 and not on pass 1) exercises
+    both kinds of pass."""
+    import test_oracle_kat as kat
+    monkeypatch.setenv("TGP_SORT_SKIP", skip)
+    x, a, w, part0 = kat.kat_r2_graph()
+    gp, op = run_both(oracle, x, a, w, 2, init_part=part0)
+    assert_same(gp, op, f"KAT-R2 skip={skip}")
+    assert gp[3].trace[1][4:7] == (161, 50, 1224)
+    skipped = gp[3].sorts_skipped
+    g = tgp.gen_rmat(13, 8, 77)
+    gp2, op2 = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 16, repair=True, floor=100)
+    assert_same(gp2, op2, f"R-MAT 13 skip={skip}")
+    if skip == "1":
+        assert skipped > 0
+    else:
+        assert skipped == 0 and gp2[3].sorts_skipped == 0

@@ -408,6 +408,7 @@ class Run {
   i32 k = 0;
   bool wide = false, repair = false;
   int passes_total = 0;
+  i64 sorts_skipped = 0;  // passes whose move sort could not change the selection
   i64 match_rounds = 0;
   double initpart_s = 0;
 
@@ -537,17 +538,39 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     DBuf<i32> stv(&C.pool, n);
     DBuf<u64> cstats;
     if (trace_match) {
-      cstats = DBuf<u64>(&C.pool, 10);
-      HIP_TRY(hipMemsetAsync(cstats.p, 0, 10 * sizeof(u64), st()));
+      cstats = DBuf<u64>(&C.pool, 12);
+      HIP_TRY(hipMemsetAsync(cstats.p, 0, 12 * sizeof(u64), st()));
     }
     launch_fill_i32(stv.p, n, -1, st());
     HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
+    // Lower-neighbour lists of the rows the kernel scans with the whole wave
+    // (longer than CHAIN_SG * CHAIN_PER): "no undecided neighbour below t"
+    // then reads only the entries below t instead of the whole row; on a
+    // power-law level the hub rows are long and their lower neighbours few.
+    // TGP_CHAIN_LOW=0 scans whole rows.
+    DBuf<i64> lowcnt, lowoff;
+    DBuf<i32> lowadj;
+    ChainLow low{nullptr, nullptr};
+    if (env_i64("TGP_CHAIN_LOW", 1) != 0) {
+      const i64 minlen = (i64)CHAIN_SG * CHAIN_PER;
+      lowcnt = DBuf<i64>(&C.pool, n);
+      lowoff = DBuf<i64>(&C.pool, n + 1);
+      DBuf<i32> llist(&C.pool, n), nll(&C.pool, 1);
+      launch_fill_i64(lowcnt.p, n, 0, st());
+      HIP_TRY(hipMemsetAsync(nll.p, 0, sizeof(i32), st()));
+      launch_chain_low_count(g, minlen, llist.p, nll.p, lowcnt.p, st());
+      launch_scan_excl(lowcnt.p, lowoff.p, n, lowoff.p + n, scratch(), st());
+      const i64 tot = read_i64(lowoff.p + n);
+      lowadj = DBuf<i32>(&C.pool, std::max<i64>(tot, 1));
+      if (tot > 0) launch_chain_low_write(g, llist.p, nll.p, lowoff.p, lowadj.p, st());
+      low = ChainLow{lowoff.p, lowadj.p};
+    }
     C.prof.end(KID_MATCH, 0);
     i64 prev = -1;
     int stuck = 0;
     while (stuck < 3) {
       C.prof.begin(KID_CHAIN);
-      launch_match_chain(g, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, chain_budget,
+      launch_match_chain(g, low, stv.p, s1.p, s2.p, prop.p, lo.p, ticket.p, chain_budget,
                          CHAIN_BLOCKS, CHAIN_SG, CHAIN_PER, cstats.p, st());
       C.prof.end(KID_CHAIN, round == 0 ? match_bytes : 0.0);
       round++;
@@ -555,6 +578,11 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
       HIP_TRY(hipMemsetAsync(ticket.p, 0, sizeof(u32), st()));
       HIP_TRY(hipMemcpyAsync(C.pinned, lo.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
       C.sync();
+      if (trace_match)
+        std::fprintf(stderr, "#   launch %u: first undecided %lld of %lld at %.2f ms\n", round,
+                     (long long)C.pinned[0], (long long)n,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                               t_start).count());
       if (C.pinned[0] >= n) break;
       stuck = C.pinned[0] > prev ? 0 : stuck + 1;
       prev = C.pinned[0];
@@ -565,11 +593,12 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
       C.prof.end(KID_MATCH, 0);
       match_rounds += round;
       if (trace_match) {
-        u64 cs[10];
+        u64 cs[12];
         HIP_TRY(hipMemcpy(cs, cstats.p, sizeof(cs), hipMemcpyDeviceToHost));
         std::fprintf(stderr,
                      "# match n=%lld m=%lld chained launches=%u ms=%.2f iter=%llu idle=%llu "
-                     "apS=%llu aL=%llu pL=%llu fetch=%llu cS=%llu cL=%llu dec=%llu arcsL=%llu\n",
+                     "apS=%llu aL=%llu pL=%llu fetch=%llu cS=%llu cL=%llu dec=%llu arcsA=%llu "
+                     "arcsP=%llu arcsC=%llu\n",
                      (long long)n, (long long)lv.m, round,
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
                                                                t_start).count(),
@@ -577,7 +606,8 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
                      (unsigned long long)cs[2], (unsigned long long)cs[3],
                      (unsigned long long)cs[4], (unsigned long long)cs[5],
                      (unsigned long long)cs[6], (unsigned long long)cs[7],
-                     (unsigned long long)cs[8], (unsigned long long)cs[9]);
+                     (unsigned long long)cs[8], (unsigned long long)cs[9],
+                     (unsigned long long)cs[10], (unsigned long long)cs[11]);
       }
       return;
     }
@@ -1015,8 +1045,13 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     DBuf<i32> bcnt(&C.pool, compaction_blocks()), boff(&C.pool, compaction_blocks());
     DBuf<i64> avail(&C.pool, k);
     DBuf<float> bg(&C.pool, 2 * (size_t)k);
-    DBuf<i32> outc(&C.pool, k), ctr(&C.pool, 4);
+    // outc[0..k) moves out of each part, outc[k..2k) moves into each part;
+    // fit[0] = 1 while every destination's moves fit its capacity
+    DBuf<i32> outc(&C.pool, 2 * (size_t)k), ctr(&C.pool, 4), fit(&C.pool, 1);
     DBuf<i32> H(&C.pool, select_scratch_elems(n, k));
+    // the move sort is skipped when it cannot change the selection: all M
+    // moves fit their destinations and the network would sort them fully
+    const bool sort_skip = env_i64("TGP_SORT_SKIP", 1) != 0;
     DBuf<i32> gscr(&C.pool, std::max<i64>(gain_gscratch_elems(k), 1));
     std::vector<i32> hpc((size_t)k);
     // a pass's count, cut, selection counters and part counts go to the host
@@ -1044,8 +1079,9 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         i32* ctr_prev = ctr.p + 2 * ((pass + 1) & 1);
         if (run) {
           C.prof.begin(KID_MISC);
-          // also clears outc[0..k) and the pass's two selection counters
-          launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, outc.p, ctr_cur, st());
+          // also clears outc[0..2k) and the pass's two selection counters, sets fit
+          launch_bg_table(pc.p, k, n, phase, bg.p, src_big.p, outc.p, 2 * k, ctr_cur, fit.p,
+                          st());
           C.prof.end(KID_MISC, 0);
           C.prof.begin(KID_GAIN);
           bool fused;
@@ -1076,8 +1112,13 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
           C.prof.begin(KID_COMPACT);
           launch_compact_moves(flag.p, n, part.p, target.p, gain.p, k, bcnt.p, boff.p, small.p,
-                               mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, st());
+                               mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, outc.p + k, st());
           C.prof.end(KID_COMPACT, 2.0 * n);
+          // capacities from this pass's counts (the previous commit is done)
+          C.prof.begin(KID_MISC);
+          launch_capacity(pc.p, outc.p, k, n, phase, avail.p, tmode ? napp.p : nullptr,
+                          outc.p + k, fit.p, st());
+          C.prof.end(KID_MISC, 0);
           if (!pack)
             HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
         } else if (pending) {
@@ -1090,8 +1131,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         }
         if (pack) {
           launch_pass_status(run ? (const i64*)small.p : nullptr, pending ? cutacc.p : nullptr,
-                             pending ? ctr_prev : nullptr, pending ? pc.p : nullptr, k, pstat.p,
-                             st());
+                             pending ? ctr_prev : nullptr, pending ? pc.p : nullptr, k,
+                             run ? fit.p : nullptr, pstat.p, st());
           const size_t bytes = pending ? 24 + 4 * (size_t)k : 8;
           HIP_TRY(hipMemcpyAsync(C.pinned, pstat.p, bytes, hipMemcpyDeviceToHost, st()));
         } else if (pending) {
@@ -1133,24 +1174,34 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           if (phase && balanced) break;                    // loop test, gpu_part.cu:647
         }
         if (!run) break;
-        i64 M = std::min<i64>(C.pinned[0], n);             // gpu_part.cu:585
+        const i64 M = pack ? std::min<i64>(((const i32*)C.pinned)[0], n)
+                           : std::min<i64>(C.pinned[0], n);  // gpu_part.cu:585
+        const bool all_fit = pack && ((const i32*)C.pinned)[1] != 0;
         passes++;
         if (M <= 0) break;                                 // gpu_part.cu:638-640, 742-744
-        C.prof.begin(KID_MISC);
-        launch_capacity(pc.p, outc.p, k, n, phase, avail.p, tmode ? napp.p : nullptr, st());
-        C.prof.end(KID_MISC, 0);
         i64 len = M;                                       // gpu_part.cu:487-497
         i64 padded = 1;
         while (padded < len) padded <<= 1;
+        const bool full_network = padded <= n;
         if (padded > n) padded = n;
         if (padded > len) len = padded;
-        C.prof.begin(KID_SORT);
-        launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st());
-        C.prof.end(KID_SORT, 16.0 * (double)len);
-        C.prof.begin(KID_SELECT);
-        launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr_cur,
-                      ctr_cur + 1, st());
-        C.prof.end(KID_SELECT, 13.0 * M);
+        if (sort_skip && all_fit && full_network) {
+          // every move is selected whatever the network's order, and with a
+          // full network no pad lands among the first M (gains of buffered
+          // moves exceed the pads' -1e9)
+          C.prof.begin(KID_SELECT);
+          launch_select_all(M, sk_idx.p, sel.p, ctr_cur, ctr_cur + 1, st());
+          C.prof.end(KID_SELECT, 5.0 * M);
+          sorts_skipped++;
+        } else {
+          C.prof.begin(KID_SORT);
+          launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st());
+          C.prof.end(KID_SORT, 16.0 * (double)len);
+          C.prof.begin(KID_SELECT);
+          launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr_cur,
+                        ctr_cur + 1, st());
+          C.prof.end(KID_SELECT, 13.0 * M);
+        }
         C.prof.begin(KID_COMMIT);
         launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, part8.p, pc.p, k,
                       tmode ? app.p : nullptr, napp.p, st());
@@ -1231,12 +1282,12 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
     launch_token(g, nedges, gain.p, part.p, target.p, k, flag.p, ax.heavy.p, ax.nheavy,
                  HEAVY_DEG, TOKEN_LANES, st());
     launch_compact_moves(flag.p, n, part.p, target.p, gain.p, k, bcnt.p, boff.p, small.p,
-                         mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, st());
+                         mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, nullptr, st());
     HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
     C.sync();
     const i64 M = std::min<i64>(C.pinned[0], n);
     if (M < min_moves) break;
-    launch_capacity(pc.p, outc.p, k, n, 0, avail.p, nullptr, st());
+    launch_capacity(pc.p, outc.p, k, n, 0, avail.p, nullptr, nullptr, nullptr, st());
     i64 len = M, padded = 1;
     while (padded < len) padded <<= 1;
     if (padded > n) padded = n;
@@ -1321,6 +1372,7 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     S->coarsest_n = L[pass]->n;
     S->coarsest_npairs = npairs;
     S->match_rounds = match_rounds;
+    S->sorts_skipped = sorts_skipped;
     S->seconds_initpart = initpart_s;
     S->seconds_total =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -86,12 +86,28 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 // decided unmatched, else the partner): one launch decides vertices from
 // lo[0] upward until all are decided or `budget` wall-clock ticks pass, then
 // sets lo[0] to the first undecided vertex (n when done).  *ticket must be 0.
-// stats (optional, 10 zeroed u64): wave iterations, idle iterations, short
+// stats (optional, 12 zeroed u64): wave iterations, idle iterations, short
 // A/proposal scans, long A scans, long proposal scans, proposal-row fetches,
-// short C scans, long C scans, decisions, arcs of the long scans.
-void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
-                        u32* ticket, i64 budget, int blocks, int sg, int per, u64* stats,
-                        hipStream_t s);
+// short C scans, long C scans, decisions, entries of the long A, proposal and
+// C scans.
+// low (optional, off == nullptr for none): the lower-neighbour lists of the
+// long rows, which the long-row scans of the "no undecided neighbour below"
+// conditions read instead of the whole row.
+struct ChainLow {
+  const i64* off;  // n + 1: entries of v at adj[off[v], off[v+1]) (empty for short rows)
+  const i32* adj;  // the entries u < v of row v, in row order
+};
+void launch_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA, i32* blkC, i32* prop,
+                        const i64* lo, u32* ticket, i64 budget, int blocks, int sg, int per,
+                        u64* stats, hipStream_t s);
+// Lower-neighbour lists of the rows longer than minlen: list/nlist (n ints,
+// *nlist zeroed by the caller) receive those rows, cnt[v] their entry counts
+// (cnt must be zeroed by the caller), then, with off = exclusive scan of cnt,
+// launch_chain_low_write puts the entries into out.
+void launch_chain_low_count(DGraph g, i64 minlen, i32* list, i32* nlist, i64* cnt,
+                            hipStream_t s);
+void launch_chain_low_write(DGraph g, const i32* list, const i32* nlist, const i64* off, i32* out,
+                            hipStream_t s);
 // *out = the level's longest row (xadj[v+1] - xadj[v]); *out zeroed by the caller
 void launch_max_row(DGraph g, u64* out, hipStream_t s);
 // lo[0] = first vertex at or above lo[0] with st == -1 (n if none); lo[3]
@@ -217,10 +233,10 @@ void launch_sanitize_hist(i32* part, i64 n, i32 k, i32* pc, hipStream_t s);
 void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s);
 
 // ---- refinement ---------------------------------------------------------------
-// per-pass balance table bg[2][k] and src_big[k]; also zeroes zero_k[0..k)
-// and zero_2[0..1] when given
+// per-pass balance table bg[2][k] and src_big[k]; also zeroes
+// zero_k[0..nzero) and zero_2[0..1] and sets *one1 = 1 when given
 void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
-                     i32* zero_k, i32* zero_2, hipStream_t s);
+                     i32* zero_k, i32 nzero, i32* zero_2, i32* one1, hipStream_t s);
 // Heavy rows (> 64 arcs) split into medium (<= 4096) and huge lists.
 void launch_split_heavy(DGraph g, i64 nedges, const i32* heavy, i64 nheavy, i32* mid,
                         i32* nmid, i32* medium, i32* nmedium, i32* huge, i32* nhuge,
@@ -297,17 +313,22 @@ void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, cons
                   i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
                   int lanes, hipStream_t s);
 // Winners (flag[v] = 1) to the move arrays in ascending vertex order; *total
-// = their count, outc[src] += moves out of each part.  bcnt and boff hold
-// compaction_blocks() ints each.
+// = their count, outc[src] += moves out of each part and (when given)
+// inc[dst] += moves into each part.  bcnt and boff hold compaction_blocks()
+// ints each.
 void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* target,
                           const float* gain, i32 k, i32* bcnt, i32* boff, i64* total,
                           i32* mv_v, i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
-                          hipStream_t s);
+                          i32* inc, hipStream_t s);
 inline i64 compaction_blocks() { return 4096; }
 // avail[q] per part (gpu_part.cu:358-369); also zeroes *zero1 when given (the
-// applied-move count that launch_commit appends to)
+// applied-move count that launch_commit appends to); with inc, *fit (set to 1
+// beforehand) becomes 0 when some inc[q] > avail[q]
 void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i64* avail,
-                     i32* zero1, hipStream_t s);
+                     i32* zero1, const i32* inc, i32* fit, hipStream_t s);
+// every one of the M moves selected in buffer order: sk_idx[i] = i, sel[i] = 1,
+// *S = M, *pads = 0
+void launch_select_all(i64 M, i32* sk_idx, u8* sel, i32* S, i32* pads, hipStream_t s);
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32* sk_idx,
                        hipStream_t s);
 i64 select_scratch_elems(i64 M, i32 k);
@@ -325,10 +346,11 @@ constexpr int ACC_SLOTS = 256;
 constexpr int ACC_STRIDE = 16;
 constexpr i64 ACC_ELEMS = (i64)ACC_SLOTS * ACC_STRIDE;
 void launch_acc_finish(u64* acc, u64* out, hipStream_t s);
-// out[0] = *m, out[1] = *cut, ((i32*)out)[4..5] = ctr[0..1], ((i32*)out)[6..6+k)
-// = pc; a null input leaves zeros (out holds 3 + (k + 1) / 2 words)
+// ((i32*)out)[0] = *m, [1] = *fit, out[1] = *cut, ((i32*)out)[4..5] = ctr[0..1],
+// ((i32*)out)[6..6+k) = pc; a null input leaves zeros (out holds 3 + (k + 1) / 2
+// words)
 void launch_pass_status(const i64* m, const u64* cut, const i32* ctr, const i32* pc, i32 k,
-                        i64* out, hipStream_t s);
+                        const i32* fit, i64* out, hipStream_t s);
 // asrc[i] = source vertex of arc i (built once per level)
 void launch_arc_source(DGraph g, i32* asrc, hipStream_t s);
 // Edge cut over arcs [0, e_end) with e_end = max(xadj[n-1], nedges): exactly

@@ -318,7 +318,7 @@ constexpr i32 BLK_NONE = -2;
 
 // Row scans issue SCAN_U independent loads per lane before using any of them,
 // so a long row costs about len/(64*SCAN_U) memory round trips.
-constexpr int SCAN_U = 16;
+constexpr int SCAN_U = 32;
 
 // Device-scope (coherent) accesses to the matching state, for kernels in
 // which other waves update it while they run.
@@ -656,15 +656,16 @@ __device__ __forceinline__ i32 ld_st(const i32* st, i64 i) { return ld_mate(st, 
 // `from` stay decided and not equal to me: a later scan of the same question
 // (same x, lim, me) resumes there, and every long row is read about once
 // per question instead of once per blocker.
-__device__ __forceinline__ i32 wave_scan_pending(const DGraph& g, i32 x, i32 lim, i32 me,
-                                                 const i32* st, int lane, i32& from) {
-  const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
+// The entries scanned are adj[s0, s1): the row of x, or its lower-neighbour
+// list (ChainLow), which holds every entry of the row below x.
+__device__ __forceinline__ i32 wave_scan_pending(const i32* adj, i64 s0, i64 s1, i32 x, i32 lim,
+                                                 i32 me, const i32* st, int lane, i32& from) {
   for (i64 base = s0 + from; base < s1; base += WAVE * SCAN_U) {
     i32 v[SCAN_U];
 #pragma unroll
     for (int j = 0; j < SCAN_U; j++) {
       const i64 i = base + j * WAVE + lane;
-      v[j] = i < s1 ? g.adj[i] : INT_MAX;
+      v[j] = i < s1 ? adj[i] : INT_MAX;
     }
     i32 sv[SCAN_U];
 #pragma unroll
@@ -691,16 +692,22 @@ __device__ __forceinline__ i32 wave_scan_pending(const DGraph& g, i32 x, i32 lim
   return BLK_NONE;
 }
 
-// The 4 best free (st < 0) neighbours of t by (weight, first position), as
-// keys (w << 32 | ~position), best first, 0 where there are fewer.  Every other
-// free neighbour ranks below all four, and a taken vertex stays taken, so the
-// best free neighbour at any later time is the first still-free key here
-// (when one is left).
-constexpr int TU = 8;  // loads per lane per step (fewer registers than SCAN_U)
-__device__ __forceinline__ void wave_best_free_top4(const DGraph& g, i32 t, const i32* st,
-                                                    int lane, u64 (&out)[4]) {
+// The K best free (st < 0) neighbours of t by (weight, first position), as
+// keys (w << 32 | ~position), best first: lane sgbase + r receives the r-th
+// in `mykey` (0 where there are fewer), and `nvalid` (uniform) says how many
+// of them are exact.  Every other free neighbour ranks below the valid ones,
+// and a taken vertex stays taken, so the best free neighbour at any later
+// time is the first still-free valid key (when one is left).  Each lane keeps
+// its own 4 best; the wave merges them one key per round, and stops after a
+// round that empties a lane which saw more than 4 (its fifth might be next).
+constexpr int TU = 16;  // loads per lane per step (fewer registers than SCAN_U)
+template <int K>
+__device__ __forceinline__ void wave_best_free_topk(const DGraph& g, i32 t, const i32* st,
+                                                    int lane, int sgbase, u64& mykey,
+                                                    int& nvalid) {
   const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
   u64 k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+  int seen = 0;
   for (i64 base = s0; base < s1; base += WAVE * TU) {
     i32 v[TU], w[TU], sv[TU];
 #pragma unroll
@@ -715,6 +722,7 @@ __device__ __forceinline__ void wave_best_free_top4(const DGraph& g, i32 t, cons
 #pragma unroll
     for (int j = 0; j < TU; j++) {
       if (sv[j] < 0) {
+        seen++;
         const i64 i = base + j * WAVE + lane;
         u64 c = ((u64)(u32)w[j] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
         if (c > k3) {
@@ -726,16 +734,24 @@ __device__ __forceinline__ void wave_best_free_top4(const DGraph& g, i32 t, cons
       }
     }
   }
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
+  mykey = 0;
+  nvalid = 0;
+  int popped = 0;
+#pragma unroll 1
+  for (int r = 0; r < K; r++) {
     const u64 m = wave_max_u64(k0);
-    out[r] = m;
-    if (m != 0 && k0 == m) {  // positions are distinct: one lane holds m
+    if (m == 0) break;
+    if (lane == sgbase + r) mykey = m;
+    nvalid = r + 1;
+    const bool win = k0 == m;  // positions are distinct: one lane holds m
+    if (win) {
       k0 = k1;
       k1 = k2;
       k2 = k3;
       k3 = 0;
+      popped++;
     }
+    if (__ballot(win && popped == 4 && seen > 4)) break;
   }
 }
 
@@ -764,15 +780,15 @@ __device__ __forceinline__ bool sg_any(bool f) {
 // The chained matching of one wave: groups from `ticket` starting at vertex
 // lo, until the groups run out or the wall-clock budget is spent.
 template <int CH_SG, int CH_PER>
-__device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, i32* blkC,
-                                           i32* prop, const i64 lo, u32* ticket, i64 budget,
-                                           u64* stats) {
+__device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low, i32* st,
+                                           i32* blkA, i32* blkC, i32* prop, const i64 lo,
+                                           u32* ticket, i64 budget, u64* stats) {
   constexpr int CH_VPW = WAVE / CH_SG;
   constexpr int CH_ROW = CH_SG * CH_PER;
   constexpr u64 SG_MASK = CH_SG == 64 ? ~0ull : (1ull << (CH_SG % 64)) - 1;
   // stats (optional): per-kind event counts, summed over the grid
   u64 c_iter = 0, c_sleep = 0, c_apS = 0, c_aL = 0, c_pL = 0, c_fetch = 0, c_cS = 0, c_cL = 0,
-      c_commit = 0, c_arcsL = 0;
+      c_commit = 0, c_arcsL = 0, c_arcsP = 0, c_arcsC = 0;
   const i64 n = g.n;
   const int lane = lane_id();
   const int sl = lane & (CH_SG - 1);
@@ -784,10 +800,12 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
   // long rows: where the next A scan of t's row and C scan of u's row resume
   i32 apos = 0, cpos = 0;
   i32 t = -1, a = BLK_NONE, u = -1, b = BLK_NONE;
-  // long rows: lanes 0-3 of the sub-group hold the 4 best free candidates of
-  // the last full proposal scan (pci), ph = the entry now proposed (4: none)
+  // long rows: lanes 0..pnv-1 of the sub-group hold the best free candidates
+  // of the last full proposal scan (pci), ph = the entry now proposed (PK:
+  // none)
+  constexpr int PK = CH_SG < 16 ? CH_SG : 16;
   i32 pci = -1;
-  int ph = 4;
+  int ph = PK, pnv = 0;
   i64 ts0 = 0;
   i32 tv[CH_PER], tw[CH_PER], uv[CH_PER];
   for (u32 iter = 0; iter < (1u << 26); iter++) {
@@ -807,7 +825,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
       cpos = 0;
       apos = 0;
       pci = -1;
-      ph = 4;
+      ph = PK;
+      pnv = 0;
       ts0 = own ? g.xadj[t] : 0;
       const i64 ts1 = own ? g.xadj[t + 1] : 0;
       tshort = own && ts1 - ts0 <= CH_ROW;
@@ -901,9 +920,12 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
       i32 from = __shfl(apos, l);
+      // every entry below x is on x's lower-neighbour list when it has one
+      const i32* la = low.off ? low.adj : g.adj;
+      const i64 l0 = low.off ? low.off[x] : g.xadj[x], l1 = low.off ? low.off[x + 1] : g.xadj[x + 1];
       c_aL++;
-      c_arcsL += g.xadj[x + 1] - g.xadj[x] - from;
-      const i32 r = wave_scan_pending(g, x, x, x, st, lane, from);
+      c_arcsL += l1 - l0 - from;
+      const i32 r = wave_scan_pending(la, l0, l1, x, x, x, st, lane, from);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) own = false;
         else a = r;
@@ -914,8 +936,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
     // long rows: a taken proposal is replaced by the first still-free cached
     // candidate; only when none is left is the row scanned again
     {
-      const bool tryc = own && !tshort && u == BLK_UNKNOWN && ph < 3;
-      const bool mine = tryc && sl > ph && sl < 4 && pci >= 0;
+      const bool tryc = own && !tshort && u == BLK_UNKNOWN && ph < pnv - 1;
+      const bool mine = tryc && sl > ph && sl < pnv && pci >= 0;
       const i32 sv = mine ? ld_st(st, pci) : 0;
       const u64 fm = (__ballot(mine && sv < 0) >> (lane & ~(CH_SG - 1))) & SG_MASK;
       if (tryc) {
@@ -928,7 +950,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
           cpos = 0;
           ph = r;
         } else {
-          ph = 4;
+          ph = PK;
         }
       }
     }
@@ -937,17 +959,18 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
       c_pL++;
-      c_arcsL += g.xadj[x + 1] - g.xadj[x];
-      u64 keys[4];
-      wave_best_free_top4(g, x, st, lane, keys);
+      c_arcsP += g.xadj[x + 1] - g.xadj[x];
+      u64 kk;
+      int nv;
+      wave_best_free_topk<PK>(g, x, st, lane, l, kk, nv);
+      const u64 key0 = __shfl(kk, l);
       const i64 xs0 = g.xadj[x];
-      const i32 r0 = keys[0] ? g.adj[xs0 + (i64)(0xFFFFFFFFu - (u32)keys[0])] : -1;
+      const i32 r0 = key0 ? g.adj[xs0 + (i64)(0xFFFFFFFFu - (u32)key0)] : -1;
       const i32 sr = r0 >= 0 ? ld_st(st, r0) : ST_UNDEC;
       if ((lane & ~(CH_SG - 1)) == l) {
-        const int r = lane & (CH_SG - 1);
-        const u64 kk = r == 0 ? keys[0] : r == 1 ? keys[1] : r == 2 ? keys[2] : r == 3 ? keys[3] : 0;
         pci = kk ? g.adj[xs0 + (i64)(0xFFFFFFFFu - (u32)kk)] : -1;
         ph = 0;
+        pnv = nv;
         u = r0 >= 0 ? r0 : PROP_NONE;
         s_u = sr;
         b = BLK_UNKNOWN;
@@ -1015,9 +1038,13 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
       const i32 x = __shfl(u, l);
       const i32 lim = __shfl(t, l);
       i32 from = __shfl(cpos, l);
+      // entries of u's row below t lie on u's lower-neighbour list when t <= u
+      const bool uselow = low.off && lim <= x;
+      const i32* la = uselow ? low.adj : g.adj;
+      const i64 l0 = uselow ? low.off[x] : g.xadj[x], l1 = uselow ? low.off[x + 1] : g.xadj[x + 1];
       c_cL++;
-      c_arcsL += g.xadj[x + 1] - g.xadj[x] - from;
-      const i32 r = wave_scan_pending(g, x, lim, x, st, lane, from);
+      c_arcsC += l1 - l0 - from;
+      const i32 r = wave_scan_pending(la, l0, l1, x, lim, x, st, lane, from);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) {
           u = BLK_UNKNOWN;
@@ -1051,17 +1078,59 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, i32* st, i32* blkA, 
     blkC[t] = b;
   }
   if (stats && lane == 0) {
-    const u64 c[10] = {c_iter, c_sleep, c_apS, c_aL, c_pL, c_fetch, c_cS, c_cL, c_commit,
-                       c_arcsL};
-    for (int q = 0; q < 10; q++) atomicAdd((unsigned long long*)&stats[q], (unsigned long long)c[q]);
+    const u64 c[12] = {c_iter, c_sleep, c_apS, c_aL, c_pL, c_fetch, c_cS, c_cL, c_commit,
+                       c_arcsL, c_arcsP, c_arcsC};
+    for (int q = 0; q < 12; q++) atomicAdd((unsigned long long*)&stats[q], (unsigned long long)c[q]);
   }
 }
 
 template <int CH_SG, int CH_PER>
-__global__ __launch_bounds__(256) void k_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC,
-                                                     i32* prop, const i64* lo_io, u32* ticket,
-                                                     i64 budget, u64* stats) {
-  chain_wave<CH_SG, CH_PER>(g, st, blkA, blkC, prop, lo_io[0], ticket, budget, stats);
+__global__ __launch_bounds__(256) void k_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA,
+                                                     i32* blkC, i32* prop, const i64* lo_io,
+                                                     u32* ticket, i64 budget, u64* stats) {
+  chain_wave<CH_SG, CH_PER>(g, low, st, blkA, blkC, prop, lo_io[0], ticket, budget, stats);
+}
+
+// Lower-neighbour lists of the rows longer than `minlen` (ChainLow).
+// k_long_rows lists those rows (in no particular order); then, one wave per
+// listed row with 16 loads per lane in flight, cnt[v] = number of entries
+// u < v of row v, and, after an exclusive scan of cnt into off, the entries
+// themselves in row order.
+__global__ __launch_bounds__(256) void k_long_rows(DGraph g, i64 minlen, i32* list, i32* nlist) {
+  // whole blocks iterate together, so every lane takes part in the ballot
+  for (i64 b = (i64)blockIdx.x * blockDim.x; b < g.n; b += gstride()) {
+    const i64 v = b + threadIdx.x;
+    const bool lng = v < g.n && g.xadj[v + 1] - g.xadj[v] > minlen;
+    wave_append(lng, (i32)v, list, nlist);
+  }
+}
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_chain_low(DGraph g, const i32* list, const i32* nlist,
+                                                   i64* cnt, const i64* off, i32* out) {
+  constexpr int U = 16;
+  const int lane = lane_id();
+  const i64 nl = *nlist;
+  for (i64 e = gwave(); e < nl; e += nwaves()) {
+    const i32 v = list[e];
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    i64 pos = WRITE ? off[v] : 0;
+    for (i64 b = s0; b < s1; b += WAVE * U) {
+      i32 u[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const i64 i = b + j * WAVE + lane;
+        u[j] = i < s1 ? g.adj[i] : INT_MAX;
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const bool f = u[j] < v;
+        const u64 bm = __ballot(f);
+        if (WRITE && f) out[pos + __popcll(bm & lanes_below(lane))] = u[j];
+        pos += __popcll(bm);
+      }
+    }
+    if (!WRITE && lane == 0) cnt[v] = pos;
+  }
 }
 
 // st -> mate (and dec for the round kernels)
@@ -1725,12 +1794,14 @@ __global__ void k_block_init(i32* part, i64 n, i32 k) {
 // the source part is oversized), so it is tabulated per pass with the
 // reference's operation order: b = (c*(T-d))/T; [+A if src big]; [+B if dst small].
 __global__ void k_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
-                           i32* zero_k, i32* zero_2) {
+                           i32* zero_k, i32 nzero, i32* zero_2, i32* one1) {
   const i64 T = n / k;
   const float Tf = (float)T;
   if (zero_2 && gtid() < 2) zero_2[gtid()] = 0;
+  if (one1 && gtid() == 0) *one1 = 1;
+  if (zero_k)
+    for (i64 q = gtid(); q < nzero; q += gstride()) zero_k[q] = 0;
   for (i64 q = gtid(); q < k; q += gstride()) {
-    if (zero_k) zero_k[q] = 0;
     const float d = (float)pc[q];
     const float base = phase ? 500.0f * (Tf - d) / Tf : 200.0f * (Tf - d) / Tf;
     const float A = phase ? 250.0f : 100.0f;
@@ -2601,10 +2672,13 @@ __global__ __launch_bounds__(CMP_BS) void k_scatter_moves(const u8* flag, i64 n,
                                                          const i32* target, const float* gain,
                                                          i32 k, i32* mv_v, i32* mv_src,
                                                          i32* mv_dst, float* mv_gain,
-                                                         i32* outc) {
-  __shared__ i32 h[MAXK_LDS];
+                                                         i32* outc, i32* inc) {
+  __shared__ i32 h[MAXK_LDS], h2[MAXK_LDS];
   __shared__ i32 sw[CMP_BS / WAVE];
-  for (int q = threadIdx.x; q < k && q < MAXK_LDS; q += CMP_BS) h[q] = 0;
+  for (int q = threadIdx.x; q < k && q < MAXK_LDS; q += CMP_BS) {
+    h[q] = 0;
+    h2[q] = 0;
+  }
   __syncthreads();
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const i64 b0 = (i64)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
@@ -2623,24 +2697,35 @@ __global__ __launch_bounds__(CMP_BS) void k_scatter_moves(const u8* flag, i64 n,
     if (f) {
       const i64 p = run + before + __popcll(bm & lanes_below(lane));
       const i32 src = part[v];
+      const i32 dst = target[v];
       mv_v[p] = (i32)v;
       mv_src[p] = src;
-      mv_dst[p] = target[v];
+      mv_dst[p] = dst;
       mv_gain[p] = gain[v];
-      if (k <= MAXK_LDS) atomicAdd(&h[src], 1);
-      else atomicAdd(&outc[src], 1);
+      if (k <= MAXK_LDS) {
+        atomicAdd(&h[src], 1);
+        if (inc) atomicAdd(&h2[dst], 1);
+      } else {
+        atomicAdd(&outc[src], 1);
+        if (inc) atomicAdd(&inc[dst], 1);
+      }
     }
     run += tot;
     __syncthreads();
   }
   if (k <= MAXK_LDS)
-    for (int q = threadIdx.x; q < k; q += CMP_BS)
+    for (int q = threadIdx.x; q < k; q += CMP_BS) {
       if (h[q]) atomicAdd(&outc[q], h[q]);
+      if (inc && h2[q]) atomicAdd(&inc[q], h2[q]);
+    }
 }
 
 // gpu_part.cu:357-369.
+// With `inc` (moves per destination), *fit (set to 1 by k_bg_table) becomes 0
+// when some destination has more moves than capacity: while it stays 1 every
+// buffered move is selected whatever their order.
 __global__ void k_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase,
-                           i64* avail, i32* zero1) {
+                           i64* avail, i32* zero1, const i32* inc, i32* fit) {
   if (zero1 && gtid() == 0) *zero1 = 0;
   const i64 T = n / k;
   const float Tf = (float)T;
@@ -2656,6 +2741,21 @@ __global__ void k_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int pha
     if (a > maxSize - current + slack) a = maxSize - current + slack;
     if (current <= minSize && !phase) a = a < slack / 2 ? a : slack / 2;
     avail[q] = a;
+    if (inc && (i64)inc[q] > a) *fit = 0;
+  }
+}
+
+// Selection when every move fits its destination (the capacity kernel's
+// flag) and the network would have sorted all M moves ahead of the pads:
+// all are selected, in buffer order (commit order does not change the result).
+__global__ void k_select_all(i64 M, i32* sk_idx, u8* sel, i32* S, i32* pads) {
+  for (i64 i = gtid(); i < M; i += gstride()) {
+    sk_idx[i] = (i32)i;
+    sel[i] = 1;
+  }
+  if (gtid() == 0) {
+    *S = (i32)M;
+    *pads = 0;
   }
 }
 
@@ -3126,12 +3226,14 @@ __global__ __launch_bounds__(256) void k_cut_arcs(const i32* asrc, const i32* ad
 // One word block with a pass's host-side inputs, copied to the host at once:
 // out[0] = *m, out[1] = *cut, out32[4..5] = ctr[0..1], out32[6..6+k) = pc
 // (absent inputs leave zeros).
+// out: [M (i32), fit (i32)], cut, S, pads, pc[k]
 __global__ __launch_bounds__(256) void k_pass_status(const i64* m, const u64* cut,
                                                      const i32* ctr, const i32* pc, i32 k,
-                                                     i64* out) {
+                                                     const i32* fit, i64* out) {
   i32* o32 = (i32*)out;
   if (threadIdx.x == 0) {
-    out[0] = m ? *m : 0;
+    o32[0] = m ? (i32)*m : 0;
+    o32[1] = fit ? *fit : 0;
     out[1] = cut ? (i64)*cut : 0;
     o32[4] = ctr ? ctr[0] : 0;
     o32[5] = ctr ? ctr[1] : 0;
@@ -3229,9 +3331,9 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
   k_fill<i64><<<1, 1, 0, s>>>(lo + 3, 1, n);  // first-undecided accumulator
 }
 
-void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, const i64* lo,
-                        u32* ticket, i64 budget, int blocks, int sg, int per, u64* stats,
-                        hipStream_t s) {
+void launch_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA, i32* blkC, i32* prop,
+                        const i64* lo, u32* ticket, i64 budget, int blocks, int sg, int per,
+                        u64* stats, hipStream_t s) {
   // sub-group width (lanes per vertex) and register entries per lane: rows
   // of up to sg * per arcs are held in registers
   // blocks <= 0: as many as the device holds at once (every wave then takes
@@ -3249,15 +3351,26 @@ void launch_match_chain(DGraph g, i32* st, i32* blkA, i32* blkC, i32* prop, cons
 #define TGP_CHAIN_CASE(SG, PER)                                                               \
   if (sg == SG && per == PER) {                                                               \
     const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<SG, PER>);       \
-    k_match_chain<SG, PER><<<nb, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget,     \
-                                              stats);                                         \
+    k_match_chain<SG, PER><<<nb, 256, 0, s>>>(g, low, st, blkA, blkC, prop, lo, ticket,       \
+                                              budget, stats);                                 \
     return;                                                                                   \
   }
   TGP_CHAIN_CASE(16, 8)
   TGP_CHAIN_CASE(32, 8)
 #undef TGP_CHAIN_CASE
   const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<8, 8>);
-  k_match_chain<8, 8><<<nb, 256, 0, s>>>(g, st, blkA, blkC, prop, lo, ticket, budget, stats);
+  k_match_chain<8, 8><<<nb, 256, 0, s>>>(g, low, st, blkA, blkC, prop, lo, ticket, budget,
+                                         stats);
+}
+
+void launch_chain_low_count(DGraph g, i64 minlen, i32* list, i32* nlist, i64* cnt,
+                            hipStream_t s) {
+  k_long_rows<<<grid_for(g.n, 256, 1u << 14), 256, 0, s>>>(g, minlen, list, nlist);
+  k_chain_low<false><<<8192, 256, 0, s>>>(g, list, nlist, cnt, nullptr, nullptr);
+}
+void launch_chain_low_write(DGraph g, const i32* list, const i32* nlist, const i64* off, i32* out,
+                            hipStream_t s) {
+  k_chain_low<true><<<8192, 256, 0, s>>>(g, list, nlist, nullptr, off, out);
 }
 
 void launch_max_row(DGraph g, u64* out, hipStream_t s) {
@@ -3377,8 +3490,9 @@ void launch_block_init(i32* part, i64 n, i32 k, hipStream_t s) {
 }
 
 void launch_bg_table(const i32* pc, i32 k, i64 n, int phase, float* bg, u8* src_big,
-                     i32* zero_k, i32* zero_2, hipStream_t s) {
-  k_bg_table<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, k, n, phase, bg, src_big, zero_k, zero_2);
+                     i32* zero_k, i32 nzero, i32* zero_2, i32* one1, hipStream_t s) {
+  k_bg_table<<<grid_for(std::max<i64>(k, nzero), 256, 64), 256, 0, s>>>(
+      pc, k, n, phase, bg, src_big, zero_k, nzero, zero_2, one1);
 }
 
 i64 gain_gscratch_elems(i32 k) { return k <= MAXK_LDS ? 0 : (i64)65536 * k; }
@@ -3507,7 +3621,7 @@ void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, cons
 void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* target,
                           const float* gain, i32 k, i32* bcnt, i32* boff, i64* total,
                           i32* mv_v, i32* mv_src, i32* mv_dst, float* mv_gain, i32* outc,
-                          hipStream_t s) {
+                          i32* inc, hipStream_t s) {
   i64 nb = (n + 4095) / 4096;
   if (nb < 1) nb = 1;
   if (nb > CMP_MAXB) nb = CMP_MAXB;
@@ -3518,12 +3632,16 @@ void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* tar
   k_flag_count<<<(unsigned)nb, CMP_BS, 0, s>>>(flag, n, per, bcnt);
   k_flag_offsets<<<1, 1024, 0, s>>>(bcnt, (i32)nb, boff, total);
   k_scatter_moves<<<(unsigned)nb, CMP_BS, 0, s>>>(flag, n, per, boff, part, target, gain, k,
-                                                   mv_v, mv_src, mv_dst, mv_gain, outc);
+                                                   mv_v, mv_src, mv_dst, mv_gain, outc, inc);
 }
 
 void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i64* avail,
-                     i32* zero1, hipStream_t s) {
-  k_capacity<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, outc, k, n, phase, avail, zero1);
+                     i32* zero1, const i32* inc, i32* fit, hipStream_t s) {
+  k_capacity<<<grid_for(k, 256, 64), 256, 0, s>>>(pc, outc, k, n, phase, avail, zero1, inc, fit);
+}
+
+void launch_select_all(i64 M, i32* sk_idx, u8* sel, i32* S, i32* pads, hipStream_t s) {
+  k_select_all<<<grid_for(M, 256 * 4, 1u << 14), 256, 0, s>>>(M, sk_idx, sel, S, pads);
 }
 
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* idx,
@@ -3619,8 +3737,8 @@ void launch_arc_source(DGraph g, i32* asrc, hipStream_t s) {
 }
 
 void launch_pass_status(const i64* m, const u64* cut, const i32* ctr, const i32* pc, i32 k,
-                        i64* out, hipStream_t s) {
-  k_pass_status<<<1, 256, 0, s>>>(m, cut, ctr, pc, k, out);
+                        const i32* fit, i64* out, hipStream_t s) {
+  k_pass_status<<<1, 256, 0, s>>>(m, cut, ctr, pc, k, fit, out);
 }
 void launch_acc_finish(u64* acc, u64* out, hipStream_t s) {
   k_acc_finish<<<1, ACC_SLOTS, 0, s>>>(acc, out);
