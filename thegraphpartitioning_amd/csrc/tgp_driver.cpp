@@ -1123,9 +1123,18 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
             HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));
         } else if (pending) {
           C.prof.begin(KID_CUT);
-          launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
-          launch_acc_finish(accs.p, cutacc.p, st());
-          C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
+          if (tmode && T_valid) {
+            // the tallies are current after the last commit (the last row's
+            // partial range is rebuilt as the gain pass does)
+            if (last_partial) launch_tally_last(g, nedges, part.p, T.p, k, unit, st());
+            launch_tally_cut(n, part.p, T.p, k, accs.p, st());
+            launch_acc_finish(accs.p, cutacc.p, st());
+            C.prof.end(KID_CUT, 4.0 * tally_row_stride(k) * n + 4.0 * n);
+          } else {
+            launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
+            launch_acc_finish(accs.p, cutacc.p, st());
+            C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
+          }
         } else {
           break;
         }
@@ -1219,7 +1228,10 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       if (phase == 0) *cut = bestCut;
     }
   }
-  *cut = compute_cut(lv, nedges, part.p);  // gpu_part.cu:749
+  // gpu_part.cu:749 recomputes the cut of the restored best partition: that
+  // partition's cut is bestCut (the cut taken when it was recorded, or the
+  // entry cut, with the same range rules)
+  *cut = bestCut;
   passes_total += passes;
   tr.push(TGP_TRACE_REFINE, level, 0, passes, n, nedges, *cut, 0,
           O.check_level ? hash_dev(part.p, n) : 0);
@@ -1360,7 +1372,9 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     cut = compute_cut(*L[level], ecount[level], part.p);
     refine(level, ecount[level], part, pc, &cut);
   }
-  const i64 fcut = compute_cut(*L[0], m, part.p);  // gpu_part.cu:1285
+  // gpu_part.cu:1285: the level-0 refinement ended with this partition's cut
+  // over the same ranges unless level 0 is the coarsest level (B5 ranges there)
+  const i64 fcut = pass > 0 ? cut : compute_cut(*L[0], m, part.p);
   HIP_TRY(hipMemcpyAsync(part_out, part.p, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
   HIP_TRY(hipMemcpyAsync(pc_out, pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();

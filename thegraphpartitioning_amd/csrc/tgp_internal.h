@@ -299,6 +299,9 @@ void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* nap
                          const i32* mv_v, const i32* mv_src, const i32* mv_dst, int unit,
                          int skip_last, hipStream_t s);
 // T[n-1] from its range [xadj[n-1], nedges)
+// Edge cut of part from kept tallies (sum over v of T[v][q] for q != part[v]),
+// added into the accumulator slots.
+void launch_tally_cut(i64 n, const i32* part, const i32* T, i32 k, u64* cut_acc, hipStream_t s);
 void launch_tally_last(DGraph g, i64 nedges, const i32* part, i32* T, i32 k, int unit,
                        hipStream_t s);
 // part8[v] = part[v] if 0 <= part[v] < k, else 255

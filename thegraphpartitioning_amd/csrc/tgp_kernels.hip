@@ -2381,6 +2381,36 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, co
   if (cut_acc) block_add_u64(cutv, cut_acc);
 }
 
+// Edge cut from kept tallies (gpu_part.cu:400-422 on a level whose tallies
+// T[v][q] are the weights of v's arcs, over its row range, into part q):
+// sum over v with part[v] in [0, k) of sum_{q != part[v]} T[v][q], into the
+// accumulator slots.  Four lanes per vertex, 16-byte loads.
+__global__ __launch_bounds__(256) void k_tally_cut(i64 n, const i32* part, const i32* T, i32 k,
+                                                   u64* cut_acc) {
+  const int gl = threadIdx.x & 3;
+  const i64 ngroups = (i64)gridDim.x * (blockDim.x / 4);
+  const i64 ts = tally_stride(k);
+  const int nv4 = (int)(ts >> 2);
+  u64 cutv = 0;
+  for (i64 v = (i64)blockIdx.x * (blockDim.x / 4) + threadIdx.x / 4; v < n; v += ngroups) {
+    const i32 cur = part[v];
+    if (cur >= k || cur < 0) continue;
+    const int4* T4 = (const int4*)(T + v * ts);
+    i64 other = 0;
+    for (int c = gl; c < nv4; c += 4) {
+      const int4 t = T4[c];
+      const i32 tv[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const i32 q = 4 * c + j;
+        if (q < k && q != cur) other += tv[j];
+      }
+    }
+    cutv += (u64)other;
+  }
+  block_add_u64(cutv, cut_acc);
+}
+
 // After a commit: every applied move (v: src -> dst) shifts the weight of each
 // of v's arcs (v, u) from T[u][src] to T[u][dst]; the reverse arc (u, v) has
 // the same weight on these levels.  T[n-1] is skipped when the last row's
@@ -3722,6 +3752,9 @@ void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* nap
                          int skip_last, hipStream_t s) {
   k_tally_update<<<grid_for(max_app, 4, 4096), 256, 0, s>>>(g, T, k, app, napp, mv_v, mv_src,
                                                             mv_dst, unit, skip_last);
+}
+void launch_tally_cut(i64 n, const i32* part, const i32* T, i32 k, u64* cut_acc, hipStream_t s) {
+  k_tally_cut<<<grid_for(n, 64, 4096), 256, 0, s>>>(n, part, T, k, cut_acc);
 }
 void launch_tally_last(DGraph g, i64 nedges, const i32* part, i32* T, i32 k, int unit,
                        hipStream_t s) {
