@@ -1,18 +1,19 @@
 """Benchmark: end-to-end partitioning throughput on one MI355X.
 
 Workload (one step = one full multilevel partition of the graph, the CSR
-already resident in HBM): seeded R-MAT scale 22, edge factor 16, symmetrized
-and deduplicated, isolated vertices removed (~2.4M vertices, ~128M arcs),
-unit weights, int32 CSR, k = 64.  The reference's exact coarsening cannot
-finish this graph (it stalls at ~663K vertices, one merge per level, and ends
-in its "Invalid new_coarsed_edge" failure), so the step runs the opt-in
-repaired coarsening; matching, refinement, move selection and cut are the
+already resident in HBM): seeded R-MAT scale 24, edge factor 16, symmetrized
+and deduplicated, isolated vertices removed (8,871,471 vertices, 520,761,282
+arcs), unit weights, int32 CSR, k = 64.  The reference's exact coarsening
+cannot finish R-MAT graphs (it stalls, one merge per level, and ends in its
+"Invalid new_coarsed_edge" failure), so the step runs the opt-in repaired
+coarsening; matching, refinement, move selection and cut are the
 reference's.  The initial partition of the coarsest graph is the
 deterministic block rule (no METIS on the machine).
 
 Prints one JSON line: M edges/s (level-0 arcs x steps / seconds), the roofline
 of the dominant kernel (the chained matching kernel, k_match_chain) from
-HIP-event timing of that kernel alone, and the oracle's CPU throughput on a
+HIP-event timing of that kernel alone, the whole step's algorithmic bytes by
+the per-kernel model over the step time, and the oracle's CPU throughput on a
 bounded sample of the same workload family.  `roofline.traffic` (HBM bytes per
 launch of that kernel) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
 committed under profiles/, scaled by the same counters' reading for a plain
@@ -32,7 +33,7 @@ SEED = 0x7C0DE5EED + 1
 HBM_PEAK_GBS = 8000.0
 ROOFLINE_FAMILY = "match_chain"          # HIP-event bracket around k_match_chain only
 ROOFLINE_KERNEL = "k_match_chain"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_match_chain_rmat22_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_match_chain_rmat24_traffic.json")
 
 
 def measured_traffic():
@@ -60,18 +61,95 @@ def copy_probe(mib):
                       "bytes_written_per_copy": n * 4}), flush=True)
 
 
+def host_cpu():
+    """The host's logical CPU count and model name (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def gpmetis_baseline(g, k):
+    """The reference's CPU path, gpmetis k-way with METIS defaults (ufactor 30 =
+    3 %), on the METIS text of g: median wall time of 3 runs, or None when no
+    gpmetis is installed (nothing else is substituted for it)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import thegraphpartitioning_amd as tgp
+    exe = shutil.which("gpmetis")
+    if not exe:
+        return None
+    text, _ = tgp.metis_text(g)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "graph.metis")
+        with open(path, "wb") as f:
+            f.write(text)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            subprocess.run([exe, path, str(k)], check=True, stdout=subprocess.DEVNULL)
+            times.append(time.perf_counter() - t0)
+    return sorted(times)[1]
+
+
 def cpu_baseline(scale, ef, k):
-    """Oracle (single thread) end-to-end on R-MAT(scale, ef), repaired mode."""
+    """The oracle end-to-end on R-MAT(scale, ef), repaired mode, timed with one
+    thread and with every host thread (its order-independent loops use
+    OpenMP; the matching, move buffering and selection stay sequential), plus
+    gpmetis when installed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
     import thegraphpartitioning_amd as tgp
+    nproc, model = host_cpu()
     g = tgp.gen_rmat(scale, ef, SEED)
-    t0 = time.perf_counter()
-    oracle_ffi.partition(g.xadj, g.adjncy, g.adjwgt, k, repair=1)
-    dt = time.perf_counter() - t0
-    return {"value": round(g.m / dt / 1e6, 4), "unit": "M edges/s", "cores": 1, "kind": "port",
+    rates = {}
+    for threads in (1, nproc):
+        oracle_ffi.set_threads(threads)
+        t0 = time.perf_counter()
+        oracle_ffi.partition(g.xadj, g.adjncy, g.adjwgt, k, repair=1)
+        rates[threads] = (time.perf_counter() - t0, g.m / (time.perf_counter() - t0) / 1e6)
+    gm = gpmetis_baseline(g, k)
+    return {"value": round(rates[nproc][1], 4), "unit": "M edges/s", "cores": nproc,
+            "kind": "port",
             "sample": f"oracle end-to-end on R-MAT scale {scale} ef {ef} k={k} "
-                      f"({g.n} vertices, {g.m} arcs) in {dt:.1f} s, same repaired semantics"}
+                      f"({g.n} vertices, {g.m} arcs), same repaired semantics: "
+                      f"{rates[nproc][0]:.1f} s with {nproc} threads, "
+                      f"{rates[1][0]:.1f} s with 1",
+            "value_1_thread": round(rates[1][1], 4), "nproc": nproc, "cpu_model": model,
+            "gpmetis": ("unavailable (not installed on this host)" if gm is None else
+                        {"value": round(g.m / gm / 1e6, 4), "unit": "M edges/s", "cores": 1,
+                         "median_seconds": round(gm, 3)})}
+
+
+def model_bytes(trace, o=8):
+    """Algorithmic bytes of one partition by the per-kernel model (offsets of o
+    bytes, ids and weights of 4): per level built, matching o*n + 12m + 4n,
+    relabel 8n + 8n_c, contraction 3o*n_c + 12m + 8m_c, projection 12n_c + 8n;
+    per refinement pass 33m + (3o + 24)n + 77M + 4S (gains, token passing,
+    compaction, sort, selection and commit, edge cut), with m the level's arcs
+    and M, S the pass's buffered and selected moves."""
+    total = 0.0
+    levels = {}
+    for t in trace:
+        kind, level = t[0], t[1]
+        if kind == 1:                                   # LEVEL: n, m, n_c, m_c
+            n, m, nc, mc = t[4], t[5], t[6], t[7]
+            total += o * n + 12 * m + 4 * n + 8 * n + 8 * nc + 3 * o * nc + 12 * m + 8 * mc
+            total += 12 * nc + 8 * n
+            levels[level] = (n, m)
+            levels[level + 1] = (nc, mc)
+    for t in trace:
+        if t[0] == 3:                                   # PASS: M, S
+            n, m = levels.get(t[1], (0, 0))
+            total += 33 * m + (3 * o + 24) * n + 77 * t[4] + 4 * t[5]
+    return total
 
 
 def batched_workload(count, seed):
@@ -320,7 +398,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -410,7 +488,7 @@ def main():
     pcie_value = g.m / (time.perf_counter() - t2) / 1e6
 
     # one extra step with per-kernel HIP-event timing (not part of the timed region)
-    rp = tgp.partition(g, args.k, profile=True, **kw)
+    rp = tgp.partition(g, args.k, profile=True, trace_cap=1 << 14, **kw)
     stp = rp[3]
     ms = stp.kernel_ms[ROOFLINE_FAMILY]
     byts = stp.kernel_bytes[ROOFLINE_FAMILY]
@@ -430,15 +508,22 @@ def main():
         "kernel_ms_total": round(total_kernel_ms, 3),
         "profiled_step_s": round(stp.seconds_total, 4),
         "match_launches": stp.match_rounds,
+        "sorts_skipped": stp.sorts_skipped,
         "pcie_inclusive_value": round(pcie_value, 3),
     }
     traffic = measured_traffic()
+    # the whole step against the same peak: the per-kernel model's bytes of
+    # every level and pass this partition ran, over the timed step
+    e2e_bytes = model_bytes(stp.trace)
+    e2e_gbs = e2e_bytes / (elapsed / args.steps) / 1e9
     roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": traffic, "launches": launches,
                 "avg_launch_ms": round(ms / max(launches, 1), 4),
                 "bytes_per_launch": round(byts / max(launches, 1)),
-                "kernel_share_of_step": round(ms / max(total_kernel_ms, 1e-9), 3)}
+                "kernel_share_of_step": round(ms / max(total_kernel_ms, 1e-9), 3),
+                "e2e_model_bytes": round(e2e_bytes), "e2e_achieved": round(e2e_gbs, 1),
+                "e2e_frac": round(e2e_gbs / HBM_PEAK_GBS, 4)}
     dg.free()
 
     cpu = None
@@ -461,7 +546,9 @@ def main():
         "config": {"workload": f"rmat{args.scale}-ef{args.edge_factor}-k{args.k}",
                    "vertices": g.n, "arcs": g.m, "k": args.k,
                    "coarsening": "repaired (reference coarsening stalls on this graph)",
-                   "initial_partition": "block i*k/n_c", "parallelism": "single GPU"},
+                   "initial_partition": "block i*k/n_c (no METIS call)",
+                   "input": "CSR resident in HBM when the timed region starts",
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": detail,

@@ -31,6 +31,9 @@
 #include <sstream>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -269,6 +272,7 @@ std::string metis_text_seq(const View& g, int64_t* npairs) {
 int64_t cut_seq(const View& g, int64_t nedges, const int32_t* part, int32_t k, int wide) {
   const int64_t n = g.n;
   uint64_t s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(dynamic, 1024)
   for (int64_t v = 0; v < n; v++) {
     const int64_t st = g.xadj[v];
     const int64_t en = row_end_param(g, v, nedges);
@@ -296,18 +300,23 @@ void gains_seq(const View& g, int64_t nedges, const int32_t* part, const int32_t
   const int64_t n = g.n;
   const int64_t T = n / k;
   const float Tf = (float)T;
-  counts.assign((size_t)k, 0.0f);
+  (void)counts;
+  // vertices are independent (each writes only its own gain and target)
+#pragma omp parallel
+  {
+  std::vector<float> cnt((size_t)k, 0.0f);
+#pragma omp for schedule(dynamic, 256)
   for (int64_t v = 0; v < n; v++) {
     const int32_t cur = part[v];
     if (cur >= k || cur < 0) continue;                      // 262
-    std::fill(counts.begin(), counts.end(), 0.0f);
+    std::fill(cnt.begin(), cnt.end(), 0.0f);
     const int64_t st = g.xadj[v];
     const int64_t en = row_end_param(g, v, nedges);
     for (int64_t i = st; i < en; i++) {
       const int32_t u = g.adj[i];
       if (u < n) {
         const int32_t p = part[u];
-        if (p < k && p >= 0) counts[p] += (float)g.w[i];
+        if (p < k && p >= 0) cnt[p] += (float)g.w[i];
       }
     }
     float best = -1e9f;
@@ -315,7 +324,7 @@ void gains_seq(const View& g, int64_t nedges, const int32_t* part, const int32_t
     const float srcSize = (float)pc[cur];
     for (int32_t q = 0; q < k; q++) {
       if (q == cur) continue;
-      const float edgeGain = counts[q] - counts[cur];
+      const float edgeGain = cnt[q] - cnt[cur];
       const float destSize = (float)pc[q];
       float bg = phase ? 500.0f * (Tf - destSize) / Tf : 200.0f * (Tf - destSize) / Tf;
       if (srcSize > Tf * 1.05f) bg += phase ? 250.0f : 100.0f;
@@ -329,6 +338,7 @@ void gains_seq(const View& g, int64_t nedges, const int32_t* part, const int32_t
     gain[v] = best;
     target[v] = tgt;
   }
+  }
 }
 
 // gpu_part.cu:472-484 executed for the stage sequence of 487-506.
@@ -336,6 +346,8 @@ template <class E, class Key>
 void bitonic_seq(E* d, int64_t len, Key key) {
   for (int64_t kk = 2; kk <= len; kk <<= 1) {
     for (int64_t j = kk >> 1; j > 0; j >>= 1) {
+      // the pairs of one stage are disjoint
+#pragma omp parallel for schedule(static) if (len >= 65536)
       for (int64_t idx = 0; idx < len; idx++) {
         const int64_t ixj = idx ^ j;
         if (ixj > idx || ixj >= len) continue;
@@ -375,8 +387,10 @@ void refine_seq(const View& g, int64_t nedges, int32_t k, int wide, int32_t* par
       std::fill(outc.begin(), outc.end(), 0);
       std::fill(inc.begin(), inc.end(), 0);
       gains_seq(g, nedges, part, pc, k, phase, gain.data(), target.data(), counts);
+#pragma omp parallel for schedule(static)
       for (int64_t v = 0; v < n; v++)                       // markCandidates 308-312
         cand[v] = phase ? (gain[v] > -50.0f) : (gain[v] > -10.0f);
+#pragma omp parallel for schedule(dynamic, 1024)
       for (int64_t v = 0; v < n; v++) {                     // tokenPassing 316-333
         if (!cand[v]) { fin[v] = 0; continue; }
         bool win = true;
@@ -517,6 +531,16 @@ int read_part_file(const std::string& path, int64_t n_c, int32_t k, std::vector<
 extern "C" {
 
 void tgpo_plant_mistake(int32_t which) { g_plant = which; }
+
+int32_t tgpo_set_threads(int32_t threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+  return omp_get_max_threads();
+#else
+  (void)threads;
+  return 1;
+#endif
+}
 
 uint64_t tgpo_hash(const int32_t* v, int64_t n) { return fnv(v, n); }
 

@@ -119,6 +119,10 @@ int64_t tgpo_final_text(int64_t n, const int32_t* part, int32_t k, char* buf,
    ranked in buffer order instead of after the network (gpu_part.cu:596-600),
    3 the repaired contraction keeps the first weight instead of the sum.     */
 void tgpo_plant_mistake(int32_t which);
+/* Threads of the oracle's order-independent loops (gains, candidates, token
+   passing, the network's stages, the cut); results do not depend on it.
+   threads <= 0 leaves the setting; returns the number now in use.          */
+int32_t tgpo_set_threads(int32_t threads);
 /* FNV-1a 64 over int32 values, the hash used in trace records */
 uint64_t tgpo_hash(const int32_t* v, int64_t n);
 

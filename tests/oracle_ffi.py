@@ -64,6 +64,8 @@ def lib():
         L.tgpo_final_text.argtypes = [i64, P, i32, C.c_char_p, i64]
         L.tgpo_final_text.restype = i64
         L.tgpo_plant_mistake.argtypes = [i32]
+        L.tgpo_set_threads.argtypes = [i32]
+        L.tgpo_set_threads.restype = i32
         L.tgpo_hash.argtypes = [P, i64]
         L.tgpo_hash.restype = C.c_uint64
         _lib = L
@@ -234,6 +236,11 @@ class planted:
 
     def __exit__(self, *exc):
         lib().tgpo_plant_mistake(0)
+
+
+def set_threads(threads):
+    """Threads of the oracle's parallel loops (0: keep); returns the count in use."""
+    return int(lib().tgpo_set_threads(threads))
 
 
 def hash_part(part):
