@@ -117,6 +117,21 @@ typedef struct tgp_options {
    * reference): when set, the finest level runs learned passes before its
    * refinement (see tgp_learned_model).                                      */
   const struct tgp_learned_model* learned;
+  /* Imbalance bound of a balance stage at the finest level (no counterpart
+   * in the reference, whose refinement keeps only 5 % + slack per level and
+   * ends far above it on power-law graphs).  0: off, the reference's result.
+   * eps > 0: after the finest level's refinement, with cap =
+   * floor((1 + eps) * ceil(n / k)): balance rounds move vertices out of the
+   * parts above cap (each to the part with room of largest exact integer
+   * gain; ranked per destination, then per source, by gain desc and vertex
+   * id asc, within room and excess) until every part holds at most cap; then
+   * the learned passes, when a model is given, with capacity cap - count;
+   * then refine passes of positive exact gains that keep every part within
+   * cap (an independent set of winners: no neighbouring candidate with a
+   * larger gain or the same gain and a smaller id; ranked per destination
+   * within room).  part_count then holds the final partition's counts.
+   * k <= 1024.                                                              */
+  float balance_eps;
 } tgp_options;
 
 /* The learned move proposer's weights (device memory) for the GNN of
@@ -145,7 +160,7 @@ typedef struct tgp_learned_model {
 
 /* trace record kinds */
 enum { TGP_TRACE_LEVEL = 1, TGP_TRACE_INIT = 2, TGP_TRACE_PASS = 3, TGP_TRACE_REFINE = 4,
-       TGP_TRACE_FINAL = 5 };
+       TGP_TRACE_FINAL = 5, TGP_TRACE_BALANCE = 6 };
 
 /* trace records (same layout the oracle emits, used by differential tests)  */
 typedef struct tgp_trace {
@@ -153,7 +168,9 @@ typedef struct tgp_trace {
   int32_t level, phase, pass;
   int64_t a, b, c, d;      /* LEVEL: n, m, n_c, m_c    INIT: n_c, npairs, cut0
                               PASS: moves M, selected S, cut, pads in prefix
-                              REFINE: n, nedges, cut, passes  FINAL: n, m, cut  */
+                              REFINE: n, nedges, cut, passes  FINAL: n, m, cut
+                              BALANCE: moves in balance rounds, moves in refine
+                              passes, cut, largest part (pass = rounds)        */
   uint64_t hash;           /* FNV-1a of the partition (only with check_level)  */
 } tgp_trace;
 

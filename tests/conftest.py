@@ -12,6 +12,21 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def torch_gpu_first(request):
+    """When GPU tests run, torch's HIP runtime is initialised before the
+    library's first HIP call: torch's wheel carries its own copy of the runtime,
+    which finds no device when it starts after the system one."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
+    yield
+
+
 @pytest.fixture(scope="session")
 def oracle():
     import oracle_ffi

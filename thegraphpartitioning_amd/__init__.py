@@ -53,7 +53,7 @@ class COptions(C.Structure):
                 ("device_input", C.c_int32), ("profile", C.c_int32),
                 ("check_level", C.c_int32), ("repair", C.c_int32),
                 ("pass_dump", C.c_void_p), ("pass_dump_user", C.c_void_p),
-                ("learned", C.c_void_p)]
+                ("learned", C.c_void_p), ("balance_eps", C.c_float)]
 
 
 class CGpmetisDir(C.Structure):
@@ -354,7 +354,7 @@ class Stats:
 
 def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=False,
               coarsen_floor=0, profile=False, check_level=0, trace_cap=0, device_graph=None,
-              pass_dump=None, learned=None, noalloc=False):
+              pass_dump=None, learned=None, noalloc=False, balance_eps=0.0):
     """Partitions `g` (a Graph) into k parts on the GPU.
 
     Returns (part[n] int32, part_count[k] int32, edge_cut, Stats).  `init`
@@ -367,6 +367,8 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
     PASS_DUMP_FN called with each level's partition at every phase start and
     after every committed pass; `learned` a tgp_learned_model (learn.GNN.handle())
     whose learned passes run at the finest level before its refinement.
+    `balance_eps` > 0 adds the balance stage at the finest level (every part
+    at most floor((1 + eps) * ceil(n / k)) vertices; see tgp_options).
     `noalloc` selects tgp_partition_noalloc (same result; every device buffer
     from the library's fixed scratch; size limits TGP_NOALLOC_MAX_*).
     """
@@ -413,6 +415,7 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
     if pass_dump is not None:
         o.pass_dump = C.cast(pass_dump, C.c_void_p).value
         keep.append(pass_dump)
+    o.balance_eps = float(balance_eps)
     if learned is not None:
         o.learned = C.addressof(learned)
         keep.append(learned)

@@ -21,7 +21,8 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include")]
 # packed-fp32 instructions are kept out of every device code object
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HIP_SOURCES = ["tgp_kernels.hip", "tgp_learn.hip", "tgp_driver.cpp", "tgp_host.cpp"]
+HIP_SOURCES = ["tgp_kernels.hip", "tgp_learn.hip", "tgp_balance.hip", "tgp_driver.cpp",
+               "tgp_host.cpp"]
 GXX_SOURCES = ["tgp_gen.cpp"]
 
 

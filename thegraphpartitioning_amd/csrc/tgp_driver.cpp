@@ -451,7 +451,9 @@ class Run {
   i64 compute_cut(Level& lv, i64 nedges, const i32* part);
   LevelAux& aux(Level& lv, i64 nedges);
   void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut);
-  void learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc);
+  // cap > 0: capacity cap - count per part instead of the reference's rule
+  void learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64 cap = 0);
+  void balance_finest(DBuf<i32>& part, DBuf<i32>& pc);
   // the refinement trace callback with a host copy of part
   void dump_part(int level, int phase, int pass, const i32* part, i64 n) {
     std::vector<i32> h((size_t)n);
@@ -1242,7 +1244,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
 // then the reference's token rule on those gains, move buffer, capacity,
 // sort, selection and commit (gpu_part.cu:316-397, 472-506).  Stops when
 // fewer than min_admit * n moves are admitted.
-void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) {
+void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64 cap) {
   const int TOKEN_LANES = token_lanes();
   const tgp_learned_model& Mdl = *O.learned;
   Level& lv = *L[level];
@@ -1299,7 +1301,8 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
     C.sync();
     const i64 M = std::min<i64>(C.pinned[0], n);
     if (M < min_moves) break;
-    launch_capacity(pc.p, outc.p, k, n, 0, avail.p, nullptr, nullptr, nullptr, st());
+    if (cap > 0) launch_bal_avail(pc.p, k, cap, avail.p, st());
+    else launch_capacity(pc.p, outc.p, k, n, 0, avail.p, nullptr, nullptr, nullptr, st());
     i64 len = M, padded = 1;
     while (padded < len) padded <<= 1;
     if (padded > n) padded = n;
@@ -1310,6 +1313,85 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc) 
                   nullptr, nullptr, st());
     passes_total++;
   }
+}
+
+// The balance stage of the finest level (tgp_options.balance_eps): balance
+// rounds until every part holds at most cap, the learned passes (when a
+// model is given) within cap, then refine passes of positive exact gains
+// within cap.  No counterpart in the reference.
+void Run::balance_finest(DBuf<i32>& part, DBuf<i32>& pc) {
+  static const int REFINE_PASSES = 30;
+  static const int MAX_ROUNDS = 1000;
+  Level& lv = *L[0];
+  const i64 n = lv.n;
+  const DGraph g = lv.dg();
+  const i64 cap = (i64)std::floor((1.0 + (double)O.balance_eps) * (double)((n + k - 1) / k));
+  DBuf<i32> T(&C.pool, (size_t)n * k), target(&C.pool, n), gain(&C.pool, n), room(&C.pool, k),
+      lim(&C.pool, k), bcnt(&C.pool, 4096), vals(&C.pool, n), vals2(&C.pool, n);
+  DBuf<u8> cand(&C.pool, n), keep(&C.pool, n), keep2(&C.pool, n);
+  DBuf<u64> keys(&C.pool, n), keys2(&C.pool, n), moved(&C.pool, 1);
+  DBuf<i64> total(&C.pool, 1), start(&C.pool, k);
+  const i64 sb = std::max<i64>(balance_sort_bytes(n), 16);
+  DBuf<u8> sort_tmp(&C.pool, sb);
+  const BalScratch S{bcnt.p, total.p, keys.p, keys2.p, vals.p, vals2.p, start.p, sort_tmp.p, sb};
+  // the counts of the partition itself (the reference's counts are those of
+  // its last pass, not of the best partition it restores)
+  HIP_TRY(hipMemsetAsync(pc.p, 0, k * sizeof(i32), st()));
+  launch_sanitize_hist(part.p, n, k, pc.p, st());
+  std::vector<i32> hpc((size_t)k), hroom((size_t)k), hlim((size_t)k);
+  auto counts = [&]() {
+    HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    bool over = false;
+    for (i32 q = 0; q < k; q++) {
+      hroom[q] = (i32)std::max<i64>(std::min<i64>(cap - hpc[q], INT_MAX), INT_MIN);
+      hlim[q] = (i32)std::max<i64>(hpc[q] - cap, 0);
+      over |= hpc[q] > cap;
+    }
+    HIP_TRY(hipMemcpyAsync(room.p, hroom.data(), k * sizeof(i32), hipMemcpyHostToDevice, st()));
+    HIP_TRY(hipMemcpyAsync(lim.p, hlim.data(), k * sizeof(i32), hipMemcpyHostToDevice, st()));
+    return over;
+  };
+  auto apply = [&](const u8* kp) {
+    HIP_TRY(hipMemsetAsync(moved.p, 0, sizeof(u64), st()));
+    launch_bal_apply(kp, n, target.p, part.p, pc.p, moved.p, st());
+    HIP_TRY(hipMemcpyAsync(C.pinned, moved.p, sizeof(u64), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    return (i64)C.pinned[0];
+  };
+  C.prof.begin(KID_MISC);
+  i64 bal_moves = 0, ref_moves = 0;
+  int rounds = 0;
+  while (rounds < MAX_ROUNDS && counts()) {
+    launch_bal_tally(g, part.p, k, T.p, st());
+    launch_bal_best(n, part.p, T.p, k, room.p, 0, cand.p, target.p, gain.p, st());
+    launch_bal_rank_keep(cand.p, n, target.p, gain.p, k, room.p, keep.p, S, st());
+    launch_bal_rank_keep(keep.p, n, part.p, gain.p, k, lim.p, keep2.p, S, st());
+    const i64 mv = apply(keep2.p);
+    rounds++;
+    bal_moves += mv;
+    if (mv == 0) break;
+  }
+  C.prof.end(KID_MISC, 0);
+  if (O.learned) learned_passes(0, lv.m, part, pc, cap);
+  C.prof.begin(KID_MISC);
+  for (int pass = 0; pass < REFINE_PASSES; pass++) {
+    counts();
+    launch_bal_tally(g, part.p, k, T.p, st());
+    launch_bal_best(n, part.p, T.p, k, room.p, 1, cand.p, target.p, gain.p, st());
+    launch_bal_token(g, cand.p, gain.p, keep.p, st());
+    launch_bal_rank_keep(keep.p, n, target.p, gain.p, k, room.p, keep2.p, S, st());
+    const i64 mv = apply(keep2.p);
+    ref_moves += mv;
+    if (mv == 0) break;
+  }
+  C.prof.end(KID_MISC, 0);
+  counts();
+  i32 largest = 0;
+  for (i32 q = 0; q < k; q++) largest = std::max(largest, hpc[q]);
+  const i64 cut = compute_cut(lv, lv.m, part.p);
+  tr.push(TGP_TRACE_BALANCE, 0, 0, rounds, bal_moves, ref_moves, cut, largest,
+          O.check_level ? hash_dev(part.p, n) : 0);
 }
 
 int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
@@ -1368,13 +1450,17 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     launch_sanitize_hist(pf.p, nf, k, pc.p, st());
     C.prof.end(KID_PROJECT, 12.0 * ncs + 12.0 * nf);
     part = std::move(pf);
-    if (O.learned && level == 0) learned_passes(level, ecount[level], part, pc);
+    if (O.learned && level == 0 && !(O.balance_eps > 0.0f))
+      learned_passes(level, ecount[level], part, pc);
     cut = compute_cut(*L[level], ecount[level], part.p);
     refine(level, ecount[level], part, pc, &cut);
   }
+  if (O.balance_eps > 0.0f) balance_finest(part, pc);
   // gpu_part.cu:1285: the level-0 refinement ended with this partition's cut
   // over the same ranges unless level 0 is the coarsest level (B5 ranges there)
-  const i64 fcut = pass > 0 ? cut : compute_cut(*L[0], m, part.p);
+  // or the balance stage moved vertices since
+  const i64 fcut =
+      pass > 0 && !(O.balance_eps > 0.0f) ? cut : compute_cut(*L[0], m, part.p);
   HIP_TRY(hipMemcpyAsync(part_out, part.p, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
   HIP_TRY(hipMemcpyAsync(pc_out, pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
@@ -1449,6 +1535,7 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
     const bool ok = it.o.repair && it.o.initpart == tgp_initpart_block && !it.o.profile &&
                     !it.o.pass_dump && !it.o.learned &&
                     !it.o.check_level && (!it.stats || !it.stats->trace) && it.o.k >= 1 &&
+                    !(it.o.balance_eps > 0.0f) &&
                     it.o.k <= BATCH_MAXK && it.g.n >= 2 && it.g.n <= INT_MAX && it.g.m > 0 &&
                     it.g.m <= INT_MAX && it.part && it.part_count &&
                     (it.o.device_input || validate_host_graph(&it.g) == TGP_OK);
@@ -1667,6 +1754,11 @@ int partition_on(Context& Cx, const tgp_graph* g, const tgp_options* o, int32_t*
     }
   } else if (g->n <= 0 || g->m <= 0) {
     set_error("empty graph");
+    return TGP_EINVAL;
+  }
+  if (o->balance_eps < 0.0f || o->balance_eps != o->balance_eps ||
+      (o->balance_eps > 0.0f && o->k > 1024)) {
+    set_error("balance_eps must be >= 0 (and k <= 1024 when it is set)");
     return TGP_EINVAL;
   }
   if (!o->wide) {  // reference int32 limits (gpu_part.cu:531, 886-888)

@@ -363,4 +363,36 @@ void launch_arc_source(DGraph g, i32* asrc, hipStream_t s);
 void launch_cut_arcs(DGraph g, i64 e_end, const i32* asrc, const i32* part, const u8* part8,
                      i32 k, int unit, u64* acc, hipStream_t s);
 
+// ---- balance stage (tgp_balance.hip): finest level, cap per part --------------
+struct BalScratch {
+  i32* bcnt;       // 4096 block counts
+  i64* total;      // 1
+  u64* keys;       // n
+  u64* keys2;      // n
+  i32* vals;       // n
+  i32* vals2;      // n
+  i64* start;      // k
+  void* sort_tmp;  // sort_bytes
+  i64 sort_bytes;
+};
+i64 balance_sort_bytes(i64 max_items);
+// T[v * k + q] = weight of v's arcs into part q (self loops excluded)
+void launch_bal_tally(DGraph g, const i32* part, i32 k, i32* T, hipStream_t s);
+// best destination with room (room[q] = cap - count): mode 0 vertices of parts
+// over cap (room < 0), any gain; mode 1 every vertex, positive gain
+void launch_bal_best(i64 n, const i32* part, const i32* T, i32 k, const i32* room, int mode,
+                     u8* cand, i32* target, i32* gain, hipStream_t s);
+// winners: candidates with no neighbouring candidate of larger gain, or equal
+// gain and smaller id
+void launch_bal_token(DGraph g, const u8* cand, const i32* gain, u8* win, hipStream_t s);
+// keep[v] = flagged and ranked (gain desc, v asc) below limit within keypart
+// (synchronises the stream once to read the candidate count)
+void launch_bal_rank_keep(const u8* flag, i64 n, const i32* keypart, const i32* gain, i32 k,
+                          const i32* limit, u8* keep, const BalScratch& S, hipStream_t s);
+// avail[q] = max(cap - pc[q], 0)
+void launch_bal_avail(const i32* pc, i32 k, i64 cap, i64* avail, hipStream_t s);
+// part[v] = target[v] for kept v, counts updated, *moved += number applied
+void launch_bal_apply(const u8* keep, i64 n, const i32* target, i32* part, i32* pc, u64* moved,
+                      hipStream_t s);
+
 }  // namespace tgp
