@@ -367,30 +367,42 @@ def bench_learned(args):
     kw = dict(init=tgp.INIT_BLOCK, repair=True, wide=bool(grid), device_graph=dg.pointers())
     handle = model.handle(max_passes=20, min_admit=0.0005, margin=args.learn_margin)
 
-    def run(learned):
+    eps = args.balance_eps
+
+    def run(learned, bal):
         extra = dict(learned=handle) if learned else {}
+        if bal:
+            extra["balance_eps"] = eps
         tgp.partition(g, k, **kw, **extra)  # warm-up
         t = time.perf_counter()
-        r = tgp.partition(g, k, **kw, **extra)
+        r = tgp.partition(g, k, trace_cap=1 << 14, **kw, **extra)
         secs = time.perf_counter() - t
         part = r[0]
         imb = float(np.bincount(part, minlength=k).max()) / math.ceil(g.n / k) - 1.0
-        return dict(seconds=round(secs, 4), value=round(g.m / secs / 1e6, 3), edge_cut=r[2],
-                    imbalance=round(imb, 4), passes=r[3].passes)
-    plain = run(False)
-    lrn = run(True)
+        out = dict(seconds=round(secs, 4), value=round(g.m / secs / 1e6, 3), edge_cut=r[2],
+                   imbalance=round(imb, 4), passes=r[3].passes)
+        rec = [t for t in r[3].trace if t[0] == 6]
+        if rec:
+            out["balance"] = {"rounds": rec[0][3], "balance_moves": rec[0][4],
+                              "refine_moves": rec[0][5]}
+        return out
+    plain = run(False, False)
+    plain_eps = run(False, True)
+    lrn_eps = run(True, True)
     dg.free()
     print(json.dumps({
-        "metric": f"M edges/s end-to-end at k={k} (learned)", "value": lrn["value"],
+        "metric": f"M edges/s end-to-end at k={k} (learned, eps={eps})", "value": lrn_eps["value"],
         "unit": "M edges/s", "n_gpus": 1, "dtype": "bf16+fp32", "data": "synthetic",
         "config": {"workload": f"grid{grid}^3-k{k}" if grid else
                    f"rmat{args.scale}-ef{args.edge_factor}-k{k}", "vertices": g.n,
                    "arcs": g.m, "train_graphs": ntrain, "train_passes": npairs,
                    "train_epochs": args.learn_epochs, "train_seconds": round(train_s, 2),
-                   "coarsening": "repaired", "initial_partition": "block i*k/n_c"},
-        "plain": plain, "learned": lrn,
-        "cut_vs_plain": round(lrn["edge_cut"] / max(plain["edge_cut"], 1), 4),
-        "speed_vs_plain": round(lrn["value"] / plain["value"], 4)}), flush=True)
+                   "coarsening": "repaired", "initial_partition": "block i*k/n_c",
+                   "balance_eps": eps},
+        "plain": plain, "plain_eps": plain_eps, "learned_eps": lrn_eps,
+        "cut_vs_plain_eps": round(lrn_eps["edge_cut"] / max(plain_eps["edge_cut"], 1), 4),
+        "speed_vs_plain_eps": round(lrn_eps["value"] / plain_eps["value"], 4),
+        "speed_vs_plain": round(lrn_eps["value"] / plain["value"], 4)}), flush=True)
 
 
 def main():
@@ -422,6 +434,8 @@ def main():
     ap.add_argument("--learn-grid", type=int, default=0, metavar="SIDE",
                     help="with --learned: train on 3-D grids and evaluate on the SIDE^3 grid "
                          "(wide mode) instead of R-MAT")
+    ap.add_argument("--balance-eps", type=float, default=0.03,
+                    help="imbalance bound of the balance stage in the learned comparison")
     ap.add_argument("--learn-margin", type=float, default=3.0,
                     help="a proposal's logit may trail the own part's by this much")
     ap.add_argument("--hbm-scale", action="store_true",
