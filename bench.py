@@ -365,7 +365,8 @@ def bench_learned(args):
     g = tgp.gen_grid3d(grid) if grid else tgp.gen_rmat(args.scale, args.edge_factor, SEED)
     dg = tgp.DeviceGraph(g)
     kw = dict(init=tgp.INIT_BLOCK, repair=True, wide=bool(grid), device_graph=dg.pointers())
-    handle = model.handle(max_passes=20, min_admit=0.0005, margin=args.learn_margin)
+    handle = model.handle(max_passes=20, min_admit=0.0005, margin=args.learn_margin,
+                          infer_every=args.learn_infer_every)
 
     eps = args.balance_eps
 
@@ -398,7 +399,7 @@ def bench_learned(args):
                    "arcs": g.m, "train_graphs": ntrain, "train_passes": npairs,
                    "train_epochs": args.learn_epochs, "train_seconds": round(train_s, 2),
                    "coarsening": "repaired", "initial_partition": "block i*k/n_c",
-                   "balance_eps": eps},
+                   "balance_eps": eps, "infer_every": args.learn_infer_every},
         "plain": plain, "plain_eps": plain_eps, "learned_eps": lrn_eps,
         "cut_vs_plain_eps": round(lrn_eps["edge_cut"] / max(plain_eps["edge_cut"], 1), 4),
         "speed_vs_plain_eps": round(lrn_eps["value"] / plain_eps["value"], 4),
@@ -436,6 +437,8 @@ def main():
                          "(wide mode) instead of R-MAT")
     ap.add_argument("--balance-eps", type=float, default=0.03,
                     help="imbalance bound of the balance stage in the learned comparison")
+    ap.add_argument("--learn-infer-every", type=int, default=4,
+                    help="the GNN runs every this many learned passes (logits reused between)")
     ap.add_argument("--learn-margin", type=float, default=3.0,
                     help="a proposal's logit may trail the own part's by this much")
     ap.add_argument("--hbm-scale", action="store_true",

@@ -156,6 +156,10 @@ typedef struct tgp_learned_model {
   const float* b2;      /* k */
   float min_admit;
   float margin;         /* a proposal needs logit(q) > logit(part_v) - margin */
+  int32_t infer_every;  /* the GNN runs on passes 0, r, 2r, ... (r = infer_every;
+                           <= 1: every pass); the passes between propose from
+                           the last logits, still gated by the current exact
+                           gains and capacities                              */
 } tgp_learned_model;
 
 /* trace record kinds */

@@ -64,20 +64,23 @@ __global__ __launch_bounds__(BAL_BS) void k_bal_tally(DGraph g, const i32* part,
 
 // Best destination with room for each vertex (mode 0: vertices of parts over
 // cap, any gain; mode 1: every vertex, gain > 0).  cand[v] = 1 for a
-// candidate, with target[v] and gain[v].
+// candidate, with target[v] and gain[v].  Four lanes per vertex, each taking
+// every fourth part of the row; the lanes' bests are combined with the same
+// rule (larger gain, then lower part).
 __global__ __launch_bounds__(BAL_BS) void k_bal_best(i64 n, const i32* part, const i32* T, i32 k,
                                                      const i32* room, int mode, u8* cand,
                                                      i32* target, i32* gain) {
-  for (i64 v = (i64)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (i64)gridDim.x * blockDim.x) {
+  const int gl = threadIdx.x & 3;
+  const i64 ngroups = (i64)gridDim.x * (BAL_BS / 4);
+  for (i64 v = (i64)blockIdx.x * (BAL_BS / 4) + threadIdx.x / 4; v < n; v += ngroups) {
     const i32 p = part[v];
-    bool c = false;
     i32 bq = -1;
     i64 bg = LLONG_MIN;
-    if (p >= 0 && p < k && (mode == 1 || room[p] < 0)) {
+    const bool look = p >= 0 && p < k && (mode == 1 || room[p] < 0);
+    if (look) {
       const i32* Tv = T + v * k;
       const i64 own = Tv[p];
-      for (i32 q = 0; q < k; q++) {
+      for (i32 q = gl; q < k; q += 4) {
         if (q == p || room[q] <= 0) continue;
         const i64 gq = (i64)Tv[q] - own;
         if (gq > bg) {
@@ -85,11 +88,53 @@ __global__ __launch_bounds__(BAL_BS) void k_bal_best(i64 n, const i32* part, con
           bq = q;
         }
       }
-      c = bq >= 0 && (mode == 0 || bg > 0);
     }
-    cand[v] = c;
-    target[v] = c ? bq : -1;
-    gain[v] = c ? (i32)(bg > INT_MAX ? INT_MAX : bg < INT_MIN ? INT_MIN : bg) : 0;
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+      const i64 og = __shfl_xor(bg, o, 4);
+      const i32 oq = __shfl_xor(bq, o, 4);
+      if (oq >= 0 && (bq < 0 || og > bg || (og == bg && oq < bq))) {
+        bg = og;
+        bq = oq;
+      }
+    }
+    if (gl == 0) {
+      const bool c = look && bq >= 0 && (mode == 0 || bg > 0);
+      cand[v] = c;
+      target[v] = c ? bq : -1;
+      gain[v] = c ? (i32)(bg > INT_MAX ? INT_MAX : bg < INT_MIN ? INT_MIN : bg) : 0;
+    }
+  }
+}
+
+// Tallies after a round: each kept vertex v moving from part[v] to target[v]
+// shifts the weight of each of its arcs (v, u) from T[u][part[v]] to
+// T[u][target[v]] (the reverse arc (u, v) has the same weight on these
+// levels).  Run before the moves are applied.  Each wave takes 64 vertices
+// and walks the rows of the kept ones with the whole wave.
+__global__ __launch_bounds__(BAL_BS) void k_bal_tally_update(DGraph g, const u8* keep,
+                                                             const i32* part,
+                                                             const i32* target, i32 k, i32* T) {
+  const int lane = bl_lane();
+  const i64 gw = ((i64)blockIdx.x * BAL_BS + threadIdx.x) >> 6;
+  const i64 nw = ((i64)gridDim.x * BAL_BS) >> 6;
+  for (i64 v0 = gw * 64; v0 < g.n; v0 += nw * 64) {
+    const i64 mv = v0 + lane;
+    u64 todo = __ballot(mv < g.n && keep[mv]);
+    while (todo) {
+      const int l = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const i64 v = v0 + l;
+      const i32 src = part[v], dst = target[v];
+      const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+      for (i64 i = s0 + lane; i < s1; i += 64) {
+        const i32 u = g.adj[i];
+        if (u == (i32)v || u < 0 || u >= g.n) continue;
+        const i32 w = g.w[i];
+        atomicAdd(&T[(i64)u * k + src], -w);
+        atomicAdd(&T[(i64)u * k + dst], w);
+      }
+    }
   }
 }
 
@@ -256,8 +301,14 @@ void launch_bal_tally(DGraph g, const i32* part, i32 k, i32* T, hipStream_t s) {
 
 void launch_bal_best(i64 n, const i32* part, const i32* T, i32 k, const i32* room, int mode,
                      u8* cand, i32* target, i32* gain, hipStream_t s) {
-  k_bal_best<<<bl_grid(n, BAL_BS, 1u << 16), BAL_BS, 0, s>>>(n, part, T, k, room, mode, cand,
-                                                             target, gain);
+  k_bal_best<<<bl_grid(n, BAL_BS / 4, 1u << 16), BAL_BS, 0, s>>>(n, part, T, k, room, mode,
+                                                                 cand, target, gain);
+}
+
+void launch_bal_tally_update(DGraph g, const u8* keep, const i32* part, const i32* target, i32 k,
+                             i32* T, hipStream_t s) {
+  k_bal_tally_update<<<bl_grid(g.n, BAL_BS, 1u << 14), BAL_BS, 0, s>>>(g, keep, part, target, k,
+                                                                      T);
 }
 
 void launch_bal_token(DGraph g, const u8* cand, const i32* gain, u8* win, hipStream_t s) {

@@ -1277,19 +1277,22 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, 
     }
   };
   const i64 min_moves = std::max<i64>(1, (i64)(Mdl.min_admit * (double)n));
+  const int every = std::max(1, Mdl.infer_every);
   for (int p = 0; p < Mdl.max_passes; p++) {
     ck(tgpl_features(n, g.xadj, g.adj, g.w, part.p, k, F, X.p, tally.p, s));
-    // H = ReLU(X W1a + mean_N(X W1b) + b1): the mean over the 64 columns of X W1b
-    ck(tgpl_gemm(X.p, F, 0, Mdl.W1b, Hd, 0, Pf.p, Hd, Pb.p, (int)n, Hd, F, 1.0f, 0.0f, nullptr, 0,
-                 s));
-    ck(tgpl_mean_agg(n, g.xadj, g.adj, Pb.p, Hd, 0, M1.p, Hf.p, s));
-    ck(tgpl_gemm(X.p, F, 0, Mdl.W1a, Hd, 0, Hf.p, Hd, Hb.p, (int)n, Hd, F, 1.0f, 1.0f, Mdl.b1, 1,
-                 s));
-    ck(tgpl_mean_agg(n, g.xadj, g.adj, Hb.p, Hd, 0, M1.p, nullptr, s));
-    ck(tgpl_gemm(Hb.p, Hd, 0, Mdl.W2a, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 0.0f, nullptr,
-                 0, s));
-    ck(tgpl_gemm(M1.p, Hd, 0, Mdl.W2b, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 1.0f, Mdl.b2,
-                 0, s));
+    if (p % every == 0) {
+      // H = ReLU(X W1a + mean_N(X W1b) + b1): the mean over the 64 columns of X W1b
+      ck(tgpl_gemm(X.p, F, 0, Mdl.W1b, Hd, 0, Pf.p, Hd, Pb.p, (int)n, Hd, F, 1.0f, 0.0f, nullptr,
+                   0, s));
+      ck(tgpl_mean_agg(n, g.xadj, g.adj, Pb.p, Hd, 0, M1.p, Hf.p, s));
+      ck(tgpl_gemm(X.p, F, 0, Mdl.W1a, Hd, 0, Hf.p, Hd, Hb.p, (int)n, Hd, F, 1.0f, 1.0f, Mdl.b1,
+                   1, s));
+      ck(tgpl_mean_agg(n, g.xadj, g.adj, Hb.p, Hd, 0, M1.p, nullptr, s));
+      ck(tgpl_gemm(Hb.p, Hd, 0, Mdl.W2a, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 0.0f, nullptr,
+                   0, s));
+      ck(tgpl_gemm(M1.p, Hd, 0, Mdl.W2b, k, 0, Z.p, k, nullptr, (int)n, k, Hd, 1.0f, 1.0f, Mdl.b2,
+                   0, s));
+    }
     ck(tgpl_propose(n, k, Z.p, part.p, tally.p, Mdl.margin, gain.p, target.p, s));
     HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
     HIP_TRY(hipMemsetAsync(ctr.p, 0, 2 * sizeof(i32), st()));
@@ -1352,8 +1355,17 @@ void Run::balance_finest(DBuf<i32>& part, DBuf<i32>& pc) {
     HIP_TRY(hipMemcpyAsync(lim.p, hlim.data(), k * sizeof(i32), hipMemcpyHostToDevice, st()));
     return over;
   };
+  // tallies kept up to date by each round's moves on levels whose reverse
+  // arcs carry the same weight; rebuilt every round elsewhere
+  const bool incremental = lv.simple_ws;
+  bool T_valid = false;
+  auto tally = [&]() {
+    if (!(incremental && T_valid)) launch_bal_tally(g, part.p, k, T.p, st());
+    T_valid = true;
+  };
   auto apply = [&](const u8* kp) {
     HIP_TRY(hipMemsetAsync(moved.p, 0, sizeof(u64), st()));
+    if (incremental) launch_bal_tally_update(g, kp, part.p, target.p, k, T.p, st());
     launch_bal_apply(kp, n, target.p, part.p, pc.p, moved.p, st());
     HIP_TRY(hipMemcpyAsync(C.pinned, moved.p, sizeof(u64), hipMemcpyDeviceToHost, st()));
     C.sync();
@@ -1363,7 +1375,7 @@ void Run::balance_finest(DBuf<i32>& part, DBuf<i32>& pc) {
   i64 bal_moves = 0, ref_moves = 0;
   int rounds = 0;
   while (rounds < MAX_ROUNDS && counts()) {
-    launch_bal_tally(g, part.p, k, T.p, st());
+    tally();
     launch_bal_best(n, part.p, T.p, k, room.p, 0, cand.p, target.p, gain.p, st());
     launch_bal_rank_keep(cand.p, n, target.p, gain.p, k, room.p, keep.p, S, st());
     launch_bal_rank_keep(keep.p, n, part.p, gain.p, k, lim.p, keep2.p, S, st());
@@ -1373,11 +1385,14 @@ void Run::balance_finest(DBuf<i32>& part, DBuf<i32>& pc) {
     if (mv == 0) break;
   }
   C.prof.end(KID_MISC, 0);
-  if (O.learned) learned_passes(0, lv.m, part, pc, cap);
+  if (O.learned) {
+    learned_passes(0, lv.m, part, pc, cap);
+    T_valid = false;
+  }
   C.prof.begin(KID_MISC);
   for (int pass = 0; pass < REFINE_PASSES; pass++) {
     counts();
-    launch_bal_tally(g, part.p, k, T.p, st());
+    tally();
     launch_bal_best(n, part.p, T.p, k, room.p, 1, cand.p, target.p, gain.p, st());
     launch_bal_token(g, cand.p, gain.p, keep.p, st());
     launch_bal_rank_keep(keep.p, n, target.p, gain.p, k, room.p, keep2.p, S, st());

@@ -389,6 +389,10 @@ void launch_bal_token(DGraph g, const u8* cand, const i32* gain, u8* win, hipStr
 // (synchronises the stream once to read the candidate count)
 void launch_bal_rank_keep(const u8* flag, i64 n, const i32* keypart, const i32* gain, i32 k,
                           const i32* limit, u8* keep, const BalScratch& S, hipStream_t s);
+// before applying a round's moves on a level with equal reverse weights: the
+// tallies of the moved vertices' neighbours, moved from part[v] to target[v]
+void launch_bal_tally_update(DGraph g, const u8* keep, const i32* part, const i32* target, i32 k,
+                             i32* T, hipStream_t s);
 // avail[q] = max(cap - pc[q], 0)
 void launch_bal_avail(const i32* pc, i32 k, i64 cap, i64* avail, hipStream_t s);
 // part[v] = target[v] for kept v, counts updated, *moved += number applied

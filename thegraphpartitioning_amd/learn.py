@@ -30,7 +30,8 @@ class CModel(C.Structure):
     _fields_ = [("k", C.c_int32), ("features", C.c_int32), ("hidden", C.c_int32),
                 ("max_passes", C.c_int32), ("W1a", C.c_void_p), ("W1b", C.c_void_p),
                 ("W2a", C.c_void_p), ("W2b", C.c_void_p), ("b1", C.c_void_p),
-                ("b2", C.c_void_p), ("min_admit", C.c_float), ("margin", C.c_float)]
+                ("b2", C.c_void_p), ("min_admit", C.c_float), ("margin", C.c_float),
+                ("infer_every", C.c_int32)]
 
 
 def _lib():
@@ -221,7 +222,7 @@ class GNN:
         return a[0] / max(a[1], 1e-30), a[2] / max(a[3], 1), rec
 
     # ---- use in the partitioner ------------------------------------------------
-    def handle(self, max_passes=20, min_admit=0.0005, margin=3.0):
+    def handle(self, max_passes=20, min_admit=0.0005, margin=3.0, infer_every=1):
         """A tgp_learned_model for tgp.partition(..., learned=...); keep self alive.
 
         The library reads the weights on its own stream, so the work that
@@ -231,7 +232,8 @@ class GNN:
         self._cm = CModel(self.k, self.F, self.hidden, max_passes,
                           self.Wb["W1a"].data_ptr(), self.Wb["W1b"].data_ptr(),
                           self.Wb["W2a"].data_ptr(), self.Wb["W2b"].data_ptr(),
-                          self.W["b1"].data_ptr(), self.W["b2"].data_ptr(), min_admit, margin)
+                          self.W["b1"].data_ptr(), self.W["b2"].data_ptr(), min_admit, margin,
+                          infer_every)
         return self._cm
 
     def state(self):
