@@ -186,6 +186,8 @@ def bench_batched(args):
     dgs = [tgp.DeviceGraph(g) for g in gs]
     ptrs = [d.pointers() for d in dgs]
     kw = dict(init=tgp.INIT_BLOCK, repair=True)
+    if args.batch_balance_eps > 0:
+        kw["balance_eps"] = args.batch_balance_eps
     wn = min(len(gs), 2 * args.workers)
     tgp.partition_batch(gs[:wn], ks[:wn], workers=args.workers, device_graphs=ptrs[:wn], **kw)
     t0 = time.perf_counter()
@@ -219,6 +221,7 @@ def bench_batched(args):
                                "half R-MAT ef 8, half Erdos-Renyi mean degree 10, k in {2,4,8,16}",
                    "graphs": len(gs), "arcs": arcs, "workers": args.workers,
                    "coarsening": "repaired", "initial_partition": "block i*k/n_c",
+                   "balance_eps": args.batch_balance_eps or None,
                    "parallelism": "single GPU, graphs on concurrent streams"},
         "detail": {"plain_value": round(p_arcs / p_s / 1e6, 3),
                    "plain_graphs": len(sample), "plain_seconds": round(p_s, 3),
@@ -437,6 +440,9 @@ def main():
                          "(wide mode) instead of R-MAT")
     ap.add_argument("--balance-eps", type=float, default=0.03,
                     help="imbalance bound of the balance stage in the learned comparison")
+    ap.add_argument("--batch-balance-eps", type=float, default=0.0,
+                    help="with --batched: run every graph with this balance bound "
+                         "(tgp_options.balance_eps; 0 = the reference's pipeline)")
     ap.add_argument("--learn-infer-every", type=int, default=4,
                     help="the GNN runs every this many learned passes (logits reused between)")
     ap.add_argument("--learn-margin", type=float, default=3.0,

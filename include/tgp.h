@@ -234,6 +234,18 @@ typedef struct tgp_batch_item {
 } tgp_batch_item;
 int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers);
 
+/* The balance stage of tgp_options.balance_eps alone, on a partition the
+ * caller holds (no counterpart in the reference): part[n] (host memory, in
+ * and out; values outside [0, k) count as part 0) is balanced to at most
+ * floor((1 + eps) * ceil(n / k)) vertices per part and refined within that
+ * bound exactly as the last step of tgp_partition with balance_eps does;
+ * part_count[k] and edge_cut receive the result's counts and cut.  Uses
+ * o->k (<= 1024), o->balance_eps (> 0), o->device_input, o->wide and
+ * o->check_level; the graph is g.  The batched form calls it for graphs its
+ * engine partitioned with a balance bound.                                  */
+int tgp_balance(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
+                int64_t* edge_cut, tgp_stats* stats);
+
 /* ---- partition without allocating (no counterpart in the reference) -------
  * For callers that cannot allocate device memory: code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc.  The same

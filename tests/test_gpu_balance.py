@@ -131,6 +131,55 @@ def test_balance_stage_matches_restatement(oracle, spec):
     assert rec and rec[0][3:8] == (rounds, nb, nr, cut, int(counts.max()))
 
 
+@pytest.mark.parametrize("spec", CASES[:3], ids=[f"{c[0]}{c[1]}-k{c[3]}" for c in CASES[:3]])
+def test_balance_entry_equals_stage_in_partition(oracle, spec):
+    """tgp_balance on the plain partition equals tgp_partition with the same
+    balance_eps: partition, counts, cut and the BALANCE trace record."""
+    kind, a1, a2, k = spec
+    g = tgp.gen_rmat(a1, a2, 0x7C0DE5EED + a1) if kind == "rmat" else tgp.gen_er(a1, a2, 7)
+    eps = 0.03
+    kw = dict(init=tgp.INIT_BLOCK, repair=True, coarsen_floor=100)
+    plain = tgp.partition(g, k, **kw)
+    full = tgp.partition(g, k, balance_eps=eps, trace_cap=1 << 14, **kw)
+    part, pc, cut, st = tgp.balance(g, plain[0], k, eps, trace_cap=64)
+    np.testing.assert_array_equal(part, full[0])
+    np.testing.assert_array_equal(pc, full[1])
+    assert cut == full[2]
+    rec = [t for t in st.trace if t[0] == 6]
+    frec = [t for t in full[3].trace if t[0] == 6]
+    assert rec and rec[0][3:] == frec[0][3:]
+    # a second call starts balanced (no balance round) and its refine passes
+    # move non-adjacent vertices by positive exact gains only: the cut cannot rise
+    p2, pc2, cut2, st2 = tgp.balance(g, part, k, eps, trace_cap=64)
+    rec2 = [t for t in st2.trace if t[0] == 6]
+    assert rec2[0][3] == 0 and rec2[0][4] == 0
+    assert cut2 <= cut and pc2.max() <= max(int(pc.max()), rec2[0][7])
+    assert cut2 == oracle.edge_cut(g.xadj, g.adjncy, g.adjwgt, g.m, p2, k)
+    # device-resident graph
+    dg = tgp.DeviceGraph(g)
+    try:
+        p3, _, cut3, _ = tgp.balance(g, plain[0], k, eps, device_graph=dg.pointers())
+    finally:
+        dg.free()
+    np.testing.assert_array_equal(p3, part)
+    assert cut3 == cut
+
+
+def test_balance_entry_rejects_bad_arguments():
+    g = tgp.gen_er(500, 6.0, 3)
+    part = np.zeros(g.n, np.int32)
+    with pytest.raises(tgp.TGPError, match="balance_eps > 0"):
+        tgp.balance(g, part, 4, 0.0)
+    with pytest.raises(tgp.TGPError, match="k in"):
+        tgp.balance(g, part, 2000, 0.03)
+    # out-of-range part values count as part 0 (as the stage's recount does)
+    bad = np.full(g.n, 7, np.int32)
+    ok = np.zeros(g.n, np.int32)
+    a = tgp.balance(g, bad, 4, 0.03)
+    b = tgp.balance(g, ok, 4, 0.03)
+    np.testing.assert_array_equal(a[0], b[0])
+
+
 def test_balance_at_evaluation_size():
     g = tgp.gen_rmat(24, 16, 0x7C0DE5EED + 1)
     k, eps = 64, 0.03

@@ -44,6 +44,27 @@ def test_batch_equals_solo(workers, engine, monkeypatch):
         np.testing.assert_array_equal(bpc, pc, err_msg=f"graph {i}")
 
 
+@pytest.mark.parametrize("engine", ["1", "0"])
+def test_batch_at_balance_bound_equals_solo(engine, monkeypatch):
+    """balance_eps in the batch: engine graphs get the balance stage after the
+    engine (tgp_balance on a worker stream), worker graphs inside their solo
+    run; both must equal the solo partition with the same bound."""
+    monkeypatch.setenv("TGP_BATCH_ENGINE", engine)
+    gs, ks = batch_graphs(12, 14, 20000)
+    eps = 0.03
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=4,
+                              balance_eps=eps)
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True,
+                                         balance_eps=eps)
+        bp, bpc, bcut, _ = out[i]
+        assert bcut == cut, f"graph {i} (n={g.n}, k={k})"
+        np.testing.assert_array_equal(bp, part, err_msg=f"graph {i}")
+        np.testing.assert_array_equal(bpc, pc, err_msg=f"graph {i}")
+        cap = int((1.0 + float(np.float32(eps))) * ((g.n + k - 1) // k))
+        assert bpc.max() <= cap
+
+
 def test_batch_matches_oracle(oracle):
     gs, ks = batch_graphs(6, 12, 8000)
     out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=3)

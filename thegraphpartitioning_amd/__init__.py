@@ -109,6 +109,8 @@ def lib():
     L.tgp_partition.restype = C.c_int
     L.tgp_partition_noalloc.argtypes = L.tgp_partition.argtypes
     L.tgp_partition_noalloc.restype = C.c_int
+    L.tgp_balance.argtypes = L.tgp_partition.argtypes
+    L.tgp_balance.restype = C.c_int
     L.tgp_strerror.argtypes = [C.c_int]
     L.tgp_strerror.restype = C.c_char_p
     L.tgp_last_error.restype = C.c_char_p
@@ -450,8 +452,44 @@ def partition(g, k, *, init=INIT_GPMETIS, init_part=None, wide=False, repair=Fal
     return part, pc, cut.value, Stats(st, trace)
 
 
+def balance(g, part, k, eps, *, wide=False, device_graph=None, trace_cap=0):
+    """The balance stage alone (tgp_balance) on partition `part` of g: returns
+    (part, part_count, edge_cut, Stats) with every part within
+    floor((1 + eps) * ceil(n / k)) vertices."""
+    L = lib()
+    o = COptions()
+    o.k = k
+    o.wide = int(bool(wide))
+    o.balance_eps = float(eps)
+    if device_graph is not None:
+        o.device_input = 1
+        cg = CGraph(g.n, g.m, device_graph[0], device_graph[1], device_graph[2])
+    else:
+        cg = g.cgraph()
+    p = np.array(part, dtype=np.int32)
+    pc = np.zeros(k, np.int32)
+    cut = C.c_int64()
+    st = CStats()
+    tbuf = None
+    if trace_cap:
+        tbuf = (CTrace * trace_cap)()
+        st.trace = C.cast(tbuf, C.POINTER(CTrace))
+        st.trace_cap = trace_cap
+    rc = L.tgp_balance(C.byref(cg), C.byref(o), p.ctypes.data, pc.ctypes.data,
+                       C.addressof(cut), C.byref(st))
+    trace = []
+    if tbuf is not None:
+        for i in range(min(st.trace_len, trace_cap)):
+            t = tbuf[i]
+            trace.append((t.kind, t.level, t.phase, t.pass_, t.a, t.b, t.c, t.d, t.hash))
+    if rc != 0:
+        raise TGPError(rc, L.tgp_last_error().decode())
+    return p, pc, cut.value, Stats(st, trace)
+
+
 def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, coarsen_floor=0,
-                    workers=8, device_graphs=None, profile=False, workdir=None):
+                    workers=8, device_graphs=None, profile=False, workdir=None,
+                    balance_eps=0.0):
     """Partitions many independent graphs through tgp_partition_batch.
 
     Each result equals what partition() returns for that graph alone (bit for
@@ -500,6 +538,7 @@ def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, co
         it.o.coarsen_floor = int(coarsen_floor)
         it.o.profile = int(bool(profile))
         it.o.initpart = fn
+        it.o.balance_eps = float(balance_eps)
         part = np.zeros(g.n, np.int32)
         pc = np.zeros(int(ks[i]), np.int32)
         st = CStats()

@@ -399,6 +399,9 @@ class Run {
   }
 
   int go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out);
+  void setup_level0(const tgp_graph* g);
+  // the balance stage alone on a given partition of g (tgp_balance)
+  int balance_only(const tgp_graph* g, i32* part_io, i32* pc_out, i64* cut_out);
 
  private:
   Context& C;
@@ -1409,8 +1412,28 @@ void Run::balance_finest(DBuf<i32>& part, DBuf<i32>& pc) {
           O.check_level ? hash_dev(part.p, n) : 0);
 }
 
-int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
+int Run::balance_only(const tgp_graph* g, i32* part_io, i32* pc_out, i64* cut_out) {
   const auto t_start = std::chrono::steady_clock::now();
+  const i64 n = g->n, m = g->m;
+  setup_level0(g);
+  check_symmetric(*L[0]);
+  DBuf<i32> part(&C.pool, n), pc(&C.pool, k);
+  HIP_TRY(hipMemcpyAsync(part.p, part_io, n * sizeof(i32), hipMemcpyHostToDevice, st()));
+  balance_finest(part, pc);
+  const i64 cut = compute_cut(*L[0], m, part.p);
+  HIP_TRY(hipMemcpyAsync(part_io, part.p, n * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(pc_out, pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  *cut_out = cut;
+  if (S) {
+    S->seconds_total =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    S->trace_len = tr.len;
+  }
+  return TGP_OK;
+}
+
+void Run::setup_level0(const tgp_graph* g) {
   const i64 n = g->n, m = g->m;
   scan_sc = DBuf<i64>(&C.pool, scan_scratch_elems(std::max<i64>(n, 2) * 2 + 64));
   small = DBuf<i64>(&C.pool, 16);
@@ -1439,7 +1462,12 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
   }
   L.push_back(std::move(l0));
   ecount.push_back(m);
+}
 
+int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
+  const auto t_start = std::chrono::steady_clock::now();
+  const i64 n = g->n, m = g->m;
+  setup_level0(g);
   coarsen();
   const int pass = (int)L.size() - 1;
 
@@ -1550,7 +1578,6 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
     const bool ok = it.o.repair && it.o.initpart == tgp_initpart_block && !it.o.profile &&
                     !it.o.pass_dump && !it.o.learned &&
                     !it.o.check_level && (!it.stats || !it.stats->trace) && it.o.k >= 1 &&
-                    !(it.o.balance_eps > 0.0f) &&
                     it.o.k <= BATCH_MAXK && it.g.n >= 2 && it.g.n <= INT_MAX && it.g.m > 0 &&
                     it.g.m <= INT_MAX && it.part && it.part_count &&
                     (it.o.device_input || validate_host_graph(&it.g) == TGP_OK);
@@ -1817,11 +1844,62 @@ int partition_on(Context& Cx, const tgp_graph* g, const tgp_options* o, int32_t*
   }
 }
 
+// The balance stage alone (tgp_balance) on context C.
+int balance_on(Context& Cx, const tgp_graph* g, const tgp_options* o, int32_t* part,
+               int32_t* part_count, int64_t* edge_cut, tgp_stats* stats) {
+  last_error().clear();
+  if (!g || !o || !part || !part_count || !edge_cut) {
+    set_error("null argument");
+    return TGP_EINVAL;
+  }
+  if (o->k <= 0 || o->k > 1024 || !(o->balance_eps > 0.0f)) {
+    set_error("tgp_balance needs k in [1, 1024] and balance_eps > 0");
+    return TGP_EINVAL;
+  }
+  if (!o->device_input) {
+    const int rc = validate_host_graph(g);
+    if (rc != TGP_OK) {
+      set_error("malformed CSR: need xadj[0]=0, non-decreasing offsets, xadj[n]=m>0, "
+                "columns in [0,n)");
+      return rc;
+    }
+  } else if (g->n <= 0 || g->m <= 0) {
+    set_error("empty graph");
+    return TGP_EINVAL;
+  }
+  if (stats) {
+    tgp_trace* tb = stats->trace;
+    const i64 tc = stats->trace_cap;
+    std::memset(stats, 0, sizeof(*stats));
+    stats->trace = tb;
+    stats->trace_cap = tc;
+  }
+  try {
+    Cx.prof.on = false;
+    int rc;
+    {
+      Run r(Cx, *o, stats);
+      rc = r.balance_only(g, part, part_count, edge_cut);
+    }
+    Cx.sync();
+    return rc;
+  } catch (const Fail& f) {
+    (void)hipStreamSynchronize(Cx.s);
+    return f.code;
+  }
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* tgp_last_error(void) { return last_error().c_str(); }
+
+int tgp_balance(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
+                int64_t* edge_cut, tgp_stats* stats) {
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  return balance_on(*default_context(), g, o, part, part_count, edge_cut, stats);
+}
 
 int tgp_partition(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
                   int64_t* edge_cut, tgp_stats* stats) {
@@ -1850,8 +1928,18 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
       return f.code;
     }
   }
-  const i64 nrest = (i64)rest.size();
-  const int W = (int)std::min<i64>(workers, std::max<i64>(nrest, 1));
+  // graphs the engine partitioned with a balance bound: the balance stage
+  // runs on the engine's result (which equals the solo plain result, so the
+  // outcome equals the solo run with the stage)
+  std::vector<i64> post;
+  {
+    std::vector<bool> in_rest((size_t)count, false);
+    for (i64 r : rest) in_rest[r] = true;
+    for (i64 i = 0; i < count; i++)
+      if (!in_rest[i] && items[i].o.balance_eps > 0.0f) post.push_back(i);
+  }
+  const i64 nrest = (i64)rest.size(), npost = (i64)post.size();
+  const int W = (int)std::min<i64>(workers, std::max<i64>(nrest + npost, 1));
   std::vector<Context*> ctx;
   try {
     for (int w = 0; w < W; w++) ctx.push_back(worker_context(w));
@@ -1862,9 +1950,14 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
   auto work = [&](Context* C) {
     for (;;) {
       const i64 r = next.fetch_add(1);
-      if (r >= nrest) break;
-      tgp_batch_item& it = items[rest[r]];
-      it.rc = partition_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
+      if (r >= nrest + npost) break;
+      if (r < nrest) {
+        tgp_batch_item& it = items[rest[r]];
+        it.rc = partition_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
+      } else {
+        tgp_batch_item& it = items[post[r - nrest]];
+        it.rc = balance_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
+      }
     }
   };
   std::vector<std::thread> th;
