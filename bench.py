@@ -229,6 +229,26 @@ def bench_batched(args):
                    "bit_identical_to_solo": f"{same}/{len(sample)}"}}), flush=True)
 
 
+def bench_batched_autotune(args):
+    """The batched engine's tuner on a seeded random sample of the batched
+    workload (both graph families): prints its measurements and writes the
+    tuning table to --tune-out."""
+    import numpy as np
+    import thegraphpartitioning_amd as tgp
+    gs, ks = batched_workload(args.batched, SEED)
+    pick = np.random.default_rng(SEED + 7).choice(len(gs), min(args.batched_autotune, len(gs)),
+                                                  replace=False)
+    idx = sorted(int(i) for i in pick)
+    sg, sk = [gs[i] for i in idx], [ks[i] for i in idx]
+    print(f"# tuning sample: {len(sg)} of {len(gs)} graphs, {sum(g.m for g in sg)} arcs",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    report, _ = tgp.batch_autotune(sg, sk, path=args.tune_out)
+    print(report, flush=True)
+    print(f"# tuner: {time.perf_counter() - t0:.1f}s; table written to {args.tune_out}",
+          file=sys.stderr, flush=True)
+
+
 def bench_hbm_scale(args):
     """The HBM-scale configuration: R-MAT scale 26, edge factor 16, symmetrized
     (int64 offsets, ~2.1G arcs), int32 weights in [1, 16], k = 256, wide mode,
@@ -440,6 +460,11 @@ def main():
                          "(wide mode) instead of R-MAT")
     ap.add_argument("--balance-eps", type=float, default=0.03,
                     help="imbalance bound of the balance stage in the learned comparison")
+    ap.add_argument("--batched-autotune", type=int, default=0, metavar="GRAPHS",
+                    help="with --batched: run the engine's tuner on a random sample of this "
+                         "many graphs of the workload and write its table")
+    ap.add_argument("--tune-out", default="batch_tune.txt",
+                    help="where --batched-autotune writes the tuning table")
     ap.add_argument("--batch-balance-eps", type=float, default=0.0,
                     help="with --batched: run every graph with this balance bound "
                          "(tgp_options.balance_eps; 0 = the reference's pipeline)")
@@ -460,6 +485,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.batched and args.batched_autotune:
+        bench_batched_autotune(args)
+        return
     if args.batched:
         bench_batched(args)
         return

@@ -36,7 +36,9 @@ enum {
   TGP_EPARTFILE = -5,   /* part file missing (gpu_part.cu:1163-1168)           */
   TGP_EHIP = -6,        /* HIP runtime error (replaces CUDA_CHECK, gpu_part.cu:40-47) */
   TGP_ENOMEM = -7,      /* device or host allocation failed                    */
-  TGP_EIO = -8          /* file I/O error (gpu_part.cu:786-800, 830-833, 1306) */
+  TGP_EIO = -8,         /* file I/O error (gpu_part.cu:786-800, 830-833, 1306) */
+  TGP_EDIFF = -9        /* tgp_batch_autotune: two engine settings gave
+                           different results (no counterpart in the reference) */
 };
 
 /* ---- graphs ---------------------------------------------------------------- */
@@ -233,6 +235,40 @@ typedef struct tgp_batch_item {
   int32_t reserved;
 } tgp_batch_item;
 int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers);
+
+/* Batched-engine tuning (no counterpart in the reference).  The engine's
+ * refinement launch runs with a number of lanes per light vertex (2, 4 or 8)
+ * and a sort tile held in LDS (2048, 4096 or 8192 entries), and each graph
+ * with a heavy-row threshold (rows longer than it are refined by a whole
+ * wave) chosen by its mean degree m/n.  None of them changes a result.  They
+ * come from the table file TGP_BATCH_TUNE names, else batch_tune.txt in the
+ * library's directory, else the built-in 4 lanes, 4096 entries, threshold
+ * 96; TGP_BATCH_SETTING="lanes,heavy,tile" sets one setting for every graph.
+ * Table lines ('#' starts a comment):
+ *   lanes <2|4|8>
+ *   sort_tile <2048|4096|8192>
+ *   heavy_default <threshold>
+ *   heavy <mean degree from> <mean degree below> <threshold>   (first match)
+ * A malformed table makes tgp_partition_batch return TGP_EINVAL with the
+ * file and line in tgp_last_error().
+ *
+ * tgp_batch_tuning: the setting the engine would use for a graph of this
+ * mean degree (reads the table as the engine does; needs no GPU).          */
+int tgp_batch_tuning(double mean_degree, int32_t* lanes, int32_t* heavy, int32_t* sort_tile);
+
+/* tgp_batch_autotune: measures the settings on the items' graphs (a sample
+ * of the workload; the items the engine takes, as in tgp_partition_batch)
+ * and writes the table to table_path (if not null).  One coarsening launch,
+ * then the refinement launch under every setting -- lanes {2, 4, 8} x
+ * thresholds {48, 96, 192, 384, 768, 1536} for each mean-degree class [0, 12), [12, 20),
+ * [20, inf), then tiles {2048, 4096, 8192} -- scored by the sum of the
+ * graphs' workgroup times.  Every setting's results must equal the first's
+ * (else TGP_EDIFF).  The items receive the chosen setting's results (rc
+ * TGP_OK; TGP_EINVAL for graphs the engine does not take).  *report (if not
+ * null) receives the measurements and the table as text; free it with
+ * tgp_free.                                                                  */
+int tgp_batch_autotune(tgp_batch_item* items, int64_t count, const char* table_path,
+                       char** report);
 
 /* The balance stage of tgp_options.balance_eps alone, on a partition the
  * caller holds (no counterpart in the reference): part[n] (host memory, in

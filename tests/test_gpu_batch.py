@@ -65,6 +65,61 @@ def test_batch_at_balance_bound_equals_solo(engine, monkeypatch):
         assert bpc.max() <= cap
 
 
+# lanes per light vertex, heavy-row threshold, sort tile: the tuner's range
+# and its ends (threshold 0: every row with an arc goes to a whole wave;
+# 10^6: none does)
+SETTINGS = [(2, 48, 2048), (8, 384, 8192), (4, 0, 4096), (2, 1000000, 8192), (8, 96, 2048)]
+
+
+@pytest.mark.parametrize("setting", SETTINGS, ids=["-".join(map(str, x)) for x in SETTINGS])
+def test_batch_settings_give_solo_results(setting, monkeypatch):
+    """every engine setting gives each graph its solo result, bit for bit"""
+    monkeypatch.setenv("TGP_BATCH_SETTING", ",".join(map(str, setting)))
+    gs, ks = batch_graphs(16, 16, 40000)
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=2)
+    monkeypatch.delenv("TGP_BATCH_SETTING")
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
+        assert out[i][2] == cut, f"graph {i} (n={g.n}, k={k})"
+        np.testing.assert_array_equal(out[i][0], part, err_msg=f"graph {i}")
+        np.testing.assert_array_equal(out[i][1], pc, err_msg=f"graph {i}")
+
+
+def test_autotune_table_read_by_engine(tmp_path, monkeypatch):
+    """the tuner measures every setting on a sample, finds the results
+    identical, writes a table; the engine then runs with that table and
+    still gives the solo results"""
+    monkeypatch.delenv("TGP_BATCH_SETTING", raising=False)
+    gs, ks = batch_graphs(24, 17, 30000)
+    path = str(tmp_path / "tune.txt")
+    report, res = tgp.batch_autotune(gs, ks, path=path)
+    print(report, flush=True)
+    assert "results identical in every setting" in report
+    assert report.count(" launch ") == 21
+    text = open(path).read()
+    assert text in report
+    monkeypatch.setenv("TGP_BATCH_TUNE", path)
+    lanes, heavy, tile = tgp.batch_tuning(10.0)
+    assert f"lanes {lanes}\n" in text and f"sort_tile {tile}\n" in text
+    out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=2)
+    for i, (g, k) in enumerate(zip(gs, ks)):
+        part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True)
+        assert res[i][3] == 0 and res[i][2] == cut, f"graph {i}"
+        np.testing.assert_array_equal(res[i][0], part, err_msg=f"graph {i}")
+        assert out[i][2] == cut, f"graph {i}"
+        np.testing.assert_array_equal(out[i][0], part, err_msg=f"graph {i}")
+        np.testing.assert_array_equal(out[i][1], pc, err_msg=f"graph {i}")
+
+
+def test_malformed_table_fails_the_batch(tmp_path, monkeypatch):
+    p = tmp_path / "bad.txt"
+    p.write_text("lanes 5\n")
+    monkeypatch.setenv("TGP_BATCH_TUNE", str(p))
+    gs, ks = batch_graphs(2, 18, 3000)
+    with pytest.raises(tgp.TGPError):
+        tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=1)
+
+
 def test_batch_matches_oracle(oracle):
     gs, ks = batch_graphs(6, 12, 8000)
     out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=3)

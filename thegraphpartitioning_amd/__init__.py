@@ -111,6 +111,10 @@ def lib():
     L.tgp_partition_noalloc.restype = C.c_int
     L.tgp_balance.argtypes = L.tgp_partition.argtypes
     L.tgp_balance.restype = C.c_int
+    L.tgp_batch_tuning.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tgp_batch_tuning.restype = C.c_int
+    L.tgp_batch_autotune.argtypes = [C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p]
+    L.tgp_batch_autotune.restype = C.c_int
     L.tgp_strerror.argtypes = [C.c_int]
     L.tgp_strerror.restype = C.c_char_p
     L.tgp_last_error.restype = C.c_char_p
@@ -487,22 +491,10 @@ def balance(g, part, k, eps, *, wide=False, device_graph=None, trace_cap=0):
     return p, pc, cut.value, Stats(st, trace)
 
 
-def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, coarsen_floor=0,
-                    workers=8, device_graphs=None, profile=False, workdir=None,
-                    balance_eps=0.0):
-    """Partitions many independent graphs through tgp_partition_batch.
-
-    Each result equals what partition() returns for that graph alone (bit for
-    bit); up to `workers` graphs run at once, each on its own HIP stream.
-    `init` is "block" or "gpmetis"; with "gpmetis" and a `workdir`, graph i
-    calls gpmetis in its own directory <workdir>/graph<i> (created), so the
-    calls run concurrently on host threads; without one they share the
-    working directory and run one at a time.  `device_graphs`, if given,
-    holds one (xadj, adjncy, adjwgt) triple of device pointers per graph.
-    Returns a list of (part, part_count, edge_cut, Stats); raises TGPError
-    naming the first failing graph.
-    """
-    L = lib()
+def _batch_items(L, graphs, ks, init, wide, repair, coarsen_floor, device_graphs, profile,
+                 workdir, balance_eps):
+    """tgp_batch_item array for graphs/ks, their output arrays, and the
+    objects the items point into (kept alive by the caller)."""
     cnt = len(graphs)
     if len(ks) != cnt:
         raise ValueError("one k per graph")
@@ -546,6 +538,62 @@ def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, co
         it.part_count = pc.ctypes.data
         it.stats = C.pointer(st)
         outs.append((part, pc, st))
+    return items, outs, dirs
+
+
+def batch_tuning(mean_degree):
+    """(lanes, heavy_threshold, sort_tile) the batched engine uses for a graph
+    of this mean degree (arcs / vertices), from its tuning table
+    (tgp_batch_tuning)."""
+    L = lib()
+    a, h, t = C.c_int32(), C.c_int32(), C.c_int32()
+    rc = L.tgp_batch_tuning(float(mean_degree), C.byref(a), C.byref(h), C.byref(t))
+    if rc != 0:
+        raise TGPError(rc, L.tgp_last_error().decode())
+    return a.value, h.value, t.value
+
+
+def batch_autotune(graphs, ks, *, path=None, device_graphs=None, wide=False, coarsen_floor=0):
+    """Measures the batched engine's settings on these graphs (a sample of the
+    workload; repaired coarsening, block initial partition) and writes the
+    tuning table to `path` if given (tgp_batch_autotune).  Returns (report,
+    results): the measurements and the table as text, and per graph (part,
+    part_count, edge_cut, rc) of the chosen setting."""
+    L = lib()
+    items, outs, _keep = _batch_items(L, graphs, ks, INIT_BLOCK, wide, True, coarsen_floor,
+                                      device_graphs, False, None, 0.0)
+    rep = C.c_void_p()
+    rc = L.tgp_batch_autotune(items, len(graphs), path.encode() if path else None,
+                              C.byref(rep))
+    text = ""
+    if rep.value:
+        text = C.string_at(rep.value).decode()
+        L.tgp_free(rep)
+    if rc != 0:
+        raise TGPError(rc, L.tgp_last_error().decode() + "\n" + text)
+    return text, [(part, pc, items[i].edge_cut, items[i].rc)
+                  for i, (part, pc, _st) in enumerate(outs)]
+
+
+def partition_batch(graphs, ks, *, init=INIT_BLOCK, wide=False, repair=False, coarsen_floor=0,
+                    workers=8, device_graphs=None, profile=False, workdir=None,
+                    balance_eps=0.0):
+    """Partitions many independent graphs through tgp_partition_batch.
+
+    Each result equals what partition() returns for that graph alone (bit for
+    bit); up to `workers` graphs run at once, each on its own HIP stream.
+    `init` is "block" or "gpmetis"; with "gpmetis" and a `workdir`, graph i
+    calls gpmetis in its own directory <workdir>/graph<i> (created), so the
+    calls run concurrently on host threads; without one they share the
+    working directory and run one at a time.  `device_graphs`, if given,
+    holds one (xadj, adjncy, adjwgt) triple of device pointers per graph.
+    Returns a list of (part, part_count, edge_cut, Stats); raises TGPError
+    naming the first failing graph.
+    """
+    L = lib()
+    cnt = len(graphs)
+    items, outs, _keep = _batch_items(L, graphs, ks, init, wide, repair, coarsen_floor,
+                                      device_graphs, profile, workdir, balance_eps)
     rc = L.tgp_partition_batch(items, cnt, int(workers))
     if rc != 0:
         bad = next(i for i in range(cnt) if items[i].rc != 0)

@@ -38,22 +38,26 @@ def _run(cmd, verbose):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    objs, steps = [], []
     for src in HIP_SOURCES:
         o = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        _run([HIPCC] + DEVICE + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", o], verbose)
+        steps.append([HIPCC] + DEVICE + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in GXX_SOURCES:
         o = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-        _run(["g++"] + COMMON + ["-fopenmp", "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        steps.append(["g++"] + COMMON + ["-fopenmp", "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    tmp = LIB + ".tmp"
-    _run([HIPCC] + DEVICE + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"], verbose)
-    os.replace(tmp, LIB)
     main_o = os.path.join(OBJ, "gpu_part_main.o")
-    _run([HIPCC] + COMMON + ["-x", "hip", "--offload-arch=" + ARCH, "-c",
-                             os.path.join(CSRC, "gpu_part_main.cpp"), "-o", main_o], verbose)
+    steps.append([HIPCC] + COMMON + ["-x", "hip", "--offload-arch=" + ARCH, "-c",
+                                     os.path.join(CSRC, "gpu_part_main.cpp"), "-o", main_o])
+    # the compile steps are independent: run them side by side
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(len(steps), os.cpu_count() or 1, 8)) as ex:
+        list(ex.map(lambda c: _run(c, verbose), steps))
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + DEVICE + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp", "-ldl"], verbose)
+    os.replace(tmp, LIB)
     tmp = CLI + ".tmp"
     _run([HIPCC, "-o", tmp, main_o, "-L" + PKG, "-ltgp", "-Wl,-rpath,$ORIGIN", "-lgomp"], verbose)
     os.replace(tmp, CLI)

@@ -21,6 +21,8 @@
 #include <mutex>
 #include <thread>
 #include <atomic>
+#include <algorithm>
+#include <dlfcn.h>
 #include <string>
 #include <vector>
 
@@ -1568,117 +1570,356 @@ Context* default_context() {
   return ctx;
 }
 
-// The batched engine over the items it can take (repaired coarsening, block
-// initial partition, k <= BATCH_MAXK, no profiling or trace); the others,
-// and every graph the engine hands back, are appended to `rest`.
-void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>& rest) {
-  std::vector<i64> take;
+// ------------------------------------------------------------ batch tuning --
+// Settings of the batched engine: lanes per light vertex and the sort tile of
+// the refinement launch, and each graph's heavy-row threshold by its mean
+// degree (arcs / vertices).  Read from the table TGP_BATCH_TUNE names, else
+// batch_tune.txt beside the library, else built in (4 lanes, 4096-entry tile,
+// threshold 96).  TGP_BATCH_SETTING="lanes,heavy,tile" overrides all of it
+// with one setting for every graph.  Every setting gives the same results.
+struct BatchTuning {
+  int g = 4, stile = 4096, heavy = 96;
+  struct Cls {
+    double lo, hi;
+    int heavy;
+  };
+  std::vector<Cls> cls;
+  std::string source = "built-in";
+  int heavy_for(i64 n, i64 m) const {
+    const double d = n > 0 ? (double)m / (double)n : 0.0;
+    for (const Cls& c : cls)
+      if (d >= c.lo && d < c.hi) return c.heavy;
+    return heavy;
+  }
+};
+
+bool valid_lanes(int g) { return g == 2 || g == 4 || g == 8; }
+bool valid_tile(int t) { return t == 2048 || t == 4096 || t == 8192; }
+
+std::string library_dir() {
+  Dl_info info{};
+  if (dladdr((void*)&tgp_batch_tuning, &info) && info.dli_fname) {
+    const std::string p = info.dli_fname;
+    const size_t sl = p.rfind('/');
+    return sl == std::string::npos ? std::string(".") : p.substr(0, sl);
+  }
+  return ".";
+}
+
+// Table lines ('#' starts a comment):
+//   lanes <2|4|8>
+//   sort_tile <2048|4096|8192>
+//   heavy_default <threshold>
+//   heavy <mean degree from> <mean degree below> <threshold>
+bool parse_tuning(const std::string& path, BatchTuning* t) {
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) {
+    set_error("cannot open the batch tuning table " + path);
+    return false;
+  }
+  char line[512];
+  int ln = 0;
+  std::string why;
+  while (why.empty() && std::fgets(line, sizeof(line), f)) {
+    ln++;
+    if (char* h = std::strchr(line, '#')) *h = 0;
+    char key[32] = {0};
+    if (std::sscanf(line, "%31s", key) != 1) continue;  // blank
+    const std::string k = key;
+    int a = 0;
+    double lo = 0, hi = 0;
+    char extra[8];
+    if (k == "lanes") {
+      if (std::sscanf(line, "%*s %d %7s", &a, extra) != 1 || !valid_lanes(a)) why = "lanes takes 2, 4 or 8";
+      else t->g = a;
+    } else if (k == "sort_tile") {
+      if (std::sscanf(line, "%*s %d %7s", &a, extra) != 1 || !valid_tile(a))
+        why = "sort_tile takes 2048, 4096 or 8192";
+      else t->stile = a;
+    } else if (k == "heavy_default") {
+      if (std::sscanf(line, "%*s %d %7s", &a, extra) != 1 || a < 0) why = "heavy_default takes a threshold >= 0";
+      else t->heavy = a;
+    } else if (k == "heavy") {
+      if (std::sscanf(line, "%*s %lf %lf %d %7s", &lo, &hi, &a, extra) != 3 || !(lo < hi) ||
+          lo < 0 || a < 0)
+        why = "heavy takes <degree from> <degree below> <threshold >= 0> with from < below";
+      else t->cls.push_back({lo, hi, a});
+    } else {
+      why = "unknown entry '" + k + "'";
+    }
+  }
+  std::fclose(f);
+  if (!why.empty()) {
+    set_error("batch tuning table " + path + ", line " + std::to_string(ln) + ": " + why);
+    return false;
+  }
+  return true;
+}
+
+int load_batch_tuning(BatchTuning* t) {
+  *t = BatchTuning{};
+  std::string path;
+  const char* env = std::getenv("TGP_BATCH_TUNE");
+  if (env && *env) {
+    path = env;
+  } else {
+    const std::string p = library_dir() + "/batch_tune.txt";
+    if (FILE* f = std::fopen(p.c_str(), "r")) {
+      std::fclose(f);
+      path = p;
+    }
+  }
+  if (!path.empty()) {
+    if (!parse_tuning(path, t)) return TGP_EINVAL;
+    t->source = path;
+  }
+  const char* set = std::getenv("TGP_BATCH_SETTING");
+  if (set && *set) {
+    int g = 0, h = 0, st = 0;
+    char extra[8];
+    if (std::sscanf(set, "%d,%d,%d%7s", &g, &h, &st, extra) != 3 || !valid_lanes(g) ||
+        !valid_tile(st) || h < 0) {
+      set_error("TGP_BATCH_SETTING takes lanes,heavy,tile with lanes 2, 4 or 8, heavy >= 0 and "
+                "tile 2048, 4096 or 8192");
+      return TGP_EINVAL;
+    }
+    t->g = g;
+    t->heavy = h;
+    t->stile = st;
+    t->cls.clear();
+    t->source = "TGP_BATCH_SETTING";
+  }
+  return TGP_OK;
+}
+
+// Whether the batched engine takes the item (repaired coarsening, block
+// initial partition, k <= BATCH_MAXK, no profiling or trace).
+bool engine_takes(const tgp_batch_item& it) {
+  return it.o.repair && it.o.initpart == tgp_initpart_block && !it.o.profile &&
+         !it.o.pass_dump && !it.o.learned && !it.o.check_level &&
+         (!it.stats || !it.stats->trace) && it.o.k >= 1 && it.o.k <= BATCH_MAXK &&
+         it.g.n >= 2 && it.g.n <= INT_MAX && it.g.m > 0 && it.g.m <= INT_MAX && it.part &&
+         it.part_count && (it.o.device_input || validate_host_graph(&it.g) == TGP_OK);
+}
+
+// The batched engine over a set of items: their graphs uploaded (when held
+// in host memory), per-graph output slots, workgroup scratch and the level
+// pool; then the coarsening and refinement launches, and the results.
+struct BatchEngine {
+  Context& C;
+  tgp_batch_item* items;
+  std::vector<i64> take;  // item index of each engine graph
+  i64 T = 0;
+  DBuf<u8> up;
+  DBuf<i32> parts, pcs, status, order;
+  DBuf<i64> cuts;
+  DBuf<BatchGraph> dg;
+  std::vector<BatchGraph> hg;
+  std::vector<i64> part_off;
+  std::vector<i32> ord;
+  BatchBuffers B{};
+  void* mem = nullptr;
+  i64 budget = 0;
+
+  BatchEngine(Context& Cx, tgp_batch_item* it) : C(Cx), items(it) {}
+  ~BatchEngine() {
+    if (mem) (void)hipFree(mem);
+  }
+
+  // false when there is no memory for the level pool
+  bool setup(const BatchTuning& tun) {
+    T = (i64)take.size();
+    i64 maxn = 0, sum_n = 0, up_bytes = 0;
+    for (i64 i : take) {
+      const tgp_graph& g = items[i].g;
+      maxn = std::max<i64>(maxn, g.n);
+      sum_n += g.n;
+      if (!items[i].o.device_input) up_bytes += 8 * (g.n + 1) + 8 * g.m + 64;
+    }
+    // inputs held in host memory go up once, into one buffer
+    hg.assign((size_t)T, BatchGraph{});
+    if (up_bytes) up = DBuf<u8>(&C.pool, (size_t)up_bytes);
+    i64 uo = 0;
+    parts = DBuf<i32>(&C.pool, (size_t)sum_n);
+    pcs = DBuf<i32>(&C.pool, (size_t)T * BATCH_MAXK);
+    status = DBuf<i32>(&C.pool, (size_t)T);
+    order = DBuf<i32>(&C.pool, (size_t)T);
+    cuts = DBuf<i64>(&C.pool, (size_t)T);
+    dg = DBuf<BatchGraph>(&C.pool, (size_t)T);
+    part_off.assign((size_t)T, 0);
+    i64 po = 0;
+    for (i64 j = 0; j < T; j++) {
+      const tgp_batch_item& it = items[take[j]];
+      BatchGraph& b = hg[j];
+      b.n = it.g.n;
+      b.m = it.g.m;
+      if (it.o.device_input) {
+        b.xadj = it.g.xadj;
+        b.adj = it.g.adjncy;
+        b.w = it.g.adjwgt;
+      } else {
+        u8* base = up.p + uo;
+        HIP_TRY(hipMemcpyAsync(base, it.g.xadj, 8 * (b.n + 1), hipMemcpyHostToDevice, C.s));
+        HIP_TRY(hipMemcpyAsync(base + 8 * (b.n + 1), it.g.adjncy, 4 * b.m,
+                               hipMemcpyHostToDevice, C.s));
+        HIP_TRY(hipMemcpyAsync(base + 8 * (b.n + 1) + 4 * b.m, it.g.adjwgt, 4 * b.m,
+                               hipMemcpyHostToDevice, C.s));
+        b.xadj = (const i64*)base;
+        b.adj = (const i32*)(base + 8 * (b.n + 1));
+        b.w = (const i32*)(base + 8 * (b.n + 1) + 4 * b.m);
+        uo += 8 * (b.n + 1) + 8 * b.m + 64;
+        uo = (uo + 255) & ~(i64)255;
+      }
+      b.k = it.o.k;
+      b.wide = it.o.wide;
+      b.target = it.o.coarsen_floor > 0 ? it.o.coarsen_floor : std::max<i64>(500, b.n / 500);
+      part_off[j] = po;
+      b.part_out = parts.p + po;
+      po += b.n;
+      b.pc_out = pcs.p + j * BATCH_MAXK;
+      b.cut_out = cuts.p + j;
+      b.status = status.p + j;
+      b.heavy = tun.heavy_for(b.n, b.m);
+      b.ticks = nullptr;
+    }
+    // largest graphs first, so the long ones start early
+    ord.resize((size_t)T);
+    for (i64 j = 0; j < T; j++) ord[j] = (i32)j;
+    std::stable_sort(ord.begin(), ord.end(), [&](i32 a, i32 b) { return hg[a].m > hg[b].m; });
+    HIP_TRY(hipMemcpyAsync(dg.p, hg.data(), sizeof(BatchGraph) * T, hipMemcpyHostToDevice, C.s));
+    HIP_TRY(hipMemcpyAsync(order.p, ord.data(), sizeof(i32) * T, hipMemcpyHostToDevice, C.s));
+    launch_fill_i32(status.p, T, 1, C.s);
+    // Coarsening: one workgroup per CU; refinement: two per CU; each with
+    // scratch for the largest graph.  The coarse levels of the graphs share
+    // a pool (up to 70% of the free memory); graphs that find it full wait
+    // for the next round.
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
+    cus = std::max(cus, 1);
+    B.maxn = maxn;
+    B.stile = tun.stile;
+    B.g = tun.g;
+    B.blocks_c = (int)std::min<i64>(T, cus);
+    B.blocks_r = (int)std::min<i64>(T, 2 * (i64)cus);
+    const i64 bc = batch_coarsen_bytes(maxn), br = batch_refine_bytes(maxn);
+    i64 want = 1 << 24;
+    for (i64 i : take) want += 6 * (8 * items[i].g.m + 16 * (items[i].g.n + 1));
+    size_t freeb = 0, totb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totb));
+    const i64 fixed = bc * B.blocks_c + br * B.blocks_r + batch_record_bytes() * T + 4096;
+    B.pool_cap = std::min<i64>(want, (i64)((double)freeb * 0.7) - fixed);
+    // test knob: a smaller level pool, so graphs find it full and go again
+    // in later rounds
+    const i64 pool_limit = env_i64("TGP_BATCH_POOL_BYTES", 0);
+    if (pool_limit > 0) B.pool_cap = std::min<i64>(B.pool_cap, pool_limit);
+    if (B.pool_cap <= 0 || hipMalloc(&mem, (size_t)(fixed + B.pool_cap)) != hipSuccess) {
+      (void)hipGetLastError();
+      mem = nullptr;
+      return false;
+    }
+    u8* m8 = (u8*)mem;
+    B.next = (u32*)m8;
+    B.pool_used = (u64*)(m8 + 64);
+    B.recs = m8 + 4096;
+    B.scratch_c = B.recs + ((batch_record_bytes() * T + 255) & ~(i64)255);
+    B.scratch_r = B.scratch_c + bc * B.blocks_c;
+    B.pool = m8 + fixed;
+    budget = C.clock_khz * 2000;  // 2 s per level's matching
+    return true;
+  }
+
+  // the graphs' statuses after the launches
+  std::vector<i32> statuses() {
+    std::vector<i32> h((size_t)T);
+    HIP_TRY(hipMemcpyAsync(h.data(), status.p, sizeof(i32) * T, hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    return h;
+  }
+
+  // Coarsening and refinement rounds until every graph is done or handed
+  // back (status 1); returns the number of rounds.
+  int run(u64* bst) {
+    std::vector<i32> hstat;
+    std::vector<i32> queue = ord;
+    int rounds = 0;
+    while (!queue.empty()) {
+      rounds++;
+      HIP_TRY(hipMemsetAsync(mem, 0, 4096, C.s));  // queue counters, pool offset
+      if (rounds > 1)
+        HIP_TRY(hipMemcpyAsync(order.p, queue.data(), sizeof(i32) * queue.size(),
+                               hipMemcpyHostToDevice, C.s));
+      launch_batch_engine(dg.p, order.p, (i32)queue.size(), B, budget, bst, C.s);
+      hstat = statuses();
+      // graphs that found the pool full go again, with the pool to themselves
+      std::vector<i32> again;
+      for (i32 j : queue)
+        if (hstat[j] == 3) again.push_back(j);
+      for (i32 j : again) hstat[j] = 1;
+      if (again.size() == queue.size()) break;  // no progress: the pool is too small
+      queue.swap(again);
+      if (!queue.empty()) {
+        HIP_TRY(hipMemcpyAsync(status.p, hstat.data(), sizeof(i32) * T, hipMemcpyHostToDevice,
+                               C.s));
+        C.sync();
+      }
+    }
+    HIP_TRY(hipMemcpyAsync(status.p, hstat.data(), sizeof(i32) * T, hipMemcpyHostToDevice, C.s));
+    C.sync();
+    return rounds;
+  }
+
+  // Results of the finished graphs into their items; the others are
+  // appended to rest.
+  void collect(std::vector<i64>& rest) {
+    const std::vector<i32> hstat = statuses();
+    std::vector<i32> hpc((size_t)T * BATCH_MAXK);
+    std::vector<i64> hcut((size_t)T);
+    HIP_TRY(hipMemcpyAsync(hpc.data(), pcs.p, sizeof(i32) * T * BATCH_MAXK,
+                           hipMemcpyDeviceToHost, C.s));
+    HIP_TRY(hipMemcpyAsync(hcut.data(), cuts.p, sizeof(i64) * T, hipMemcpyDeviceToHost, C.s));
+    for (i64 j = 0; j < T; j++) {
+      if (hstat[j] != 0) continue;
+      tgp_batch_item& it = items[take[j]];
+      HIP_TRY(hipMemcpyAsync(it.part, parts.p + part_off[j], sizeof(i32) * it.g.n,
+                             hipMemcpyDeviceToHost, C.s));
+    }
+    C.sync();
+    for (i64 j = 0; j < T; j++) {
+      tgp_batch_item& it = items[take[j]];
+      if (hstat[j] != 0) {
+        rest.push_back(take[j]);
+        continue;
+      }
+      std::memcpy(it.part_count, hpc.data() + j * BATCH_MAXK, sizeof(i32) * it.o.k);
+      it.edge_cut = hcut[j];
+      it.rc = TGP_OK;
+      if (it.stats) {
+        tgp_trace* tb = it.stats->trace;
+        const i64 tc = it.stats->trace_cap;
+        std::memset(it.stats, 0, sizeof(*it.stats));
+        it.stats->trace = tb;
+        it.stats->trace_cap = tc;
+      }
+    }
+  }
+};
+
+// The batched engine over the items it takes; the others, and every graph
+// the engine hands back, are appended to `rest`.
+int engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>& rest) {
+  BatchTuning tun;
+  const int trc = load_batch_tuning(&tun);
+  if (trc != TGP_OK) return trc;
+  BatchEngine E(C, items);
   for (i64 i = 0; i < count; i++) {
-    const tgp_batch_item& it = items[i];
-    const bool ok = it.o.repair && it.o.initpart == tgp_initpart_block && !it.o.profile &&
-                    !it.o.pass_dump && !it.o.learned &&
-                    !it.o.check_level && (!it.stats || !it.stats->trace) && it.o.k >= 1 &&
-                    it.o.k <= BATCH_MAXK && it.g.n >= 2 && it.g.n <= INT_MAX && it.g.m > 0 &&
-                    it.g.m <= INT_MAX && it.part && it.part_count &&
-                    (it.o.device_input || validate_host_graph(&it.g) == TGP_OK);
-    if (ok) take.push_back(i);
+    if (engine_takes(items[i])) E.take.push_back(i);
     else rest.push_back(i);
   }
-  if (take.empty()) return;
-  const i64 T = (i64)take.size();
-  i64 maxn = 0, sum_n = 0, up_bytes = 0;
-  for (i64 i : take) {
-    const tgp_graph& g = items[i].g;
-    maxn = std::max<i64>(maxn, g.n);
-    sum_n += g.n;
-    if (!items[i].o.device_input) up_bytes += 8 * (g.n + 1) + 8 * g.m + 64;
+  if (E.take.empty()) return TGP_OK;
+  if (!E.setup(tun)) {
+    for (i64 i : E.take) rest.push_back(i);
+    return TGP_OK;
   }
-  // inputs held in host memory go up once, into one buffer
-  DBuf<u8> up;
-  std::vector<BatchGraph> hg((size_t)T);
-  if (up_bytes) up = DBuf<u8>(&C.pool, (size_t)up_bytes);
-  i64 uo = 0;
-  DBuf<i32> parts(&C.pool, (size_t)sum_n), pcs(&C.pool, (size_t)T * BATCH_MAXK),
-      status(&C.pool, (size_t)T), order(&C.pool, (size_t)T);
-  DBuf<i64> cuts(&C.pool, (size_t)T);
-  DBuf<BatchGraph> dg(&C.pool, (size_t)T);
-  std::vector<i64> part_off((size_t)T);
-  i64 po = 0;
-  for (i64 j = 0; j < T; j++) {
-    const tgp_batch_item& it = items[take[j]];
-    BatchGraph& b = hg[j];
-    b.n = it.g.n;
-    b.m = it.g.m;
-    if (it.o.device_input) {
-      b.xadj = it.g.xadj;
-      b.adj = it.g.adjncy;
-      b.w = it.g.adjwgt;
-    } else {
-      u8* base = up.p + uo;
-      HIP_TRY(hipMemcpyAsync(base, it.g.xadj, 8 * (b.n + 1), hipMemcpyHostToDevice, C.s));
-      HIP_TRY(hipMemcpyAsync(base + 8 * (b.n + 1), it.g.adjncy, 4 * b.m, hipMemcpyHostToDevice,
-                             C.s));
-      HIP_TRY(hipMemcpyAsync(base + 8 * (b.n + 1) + 4 * b.m, it.g.adjwgt, 4 * b.m,
-                             hipMemcpyHostToDevice, C.s));
-      b.xadj = (const i64*)base;
-      b.adj = (const i32*)(base + 8 * (b.n + 1));
-      b.w = (const i32*)(base + 8 * (b.n + 1) + 4 * b.m);
-      uo += 8 * (b.n + 1) + 8 * b.m + 64;
-      uo = (uo + 255) & ~(i64)255;
-    }
-    b.k = it.o.k;
-    b.wide = it.o.wide;
-    b.target = it.o.coarsen_floor > 0 ? it.o.coarsen_floor : std::max<i64>(500, b.n / 500);
-    part_off[j] = po;
-    b.part_out = parts.p + po;
-    po += b.n;
-    b.pc_out = pcs.p + j * BATCH_MAXK;
-    b.cut_out = cuts.p + j;
-    b.status = status.p + j;
-  }
-  // largest graphs first, so the long ones start early
-  std::vector<i32> ord((size_t)T);
-  for (i64 j = 0; j < T; j++) ord[j] = (i32)j;
-  std::stable_sort(ord.begin(), ord.end(), [&](i32 a, i32 b) { return hg[a].m > hg[b].m; });
-  HIP_TRY(hipMemcpyAsync(dg.p, hg.data(), sizeof(BatchGraph) * T, hipMemcpyHostToDevice, C.s));
-  HIP_TRY(hipMemcpyAsync(order.p, ord.data(), sizeof(i32) * T, hipMemcpyHostToDevice, C.s));
-  launch_fill_i32(status.p, T, 1, C.s);
-  // Coarsening: one workgroup per CU; refinement: two per CU; each with
-  // scratch for the largest graph.  The coarse levels of the graphs share a
-  // pool (up to 70% of the free memory); graphs that find it full wait for
-  // the next round.
-  int cus = 0;
-  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
-  cus = std::max(cus, 1);
-  BatchBuffers B{};
-  B.maxn = maxn;
-  B.blocks_c = (int)std::min<i64>(T, cus);
-  B.blocks_r = (int)std::min<i64>(T, 2 * (i64)cus);
-  const i64 bc = batch_coarsen_bytes(maxn), br = batch_refine_bytes(maxn);
-  i64 want = 1 << 24;
-  for (i64 i : take) want += 6 * (8 * items[i].g.m + 16 * (items[i].g.n + 1));
-  size_t freeb = 0, totb = 0;
-  HIP_TRY(hipMemGetInfo(&freeb, &totb));
-  const i64 fixed = bc * B.blocks_c + br * B.blocks_r + batch_record_bytes() * T + 4096;
-  B.pool_cap = std::min<i64>(want, (i64)((double)freeb * 0.7) - fixed);
-  // test knob: a smaller level pool, so graphs find it full and go again in
-  // later rounds
-  const i64 pool_limit = env_i64("TGP_BATCH_POOL_BYTES", 0);
-  if (pool_limit > 0) B.pool_cap = std::min<i64>(B.pool_cap, pool_limit);
-  void* mem = nullptr;
-  if (B.pool_cap <= 0 || hipMalloc(&mem, (size_t)(fixed + B.pool_cap)) != hipSuccess) {
-    (void)hipGetLastError();
-    for (i64 i : take) rest.push_back(i);
-    return;
-  }
-  u8* m8 = (u8*)mem;
-  B.next = (u32*)m8;
-  B.pool_used = (u64*)(m8 + 64);
-  B.recs = m8 + 4096;
-  B.scratch_c = B.recs + ((batch_record_bytes() * T + 255) & ~(i64)255);
-  B.scratch_r = B.scratch_c + bc * B.blocks_c;
-  B.pool = m8 + fixed;
-  const i64 budget = C.clock_khz * 2000;  // 2 s per level's matching
   const bool trace = env_i64("TGP_BATCH_TRACE", 0) != 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DBuf<u64> bst;
@@ -1689,81 +1930,238 @@ void engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, C.s));
   }
-  std::vector<i32> hstat((size_t)T), hpc((size_t)T * BATCH_MAXK);
-  std::vector<i32> queue = ord;
-  int rounds = 0;
-  while (!queue.empty()) {
-    rounds++;
-    HIP_TRY(hipMemsetAsync(m8, 0, 4096, C.s));  // queue counters, pool offset
-    if (rounds > 1)
-      HIP_TRY(hipMemcpyAsync(order.p, queue.data(), sizeof(i32) * queue.size(),
-                             hipMemcpyHostToDevice, C.s));
-    launch_batch_engine(dg.p, order.p, (i32)queue.size(), B, budget, bst.p, C.s);
-    HIP_TRY(hipMemcpyAsync(hstat.data(), status.p, sizeof(i32) * T, hipMemcpyDeviceToHost, C.s));
-    C.sync();
-    // graphs that found the pool full go again, with the pool to themselves
-    std::vector<i32> again;
-    for (i32 j : queue)
-      if (hstat[j] == 3) again.push_back(j);
-    for (i32 j : again) hstat[j] = 1;
-    if (again.size() == queue.size()) break;  // no progress: the pool is too small
-    queue.swap(again);
-    if (!queue.empty()) {
-      HIP_TRY(hipMemcpyAsync(status.p, hstat.data(), sizeof(i32) * T, hipMemcpyHostToDevice,
-                             C.s));
-      C.sync();
-    }
-  }
+  const int rounds = E.run(bst.p);
   if (trace) HIP_TRY(hipEventRecord(e1, C.s));
-  std::vector<i64> hcut((size_t)T);
-  HIP_TRY(hipMemcpyAsync(hpc.data(), pcs.p, sizeof(i32) * T * BATCH_MAXK, hipMemcpyDeviceToHost,
-                         C.s));
-  HIP_TRY(hipMemcpyAsync(hcut.data(), cuts.p, sizeof(i64) * T, hipMemcpyDeviceToHost, C.s));
-  for (i64 j = 0; j < T; j++) {
-    tgp_batch_item& it = items[take[j]];
-    HIP_TRY(hipMemcpyAsync(it.part, parts.p + part_off[j], sizeof(i32) * it.g.n,
-                           hipMemcpyDeviceToHost, C.s));
-  }
-  C.sync();
-  (void)hipFree(mem);
+  const std::vector<i32> hstat = E.statuses();
+  E.collect(rest);
   if (trace) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     i64 back = 0;
-    for (i64 j = 0; j < T; j++) back += hstat[j] != 0;
+    for (i64 j = 0; j < E.T; j++) back += hstat[j] != 0;
     std::fprintf(stderr,
                  "# batch engine: %lld graphs, %lld handed back, %d + %d workgroups, "
-                 "level pool %.1f GB, %d rounds, %.2f ms\n",
-                 (long long)T, (long long)back, B.blocks_c, B.blocks_r, B.pool_cap / 1e9, rounds,
-                 ms);
+                 "level pool %.1f GB, %d rounds, %.2f ms (lanes %d, sort tile %d, tuning %s)\n",
+                 (long long)E.T, (long long)back, E.B.blocks_c, E.B.blocks_r,
+                 E.B.pool_cap / 1e9, rounds, ms, E.B.g, E.B.stile, tun.source.c_str());
     u64 hb[12];
     HIP_TRY(hipMemcpy(hb, bst.p, sizeof(hb), hipMemcpyDeviceToHost));
     const char* nm[10] = {"check", "match", "contract", "gain", "token",
                           "moves", "sort",  "select",   "cut",  "misc"};
     std::fprintf(stderr, "# batch engine phases (workgroup-ms):");
-    for (int q = 0; q < 10; q++) std::fprintf(stderr, " %s=%.1f", nm[q], hb[q] / (double)C.clock_khz);
+    for (int q = 0; q < 10; q++)
+      std::fprintf(stderr, " %s=%.1f", nm[q], hb[q] / (double)C.clock_khz);
     std::fprintf(stderr, " passes=%llu levels=%llu\n", (unsigned long long)hb[10],
                  (unsigned long long)hb[11]);
   }
-  for (i64 j = 0; j < T; j++) {
-    tgp_batch_item& it = items[take[j]];
-    if (hstat[j] != 0) {
-      rest.push_back(take[j]);
+  return TGP_OK;
+}
+
+// The tuner: one coarsening launch over the sample, then the refinement
+// launch again under each setting, timing every graph (wall-clock ticks of
+// its workgroup) and checking that every setting gives the first one's
+// results.  Settings: lanes {2, 4, 8} x heavy-row thresholds {48, 96, 192,
+// 384, 768, 1536} at the 4096-entry tile, the threshold chosen per mean-degree class
+// ([0, 12), [12, 20), [20, inf)); then the tiles {2048, 4096, 8192} at the
+// chosen lanes and thresholds.  The sum of the graphs' workgroup times is
+// the measure (the persistent launch's length follows it when the queue
+// keeps every workgroup busy).
+int autotune_on(Context& C, tgp_batch_item* items, i64 count, const char* path,
+                std::string& rep) {
+  BatchEngine E(C, items);
+  for (i64 i = 0; i < count; i++)
+    if (engine_takes(items[i])) E.take.push_back(i);
+    else items[i].rc = TGP_EINVAL;
+  if (E.take.empty()) {
+    set_error("autotune: no graph within the batched engine's scope (repaired coarsening, "
+              "block initial partition, k <= 16)");
+    return TGP_EINVAL;
+  }
+  const BatchTuning base;
+  if (!E.setup(base)) {
+    set_error("autotune: no device memory for the level pool");
+    return TGP_ENOMEM;
+  }
+  HIP_TRY(hipMemsetAsync(E.mem, 0, 4096, C.s));
+  launch_batch_coarsen(E.dg.p, E.order.p, (i32)E.T, E.B, E.budget, nullptr, C.s);
+  const std::vector<i32> st0 = E.statuses();
+  static const double CLS_LO[3] = {0.0, 12.0, 20.0}, CLS_HI[3] = {12.0, 20.0, INFINITY};
+  std::vector<int> cls((size_t)E.T, -1);
+  i64 ncls[3] = {0, 0, 0}, ntuned = 0;
+  for (i64 j = 0; j < E.T; j++) {
+    if (st0[j] != 2) {
+      items[E.take[j]].rc = TGP_EINVAL;  // outside the engine's scope, or the pool was full
       continue;
     }
-    std::memcpy(it.part_count, hpc.data() + j * BATCH_MAXK, sizeof(i32) * it.o.k);
-    it.edge_cut = hcut[j];
-    it.rc = TGP_OK;
-    if (it.stats) {
-      tgp_trace* tb = it.stats->trace;
-      const i64 tc = it.stats->trace_cap;
-      std::memset(it.stats, 0, sizeof(*it.stats));
-      it.stats->trace = tb;
-      it.stats->trace_cap = tc;
+    const double d = (double)E.hg[j].m / (double)E.hg[j].n;
+    cls[j] = d < CLS_HI[0] ? 0 : d < CLS_HI[1] ? 1 : 2;
+    ncls[cls[j]]++;
+    ntuned++;
+  }
+  if (!ntuned) {
+    set_error("autotune: no graph of the sample was coarsened by the engine");
+    return TGP_EINVAL;
+  }
+  DBuf<i32> status0(&C.pool, (size_t)E.T);
+  DBuf<u64> ticks(&C.pool, (size_t)E.T);
+  HIP_TRY(hipMemcpyAsync(status0.p, E.status.p, sizeof(i32) * E.T, hipMemcpyDeviceToDevice, C.s));
+  for (i64 j = 0; j < E.T; j++) E.hg[j].ticks = ticks.p + j;
+  const i64 sum_n = E.part_off.back() + E.hg.back().n;
+  std::vector<i32> ref_part, cur_part((size_t)sum_n), ref_pc, cur_pc((size_t)E.T * BATCH_MAXK);
+  std::vector<i64> ref_cut, cur_cut((size_t)E.T);
+  std::vector<u64> ht((size_t)E.T);
+  int trials = 0, differ = 0;
+  char line[512];
+  // one refinement launch; per-class workgroup-ms, then the launch's ms
+  auto trial = [&](int g, int tile, const int* heavy, double* out) {
+    for (i64 j = 0; j < E.T; j++)
+      if (cls[j] >= 0) E.hg[j].heavy = heavy[cls[j]];
+    HIP_TRY(hipMemcpyAsync(E.dg.p, E.hg.data(), sizeof(BatchGraph) * E.T, hipMemcpyHostToDevice,
+                           C.s));
+    HIP_TRY(hipMemcpyAsync(E.status.p, status0.p, sizeof(i32) * E.T, hipMemcpyDeviceToDevice,
+                           C.s));
+    HIP_TRY(hipMemsetAsync(E.B.next + 1, 0, sizeof(u32), C.s));
+    E.B.g = g;
+    E.B.stile = tile;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, C.s));
+    launch_batch_refine(E.dg.p, E.order.p, (i32)E.T, E.B, nullptr, C.s);
+    HIP_TRY(hipEventRecord(e1, C.s));
+    HIP_TRY(hipMemcpyAsync(ht.data(), ticks.p, sizeof(u64) * E.T, hipMemcpyDeviceToHost, C.s));
+    HIP_TRY(hipMemcpyAsync(cur_part.data(), E.parts.p, sizeof(i32) * sum_n,
+                           hipMemcpyDeviceToHost, C.s));
+    HIP_TRY(hipMemcpyAsync(cur_pc.data(), E.pcs.p, sizeof(i32) * E.T * BATCH_MAXK,
+                           hipMemcpyDeviceToHost, C.s));
+    HIP_TRY(hipMemcpyAsync(cur_cut.data(), E.cuts.p, sizeof(i64) * E.T, hipMemcpyDeviceToHost,
+                           C.s));
+    C.sync();
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const std::vector<i32> st = E.statuses();
+    for (int c = 0; c < 3; c++) out[c] = 0.0;
+    bool same = true;
+    for (i64 j = 0; j < E.T; j++) {
+      if (cls[j] < 0) continue;
+      if (st[j] != 0) same = false;
+      out[cls[j]] += (double)ht[j] / (double)C.clock_khz;
+    }
+    out[3] = ms;
+    if (ref_part.empty()) {
+      ref_part = cur_part;
+      ref_pc = cur_pc;
+      ref_cut = cur_cut;
+    } else {
+      for (i64 j = 0; j < E.T && same; j++) {
+        if (cls[j] < 0) continue;
+        same = ref_cut[j] == cur_cut[j] &&
+               !std::memcmp(&ref_part[E.part_off[j]], &cur_part[E.part_off[j]],
+                            sizeof(i32) * E.hg[j].n) &&
+               !std::memcmp(&ref_pc[j * BATCH_MAXK], &cur_pc[j * BATCH_MAXK],
+                            sizeof(i32) * E.hg[j].k);
+      }
+    }
+    trials++;
+    differ += !same;
+    std::snprintf(line, sizeof(line),
+                  "lanes %d tile %4d heavy %4d/%4d/%4d: workgroup-ms %10.1f %10.1f %10.1f  "
+                  "sum %10.1f  launch %8.1f ms%s\n",
+                  g, tile, heavy[0], heavy[1], heavy[2], out[0], out[1], out[2],
+                  out[0] + out[1] + out[2], out[3], same ? "" : "  RESULTS DIFFER");
+    rep += line;
+  };
+  std::snprintf(line, sizeof(line),
+                "# sample: %lld graphs tuned (mean degree [0,12): %lld, [12,20): %lld, >= 20: "
+                "%lld)\n# per class: the sum of the graphs' workgroup-ms\n",
+                (long long)ntuned, (long long)ncls[0], (long long)ncls[1], (long long)ncls[2]);
+  rep += line;
+  static const int LANES[3] = {2, 4, 8}, HEAVY[6] = {48, 96, 192, 384, 768, 1536};
+  double best_total = INFINITY;
+  int bg = 4, bh[3] = {96, 96, 96}, bu = 96;
+  for (int gi = 0; gi < 3; gi++) {
+    double cbest[3] = {INFINITY, INFINITY, INFINITY}, ubest = INFINITY;
+    int ch[3] = {96, 96, 96}, uh = 96;
+    for (int hi = 0; hi < 6; hi++) {
+      const int h3[3] = {HEAVY[hi], HEAVY[hi], HEAVY[hi]};
+      double r[4];
+      trial(LANES[gi], 4096, h3, r);
+      for (int c = 0; c < 3; c++)
+        if (ncls[c] && r[c] < cbest[c]) {
+          cbest[c] = r[c];
+          ch[c] = HEAVY[hi];
+        }
+      if (r[0] + r[1] + r[2] < ubest) {
+        ubest = r[0] + r[1] + r[2];
+        uh = HEAVY[hi];
+      }
+    }
+    double tot = 0;
+    for (int c = 0; c < 3; c++)
+      if (ncls[c]) tot += cbest[c];
+    if (tot < best_total) {
+      best_total = tot;
+      bg = LANES[gi];
+      for (int c = 0; c < 3; c++) bh[c] = ch[c];
+      bu = uh;
     }
   }
+  static const int TILES[3] = {2048, 4096, 8192};
+  int bt = 4096;
+  double bt_ms = INFINITY;
+  for (int ti = 0; ti < 3; ti++) {
+    double r[4];
+    trial(bg, TILES[ti], bh, r);
+    if (r[0] + r[1] + r[2] < bt_ms) {
+      bt_ms = r[0] + r[1] + r[2];
+      bt = TILES[ti];
+    }
+  }
+  std::string table = "# batched engine tuning table, written by tgp_batch_autotune\n";
+  {
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, C.device) == hipSuccess) {
+      std::snprintf(line, sizeof(line), "# device: %s%s%s, %d CUs\n", prop.name,
+                    prop.name[0] ? " " : "", prop.gcnArchName, prop.multiProcessorCount);
+      table += line;
+    }
+  }
+  std::snprintf(line, sizeof(line),
+                "# sample: %lld graphs; %d settings measured, results %s\n"
+                "lanes %d\nsort_tile %d\nheavy_default %d\n",
+                (long long)ntuned, trials, differ ? "DIFFER" : "identical in every setting",
+                bg, bt, bu);
+  table += line;
+  for (int c = 0; c < 3; c++) {
+    if (!ncls[c]) continue;
+    if (c < 2) std::snprintf(line, sizeof(line), "heavy %g %g %d\n", CLS_LO[c], CLS_HI[c], bh[c]);
+    else std::snprintf(line, sizeof(line), "heavy %g inf %d\n", CLS_LO[c], bh[c]);
+    table += line;
+  }
+  rep += table;
+  if (differ) {
+    set_error("autotune: results differ between engine settings in " + std::to_string(differ) +
+              " of " + std::to_string(trials) + " trials");
+    return TGP_EDIFF;
+  }
+  if (path && *path) {
+    FILE* f = std::fopen(path, "w");
+    if (!f || std::fputs(table.c_str(), f) < 0) {
+      if (f) std::fclose(f);
+      set_error(std::string("autotune: cannot write ") + path);
+      return TGP_EIO;
+    }
+    std::fclose(f);
+  }
+  // the last trial's results (the chosen setting's) into the items
+  std::vector<i64> rest;
+  E.collect(rest);
+  for (i64 i : rest) items[i].rc = TGP_EINVAL;
+  return TGP_OK;
 }
 
 // Contexts of the batched form's workers, created on first use and, like the
@@ -1895,6 +2293,42 @@ extern "C" {
 
 const char* tgp_last_error(void) { return last_error().c_str(); }
 
+int tgp_batch_tuning(double mean_degree, int32_t* lanes, int32_t* heavy, int32_t* sort_tile) {
+  last_error().clear();
+  BatchTuning t;
+  const int rc = load_batch_tuning(&t);
+  if (rc != TGP_OK) return rc;
+  const i64 scale = 1 << 20;
+  if (lanes) *lanes = t.g;
+  if (sort_tile) *sort_tile = t.stile;
+  if (heavy) *heavy = t.heavy_for(scale, (i64)std::llround(mean_degree * (double)scale));
+  return TGP_OK;
+}
+
+int tgp_batch_autotune(tgp_batch_item* items, int64_t count, const char* table_path,
+                       char** report) {
+  last_error().clear();
+  if (report) *report = nullptr;
+  if (count <= 0 || !items) {
+    set_error("autotune: need items and count > 0");
+    return TGP_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  std::string rep;
+  int rc;
+  try {
+    rc = autotune_on(*default_context(), items, count, table_path, rep);
+  } catch (const Fail& f) {
+    (void)hipStreamSynchronize(default_context()->s);
+    rc = f.code;
+  }
+  if (report) {
+    *report = (char*)std::malloc(rep.size() + 1);
+    if (*report) std::memcpy(*report, rep.c_str(), rep.size() + 1);
+  }
+  return rc;
+}
+
 int tgp_balance(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
                 int64_t* edge_cut, tgp_stats* stats) {
   std::lock_guard<std::mutex> lock(g_ctx_mu);
@@ -1922,7 +2356,11 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
     for (i64 i = 0; i < count; i++) rest.push_back(i);
   } else {
     try {
-      engine_batch(*default_context(), items, count, rest);
+      const int erc = engine_batch(*default_context(), items, count, rest);
+      if (erc != TGP_OK) {
+        for (i64 i = 0; i < count; i++) items[i].rc = erc;
+        return erc;
+      }
     } catch (const Fail& f) {
       (void)hipStreamSynchronize(default_context()->s);
       return f.code;

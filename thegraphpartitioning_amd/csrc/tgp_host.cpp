@@ -101,6 +101,7 @@ const char* tgp_strerror(int code) {
     case TGP_EHIP: return "HIP runtime error";
     case TGP_ENOMEM: return "out of memory";
     case TGP_EIO: return "file I/O error";
+    case TGP_EDIFF: return "engine settings gave different results";
     default: return "unknown error";
   }
 }

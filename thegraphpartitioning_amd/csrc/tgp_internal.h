@@ -131,8 +131,14 @@ struct BatchGraph {
   i32* pc_out;
   i64* cut_out;
   i32* status;
+  i32 heavy;   // rows longer than this are refined by a whole wave
+  u64* ticks;  // optional: the refinement's wall-clock ticks for this graph
 };
 constexpr int BATCH_MAXK = 16;
+// sort tiles the refinement launch is built for (entries held in LDS)
+constexpr int BATCH_STILES[3] = {2048, 4096, 8192};
+// lanes per light vertex it is built for
+constexpr int BATCH_LANES[3] = {2, 4, 8};
 // Scratch of one coarsening / refinement workgroup for graphs of at most
 // maxn vertices, and the size of one graph's level record.
 i64 batch_coarsen_bytes(i64 maxn);
@@ -149,6 +155,8 @@ struct BatchBuffers {
   u64* pool_used; // zero before the launch
   u8* recs;       // count x batch_record_bytes()
   u32* next;      // two queue counters, zero before the launch
+  int stile;      // the refinement launch's sort tile (one of BATCH_STILES)
+  int g;          // its lanes per light vertex (one of BATCH_LANES)
 };
 // The coarsening launch over the queue, then the refinement launch.
 // stats (optional, 12 u64): wall-clock ticks per phase summed over
@@ -156,6 +164,12 @@ struct BatchBuffers {
 // misc), then refinement passes and levels built.
 void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
                          const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s);
+// The two launches apart (the tuner coarsens once and refines many times;
+// the refinement takes the graphs whose status is 2 and reads next[1]).
+void launch_batch_coarsen(const BatchGraph* graphs, const i32* order, i32 count,
+                          const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s);
+void launch_batch_refine(const BatchGraph* graphs, const i32* order, i32 count,
+                         const BatchBuffers& B, u64* stats, hipStream_t s);
 void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, i32* prop,
                             i32* decision, i64* lo, i64 W, i32* req, i32* nreq, u32 round,
                             hipStream_t s);

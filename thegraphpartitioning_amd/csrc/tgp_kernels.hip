@@ -3802,15 +3802,41 @@ i64 batch_refine_bytes(i64 maxn) {
 }
 i64 batch_record_bytes() { return (i64)sizeof(BeLevRec); }
 
-void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
-                         const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s) {
+void launch_batch_coarsen(const BatchGraph* graphs, const i32* order, i32 count,
+                          const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s) {
   const BePool pool{B.pool, B.pool_cap, B.pool_used};
   k_batch_coarsen<<<B.blocks_c, BE_T, 0, s>>>(graphs, order, count, B.next, B.scratch_c,
                                                B.maxn, batch_coarsen_bytes(B.maxn), pool,
                                                (BeLevRec*)B.recs, budget, stats);
-  k_batch_refine<<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 1, B.scratch_r,
-                                              B.maxn, batch_refine_bytes(B.maxn),
-                                              (const BeLevRec*)B.recs, stats);
+}
+template <int STILE>
+void batch_refine_at(const BatchGraph* graphs, const i32* order, i32 count,
+                     const BatchBuffers& B, u64* stats, hipStream_t s) {
+  const i64 per = batch_refine_bytes(B.maxn);
+  const BeLevRec* recs = (const BeLevRec*)B.recs;
+  if (B.g == 2)
+    k_batch_refine<STILE, 2><<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 1,
+                                                         B.scratch_r, B.maxn, per, recs, stats);
+  else if (B.g == 8)
+    k_batch_refine<STILE, 8><<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 1,
+                                                         B.scratch_r, B.maxn, per, recs, stats);
+  else
+    k_batch_refine<STILE, 4><<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 1,
+                                                         B.scratch_r, B.maxn, per, recs, stats);
+}
+void launch_batch_refine(const BatchGraph* graphs, const i32* order, i32 count,
+                         const BatchBuffers& B, u64* stats, hipStream_t s) {
+  if (B.stile == 2048)
+    batch_refine_at<2048>(graphs, order, count, B, stats, s);
+  else if (B.stile == 8192)
+    batch_refine_at<8192>(graphs, order, count, B, stats, s);
+  else
+    batch_refine_at<4096>(graphs, order, count, B, stats, s);
+}
+void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
+                         const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s) {
+  launch_batch_coarsen(graphs, order, count, B, budget, stats, s);
+  launch_batch_refine(graphs, order, count, B, stats, s);
 }
 
 }  // namespace tgp
