@@ -14,7 +14,8 @@ Prints one JSON line: M edges/s (level-0 arcs x steps / seconds), the roofline
 of the dominant kernel (the chained matching kernel, k_match_chain) from
 HIP-event timing of that kernel alone, the whole step's algorithmic bytes by
 the per-kernel model over the step time, and the oracle's CPU throughput on a
-bounded sample of the same workload family.  `roofline.traffic` (HBM bytes per
+bounded sample of the same workload family (the same generator, edge factor,
+k and semantics at R-MAT scale 20 on every host thread, and scale 18 on one).  `roofline.traffic` (HBM bytes per
 launch of that kernel) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
 committed under profiles/, scaled by the same counters' reading for a plain
 copy of known size (`--copy-probe`).
@@ -99,30 +100,35 @@ def gpmetis_baseline(g, k):
     return sorted(times)[1]
 
 
-def cpu_baseline(scale, ef, k):
-    """The oracle end-to-end on R-MAT(scale, ef), repaired mode, timed with one
-    thread and with every host thread (its order-independent loops use
-    OpenMP; the matching, move buffering and selection stay sequential), plus
-    gpmetis when installed."""
+def cpu_baseline(scale, ef, k, scale_one):
+    """The oracle end-to-end, repaired mode: on R-MAT(scale, ef) with every
+    host thread (its order-independent loops use OpenMP; the matching, move
+    buffering and selection stay sequential), and on the smaller
+    R-MAT(scale_one, ef) with one thread; plus gpmetis when installed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
     import thegraphpartitioning_amd as tgp
     nproc, model = host_cpu()
-    g = tgp.gen_rmat(scale, ef, SEED)
-    rates = {}
-    for threads in (1, nproc):
+    runs = {}
+    for threads, sc in ((nproc, scale), (1, scale_one)):
+        g = tgp.gen_rmat(sc, ef, SEED)
         oracle_ffi.set_threads(threads)
         t0 = time.perf_counter()
         oracle_ffi.partition(g.xadj, g.adjncy, g.adjwgt, k, repair=1)
-        rates[threads] = (time.perf_counter() - t0, g.m / (time.perf_counter() - t0) / 1e6)
+        dt = time.perf_counter() - t0
+        runs[threads] = (dt, g.m / dt / 1e6, g)
+        print(f"# oracle R-MAT {sc}: {dt:.1f}s with {threads} threads", file=sys.stderr,
+              flush=True)
+    g = runs[nproc][2]
+    g1 = runs[1][2]
     gm = gpmetis_baseline(g, k)
-    return {"value": round(rates[nproc][1], 4), "unit": "M edges/s", "cores": nproc,
+    return {"value": round(runs[nproc][1], 4), "unit": "M edges/s", "cores": nproc,
             "kind": "port",
-            "sample": f"oracle end-to-end on R-MAT scale {scale} ef {ef} k={k} "
-                      f"({g.n} vertices, {g.m} arcs), same repaired semantics: "
-                      f"{rates[nproc][0]:.1f} s with {nproc} threads, "
-                      f"{rates[1][0]:.1f} s with 1",
-            "value_1_thread": round(rates[1][1], 4), "nproc": nproc, "cpu_model": model,
+            "sample": f"oracle end-to-end, same repaired semantics, k={k}: R-MAT scale {scale} "
+                      f"ef {ef} ({g.n} vertices, {g.m} arcs) in {runs[nproc][0]:.1f} s with "
+                      f"{nproc} threads; value_1_thread on R-MAT scale {scale_one} ef {ef} "
+                      f"({g1.n} vertices, {g1.m} arcs) in {runs[1][0]:.1f} s",
+            "value_1_thread": round(runs[1][1], 4), "nproc": nproc, "cpu_model": model,
             "gpmetis": ("unavailable (not installed on this host)" if gm is None else
                         {"value": round(g.m / gm / 1e6, 4), "unit": "M edges/s", "cores": 1,
                          "median_seconds": round(gm, 3)})}
@@ -438,7 +444,10 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-scale", type=int, default=17)
+    ap.add_argument("--cpu-scale", type=int, default=20,
+                    help="R-MAT scale of the oracle's all-threads CPU baseline run")
+    ap.add_argument("--cpu-scale-one", type=int, default=18,
+                    help="R-MAT scale of the oracle's one-thread CPU baseline run")
     ap.add_argument("--profile-only", action="store_true",
                     help="run one profiled step and print per-kernel times")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -579,7 +588,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_scale, args.edge_factor, args.k)
+        cpu = cpu_baseline(args.cpu_scale, args.edge_factor, args.k, args.cpu_scale_one)
 
     out = {
         "metric": "M edges/s end-to-end at k=64 (plain)",
