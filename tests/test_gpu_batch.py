@@ -44,16 +44,20 @@ def test_batch_equals_solo(workers, engine, monkeypatch):
         np.testing.assert_array_equal(bpc, pc, err_msg=f"graph {i}")
 
 
-@pytest.mark.parametrize("engine", ["1", "0"])
-def test_batch_at_balance_bound_equals_solo(engine, monkeypatch):
-    """balance_eps in the batch: engine graphs get the balance stage after the
-    engine (tgp_balance on a worker stream), worker graphs inside their solo
-    run; both must equal the solo partition with the same bound."""
+@pytest.mark.parametrize("engine,setting", [("1", ""), ("1", "2,48,2048"), ("1", "8,1536,8192"),
+                                            ("0", "")])
+def test_batch_at_balance_bound_equals_solo(engine, setting, monkeypatch):
+    """balance_eps in the batch: engine graphs get the balance stage in the
+    engine's own balance launch (at each lane width), worker graphs inside
+    their solo run; both must equal the solo partition with the same bound."""
     monkeypatch.setenv("TGP_BATCH_ENGINE", engine)
+    if setting:
+        monkeypatch.setenv("TGP_BATCH_SETTING", setting)
     gs, ks = batch_graphs(12, 14, 20000)
     eps = 0.03
     out = tgp.partition_batch(gs, ks, init=tgp.INIT_BLOCK, repair=True, workers=4,
                               balance_eps=eps)
+    monkeypatch.delenv("TGP_BATCH_SETTING", raising=False)
     for i, (g, k) in enumerate(zip(gs, ks)):
         part, pc, cut, _ = tgp.partition(g, k, init=tgp.INIT_BLOCK, repair=True,
                                          balance_eps=eps)

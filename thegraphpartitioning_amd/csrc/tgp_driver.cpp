@@ -1783,6 +1783,11 @@ struct BatchEngine {
       b.status = status.p + j;
       b.heavy = tun.heavy_for(b.n, b.m);
       b.ticks = nullptr;
+      // the balance bound of Run::balance_finest
+      b.cap = it.o.balance_eps > 0.0f
+                  ? (i64)std::floor((1.0 + (double)it.o.balance_eps) *
+                                    (double)((b.n + b.k - 1) / b.k))
+                  : 0;
     }
     // largest graphs first, so the long ones start early
     ord.resize((size_t)T);
@@ -1799,6 +1804,8 @@ struct BatchEngine {
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device));
     cus = std::max(cus, 1);
     B.maxn = maxn;
+    B.balance = 0;
+    for (const BatchGraph& b : hg) B.balance |= b.cap > 0;
     B.stile = tun.stile;
     B.g = tun.g;
     B.blocks_c = (int)std::min<i64>(T, cus);
@@ -1926,8 +1933,8 @@ int engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>&
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DBuf<u64> bst;
   if (trace) {
-    bst = DBuf<u64>(&C.pool, 12);
-    HIP_TRY(hipMemsetAsync(bst.p, 0, 12 * sizeof(u64), C.s));
+    bst = DBuf<u64>(&C.pool, 13);
+    HIP_TRY(hipMemsetAsync(bst.p, 0, 13 * sizeof(u64), C.s));
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, C.s));
@@ -1948,14 +1955,15 @@ int engine_batch(Context& C, tgp_batch_item* items, i64 count, std::vector<i64>&
                  "level pool %.1f GB, %d rounds, %.2f ms (lanes %d, sort tile %d, tuning %s)\n",
                  (long long)E.T, (long long)back, E.B.blocks_c, E.B.blocks_r,
                  E.B.pool_cap / 1e9, rounds, ms, E.B.g, E.B.stile, tun.source.c_str());
-    u64 hb[12];
+    u64 hb[13];
     HIP_TRY(hipMemcpy(hb, bst.p, sizeof(hb), hipMemcpyDeviceToHost));
     const char* nm[10] = {"check", "match", "contract", "gain", "token",
                           "moves", "sort",  "select",   "cut",  "misc"};
     std::fprintf(stderr, "# batch engine phases (workgroup-ms):");
     for (int q = 0; q < 10; q++)
       std::fprintf(stderr, " %s=%.1f", nm[q], hb[q] / (double)C.clock_khz);
-    std::fprintf(stderr, " passes=%llu levels=%llu\n", (unsigned long long)hb[10],
+    std::fprintf(stderr, " balance=%.1f passes=%llu levels=%llu\n",
+                 hb[12] / (double)C.clock_khz, (unsigned long long)hb[10],
                  (unsigned long long)hb[11]);
   }
   return TGP_OK;
@@ -2368,18 +2376,8 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
       return f.code;
     }
   }
-  // graphs the engine partitioned with a balance bound: the balance stage
-  // runs on the engine's result (which equals the solo plain result, so the
-  // outcome equals the solo run with the stage)
-  std::vector<i64> post;
-  {
-    std::vector<bool> in_rest((size_t)count, false);
-    for (i64 r : rest) in_rest[r] = true;
-    for (i64 i = 0; i < count; i++)
-      if (!in_rest[i] && items[i].o.balance_eps > 0.0f) post.push_back(i);
-  }
-  const i64 nrest = (i64)rest.size(), npost = (i64)post.size();
-  const int W = (int)std::min<i64>(workers, std::max<i64>(nrest + npost, 1));
+  const i64 nrest = (i64)rest.size();
+  const int W = (int)std::min<i64>(workers, std::max<i64>(nrest, 1));
   std::vector<Context*> ctx;
   try {
     for (int w = 0; w < W; w++) ctx.push_back(worker_context(w));
@@ -2390,14 +2388,9 @@ int tgp_partition_batch(tgp_batch_item* items, int64_t count, int32_t workers) {
   auto work = [&](Context* C) {
     for (;;) {
       const i64 r = next.fetch_add(1);
-      if (r >= nrest + npost) break;
-      if (r < nrest) {
-        tgp_batch_item& it = items[rest[r]];
-        it.rc = partition_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
-      } else {
-        tgp_batch_item& it = items[post[r - nrest]];
-        it.rc = balance_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
-      }
+      if (r >= nrest) break;
+      tgp_batch_item& it = items[rest[r]];
+      it.rc = partition_on(*C, &it.g, &it.o, it.part, it.part_count, &it.edge_cut, it.stats);
     }
   };
   std::vector<std::thread> th;

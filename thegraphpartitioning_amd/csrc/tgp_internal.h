@@ -133,6 +133,7 @@ struct BatchGraph {
   i32* status;
   i32 heavy;   // rows longer than this are refined by a whole wave
   u64* ticks;  // optional: the refinement's wall-clock ticks for this graph
+  i64 cap;     // > 0: the balance stage's bound (tgp_options.balance_eps), 0: none
 };
 constexpr int BATCH_MAXK = 16;
 // sort tiles the refinement launch is built for (entries held in LDS)
@@ -154,16 +155,21 @@ struct BatchBuffers {
   i64 pool_cap;
   u64* pool_used; // zero before the launch
   u8* recs;       // count x batch_record_bytes()
-  u32* next;      // two queue counters, zero before the launch
+  u32* next;      // three queue counters, zero before the launches
   int stile;      // the refinement launch's sort tile (one of BATCH_STILES)
   int g;          // its lanes per light vertex (one of BATCH_LANES)
+  int balance;    // some graph has a balance bound: the balance launch runs
 };
-// The coarsening launch over the queue, then the refinement launch.
-// stats (optional, 12 u64): wall-clock ticks per phase summed over
+// The coarsening launch over the queue, then the refinement launch, then
+// (B.balance) the balance launch.
+// stats (optional, 13 u64): wall-clock ticks per phase summed over
 // workgroups (check, match, contract, gain, token, moves, sort, select, cut,
-// misc), then refinement passes and levels built.
+// misc), refinement passes, levels built, then balance-stage ticks.
 void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
                          const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s);
+// The balance stage's launch (graphs with cap > 0 and status 0; next[2]).
+void launch_batch_balance(const BatchGraph* graphs, const i32* order, i32 count,
+                          const BatchBuffers& B, u64* stats, hipStream_t s);
 // The two launches apart (the tuner coarsens once and refines many times;
 // the refinement takes the graphs whose status is 2 and reads next[1]).
 void launch_batch_coarsen(const BatchGraph* graphs, const i32* order, i32 count,

@@ -3833,10 +3833,24 @@ void launch_batch_refine(const BatchGraph* graphs, const i32* order, i32 count,
   else
     batch_refine_at<4096>(graphs, order, count, B, stats, s);
 }
+void launch_batch_balance(const BatchGraph* graphs, const i32* order, i32 count,
+                          const BatchBuffers& B, u64* stats, hipStream_t s) {
+  const i64 per = batch_refine_bytes(B.maxn);
+  if (B.g == 2)
+    k_batch_balance<2><<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 2, B.scratch_r,
+                                                   B.maxn, per, stats);
+  else if (B.g == 8)
+    k_batch_balance<8><<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 2, B.scratch_r,
+                                                   B.maxn, per, stats);
+  else
+    k_batch_balance<4><<<B.blocks_r, BE_T, 0, s>>>(graphs, order, count, B.next + 2, B.scratch_r,
+                                                   B.maxn, per, stats);
+}
 void launch_batch_engine(const BatchGraph* graphs, const i32* order, i32 count,
                          const BatchBuffers& B, i64 budget, u64* stats, hipStream_t s) {
   launch_batch_coarsen(graphs, order, count, B, budget, stats, s);
   launch_batch_refine(graphs, order, count, B, stats, s);
+  if (B.balance) launch_batch_balance(graphs, order, count, B, stats, s);
 }
 
 }  // namespace tgp
