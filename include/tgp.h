@@ -277,8 +277,8 @@ int tgp_batch_autotune(tgp_batch_item* items, int64_t count, const char* table_p
  * bound exactly as the last step of tgp_partition with balance_eps does;
  * part_count[k] and edge_cut receive the result's counts and cut.  Uses
  * o->k (<= 1024), o->balance_eps (> 0), o->device_input, o->wide and
- * o->check_level; the graph is g.  The batched form calls it for graphs its
- * engine partitioned with a balance bound.                                  */
+ * o->check_level; the graph is g.  For callers that already hold a
+ * partition (from another partitioner, or an earlier call without a bound). */
 int tgp_balance(const tgp_graph* g, const tgp_options* o, int32_t* part, int32_t* part_count,
                 int64_t* edge_cut, tgp_stats* stats);
 
