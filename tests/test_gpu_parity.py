@@ -296,3 +296,43 @@ def test_sort_skip_is_exact(oracle, monkeypatch, skip):
         assert skipped > 0
     else:
         assert skipped == 0 and gp2[3].sorts_skipped == 0
+
+
+def twin_hub_graph(nleaves, extra, seed, wmax=5):
+    """Vertices 0 and 1 joined by the heaviest edge (so they match) and each
+    joined to every leaf 2..nleaves+1, plus `extra` random leaf edges; weights
+    1..wmax.  The coarse row of {0, 1} holds every leaf's coarse id at least
+    twice."""
+    rng = np.random.default_rng(seed)
+    n = nleaves + 2
+    edges = {(0, 1)}
+    for h in (0, 1):
+        for v in range(2, n):
+            edges.add((h, v))
+    while len(edges) < 1 + 2 * nleaves + extra:
+        u, v = int(rng.integers(2, n)), int(rng.integers(2, n))
+        if u != v:
+            edges.add((min(u, v), max(u, v)))
+    edges = sorted(edges)
+    w = rng.integers(1, wmax + 1, size=len(edges))
+    w[edges.index((0, 1))] = wmax + 5
+    return graphs.csr_from_edges(n, edges, weights=w)
+
+
+@pytest.mark.parametrize("env", [{}, {"TGP_REP_BUCKETS": "0"}, {"TGP_REP_BUCKET_FILL": "1"},
+                                 {"TGP_REP_BUCKET_TARGET": "8192"},
+                                 {"TGP_REP_BUCKET_TARGET": "65536"}],
+                         ids=["buckets", "tables", "fill1", "slices", "overfill"])
+def test_long_row_contraction_forms(oracle, monkeypatch, env):
+    """The repaired contraction of long rows in each of its forms: LDS buckets
+    (default), the global tables, a fill limit of one key (every level falls
+    back to the tables), buckets of ~5000 records read in slices, and one
+    bucket per row whose 5000 keys pass the fill limit (falls back)."""
+    for key, val in env.items():
+        monkeypatch.setenv(key, val)
+    xadj, adj, w = twin_hub_graph(5000, 12000, 3)
+    gp, op = run_both(oracle, xadj, adj, w, 8, repair=True, floor=50)
+    assert_same(gp, op, f"twin hubs {env}")
+    g = tgp.gen_rmat(15, 16, 0x7C0DE5EED + 15)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 32, repair=True)
+    assert_same(gp, op, f"rmat15 {env}")

@@ -744,18 +744,23 @@ void Run::coarsen() {
       DBuf<u64> T;
       DBuf<i32> W, E, ch_row;
       DBuf<i64> ch_beg, chunk_cnt, choff;
+      // bucketed long rows (default): TGP_REP_BUCKETS=0 takes the global tables
+      // instead; TGP_REP_BUCKET_FILL lowers the per-bucket key limit and
+      // TGP_REP_BUCKET_TARGET sets the entries per bucket aimed at (tests)
+      RepBuckets RB;
+      bool bucketed = false;
+      DBuf<i32> lgb, brow, over;
+      DBuf<i64> nbk, bbase, boff;
+      DBuf<u64> bcnt, R, BK;
+      DBuf<u32> bcur;
+      DBuf<i32> BW;
       if (nlng > 0) {
         launch_scan_excl(nchk.p, cbase.p, nlng, cbase.p + nlng, scratch(), st());
         const i64 nch = read_i64(cbase.p + nlng);
-        T = DBuf<u64>(&C.pool, ttot);
-        W = DBuf<i32>(&C.pool, ttot);
-        E = DBuf<i32>(&C.pool, etot);
         ch_row = DBuf<i32>(&C.pool, nch);
         ch_beg = DBuf<i64>(&C.pool, nch);
         chunk_cnt = DBuf<i64>(&C.pool, nch + 1);
         choff = DBuf<i64>(&C.pool, nch + 1);
-        launch_fill_u64(T.p, ttot, ~0ull, st());
-        launch_fill_i32(W.p, ttot, 0, st());
         launch_rep_chunks(lng.p, nlng, dsz.p, cbase.p, ch_row.p, ch_beg.p, st());
         RL.lng = lng.p;
         RL.nl = nlng;
@@ -764,15 +769,65 @@ void Run::coarsen() {
         RL.ch_row = ch_row.p;
         RL.ch_beg = ch_beg.p;
         RL.nch = nch;
-        RL.toff = toff.p;
-        RL.csz = csz.p;
-        RL.T = T.p;
-        RL.W = W.p;
         RL.eoff = eoff.p;
-        RL.E = E.p;
         RL.chunk_cnt = chunk_cnt.p;
         RL.choff = choff.p;
-        launch_rep_long(g, tb.p, new_ids.p, RL, 0, nullptr, nullptr, nullptr, st());
+        if (env_i64("TGP_REP_BUCKETS", 1) != 0) {
+          lgb = DBuf<i32>(&C.pool, nlng);
+          nbk = DBuf<i64>(&C.pool, nlng);
+          bbase = DBuf<i64>(&C.pool, nlng + 1);
+          launch_rl_buckets(RL, env_i64("TGP_REP_BUCKET_TARGET", 0), lgb.p, nbk.p, st());
+          launch_scan_excl(nbk.p, bbase.p, nlng, bbase.p + nlng, scratch(), st());
+          const i64 nbt = read_i64(bbase.p + nlng);
+          brow = DBuf<i32>(&C.pool, nbt);
+          bcnt = DBuf<u64>(&C.pool, nbt);
+          boff = DBuf<i64>(&C.pool, nbt + 1);
+          bcur = DBuf<u32>(&C.pool, nbt);
+          over = DBuf<i32>(&C.pool, 1);
+          R = DBuf<u64>(&C.pool, etot);
+          BK = DBuf<u64>(&C.pool, etot);
+          BW = DBuf<i32>(&C.pool, etot);
+          RB.lgb = lgb.p;
+          RB.nbk = nbk.p;
+          RB.bbase = bbase.p;
+          RB.nbt = nbt;
+          RB.brow = brow.p;
+          RB.bcnt = bcnt.p;
+          RB.boff = boff.p;
+          RB.bcur = bcur.p;
+          RB.R = R.p;
+          RB.K = BK.p;
+          RB.W = BW.p;
+          RB.over = over.p;
+          launch_fill_u64(bcnt.p, nbt, 0, st());
+          launch_fill_i32((i32*)bcur.p, nbt, 0, st());
+          launch_fill_i32(over.p, 1, 0, st());
+          launch_rl_brow(nlng, bbase.p, brow.p, st());
+          launch_rl_fetch(g, tb.p, new_ids.p, RL, RB, st());
+          launch_scan_excl((const i64*)bcnt.p, boff.p, nbt, boff.p + nbt, scratch(), st());
+          launch_rl_scatter(RL, RB, st());
+          launch_rl_dedup(RL, RB, (i32)env_i64("TGP_REP_BUCKET_FILL", REP_BUCKET_FILL), st());
+          HIP_TRY(hipMemcpyAsync(C.pinned, over.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+          C.sync();
+          bucketed = ((const i32*)C.pinned)[0] == 0;
+          BK = DBuf<u64>();
+          BW = DBuf<i32>();
+          if (bucketed) launch_rl_count(RL, RB, st());
+          else R = DBuf<u64>();
+        }
+        if (!bucketed) {
+          T = DBuf<u64>(&C.pool, ttot);
+          W = DBuf<i32>(&C.pool, ttot);
+          E = DBuf<i32>(&C.pool, etot);
+          launch_fill_u64(T.p, ttot, ~0ull, st());
+          launch_fill_i32(W.p, ttot, 0, st());
+          RL.toff = toff.p;
+          RL.csz = csz.p;
+          RL.T = T.p;
+          RL.W = W.p;
+          RL.E = E.p;
+          launch_rep_long(g, tb.p, new_ids.p, RL, 0, nullptr, nullptr, nullptr, st());
+        }
         launch_scan_excl(chunk_cnt.p, choff.p, nch, choff.p + nch, scratch(), st());
         launch_rep_rowcnt(RL, cnt.p, st());
       }
@@ -785,7 +840,8 @@ void Run::coarsen() {
       if (m_c > 0) {
         launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, 1, nullptr, c->xadj.p,
                         c->adj.p, c->w.p, st());
-        launch_rep_long(g, tb.p, new_ids.p, RL, 1, c->xadj.p, c->adj.p, c->w.p, st());
+        if (bucketed) launch_rl_emit(RL, RB, c->xadj.p, c->adj.p, c->w.p, st());
+        else launch_rep_long(g, tb.p, new_ids.p, RL, 1, c->xadj.p, c->adj.p, c->w.p, st());
       }
     }
     C.prof.end(KID_CONTRACT, 24.0 * n_c + 12.0 * lv.m + 8.0 * m_c);

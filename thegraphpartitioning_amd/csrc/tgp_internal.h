@@ -239,6 +239,41 @@ void launch_rep_long(DGraph g, const i32* tb, const i32* new_ids, const RepLong&
                      const i64* ox, i32* oa, i32* ow, hipStream_t s);
 // cnt[c] of the long rows from the scanned chunk counts
 void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s);
+// Long rows, bucketed form (the default; the global tables are the fallback).
+// Entry words R (x << 32 | w in position order, at eoff); per long row e,
+// 2^lgb[e] buckets (nbk[e]) from bucket bbase[e] on (exclusive scan, nbt in
+// all); brow = row of each bucket; bcnt = entries per bucket (zeroed before
+// fetch), boff = their exclusive scan, bcur = fill cursors (zeroed); K / W =
+// bucket records (x << 32 | position, weight); over = set when a bucket's
+// distinct neighbours pass the fill limit (the level then takes the tables).
+struct RepBuckets {
+  i32* lgb = nullptr;
+  i64* nbk = nullptr;
+  const i64* bbase = nullptr;
+  i64 nbt = 0;
+  i32* brow = nullptr;
+  u64* bcnt = nullptr;
+  const i64* boff = nullptr;
+  u32* bcur = nullptr;
+  u64* R = nullptr;
+  u64* K = nullptr;
+  i32* W = nullptr;
+  i32* over = nullptr;
+};
+// most distinct neighbours one bucket's LDS table takes before the level
+// falls back to the global tables
+constexpr i32 REP_BUCKET_FILL = 3072;
+// target: entries per bucket aimed at (0: the built-in 1024)
+void launch_rl_buckets(const RepLong& L, i64 target, i32* lgb, i64* nbk, hipStream_t s);
+void launch_rl_brow(i64 nl, const i64* bbase, i32* brow, hipStream_t s);
+void launch_rl_fetch(DGraph g, const i32* tb, const i32* new_ids, const RepLong& L,
+                     const RepBuckets& B, hipStream_t s);
+void launch_rl_scatter(const RepLong& L, const RepBuckets& B, hipStream_t s);
+void launch_rl_dedup(const RepLong& L, const RepBuckets& B, i32 fill, hipStream_t s);
+// chunk_cnt of every chunk from R
+void launch_rl_count(const RepLong& L, const RepBuckets& B, hipStream_t s);
+void launch_rl_emit(const RepLong& L, const RepBuckets& B, const i64* ox, i32* oa, i32* ow,
+                    hipStream_t s);
 // Unique undirected pairs (min,max), u != v, 0 <= u < n, true row ends.
 void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s);
 // count += arcs v -> u with 0 <= u < n, u != v (same row ranges as

@@ -1578,6 +1578,321 @@ __global__ __launch_bounds__(256) void k_rep_emit(const i32* lng, const i32* ch_
   }
 }
 
+// Long rows, bucketed form (the default; the global tables above are the
+// fallback).  Every entry of a long row gets a word R[p] = x << 32 | w in
+// position order (x = ~0 for a self loop).  Entries are then scattered into
+// 2^lg buckets of the row by the top lg bits of hash32(x) (about RL_BK_TARGET
+// entries each, or the target the host passes), and one block per bucket finds, in an LDS table, each
+// coarse neighbour's first position and weight sum.  Only neighbours that
+// occur more than once change R: the first occurrence gets the summed
+// weight, the others are dropped.  The row is then compacted in position
+// order from R.  A bucket whose distinct neighbours would fill more than
+// RL_TFILL of its table stops the bucketed form for the whole level (the
+// host reruns the level's long rows through the global tables).
+constexpr int RL_BK_LGMAX = 13;  // at most 8192 buckets per row (LDS histogram)
+constexpr i64 RL_BK_TARGET = 1024;
+constexpr int RL_TCAP = 4096;
+constexpr int RL_TFILL = REP_BUCKET_FILL;
+static_assert(RL_TFILL + 256 < RL_TCAP, "a free slot must remain after the fill limit");
+constexpr u32 RL_NONE = 0xFFFFFFFFu;
+constexpr int RL_PER = (int)(REP_CHUNK / 256);  // entries per thread of a chunk
+
+__device__ __forceinline__ u32 rl_bucket(u32 x, int lg) {
+  return lg ? hash32(x) >> (32 - lg) : 0u;
+}
+
+// lgb[e] = log2 of the bucket count of long row e; nbk[e] = 2^lgb[e]
+__global__ void k_rl_buckets(const i32* lng, i64 nl, const i64* dsz, i64 target, i32* lgb,
+                             i64* nbk) {
+  for (i64 e = gtid(); e < nl; e += gstride()) {
+    const i64 want = (dsz[lng[e]] + target - 1) / target;
+    int lg = 0;
+    while (lg < RL_BK_LGMAX && ((i64)1 << lg) < want) lg++;
+    lgb[e] = lg;
+    nbk[e] = (i64)1 << lg;
+  }
+}
+// brow[b] = the long row that bucket b belongs to
+__global__ void k_rl_brow(i64 nl, const i64* bbase, i32* brow) {
+  for (i64 e = gtid(); e < nl; e += gstride())
+    for (i64 b = bbase[e]; b < bbase[e + 1]; b++) brow[b] = (i32)e;
+}
+
+// R words of a chunk, and its per-bucket entry counts added to bcnt
+__global__ __launch_bounds__(256) void k_rl_fetch(DGraph g, const i32* tb, const i32* new_ids,
+                                                  const i32* lng, const i32* ch_row,
+                                                  const i64* ch_beg, i64 nch, const i64* dsz,
+                                                  const i64* eoff, const i32* lgb,
+                                                  const i64* bbase, u64* R, u64* bcnt) {
+  __shared__ i32 H[1 << RL_BK_LGMAX];
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i32 e0 = ch_row[k2];
+    const i64 c = lng[e0];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 end = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const int lg = lgb[e0];
+    const i32 nb = 1 << lg;
+    for (i32 h = threadIdx.x; h < nb; h += blockDim.x) H[h] = 0;
+    __syncthreads();
+    const i32 r = tb[2 * c], mt = tb[2 * c + 1];
+    const i64 a0 = g.xadj[r], la = g.xadj[r + 1] - a0;
+    const i64 b0 = mt >= 0 ? g.xadj[mt] : 0;
+    u64* Rc = R + eoff[c];
+    i32 v[RL_PER], w[RL_PER];
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const i64 p = b + j * 256 + threadIdx.x;
+      const i64 i = p < la ? a0 + p : b0 + (p - la);
+      v[j] = p < end ? g.adj[i] : -1;
+      w[j] = p < end ? g.w[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) v[j] = v[j] >= 0 ? new_ids[v[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const i64 p = b + j * 256 + threadIdx.x;
+      if (p < end) {
+        const bool keep = !(v[j] == (i32)c || v[j] == -1);
+        Rc[p] = keep ? (((u64)(u32)v[j] << 32) | (u64)(u32)w[j]) : ((u64)RL_NONE << 32);
+        if (keep) atomicAdd(&H[rl_bucket((u32)v[j], lg)], 1);
+      }
+    }
+    __syncthreads();
+    const i64 bb = bbase[e0];
+    for (i32 h = threadIdx.x; h < nb; h += blockDim.x)
+      if (H[h]) atomicAdd((unsigned long long*)&bcnt[bb + h], (unsigned long long)H[h]);
+    __syncthreads();
+  }
+}
+
+// the entries of a chunk into their buckets: K = x << 32 | position, W = weight
+// (order within a bucket is arbitrary; the bucket's result does not depend on it)
+__global__ __launch_bounds__(256) void k_rl_scatter(const i32* lng, const i32* ch_row,
+                                                    const i64* ch_beg, i64 nch, const i64* dsz,
+                                                    const i64* eoff, const i32* lgb,
+                                                    const i64* bbase, const i64* boff,
+                                                    u32* bcur, const u64* R, u64* K, i32* W) {
+  __shared__ i32 H[1 << RL_BK_LGMAX];
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i32 e0 = ch_row[k2];
+    const i64 c = lng[e0];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 end = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const int lg = lgb[e0];
+    const i32 nb = 1 << lg;
+    const i64 bb = bbase[e0];
+    for (i32 h = threadIdx.x; h < nb; h += blockDim.x) H[h] = 0;
+    __syncthreads();
+    const u64* Rc = R + eoff[c];
+    u64 rv[RL_PER];
+    i32 rk[RL_PER];
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const i64 p = b + j * 256 + threadIdx.x;
+      rv[j] = p < end ? Rc[p] : ((u64)RL_NONE << 32);
+    }
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const u32 x = (u32)(rv[j] >> 32);
+      rk[j] = x != RL_NONE ? atomicAdd(&H[rl_bucket(x, lg)], 1) : -1;
+    }
+    __syncthreads();
+    // counts become this chunk's first slot within each bucket
+    for (i32 h = threadIdx.x; h < nb; h += blockDim.x)
+      if (H[h]) H[h] = (i32)atomicAdd(&bcur[bb + h], (u32)H[h]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      if (rk[j] < 0) continue;
+      const u32 x = (u32)(rv[j] >> 32);
+      const u32 bk = rl_bucket(x, lg);
+      const i64 o = boff[bb + bk] + H[bk] + rk[j];
+      const i64 p = b + j * 256 + threadIdx.x;
+      K[o] = ((u64)x << 32) | (u64)(u32)p;
+      W[o] = (i32)(u32)rv[j];
+    }
+    __syncthreads();
+  }
+}
+
+// One block per bucket: first position, weight sum and multiplicity of each
+// neighbour in an LDS table; R of repeated neighbours rewritten.  Records are
+// taken 2048 at a time (8 per thread); a bucket of at most 2048 records keeps
+// them in registers between the two passes.
+constexpr int RL_SLICE = 2048;
+__device__ __forceinline__ void rl_insert(const u64* kk, const i32* ww, i32* TK, i32* TP,
+                                          i32* TW, u8* TD, i32* s_n, i32* s_over, i32 fill) {
+#pragma unroll
+  for (int j = 0; j < RL_SLICE / 256; j++) {
+    // once the table holds `fill` keys nothing more is inserted: at most one
+    // insert per thread is then still in flight, so a free slot always remains
+    // (fill + 256 < RL_TCAP) and every probe ends
+    if (kk[j] == ~0ull || *(volatile i32*)s_over) continue;
+    const i32 x = (i32)(kk[j] >> 32), p = (i32)(u32)kk[j];
+    u32 h = hash32((u32)x) & (RL_TCAP - 1);
+    while (true) {
+      const i32 old = atomicCAS(&TK[h], -1, x);
+      if (old == -1) {
+        if (atomicAdd(s_n, 1) >= fill) *s_over = 1;
+        break;
+      }
+      if (old == x) {
+        TD[h] = 1;
+        break;
+      }
+      h = (h + 1) & (RL_TCAP - 1);
+    }
+    atomicMin(&TP[h], p);
+    atomicAdd(&TW[h], ww[j]);
+  }
+}
+__device__ __forceinline__ void rl_apply(const u64* kk, const i32* TK, const i32* TP,
+                                         const i32* TW, const u8* TD, u64* Rc) {
+#pragma unroll
+  for (int j = 0; j < RL_SLICE / 256; j++) {
+    if (kk[j] == ~0ull) continue;
+    const i32 x = (i32)(kk[j] >> 32), p = (i32)(u32)kk[j];
+    u32 h = hash32((u32)x) & (RL_TCAP - 1);
+    while (TK[h] != x) h = (h + 1) & (RL_TCAP - 1);
+    if (!TD[h]) continue;
+    Rc[p] = TP[h] == p ? (((u64)(u32)x << 32) | (u64)(u32)TW[h]) : ((u64)RL_NONE << 32);
+  }
+}
+__device__ __forceinline__ void rl_load(const u64* K, const i32* W, i64 base, i64 r1, u64* kk,
+                                        i32* ww) {
+#pragma unroll
+  for (int j = 0; j < RL_SLICE / 256; j++) {
+    const i64 i = base + j * 256 + threadIdx.x;
+    kk[j] = i < r1 ? K[i] : ~0ull;
+    ww[j] = i < r1 ? W[i] : 0;
+  }
+}
+__global__ __launch_bounds__(256) void k_rl_dedup(const i64* boff, i64 nbt, const i32* brow,
+                                                  const i32* lng, const i64* eoff, const u64* K,
+                                                  const i32* W, u64* R, i32 fill, i32* over) {
+  __shared__ i32 TK[RL_TCAP], TP[RL_TCAP], TW[RL_TCAP];
+  __shared__ u8 TD[RL_TCAP];
+  __shared__ i32 s_n, s_over;
+  for (i64 bk = blockIdx.x; bk < nbt; bk += gridDim.x) {
+    const i64 r0 = boff[bk], r1 = boff[bk + 1];
+    if (r1 == r0) continue;  // uniform over the block
+    u64* Rc = R + eoff[lng[brow[bk]]];
+    for (int h = threadIdx.x; h < RL_TCAP; h += blockDim.x) {
+      TK[h] = -1;
+      TP[h] = INT_MAX;
+      TW[h] = 0;
+      TD[h] = 0;
+    }
+    if (threadIdx.x == 0) {
+      s_n = 0;
+      s_over = 0;
+    }
+    __syncthreads();
+    u64 kk[RL_SLICE / 256];
+    i32 ww[RL_SLICE / 256];
+    const bool one = r1 - r0 <= RL_SLICE;
+    bool ov = false;
+    for (i64 base = r0; base < r1; base += RL_SLICE) {
+      rl_load(K, W, base, r1, kk, ww);
+      rl_insert(kk, ww, TK, TP, TW, TD, &s_n, &s_over, fill);
+      __syncthreads();
+      ov = s_over != 0;
+      __syncthreads();  // every thread has read s_over before the next slice may set it
+      if (ov) break;
+    }
+    if (ov) {
+      if (threadIdx.x == 0) atomicOr(over, 1);
+      __syncthreads();
+      continue;
+    }
+    if (one) {
+      rl_apply(kk, TK, TP, TW, TD, Rc);
+    } else {
+      for (i64 base = r0; base < r1; base += RL_SLICE) {
+        rl_load(K, W, base, r1, kk, ww);
+        rl_apply(kk, TK, TP, TW, TD, Rc);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// entries left in R per chunk (chunk_cnt), then the chunk's entries written in
+// position order at ox[c] + (the earlier chunks' counts)
+__global__ __launch_bounds__(256) void k_rl_count(const i32* lng, const i32* ch_row,
+                                                  const i64* ch_beg, i64 nch, const i64* dsz,
+                                                  const i64* eoff, const u64* R,
+                                                  i64* chunk_cnt) {
+  __shared__ i32 s_c[256 / WAVE];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i64 c = lng[ch_row[k2]];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 end = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const u64* Rc = R + eoff[c];
+    i32 cnt = 0;
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const i64 p = b + j * 256 + threadIdx.x;
+      cnt += (p < end && (u32)(Rc[p] >> 32) != RL_NONE) ? 1 : 0;
+    }
+    cnt = wave_sum(cnt);
+    if (lane == 0) s_c[wv] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      i64 t = 0;
+      for (int w2 = 0; w2 < 256 / WAVE; w2++) t += s_c[w2];
+      chunk_cnt[k2] = t;
+    }
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(256) void k_rl_emit(const i32* lng, const i32* ch_row,
+                                                 const i64* ch_beg, i64 nch, const i64* dsz,
+                                                 const i64* cbase, const i64* choff,
+                                                 const i64* eoff, const u64* R, const i64* ox,
+                                                 i32* oa, i32* ow) {
+  __shared__ i32 s_w[256 / WAVE];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 k2 = blockIdx.x; k2 < nch; k2 += gridDim.x) {
+    const i32 e0 = ch_row[k2];
+    const i64 c = lng[e0];
+    const i64 d = dsz[c];
+    const i64 b = ch_beg[k2];
+    const i64 end = b + REP_CHUNK < d ? b + REP_CHUNK : d;
+    const u64* Rc = R + eoff[c];
+    u64 rv[RL_PER];
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const i64 p = b + j * 256 + threadIdx.x;
+      rv[j] = p < end ? Rc[p] : ((u64)RL_NONE << 32);
+    }
+    i64 run = ox[c] + (choff[k2] - choff[cbase[e0]]);
+#pragma unroll
+    for (int j = 0; j < RL_PER; j++) {
+      const bool keep = (u32)(rv[j] >> 32) != RL_NONE;
+      const u64 bm = __ballot(keep);
+      if (lane == 0) s_w[wv] = __popcll(bm);
+      __syncthreads();
+      i64 before = 0, tot = 0;
+      for (int w2 = 0; w2 < 256 / WAVE; w2++) {
+        if (w2 < wv) before += s_w[w2];
+        tot += s_w[w2];
+      }
+      if (keep) {
+        const i64 o = run + before + __popcll(bm & lanes_below(lane));
+        oa[o] = (i32)(rv[j] >> 32);
+        ow[o] = (i32)(u32)rv[j];
+      }
+      run += tot;
+      __syncthreads();
+    }
+  }
+}
+
 // Rows with d <= SG entries: SG lanes per row, one entry per lane.  An entry
 // is the first occurrence of its coarse neighbour x when no earlier lane holds
 // x; its weight is the sum over all lanes holding x.  PASS 0 writes the row's
@@ -3481,6 +3796,46 @@ void launch_rep_long(DGraph g, const i32* tb, const i32* new_ids, const RepLong&
     k_rep_emit<<<gr, 256, 0, s>>>(L.lng, L.ch_row, L.ch_beg, L.nch, L.dsz, L.cbase, L.choff,
                                   L.toff, L.eoff, L.T, L.W, L.E, ox, oa, ow);
   }
+}
+void launch_rl_buckets(const RepLong& L, i64 target, i32* lgb, i64* nbk, hipStream_t s) {
+  if (L.nl > 0)
+    k_rl_buckets<<<grid_for(L.nl, 256, 4096), 256, 0, s>>>(L.lng, L.nl, L.dsz,
+                                                           target > 0 ? target : RL_BK_TARGET,
+                                                           lgb, nbk);
+}
+void launch_rl_brow(i64 nl, const i64* bbase, i32* brow, hipStream_t s) {
+  if (nl > 0) k_rl_brow<<<grid_for(nl, 256, 4096), 256, 0, s>>>(nl, bbase, brow);
+}
+void launch_rl_fetch(DGraph g, const i32* tb, const i32* new_ids, const RepLong& L,
+                     const RepBuckets& B, hipStream_t s) {
+  if (L.nch <= 0) return;
+  k_rl_fetch<<<grid_for(L.nch, 1, 1u << 16), 256, 0, s>>>(g, tb, new_ids, L.lng, L.ch_row,
+                                                          L.ch_beg, L.nch, L.dsz, L.eoff, B.lgb,
+                                                          B.bbase, B.R, B.bcnt);
+}
+void launch_rl_scatter(const RepLong& L, const RepBuckets& B, hipStream_t s) {
+  if (L.nch <= 0) return;
+  k_rl_scatter<<<grid_for(L.nch, 1, 1u << 16), 256, 0, s>>>(L.lng, L.ch_row, L.ch_beg, L.nch,
+                                                            L.dsz, L.eoff, B.lgb, B.bbase,
+                                                            B.boff, B.bcur, B.R, B.K, B.W);
+}
+void launch_rl_dedup(const RepLong& L, const RepBuckets& B, i32 fill, hipStream_t s) {
+  if (B.nbt <= 0) return;
+  fill = std::max<i32>(1, std::min<i32>(fill, RL_TFILL));
+  k_rl_dedup<<<grid_for(B.nbt, 1, 1u << 16), 256, 0, s>>>(B.boff, B.nbt, B.brow, L.lng, L.eoff,
+                                                          B.K, B.W, B.R, fill, B.over);
+}
+void launch_rl_count(const RepLong& L, const RepBuckets& B, hipStream_t s) {
+  if (L.nch <= 0) return;
+  k_rl_count<<<grid_for(L.nch, 1, 1u << 16), 256, 0, s>>>(L.lng, L.ch_row, L.ch_beg, L.nch,
+                                                          L.dsz, L.eoff, B.R, L.chunk_cnt);
+}
+void launch_rl_emit(const RepLong& L, const RepBuckets& B, const i64* ox, i32* oa, i32* ow,
+                    hipStream_t s) {
+  if (L.nch <= 0) return;
+  k_rl_emit<<<grid_for(L.nch, 1, 1u << 16), 256, 0, s>>>(L.lng, L.ch_row, L.ch_beg, L.nch, L.dsz,
+                                                         L.cbase, L.choff, L.eoff, B.R, ox, oa,
+                                                         ow);
 }
 void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s) {
   if (L.nl > 0)
