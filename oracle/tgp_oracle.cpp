@@ -660,6 +660,10 @@ int tgpo_partition(int64_t n, const int64_t* xadj, const int32_t* adjncy,
   int64_t ncur = n;
   std::vector<int32_t> match, new_ids, tb;
   while (ncur > target) {                                   // 1003-1152
+    // repaired mode: a contraction that kept more than 93 % of the vertices
+    // ends the coarsening before the next matching (whose level would keep
+    // more than 95 % on power-law graphs and be dropped by the rule below)
+    if (opt->repair && L.size() >= 2 && 100 * L.back().n > 93 * L[L.size() - 2].n) break;
     const View g = view_of(L.back());
     match.resize(ncur);
     new_ids.resize(ncur);

@@ -698,6 +698,10 @@ void Run::coarsen() {
   DBuf<i32> mate(&C.pool, n0), new_ids(&C.pool, n0);
   DBuf<i64> flags(&C.pool, n0), pos(&C.pool, n0);
   while (ncur > target) {  // gpu_part.cu:1003
+    // repaired mode: a contraction that kept more than 93 % of the vertices
+    // ends the coarsening before the next matching (whose level would keep
+    // more than 95 % on power-law graphs and be dropped by the stall stop)
+    if (repair && L.size() >= 2 && 100 * L.back()->n > 93 * L[L.size() - 2]->n) break;
     Level& lv = *L.back();
     match_level(lv, mate);
     auto tb = DBuf<i32>(&C.pool, 2 * ncur);
@@ -754,6 +758,8 @@ void Run::coarsen() {
       DBuf<u64> bcnt, R, BK;
       DBuf<u32> bcur;
       DBuf<i32> BW;
+      // entry words of the rows past 64 entries (medium and long)
+      if (etot > 0) R = DBuf<u64>(&C.pool, etot);
       if (nlng > 0) {
         launch_scan_excl(nchk.p, cbase.p, nlng, cbase.p + nlng, scratch(), st());
         const i64 nch = read_i64(cbase.p + nlng);
@@ -784,7 +790,6 @@ void Run::coarsen() {
           boff = DBuf<i64>(&C.pool, nbt + 1);
           bcur = DBuf<u32>(&C.pool, nbt);
           over = DBuf<i32>(&C.pool, 1);
-          R = DBuf<u64>(&C.pool, etot);
           BK = DBuf<u64>(&C.pool, etot);
           BW = DBuf<i32>(&C.pool, etot);
           RB.lgb = lgb.p;
@@ -813,7 +818,6 @@ void Run::coarsen() {
           BK = DBuf<u64>();
           BW = DBuf<i32>();
           if (bucketed) launch_rl_count(RL, RB, st());
-          else R = DBuf<u64>();
         }
         if (!bucketed) {
           T = DBuf<u64>(&C.pool, ttot);
@@ -831,15 +835,15 @@ void Run::coarsen() {
         launch_scan_excl(chunk_cnt.p, choff.p, nch, choff.p + nch, scratch(), st());
         launch_rep_rowcnt(RL, cnt.p, st());
       }
-      launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, 0, cnt.p, nullptr, nullptr,
-                      nullptr, st());
+      launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, eoff.p, R.p, 0, cnt.p,
+                      nullptr, nullptr, nullptr, st());
       launch_scan_excl(cnt.p, c->xadj.p, n_c, c->xadj.p + n_c, scratch(), st());
       m_c = read_i64(c->xadj.p + n_c);
       c->adj = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
       c->w = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
       if (m_c > 0) {
-        launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, 1, nullptr, c->xadj.p,
-                        c->adj.p, c->w.p, st());
+        launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, eoff.p, R.p, 1, nullptr,
+                        c->xadj.p, c->adj.p, c->w.p, st());
         if (bucketed) launch_rl_emit(RL, RB, c->xadj.p, c->adj.p, c->w.p, st());
         else launch_rep_long(g, tb.p, new_ids.p, RL, 1, c->xadj.p, c->adj.p, c->w.p, st());
       }

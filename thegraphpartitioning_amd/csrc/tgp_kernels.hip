@@ -1413,7 +1413,7 @@ __global__ void k_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz
     if (mt >= 0) d += xadj[mt + 1] - xadj[mt];
     dsz[c] = d;
     const bool is_long = d > REP_MED;
-    dlong[c] = is_long ? d : 0;
+    dlong[c] = d > REP_SMALL ? d : 0;
     csz[c] = is_long ? next_pow2_i64(2 * d) : 0;
     if (is_long) {
       const i32 e = atomicAdd(nlng, 1);
@@ -1942,15 +1942,18 @@ __global__ __launch_bounds__(256) void k_rep_small(DGraph g, const i32* tb, cons
   }
 }
 
-// Rows with REP_SMALL < d <= REP_MED: one block per row, an LDS table of
-// capacity pow2 >= 2d recording each x's first position (atomicMin) and
-// weight sum (atomicAdd); PASS 0 counts the first occurrences, PASS 1 writes
-// them in position order (block-wide scan over 256-entry chunks).
+// Rows with REP_SMALL < d <= REP_MED: one block per row.  Pass 0 loads the
+// row's entries (8 per thread, all loads issued before the gathers), records
+// each coarse neighbour's first position and weight sum in an LDS table of
+// capacity pow2 >= 2d, writes every entry's word R[eoff[c] + p] (x << 32 |
+// summed weight for a first occurrence, x = ~0 otherwise) and the row's count;
+// pass 1 writes the kept words in position order (block-wide ranks over
+// 256-entry steps) without touching the fine rows again.
 constexpr int REP_TCAP = 4096;
-template <int PASS>
-__global__ __launch_bounds__(256) void k_rep_med(DGraph g, const i32* tb, const i32* new_ids,
-                                                 const i32* lst, i64 nl, const i64* dsz,
-                                                 i64* cnt, const i64* ox, i32* oa, i32* ow) {
+constexpr int REP_MED_PER = (int)(REP_MED / 256);
+__global__ __launch_bounds__(256) void k_rep_med0(DGraph g, const i32* tb, const i32* new_ids,
+                                                  const i32* lst, i64 nl, const i64* dsz,
+                                                  const i64* eoff, u64* R, i64* cnt) {
   __shared__ i32 K[REP_TCAP], P[REP_TCAP], W[REP_TCAP];
   __shared__ i32 s_w[256 / WAVE];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -1967,54 +1970,97 @@ __global__ __launch_bounds__(256) void k_rep_med(DGraph g, const i32* tb, const 
       P[h] = INT_MAX;
       W[h] = 0;
     }
-    __syncthreads();
-    for (i64 p = threadIdx.x; p < d; p += blockDim.x) {
+    i32 x[REP_MED_PER], w[REP_MED_PER];
+#pragma unroll
+    for (int j = 0; j < REP_MED_PER; j++) {
+      const i64 p = j * 256 + threadIdx.x;
       const i64 i = p < la ? a0 + p : b0 + (p - la);
-      const i32 x = new_ids[g.adj[i]];
-      if (x == (i32)c || x == -1) continue;
-      u32 h = hash32((u32)x) & mask;
+      x[j] = p < d ? g.adj[i] : -1;
+      w[j] = p < d ? g.w[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < REP_MED_PER; j++) {
+      x[j] = x[j] >= 0 ? new_ids[x[j]] : -1;
+      if (x[j] == (i32)c) x[j] = -1;
+    }
+    __syncthreads();  // table cleared
+#pragma unroll
+    for (int j = 0; j < REP_MED_PER; j++) {
+      if (x[j] < 0) continue;
+      u32 h = hash32((u32)x[j]) & mask;
       while (true) {
-        const i32 old = atomicCAS(&K[h], -1, x);
-        if (old == -1 || old == x) break;
+        const i32 old = atomicCAS(&K[h], -1, x[j]);
+        if (old == -1 || old == x[j]) break;
         h = (h + 1) & mask;
       }
-      atomicMin(&P[h], (i32)p);
-      atomicAdd(&W[h], g.w[i]);
+      atomicMin(&P[h], j * 256 + (i32)threadIdx.x);
+      atomicAdd(&W[h], w[j]);
     }
     __syncthreads();
-    i64 run = PASS ? ox[c] : 0;
-    for (i64 base = 0; base < d; base += blockDim.x) {
-      const i64 p = base + threadIdx.x;
-      bool first = false;
-      i32 x = -1;
-      u32 h = 0;
-      if (p < d) {
-        const i64 i = p < la ? a0 + p : b0 + (p - la);
-        x = new_ids[g.adj[i]];
-        if (!(x == (i32)c || x == -1)) {
-          h = hash32((u32)x) & mask;
-          while (K[h] != x) h = (h + 1) & mask;
-          first = P[h] == (i32)p;
+    u64* Rc = R + eoff[c];
+    i32 kept = 0;
+#pragma unroll
+    for (int j = 0; j < REP_MED_PER; j++) {
+      const i64 p = j * 256 + threadIdx.x;
+      if (p >= d) continue;
+      u64 word = (u64)RL_NONE << 32;
+      if (x[j] >= 0) {
+        u32 h = hash32((u32)x[j]) & mask;
+        while (K[h] != x[j]) h = (h + 1) & mask;
+        if (P[h] == (i32)p) {
+          word = ((u64)(u32)x[j] << 32) | (u64)(u32)W[h];
+          kept++;
         }
       }
-      const u64 bm = __ballot(first);
+      Rc[p] = word;
+    }
+    kept = wave_sum(kept);
+    if (lane == 0) s_w[wv] = kept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      i64 t = 0;
+      for (int w2 = 0; w2 < 256 / WAVE; w2++) t += s_w[w2];
+      cnt[c] = t;
+    }
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(256) void k_rep_med1(const i32* lst, i64 nl, const i64* dsz,
+                                                  const i64* eoff, const u64* R, const i64* ox,
+                                                  i32* oa, i32* ow) {
+  __shared__ i32 s_w[256 / WAVE];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (i64 e = blockIdx.x; e < nl; e += gridDim.x) {
+    const i64 c = lst[e];
+    const i64 d = dsz[c];
+    const u64* Rc = R + eoff[c];
+    u64 rv[REP_MED_PER];
+#pragma unroll
+    for (int j = 0; j < REP_MED_PER; j++) {
+      const i64 p = j * 256 + threadIdx.x;
+      rv[j] = p < d ? Rc[p] : ((u64)RL_NONE << 32);
+    }
+    i64 run = ox[c];
+#pragma unroll
+    for (int j = 0; j < REP_MED_PER; j++) {
+      if (j * 256 >= d) break;  // uniform over the block
+      const bool keep = (u32)(rv[j] >> 32) != RL_NONE;
+      const u64 bm = __ballot(keep);
       if (lane == 0) s_w[wv] = __popcll(bm);
       __syncthreads();
       i64 before = 0, tot = 0;
-      for (int w2 = 0; w2 < (int)(blockDim.x / WAVE); w2++) {
+      for (int w2 = 0; w2 < 256 / WAVE; w2++) {
         if (w2 < wv) before += s_w[w2];
         tot += s_w[w2];
       }
-      if (PASS == 1 && first) {
+      if (keep) {
         const i64 o = run + before + __popcll(bm & lanes_below(lane));
-        oa[o] = x;
-        ow[o] = W[h];
+        oa[o] = (i32)(rv[j] >> 32);
+        ow[o] = (i32)(u32)rv[j];
       }
       run += tot;
       __syncthreads();
     }
-    if (PASS == 0 && threadIdx.x == 0) cnt[c] = run;
-    __syncthreads();
   }
 }
 
@@ -3842,18 +3888,18 @@ void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s) {
     k_rep_rowcnt<<<grid_for(L.nl, 256, 4096), 256, 0, s>>>(L.lng, L.nl, L.cbase, L.choff, cnt);
 }
 void launch_rep_rows(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* dsz,
-                     const i32* med, i64 nmed, int pass, i64* cnt, const i64* ox, i32* oa,
-                     i32* ow, hipStream_t s) {
+                     const i32* med, i64 nmed, const i64* eoff, u64* R, int pass, i64* cnt,
+                     const i64* ox, i32* oa, i32* ow, hipStream_t s) {
   const unsigned g8 = grid_for(n_c, 256 / 8, 1u << 16), g64 = grid_for(n_c, 4, 1u << 16);
   const unsigned gm = grid_for(nmed, 1, 1u << 16);
   if (pass == 0) {
     k_rep_small<8, 0><<<g8, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 0, cnt, ox, oa, ow);
     k_rep_small<64, 0><<<g64, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 9, cnt, ox, oa, ow);
-    if (nmed > 0) k_rep_med<0><<<gm, 256, 0, s>>>(g, tb, new_ids, med, nmed, dsz, cnt, ox, oa, ow);
+    if (nmed > 0) k_rep_med0<<<gm, 256, 0, s>>>(g, tb, new_ids, med, nmed, dsz, eoff, R, cnt);
   } else {
     k_rep_small<8, 1><<<g8, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 0, cnt, ox, oa, ow);
     k_rep_small<64, 1><<<g64, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 9, cnt, ox, oa, ow);
-    if (nmed > 0) k_rep_med<1><<<gm, 256, 0, s>>>(g, tb, new_ids, med, nmed, dsz, cnt, ox, oa, ow);
+    if (nmed > 0) k_rep_med1<<<gm, 256, 0, s>>>(med, nmed, dsz, eoff, R, ox, oa, ow);
   }
 }
 void launch_pairs(DGraph g, u64* table, i64 cap, u64* count, hipStream_t s) {
