@@ -304,6 +304,8 @@ struct Level {
   i64 n = 0, m = 0;
   bool symmetric = false;  // every arc s->x (s != x) has its reverse
   bool simple_ws = false;  // and every row is simple and each reverse has the same weight
+  bool sorted = false;     // every row strictly ascending (checked on level 0 only)
+  bool unitw = false;      // every arc weight is 1 (checked on level 0 only)
   LevelAux aux;
   DBuf<i64> xadj;
   DBuf<i32> adj, w;
@@ -574,6 +576,11 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
       if (tot > 0) launch_chain_low_write(g, llist.p, nll.p, lowoff.p, lowadj.p, st());
       low = ChainLow{lowoff.p, lowadj.p};
     }
+    // TGP_CHAIN_EXACT_SCANS=0 keeps the full scans on sorted / unit-weight levels
+    if (env_i64("TGP_CHAIN_EXACT_SCANS", 1) != 0) {
+      low.sorted = lv.sorted ? 1 : 0;
+      low.unitw = lv.unitw ? 1 : 0;
+    }
     C.prof.end(KID_MATCH, 0);
     i64 prev = -1;
     int stuck = 0;
@@ -655,16 +662,18 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
 }
 
 void Run::check_symmetric(Level& lv) {
-  DBuf<i32> flags(&C.pool, 4);
+  DBuf<i32> flags(&C.pool, 5);
   DBuf<u64> cnt(&C.pool, 2);
-  HIP_TRY(hipMemsetAsync(flags.p, 0, 4 * sizeof(i32), st()));
+  HIP_TRY(hipMemsetAsync(flags.p, 0, 5 * sizeof(i32), st()));
   HIP_TRY(hipMemsetAsync(cnt.p, 0, 2 * sizeof(u64), st()));
   C.prof.begin(KID_MISC);
   launch_rows_strict(lv.dg(), flags.p, st());
   C.prof.end(KID_MISC, 4.0 * lv.m);
-  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 5 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
   const bool strict = ((const i32*)C.pinned)[3] == 0;
+  lv.sorted = strict;
+  lv.unitw = ((const i32*)C.pinned)[4] == 0;
   // strictly ascending rows hold no duplicates: each arc down finds its
   // reverse by binary search in the sorted row; other rows go through a hash
   // table of every arc

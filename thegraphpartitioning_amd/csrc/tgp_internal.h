@@ -96,6 +96,8 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
 struct ChainLow {
   const i64* off;  // n + 1: entries of v at adj[off[v], off[v+1]) (empty for short rows)
   const i32* adj;  // the entries u < v of row v, in row order
+  int sorted = 0;  // every row is strictly ascending (so are the lists above)
+  int unitw = 0;   // every arc weight is 1 (the heaviest free neighbour is the first one)
 };
 void launch_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA, i32* blkC, i32* prop,
                         const i64* lo, u32* ticket, i64 budget, int blocks, int sg, int per,
@@ -183,7 +185,8 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 // flags[0] |= some arc s->x (s != x, x < n) has no reverse; flags[1] |= some
 // (s, x) appears twice in s's row; flags[2] |= some reverse has another weight.
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s);
-// flags[3] |= some row is not strictly ascending
+// flags[3] |= some row is not strictly ascending; flags[4] |= some arc weight
+// is not 1
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
 // the same checks for strictly ascending rows, by binary search of the
 // reverse arc in its (sorted) row: flags[0], flags[2] as above; cnt[0] = arcs

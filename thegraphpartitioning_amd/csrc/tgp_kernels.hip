@@ -319,6 +319,7 @@ constexpr i32 BLK_NONE = -2;
 // Row scans issue SCAN_U independent loads per lane before using any of them,
 // so a long row costs about len/(64*SCAN_U) memory round trips.
 constexpr int SCAN_U = 32;
+constexpr int TU = 16;  // loads per lane per step of the proposal scans (fewer registers)
 
 // Device-scope (coherent) accesses to the matching state, for kernels in
 // which other waves update it while they run.
@@ -659,7 +660,8 @@ __device__ __forceinline__ i32 ld_st(const i32* st, i64 i) { return ld_mate(st, 
 // The entries scanned are adj[s0, s1): the row of x, or its lower-neighbour
 // list (ChainLow), which holds every entry of the row below x.
 __device__ __forceinline__ i32 wave_scan_pending(const i32* adj, i64 s0, i64 s1, i32 x, i32 lim,
-                                                 i32 me, const i32* st, int lane, i32& from) {
+                                                 i32 me, const i32* st, int lane, i32& from,
+                                                 bool sorted = false) {
   for (i64 base = s0 + from; base < s1; base += WAVE * SCAN_U) {
     i32 v[SCAN_U];
 #pragma unroll
@@ -687,9 +689,55 @@ __device__ __forceinline__ i32 wave_scan_pending(const i32* adj, i64 s0, i64 s1,
       from = (i32)(base - s0);
       return best;
     }
+    // ascending entries: once a trip reaches lim, no later entry is below it
+    if (sorted && __ballot(v[SCAN_U - 1] >= lim)) break;
   }
   from = (i32)(s1 - s0);
   return BLK_NONE;
+}
+
+// Unit-weight levels: the heaviest free neighbour of t (first position on
+// ties) is its first free entry, v != t, 0 <= v < n, st[v] < 0.  Entries
+// before `from` are not free and stay so (a taken vertex stays taken), so the
+// scan resumes there; returns the neighbour (-1 if none) with its state in
+// sval, and leaves `from` at its position.
+__device__ __forceinline__ i32 wave_first_free(const DGraph& g, i32 t, const i32* st, int lane,
+                                               i32& from, i32& sval) {
+  const i64 s0 = g.xadj[t], s1 = g.xadj[t + 1];
+  for (i64 base = s0 + from; base < s1; base += WAVE * TU) {
+    i32 v[TU], sv[TU];
+#pragma unroll
+    for (int j = 0; j < TU; j++) {
+      const i64 i = base + j * WAVE + lane;
+      v[j] = i < s1 ? g.adj[i] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < TU; j++)
+      sv[j] = (v[j] >= 0 && v[j] < g.n && v[j] != t) ? ld_st(st, v[j]) : 0;
+    i32 key = INT_MAX, myv = -1, mys = 0;
+#pragma unroll
+    for (int j = TU - 1; j >= 0; j--) {
+      if (sv[j] < 0) {
+        key = j * WAVE + lane;
+        myv = v[j];
+        mys = sv[j];
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const i32 k2 = __shfl_xor(key, o);
+      key = k2 < key ? k2 : key;
+    }
+    if (key != INT_MAX) {
+      const int src = key & (WAVE - 1);
+      from = (i32)(base - s0) + key;
+      sval = __shfl(mys, src);
+      return __shfl(myv, src);
+    }
+  }
+  from = (i32)(s1 - s0);
+  sval = 0;
+  return -1;
 }
 
 // The K best free (st < 0) neighbours of t by (weight, first position), as
@@ -700,7 +748,6 @@ __device__ __forceinline__ i32 wave_scan_pending(const i32* adj, i64 s0, i64 s1,
 // time is the first still-free valid key (when one is left).  Each lane keeps
 // its own 4 best; the wave merges them one key per round, and stops after a
 // round that empties a lane which saw more than 4 (its fifth might be next).
-constexpr int TU = 16;  // loads per lane per step (fewer registers than SCAN_U)
 template <int K>
 __device__ __forceinline__ void wave_best_free_topk(const DGraph& g, i32 t, const i32* st,
                                                     int lane, int sgbase, u64& mykey,
@@ -797,8 +844,9 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
   i64 base = -1;
   // per-vertex state, uniform within a sub-group
   bool own = false, tshort = false, ushort_ = false, uknown = false;
-  // long rows: where the next A scan of t's row and C scan of u's row resume
-  i32 apos = 0, cpos = 0;
+  // long rows: where the next A scan of t's row and C scan of u's row resume,
+  // and (unit-weight levels) the next proposal scan of t's row
+  i32 apos = 0, cpos = 0, ppos = 0;
   i32 t = -1, a = BLK_NONE, u = -1, b = BLK_NONE;
   // long rows: lanes 0..pnv-1 of the sub-group hold the best free candidates
   // of the last full proposal scan (pci), ph = the entry now proposed (PK:
@@ -824,6 +872,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       uknown = false;
       cpos = 0;
       apos = 0;
+      ppos = 0;
       pci = -1;
       ph = PK;
       pnv = 0;
@@ -925,7 +974,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       const i64 l0 = low.off ? low.off[x] : g.xadj[x], l1 = low.off ? low.off[x + 1] : g.xadj[x + 1];
       c_aL++;
       c_arcsL += l1 - l0 - from;
-      const i32 r = wave_scan_pending(la, l0, l1, x, x, x, st, lane, from);
+      const i32 r = wave_scan_pending(la, l0, l1, x, x, x, st, lane, from, low.sorted);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) own = false;
         else a = r;
@@ -955,6 +1004,24 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       }
     }
     need = __ballot(own && !tshort && sl == 0 && u == BLK_UNKNOWN);
+    while (need && low.unitw) {
+      const int l = __ffsll((unsigned long long)need) - 1;
+      const i32 x = __shfl(t, l);
+      i32 from = __shfl(ppos, l), sr = 0;
+      c_pL++;
+      c_arcsP -= from;
+      const i32 r0 = wave_first_free(g, x, st, lane, from, sr);
+      c_arcsP += from;
+      if ((lane & ~(CH_SG - 1)) == l) {
+        ppos = from;
+        u = r0 >= 0 ? r0 : PROP_NONE;
+        s_u = r0 >= 0 ? sr : ST_UNDEC;
+        b = BLK_UNKNOWN;
+        uknown = false;
+        cpos = 0;
+      }
+      need &= need - 1;
+    }
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
@@ -1044,7 +1111,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       const i64 l0 = uselow ? low.off[x] : g.xadj[x], l1 = uselow ? low.off[x + 1] : g.xadj[x + 1];
       c_cL++;
       c_arcsC += l1 - l0 - from;
-      const i32 r = wave_scan_pending(la, l0, l1, x, lim, x, st, lane, from);
+      const i32 r = wave_scan_pending(la, l0, l1, x, lim, x, st, lane, from, low.sorted);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) {
           u = BLK_UNKNOWN;
@@ -1241,13 +1308,17 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, c
 
 // flags[3] |= some row is not strictly ascending (so it may hold duplicates)
 __global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
-  bool bad = false;
+  bool bad = false, nonunit = false;
   const int lane = lane_id();
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
-    for (i64 i = s0 + 1 + lane; i < s1; i += WAVE) bad |= g.adj[i] <= g.adj[i - 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      bad |= i > s0 && g.adj[i] <= g.adj[i - 1];
+      nonunit |= g.w[i] != 1;
+    }
   }
   if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
+  if (__ballot(nonunit) && lane == 0) atomicOr(&flags[4], 1);
 }
 
 // Strictly ascending rows: each arc v->u with u > v is counted (cnt[0]); each
