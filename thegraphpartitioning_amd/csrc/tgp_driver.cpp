@@ -669,11 +669,10 @@ void Run::check_symmetric(Level& lv) {
   C.prof.begin(KID_MISC);
   launch_rows_strict(lv.dg(), flags.p, st());
   C.prof.end(KID_MISC, 4.0 * lv.m);
-  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 5 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
   const bool strict = ((const i32*)C.pinned)[3] == 0;
   lv.sorted = strict;
-  lv.unitw = ((const i32*)C.pinned)[4] == 0;
   // strictly ascending rows hold no duplicates: each arc down finds its
   // reverse by binary search in the sorted row; other rows go through a hash
   // table of every arc
@@ -690,13 +689,16 @@ void Run::check_symmetric(Level& lv) {
     launch_sym_check(lv.dg(), table.p, wt.p, cap, flags.p, st());
   }
   C.prof.end(KID_MISC, (strict ? 14.0 : 28.0) * lv.m);
-  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
-  HIP_TRY(hipMemcpyAsync(C.pinned + 2, cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 5 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned + 3, cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, st()));
   C.sync();
   const i32* f = (const i32*)C.pinned;
-  const bool sym = f[0] == 0 && (!strict || C.pinned[2] == C.pinned[3]);
+  const bool sym = f[0] == 0 && (!strict || C.pinned[3] == C.pinned[4]);
   lv.symmetric = sym;
   lv.simple_ws = sym && f[1] == 0 && f[2] == 0;
+  // unit weights are read off the sorted check's found pairs (all arcs of a
+  // symmetric level between distinct vertices in range)
+  lv.unitw = sym && strict && f[4] == 0;
 }
 
 void Run::coarsen() {

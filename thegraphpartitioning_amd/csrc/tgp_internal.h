@@ -185,13 +185,12 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 // flags[0] |= some arc s->x (s != x, x < n) has no reverse; flags[1] |= some
 // (s, x) appears twice in s's row; flags[2] |= some reverse has another weight.
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s);
-// flags[3] |= some row is not strictly ascending; flags[4] |= some arc weight
-// is not 1
+// flags[3] |= some row is not strictly ascending
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
 // the same checks for strictly ascending rows, by binary search of the
 // reverse arc in its (sorted) row: flags[0], flags[2] as above; cnt[0] = arcs
 // up, cnt[1] = arcs down whose reverse was found (symmetric iff flags[0] == 0
-// and they agree)
+// and they agree); flags[4] |= a found pair's weight is not 1
 void launch_sym_check_sorted(DGraph g, i32* flags, u64* cnt, hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);

@@ -1308,27 +1308,25 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, c
 
 // flags[3] |= some row is not strictly ascending (so it may hold duplicates)
 __global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
-  bool bad = false, nonunit = false;
+  bool bad = false;
   const int lane = lane_id();
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
-    for (i64 i = s0 + lane; i < s1; i += WAVE) {
-      bad |= i > s0 && g.adj[i] <= g.adj[i - 1];
-      nonunit |= g.w[i] != 1;
-    }
+    for (i64 i = s0 + 1 + lane; i < s1; i += WAVE) bad |= g.adj[i] <= g.adj[i - 1];
   }
   if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
-  if (__ballot(nonunit) && lane == 0) atomicOr(&flags[4], 1);
 }
 
 // Strictly ascending rows: each arc v->u with u > v is counted (cnt[0]); each
 // arc v->u with u < v binary-searches row u for v and compares weights
 // (counted in cnt[1] when found, flags[0] when missing, flags[2] on a weight
 // mismatch).  Rows hold no repeats, so distinct lower arcs find distinct upper
-// arcs: symmetric iff nothing is missing and both counts agree.
+// arcs: symmetric iff nothing is missing and both counts agree.  flags[4] is
+// set when a found pair carries a weight other than 1 (on a symmetric level
+// the found pairs hold every arc between distinct vertices in range).
 __global__ __launch_bounds__(256) void k_sym_search_sorted(DGraph g, i32* flags, u64* cnt) {
   const int lane = lane_id();
-  bool miss = false, wmis = false;
+  bool miss = false, wmis = false, nonunit = false;
   u64 cu = 0, cd = 0;
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
@@ -1346,7 +1344,9 @@ __global__ __launch_bounds__(256) void k_sym_search_sorted(DGraph g, i32* flags,
         else hi = mid;
       }
       if (lo < g.xadj[u + 1] && g.adj[lo] == v) {
-        wmis |= g.w[lo] != g.w[i];
+        const i32 wl = g.w[lo], wi = g.w[i];
+        wmis |= wl != wi;
+        nonunit |= wl != 1 || wi != 1;
         cd++;
       } else {
         miss = true;
@@ -1359,6 +1359,7 @@ __global__ __launch_bounds__(256) void k_sym_search_sorted(DGraph g, i32* flags,
   if (lane == 0 && cd) atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)cd);
   if (__ballot(miss) && lane == 0) atomicOr(&flags[0], 1);
   if (__ballot(wmis) && lane == 0) atomicOr(&flags[2], 1);
+  if (__ballot(nonunit) && lane == 0) atomicOr(&flags[4], 1);
 }
 
 // ------------------------------------------------------------------ relabel --
