@@ -480,9 +480,10 @@ static i64 env_i64(const char* name, i64 dflt) {
   return (v && *v) ? std::atoll(v) : dflt;
 }
 
-// Lanes per row in the token pass (1, 4 or 8: TGP_TOKEN_LANES) and in the
-// gain pass over kept tallies (2, 4, 8 or 16: TGP_GAIN_LANES).
-static int token_lanes() { return (int)env_i64("TGP_TOKEN_LANES", 1); }
+// Lanes per light row in the token pass (0 = one arc stream per 64 rows,
+// 1, 4 or 8: TGP_TOKEN_LANES) and in the gain pass over kept tallies (2, 4,
+// 8 or 16: TGP_GAIN_LANES).
+static int token_lanes() { return (int)env_i64("TGP_TOKEN_LANES", 0); }
 static int gain_lanes() { return (int)env_i64("TGP_GAIN_LANES", 4); }
 
 void Run::match_level(const Level& lv, DBuf<i32>& mate) {

@@ -370,7 +370,8 @@ void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* h
                         hipStream_t s);
 i64 gain_gscratch_elems(i32 k);
 // cand[v] = gain of v when v is a move candidate, NaN otherwise (the gain
-// kernels write it); lanes (1, 4 or 8) per row of at most heavy_deg arcs
+// kernels write it); rows of at most heavy_deg arcs: lanes = 0 streams the
+// arcs of 64 consecutive rows per wave, 1, 4 or 8 lanes per row otherwise
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
                   int lanes, hipStream_t s);

@@ -2998,11 +2998,82 @@ __device__ __forceinline__ void token_light_groups(const DGraph& g, i64 nedges,
   }
 }
 
+// G = 0: one wave per 64 consecutive vertices, walking the arcs of their
+// light candidate rows as one stream (lane l takes the stream's entry c + l;
+// its row comes from a binary search over the rows' stream offsets held one
+// per lane), so the row loads are coalesced; a row's losing neighbour sets
+// its bit in a wave-wide mask, and later entries of a row already lost skip
+// their gather.
+__device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
+                                                   const float* cand, const i32* part,
+                                                   const i32* target, i32 k, u8* flag,
+                                                   i64 heavy_deg, unsigned bh) {
+  const int lane = lane_id();
+  const i64 ngw = (i64)(gridDim.x - bh) * (blockDim.x / WAVE);
+  for (i64 gw = (i64)(blockIdx.x - bh) * (blockDim.x / WAVE) + (threadIdx.x >> 6);
+       gw * WAVE < g.n; gw += ngw) {
+    const i64 v = gw * WAVE + lane;
+    const bool valid = v < g.n;
+    const i64 s0 = valid ? g.xadj[v] : 0;
+    const i64 s1 = valid ? ((v + 1 < g.n) ? g.xadj[v + 1] : nedges) : 0;
+    const bool light = valid && s1 - s0 <= heavy_deg;
+    const float gv = light ? cand[v] : __int_as_float(0x7fc00000);
+    const bool isc = gv == gv;
+    // the last row ends at nedges, which may lie before its start: no entries
+    const i32 len = (isc && s1 > s0) ? (i32)(s1 - s0) : 0;
+    // exclusive prefix of len over the wave
+    i32 incl = len;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const i32 t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    const i32 start = incl - len;
+    const i32 total = __shfl(incl, WAVE - 1);
+    const float gv10 = gv + 10.0f;
+    u64 lost = 0;  // bit r: row r has a losing neighbour
+    for (i32 c = 0; c < total; c += WAVE) {
+      const i32 idx = c + lane;
+      // the last row whose stream offset is <= idx (rows of len 0 share the
+      // next row's offset and lie before it)
+      int r = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        const int q = r + step;
+        const i32 sq = __shfl(start, q < WAVE ? q : WAVE - 1);
+        if (q < WAVE && sq <= idx) r = q;
+      }
+      const i64 rs0 = __shfl(s0, r);
+      const i32 rst = __shfl(start, r);
+      const float rg = __shfl(gv, r), rg10 = __shfl(gv10, r);
+      const bool act = idx < total && !((lost >> r) & 1);
+      const i32 u = act ? g.adj[rs0 + (idx - rst)] : -1;
+      bool l = false;
+      if (act && u >= 0 && u < g.n) {
+        const float gu = cand[u];
+        l = gu > rg10 || (gu == rg && (i64)u < gw * WAVE + r);
+      }
+      u64 m = __ballot(l);
+      while (m) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        const int rb = __shfl(r, b);
+        lost |= 1ull << rb;
+        m &= ~__ballot(l && r == rb);
+      }
+    }
+    if (light) flag[v] = (isc && !((lost >> lane) & 1) && move_ok(part[v], target[v], k)) ? 1 : 0;
+  }
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float* cand,
                                                const i32* part, const i32* target, i32 k,
                                                u8* flag, i64 heavy_deg, const i32* list,
                                                i64 cnt, unsigned bh) {
+  if (blockIdx.x >= bh && G == 0) {
+    token_light_stream(g, nedges, cand, part, target, k, flag, heavy_deg, bh);
+    return;
+  }
   if (blockIdx.x >= bh && G > 1) {
     token_light_groups<(G > 1 ? G : 2), 4>(g, nedges, cand, part, target, k, flag, heavy_deg,
                                            bh);
@@ -4107,10 +4178,13 @@ void launch_gain_rows(DGraph g, i64 nedges, const i32* part, i32 k, const float*
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
                   int lanes, hipStream_t s) {
-  const int G = (lanes == 4 || lanes == 8) ? lanes : 1;
-  const unsigned bl = grid_for(g.n, 256 / G, 1u << 16);
+  const int G = (lanes == 0 || lanes == 4 || lanes == 8) ? lanes : 1;
+  const unsigned bl = grid_for(g.n, G ? 256 / G : 256, 1u << 16);
   const unsigned bh = heavy_count > 0 ? grid_for(heavy_count, 4, 8192) : 0;
-  if (G == 8)
+  if (G == 0)
+    k_token<0><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
+                                       heavy_list, heavy_count, bh);
+  else if (G == 8)
     k_token<8><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
                                        heavy_list, heavy_count, bh);
   else if (G == 4)
@@ -4156,11 +4230,27 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
   }
   const unsigned tiles = (unsigned)((len + TILE - 1) / TILE);
   constexpr int NT = 1 << (BT_TB - 4);
-  // the first pass reads the moves itself (pads as k_sort_init writes them)
-  k_bt_tile<BT_TB, true><<<tiles, NT, 0, s>>>(key, idx, len, 0, mv_gain, M);
+  // Positions [M, len) hold identical (-1e9, -1) pads, and within a merge
+  // block every comparator acts on that block alone, so a block lying wholly
+  // at or past M compares pads with pads and changes nothing: each merge of
+  // size kk runs only on the blocks below M rounded up to kk (nothing is
+  // skipped under TGP_SORT_ALL=1).  The first pass reads the moves itself and
+  // writes the pads of its tiles; the tiles past them take the pads by a fill.
+  const char* all_env = std::getenv("TGP_SORT_ALL");
+  const bool all = all_env && std::atoi(all_env) != 0;
+  auto upto = [&](i64 kk) { return all ? len : std::min(len, (M + kk - 1) / kk * kk); };
+  const unsigned t0 = (unsigned)((upto(TILE) + TILE - 1) / TILE);
+  k_bt_tile<BT_TB, true><<<t0, NT, 0, s>>>(key, idx, len, 0, mv_gain, M);
+  const i64 pad0 = (i64)t0 * TILE;
+  if (pad0 < len) {
+    const unsigned gf = grid_for(len - pad0, 256, 1u << 16);
+    k_fill<float><<<gf, 256, 0, s>>>(key + pad0, len - pad0, -1e9f);
+    k_fill<i32><<<gf, 256, 0, s>>>(idx + pad0, len - pad0, -1);
+  }
   i64 p2 = 1;
   while (p2 < len) p2 <<= 1;
   for (i64 kk = 2 * TILE; kk <= len; kk <<= 1) {
+    const i64 mk = upto(kk);
     int bhi = 0;  // log2 of the largest stride of this merge
     while (((i64)1 << (bhi + 1)) < kk) bhi++;
     for (int b = bhi; b >= BT_TB;) {
@@ -4168,7 +4258,10 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
       // (one pass instead of two) measured slower: 83 vs 78 ms per partition.
       const int S = std::min(BT_SMAX, b - BT_TB + 1);
       const int blo = b - S + 1;
-      const i64 nsets = p2 >> S;
+      // set t starts at ((t >> blo) << (blo + S)) | (t & (2^blo - 1)): the sets
+      // below mk are the first ones
+      const i64 span = (i64)1 << (blo + S);
+      const i64 nsets = std::min(p2 >> S, ((mk + span - 1) / span) << blo);
       const unsigned gr = grid_for(nsets, 256, 1u << 16);
       if (S == 4) k_bt_global<4><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       else if (S == 3) k_bt_global<3><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
@@ -4176,7 +4269,8 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
       else k_bt_global<1><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       b -= S;
     }
-    k_bt_tile<BT_TB, false><<<tiles, NT, 0, s>>>(key, idx, len, kk, nullptr, 0);
+    const unsigned tk = (unsigned)((mk + TILE - 1) / TILE);
+    k_bt_tile<BT_TB, false><<<tk, NT, 0, s>>>(key, idx, len, kk, nullptr, 0);
   }
 }
 
