@@ -11,14 +11,16 @@ reference's.  The initial partition of the coarsest graph is the
 deterministic block rule (no METIS on the machine).
 
 Prints one JSON line: M edges/s (level-0 arcs x steps / seconds), the roofline
-of the dominant kernel (the chained matching kernel, k_match_chain) from
-HIP-event timing of that kernel alone, the whole step's algorithmic bytes by
-the per-kernel model over the step time, and the oracle's CPU throughput on a
-bounded sample of the same workload family (the same generator, edge factor,
-k and semantics at R-MAT scale 20 on every host thread, and scale 18 on one).  `roofline.traffic` (HBM bytes per
-launch of that kernel) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-committed under profiles/, scaled by the same counters' reading for a plain
-copy of known size (`--copy-probe`).
+of the dominant kernel (of token passing, k_token, and the chained matching,
+k_match_chain, the one with the larger share of the step; the other is
+reported as roofline.next_kernel) from HIP-event timing of that kernel alone,
+the whole step's algorithmic bytes by the per-kernel model over the step time,
+and the oracle's CPU throughput on a bounded sample of the same workload family
+(the same generator, edge factor, k and semantics at R-MAT scale 20 on every
+host thread, and scale 18 on one).  `traffic` (HBM bytes per launch of each
+kernel) comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed
+under profiles/ (<kernel>_rmat24_traffic.json), scaled by the same counters'
+reading for a plain copy of known size (`--copy-probe`).
 """
 import argparse
 import json
@@ -32,15 +34,17 @@ sys.path.insert(0, ROOT)
 
 SEED = 0x7C0DE5EED + 1
 HBM_PEAK_GBS = 8000.0
-ROOFLINE_FAMILY = "match_chain"          # HIP-event bracket around k_match_chain only
-ROOFLINE_KERNEL = "k_match_chain"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_match_chain_rmat24_traffic.json")
+# Single-kernel families the driver brackets with HIP events on its stream
+# (family -> kernel); the roofline line is the one with the larger share of
+# the profiled step, the other is reported beside it.
+ROOFLINE_FAMILIES = {"token": "k_token", "match_chain": "k_match_chain"}
 
 
-def measured_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed counter summary."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/<kernel>_rmat24_traffic.json), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", f"{kernel}_rmat24_traffic.json")) as f:
             t = json.load(f)
         return t["traffic_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
@@ -550,11 +554,21 @@ def main():
     # one extra step with per-kernel HIP-event timing (not part of the timed region)
     rp = tgp.partition(g, args.k, profile=True, trace_cap=1 << 14, **kw)
     stp = rp[3]
-    ms = stp.kernel_ms[ROOFLINE_FAMILY]
-    byts = stp.kernel_bytes[ROOFLINE_FAMILY]
-    launches = stp.kernel_launches[ROOFLINE_FAMILY]
-    achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     total_kernel_ms = sum(stp.kernel_ms.values())
+
+    def kernel_line(fam):
+        ms = stp.kernel_ms[fam]
+        byts = stp.kernel_bytes[fam]
+        launches = stp.kernel_launches[fam]
+        achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        kernel = ROOFLINE_FAMILIES[fam]
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(kernel), "launches": launches,
+                "avg_launch_ms": round(ms / max(launches, 1), 4),
+                "bytes_per_launch": round(byts / max(launches, 1)),
+                "kernel_share_of_step": round(ms / max(total_kernel_ms, 1e-9), 3)}
+    fams = sorted(ROOFLINE_FAMILIES, key=lambda f: -stp.kernel_ms.get(f, 0.0))
     import numpy as np
     sizes = np.bincount(part[part >= 0], minlength=args.k)
     detail = {
@@ -571,19 +585,14 @@ def main():
         "sorts_skipped": stp.sorts_skipped,
         "pcie_inclusive_value": round(pcie_value, 3),
     }
-    traffic = measured_traffic()
     # the whole step against the same peak: the per-kernel model's bytes of
     # every level and pass this partition ran, over the timed step
     e2e_bytes = model_bytes(stp.trace)
     e2e_gbs = e2e_bytes / (elapsed / args.steps) / 1e9
-    roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic, "launches": launches,
-                "avg_launch_ms": round(ms / max(launches, 1), 4),
-                "bytes_per_launch": round(byts / max(launches, 1)),
-                "kernel_share_of_step": round(ms / max(total_kernel_ms, 1e-9), 3),
-                "e2e_model_bytes": round(e2e_bytes), "e2e_achieved": round(e2e_gbs, 1),
-                "e2e_frac": round(e2e_gbs / HBM_PEAK_GBS, 4)}
+    roofline = kernel_line(fams[0])
+    roofline.update({"e2e_model_bytes": round(e2e_bytes), "e2e_achieved": round(e2e_gbs, 1),
+                     "e2e_frac": round(e2e_gbs / HBM_PEAK_GBS, 4)})
+    roofline["next_kernel"] = kernel_line(fams[1])
     dg.free()
 
     cpu = None

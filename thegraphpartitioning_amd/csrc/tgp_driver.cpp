@@ -1190,7 +1190,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           C.prof.begin(KID_TOKEN);
           launch_token(g, nedges, cand.p, part.p, target.p, k, flag.p, heavy, nheavy, HEAVY_DEG,
                        TOKEN_LANES, st());
-          C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n + 8.0 * n);
+          // 9m + (o + 6)n: arcs and the candidates' gains, offsets, own gain and flag
+          C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n);
           C.prof.begin(KID_COMPACT);
           launch_compact_moves(flag.p, n, part.p, target.p, gain.p, k, bcnt.p, boff.p, small.p,
                                mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, outc.p + k, st());
