@@ -1142,6 +1142,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     const double arc_b = unit ? 8.0 : 12.0;
 
     DBuf<u64> cutacc(&C.pool, 1);
+    i64 lastM = n;  // moves buffered by the last pass of this level (token lane choice)
     for (int phase = 0; phase < 2; phase++) {
       const int maxPass = phase ? 40 : 30;               // gpu_part.cu:545, 647
       bool balanced = false;
@@ -1188,8 +1189,13 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           }
           if (pending) launch_acc_finish(accs.p, cutacc.p, st());
           C.prof.begin(KID_TOKEN);
+          // Light rows: the arc stream (lanes 0) pays off when most candidates
+          // win and read their whole rows; with few winners (under 30 % of the
+          // vertices moved in the last pass) one thread per row, which stops at
+          // the first losing neighbour, measured faster.  Both are exact.
+          const int tlanes = (TOKEN_LANES == 0 && 10 * lastM < 3 * n) ? 1 : TOKEN_LANES;
           launch_token(g, nedges, cand.p, part.p, target.p, k, flag.p, heavy, nheavy, HEAVY_DEG,
-                       TOKEN_LANES, st());
+                       tlanes, st());
           // 9m + (o + 6)n: arcs and the candidates' gains, offsets, own gain and flag
           C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n);
           C.prof.begin(KID_COMPACT);
@@ -1302,6 +1308,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         C.prof.end(KID_COMMIT, 17.0 * M);
         pending = true;
         prevM = M;
+        lastM = M;
         prevPass = pass;
       }
       HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));

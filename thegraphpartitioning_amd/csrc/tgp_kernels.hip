@@ -2998,12 +2998,12 @@ __device__ __forceinline__ void token_light_groups(const DGraph& g, i64 nedges,
   }
 }
 
-// G = 0: one wave per 64 consecutive vertices, walking the arcs of their
-// light candidate rows as one stream (lane l takes the stream's entry c + l;
-// its row comes from a binary search over the rows' stream offsets held one
-// per lane), so the row loads are coalesced; a row's losing neighbour sets
-// its bit in a wave-wide mask, and later entries of a row already lost skip
-// their gather.
+// G = 0: one wave per 64 consecutive vertices, whose light candidate rows'
+// arcs form one stream, 64 entries per step (lane l takes the step's entry l,
+// its row found by binary search over the rows' offsets in the step, held one
+// per lane), so the row loads are coalesced.  The stream is rebuilt at every
+// step from the rows still undecided: a row that met a losing neighbour drops
+// out at once and its lanes go to the other rows' next entries.
 __device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
                                                    const float* cand, const i32* part,
                                                    const i32* target, i32 k, u8* flag,
@@ -3019,49 +3019,60 @@ __device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
     const bool light = valid && s1 - s0 <= heavy_deg;
     const float gv = light ? cand[v] : __int_as_float(0x7fc00000);
     const bool isc = gv == gv;
-    // the last row ends at nedges, which may lie before its start: no entries
-    const i32 len = (isc && s1 > s0) ? (i32)(s1 - s0) : 0;
-    // exclusive prefix of len over the wave
-    i32 incl = len;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-      const i32 t = __shfl_up(incl, o);
-      if (lane >= o) incl += t;
-    }
-    const i32 start = incl - len;
-    const i32 total = __shfl(incl, WAVE - 1);
     const float gv10 = gv + 10.0f;
-    u64 lost = 0;  // bit r: row r has a losing neighbour
-    for (i32 c = 0; c < total; c += WAVE) {
-      const i32 idx = c + lane;
-      // the last row whose stream offset is <= idx (rows of len 0 share the
-      // next row's offset and lie before it)
+    // entries of this lane's row not yet read (the last row ends at nedges,
+    // which may lie before its start: none); 0 once the row has lost
+    i32 rem = (isc && s1 > s0) ? (i32)(s1 - s0) : 0;
+    i64 next = s0;  // the row's next entry
+    bool lost = false;
+    while (true) {
+      i32 incl = rem;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const i32 t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+      }
+      if (__shfl(incl, WAVE - 1) == 0) break;
+      const i32 start = incl - rem;  // this row's first slot in the step
+      // the last row whose first slot is <= lane (rows with nothing left share
+      // the next row's slot and lie before it)
       int r = 0;
 #pragma unroll
       for (int step = 32; step > 0; step >>= 1) {
         const int q = r + step;
-        const i32 sq = __shfl(start, q < WAVE ? q : WAVE - 1);
-        if (q < WAVE && sq <= idx) r = q;
+        if (__shfl(start, q) <= lane) r = q;
       }
-      const i64 rs0 = __shfl(s0, r);
-      const i32 rst = __shfl(start, r);
+      const i32 rst = __shfl(start, r), rrem = __shfl(rem, r);
+      const bool act = lane - rst < rrem;
+      const i64 at = __shfl(next, r) + (lane - rst);
       const float rg = __shfl(gv, r), rg10 = __shfl(gv10, r);
-      const bool act = idx < total && !((lost >> r) & 1);
-      const i32 u = act ? g.adj[rs0 + (idx - rst)] : -1;
       bool l = false;
-      if (act && u >= 0 && u < g.n) {
-        const float gu = cand[u];
-        l = gu > rg10 || (gu == rg && (i64)u < gw * WAVE + r);
+      if (act) {
+        const i32 u = g.adj[at];
+        if (u >= 0 && u < g.n) {
+          const float gu = cand[u];
+          l = gu > rg10 || (gu == rg && (i64)u < gw * WAVE + r);
+        }
       }
+      // rows with a losing neighbour in this step
+      u64 lm = 0;
       u64 m = __ballot(l);
       while (m) {
         const int b = __ffsll((unsigned long long)m) - 1;
         const int rb = __shfl(r, b);
-        lost |= 1ull << rb;
+        lm |= 1ull << rb;
         m &= ~__ballot(l && r == rb);
       }
+      // this row's entries read in the step: min(rem, 64 - start) when start < 64
+      const i32 took = start < WAVE ? (rem < WAVE - start ? rem : WAVE - start) : 0;
+      next += took;
+      rem -= took;
+      if ((lm >> lane) & 1) {
+        lost = true;
+        rem = 0;
+      }
     }
-    if (light) flag[v] = (isc && !((lost >> lane) & 1) && move_ok(part[v], target[v], k)) ? 1 : 0;
+    if (light) flag[v] = (isc && !lost && move_ok(part[v], target[v], k)) ? 1 : 0;
   }
 }
 
