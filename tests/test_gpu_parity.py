@@ -264,7 +264,8 @@ def test_nearly_symmetric_sorted_rows(oracle, kind, repair):
     assert_same(gp, op, f"nearly symmetric ({kind}) repair={repair}")
 
 
-@pytest.mark.parametrize("token,gain", [("0", "4"), ("1", "16"), ("4", "8"), ("8", "2")])
+@pytest.mark.parametrize("token,gain", [("0", "1"), ("0", "4"), ("1", "16"), ("4", "8"),
+                                        ("8", "2")])
 def test_token_and_gain_lane_widths(oracle, monkeypatch, token, gain):
     """Every lanes-per-row setting of token passing (TGP_TOKEN_LANES; 0, the
     default, streams the arcs of 64 rows per wave) and of the gain pass over

@@ -481,10 +481,10 @@ static i64 env_i64(const char* name, i64 dflt) {
 }
 
 // Lanes per light row in the token pass (0 = one arc stream per 64 rows,
-// 1, 4 or 8: TGP_TOKEN_LANES) and in the gain pass over kept tallies (2, 4,
-// 8 or 16: TGP_GAIN_LANES).
+// 1, 4 or 8: TGP_TOKEN_LANES) and per vertex in the gain pass over kept
+// tallies (1 when k <= 64, else 4 by default; 2, 4, 8 or 16: TGP_GAIN_LANES).
 static int token_lanes() { return (int)env_i64("TGP_TOKEN_LANES", 0); }
-static int gain_lanes() { return (int)env_i64("TGP_GAIN_LANES", 4); }
+static int gain_lanes() { return (int)env_i64("TGP_GAIN_LANES", 1); }
 
 void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   const i64 n = lv.n;
@@ -1108,16 +1108,47 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     }
     // A few rows whose sum of |w| passes 2^24 keep the kept-tally form: their
     // gains are gathered again in fp32 row order after every tally pass.
-    const bool tmode = k <= 512 && (ax.all_exact || ax.nover <= OVER_ROWS_MAX) &&
-                       lv.simple_ws &&
-                       (double)n * tally_row_stride(k) * 4 <= 0.4 * (double)free_b &&
-                       env_i64("TGP_KEEP_TALLIES", 1) != 0;
-    DBuf<i32> T, app, napp;
+    // Rows whose weights sum to at most 255 keep bytes, the others ints
+    // (TallyRef); a level where over half the rows need ints keeps ints for
+    // every row (as does TGP_TALLY_BYTES=0).
+    bool tmode = k <= 512 && (ax.all_exact || ax.nover <= OVER_ROWS_MAX) && lv.simple_ws &&
+                 env_i64("TGP_KEEP_TALLIES", 1) != 0;
+    DBuf<i32> T32, bix, app, napp;
+    DBuf<u32> bigmask;
+    DBuf<u8> T8;
+    TallyRef TR;
+    i64 nbig = n;
+    bool all_int = true;
+    // byte rows pay off with one lane per vertex (k <= 64, gain lanes 1); the
+    // kernels for other widths read int rows only
+    if (tmode && env_i64("TGP_TALLY_BYTES", 1) != 0 && k <= 64 && GAIN_LANES == 1) {
+      bix = DBuf<i32>(&C.pool, n);
+      bigmask = DBuf<u32>(&C.pool, (n + 31) / 32 + 2);
+      DBuf<i32> nb(&C.pool, 1);
+      HIP_TRY(hipMemsetAsync(nb.p, 0, sizeof(i32), st()));
+      launch_tally_class(g, nedges, unit, 0, bix.p, bigmask.p, nb.p, st());
+      HIP_TRY(hipMemcpyAsync(C.pinned, nb.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+      C.sync();
+      nbig = ((const i32*)C.pinned)[0];
+      all_int = 2 * nbig > n;
+      if (all_int) {
+        nbig = n;
+        bix = DBuf<i32>();
+        bigmask = DBuf<u32>();
+      }
+    }
+    // bytes one pass reads from the kept tallies (and the row classes)
+    const double tally_bytes =
+        all_int ? (double)n * tally_row_stride(k) * 4
+                : (double)n * tally8_stride(k) + (double)nbig * tally_row_stride(k) * 4 + n / 8.0;
+    if (tmode) tmode = tally_bytes <= 0.4 * (double)free_b;
     bool T_valid = false;
     const bool last_full = nedges == lv.m;
     const bool last_partial = !last_full && nedges > ax.last_begin;
     if (tmode) {
-      T = DBuf<i32>(&C.pool, n * tally_row_stride(k));
+      if (!all_int) T8 = DBuf<u8>(&C.pool, n * tally8_stride(k));
+      T32 = DBuf<i32>(&C.pool, std::max<i64>(nbig, 1) * tally_row_stride(k));
+      TR = TallyRef{T8.p, T32.p, bix.p, bigmask.p, all_int ? 1 : 0};
       app = DBuf<i32>(&C.pool, n);
       napp = DBuf<i32>(&C.pool, 1);
     }
@@ -1168,17 +1199,17 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           C.prof.begin(KID_GAIN);
           bool fused;
           if (tmode && T_valid) {
-            if (last_partial) launch_tally_last(g, nedges, part.p, T.p, k, unit, st());
-            launch_gain_tallies(n, part.p, T.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
+            if (last_partial) launch_tally_last(g, nedges, part.p, TR, k, unit, st());
+            launch_gain_tallies(n, part.p, TR, k, bg.p, src_big.p, gain.p, target.p, cand.p,
                                 phase, pending ? accs.p : nullptr, GAIN_LANES, st());
             launch_gain_rows(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
                              phase, unit, ax.over.p + 1, ax.nover, st());
             fused = true;
-            C.prof.end(KID_GAIN, 4.0 * k * n + 17.0 * n);
+            C.prof.end(KID_GAIN, tally_bytes + 17.0 * n);
           } else {
             fused = launch_gain(g, nedges, part.p, part8.p, k, bg.p, src_big.p, gain.p, target.p,
                                 cand.p, phase, unit, gscr.p, ax.lists(),
-                                pending ? accs.p : nullptr, tmode ? T.p : nullptr, st());
+                                pending ? accs.p : nullptr, tmode ? &TR : nullptr, st());
             T_valid = tmode;
             C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
           }
@@ -1214,10 +1245,10 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           if (tmode && T_valid) {
             // the tallies are current after the last commit (the last row's
             // partial range is rebuilt as the gain pass does)
-            if (last_partial) launch_tally_last(g, nedges, part.p, T.p, k, unit, st());
-            launch_tally_cut(n, part.p, T.p, k, accs.p, st());
+            if (last_partial) launch_tally_last(g, nedges, part.p, TR, k, unit, st());
+            launch_tally_cut(n, part.p, TR, k, accs.p, st());
             launch_acc_finish(accs.p, cutacc.p, st());
-            C.prof.end(KID_CUT, 4.0 * tally_row_stride(k) * n + 4.0 * n);
+            C.prof.end(KID_CUT, tally_bytes + 4.0 * n);
           } else {
             launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
             launch_acc_finish(accs.p, cutacc.p, st());
@@ -1303,7 +1334,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
         launch_commit(sk_idx.p, sel.p, M, mv_v.p, mv_src.p, mv_dst.p, part.p, part8.p, pc.p, k,
                       tmode ? app.p : nullptr, napp.p, st());
         if (tmode && T_valid)
-          launch_tally_update(g, T.p, k, app.p, napp.p, M, mv_v.p, mv_src.p, mv_dst.p, unit,
+          launch_tally_update(g, TR, k, app.p, napp.p, M, mv_v.p, mv_src.p, mv_dst.p, unit,
                               last_full ? 0 : 1, st());
         C.prof.end(KID_COMMIT, 17.0 * M);
         pending = true;

@@ -335,13 +335,36 @@ void launch_row_abs(DGraph g, i64 nedges, i64 heavy_deg, const i32* heavy, i64 n
 // part8 (optional, k <= PART8_MAXK): byte copy of part that the neighbour
 // gathers read instead of part.
 constexpr i32 PART8_MAXK = 255;
-// Tout (optional, n * k ints): also writes each vertex's integer tally.
+// Kept tallies (T[v][q] = weight of v's arcs into part q).  A row whose arcs
+// all weigh >= 0 and sum to at most 255 (and that is not the last row) keeps
+// its tallies as bytes: t8 + v * tally8_stride(k) (k rounded up to 16), marked
+// bix[v] < 0; every other row keeps ints at t32 + bix[v] * tally_row_stride(k)
+// (k rounded up to 4).  Rows are 16-byte aligned.  A byte tally never leaves
+// [0, 255], so byte updates are exact 32-bit atomic adds on the word holding
+// the byte (no carry or borrow crosses a byte).
+// With all_int set every row is an int row at t32 + v * tally_row_stride(k)
+// (no bix); a level where most rows need ints takes that layout.  bigmask has
+// bit v set for the int rows of a mixed level (one bit per vertex, so the
+// class test of a row is an L2 hit while its byte row is already in flight).
+struct TallyRef {
+  u8* t8 = nullptr;
+  i32* t32 = nullptr;
+  const i32* bix = nullptr;
+  const u32* bigmask = nullptr;
+  int all_int = 0;
+};
+inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
+inline i64 tally8_stride(i32 k) { return (i64)((k + 15) & ~15); }
+// bix[v] = -1 for a byte row, else its int row (wave-aggregated from *nbig,
+// zeroed by the caller), bigmask[(n + 31) / 32] the int rows; all_int puts
+// every row in the int table.
+void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i32* bix, u32* bigmask,
+                        i32* nbig, hipStream_t s);
+// Tout (optional): also writes each vertex's integer tally.
 bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
-                 i32* gscratch, const GainLists& L, u64* cut_acc, i32* Tout, hipStream_t s);
-// Kept tallies: row v of T holds k ints at v * tally_stride(k) (k rounded up
-// to 4, so rows are 16-byte aligned); Tout of launch_gain uses the same layout.
-inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
+                 i32* gscratch, const GainLists& L, u64* cut_acc, const TallyRef* Tout,
+                 hipStream_t s);
 // Gains of the listed rows only, gathered from their arcs (k <= 512): rows
 // whose sum of |w| passes 2^24 are tallied in fp32 in row order.
 void launch_gain_rows(DGraph g, i64 nedges, const i32* part, i32 k, const float* bg,
@@ -349,19 +372,20 @@ void launch_gain_rows(DGraph g, i64 nedges, const i32* part, i32 k, const float*
                       int unit, const i32* list, i64 cnt, hipStream_t s);
 // Gains (and the cut into cut_acc slots) from kept tallies T; lanes (2, 4, 8 or
 // 16) read each row.
-void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
+void launch_gain_tallies(i64 n, const i32* part, const TallyRef& T, i32 k, const float* bg,
                          const u8* src_big, float* gain, i32* target, float* cand, int phase,
                          u64* cut_acc, int lanes, hipStream_t s);
 // T update for the applied moves app[0..*napp) (at most max_app) of a commit;
 // skip_last leaves T[n-1] alone.
-void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
-                         const i32* mv_v, const i32* mv_src, const i32* mv_dst, int unit,
-                         int skip_last, hipStream_t s);
+void launch_tally_update(DGraph g, const TallyRef& T, i32 k, const i32* app, const i32* napp,
+                         i64 max_app, const i32* mv_v, const i32* mv_src, const i32* mv_dst,
+                         int unit, int skip_last, hipStream_t s);
 // T[n-1] from its range [xadj[n-1], nedges)
 // Edge cut of part from kept tallies (sum over v of T[v][q] for q != part[v]),
 // added into the accumulator slots.
-void launch_tally_cut(i64 n, const i32* part, const i32* T, i32 k, u64* cut_acc, hipStream_t s);
-void launch_tally_last(DGraph g, i64 nedges, const i32* part, i32* T, i32 k, int unit,
+void launch_tally_cut(i64 n, const i32* part, const TallyRef& T, i32 k, u64* cut_acc,
+                      hipStream_t s);
+void launch_tally_last(DGraph g, i64 nedges, const i32* part, const TallyRef& T, i32 k, int unit,
                        hipStream_t s);
 // part8[v] = part[v] if 0 <= part[v] < k, else 255
 void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s);

@@ -26,6 +26,29 @@ constexpr int WAVE = 64;
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 // row stride of a kept tally array: k rounded up to 4 ints (16-byte rows)
 __host__ __device__ __forceinline__ i64 tally_stride(i32 k) { return (i64)((k + 3) & ~3); }
+// byte rows of the kept tallies: k rounded up to 16
+__host__ __device__ __forceinline__ i64 tally8_stride_d(i32 k) { return (i64)((k + 15) & ~15); }
+// int row of v, or -1 for a byte row
+__device__ __forceinline__ i64 tally_int_row(const TallyRef& T, i64 v) {
+  return T.all_int ? v : (i64)T.bix[v];
+}
+__device__ __forceinline__ void tally_put(const TallyRef& T, i64 v, i32 k, i32 q, i32 val) {
+  const i64 b = tally_int_row(T, v);
+  if (b >= 0) T.t32[(i64)b * tally_stride(k) + q] = val;
+  else T.t8[v * tally8_stride_d(k) + q] = (u8)val;
+}
+// T[u][q] += d; a byte row's add goes to the 32-bit word holding the byte:
+// the byte stays in [0, 255], so nothing carries into or borrows from its
+// neighbours ((u32)d << sh is d * 2^sh modulo 2^32 for either sign)
+__device__ __forceinline__ void tally_add(const TallyRef& T, i64 u, i32 k, i32 q, i32 d) {
+  const i64 b = tally_int_row(T, u);
+  if (b >= 0) {
+    atomicAdd(&T.t32[(i64)b * tally_stride(k) + q], d);
+  } else {
+    const i64 byte = u * tally8_stride_d(k) + q;
+    atomicAdd((u32*)(T.t8 + (byte & ~(i64)3)), (u32)d << (8 * (u32)(byte & 3)));
+  }
+}
 __device__ __forceinline__ u64 lanes_below(int lane) { return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane)); }
 
 template <typename T>
@@ -2380,7 +2403,8 @@ __global__ __launch_bounds__(256) void k_gain_huge_final(DGraph g, i64 nedges, c
                                                          const u8* src_big, float* gain,
                                                          i32* target, float* cand, int phase,
                                                          const i32* huge, i64 nhuge,
-                                                         i32* gtally, i64* gabs, i32* Tout) {
+                                                         i32* gtally, i64* gabs,
+                                                         TallyRef Tout) {
   extern __shared__ i32 lds[];
   __shared__ float s_g[256 / WAVE];
   __shared__ i32 s_q[256 / WAVE];
@@ -2465,7 +2489,7 @@ __global__ __launch_bounds__(256) void k_gain_huge_final(DGraph g, i64 nedges, c
     }
     __syncthreads();
     for (int q = threadIdx.x; q < k; q += blockDim.x) {
-      if (Tout) Tout[v * tally_stride(k) + q] = gt[q];
+      if (Tout.t32) tally_put(Tout, v, k, q, gt[q]);
       gt[q] = 0;
     }
     if (threadIdx.x == 0) gabs[h] = 0;
@@ -2499,7 +2523,8 @@ template <int G, bool UNIT, typename PT>
 __device__ __forceinline__ void gain_vertex(const DGraph& g, i64 nedges, const PT* part, i32 k,
                                             const float* sbg, const u8* src_big, float* gain,
                                             i32* target, float* cand, int phase, int all_exact,
-                                            i64 v, i32* tally, u64& cutv, int gl, i32* Tout) {
+                                            i64 v, i32* tally, u64& cutv, int gl,
+                                            const TallyRef& Tout) {
   float* ftally = (float*)tally;
   const i64 s0 = g.xadj[v];
   const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
@@ -2576,7 +2601,7 @@ __device__ __forceinline__ void gain_vertex(const DGraph& g, i64 nedges, const P
     const i32 tq = tally[q];
     const float cq = exact_int ? (float)tq : ftally[q];
     tally[q] = 0;
-    if (Tout) Tout[v * tally_stride(k) + q] = q == cur ? own : tq;  // integers: exact
+    if (Tout.t32) tally_put(Tout, v, k, q, q == cur ? own : tq);  // integers: exact
     if (q == cur) continue;
     const float eg = cq - ccur;
     const float gg = eg + bgr[q];
@@ -2670,7 +2695,7 @@ __global__ __launch_bounds__(256) void k_gain_fused(DGraph g, i64 nedges, const 
                                                     const float* bg_g, const u8* src_big,
                                                     float* gain, i32* target, float* cand,
                                                     int phase, GainLists L, u32* ctr,
-                                                    u64* cut_acc, i32* Tout) {
+                                                    u64* cut_acc, TallyRef Tout) {
   extern __shared__ i32 lds[];
   float* sbg = (float*)lds;
   for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbg[q] = bg_g[q];
@@ -2733,8 +2758,10 @@ __global__ __launch_bounds__(256) void k_gain_fused(DGraph g, i64 nedges, const 
 
 // The pick of gain_vertex from T, 16 lanes per vertex; also the vertex's
 // share of the edge cut (its arcs into other parts).
+// Levels whose tallies are all int rows (row v at v * tally_stride(k)): G lanes
+// per vertex, R 16-byte loads per lane in flight.
 template <int G, int R>
-__global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, const i32* T,
+__global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, TallyRef T,
                                                       i32 k, const float* bg_g,
                                                       const u8* src_big, float* gain,
                                                       i32* target, float* cand, int phase,
@@ -2750,7 +2777,7 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, co
   for (i64 v = (i64)blockIdx.x * (blockDim.x / G) + threadIdx.x / G; v < n; v += ngroups) {
     const i32 cur = part[v];
     if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
-    const i32* Tv = T + v * ts;
+    const i32* Tv = T.t32 + v * ts;
     const int4* T4 = (const int4*)Tv;  // rows are 16-byte aligned (stride multiple of 4)
     const float ccur = (float)Tv[cur];
     const float* bgr = sbgt + (src_big[cur] ? k : 0);
@@ -2814,29 +2841,131 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, co
   if (cut_acc) block_add_u64(cutv, cut_acc);
 }
 
+// k <= 64 on a level with byte rows: one lane per vertex.  A byte row is four
+// 16-byte loads, issued together with the other 63 lanes' rows; an int row
+// (the level's few heavy rows) is read in four chunks of 16 tallies.  The pick runs over q in lockstep across the
+// wave, so each balance term is one LDS broadcast; outputs are coalesced.
+constexpr int GT1_MAXK = 64;
+__device__ __forceinline__ void gt1_pick(i32 q, i32 tq, i32 cur, i32 k, float ccur,
+                                         const float* bgr, float& bgv, i32& bq, i64& other) {
+  if (q >= k || q == cur) return;
+  other += tq;
+  const float eg = (float)tq - ccur;
+  const float gg = eg + bgr[q];
+  if (gg > bgv || (gg == bgv && q < bq)) {
+    bgv = gg;
+    bq = q;
+  }
+}
+__global__ __launch_bounds__(256) void k_gain_tallies1(i64 n, const i32* part, TallyRef T,
+                                                       i32 k, const float* bg_g,
+                                                       const u8* src_big, float* gain,
+                                                       i32* target, float* cand, int phase,
+                                                       u64* cut_acc) {
+  extern __shared__ float sbgt[];
+  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbgt[q] = bg_g[q];
+  __syncthreads();
+  const i64 ts = tally_stride(k), ts8 = tally8_stride_d(k);
+  u64 cutv = 0;
+  for (i64 v0 = (i64)blockIdx.x * blockDim.x; v0 < n; v0 += (i64)gridDim.x * blockDim.x) {
+    const i64 v = v0 + threadIdx.x;
+    if (v >= n) continue;
+    const i32 cur = part[v];
+    if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
+    const bool big = (T.bigmask[v >> 5] >> (v & 31)) & 1u;
+    const float* bgr = sbgt + (src_big[cur] ? k : 0);
+    float bgv = -INFINITY;
+    i32 bq = INT_MAX;
+    i64 other = 0;
+    if (!big) {
+      const int4* R16 = (const int4*)(T.t8 + v * ts8);
+      int4 c4[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) c4[c] = 16 * c < k ? R16[c] : make_int4(0, 0, 0, 0);
+      const float ccur = (float)T.t8[v * ts8 + cur];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const u32 w4[4] = {(u32)c4[c].x, (u32)c4[c].y, (u32)c4[c].z, (u32)c4[c].w};
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+          gt1_pick(16 * c + j, (i32)((w4[j >> 2] >> (8 * (j & 3))) & 255u), cur, k, ccur, bgr,
+                   bgv, bq, other);
+      }
+    } else {
+      const i32* Tv = T.t32 + (i64)T.bix[v] * ts;
+      const int4* R4 = (const int4*)Tv;
+      const float ccur = (float)Tv[cur];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        int4 c4[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) c4[r] = 16 * c + 4 * r < k ? R4[4 * c + r] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const i32 t4[4] = {c4[r].x, c4[r].y, c4[r].z, c4[r].w};
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            gt1_pick(16 * c + 4 * r + j, t4[j], cur, k, ccur, bgr, bgv, bq, other);
+        }
+      }
+    }
+    cutv += (u64)other;
+    float best;
+    i32 tgt;
+    if (bq == INT_MAX || bgv < -1e9f) {
+      best = -1e9f;
+      tgt = cur;
+    } else if (bgv == -1e9f) {
+      best = -1e9f;
+      tgt = bq < cur ? bq : cur;
+    } else {
+      best = bgv;
+      tgt = bq;
+    }
+    gain[v] = best;
+    target[v] = tgt;
+    cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
+  }
+  if (cut_acc) block_add_u64(cutv, cut_acc);
+}
+
 // Edge cut from kept tallies (gpu_part.cu:400-422 on a level whose tallies
 // T[v][q] are the weights of v's arcs, over its row range, into part q):
 // sum over v with part[v] in [0, k) of sum_{q != part[v]} T[v][q], into the
 // accumulator slots.  Four lanes per vertex, 16-byte loads.
-__global__ __launch_bounds__(256) void k_tally_cut(i64 n, const i32* part, const i32* T, i32 k,
+__global__ __launch_bounds__(256) void k_tally_cut(i64 n, const i32* part, TallyRef T, i32 k,
                                                    u64* cut_acc) {
   const int gl = threadIdx.x & 3;
   const i64 ngroups = (i64)gridDim.x * (blockDim.x / 4);
-  const i64 ts = tally_stride(k);
-  const int nv4 = (int)(ts >> 2);
+  const i64 ts = tally_stride(k), ts8 = tally8_stride_d(k);
+  const int nv4 = (int)(ts >> 2), nv16 = (int)(ts8 >> 4);
   u64 cutv = 0;
   for (i64 v = (i64)blockIdx.x * (blockDim.x / 4) + threadIdx.x / 4; v < n; v += ngroups) {
     const i32 cur = part[v];
     if (cur >= k || cur < 0) continue;
-    const int4* T4 = (const int4*)(T + v * ts);
+    const i64 b = tally_int_row(T, v);
     i64 other = 0;
-    for (int c = gl; c < nv4; c += 4) {
-      const int4 t = T4[c];
-      const i32 tv[4] = {t.x, t.y, t.z, t.w};
+    if (b >= 0) {
+      const int4* T4 = (const int4*)(T.t32 + (i64)b * ts);
+      for (int c = gl; c < nv4; c += 4) {
+        const int4 t = T4[c];
+        const i32 tv[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const i32 q = 4 * c + j;
-        if (q < k && q != cur) other += tv[j];
+        for (int j = 0; j < 4; j++) {
+          const i32 q = 4 * c + j;
+          if (q < k && q != cur) other += tv[j];
+        }
+      }
+    } else {
+      const int4* T16 = (const int4*)(T.t8 + v * ts8);
+      for (int c = gl; c < nv16; c += 4) {
+        const int4 t = T16[c];
+        const u32 w4[4] = {(u32)t.x, (u32)t.y, (u32)t.z, (u32)t.w};
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const i32 q = 16 * c + j;
+          if (q < k && q != cur) other += (w4[j >> 2] >> (8 * (j & 3))) & 255u;
+        }
       }
     }
     cutv += (u64)other;
@@ -2848,13 +2977,12 @@ __global__ __launch_bounds__(256) void k_tally_cut(i64 n, const i32* part, const
 // of v's arcs (v, u) from T[u][src] to T[u][dst]; the reverse arc (u, v) has
 // the same weight on these levels.  T[n-1] is skipped when the last row's
 // range is not the whole row (it is rebuilt by k_tally_last then).
-__global__ __launch_bounds__(256) void k_tally_update(DGraph g, i32* T, i32 k, const i32* app,
+__global__ __launch_bounds__(256) void k_tally_update(DGraph g, TallyRef T, i32 k, const i32* app,
                                                       const i32* napp, const i32* mv_v,
                                                       const i32* mv_src, const i32* mv_dst,
                                                       int unit, int skip_last) {
   const int lane = lane_id();
   const i64 cnt = *napp;
-  const i64 ts = tally_stride(k);
   for (i64 e = gwave(); e < cnt; e += nwaves()) {
     const i32 id = app[e];
     const i32 v = mv_v[id], src = mv_src[id], dst = mv_dst[id];
@@ -2863,17 +2991,55 @@ __global__ __launch_bounds__(256) void k_tally_update(DGraph g, i32* T, i32 k, c
       const i32 u = g.adj[i];
       if (u < 0 || u >= g.n || (skip_last && u == g.n - 1)) continue;
       const i32 w = unit ? 1 : g.w[i];
-      atomicAdd(&T[(i64)u * ts + src], -w);
-      atomicAdd(&T[(i64)u * ts + dst], w);
+      tally_add(T, u, k, src, -w);
+      tally_add(T, u, k, dst, w);
     }
   }
 }
 
-// T[n-1] from its range [xadj[n-1], nedges) (one block)
+// Rows kept as bytes: every arc weighs >= 0 and the row (the last row ending
+// at nedges) sums to at most 255; the last row always keeps ints (its range
+// may change with nedges).
+__global__ __launch_bounds__(256) void k_tally_class(DGraph g, i64 nedges, int unit, int all_int,
+                                                     i32* bix, u32* bigmask, i32* nbig) {
+  for (i64 v0 = (i64)blockIdx.x * blockDim.x; v0 < g.n; v0 += (i64)gridDim.x * blockDim.x) {
+    const i64 v = v0 + threadIdx.x;
+    bool big = true;
+    if (v < g.n && !all_int && v != g.n - 1) {
+      const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+      if (unit) {
+        big = s1 - s0 > 255;
+      } else {
+        big = false;
+        i64 sum = 0;
+        for (i64 i = s0; i < s1; i++) {
+          const i32 w = g.w[i];
+          sum += w;
+          if (w < 0 || sum > 255) {
+            big = true;
+            break;
+          }
+        }
+      }
+    }
+    const bool want = v < g.n && big;
+    const u64 m = __ballot(want);
+    i32 base = 0;
+    const int lane = lane_id();
+    const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+    if (m && lane == leader) base = atomicAdd(nbig, (i32)__popcll(m));
+    base = __shfl(base, leader);
+    if (v < g.n) bix[v] = want ? base + (i32)__popcll(m & lanes_below(lane)) : -1;
+    // 64 consecutive vertices per wave (v0 and blockDim are multiples of 64)
+    if (lane < 2 && v - lane + 32 * lane < g.n) bigmask[(v - lane) / 32 + lane] = (u32)(m >> (32 * lane));
+  }
+}
+
+// T[n-1] from its range [xadj[n-1], nedges) (one block; the last row keeps ints)
 __global__ __launch_bounds__(256) void k_tally_last(DGraph g, i64 nedges, const i32* part,
-                                                    i32* T, i32 k, int unit) {
+                                                    TallyRef T, i32 k, int unit) {
   const i64 v = g.n - 1;
-  i32* Tv = T + v * tally_stride(k);
+  i32* Tv = T.t32 + tally_int_row(T, v) * tally_stride(k);
   for (int q = threadIdx.x; q < k; q += blockDim.x) Tv[q] = 0;
   __syncthreads();
   for (i64 i = g.xadj[v] + threadIdx.x; i < nedges; i += blockDim.x) {
@@ -4109,7 +4275,7 @@ template <typename PT>
 static void launch_gain_small_k(DGraph g, i64 nedges, const PT* part, const i32* part32, i32 k,
                                 const float* bg, const u8* src_big, float* gain, i32* target,
                                 float* cand, int phase, int unit, const GainLists& L,
-                                u32* ctr, u64* cut_acc, i32* Tout, hipStream_t s) {
+                                u32* ctr, u64* cut_acc, TallyRef Tout, hipStream_t s) {
   const size_t shm = (size_t)(2 * k + 4 * 8 * (k | 1)) * 4;
   k_fill<u32><<<1, 256, 0, s>>>(ctr, GAIN_CTRS * 32, 0u);
   if (unit)
@@ -4136,14 +4302,16 @@ static void launch_gain_small_k(DGraph g, i64 nedges, const PT* part, const i32*
 
 bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,
-                 i32* gscratch, const GainLists& L, u64* cut_acc, i32* Tout, hipStream_t s) {
+                 i32* gscratch, const GainLists& L, u64* cut_acc, const TallyRef* Tout,
+                 hipStream_t s) {
+  const TallyRef tout = Tout ? *Tout : TallyRef{};
   if (k <= 512) {
     if (part8 && k <= PART8_MAXK)
       launch_gain_small_k<u8>(g, nedges, part8, part, k, bg, src_big, gain, target, cand, phase,
-                              unit, L, L.ctr, cut_acc, Tout, s);
+                              unit, L, L.ctr, cut_acc, tout, s);
     else
       launch_gain_small_k<i32>(g, nedges, part, part, k, bg, src_big, gain, target, cand, phase,
-                               unit, L, L.ctr, cut_acc, Tout, s);
+                               unit, L, L.ctr, cut_acc, tout, s);
     return cut_acc != nullptr;
   }
   // large k: one wave per vertex for every vertex
@@ -4308,10 +4476,18 @@ void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, con
                                                                napp);
 }
 
-void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const float* bg,
+void launch_gain_tallies(i64 n, const i32* part, const TallyRef& T, i32 k, const float* bg,
                          const u8* src_big, float* gain, i32* target, float* cand, int phase,
                          u64* cut_acc, int lanes, hipStream_t s) {
   const size_t lds = (size_t)2 * k * 4;
+  // one lane per vertex on a level with byte rows (k <= 64); int rows are read
+  // by 4 lanes per vertex unless the caller asks for another width
+  if (k <= GT1_MAXK && lanes == 1 && !T.all_int) {
+    k_gain_tallies1<<<grid_for(n, 256, 8192), 256, lds, s>>>(
+        n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+    return;
+  }
+  if (lanes == 1) lanes = 4;
   if (lanes == 2)
     k_gain_tallies<2, 8><<<grid_for(n, 128, 4096), 256, lds, s>>>(
         n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
@@ -4325,18 +4501,24 @@ void launch_gain_tallies(i64 n, const i32* part, const i32* T, i32 k, const floa
     k_gain_tallies<16, 1><<<grid_for(n, 16, 4096), 256, lds, s>>>(
         n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
 }
-void launch_tally_update(DGraph g, i32* T, i32 k, const i32* app, const i32* napp, i64 max_app,
-                         const i32* mv_v, const i32* mv_src, const i32* mv_dst, int unit,
-                         int skip_last, hipStream_t s) {
+void launch_tally_update(DGraph g, const TallyRef& T, i32 k, const i32* app, const i32* napp,
+                         i64 max_app, const i32* mv_v, const i32* mv_src, const i32* mv_dst,
+                         int unit, int skip_last, hipStream_t s) {
   k_tally_update<<<grid_for(max_app, 4, 4096), 256, 0, s>>>(g, T, k, app, napp, mv_v, mv_src,
                                                             mv_dst, unit, skip_last);
 }
-void launch_tally_cut(i64 n, const i32* part, const i32* T, i32 k, u64* cut_acc, hipStream_t s) {
+void launch_tally_cut(i64 n, const i32* part, const TallyRef& T, i32 k, u64* cut_acc,
+                      hipStream_t s) {
   k_tally_cut<<<grid_for(n, 64, 4096), 256, 0, s>>>(n, part, T, k, cut_acc);
 }
-void launch_tally_last(DGraph g, i64 nedges, const i32* part, i32* T, i32 k, int unit,
+void launch_tally_last(DGraph g, i64 nedges, const i32* part, const TallyRef& T, i32 k, int unit,
                        hipStream_t s) {
   k_tally_last<<<1, 256, 0, s>>>(g, nedges, part, T, k, unit);
+}
+void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i32* bix, u32* bigmask,
+                        i32* nbig, hipStream_t s) {
+  k_tally_class<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, unit, all_int, bix,
+                                                              bigmask, nbig);
 }
 
 void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s) {
