@@ -3838,6 +3838,62 @@ __global__ __launch_bounds__(SEL_BLK) void k_sel_mark(const i32* sdst, i64 M, i3
   }
 }
 
+// Ranks within the block in position order: every wave ranks its own moves
+// per destination (one ballot round per distinct destination in the wave) and
+// records its per-destination counts; a scan over the block's waves per
+// destination then gives each wave's first rank (k <= SELW_MAXK).
+constexpr int SELW_MAXK = 256;
+__global__ __launch_bounds__(SEL_BLK) void k_sel_mark_w(const i32* sdst, i64 M, i32 k,
+                                                      const i32* H, const i64* avail, u8* sel,
+                                                      i32* S, i32* pads) {
+  constexpr int NW = SEL_BLK / WAVE;
+  __shared__ i32 cntd[SELW_MAXK];
+  __shared__ i32 wcnt[NW][SELW_MAXK];  // per wave and destination; then the wave's first rank
+  __shared__ i32 ssel[NW];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (int q = threadIdx.x; q < k; q += SEL_BLK) cntd[q] = H[(i64)q * gridDim.x + blockIdx.x];
+  for (int q = lane; q < k; q += WAVE) wcnt[wv][q] = 0;
+  const i64 i = (i64)blockIdx.x * SEL_BLK + threadIdx.x;
+  const i32 d = i < M ? sdst[i] : -1;
+  const bool valid = d >= 0 && d < k;
+  i32 inwave = 0;  // rank among this wave's earlier moves to d
+  __builtin_amdgcn_wave_barrier();
+  u64 active = __ballot(valid);
+  while (active) {
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const i32 dl = __shfl(d, leader);
+    const u64 same = __ballot(valid && d == dl);
+    if (valid && d == dl) inwave = __popcll(same & lanes_below(lane));
+    if (lane == leader) wcnt[wv][dl] = __popcll(same);
+    active &= ~same;
+  }
+  __syncthreads();
+  // exclusive scan over the waves, per destination, from the block's base
+  for (int q = threadIdx.x; q < k; q += SEL_BLK) {
+    i32 run = cntd[q];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      const i32 c = wcnt[w][q];
+      wcnt[w][q] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  bool mine = false;
+  if (valid) mine = (i64)wcnt[wv][d] + inwave < avail[d];
+  if (i < M) sel[i] = mine ? 1 : 0;
+  const u64 pm = __ballot(i < M && !valid);
+  if (lane == 0 && pm) atomicAdd(pads, (i32)__popcll(pm));
+  const u64 bm = __ballot(mine);
+  if (lane == 0) ssel[wv] = __popcll(bm);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    i32 s = 0;
+    for (int w = 0; w < NW; w++) s += ssel[w];
+    if (s) atomicAdd(S, s);
+  }
+}
+
 // part8[v] = part[v] when it names a part, 255 otherwise (k <= PART8_MAXK)
 __global__ void k_part8(const i32* part, i64 n, i32 k, u8* part8) {
   for (i64 i = gtid(); i < n; i += gstride()) {
@@ -4460,7 +4516,10 @@ void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64
   const i64 nb = (M + SEL_BLK - 1) / SEL_BLK;
   k_sel_hist<<<(unsigned)nb, SEL_BLK, 0, s>>>(sk_idx, mv_dst, M, k, H, sdst);
   k_sel_scan<<<(unsigned)k, SELSCAN_BS, 0, s>>>(H, nb);
-  k_sel_mark<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H, avail, sel, S, pads);
+  if (k <= SELW_MAXK)
+    k_sel_mark_w<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H, avail, sel, S, pads);
+  else
+    k_sel_mark<<<(unsigned)nb, SEL_BLK, 0, s>>>(sdst, M, k, H, avail, sel, S, pads);
 }
 
 void launch_commit(const i32* sk_idx, const u8* sel, i64 M, const i32* mv_v, const i32* mv_src,
