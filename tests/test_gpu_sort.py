@@ -1,9 +1,9 @@
 """The move sort (tgp_sort_moves) against the oracle's restatement of the
 reference's bitonic network (gpu_part.cu:472-506), bit for bit, at lengths
-the partition tests do not reach: global passes of 1, 2, 3 and 4 strides in
-registers (all four occur at 2^21 elements) and a length that is not a power
-of two.  Gains are small integers, so most keys tie and the network's own tie
-order is what is compared."""
+the partition tests do not reach: global passes of 1 to 5 strides in
+registers (all five occur at 2^21 elements) and a length that is not a power
+of two.  Gains are small integers, so most keys tie and the network's own
+tie order is what is compared."""
 import ctypes as C
 
 import numpy as np

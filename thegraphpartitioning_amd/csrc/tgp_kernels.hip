@@ -3478,7 +3478,7 @@ __global__ void k_select_all(i64 M, i32* sk_idx, u8* sel, i32* S, i32* pads) {
 // tiles held in registers (k_bt_tile); larger strides run as global passes
 // that apply up to four consecutive strides at once (k_bt_global).
 constexpr int BT_TB = 12;
-constexpr int BT_SMAX = 4;
+constexpr int BT_SMAX = 5;
 
 __global__ void k_sort_init(const float* mv_gain, i64 M, i64 len, float* key, i32* idx) {
   for (i64 i = gtid(); i < len; i += gstride()) {
@@ -4471,6 +4471,8 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
   // size kk runs only on the blocks below M rounded up to kk (nothing is
   // skipped under TGP_SORT_ALL=1).  The first pass reads the moves itself and
   // writes the pads of its tiles; the tiles past them take the pads by a fill.
+  const char* smax_env = std::getenv("TGP_SORT_SMAX");  // strides per global pass (1-5)
+  const int smax = smax_env ? std::max(1, std::min(BT_SMAX, std::atoi(smax_env))) : BT_SMAX;
   const char* all_env = std::getenv("TGP_SORT_ALL");
   const bool all = all_env && std::atoi(all_env) != 0;
   auto upto = [&](i64 kk) { return all ? len : std::min(len, (M + kk - 1) / kk * kk); };
@@ -4489,16 +4491,18 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
     int bhi = 0;  // log2 of the largest stride of this merge
     while (((i64)1 << (bhi + 1)) < kk) bhi++;
     for (int b = bhi; b >= BT_TB;) {
-      // Up to 4 strides per pass in registers.  Staging 5-8 strides in LDS
-      // (one pass instead of two) measured slower: 83 vs 78 ms per partition.
-      const int S = std::min(BT_SMAX, b - BT_TB + 1);
+      // Up to 5 strides per pass in registers (32 elements per thread; 4:
+      // 171 vs 165 ms, 6: 170 ms per partition on R-MAT 24).  Staging 5-8
+      // strides in LDS measured slower than registers.
+      const int S = std::min(smax, b - BT_TB + 1);
       const int blo = b - S + 1;
       // set t starts at ((t >> blo) << (blo + S)) | (t & (2^blo - 1)): the sets
       // below mk are the first ones
       const i64 span = (i64)1 << (blo + S);
       const i64 nsets = std::min(p2 >> S, ((mk + span - 1) / span) << blo);
       const unsigned gr = grid_for(nsets, 256, 1u << 16);
-      if (S == 4) k_bt_global<4><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      if (S == 5) k_bt_global<5><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
+      else if (S == 4) k_bt_global<4><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       else if (S == 3) k_bt_global<3><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       else if (S == 2) k_bt_global<2><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
       else k_bt_global<1><<<gr, 256, 0, s>>>(key, idx, len, nsets, kk, blo);
