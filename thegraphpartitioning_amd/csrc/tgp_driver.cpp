@@ -1174,6 +1174,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
 
     DBuf<u64> cutacc(&C.pool, 1);
     i64 lastM = n;  // moves buffered by the last pass of this level (token lane choice)
+    bool best_cur = true;  // part equals best (no commit since best was recorded)
     for (int phase = 0; phase < 2; phase++) {
       const int maxPass = phase ? 40 : 30;               // gpu_part.cu:545, 647
       bool balanced = false;
@@ -1292,6 +1293,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
             bestScore = score;
             HIP_TRY(hipMemcpyAsync(best.p, part.p, n * sizeof(i32), hipMemcpyDeviceToDevice,
                                    st()));
+            best_cur = true;
           }
           *cut = cur;
           tr.push(TGP_TRACE_PASS, level, phase, prevPass, prevM, Ssel, cur, pads,
@@ -1337,14 +1339,20 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           launch_tally_update(g, TR, k, app.p, napp.p, M, mv_v.p, mv_src.p, mv_dst.p, unit,
                               last_full ? 0 : 1, st());
         C.prof.end(KID_COMMIT, 17.0 * M);
+        best_cur = false;
         pending = true;
         prevM = M;
         lastM = M;
         prevPass = pass;
       }
-      HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
-      if (part8.p) launch_part8(part.p, n, k, part8.p, st());
-      T_valid = false;
+      // restore the best partition (gpu_part.cu:642, 746); when nothing
+      // was committed since it was recorded, part already equals it and the
+      // kept tallies and the byte copy stay valid
+      if (!best_cur) {
+        HIP_TRY(hipMemcpyAsync(part.p, best.p, n * sizeof(i32), hipMemcpyDeviceToDevice, st()));
+        if (part8.p) launch_part8(part.p, n, k, part8.p, st());
+        T_valid = false;
+      }
       if (phase == 0) *cut = bestCut;
     }
   }
