@@ -509,3 +509,35 @@ def test_reference_coarsening_fails_like_gpu_part(oracle):
 
 def test_final_text_format(oracle):
     assert oracle.final_text([1, 0, 1, 7], 3) == b"Partition 0: 1,3\nPartition 1: 0,2\nPartition 2: \n"
+
+
+@pytest.mark.parametrize("scale,ef,seed,clause", [(12, 8, 1, "95"), (12, 16, 3, "93"),
+                                                  (13, 16, 6, "93")])
+def test_repaired_stall_stops(oracle, scale, ef, seed, clause):
+    """The repaired coarsening's two stall stops, replayed by a loop of the
+    oracle's own pieces (matching, relabel, repaired contraction): stop when a
+    level's matching would keep more than 95 % of its vertices (that level is
+    dropped), and stop before matching when the contraction that made the
+    level kept more than 93 %.  The cases end on each clause; the oracle's
+    LEVEL records must list the same contractions."""
+    import thegraphpartitioning_amd as tgp
+    g = tgp.gen_rmat(scale, ef, 0x7C0DE5EED + seed)
+    floor = 20
+    x, a, w = g.xadj, g.adjncy, g.adjwgt
+    ns, why = [g.n], None
+    while ns[-1] > floor:
+        if len(ns) >= 2 and 100 * ns[-1] > 93 * ns[-2]:
+            why = "93"
+            break
+        nid, tb, nc = oracle.relabel(oracle.match(x, a, w))
+        if nc <= 0 or nc >= ns[-1]:
+            break
+        if 20 * nc > 19 * ns[-1]:
+            why = "95"
+            break
+        x, a, w = oracle.contract_repaired(x, a, w, tb, nid, nc)
+        ns.append(nc)
+    assert why == clause
+    _, _, _, tr = oracle.partition(g.xadj, g.adjncy, g.adjwgt, 8, coarsen_floor=floor, repair=1)
+    levels = [t for t in tr if t[0] == oracle.TR_LEVEL]
+    assert [(t[4], t[6]) for t in levels] == list(zip(ns[:-1], ns[1:]))
