@@ -1113,7 +1113,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     // every row (as does TGP_TALLY_BYTES=0).
     bool tmode = k <= 512 && (ax.all_exact || ax.nover <= OVER_ROWS_MAX) && lv.simple_ws &&
                  env_i64("TGP_KEEP_TALLIES", 1) != 0;
-    DBuf<i32> T32, bix, app, napp;
+    DBuf<i32> T32, bix, bigv, app, napp;
     DBuf<u32> bigmask;
     DBuf<u8> T8;
     TallyRef TR;
@@ -1146,9 +1146,18 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     const bool last_full = nedges == lv.m;
     const bool last_partial = !last_full && nedges > ax.last_begin;
     if (tmode) {
-      if (!all_int) T8 = DBuf<u8>(&C.pool, n * tally8_stride(k));
+      // row padding (entries past k) stays zero: the updates touch parts < k
+      if (!all_int) {
+        T8 = DBuf<u8>(&C.pool, n * tally8_stride(k));
+        if (k % 16) HIP_TRY(hipMemsetAsync(T8.p, 0, (size_t)n * tally8_stride(k), st()));
+        bigv = DBuf<i32>(&C.pool, std::max<i64>(nbig, 1));
+        launch_big_list(bix.p, n, bigv.p, st());
+      }
       T32 = DBuf<i32>(&C.pool, std::max<i64>(nbig, 1) * tally_row_stride(k));
-      TR = TallyRef{T8.p, T32.p, bix.p, bigmask.p, all_int ? 1 : 0};
+      if (k % 4)
+        HIP_TRY(hipMemsetAsync(T32.p, 0, (size_t)std::max<i64>(nbig, 1) * tally_row_stride(k) * 4,
+                               st()));
+      TR = TallyRef{T8.p, T32.p, bix.p, bigmask.p, all_int ? 1 : 0, bigv.p, (i32)nbig};
       app = DBuf<i32>(&C.pool, n);
       napp = DBuf<i32>(&C.pool, 1);
     }

@@ -346,12 +346,16 @@ constexpr i32 PART8_MAXK = 255;
 // (no bix); a level where most rows need ints takes that layout.  bigmask has
 // bit v set for the int rows of a mixed level (one bit per vertex, so the
 // class test of a row is an L2 hit while its byte row is already in flight).
+// bigv[b] (b < nbig) is the vertex of int row b on a mixed level.  Entries of
+// a row past k (its padding) hold zero.
 struct TallyRef {
   u8* t8 = nullptr;
   i32* t32 = nullptr;
   const i32* bix = nullptr;
   const u32* bigmask = nullptr;
   int all_int = 0;
+  const i32* bigv = nullptr;
+  i32 nbig = 0;
 };
 inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
 inline i64 tally8_stride(i32 k) { return (i64)((k + 15) & ~15); }
@@ -360,6 +364,8 @@ inline i64 tally8_stride(i32 k) { return (i64)((k + 15) & ~15); }
 // every row in the int table.
 void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i32* bix, u32* bigmask,
                         i32* nbig, hipStream_t s);
+// bigv[bix[v]] = v for the int rows of a mixed level
+void launch_big_list(const i32* bix, i64 n, i32* bigv, hipStream_t s);
 // Tout (optional): also writes each vertex's integer tally.
 bool launch_gain(DGraph g, i64 nedges, const i32* part, const u8* part8, i32 k, const float* bg,
                  const u8* src_big, float* gain, i32* target, float* cand, int phase, int unit,

@@ -2841,92 +2841,135 @@ __global__ __launch_bounds__(256) void k_gain_tallies(i64 n, const i32* part, Ta
   if (cut_acc) block_add_u64(cutv, cut_acc);
 }
 
-// k <= 64 on a level with byte rows: one lane per vertex.  A byte row is four
-// 16-byte loads, issued together with the other 63 lanes' rows; an int row
-// (the level's few heavy rows) is read in four chunks of 16 tallies.  The pick runs over q in lockstep across the
-// wave, so each balance term is one LDS broadcast; outputs are coalesced.
+// k <= 64 on a level with byte rows: one lane per vertex.  The first nb_byte
+// blocks take the byte rows (up to four 16-byte loads each, issued together
+// with the other 63 lanes' rows), the others the int rows from their list.
+// The pick runs over q in lockstep across the wave without branches: the
+// balance terms come from LDS four at a time (entries past k hold NaN, which
+// never wins), and since q rises, the first q of the largest gain is the one a
+// strict comparison keeps.  The arcs into other parts are the row's sum (its
+// padding holds zero) less the own tally.
 constexpr int GT1_MAXK = 64;
-__device__ __forceinline__ void gt1_pick(i32 q, i32 tq, i32 cur, i32 k, float ccur,
-                                         const float* bgr, float& bgv, i32& bq, i64& other) {
-  if (q >= k || q == cur) return;
-  other += tq;
-  const float eg = (float)tq - ccur;
-  const float gg = eg + bgr[q];
-  if (gg > bgv || (gg == bgv && q < bq)) {
-    bgv = gg;
-    bq = q;
+__device__ __forceinline__ void gt1_pick(i32 q, float tq, i32 cur, float ccur, float b,
+                                         float& bgv, i32& bq) {
+  const float gg = (tq - ccur) + b;
+  const bool better = gg > bgv && q != cur;
+  bgv = better ? gg : bgv;
+  bq = better ? q : bq;
+}
+__device__ __forceinline__ void gt1_store(i64 v, i32 cur, float bgv, i32 bq, float* gain,
+                                          i32* target, float* cand, int phase) {
+  float best;
+  i32 tgt;
+  if (bq == INT_MAX || bgv < -1e9f) {
+    best = -1e9f;
+    tgt = cur;
+  } else if (bgv == -1e9f) {
+    best = -1e9f;
+    tgt = bq < cur ? bq : cur;
+  } else {
+    best = bgv;
+    tgt = bq;
   }
+  gain[v] = best;
+  target[v] = tgt;
+  cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
 }
 __global__ __launch_bounds__(256) void k_gain_tallies1(i64 n, const i32* part, TallyRef T,
                                                        i32 k, const float* bg_g,
                                                        const u8* src_big, float* gain,
                                                        i32* target, float* cand, int phase,
-                                                       u64* cut_acc) {
-  extern __shared__ float sbgt[];
-  for (int q = threadIdx.x; q < 2 * k; q += blockDim.x) sbgt[q] = bg_g[q];
+                                                       u64* cut_acc, unsigned nb_byte) {
+  // two rows of GT1_MAXK balance terms (source part not big / big)
+  __shared__ __align__(16) float sbgt[2 * GT1_MAXK];
+  for (int i = threadIdx.x; i < 2 * GT1_MAXK; i += blockDim.x) {
+    const int q = i & (GT1_MAXK - 1);
+    sbgt[i] = q < k ? bg_g[(i >= GT1_MAXK ? k : 0) + q] : CAND_NO;
+  }
   __syncthreads();
-  const i64 ts = tally_stride(k), ts8 = tally8_stride_d(k);
+  const int nc = (k + 15) >> 4;  // 16-tally chunks
   u64 cutv = 0;
-  for (i64 v0 = (i64)blockIdx.x * blockDim.x; v0 < n; v0 += (i64)gridDim.x * blockDim.x) {
-    const i64 v = v0 + threadIdx.x;
-    if (v >= n) continue;
-    const i32 cur = part[v];
-    if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
-    const bool big = (T.bigmask[v >> 5] >> (v & 31)) & 1u;
-    const float* bgr = sbgt + (src_big[cur] ? k : 0);
-    float bgv = -INFINITY;
-    i32 bq = INT_MAX;
-    i64 other = 0;
-    if (!big) {
+  if (blockIdx.x < nb_byte) {
+    const i64 ts8 = tally8_stride_d(k);
+    for (i64 v0 = (i64)blockIdx.x * blockDim.x; v0 < n; v0 += (i64)nb_byte * blockDim.x) {
+      const i64 v = v0 + threadIdx.x;
+      if (v >= n || ((T.bigmask[v >> 5] >> (v & 31)) & 1u)) continue;
+      const i32 cur = part[v];
+      if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
+      const float4* B4 = (const float4*)(sbgt + (src_big[cur] ? GT1_MAXK : 0));
       const int4* R16 = (const int4*)(T.t8 + v * ts8);
       int4 c4[4];
 #pragma unroll
-      for (int c = 0; c < 4; c++) c4[c] = 16 * c < k ? R16[c] : make_int4(0, 0, 0, 0);
-      const float ccur = (float)T.t8[v * ts8 + cur];
+      for (int c = 0; c < 4; c++) c4[c] = c < nc ? R16[c] : make_int4(0, 0, 0, 0);
+      const u32 own = T.t8[v * ts8 + cur];
+      const float ccur = (float)own;
+      float bgv = -INFINITY;
+      i32 bq = INT_MAX;
+      u32 tot = 0;
 #pragma unroll
       for (int c = 0; c < 4; c++) {
+        if (c >= nc) break;
         const u32 w4[4] = {(u32)c4[c].x, (u32)c4[c].y, (u32)c4[c].z, (u32)c4[c].w};
 #pragma unroll
-        for (int j = 0; j < 16; j++)
-          gt1_pick(16 * c + j, (i32)((w4[j >> 2] >> (8 * (j & 3))) & 255u), cur, k, ccur, bgr,
-                   bgv, bq, other);
-      }
-    } else {
-      const i32* Tv = T.t32 + (i64)T.bix[v] * ts;
-      const int4* R4 = (const int4*)Tv;
-      const float ccur = (float)Tv[cur];
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        int4 c4[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) c4[r] = 16 * c + 4 * r < k ? R4[4 * c + r] : make_int4(0, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const i32 t4[4] = {c4[r].x, c4[r].y, c4[r].z, c4[r].w};
-#pragma unroll
-          for (int j = 0; j < 4; j++)
-            gt1_pick(16 * c + 4 * r + j, t4[j], cur, k, ccur, bgr, bgv, bq, other);
+        for (int i = 0; i < 4; i++) {
+          const int q0 = 16 * c + 4 * i;
+          const u32 w = w4[i];
+          tot = __builtin_amdgcn_sad_u8(w, 0u, tot);
+          const float4 b = B4[q0 >> 2];
+          gt1_pick(q0, (float)(w & 255u), cur, ccur, b.x, bgv, bq);
+          gt1_pick(q0 + 1, (float)((w >> 8) & 255u), cur, ccur, b.y, bgv, bq);
+          gt1_pick(q0 + 2, (float)((w >> 16) & 255u), cur, ccur, b.z, bgv, bq);
+          gt1_pick(q0 + 3, (float)(w >> 24), cur, ccur, b.w, bgv, bq);
         }
       }
+      cutv += (u64)(tot - own);
+      gt1_store(v, cur, bgv, bq, gain, target, cand, phase);
     }
-    cutv += (u64)other;
-    float best;
-    i32 tgt;
-    if (bq == INT_MAX || bgv < -1e9f) {
-      best = -1e9f;
-      tgt = cur;
-    } else if (bgv == -1e9f) {
-      best = -1e9f;
-      tgt = bq < cur ? bq : cur;
-    } else {
-      best = bgv;
-      tgt = bq;
+  } else {
+    const i64 ts = tally_stride(k);
+    const int nv4 = (int)(ts >> 2);
+    const i64 stride = (i64)(gridDim.x - nb_byte) * blockDim.x;
+    for (i64 r = (i64)(blockIdx.x - nb_byte) * blockDim.x + threadIdx.x; r < T.nbig;
+         r += stride) {
+      const i64 v = T.bigv[r];
+      const i32 cur = part[v];
+      if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
+      const float4* B4 = (const float4*)(sbgt + (src_big[cur] ? GT1_MAXK : 0));
+      const int4* R4 = (const int4*)(T.t32 + r * ts);  // bix[v] == r
+      const i32 own = T.t32[r * ts + cur];
+      const float ccur = (float)own;
+      float bgv = -INFINITY;
+      i32 bq = INT_MAX;
+      i64 tot = 0;
+#pragma unroll 1
+      for (int c = 0; c < nc; c++) {
+        int4 c4[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          c4[i] = 4 * c + i < nv4 ? R4[4 * c + i] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int q0 = 16 * c + 4 * i;
+          const float4 b = B4[q0 >> 2];
+          tot += (i64)c4[i].x + c4[i].y + c4[i].z + c4[i].w;
+          gt1_pick(q0, (float)c4[i].x, cur, ccur, b.x, bgv, bq);
+          gt1_pick(q0 + 1, (float)c4[i].y, cur, ccur, b.y, bgv, bq);
+          gt1_pick(q0 + 2, (float)c4[i].z, cur, ccur, b.z, bgv, bq);
+          gt1_pick(q0 + 3, (float)c4[i].w, cur, ccur, b.w, bgv, bq);
+        }
+      }
+      cutv += (u64)(tot - own);
+      gt1_store(v, cur, bgv, bq, gain, target, cand, phase);
     }
-    gain[v] = best;
-    target[v] = tgt;
-    cand[v] = (phase ? (best > -50.0f) : (best > -10.0f)) ? best : CAND_NO;
   }
   if (cut_acc) block_add_u64(cutv, cut_acc);
+}
+
+__global__ void k_big_list(const i32* bix, i64 n, i32* bigv) {
+  for (i64 v = gtid(); v < n; v += gstride()) {
+    const i32 b = bix[v];
+    if (b >= 0) bigv[b] = (i32)v;
+  }
 }
 
 // Edge cut from kept tallies (gpu_part.cu:400-422 on a level whose tallies
@@ -4546,8 +4589,9 @@ void launch_gain_tallies(i64 n, const i32* part, const TallyRef& T, i32 k, const
   // one lane per vertex on a level with byte rows (k <= 64); int rows are read
   // by 4 lanes per vertex unless the caller asks for another width
   if (k <= GT1_MAXK && lanes == 1 && !T.all_int) {
-    k_gain_tallies1<<<grid_for(n, 256, 8192), 256, lds, s>>>(
-        n, part, T, k, bg, src_big, gain, target, cand, phase, cut_acc);
+    const unsigned nb = grid_for(n, 256, 8192), nbb = grid_for(T.nbig, 256, 1024);
+    k_gain_tallies1<<<nb + nbb, 256, 0, s>>>(n, part, T, k, bg, src_big, gain, target, cand,
+                                             phase, cut_acc, nb);
     return;
   }
   if (lanes == 1) lanes = 4;
@@ -4582,6 +4626,10 @@ void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i32* bix, u
                         i32* nbig, hipStream_t s) {
   k_tally_class<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, unit, all_int, bix,
                                                               bigmask, nbig);
+}
+
+void launch_big_list(const i32* bix, i64 n, i32* bigv, hipStream_t s) {
+  if (n > 0) k_big_list<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(bix, n, bigv);
 }
 
 void launch_part8(const i32* part, i64 n, i32 k, u8* part8, hipStream_t s) {
