@@ -680,8 +680,12 @@ void Run::check_symmetric(Level& lv) {
   DBuf<u64> table;
   DBuf<i32> wt;
   C.prof.begin(KID_MISC);
+  DBuf<u8> cls;
+  DBuf<i32> crow;
   if (strict) {
-    launch_sym_check_sorted(lv.dg(), flags.p, cnt.p, st());
+    cls = DBuf<u8>(&C.pool, std::max<i64>(lv.n, 1));
+    crow = DBuf<i32>(&C.pool, std::max<i64>(sym_check_chunks(lv.m), 1));
+    launch_sym_check_sorted(lv.dg(), lv.m, cls.p, crow.p, flags.p, cnt.p, st());
   } else {
     i64 cap = 1;
     while (cap < 2 * lv.m + 2) cap <<= 1;

@@ -188,10 +188,15 @@ void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStr
 // flags[3] |= some row is not strictly ascending
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
 // the same checks for strictly ascending rows, by binary search of the
-// reverse arc in its (sorted) row: flags[0], flags[2] as above; cnt[0] = arcs
-// up, cnt[1] = arcs down whose reverse was found (symmetric iff flags[0] == 0
-// and they agree); flags[4] |= a found pair's weight is not 1
-void launch_sym_check_sorted(DGraph g, i32* flags, u64* cnt, hipStream_t s);
+// reverse arc in its (sorted) row: flags[0], flags[2] as above; of each pair
+// the arc into the row of lower (degree bit length, id) searches: cnt[1] =
+// searching arcs whose reverse was found, cnt[0] = the other arcs (symmetric
+// iff flags[0] == 0 and they agree); flags[4] |= a found pair's weight is not
+// 1; cls (n bytes) receives the degree bit lengths, crow
+// (sym_check_chunks(m) entries, m = xadj[n]) the row of each chunk's first arc
+i64 sym_check_chunks(i64 m);
+void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, i32* crow, i32* flags, u64* cnt,
+                             hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);
 // Reference contraction (gpu_part.cu:145-242 semantics): padded rows of 64.

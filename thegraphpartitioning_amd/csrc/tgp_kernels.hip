@@ -1340,33 +1340,85 @@ __global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
   if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
 }
 
-// Strictly ascending rows: each arc v->u with u > v is counted (cnt[0]); each
-// arc v->u with u < v binary-searches row u for v and compares weights
-// (counted in cnt[1] when found, flags[0] when missing, flags[2] on a weight
-// mismatch).  Rows hold no repeats, so distinct lower arcs find distinct upper
-// arcs: symmetric iff nothing is missing and both counts agree.  flags[4] is
-// set when a found pair carries a weight other than 1 (on a symmetric level
-// the found pairs hold every arc between distinct vertices in range).
-__global__ __launch_bounds__(256) void k_sym_search_sorted(DGraph g, i32* flags, u64* cnt) {
+// Bit length of each row's degree (the rank of a row in the search below).
+__global__ void k_deg_class(DGraph g, u8* cls) {
+  for (i64 v = gtid(); v < g.n; v += gstride()) {
+    const i64 d = g.xadj[v + 1] - g.xadj[v];
+    cls[v] = d > 0 ? (u8)(64 - __clzll((unsigned long long)d)) : 0;
+  }
+}
+
+// crow[c] = the row holding entry c * SYM_CHUNK (chunks of the arc range)
+constexpr int SYM_CHUNK = 1024;
+__global__ void k_chunk_rows(DGraph g, i64 nch, i32* crow) {
+  for (i64 v = gtid(); v < g.n; v += gstride()) {
+    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
+    if (s1 <= s0) continue;
+    for (i64 c = (s0 + SYM_CHUNK - 1) / SYM_CHUNK; c <= (s1 - 1) / SYM_CHUNK && c < nch; c++)
+      crow[c] = (i32)v;
+  }
+}
+
+// Strictly ascending rows.  Of the two arcs of a pair {u, v}, the one whose
+// target row ranks lower by (degree bit length, id) binary-searches that row
+// for its source and compares weights (counted in cnt[1] when found, flags[0]
+// when missing, flags[2] on a weight mismatch); the other is counted in
+// cnt[0].  Rows hold no repeats, so distinct searching arcs find distinct
+// counted arcs: symmetric iff nothing is missing and both counts agree.  The
+// searched row is never much longer than the searching one, so on skewed
+// degrees the searches stay short.  One wave per chunk of SYM_CHUNK entries
+// of the arc range (long rows spread over many waves): 64 entries per step,
+// each lane finding its row by binary search over the ends of the 64 rows
+// from the current one (held one per lane).  flags[4] is set when a found
+// pair carries a weight other than 1 (on a symmetric level the found pairs
+// hold every arc between distinct vertices in range).
+__global__ __launch_bounds__(256) void k_sym_search_sorted(DGraph g, const u8* cls,
+                                                           const i32* crow, i64 nch, i64 m,
+                                                           i32* flags, u64* cnt) {
   const int lane = lane_id();
   bool miss = false, wmis = false, nonunit = false;
   u64 cu = 0, cd = 0;
-  for (i64 v = gwave(); v < g.n; v += nwaves()) {
-    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
-    for (i64 i = s0 + lane; i < s1; i += WAVE) {
-      const i32 u = g.adj[i];
-      if (u >= g.n || u < 0 || u == v) continue;
-      if (u > v) {
+  for (i64 c = gwave(); c < nch; c += nwaves()) {
+    i64 b = c * SYM_CHUNK;
+    const i64 bend = b + SYM_CHUNK < m ? b + SYM_CHUNK : m;
+    i64 r = crow[c];
+    while (b < bend) {
+      const i64 rl = r + lane;
+      const i64 e = g.xadj[rl < g.n ? rl + 1 : g.n];  // end of row rl
+      const i32 cl = rl < g.n ? cls[rl] : 0;
+      const i64 i = b + lane;
+      const i32 u = i < bend ? g.adj[i] : -1;
+      // rows of the window that end at or before b hold none of the entries
+      const int done = __popcll(__ballot(e <= b));
+      if (done) {
+        r += done;
+        continue;
+      }
+      const i64 e63 = __shfl(e, WAVE - 1);
+      const i64 lim = b + WAVE < bend ? b + WAVE : bend;
+      const i64 wend = lim < e63 ? lim : e63;
+      // lanes whose row ends at or before i: the row of i is r + that count
+      int rr = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(e, rr + step - 1) <= i) rr += step;
+      const i32 cv = __shfl(cl, rr);
+      const i64 v = r + rr;
+      b = wend;
+      if (i >= wend || u >= g.n || u < 0 || u == v) continue;
+      const i32 cuu = cls[u];
+      if (!(cuu < cv || (cuu == cv && u < v))) {
         cu++;
         continue;
       }
       i64 lo = g.xadj[u], hi = g.xadj[u + 1];  // first entry >= v in [lo, hi)
+      const i64 end = hi;
       while (lo < hi) {
         const i64 mid = lo + ((hi - lo) >> 1);
         if (g.adj[mid] < v) lo = mid + 1;
         else hi = mid;
       }
-      if (lo < g.xadj[u + 1] && g.adj[lo] == v) {
+      if (lo < end && g.adj[lo] == v) {
         const i32 wl = g.w[lo], wi = g.w[i];
         wmis |= wl != wi;
         nonunit |= wl != 1 || wi != 1;
@@ -3421,42 +3473,85 @@ __global__ __launch_bounds__(1024) void k_flag_offsets(const i32* bcnt, i32 nb, 
   }
   if (threadIdx.x == 0) *total = all;
 }
+// Each block walks its range in tiles of CMP_TILE vertices: thread t of the
+// block takes vertices t, t + CMP_BS, ... of the tile (coalesced), all loads
+// of the tile are issued at once, and one scan of the 64 per-(round, wave)
+// winner counts gives every winner its slot.
+constexpr int CMP_PT = 16;
+constexpr int CMP_TILE = CMP_BS * CMP_PT;
+static_assert(CMP_PT * (CMP_BS / WAVE) == WAVE, "one scan lane per (round, wave)");
 __global__ __launch_bounds__(CMP_BS) void k_scatter_moves(const u8* flag, i64 n, i64 per,
                                                          const i32* boff, const i32* part,
                                                          const i32* target, const float* gain,
                                                          i32 k, i32* mv_v, i32* mv_src,
                                                          i32* mv_dst, float* mv_gain,
                                                          i32* outc, i32* inc) {
-  __shared__ i32 h[MAXK_LDS], h2[MAXK_LDS];
-  __shared__ i32 sw[CMP_BS / WAVE];
-  for (int q = threadIdx.x; q < k && q < MAXK_LDS; q += CMP_BS) {
-    h[q] = 0;
-    h2[q] = 0;
-  }
-  __syncthreads();
+  // per-part move counts of the block (k <= MAXK_LDS): out of src, into dst
+  extern __shared__ i32 hsh[];
+  __shared__ i32 scnt[WAVE];
+  __shared__ i32 stot;
+  constexpr int NW = CMP_BS / WAVE;
+  const bool lds_h = k <= MAXK_LDS;
+  i32* h = hsh;
+  i32* h2 = hsh + (lds_h ? k : 0);
+  if (lds_h)
+    for (int q = threadIdx.x; q < k; q += CMP_BS) {
+      h[q] = 0;
+      h2[q] = 0;
+    }
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const i64 b0 = (i64)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
   i64 run = boff[blockIdx.x];
-  for (i64 base = b0; base < b1; base += CMP_BS) {
-    const i64 v = base + threadIdx.x;
-    const bool f = v < b1 && flag[v];
-    const u64 bm = __ballot(f);
-    if (lane == 0) sw[wv] = __popcll(bm);
-    __syncthreads();
-    i32 before = 0, tot = 0;
-    for (int w2 = 0; w2 < CMP_BS / WAVE; w2++) {
-      if (w2 < wv) before += sw[w2];
-      tot += sw[w2];
+  for (i64 sub = b0; sub < b1; sub += CMP_TILE) {
+    bool f[CMP_PT];
+    u64 bm[CMP_PT];
+#pragma unroll
+    for (int j = 0; j < CMP_PT; j++) {
+      const i64 v = sub + (i64)j * CMP_BS + threadIdx.x;
+      f[j] = v < b1 && flag[v];
     }
-    if (f) {
-      const i64 p = run + before + __popcll(bm & lanes_below(lane));
-      const i32 src = part[v];
-      const i32 dst = target[v];
+#pragma unroll
+    for (int j = 0; j < CMP_PT; j++) bm[j] = __ballot(f[j]);
+    __syncthreads();  // the previous tile's scan is read
+    if (lane < CMP_PT) {
+      i32 c = 0;
+#pragma unroll
+      for (int j = 0; j < CMP_PT; j++) c = lane == j ? __popcll(bm[j]) : c;
+      scnt[lane * NW + wv] = c;
+    }
+    __syncthreads();
+    if (wv == 0) {  // exclusive scan over (round, wave) in vertex order
+      const i32 c = scnt[lane];
+      i32 x = c;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const i32 y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      scnt[lane] = x - c;
+      if (lane == WAVE - 1) stot = x;
+    }
+    __syncthreads();
+    i32 pp[CMP_PT], tt[CMP_PT];
+    float gg[CMP_PT];
+#pragma unroll
+    for (int j = 0; j < CMP_PT; j++) {
+      const i64 v = sub + (i64)j * CMP_BS + threadIdx.x;
+      pp[j] = f[j] ? part[v] : 0;
+      tt[j] = f[j] ? target[v] : 0;
+      gg[j] = f[j] ? gain[v] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < CMP_PT; j++) {
+      if (!f[j]) continue;
+      const i64 v = sub + (i64)j * CMP_BS + threadIdx.x;
+      const i64 p = run + scnt[j * NW + wv] + __popcll(bm[j] & lanes_below(lane));
+      const i32 src = pp[j], dst = tt[j];
       mv_v[p] = (i32)v;
       mv_src[p] = src;
       mv_dst[p] = dst;
-      mv_gain[p] = gain[v];
-      if (k <= MAXK_LDS) {
+      mv_gain[p] = gg[j];
+      if (lds_h) {
         atomicAdd(&h[src], 1);
         if (inc) atomicAdd(&h2[dst], 1);
       } else {
@@ -3464,10 +3559,10 @@ __global__ __launch_bounds__(CMP_BS) void k_scatter_moves(const u8* flag, i64 n,
         if (inc) atomicAdd(&inc[dst], 1);
       }
     }
-    run += tot;
-    __syncthreads();
+    run += stot;
   }
-  if (k <= MAXK_LDS)
+  __syncthreads();
+  if (lds_h)
     for (int q = threadIdx.x; q < k; q += CMP_BS) {
       if (h[q]) atomicAdd(&outc[q], h[q]);
       if (inc && h2[q]) atomicAdd(&inc[q], h2[q]);
@@ -4211,8 +4306,15 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s) {
   k_rows_strict<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags);
 }
-void launch_sym_check_sorted(DGraph g, i32* flags, u64* cnt, hipStream_t s) {
-  k_sym_search_sorted<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags, cnt);
+i64 sym_check_chunks(i64 m) { return (m + SYM_CHUNK - 1) / SYM_CHUNK; }
+void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, i32* crow, i32* flags, u64* cnt,
+                             hipStream_t s) {
+  k_deg_class<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, cls);
+  const i64 nch = sym_check_chunks(m);
+  if (nch <= 0) return;
+  k_chunk_rows<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nch, crow);
+  k_sym_search_sorted<<<grid_for(nch, 4, 1u << 16), 256, 0, s>>>(g, cls, crow, nch, m, flags,
+                                                                 cnt);
 }
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s) {
   k_fill<u64><<<grid_for(cap, 256, 1u << 16), 256, 0, s>>>(table, cap, ~0ull);
@@ -4481,13 +4583,14 @@ void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* tar
   if (nb < 1) nb = 1;
   if (nb > CMP_MAXB) nb = CMP_MAXB;
   i64 per = (n + nb - 1) / nb;
-  per = (per + CMP_BS - 1) / CMP_BS * CMP_BS;
+  per = (per + CMP_TILE - 1) / CMP_TILE * CMP_TILE;
   nb = (n + per - 1) / per;
   if (nb < 1) nb = 1;
   k_flag_count<<<(unsigned)nb, CMP_BS, 0, s>>>(flag, n, per, bcnt);
   k_flag_offsets<<<1, 1024, 0, s>>>(bcnt, (i32)nb, boff, total);
-  k_scatter_moves<<<(unsigned)nb, CMP_BS, 0, s>>>(flag, n, per, boff, part, target, gain, k,
-                                                   mv_v, mv_src, mv_dst, mv_gain, outc, inc);
+  const size_t lds = k <= MAXK_LDS ? (size_t)2 * k * sizeof(i32) : 0;
+  k_scatter_moves<<<(unsigned)nb, CMP_BS, lds, s>>>(flag, n, per, boff, part, target, gain, k,
+                                                     mv_v, mv_src, mv_dst, mv_gain, outc, inc);
 }
 
 void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i64* avail,
