@@ -1167,6 +1167,12 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     }
     DBuf<u8> sel(&C.pool, n), src_big(&C.pool, k);
     DBuf<u8> flag(&C.pool, n);
+    // token witnesses of the level's passes (-1: none yet)
+    DBuf<i32> wit;
+    if (env_i64("TGP_TOKEN_WITNESS", 1) != 0) {
+      wit = DBuf<i32>(&C.pool, n);
+      HIP_TRY(hipMemsetAsync(wit.p, 0xFF, (size_t)n * sizeof(i32), st()));
+    }
     DBuf<i32> bcnt(&C.pool, compaction_blocks()), boff(&C.pool, compaction_blocks());
     DBuf<i64> avail(&C.pool, k);
     DBuf<float> bg(&C.pool, 2 * (size_t)k);
@@ -1240,7 +1246,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           // the first losing neighbour, measured faster.  Both are exact.
           const int tlanes = (TOKEN_LANES == 0 && 10 * lastM < 3 * n) ? 1 : TOKEN_LANES;
           launch_token(g, nedges, cand.p, part.p, target.p, k, flag.p, heavy, nheavy, HEAVY_DEG,
-                       tlanes, st());
+                       tlanes, wit.p, st());
           // 9m + (o + 6)n: arcs and the candidates' gains, offsets, own gain and flag
           C.prof.end(KID_TOKEN, 9.0 * lv.m + 14.0 * n);
           C.prof.begin(KID_COMPACT);
@@ -1436,7 +1442,7 @@ void Run::learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, 
     HIP_TRY(hipMemsetAsync(outc.p, 0, k * sizeof(i32), st()));
     HIP_TRY(hipMemsetAsync(ctr.p, 0, 2 * sizeof(i32), st()));
     launch_token(g, nedges, gain.p, part.p, target.p, k, flag.p, ax.heavy.p, ax.nheavy,
-                 HEAVY_DEG, TOKEN_LANES, st());
+                 HEAVY_DEG, TOKEN_LANES, nullptr, st());
     launch_compact_moves(flag.p, n, part.p, target.p, gain.p, k, bcnt.p, boff.p, small.p,
                          mv_v.p, mv_src.p, mv_dst.p, mv_gain.p, outc.p, nullptr, st());
     HIP_TRY(hipMemcpyAsync(C.pinned, small.p, sizeof(i64), hipMemcpyDeviceToHost, st()));

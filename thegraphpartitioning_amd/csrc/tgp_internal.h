@@ -406,10 +406,14 @@ void launch_huge_bounds(DGraph g, i64 nedges, const i32* huge, i64 nhuge, i64* h
 i64 gain_gscratch_elems(i32 k);
 // cand[v] = gain of v when v is a move candidate, NaN otherwise (the gain
 // kernels write it); rows of at most heavy_deg arcs: lanes = 0 streams the
-// arcs of 64 consecutive rows per wave, 1, 4 or 8 lanes per row otherwise
+// arcs of 64 consecutive rows per wave, 1, 4 or 8 lanes per row otherwise.
+// wit (optional, n entries, -1 or a neighbour of v, kept across the passes of
+// a level): a neighbour that beat v in an earlier pass, checked before v's row
+// is read and rewritten when the row finds another (the arc stream, one
+// thread per row and the heavy rows use it; 4 or 8 lanes per row do not)
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
-                  int lanes, hipStream_t s);
+                  int lanes, i32* wit, hipStream_t s);
 // Winners (flag[v] = 1) to the move arrays in ascending vertex order; *total
 // = their count, outc[src] += moves out of each part and (when given)
 // inc[dst] += moves into each part.  bcnt and boff hold compaction_blocks()

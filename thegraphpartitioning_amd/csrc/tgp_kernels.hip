@@ -3205,6 +3205,21 @@ __device__ __forceinline__ bool move_ok(i32 src, i32 dst, i32 k) {
   return src < k && dst < k && src != dst && src >= 0 && dst >= 0;
 }
 
+// wit[v] (optional) keeps, from an earlier pass on the level, a neighbour
+// that beat v then; when it still beats v, v loses without its row being
+// read.  The rule is the same ("some neighbour beats v"), so a stale witness
+// only costs the row read.  The last row (whose range ends at nedges) keeps
+// none.
+__device__ __forceinline__ bool token_beats(float gu, i32 u, float gv, i64 v) {
+  return gu > gv + 10.0f || (gu == gv && (i64)u < v);
+}
+__device__ __forceinline__ bool token_witness_beats(const i32* wit, const float* cand, i64 n,
+                                                    i64 v, float gv) {
+  if (!wit || v == n - 1) return false;
+  const i32 x = wit[v];
+  return x >= 0 && x < n && token_beats(cand[x], x, gv, v);
+}
+
 // cand[u] is u's gain when u is a candidate and NaN otherwise, so one load
 // answers both questions and every comparison with a non-candidate is false.
 // One launch serves both row classes at once: the first `bh` blocks (issued
@@ -3268,7 +3283,7 @@ __device__ __forceinline__ void token_light_groups(const DGraph& g, i64 nedges,
 __device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
                                                    const float* cand, const i32* part,
                                                    const i32* target, i32 k, u8* flag,
-                                                   i64 heavy_deg, unsigned bh) {
+                                                   i64 heavy_deg, unsigned bh, i32* wit) {
   const int lane = lane_id();
   const i64 ngw = (i64)(gridDim.x - bh) * (blockDim.x / WAVE);
   for (i64 gw = (i64)(blockIdx.x - bh) * (blockDim.x / WAVE) + (threadIdx.x >> 6);
@@ -3286,6 +3301,11 @@ __device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
     i32 rem = (isc && s1 > s0) ? (i32)(s1 - s0) : 0;
     i64 next = s0;  // the row's next entry
     bool lost = false;
+    if (rem > 0 && token_witness_beats(wit, cand, g.n, v, gv)) {
+      lost = true;
+      rem = 0;
+    }
+    i32 wnew = -1;  // a neighbour found beating v in this pass
     while (true) {
       i32 incl = rem;
 #pragma unroll
@@ -3308,20 +3328,20 @@ __device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
       const i64 at = __shfl(next, r) + (lane - rst);
       const float rg = __shfl(gv, r), rg10 = __shfl(gv10, r);
       bool l = false;
-      if (act) {
-        const i32 u = g.adj[at];
-        if (u >= 0 && u < g.n) {
-          const float gu = cand[u];
-          l = gu > rg10 || (gu == rg && (i64)u < gw * WAVE + r);
-        }
+      const i32 u = act ? g.adj[at] : -1;
+      if (u >= 0 && u < g.n) {
+        const float gu = cand[u];
+        l = gu > rg10 || (gu == rg && (i64)u < gw * WAVE + r);
       }
-      // rows with a losing neighbour in this step
+      // rows with a losing neighbour in this step (and one such neighbour each)
       u64 lm = 0;
       u64 m = __ballot(l);
       while (m) {
         const int b = __ffsll((unsigned long long)m) - 1;
         const int rb = __shfl(r, b);
+        const i32 ub = __shfl(u, b);
         lm |= 1ull << rb;
+        if (lane == rb) wnew = ub;
         m &= ~__ballot(l && r == rb);
       }
       // this row's entries read in the step: min(rem, 64 - start) when start < 64
@@ -3334,6 +3354,7 @@ __device__ __forceinline__ void token_light_stream(const DGraph& g, i64 nedges,
       }
     }
     if (light) flag[v] = (isc && !lost && move_ok(part[v], target[v], k)) ? 1 : 0;
+    if (wit && wnew >= 0 && v != g.n - 1) wit[v] = wnew;
   }
 }
 
@@ -3341,9 +3362,9 @@ template <int G>
 __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float* cand,
                                                const i32* part, const i32* target, i32 k,
                                                u8* flag, i64 heavy_deg, const i32* list,
-                                               i64 cnt, unsigned bh) {
+                                               i64 cnt, unsigned bh, i32* wit) {
   if (blockIdx.x >= bh && G == 0) {
-    token_light_stream(g, nedges, cand, part, target, k, flag, heavy_deg, bh);
+    token_light_stream(g, nedges, cand, part, target, k, flag, heavy_deg, bh, wit);
     return;
   }
   if (blockIdx.x >= bh && G > 1) {
@@ -3364,20 +3385,24 @@ __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float
         continue;
       }
       const float gv10 = gv + 10.0f;
-      bool lose = false;
+      bool lose = s1 > s0 && token_witness_beats(wit, cand, g.n, v, gv);
+      i32 wnew = -1;
       for (i64 base = s0; base < s1 && !lose; base += TU) {
         i32 uu[TU];
 #pragma unroll
         for (int j = 0; j < TU; j++) uu[j] = base + j < s1 ? g.adj[base + j] : INT_MAX;
 #pragma unroll
         for (int j = 0; j < TU; j++) {
-          if (uu[j] < g.n) {
+          if (uu[j] >= 0 && uu[j] < g.n) {
             const float gu = cand[uu[j]];
-            lose |= gu > gv10 || (gu == gv && uu[j] < v);
+            const bool l = gu > gv10 || (gu == gv && uu[j] < v);
+            if (l && !lose) wnew = uu[j];
+            lose |= l;
           }
         }
       }
       flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
+      if (wit && wnew >= 0 && v != g.n - 1) wit[v] = wnew;
     }
     return;
   }
@@ -3394,20 +3419,23 @@ __global__ __launch_bounds__(256) void k_token(DGraph g, i64 nedges, const float
     const i64 s0 = g.xadj[v];
     const i64 s1 = (v + 1 < g.n) ? g.xadj[v + 1] : nedges;
     const float gv10 = gv + 10.0f;
-    bool lose = false;
-    for (i64 base = s0; base < s1; base += WAVE) {
+    bool lose = s1 > s0 && token_witness_beats(wit, cand, g.n, v, gv);
+    for (i64 base = s0; base < s1 && !lose; base += WAVE) {
       const i64 i = base + lane;
       bool l = false;
+      i32 u = -1;
       if (i < s1) {
-        const i32 u = g.adj[i];
-        if (u < g.n) {
+        u = g.adj[i];
+        if (u >= 0 && u < g.n) {
           const float gu = cand[u];
           l = gu > gv10 || (gu == gv && u < v);
         }
       }
-      if (__ballot(l)) {
+      const u64 lm = __ballot(l);
+      if (lm) {
         lose = true;
-        break;
+        const i32 ub = __shfl(u, __ffsll((unsigned long long)lm) - 1);
+        if (wit && lane == 0 && v != g.n - 1) wit[v] = ub;
       }
     }
     if (lane == 0) flag[v] = (!lose && move_ok(part[v], target[v], k)) ? 1 : 0;
@@ -4557,22 +4585,22 @@ void launch_gain_rows(DGraph g, i64 nedges, const i32* part, i32 k, const float*
 
 void launch_token(DGraph g, i64 nedges, const float* cand, const i32* part, const i32* target,
                   i32 k, u8* flag, const i32* heavy_list, i64 heavy_count, i64 heavy_deg,
-                  int lanes, hipStream_t s) {
+                  int lanes, i32* wit, hipStream_t s) {
   const int G = (lanes == 0 || lanes == 4 || lanes == 8) ? lanes : 1;
   const unsigned bl = grid_for(g.n, G ? 256 / G : 256, 1u << 16);
   const unsigned bh = heavy_count > 0 ? grid_for(heavy_count, 4, 8192) : 0;
   if (G == 0)
     k_token<0><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
-                                       heavy_list, heavy_count, bh);
+                                       heavy_list, heavy_count, bh, wit);
   else if (G == 8)
     k_token<8><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
-                                       heavy_list, heavy_count, bh);
+                                       heavy_list, heavy_count, bh, wit);
   else if (G == 4)
     k_token<4><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
-                                       heavy_list, heavy_count, bh);
+                                       heavy_list, heavy_count, bh, wit);
   else
     k_token<1><<<bl + bh, 256, 0, s>>>(g, nedges, cand, part, target, k, flag, heavy_deg,
-                                       heavy_list, heavy_count, bh);
+                                       heavy_list, heavy_count, bh, wit);
 }
 
 void launch_compact_moves(const u8* flag, i64 n, const i32* part, const i32* target,
