@@ -338,3 +338,27 @@ def test_long_row_contraction_forms(oracle, monkeypatch, env):
     g = tgp.gen_rmat(15, 16, 0x7C0DE5EED + 15)
     gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 32, repair=True)
     assert_same(gp, op, f"rmat15 {env}")
+
+
+@pytest.mark.parametrize("lmin", ["off", "128", "300", "4096"])
+def test_chain_best_entry_lists(oracle, monkeypatch, lmin):
+    """The chained matching's best-entry lists of long rows on weighted levels
+    (the 64 best entries by weight, then position; a proposal is the first
+    still-free entry, and the row is scanned once the list is used up): with
+    lists on rows longer than 128 (every long row of the chain), 300 or 4096
+    arcs (the default), and with none, traces and partitions equal the
+    oracle's.  The twin hubs' rows carry weights 1..5 (ties broken by
+    position) and lose their list entries to the leaves' matches."""
+    if lmin == "off":
+        monkeypatch.setenv("TGP_CHAIN_LISTS", "0")
+    else:
+        monkeypatch.setenv("TGP_CHAIN_LIST_MIN", lmin)
+    xadj, adj, w = twin_hub_graph(5000, 12000, 5)
+    gp, op = run_both(oracle, xadj, adj, w, 8, repair=True, floor=50)
+    assert_same(gp, op, f"twin hubs lists {lmin}")
+    g = tgp.gen_rmat(15, 16, 0x7C0DE5EED + 15, "hash16")
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 32, repair=True)
+    assert_same(gp, op, f"rmat15 weighted lists {lmin}")
+    g = tgp.gen_rmat(16, 16, 0x7C0DE5EED + 16)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 64, repair=True)
+    assert_same(gp, op, f"rmat16 lists {lmin}")

@@ -577,6 +577,26 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
       if (tot > 0) launch_chain_low_write(g, llist.p, nll.p, lowoff.p, lowadj.p, st());
       low = ChainLow{lowoff.p, lowadj.p};
     }
+    // Best-entry lists of the rows longer than TGP_CHAIN_LIST_MIN (4096) arcs
+    // on weighted levels (TGP_CHAIN_LISTS=0: none): a long row's proposal is
+    // then its list's first free entry, and the row is scanned only once the
+    // list is used up.
+    DBuf<i32> lidx, lrows, lnl, llists, lcnt;
+    if (!lv.unitw && env_i64("TGP_CHAIN_LISTS", 1) != 0) {
+      const i64 lmin =
+          std::max<i64>(env_i64("TGP_CHAIN_LIST_MIN", 4096), (i64)CHAIN_SG * CHAIN_PER);
+      const i64 maxl = lv.m / lmin + 1;
+      lidx = DBuf<i32>(&C.pool, n);
+      lrows = DBuf<i32>(&C.pool, maxl);
+      lnl = DBuf<i32>(&C.pool, 1);
+      llists = DBuf<i32>(&C.pool, maxl * CHAIN_LIST_LEN);
+      lcnt = DBuf<i32>(&C.pool, maxl);
+      HIP_TRY(hipMemsetAsync(lnl.p, 0, sizeof(i32), st()));
+      launch_chain_lists(g, lmin, lidx.p, lrows.p, lnl.p, llists.p, lcnt.p, st());
+      low.lidx = lidx.p;
+      low.lists = llists.p;
+      low.lcnt = lcnt.p;
+    }
     // TGP_CHAIN_EXACT_SCANS=0 keeps the full scans on sorted / unit-weight levels
     if (env_i64("TGP_CHAIN_EXACT_SCANS", 1) != 0) {
       low.sorted = lv.sorted ? 1 : 0;

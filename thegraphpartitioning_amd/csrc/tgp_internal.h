@@ -98,7 +98,21 @@ struct ChainLow {
   const i32* adj;  // the entries u < v of row v, in row order
   int sorted = 0;  // every row is strictly ascending (so are the lists above)
   int unitw = 0;   // every arc weight is 1 (the heaviest free neighbour is the first one)
+  // Best-entry lists (optional): row v with lidx[v] >= 0 has its lcnt[lidx[v]]
+  // best entries (weight, then first position; v itself, entries outside
+  // [0, n) and negative weights left out) at lists + 64 * lidx[v], best first,
+  // and every other entry of the row ranks below them.
+  const i32* lidx = nullptr;
+  const i32* lists = nullptr;
+  const i32* lcnt = nullptr;
 };
+constexpr int CHAIN_LIST_LEN = 64;
+// Best-entry lists of the rows longer than minlen: lidx (n ints) = -1 or the
+// row's list index, lrows[index] = the row, *nl (zeroed by the caller) = the
+// count; lists (CHAIN_LIST_LEN ints per list) and lcnt receive the lists.
+// At most m / minlen + 1 rows qualify.
+void launch_chain_lists(DGraph g, i64 minlen, i32* lidx, i32* lrows, i32* nl, i32* lists,
+                        i32* lcnt, hipStream_t s);
 void launch_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA, i32* blkC, i32* prop,
                         const i64* lo, u32* ticket, i64 budget, int blocks, int sg, int per,
                         u64* stats, hipStream_t s);

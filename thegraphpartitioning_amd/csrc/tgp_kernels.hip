@@ -825,6 +825,125 @@ __device__ __forceinline__ void wave_best_free_topk(const DGraph& g, i32 t, cons
   }
 }
 
+// Best-entry lists of the long rows (ChainLow): rows longer than minlen get a
+// list index.
+__global__ void k_list_rows(DGraph g, i64 minlen, i32* lidx, i32* lrows, i32* nl) {
+  for (i64 v = gtid(); v < g.n; v += gstride()) {
+    const bool lng = g.xadj[v + 1] - g.xadj[v] > minlen;
+    i32 idx = -1;
+    if (lng) {
+      idx = atomicAdd(nl, 1);
+      lrows[idx] = (i32)v;
+    }
+    lidx[v] = idx;
+  }
+}
+
+// One block per listed row.  Every thread keeps the LST_T best keys
+// (w << 32 | ~position) of its entries i = t, t + 256, ... (sorted, in LDS
+// afterwards); wave 0 then pops the largest kept key CHAIN_LIST_LEN times
+// (lane l merges the lists of threads 4l .. 4l + 3).  The popped keys are
+// the row's best keys exactly down to the largest key a thread dropped below
+// (a thread that saw more than LST_T keeps its LST_T-th best, and every key it
+// dropped is smaller), where the list stops.  The list holds the neighbour
+// ids, best first.
+constexpr int LST_T = 8;
+constexpr int LST_BS = 256;
+__global__ __launch_bounds__(LST_BS) void k_long_lists(DGraph g, const i32* lrows, const i32* nl,
+                                                       i32* lists, i32* lcnt) {
+  constexpr int NK = LST_BS * LST_T;
+  constexpr int PER_LANE = LST_BS / WAVE;  // kept lists merged by one lane of wave 0
+  __shared__ u64 sk[NK];
+  __shared__ u64 sbound[LST_BS / WAVE];
+  const int t = threadIdx.x, lane = lane_id(), wv = t >> 6;
+  const i32 nrows = *nl;
+  for (i32 li = blockIdx.x; li < nrows; li += gridDim.x) {
+    const i32 x = lrows[li];
+    const i64 s0 = g.xadj[x], s1 = g.xadj[x + 1];
+    u64 k[LST_T];
+#pragma unroll
+    for (int j = 0; j < LST_T; j++) k[j] = 0;
+    i64 seen = 0;
+    constexpr int LST_U = 8;  // entries per thread in flight
+    for (i64 base = s0 + t; base < s1; base += LST_BS * LST_U) {
+      i32 v[LST_U], w[LST_U];
+#pragma unroll
+      for (int u = 0; u < LST_U; u++) {
+        const i64 i = base + (i64)u * LST_BS;
+        v[u] = i < s1 ? g.adj[i] : -1;
+        w[u] = i < s1 ? g.w[i] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < LST_U; u++) {
+        if (v[u] < 0 || v[u] >= g.n || v[u] == x || w[u] < 0) continue;
+        seen++;
+        const i64 i = base + (i64)u * LST_BS;
+        const u64 c = ((u64)(u32)w[u] << 32) | (u64)(0xFFFFFFFFu - (u32)(i - s0));
+        if (c > k[LST_T - 1]) {
+          k[LST_T - 1] = c;
+#pragma unroll
+          for (int j = LST_T - 1; j > 0; j--) {
+            if (k[j] > k[j - 1]) {
+              const u64 tmp = k[j];
+              k[j] = k[j - 1];
+              k[j - 1] = tmp;
+            }
+          }
+        }
+      }
+    }
+    u64 bnd = seen > LST_T ? k[LST_T - 1] : 0;
+    bnd = wave_max_u64(bnd);
+    if (lane == 0) sbound[wv] = bnd;
+#pragma unroll
+    for (int j = 0; j < LST_T; j++) sk[t * LST_T + j] = k[j];
+    __syncthreads();
+    if (wv == 0) {
+      u64 B = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < LST_BS / WAVE; w2++) B = sbound[w2] > B ? sbound[w2] : B;
+      // lane's merge state: the next key of each of its PER_LANE kept lists
+      int p[PER_LANE];
+      u64 hd[PER_LANE];
+#pragma unroll
+      for (int q = 0; q < PER_LANE; q++) {
+        p[q] = 0;
+        hd[q] = sk[(lane * PER_LANE + q) * LST_T];
+      }
+      u64 out = 0;
+      int cnt = 0;
+      for (int r = 0; r < CHAIN_LIST_LEN; r++) {
+        u64 mine = 0;
+        int qb = 0;
+#pragma unroll
+        for (int q = 0; q < PER_LANE; q++) {
+          if (hd[q] > mine) {
+            mine = hd[q];
+            qb = q;
+          }
+        }
+        const u64 m = wave_max_u64(mine);
+        if (m == 0 || m < B) break;
+        if (lane == r) out = m;
+        cnt = r + 1;
+        if (mine == m) {  // keys are distinct: one lane holds m
+#pragma unroll
+          for (int q = 0; q < PER_LANE; q++) {
+            if (q == qb) {
+              p[q]++;
+              hd[q] = p[q] < LST_T ? sk[(lane * PER_LANE + q) * LST_T + p[q]] : 0;
+            }
+          }
+        }
+      }
+      lists[(i64)li * CHAIN_LIST_LEN + lane] =
+          lane < cnt ? g.adj[s0 + (i64)(0xFFFFFFFFu - (u32)out)] : -1;
+      if (lane == 0) lcnt[li] = cnt;
+    }
+    __syncthreads();
+  }
+}
+
 // Sub-group layout: CH_SG lanes per vertex, CH_VPW vertices per wave.  A row
 // of at most CH_ROW arcs is held in registers (CH_ROW / CH_SG entries per
 // lane), so a rescan of it is one batch of state loads and all sub-groups of
@@ -868,8 +987,9 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
   // per-vertex state, uniform within a sub-group
   bool own = false, tshort = false, ushort_ = false, uknown = false;
   // long rows: where the next A scan of t's row and C scan of u's row resume,
-  // and (unit-weight levels) the next proposal scan of t's row
-  i32 apos = 0, cpos = 0, ppos = 0;
+  // and (unit-weight levels) the next proposal scan of t's row; spos: the
+  // first entry of t's best-entry list not known to be taken
+  i32 apos = 0, cpos = 0, ppos = 0, spos = 0;
   i32 t = -1, a = BLK_NONE, u = -1, b = BLK_NONE;
   // long rows: lanes 0..pnv-1 of the sub-group hold the best free candidates
   // of the last full proposal scan (pci), ph = the entry now proposed (PK:
@@ -896,6 +1016,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       cpos = 0;
       apos = 0;
       ppos = 0;
+      spos = 0;
       pci = -1;
       ph = PK;
       pnv = 0;
@@ -1050,6 +1171,36 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       const i32 x = __shfl(t, l);
       c_pL++;
       c_arcsP += g.xadj[x + 1] - g.xadj[x];
+      // x's best-entry list: its first entry still free (the entries before
+      // spos are taken, and a taken vertex stays taken) is x's best free
+      // neighbour; only when none is left is the row scanned
+      const i32 li = low.lidx ? low.lidx[x] : -1;
+      if (li >= 0) {
+        const i32 from = __shfl(spos, l);
+        const i32 nl = low.lcnt[li];
+        const i32 e = (lane >= from && lane < nl) ? low.lists[(i64)li * CHAIN_LIST_LEN + lane] : -1;
+        const i32 sv = e >= 0 ? ld_st(st, e) : 0;
+        const u64 fm = __ballot(e >= 0 && sv < 0);
+        const bool mine = (lane & ~(CH_SG - 1)) == l;
+        if (fm) {
+          const int r = __ffsll((unsigned long long)fm) - 1;
+          const i32 r0 = __shfl(e, r), sr = __shfl(sv, r);
+          if (mine) {
+            spos = r;
+            u = r0;
+            s_u = sr;
+            b = BLK_UNKNOWN;
+            uknown = false;
+            cpos = 0;
+            pci = -1;
+            ph = PK;
+            pnv = 0;
+          }
+          need &= need - 1;
+          continue;
+        }
+        if (mine) spos = CHAIN_LIST_LEN;
+      }
       u64 kk;
       int nv;
       wave_best_free_topk<PK>(g, x, st, lane, l, kk, nv);
@@ -4262,6 +4413,13 @@ void launch_match_sym_init(i64 n, i32* mate, u8* dec, i32* blkA, i32* blkC, i32*
   k_match_sym_init<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(n, mate, dec, blkA, blkC, prop);
   k_fill<i64><<<1, 3, 0, s>>>(lo, 3, 0);
   k_fill<i64><<<1, 1, 0, s>>>(lo + 3, 1, n);  // first-undecided accumulator
+}
+
+void launch_chain_lists(DGraph g, i64 minlen, i32* lidx, i32* lrows, i32* nl, i32* lists,
+                        i32* lcnt, hipStream_t s) {
+  k_list_rows<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, minlen, lidx, lrows, nl);
+  k_long_lists<<<grid_for(g.m / (minlen > 0 ? minlen : 1) + 1, 1, 1u << 14), LST_BS, 0, s>>>(
+      g, lrows, nl, lists, lcnt);
 }
 
 void launch_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA, i32* blkC, i32* prop,
