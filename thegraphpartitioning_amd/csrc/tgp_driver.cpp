@@ -237,16 +237,23 @@ struct Prof {
     pending.push_back(Rec{kid, cur_ev, b});
     cur = -1;
   }
+  // takes the finished records (events complete in stream order, so the
+  // unfinished ones are the last few: they stay pending)
   void flush() {
     if (!on || pending.empty()) return;
-    for (const Rec& r : pending) {
+    size_t done = 0;
+    for (; done < pending.size(); done++) {
+      const Rec& r = pending[done];
+      const hipError_t q = hipEventQuery(r.b);
+      if (q == hipErrorNotReady) break;
+      HIP_TRY(q);
       float t = 0;
       HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
       ms[r.kid] += t;
       pool.push_back(r.a);
       pool.push_back(r.b);
     }
-    pending.clear();
+    pending.erase(pending.begin(), pending.begin() + (std::ptrdiff_t)done);
   }
   ~Prof() {
     for (auto& r : pending) {
@@ -342,14 +349,24 @@ struct Context {
       own_pinned = true;
     }
     prof.s = s;
+    HIP_TRY(hipEventCreateWithFlags(&mark_ev, hipEventDisableTiming));
   }
   ~Context() {
     if (pinned && own_pinned) (void)hipHostFree(pinned);
+    if (mark_ev) (void)hipEventDestroy(mark_ev);
     if (s) (void)hipStreamDestroy(s);
   }
   bool own_pinned = false;
   void sync() {
     HIP_TRY(hipStreamSynchronize(s));
+    prof.flush();
+  }
+  // mark(): the work enqueued so far; wait_mark(): wait for it only (work
+  // enqueued after the mark keeps running)
+  hipEvent_t mark_ev = nullptr;
+  void mark() { HIP_TRY(hipEventRecord(mark_ev, s)); }
+  void wait_mark() {
+    HIP_TRY(hipEventSynchronize(mark_ev));
     prof.flush();
   }
 };
@@ -1203,6 +1220,9 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     // the move sort is skipped when it cannot change the selection: all M
     // moves fit their destinations and the network would sort them fully
     const bool sort_skip = env_i64("TGP_SORT_SKIP", 1) != 0;
+    // TGP_SORT_AHEAD=0: the sort's first pass waits for the host's decision
+    const bool sort_ahead = env_i64("TGP_SORT_AHEAD", 1) != 0;
+    bool last_skipped = false;
     DBuf<i32> gscr(&C.pool, std::max<i64>(gain_gscratch_elems(k), 1));
     std::vector<i32> hpc((size_t)k);
     // a pass's count, cut, selection counters and part counts go to the host
@@ -1310,7 +1330,21 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
                                  st()));
           HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
         }
-        C.sync();
+        // The first tile pass of this pass's move sort (M read on the device)
+        // runs while the host waits for the status and does the bookkeeping;
+        // it writes only the sort's buffers, which nothing reads when the host
+        // then skips the sort or ends the phase.  Not after a pass whose sort
+        // was skipped (the next one likely skips too).
+        const bool ahead = run && sort_ahead && !last_skipped;
+        if (ahead) {
+          C.mark();
+          C.prof.begin(KID_SORT);
+          launch_sort_first_dev(mv_gain.p, small.p, n, sk_gain.p, sk_idx.p, st());
+          C.prof.end(KID_SORT, 0);
+          C.wait_mark();
+        } else {
+          C.sync();
+        }
         if (pack && pending) std::memcpy(hpc.data(), (const i32*)C.pinned + 6, k * sizeof(i32));
         if (pending) {
           // the previous pass's bookkeeping, gpu_part.cu:605-640 / 707-744
@@ -1362,9 +1396,11 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           launch_select_all(M, sk_idx.p, sel.p, ctr_cur, ctr_cur + 1, st());
           C.prof.end(KID_SELECT, 5.0 * M);
           sorts_skipped++;
+          last_skipped = true;
         } else {
+          last_skipped = false;
           C.prof.begin(KID_SORT);
-          launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st());
+          launch_sort_moves(mv_gain.p, M, len, sk_gain.p, sk_idx.p, st(), ahead);
           C.prof.end(KID_SORT, 16.0 * (double)len);
           C.prof.begin(KID_SELECT);
           launch_select(sk_idx.p, mv_dst.p, M, k, avail.p, sdst.p, H.p, sel.p, ctr_cur,

@@ -445,8 +445,14 @@ void launch_capacity(const i32* pc, const i32* outc, i32 k, i64 n, int phase, i6
 // every one of the M moves selected in buffer order: sk_idx[i] = i, sel[i] = 1,
 // *S = M, *pads = 0
 void launch_select_all(i64 M, i32* sk_idx, u8* sel, i32* S, i32* pads, hipStream_t s);
+// first_done: the first tile pass already ran (launch_sort_first_dev)
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* sk_gain, i32* sk_idx,
-                       hipStream_t s);
+                       hipStream_t s, bool first_done = false);
+// The sort's first tile pass with M read from the device (min(*Mp, n); the
+// network length min(2^ceil(log2 M), n) as the refinement pads it), for a
+// launch ahead of the host's decision; nothing when M < 2.
+void launch_sort_first_dev(const float* mv_gain, const i64* Mp, i64 n, float* sk_gain,
+                           i32* sk_idx, hipStream_t s);
 i64 select_scratch_elems(i64 M, i32 k);
 void launch_select(const i32* sk_idx, const i32* mv_dst, i64 M, i32 k, const i64* avail,
                    i32* sdst, i32* H, u8* sel, i32* S, i32* pads, hipStream_t s);

@@ -4003,6 +4003,29 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_bt_tile(float* key, i32* idx,
     bt_tile_body<TB, false, FIRST>(key, idx, len, kk, sk, si, src, M);
 }
 
+// k_bt_tile<TB, true> with M from the device: len = min(2^ceil(log2 M), n),
+// the tiles below M rounded up to the tile (all: every tile below len).
+template <int TB>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_bt_tile_first_dev(float* key, i32* idx,
+                                                                     const i64* Mp, i64 n,
+                                                                     const float* src, int all) {
+  __shared__ float sk[1 << TB];
+  __shared__ i32 si[1 << TB];
+  constexpr i64 TILE = (i64)1 << TB;
+  const i64 M = *Mp < n ? *Mp : n;
+  if (M < 2) return;
+  i64 p2 = 1;
+  while (p2 < M) p2 <<= 1;
+  const i64 len = p2 < n ? p2 : n;
+  const i64 up = all ? len : ((M + TILE - 1) / TILE * TILE < len ? (M + TILE - 1) / TILE * TILE
+                                                                 : len);
+  if ((i64)blockIdx.x >= (up + TILE - 1) / TILE) return;
+  if ((i64)(blockIdx.x + 1) << TB <= len)
+    bt_tile_body<TB, true, true>(key, idx, len, 0, sk, si, src, M);
+  else
+    bt_tile_body<TB, false, true>(key, idx, len, 0, sk, si, src, M);
+}
+
 // S consecutive strides j, j/2, ..., j >> (S-1) of one merge kk in a single
 // pass: every thread owns the 2^S elements whose positions differ only in
 // those stride bits (a set closed under the S steps) and applies the steps in
@@ -4788,8 +4811,22 @@ void launch_select_all(i64 M, i32* sk_idx, u8* sel, i32* S, i32* pads, hipStream
   k_select_all<<<grid_for(M, 256 * 4, 1u << 14), 256, 0, s>>>(M, sk_idx, sel, S, pads);
 }
 
+static bool sort_all_env() {
+  const char* all_env = std::getenv("TGP_SORT_ALL");
+  return all_env && std::atoi(all_env) != 0;
+}
+
+void launch_sort_first_dev(const float* mv_gain, const i64* Mp, i64 n, float* key, i32* idx,
+                           hipStream_t s) {
+  constexpr i64 TILE = (i64)1 << BT_TB;
+  constexpr int NT = 1 << (BT_TB - 4);
+  if (n < 2) return;
+  k_bt_tile_first_dev<BT_TB><<<(unsigned)((n + TILE - 1) / TILE), NT, 0, s>>>(
+      key, idx, Mp, n, mv_gain, sort_all_env() ? 1 : 0);
+}
+
 void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* idx,
-                       hipStream_t s) {
+                       hipStream_t s, bool first_done) {
   constexpr i64 TILE = (i64)1 << BT_TB;
   if (len < 2) {
     k_sort_init<<<1, 64, 0, s>>>(mv_gain, M, len, key, idx);
@@ -4805,11 +4842,10 @@ void launch_sort_moves(const float* mv_gain, i64 M, i64 len, float* key, i32* id
   // writes the pads of its tiles; the tiles past them take the pads by a fill.
   const char* smax_env = std::getenv("TGP_SORT_SMAX");  // strides per global pass (1-5)
   const int smax = smax_env ? std::max(1, std::min(BT_SMAX, std::atoi(smax_env))) : BT_SMAX;
-  const char* all_env = std::getenv("TGP_SORT_ALL");
-  const bool all = all_env && std::atoi(all_env) != 0;
+  const bool all = sort_all_env();
   auto upto = [&](i64 kk) { return all ? len : std::min(len, (M + kk - 1) / kk * kk); };
   const unsigned t0 = (unsigned)((upto(TILE) + TILE - 1) / TILE);
-  k_bt_tile<BT_TB, true><<<t0, NT, 0, s>>>(key, idx, len, 0, mv_gain, M);
+  if (!first_done) k_bt_tile<BT_TB, true><<<t0, NT, 0, s>>>(key, idx, len, 0, mv_gain, M);
   const i64 pad0 = (i64)t0 * TILE;
   if (pad0 < len) {
     const unsigned gf = grid_for(len - pad0, 256, 1u << 16);
