@@ -340,17 +340,22 @@ def test_long_row_contraction_forms(oracle, monkeypatch, env):
     assert_same(gp, op, f"rmat15 {env}")
 
 
-@pytest.mark.parametrize("lmin", ["off", "128", "300", "4096"])
+@pytest.mark.parametrize("lmin", ["off", "128", "300", "4096", "64-sg8"])
 def test_chain_best_entry_lists(oracle, monkeypatch, lmin):
     """The chained matching's best-entry lists of long rows on weighted levels
     (the 64 best entries by weight, then position; a proposal is the first
     still-free entry, and the row is scanned once the list is used up): with
     lists on rows longer than 128 (every long row of the chain), 300 or 4096
     arcs (the default), and with none, traces and partitions equal the
-    oracle's.  The twin hubs' rows carry weights 1..5 (ties broken by
-    position) and lose their list entries to the leaves' matches."""
+    oracle's; also with 8 lanes per vertex (the choice on levels of 2^22
+    vertices and more), whose long rows start past 64 arcs.  The twin hubs'
+    rows carry weights 1..5 (ties broken by position) and lose their list
+    entries to the leaves' matches."""
     if lmin == "off":
         monkeypatch.setenv("TGP_CHAIN_LISTS", "0")
+    elif lmin == "64-sg8":
+        monkeypatch.setenv("TGP_CHAIN_SG", "8")
+        monkeypatch.setenv("TGP_CHAIN_LIST_MIN", "64")
     else:
         monkeypatch.setenv("TGP_CHAIN_LIST_MIN", lmin)
     xadj, adj, w = twin_hub_graph(5000, 12000, 5)

@@ -518,9 +518,12 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
   // Lanes per vertex: 8 when every row fits 8 lanes' registers (8 vertices
   // per wave, so 4x as many vertices in flight: on a 3-D grid the decisions
   // run as wavefronts through consecutive ids and the number in flight sets
-  // the pace, 1.54 -> 0.50 s on the 256^3 grid), else 16 (rows longer than
-  // 128 arcs are scanned by the whole wave; 16 against 32 lanes: R-MAT 24
-  // ef16 k=64 1.366 -> 1.312 s, R-MAT 22 0.431 -> 0.420 s).
+  // the pace, 1.54 -> 0.50 s on the 256^3 grid) or the level has at least
+  // 2^22 vertices (with the long rows' best-entry lists, R-MAT 24 ef16 k=64
+  // 0.796 -> 0.781 s, weighted 0.802 -> 0.791 s; R-MAT 22 even, R-MAT 20
+  // 1.5 % slower), else 16 (rows longer than 128 arcs are scanned by the
+  // whole wave; 16 against 32 lanes: R-MAT 24 ef16 k=64 1.366 -> 1.312 s,
+  // R-MAT 22 0.431 -> 0.420 s).
   int CHAIN_SG = (int)env_i64("TGP_CHAIN_SG", 0);
   if (CHAIN_SG <= 0 && lv.symmetric && MATCH_CHAIN) {
     DBuf<u64> mr(&C.pool, 1);
@@ -528,7 +531,7 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
     launch_max_row(lv.dg(), mr.p, st());
     HIP_TRY(hipMemcpyAsync(C.pinned, mr.p, sizeof(u64), hipMemcpyDeviceToHost, st()));
     C.sync();
-    CHAIN_SG = (i64)C.pinned[0] <= (i64)8 * CHAIN_PER ? 8 : 16;
+    CHAIN_SG = ((i64)C.pinned[0] <= (i64)8 * CHAIN_PER || n >= ((i64)1 << 22)) ? 8 : 16;
   }
   if (CHAIN_SG <= 0) CHAIN_SG = 32;
   const i64 chain_budget =
