@@ -37,16 +37,29 @@ __device__ __forceinline__ void tally_put(const TallyRef& T, i64 v, i32 k, i32 q
   if (b >= 0) T.t32[(i64)b * tally_stride(k) + q] = val;
   else T.t8[v * tally8_stride_d(k) + q] = (u8)val;
 }
-// T[u][q] += d; a byte row's add goes to the 32-bit word holding the byte:
-// the byte stays in [0, 255], so nothing carries into or borrows from its
-// neighbours ((u32)d << sh is d * 2^sh modulo 2^32 for either sign)
-__device__ __forceinline__ void tally_add(const TallyRef& T, i64 u, i32 k, i32 q, i32 d) {
-  const i64 b = tally_int_row(T, u);
-  if (b >= 0) {
-    atomicAdd(&T.t32[(i64)b * tally_stride(k) + q], d);
+// A move of weight w from part src to part dst seen from neighbour u:
+// T[u][src] -= w, T[u][dst] += w.  A byte row's adds go to the 32-bit words
+// holding the bytes: every byte stays in [0, 255], so nothing carries into or
+// borrows from its neighbours ((u32)d << sh is d * 2^sh modulo 2^32 for
+// either sign), and two bytes of one word take one add of both terms.  The
+// row class comes from the bigmask bit (an L2-resident bit array) before any
+// int-row index is read.
+__device__ __forceinline__ void tally_move(const TallyRef& T, i64 u, i32 k, i32 src, i32 dst,
+                                          i32 w) {
+  const bool big = T.all_int || ((T.bigmask[u >> 5] >> (u & 31)) & 1u);
+  if (big) {
+    i32* row = T.t32 + (T.all_int ? u : (i64)T.bix[u]) * tally_stride(k);
+    atomicAdd(&row[src], -w);
+    atomicAdd(&row[dst], w);
   } else {
-    const i64 byte = u * tally8_stride_d(k) + q;
-    atomicAdd((u32*)(T.t8 + (byte & ~(i64)3)), (u32)d << (8 * (u32)(byte & 3)));
+    u8* row = T.t8 + u * tally8_stride_d(k);
+    const u32 ds = (u32)(-w) << (8 * (u32)(src & 3)), dd = (u32)w << (8 * (u32)(dst & 3));
+    if ((src >> 2) == (dst >> 2)) {
+      atomicAdd((u32*)(row + (src & ~3)), ds + dd);
+    } else {
+      atomicAdd((u32*)(row + (src & ~3)), ds);
+      atomicAdd((u32*)(row + (dst & ~3)), dd);
+    }
   }
 }
 __device__ __forceinline__ u64 lanes_below(int lane) { return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane)); }
@@ -3237,8 +3250,7 @@ __global__ __launch_bounds__(256) void k_tally_update(DGraph g, TallyRef T, i32 
       const i32 u = g.adj[i];
       if (u < 0 || u >= g.n || (skip_last && u == g.n - 1)) continue;
       const i32 w = unit ? 1 : g.w[i];
-      tally_add(T, u, k, src, -w);
-      tally_add(T, u, k, dst, w);
+      tally_move(T, u, k, src, dst, w);
     }
   }
 }
