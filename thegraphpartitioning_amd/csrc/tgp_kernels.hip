@@ -981,14 +981,14 @@ __device__ __forceinline__ bool sg_any(bool f) {
 
 // The chained matching of one wave: groups from `ticket` starting at vertex
 // lo, until the groups run out or the wall-clock budget is spent.
-template <int CH_SG, int CH_PER>
+template <int CH_SG, int CH_PER, bool STATS = false>
 __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low, i32* st,
                                            i32* blkA, i32* blkC, i32* prop, const i64 lo,
                                            u32* ticket, i64 budget, u64* stats) {
   constexpr int CH_VPW = WAVE / CH_SG;
   constexpr int CH_ROW = CH_SG * CH_PER;
   constexpr u64 SG_MASK = CH_SG == 64 ? ~0ull : (1ull << (CH_SG % 64)) - 1;
-  // stats (optional): per-kind event counts, summed over the grid
+  // stats (STATS): per-kind event counts, summed over the grid
   u64 c_iter = 0, c_sleep = 0, c_apS = 0, c_aL = 0, c_pL = 0, c_fetch = 0, c_cS = 0, c_cL = 0,
       c_commit = 0, c_arcsL = 0, c_arcsP = 0, c_arcsC = 0;
   const i64 n = g.n;
@@ -1058,7 +1058,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       continue;
     }
     if (wall_clock64() - t_start > budget) break;
-    c_iter++;
+    if constexpr (STATS) c_iter++;
 
     if (own && a >= 0 && s_a != ST_UNDEC) {
       if (s_a == t) own = false;  // a took t
@@ -1068,7 +1068,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
     // A and proposal over a row held in registers: one batch of loads
     const bool needAP = own && (a == BLK_UNKNOWN || u == BLK_UNKNOWN);
     if (__ballot(needAP && tshort)) {
-      c_apS++;
+      if constexpr (STATS) c_apS++;
       i32 sv[CH_PER];
 #pragma unroll
       for (int j = 0; j < CH_PER; j++)
@@ -1129,8 +1129,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       // every entry below x is on x's lower-neighbour list when it has one
       const i32* la = low.off ? low.adj : g.adj;
       const i64 l0 = low.off ? low.off[x] : g.xadj[x], l1 = low.off ? low.off[x + 1] : g.xadj[x + 1];
-      c_aL++;
-      c_arcsL += l1 - l0 - from;
+      if constexpr (STATS) c_aL++;
+      if constexpr (STATS) c_arcsL += l1 - l0 - from;
       const i32 r = wave_scan_pending(la, l0, l1, x, x, x, st, lane, from, low.sorted);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) own = false;
@@ -1165,10 +1165,10 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
       i32 from = __shfl(ppos, l), sr = 0;
-      c_pL++;
-      c_arcsP -= from;
+      if constexpr (STATS) c_pL++;
+      if constexpr (STATS) c_arcsP -= from;
       const i32 r0 = wave_first_free(g, x, st, lane, from, sr);
-      c_arcsP += from;
+      if constexpr (STATS) c_arcsP += from;
       if ((lane & ~(CH_SG - 1)) == l) {
         ppos = from;
         u = r0 >= 0 ? r0 : PROP_NONE;
@@ -1182,8 +1182,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
     while (need) {
       const int l = __ffsll((unsigned long long)need) - 1;
       const i32 x = __shfl(t, l);
-      c_pL++;
-      c_arcsP += g.xadj[x + 1] - g.xadj[x];
+      if constexpr (STATS) c_pL++;
+      if constexpr (STATS) c_arcsP += g.xadj[x + 1] - g.xadj[x];
       // x's best-entry list: its first entry still free (the entries before
       // spos are taken, and a taken vertex stays taken) is x's best free
       // neighbour; only when none is left is the row scanned
@@ -1247,7 +1247,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
     const bool needC = okB && b == BLK_UNKNOWN;
     const bool fetch = needC && !uknown;
     if (__ballot(fetch)) {  // the proposal's row, once per proposal
-      c_fetch++;
+      if constexpr (STATS) c_fetch++;
       i64 us0 = 0, us1 = 0;
       if (fetch) {
         us0 = g.xadj[u];
@@ -1262,7 +1262,7 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       }
     }
     if (__ballot(needC && ushort_)) {
-      c_cS++;
+      if constexpr (STATS) c_cS++;
       i32 sv[CH_PER];
 #pragma unroll
       for (int j = 0; j < CH_PER; j++)
@@ -1296,8 +1296,8 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       const bool uselow = low.off && lim <= x;
       const i32* la = uselow ? low.adj : g.adj;
       const i64 l0 = uselow ? low.off[x] : g.xadj[x], l1 = uselow ? low.off[x + 1] : g.xadj[x + 1];
-      c_cL++;
-      c_arcsC += l1 - l0 - from;
+      if constexpr (STATS) c_cL++;
+      if constexpr (STATS) c_arcsC += l1 - l0 - from;
       const i32 r = wave_scan_pending(la, l0, l1, x, lim, x, st, lane, from, low.sorted);
       if ((lane & ~(CH_SG - 1)) == l) {
         if (r == BLK_TAKEN) {
@@ -1320,9 +1320,9 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
       }
     }
     if (ready) own = false;
-    c_commit += __popcll(__ballot(ready && sl == 0));
+    if constexpr (STATS) c_commit += __popcll(__ballot(ready && sl == 0));
     if (!__ballot(ready)) {
-      c_sleep++;
+      if constexpr (STATS) c_sleep++;
       __builtin_amdgcn_s_sleep(2);
     }
   }
@@ -1331,18 +1331,23 @@ __device__ __forceinline__ void chain_wave(const DGraph& g, const ChainLow& low,
     prop[t] = u;
     blkC[t] = b;
   }
-  if (stats && lane == 0) {
-    const u64 c[12] = {c_iter, c_sleep, c_apS, c_aL, c_pL, c_fetch, c_cS, c_cL, c_commit,
-                       c_arcsL, c_arcsP, c_arcsC};
-    for (int q = 0; q < 12; q++) atomicAdd((unsigned long long*)&stats[q], (unsigned long long)c[q]);
+  if constexpr (STATS) {
+    if (stats && lane == 0) {
+      const u64 c[12] = {c_iter, c_sleep, c_apS, c_aL, c_pL, c_fetch, c_cS, c_cL, c_commit,
+                         c_arcsL, c_arcsP, c_arcsC};
+      for (int q = 0; q < 12; q++)
+        atomicAdd((unsigned long long*)&stats[q], (unsigned long long)c[q]);
+    }
   }
 }
 
-template <int CH_SG, int CH_PER>
+// STATS: the event counts of the chain (kept in registers only when asked)
+template <int CH_SG, int CH_PER, bool STATS>
 __global__ __launch_bounds__(256) void k_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA,
                                                      i32* blkC, i32* prop, const i64* lo_io,
                                                      u32* ticket, i64 budget, u64* stats) {
-  chain_wave<CH_SG, CH_PER>(g, low, st, blkA, blkC, prop, lo_io[0], ticket, budget, stats);
+  chain_wave<CH_SG, CH_PER, STATS>(g, low, st, blkA, blkC, prop, lo_io[0], ticket, budget,
+                                   stats);
 }
 
 // Lower-neighbour lists of the rows longer than `minlen` (ChainLow).
@@ -4474,19 +4479,23 @@ void launch_match_chain(DGraph g, ChainLow low, i32* st, i32* blkA, i32* blkC, i
     }
     return std::max(1, per_cu * cus);
   };
-#define TGP_CHAIN_CASE(SG, PER)                                                               \
-  if (sg == SG && per == PER) {                                                               \
-    const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<SG, PER>);       \
-    k_match_chain<SG, PER><<<nb, 256, 0, s>>>(g, low, st, blkA, blkC, prop, lo, ticket,       \
-                                              budget, stats);                                 \
+  // the event counts (stats) are kept by the two default shapes' counting
+  // forms only
+#define TGP_CHAIN_CASE(SG, PER, ST)                                                           \
+  if (sg == SG && per == PER && (ST == (stats != nullptr) || !ST)) {                          \
+    const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<SG, PER, ST>);   \
+    k_match_chain<SG, PER, ST><<<nb, 256, 0, s>>>(g, low, st, blkA, blkC, prop, lo, ticket,   \
+                                                  budget, stats);                             \
     return;                                                                                   \
   }
-  TGP_CHAIN_CASE(16, 8)
-  TGP_CHAIN_CASE(32, 8)
+  TGP_CHAIN_CASE(16, 8, true)
+  TGP_CHAIN_CASE(8, 8, true)
+  TGP_CHAIN_CASE(16, 8, false)
+  TGP_CHAIN_CASE(32, 8, false)
 #undef TGP_CHAIN_CASE
-  const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<8, 8>);
-  k_match_chain<8, 8><<<nb, 256, 0, s>>>(g, low, st, blkA, blkC, prop, lo, ticket, budget,
-                                         stats);
+  const int nb = blocks > 0 ? blocks : resident((const void*)k_match_chain<8, 8, false>);
+  k_match_chain<8, 8, false><<<nb, 256, 0, s>>>(g, low, st, blkA, blkC, prop, lo, ticket, budget,
+                                                stats);
 }
 
 void launch_chain_low_count(DGraph g, i64 minlen, i32* list, i32* nlist, i64* cnt,
