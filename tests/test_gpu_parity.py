@@ -367,3 +367,20 @@ def test_chain_best_entry_lists(oracle, monkeypatch, lmin):
     g = tgp.gen_rmat(16, 16, 0x7C0DE5EED + 16)
     gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, 64, repair=True)
     assert_same(gp, op, f"rmat16 lists {lmin}")
+
+
+@pytest.mark.parametrize("k,shorts", [(100, "1"), (100, "0"), (129, "1"), (256, "1")])
+def test_short_tally_rows(oracle, monkeypatch, k, shorts):
+    """Kept tallies for 64 < k <= 256 as 16-bit rows (rows whose arcs weigh
+    >= 0 and sum to at most 65535; the rest, hubs and the last row, as int
+    rows read from their list), with row padding (k = 100, 129) and without
+    (k = 256), against the oracle; TGP_TALLY_SHORTS=0 keeps ints.  The
+    weighted graph's rows sum past 255 but below 65535, and the unit R-MAT's
+    hubs pass 65535 only on coarse levels where they absorb their mates."""
+    monkeypatch.setenv("TGP_TALLY_SHORTS", shorts)
+    g = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 40 + k, "hash16")
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, k, repair=True, floor=100)
+    assert_same(gp, op, f"weighted rmat14 k={k} shorts={shorts}")
+    g = tgp.gen_rmat(15, 16, 0x7C0DE5EED + 41 + k)
+    gp, op = run_both(oracle, g.xadj, g.adjncy, g.adjwgt, k, repair=True, floor=100)
+    assert_same(gp, op, f"rmat15 k={k} shorts={shorts}")

@@ -1160,17 +1160,22 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     DBuf<i32> T32, bix, bigv, app, napp;
     DBuf<u32> bigmask;
     DBuf<u8> T8;
+    DBuf<u16> T16;
     TallyRef TR;
     i64 nbig = n;
     bool all_int = true;
-    // byte rows pay off with one lane per vertex (k <= 64, gain lanes 1); the
+    // byte rows pay off with one lane per vertex (k <= 64, gain lanes 1),
+    // short rows with 4 lanes per vertex for 64 < k <= TALLY16_MAXK; the
     // kernels for other widths read int rows only
-    if (tmode && env_i64("TGP_TALLY_BYTES", 1) != 0 && k <= 64 && GAIN_LANES == 1) {
+    const bool bytes = env_i64("TGP_TALLY_BYTES", 1) != 0 && k <= 64 && GAIN_LANES == 1;
+    const bool shorts =
+        env_i64("TGP_TALLY_SHORTS", 1) != 0 && k > 64 && k <= TALLY16_MAXK && GAIN_LANES == 1;
+    if (tmode && (bytes || shorts)) {
       bix = DBuf<i32>(&C.pool, n);
       bigmask = DBuf<u32>(&C.pool, (n + 31) / 32 + 2);
       DBuf<i32> nb(&C.pool, 1);
       HIP_TRY(hipMemsetAsync(nb.p, 0, sizeof(i32), st()));
-      launch_tally_class(g, nedges, unit, 0, bix.p, bigmask.p, nb.p, st());
+      launch_tally_class(g, nedges, unit, 0, shorts ? 65535 : 255, bix.p, bigmask.p, nb.p, st());
       HIP_TRY(hipMemcpyAsync(C.pinned, nb.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
       C.sync();
       nbig = ((const i32*)C.pinned)[0];
@@ -1184,16 +1189,22 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
     // bytes one pass reads from the kept tallies (and the row classes)
     const double tally_bytes =
         all_int ? (double)n * tally_row_stride(k) * 4
-                : (double)n * tally8_stride(k) + (double)nbig * tally_row_stride(k) * 4 + n / 8.0;
+                : (double)n * (shorts ? 2 * tally16_stride(k) : tally8_stride(k)) +
+                      (double)nbig * tally_row_stride(k) * 4 + n / 8.0;
     if (tmode) tmode = tally_bytes <= 0.4 * (double)free_b;
     bool T_valid = false;
     const bool last_full = nedges == lv.m;
     const bool last_partial = !last_full && nedges > ax.last_begin;
     if (tmode) {
       // row padding (entries past k) stays zero: the updates touch parts < k
-      if (!all_int) {
+      if (!all_int && shorts) {
+        T16 = DBuf<u16>(&C.pool, n * tally16_stride(k));
+        if (k % 8) HIP_TRY(hipMemsetAsync(T16.p, 0, (size_t)n * tally16_stride(k) * 2, st()));
+      } else if (!all_int) {
         T8 = DBuf<u8>(&C.pool, n * tally8_stride(k));
         if (k % 16) HIP_TRY(hipMemsetAsync(T8.p, 0, (size_t)n * tally8_stride(k), st()));
+      }
+      if (!all_int) {
         bigv = DBuf<i32>(&C.pool, std::max<i64>(nbig, 1));
         launch_big_list(bix.p, n, bigv.p, st());
       }
@@ -1201,7 +1212,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
       if (k % 4)
         HIP_TRY(hipMemsetAsync(T32.p, 0, (size_t)std::max<i64>(nbig, 1) * tally_row_stride(k) * 4,
                                st()));
-      TR = TallyRef{T8.p, T32.p, bix.p, bigmask.p, all_int ? 1 : 0, bigv.p, (i32)nbig};
+      TR = TallyRef{T8.p, T32.p, bix.p, bigmask.p, all_int ? 1 : 0, bigv.p, (i32)nbig, T16.p};
       app = DBuf<i32>(&C.pool, n);
       napp = DBuf<i32>(&C.pool, 1);
     }

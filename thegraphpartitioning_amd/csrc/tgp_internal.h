@@ -20,6 +20,7 @@ typedef int32_t i32;
 typedef uint32_t u32;
 typedef uint64_t u64;
 typedef uint8_t u8;
+typedef uint16_t u16;
 
 // A CSR level resident in HBM: 64-bit row offsets, 32-bit ids and weights.
 struct DGraph {
@@ -366,7 +367,11 @@ constexpr i32 PART8_MAXK = 255;
 // bit v set for the int rows of a mixed level (one bit per vertex, so the
 // class test of a row is an L2 hit while its byte row is already in flight).
 // bigv[b] (b < nbig) is the vertex of int row b on a mixed level.  Entries of
-// a row past k (its padding) hold zero.
+// a row past k (its padding) hold zero.  With t16 set (levels with k > 64)
+// the rows that are not int rows are 16-bit rows at t16 + v * tally16_stride(k)
+// (k rounded up to 8) instead of byte rows: arcs weigh >= 0 and sum to at
+// most 65535, and the updates are exact 32-bit adds on the word holding the
+// half.
 struct TallyRef {
   u8* t8 = nullptr;
   i32* t32 = nullptr;
@@ -375,14 +380,19 @@ struct TallyRef {
   int all_int = 0;
   const i32* bigv = nullptr;
   i32 nbig = 0;
+  u16* t16 = nullptr;
 };
 inline i64 tally_row_stride(i32 k) { return (i64)((k + 3) & ~3); }
 inline i64 tally8_stride(i32 k) { return (i64)((k + 15) & ~15); }
-// bix[v] = -1 for a byte row, else its int row (wave-aggregated from *nbig,
-// zeroed by the caller), bigmask[(n + 31) / 32] the int rows; all_int puts
-// every row in the int table.
-void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i32* bix, u32* bigmask,
-                        i32* nbig, hipStream_t s);
+inline i64 tally16_stride(i32 k) { return (i64)((k + 7) & ~7); }
+// short rows serve levels with 64 < k <= TALLY16_MAXK
+constexpr i32 TALLY16_MAXK = 256;
+// bix[v] = -1 for a byte or short row (arcs >= 0 summing to at most limit:
+// 255 or 65535), else its int row (wave-aggregated from *nbig, zeroed by the
+// caller), bigmask[(n + 31) / 32] the int rows; all_int puts every row in the
+// int table.
+void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i64 limit, i32* bix,
+                        u32* bigmask, i32* nbig, hipStream_t s);
 // bigv[bix[v]] = v for the int rows of a mixed level
 void launch_big_list(const i32* bix, i64 n, i32* bigv, hipStream_t s);
 // Tout (optional): also writes each vertex's integer tally.

@@ -28,6 +28,8 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)
 __host__ __device__ __forceinline__ i64 tally_stride(i32 k) { return (i64)((k + 3) & ~3); }
 // byte rows of the kept tallies: k rounded up to 16
 __host__ __device__ __forceinline__ i64 tally8_stride_d(i32 k) { return (i64)((k + 15) & ~15); }
+// short (16-bit) rows of the kept tallies: k rounded up to 8
+__host__ __device__ __forceinline__ i64 tally16_stride_d(i32 k) { return (i64)((k + 7) & ~7); }
 // int row of v, or -1 for a byte row
 __device__ __forceinline__ i64 tally_int_row(const TallyRef& T, i64 v) {
   return T.all_int ? v : (i64)T.bix[v];
@@ -35,6 +37,7 @@ __device__ __forceinline__ i64 tally_int_row(const TallyRef& T, i64 v) {
 __device__ __forceinline__ void tally_put(const TallyRef& T, i64 v, i32 k, i32 q, i32 val) {
   const i64 b = tally_int_row(T, v);
   if (b >= 0) T.t32[(i64)b * tally_stride(k) + q] = val;
+  else if (T.t16) T.t16[v * tally16_stride_d(k) + q] = (u16)val;
   else T.t8[v * tally8_stride_d(k) + q] = (u8)val;
 }
 // A move of weight w from part src to part dst seen from neighbour u:
@@ -51,6 +54,15 @@ __device__ __forceinline__ void tally_move(const TallyRef& T, i64 u, i32 k, i32 
     i32* row = T.t32 + (T.all_int ? u : (i64)T.bix[u]) * tally_stride(k);
     atomicAdd(&row[src], -w);
     atomicAdd(&row[dst], w);
+  } else if (T.t16) {  // short row: the same on 16-bit halves of a word
+    u16* row = T.t16 + u * tally16_stride_d(k);
+    const u32 ds = (u32)(-w) << (16 * (u32)(src & 1)), dd = (u32)w << (16 * (u32)(dst & 1));
+    if ((src >> 1) == (dst >> 1)) {
+      atomicAdd((u32*)(row + (src & ~1)), ds + dd);
+    } else {
+      atomicAdd((u32*)(row + (src & ~1)), ds);
+      atomicAdd((u32*)(row + (dst & ~1)), dd);
+    }
   } else {
     u8* row = T.t8 + u * tally8_stride_d(k);
     const u32 ds = (u32)(-w) << (8 * (u32)(src & 3)), dd = (u32)w << (8 * (u32)(dst & 3));
@@ -3186,6 +3198,118 @@ __global__ __launch_bounds__(256) void k_gain_tallies1(i64 n, const i32* part, T
   if (cut_acc) block_add_u64(cutv, cut_acc);
 }
 
+// 64 < k <= TALLY16_MAXK on a level with short rows: G lanes per vertex, each
+// reading every G-th 16-byte chunk (8 tallies) of the row with all its loads
+// in flight; the pick is gt1's (NaN-padded balance rows in LDS, strict first
+// maximum within the lane), then the lanes' picks combine by (gain, lower q).
+// The blocks past nb_short take the int rows from their list, one lane each.
+template <int G>
+__global__ __launch_bounds__(256) void k_gain_tallies16(i64 n, const i32* part, TallyRef T,
+                                                        i32 k, const float* bg_g,
+                                                        const u8* src_big, float* gain,
+                                                        i32* target, float* cand, int phase,
+                                                        u64* cut_acc, unsigned nb_short) {
+  constexpr int KM = TALLY16_MAXK;
+  __shared__ __align__(16) float sbgt[2 * KM];
+  for (int i = threadIdx.x; i < 2 * KM; i += blockDim.x) {
+    const int q = i & (KM - 1);
+    sbgt[i] = q < k ? bg_g[(i >= KM ? k : 0) + q] : CAND_NO;
+  }
+  __syncthreads();
+  u64 cutv = 0;
+  if (blockIdx.x < nb_short) {
+    const i64 kp = tally16_stride_d(k);
+    const int nch = (int)(kp >> 3);  // 16-byte chunks of a row
+    constexpr int MAXC = KM / 8 / G;  // chunks per lane at most
+    const int gl = threadIdx.x & (G - 1);
+    const i64 ngroups = (i64)nb_short * (blockDim.x / G);
+    for (i64 v = (i64)blockIdx.x * (blockDim.x / G) + threadIdx.x / G; v < n; v += ngroups) {
+      if ((T.bigmask[v >> 5] >> (v & 31)) & 1u) continue;
+      const i32 cur = part[v];
+      if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
+      const float4* B4 = (const float4*)(sbgt + (src_big[cur] ? KM : 0));
+      const int4* R = (const int4*)(T.t16 + v * kp);
+      int4 c4[MAXC];
+#pragma unroll
+      for (int j = 0; j < MAXC; j++) {
+        const int c = gl + j * G;
+        c4[j] = c < nch ? R[c] : make_int4(0, 0, 0, 0);
+      }
+      const u32 own = T.t16[v * kp + cur];
+      const float ccur = (float)own;
+      float bgv = -INFINITY;
+      i32 bq = INT_MAX;
+      u32 tot = 0;
+#pragma unroll
+      for (int j = 0; j < MAXC; j++) {
+        const int c = gl + j * G;
+        if (c >= nch) break;
+        const u32 w4[4] = {(u32)c4[j].x, (u32)c4[j].y, (u32)c4[j].z, (u32)c4[j].w};
+        const float4 b0 = B4[2 * c], b1 = B4[2 * c + 1];
+        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int h = 0; h < 8; h++) {
+          const u32 t = (w4[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
+          tot += t;
+          gt1_pick(8 * c + h, (float)t, cur, ccur, bb[h], bgv, bq);
+        }
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) {
+        const float og = __shfl_xor(bgv, o, G);
+        const i32 oq = __shfl_xor(bq, o, G);
+        if (og > bgv || (og == bgv && oq < bq)) {
+          bgv = og;
+          bq = oq;
+        }
+        tot += __shfl_xor(tot, o, G);
+      }
+      if (gl == 0) {
+        cutv += (u64)(tot - own);
+        gt1_store(v, cur, bgv, bq, gain, target, cand, phase);
+      }
+    }
+  } else {
+    const i64 ts = tally_stride(k);
+    const int nv4 = (int)(ts >> 2);
+    const int nc = (k + 15) >> 4;
+    const i64 stride = (i64)(gridDim.x - nb_short) * blockDim.x;
+    for (i64 r = (i64)(blockIdx.x - nb_short) * blockDim.x + threadIdx.x; r < T.nbig;
+         r += stride) {
+      const i64 v = T.bigv[r];
+      const i32 cur = part[v];
+      if (cur >= k || cur < 0) continue;  // gpu_part.cu:262
+      const float4* B4 = (const float4*)(sbgt + (src_big[cur] ? KM : 0));
+      const int4* R4 = (const int4*)(T.t32 + r * ts);  // bix[v] == r
+      const i32 own = T.t32[r * ts + cur];
+      const float ccur = (float)own;
+      float bgv = -INFINITY;
+      i32 bq = INT_MAX;
+      i64 tot = 0;
+#pragma unroll 1
+      for (int c = 0; c < nc; c++) {
+        int4 c4[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          c4[i] = 4 * c + i < nv4 ? R4[4 * c + i] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int q0 = 16 * c + 4 * i;
+          const float4 b = B4[q0 >> 2];
+          tot += (i64)c4[i].x + c4[i].y + c4[i].z + c4[i].w;
+          gt1_pick(q0, (float)c4[i].x, cur, ccur, b.x, bgv, bq);
+          gt1_pick(q0 + 1, (float)c4[i].y, cur, ccur, b.y, bgv, bq);
+          gt1_pick(q0 + 2, (float)c4[i].z, cur, ccur, b.z, bgv, bq);
+          gt1_pick(q0 + 3, (float)c4[i].w, cur, ccur, b.w, bgv, bq);
+        }
+      }
+      cutv += (u64)(tot - own);
+      gt1_store(v, cur, bgv, bq, gain, target, cand, phase);
+    }
+  }
+  if (cut_acc) block_add_u64(cutv, cut_acc);
+}
+
 __global__ void k_big_list(const i32* bix, i64 n, i32* bigv) {
   for (i64 v = gtid(); v < n; v += gstride()) {
     const i32 b = bix[v];
@@ -3218,6 +3342,18 @@ __global__ __launch_bounds__(256) void k_tally_cut(i64 n, const i32* part, Tally
         for (int j = 0; j < 4; j++) {
           const i32 q = 4 * c + j;
           if (q < k && q != cur) other += tv[j];
+        }
+      }
+    } else if (T.t16) {
+      const int4* S4 = (const int4*)(T.t16 + v * tally16_stride_d(k));
+      const int nc8 = (int)(tally16_stride_d(k) >> 3);
+      for (int c = gl; c < nc8; c += 4) {
+        const int4 t = S4[c];
+        const u32 w4[4] = {(u32)t.x, (u32)t.y, (u32)t.z, (u32)t.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const i32 q = 8 * c + j;
+          if (q < k && q != cur) other += (w4[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
         }
       }
     } else {
@@ -3260,25 +3396,26 @@ __global__ __launch_bounds__(256) void k_tally_update(DGraph g, TallyRef T, i32 
   }
 }
 
-// Rows kept as bytes: every arc weighs >= 0 and the row (the last row ending
-// at nedges) sums to at most 255; the last row always keeps ints (its range
-// may change with nedges).
+// Rows kept as bytes (limit 255) or shorts (limit 65535): every arc weighs >= 0
+// and the row (the last row ending at nedges) sums to at most the limit; the
+// last row always keeps ints (its range may change with nedges).
 __global__ __launch_bounds__(256) void k_tally_class(DGraph g, i64 nedges, int unit, int all_int,
-                                                     i32* bix, u32* bigmask, i32* nbig) {
+                                                     i64 limit, i32* bix, u32* bigmask,
+                                                     i32* nbig) {
   for (i64 v0 = (i64)blockIdx.x * blockDim.x; v0 < g.n; v0 += (i64)gridDim.x * blockDim.x) {
     const i64 v = v0 + threadIdx.x;
     bool big = true;
     if (v < g.n && !all_int && v != g.n - 1) {
       const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
       if (unit) {
-        big = s1 - s0 > 255;
+        big = s1 - s0 > limit;
       } else {
         big = false;
         i64 sum = 0;
         for (i64 i = s0; i < s1; i++) {
           const i32 w = g.w[i];
           sum += w;
-          if (w < 0 || sum > 255) {
+          if (w < 0 || sum > limit) {
             big = true;
             break;
           }
@@ -4934,6 +5071,12 @@ void launch_gain_tallies(i64 n, const i32* part, const TallyRef& T, i32 k, const
   const size_t lds = (size_t)2 * k * 4;
   // one lane per vertex on a level with byte rows (k <= 64); int rows are read
   // by 4 lanes per vertex unless the caller asks for another width
+  if (T.t16 && !T.all_int && k <= TALLY16_MAXK) {
+    const unsigned nb = grid_for(n, 256 / 4, 8192), nbb = grid_for(T.nbig, 256, 1024);
+    k_gain_tallies16<4><<<nb + nbb, 256, 0, s>>>(n, part, T, k, bg, src_big, gain, target, cand,
+                                                 phase, cut_acc, nb);
+    return;
+  }
   if (k <= GT1_MAXK && lanes == 1 && !T.all_int) {
     const unsigned nb = grid_for(n, 256, 8192), nbb = grid_for(T.nbig, 256, 1024);
     k_gain_tallies1<<<nb + nbb, 256, 0, s>>>(n, part, T, k, bg, src_big, gain, target, cand,
@@ -4968,10 +5111,10 @@ void launch_tally_last(DGraph g, i64 nedges, const i32* part, const TallyRef& T,
                        hipStream_t s) {
   k_tally_last<<<1, 256, 0, s>>>(g, nedges, part, T, k, unit);
 }
-void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i32* bix, u32* bigmask,
-                        i32* nbig, hipStream_t s) {
-  k_tally_class<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, unit, all_int, bix,
-                                                              bigmask, nbig);
+void launch_tally_class(DGraph g, i64 nedges, int unit, int all_int, i64 limit, i32* bix,
+                        u32* bigmask, i32* nbig, hipStream_t s) {
+  k_tally_class<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nedges, unit, all_int, limit,
+                                                              bix, bigmask, nbig);
 }
 
 void launch_big_list(const i32* bix, i64 n, i32* bigv, hipStream_t s) {
