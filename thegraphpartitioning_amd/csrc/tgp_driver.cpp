@@ -474,7 +474,8 @@ class Run {
   int initial_partition(DBuf<i32>& part, i64* npairs);
   i64 compute_cut(Level& lv, i64 nedges, const i32* part);
   LevelAux& aux(Level& lv, i64 nedges);
-  void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut);
+  void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut,
+              bool cut_known = true);
   // cap > 0: capacity cap - count per part instead of the reference's rule
   void learned_passes(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64 cap = 0);
   void balance_finest(DBuf<i32>& part, DBuf<i32>& pc);
@@ -1100,7 +1101,11 @@ int Run::initial_partition(DBuf<i32>& part, i64* npairs) {
   return 0;
 }
 
-void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut) {
+// cut_known = false: *cut is not given; it is the entry partition's cut, which
+// the first pass's gain launch sums anyway (the same row ranges as
+// compute_cut), read with that pass's status.
+void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut,
+                 bool cut_known) {
   const int TOKEN_LANES = token_lanes(), GAIN_LANES = gain_lanes();
   Level& lv = *L[level];
   const i64 n = lv.n;
@@ -1114,6 +1119,11 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
   const i64 nheavy = ax.nheavy;
   const int unit = ax.unit;
 
+  if (!cut_known && n / k == 0) {
+    *cut = compute_cut(lv, nedges, part.p);
+    cut_known = true;
+  }
+  bool need_init = !cut_known;  // the first pass also reads the entry cut
   i64 bestCut = *cut;
   float bestScore = 1e9f;
   DBuf<i32> best(&C.pool, n);
@@ -1275,7 +1285,8 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           if (tmode && T_valid) {
             if (last_partial) launch_tally_last(g, nedges, part.p, TR, k, unit, st());
             launch_gain_tallies(n, part.p, TR, k, bg.p, src_big.p, gain.p, target.p, cand.p,
-                                phase, pending ? accs.p : nullptr, GAIN_LANES, st());
+                                phase, (pending || need_init) ? accs.p : nullptr, GAIN_LANES,
+                                st());
             launch_gain_rows(g, nedges, part.p, k, bg.p, src_big.p, gain.p, target.p, cand.p,
                              phase, unit, ax.over.p + 1, ax.nover, st());
             fused = true;
@@ -1283,16 +1294,17 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           } else {
             fused = launch_gain(g, nedges, part.p, part8.p, k, bg.p, src_big.p, gain.p, target.p,
                                 cand.p, phase, unit, gscr.p, ax.lists(),
-                                pending ? accs.p : nullptr, tmode ? &TR : nullptr, st());
+                                (pending || need_init) ? accs.p : nullptr, tmode ? &TR : nullptr,
+                                st());
             T_valid = tmode;
             C.prof.end(KID_GAIN, arc_b * lv.m + 21.0 * n);
           }
-          if (pending && !fused) {
+          if ((pending || need_init) && !fused) {
             C.prof.begin(KID_CUT);
             launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
             C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
           }
-          if (pending) launch_acc_finish(accs.p, cutacc.p, st());
+          if (pending || need_init) launch_acc_finish(accs.p, cutacc.p, st());
           C.prof.begin(KID_TOKEN);
           // Light rows: the arc stream (lanes 0) pays off when most candidates
           // win and read their whole rows; with few winners (under 30 % of the
@@ -1332,14 +1344,17 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           break;
         }
         if (pack) {
-          launch_pass_status(run ? (const i64*)small.p : nullptr, pending ? cutacc.p : nullptr,
+          launch_pass_status(run ? (const i64*)small.p : nullptr,
+                             (pending || need_init) ? cutacc.p : nullptr,
                              pending ? ctr_prev : nullptr, pending ? pc.p : nullptr, k,
                              run ? fit.p : nullptr, pstat.p, st());
-          const size_t bytes = pending ? 24 + 4 * (size_t)k : 8;
+          const size_t bytes = pending ? 24 + 4 * (size_t)k : need_init ? 16 : 8;
           HIP_TRY(hipMemcpyAsync(C.pinned, pstat.p, bytes, hipMemcpyDeviceToHost, st()));
-        } else if (pending) {
+        } else if (pending || need_init) {
           HIP_TRY(hipMemcpyAsync(C.pinned + 1, cutacc.p, sizeof(u64), hipMemcpyDeviceToHost,
                                  st()));
+        }
+        if (!pack && pending) {
           HIP_TRY(hipMemcpyAsync(C.pinned + 2, ctr_prev, 2 * sizeof(i32), hipMemcpyDeviceToHost,
                                  st()));
           HIP_TRY(hipMemcpyAsync(hpc.data(), pc.p, k * sizeof(i32), hipMemcpyDeviceToHost, st()));
@@ -1360,6 +1375,11 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           C.sync();
         }
         if (pack && pending) std::memcpy(hpc.data(), (const i32*)C.pinned + 6, k * sizeof(i32));
+        if (need_init) {  // the entry partition's cut, summed by this pass's gain launch
+          need_init = false;
+          bestCut = wrap_cut((u64)C.pinned[1]);
+          *cut = bestCut;
+        }
         if (pending) {
           // the previous pass's bookkeeping, gpu_part.cu:605-640 / 707-744
           pending = false;
@@ -1707,8 +1727,11 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     part = std::move(pf);
     if (O.learned && level == 0 && !(O.balance_eps > 0.0f))
       learned_passes(level, ecount[level], part, pc);
-    cut = compute_cut(*L[level], ecount[level], part.p);
-    refine(level, ecount[level], part, pc, &cut);
+    // the entry cut of the level comes with its first pass (the learned passes
+    // before level 0's refinement leave it to compute_cut)
+    const bool learned0 = O.learned && level == 0 && !(O.balance_eps > 0.0f);
+    if (learned0) cut = compute_cut(*L[level], ecount[level], part.p);
+    refine(level, ecount[level], part, pc, &cut, learned0);
   }
   if (O.balance_eps > 0.0f) balance_finest(part, pc);
   // gpu_part.cu:1285: the level-0 refinement ended with this partition's cut
