@@ -244,11 +244,18 @@ struct Prof {
     size_t done = 0;
     for (; done < pending.size(); done++) {
       const Rec& r = pending[done];
-      const hipError_t q = hipEventQuery(r.b);
-      if (q == hipErrorNotReady) break;
-      HIP_TRY(q);
+      const hipError_t qa = hipEventQuery(r.a), qb = hipEventQuery(r.b);
+      if (qa == hipErrorNotReady || qb == hipErrorNotReady) break;
+      HIP_TRY(qa);
+      HIP_TRY(qb);
       float t = 0;
-      HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+      // the runtime may still report a finished pair as not ready: keep it
+      const hipError_t e = hipEventElapsedTime(&t, r.a, r.b);
+      if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        break;
+      }
+      HIP_TRY(e);
       ms[r.kid] += t;
       pool.push_back(r.a);
       pool.push_back(r.b);
@@ -274,7 +281,8 @@ struct LevelAux {
   i64 last_begin = 0; // xadj[n-1]
   i64 nheavy = 0;     // rows longer than HEAVY_DEG
   int unit = 0;       // all weights are 1
-  DBuf<i32> asrc;     // arc -> source vertex
+  DBuf<i32> asrc;     // arc -> source vertex (built on first use: arc_source())
+  bool asrc_ok = false;
   DBuf<i32> heavy;    // heavy rows
   DBuf<i32> mid, medium, huge;  // heavy rows split at 512 and 4096 arcs
   i64 nmid = 0, nmedium = 0, nhuge = 0;
@@ -473,6 +481,7 @@ class Run {
   void check_symmetric(Level& lv);
   int initial_partition(DBuf<i32>& part, i64* npairs);
   i64 compute_cut(Level& lv, i64 nedges, const i32* part);
+  const i32* arc_source(Level& lv);
   LevelAux& aux(Level& lv, i64 nedges);
   void refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut,
               bool cut_known = true);
@@ -938,7 +947,6 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   LevelAux& a = lv.aux;
   if (a.valid && a.nedges == nedges) return a;
   a.heavy.ensure(&C.pool, std::max<i64>(lv.n, 1));
-  a.asrc.ensure(&C.pool, std::max<i64>(lv.m, 1));
   DBuf<i32> cnt32(&C.pool, 2);
   DBuf<u64> maxabs(&C.pool, 1);
   HIP_TRY(hipMemsetAsync(cnt32.p, 0, 2 * sizeof(i32), st()));
@@ -946,7 +954,6 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   C.prof.begin(KID_MISC);
   launch_level_census(lv.dg(), nedges, HEAVY_DEG, a.heavy.p, cnt32.p, cnt32.p + 1, maxabs.p,
                       st());
-  launch_arc_source(lv.dg(), a.asrc.p, st());
   C.prof.end(KID_MISC, 12.0 * lv.m);
   HIP_TRY(hipMemcpyAsync(C.pinned, cnt32.p, 2 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   HIP_TRY(hipMemcpyAsync(C.pinned + 1, lv.xadj_p + (lv.n - 1), sizeof(i64),
@@ -1018,6 +1025,20 @@ LevelAux& Run::aux(Level& lv, i64 nedges) {
   return a;
 }
 
+// the level's arc -> source array, built at its first use (the cut kernel's;
+// the passes of a level with kept tallies take their cuts without it)
+const i32* Run::arc_source(Level& lv) {
+  LevelAux& a = lv.aux;
+  if (!a.asrc_ok) {
+    a.asrc.ensure(&C.pool, std::max<i64>(lv.m, 1));
+    C.prof.begin(KID_MISC);
+    launch_arc_source(lv.dg(), a.asrc.p, st());
+    C.prof.end(KID_MISC, 4.0 * lv.m + 8.0 * lv.n);
+    a.asrc_ok = true;
+  }
+  return a.asrc.p;
+}
+
 i64 Run::compute_cut(Level& lv, i64 nedges, const i32* part) {
   LevelAux& a = aux(lv, nedges);
   C.prof.begin(KID_CUT);
@@ -1026,7 +1047,7 @@ i64 Run::compute_cut(Level& lv, i64 nedges, const i32* part) {
     part8 = DBuf<u8>(&C.pool, lv.n);
     launch_part8(part, lv.n, k, part8.p, st());
   }
-  launch_cut_arcs(lv.dg(), a.e_end, a.asrc.p, part, part8.p, k, a.unit, accs.p, st());
+  launch_cut_arcs(lv.dg(), a.e_end, arc_source(lv), part, part8.p, k, a.unit, accs.p, st());
   launch_acc_finish(accs.p, (u64*)(small.p + 1), st());
   C.prof.end(KID_CUT, (a.unit ? 12.0 : 16.0) * a.e_end);
   return wrap_cut((u64)read_i64(small.p + 1));
@@ -1301,7 +1322,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
           }
           if ((pending || need_init) && !fused) {
             C.prof.begin(KID_CUT);
-            launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
+            launch_cut_arcs(g, ax.e_end, arc_source(lv), part.p, part8.p, k, unit, accs.p, st());
             C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
           }
           if (pending || need_init) launch_acc_finish(accs.p, cutacc.p, st());
@@ -1336,7 +1357,7 @@ void Run::refine(int level, i64 nedges, DBuf<i32>& part, DBuf<i32>& pc, i64* cut
             launch_acc_finish(accs.p, cutacc.p, st());
             C.prof.end(KID_CUT, tally_bytes + 4.0 * n);
           } else {
-            launch_cut_arcs(g, ax.e_end, ax.asrc.p, part.p, part8.p, k, unit, accs.p, st());
+            launch_cut_arcs(g, ax.e_end, arc_source(lv), part.p, part8.p, k, unit, accs.p, st());
             launch_acc_finish(accs.p, cutacc.p, st());
             C.prof.end(KID_CUT, (unit ? 12.0 : 16.0) * ax.e_end);
           }
