@@ -97,7 +97,9 @@ typedef struct tgp_options {
                               vertex count below which coarsening stops         */
   tgp_initpart_fn initpart;/* NULL: tgp_initpart_gpmetis (the reference seam)  */
   void* initpart_user;
-  int32_t device_input;    /* 1: xadj/adjncy/adjwgt are device pointers         */
+  int32_t device_input;    /* 1: xadj/adjncy/adjwgt are device pointers (checked
+                              on the device as a host CSR is on the host:
+                              TGP_EINVAL when malformed)                        */
   int32_t profile;         /* 1: time kernels with HIP events into tgp_stats    */
   int32_t check_level;     /* 1: after every pass copy the partition to host and
                               hash it into the trace (slow; tests only)         */

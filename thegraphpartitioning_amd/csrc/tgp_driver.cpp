@@ -1698,6 +1698,18 @@ void Run::setup_level0(const tgp_graph* g) {
   l0->n = n;
   l0->m = m;
   if (O.device_input) {
+    // a device CSR is checked on the device before any kernel follows its
+    // offsets (a host CSR was checked by validate_host_graph)
+    DBuf<i32> bad(&C.pool, 1);
+    HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(i32), st()));
+    launch_csr_check(g->xadj, g->adjncy, n, m, bad.p, st());
+    HIP_TRY(hipMemcpyAsync(C.pinned, bad.p, sizeof(i32), hipMemcpyDeviceToHost, st()));
+    C.sync();
+    if (((const i32*)C.pinned)[0] != 0) {
+      set_error("malformed device CSR: need xadj[0]=0, non-decreasing offsets, xadj[n]=m and "
+                "every adjncy entry in [0, n)");
+      throw Fail(TGP_EINVAL);
+    }
     l0->xadj_p = g->xadj;
     l0->adj_p = g->adjncy;
     l0->w_p = g->adjwgt;
@@ -2046,7 +2058,25 @@ struct BatchEngine {
     for (i64 j = 0; j < T; j++) ord[j] = (i32)j;
     std::stable_sort(ord.begin(), ord.end(), [&](i32 a, i32 b) { return hg[a].m > hg[b].m; });
     HIP_TRY(hipMemcpyAsync(dg.p, hg.data(), sizeof(BatchGraph) * T, hipMemcpyHostToDevice, C.s));
-    HIP_TRY(hipMemcpyAsync(order.p, ord.data(), sizeof(i32) * T, hipMemcpyHostToDevice, C.s));
+    // device-resident graphs are checked on the device first; a malformed one
+    // stays out of the engine and the per-item path rejects it (TGP_EINVAL)
+    std::vector<i32> devl;
+    for (i64 j = 0; j < T; j++)
+      if (items[take[j]].o.device_input) devl.push_back((i32)j);
+    if (!devl.empty()) {
+      DBuf<i32> dl(&C.pool, devl.size()), bad(&C.pool, (size_t)T);
+      HIP_TRY(hipMemcpyAsync(dl.p, devl.data(), sizeof(i32) * devl.size(),
+                             hipMemcpyHostToDevice, C.s));
+      HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(i32) * T, C.s));
+      launch_batch_csr_check(dg.p, dl.p, (i32)devl.size(), bad.p, C.s);
+      std::vector<i32> hb((size_t)T);
+      HIP_TRY(hipMemcpyAsync(hb.data(), bad.p, sizeof(i32) * T, hipMemcpyDeviceToHost, C.s));
+      C.sync();
+      ord.erase(std::remove_if(ord.begin(), ord.end(), [&](i32 j) { return hb[j] != 0; }),
+                ord.end());
+    }
+    HIP_TRY(hipMemcpyAsync(order.p, ord.data(), sizeof(i32) * ord.size(), hipMemcpyHostToDevice,
+                           C.s));
     launch_fill_i32(status.p, T, 1, C.s);
     // Coarsening: one workgroup per CU; refinement: two per CU; each with
     // scratch for the largest graph.  The coarse levels of the graphs share

@@ -200,6 +200,14 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 // flags[0] |= some arc s->x (s != x, x < n) has no reverse; flags[1] |= some
 // (s, x) appears twice in s's row; flags[2] |= some reverse has another weight.
 void launch_sym_check(DGraph g, u64* table, i32* wt, i64 cap, i32* flags, hipStream_t s);
+// *flag = 1 (zeroed by the caller) unless xadj[0] = 0, the offsets do not
+// decrease, xadj[n] = m and every adj entry is in [0, n) (a CSR handed over
+// in device memory)
+void launch_csr_check(const i64* xadj, const i32* adj, i64 n, i64 m, i32* flag, hipStream_t s);
+// bad[j] |= 1 (bad zeroed by the caller) for each listed batch graph j whose
+// device CSR fails the same check
+void launch_batch_csr_check(const BatchGraph* dg, const i32* list, i32 cnt, i32* bad,
+                            hipStream_t s);
 // flags[3] |= some row is not strictly ascending
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
 // the same checks for strictly ascending rows, by binary search of the

@@ -1521,6 +1521,48 @@ __global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
   if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
 }
 
+// The shape of a CSR handed over in device memory: xadj[0] = 0, offsets
+// non-decreasing, xadj[n] = m, every neighbour id in [0, n); *flag = 1 on any
+// violation.  Reads the n + 1 offsets and the m ids only (within the arrays'
+// declared sizes), so a malformed graph is rejected before any kernel follows
+// its offsets.
+__global__ void k_csr_check(const i64* xadj, const i32* adj, i64 n, i64 m, i32* flag) {
+  bool bad = false;
+  for (i64 v = gtid(); v <= n; v += gstride()) {
+    const i64 x = xadj[v];
+    if (v == 0) bad |= x != 0;
+    if (v == n) bad |= x != m;
+    else bad |= xadj[v + 1] < x;
+  }
+  for (i64 i = gtid(); i < m; i += gstride()) {
+    const i32 u = adj[i];
+    bad |= u < 0 || (i64)u >= n;
+  }
+  if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1);
+}
+
+// The same check for the device-resident graphs of a batch: one block per
+// listed graph, bad[j] = 1 when graph j is malformed.
+__global__ __launch_bounds__(256) void k_batch_csr_check(const BatchGraph* dg, const i32* list,
+                                                         i32 cnt, i32* bad) {
+  for (i32 e = blockIdx.x; e < cnt; e += gridDim.x) {
+    const i32 j = list[e];
+    const BatchGraph b = dg[j];
+    bool f = false;
+    for (i64 v = threadIdx.x; v <= b.n; v += blockDim.x) {
+      const i64 x = b.xadj[v];
+      if (v == 0) f |= x != 0;
+      if (v == b.n) f |= x != b.m;
+      else f |= b.xadj[v + 1] < x;
+    }
+    for (i64 i = threadIdx.x; i < b.m; i += blockDim.x) {
+      const i32 u = b.adj[i];
+      f |= u < 0 || (i64)u >= b.n;
+    }
+    if (__ballot(f) && lane_id() == 0) atomicOr(&bad[j], 1);
+  }
+}
+
 // Bit length of each row's degree (the rank of a row in the search below).
 __global__ void k_deg_class(DGraph g, u8* cls) {
   for (i64 v = gtid(); v < g.n; v += gstride()) {
@@ -4673,6 +4715,15 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s) {
   k_rows_strict<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags);
 }
+void launch_csr_check(const i64* xadj, const i32* adj, i64 n, i64 m, i32* flag, hipStream_t s) {
+  k_csr_check<<<grid_for(std::max(n + 1, m), 256, 8192), 256, 0, s>>>(xadj, adj, n, m, flag);
+}
+
+void launch_batch_csr_check(const BatchGraph* dg, const i32* list, i32 cnt, i32* bad,
+                            hipStream_t s) {
+  if (cnt > 0) k_batch_csr_check<<<grid_for(cnt, 1, 4096), 256, 0, s>>>(dg, list, cnt, bad);
+}
+
 i64 sym_check_chunks(i64 m) { return (m + SYM_CHUNK - 1) / SYM_CHUNK; }
 void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, i32* crow, i32* flags, u64* cnt,
                              hipStream_t s) {
