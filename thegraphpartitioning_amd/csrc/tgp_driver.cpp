@@ -358,10 +358,16 @@ struct Context {
     }
     prof.s = s;
     HIP_TRY(hipEventCreateWithFlags(&mark_ev, hipEventDisableTiming));
+    HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&ev_main, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
   }
   ~Context() {
     if (pinned && own_pinned) (void)hipHostFree(pinned);
     if (mark_ev) (void)hipEventDestroy(mark_ev);
+    if (ev_main) (void)hipEventDestroy(ev_main);
+    if (ev_side) (void)hipEventDestroy(ev_side);
+    if (s2) (void)hipStreamDestroy(s2);
     if (s) (void)hipStreamDestroy(s);
   }
   bool own_pinned = false;
@@ -372,6 +378,10 @@ struct Context {
   // mark(): the work enqueued so far; wait_mark(): wait for it only (work
   // enqueued after the mark keeps running)
   hipEvent_t mark_ev = nullptr;
+  // a second stream (default priority) for work that overlaps the main one:
+  // ev_main orders it after the main stream's work so far, ev_side marks its end
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_main = nullptr, ev_side = nullptr;
   void mark() { HIP_TRY(hipEventRecord(mark_ev, s)); }
   void wait_mark() {
     HIP_TRY(hipEventSynchronize(mark_ev));
@@ -427,6 +437,12 @@ class Run {
     }
   }
 
+  // the symmetry check may still run on the second stream when a run ends
+  // early (an error): its buffers are not handed back before it finishes
+  ~Run() {
+    if (sp_active) (void)hipStreamSynchronize(C.s2);
+  }
+
   int go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out);
   void setup_level0(const tgp_graph* g);
   // the balance stage alone on a given partition of g (tgp_balance)
@@ -478,7 +494,17 @@ class Run {
 
   void coarsen();
   void match_level(const Level& lv, DBuf<i32>& mate);
-  void check_symmetric(Level& lv);
+  // overlap: on strictly ascending rows, run the search on the second stream
+  // and take the level as symmetric until join_symmetric (returns true then)
+  bool check_symmetric(Level& lv, bool overlap = false);
+  // the overlapped check's verdict into lv; false when the level turned out
+  // not symmetric (whatever was derived from the assumption must be redone)
+  bool join_symmetric(Level& lv);
+  DBuf<i32> sp_flags;
+  DBuf<u64> sp_cnt;
+  DBuf<u8> sp_cls;
+  DBuf<i32> sp_crow;
+  bool sp_active = false;
   int initial_partition(DBuf<i32>& part, i64* npairs);
   i64 compute_cut(Level& lv, i64 nedges, const i32* part);
   const i32* arc_source(Level& lv);
@@ -712,21 +738,39 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
                                                            t_start).count());
 }
 
-void Run::check_symmetric(Level& lv) {
-  DBuf<i32> flags(&C.pool, 5);
+bool Run::check_symmetric(Level& lv, bool overlap) {
+  DBuf<i32> flags(&C.pool, 6);
   DBuf<u64> cnt(&C.pool, 2);
-  HIP_TRY(hipMemsetAsync(flags.p, 0, 5 * sizeof(i32), st()));
+  HIP_TRY(hipMemsetAsync(flags.p, 0, 6 * sizeof(i32), st()));
   HIP_TRY(hipMemsetAsync(cnt.p, 0, 2 * sizeof(u64), st()));
   C.prof.begin(KID_MISC);
   launch_rows_strict(lv.dg(), flags.p, st());
-  C.prof.end(KID_MISC, 4.0 * lv.m);
-  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 4 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  C.prof.end(KID_MISC, 8.0 * lv.m);
+  HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 6 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
   const bool strict = ((const i32*)C.pinned)[3] == 0;
+  const bool nonunit = ((const i32*)C.pinned)[5] != 0;
   lv.sorted = strict;
-  // strictly ascending rows hold no duplicates: each arc down finds its
-  // reverse by binary search in the sorted row; other rows go through a hash
-  // table of every arc
+  if (strict && overlap) {
+    // the search runs beside the level's matching, which assumes symmetry
+    // (and, on a symmetric level, the weights rows_strict saw decide unitw)
+    sp_flags = std::move(flags);
+    sp_cnt = std::move(cnt);
+    sp_cls = DBuf<u8>(&C.pool, std::max<i64>(lv.n, 1));
+    sp_crow = DBuf<i32>(&C.pool, std::max<i64>(sym_check_chunks(lv.m), 1));
+    HIP_TRY(hipEventRecord(C.ev_main, C.s));
+    HIP_TRY(hipStreamWaitEvent(C.s2, C.ev_main, 0));
+    launch_sym_check_sorted(lv.dg(), lv.m, sp_cls.p, sp_crow.p, sp_flags.p, sp_cnt.p, C.s2);
+    HIP_TRY(hipEventRecord(C.ev_side, C.s2));
+    sp_active = true;
+    lv.symmetric = true;
+    lv.simple_ws = false;
+    lv.unitw = !nonunit;
+    return true;
+  }
+  // strictly ascending rows hold no duplicates: each pair's searching arc
+  // finds its reverse by binary search in the sorted row; other rows go
+  // through a hash table of every arc
   DBuf<u64> table;
   DBuf<i32> wt;
   C.prof.begin(KID_MISC);
@@ -754,13 +798,40 @@ void Run::check_symmetric(Level& lv) {
   // unit weights are read off the sorted check's found pairs (all arcs of a
   // symmetric level between distinct vertices in range)
   lv.unitw = sym && strict && f[4] == 0;
+  return false;
+}
+
+bool Run::join_symmetric(Level& lv) {
+  if (!sp_active) return true;
+  HIP_TRY(hipEventSynchronize(C.ev_side));
+  sp_active = false;
+  HIP_TRY(hipMemcpyAsync(C.pinned, sp_flags.p, 5 * sizeof(i32), hipMemcpyDeviceToHost, st()));
+  HIP_TRY(hipMemcpyAsync(C.pinned + 3, sp_cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, st()));
+  C.sync();
+  const i32* f = (const i32*)C.pinned;
+  const bool sym = f[0] == 0 && C.pinned[3] == C.pinned[4];
+  lv.symmetric = sym;
+  lv.simple_ws = sym && f[1] == 0 && f[2] == 0;
+  lv.unitw = sym && f[4] == 0;
+  sp_flags = DBuf<i32>();
+  sp_cnt = DBuf<u64>();
+  sp_cls = DBuf<u8>();
+  sp_crow = DBuf<i32>();
+  return sym;
 }
 
 void Run::coarsen() {
   const i64 n0 = L[0]->n;
   const i64 target = O.coarsen_floor > 0 ? O.coarsen_floor : std::max<i64>(500, n0 / 500);
   i64 ncur = n0;
-  if (ncur > target) check_symmetric(*L[0]);
+  // On strictly ascending rows the symmetry search of level 0 runs on the
+  // second stream beside the level's matching, which assumes symmetry; a
+  // level that turns out not symmetric is matched again by the round kernels
+  // (TGP_SYM_OVERLAP=0: check first; never in the no-allocation context,
+  // whose staging memory is not pinned)
+  const bool overlap = !C.pool.arena() && env_i64("TGP_SYM_OVERLAP", 1) != 0;
+  bool spec = false;
+  if (ncur > target) spec = check_symmetric(*L[0], overlap);
   DBuf<i32> mate(&C.pool, n0), new_ids(&C.pool, n0);
   DBuf<i64> flags(&C.pool, n0), pos(&C.pool, n0);
   while (ncur > target) {  // gpu_part.cu:1003
@@ -770,6 +841,10 @@ void Run::coarsen() {
     if (repair && L.size() >= 2 && 100 * L.back()->n > 93 * L[L.size() - 2]->n) break;
     Level& lv = *L.back();
     match_level(lv, mate);
+    if (spec) {
+      spec = false;
+      if (!join_symmetric(lv)) match_level(lv, mate);
+    }
     auto tb = DBuf<i32>(&C.pool, 2 * ncur);
     C.prof.begin(KID_RELABEL);
     launch_relabel(mate.p, ncur, flags.p, pos.p, small.p, new_ids.p, tb.p, scratch(), st());

@@ -208,7 +208,8 @@ void launch_csr_check(const i64* xadj, const i32* adj, i64 n, i64 m, i32* flag, 
 // device CSR fails the same check
 void launch_batch_csr_check(const BatchGraph* dg, const i32* list, i32 cnt, i32* bad,
                             hipStream_t s);
-// flags[3] |= some row is not strictly ascending
+// flags[3] |= some row is not strictly ascending; flags[5] |= some arc between
+// distinct vertices in range weighs other than 1
 void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
 // the same checks for strictly ascending rows, by binary search of the
 // reverse arc in its (sorted) row: flags[0], flags[2] as above; of each pair

@@ -1511,14 +1511,21 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, c
 }
 
 // flags[3] |= some row is not strictly ascending (so it may hold duplicates)
+// flags[3] |= some row is not strictly ascending; flags[5] |= some arc between
+// distinct vertices in range weighs other than 1.
 __global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
-  bool bad = false;
+  bool bad = false, nonunit = false;
   const int lane = lane_id();
   for (i64 v = gwave(); v < g.n; v += nwaves()) {
     const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
-    for (i64 i = s0 + 1 + lane; i < s1; i += WAVE) bad |= g.adj[i] <= g.adj[i - 1];
+    for (i64 i = s0 + lane; i < s1; i += WAVE) {
+      const i32 u = g.adj[i];
+      if (i > s0) bad |= u <= g.adj[i - 1];
+      nonunit |= u >= 0 && u < g.n && u != v && g.w[i] != 1;
+    }
   }
   if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
+  if (__ballot(nonunit) && lane == 0) atomicOr(&flags[5], 1);
 }
 
 // The shape of a CSR handed over in device memory: xadj[0] = 0, offsets
