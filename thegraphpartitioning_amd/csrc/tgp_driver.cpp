@@ -871,20 +871,21 @@ void Run::coarsen() {
       DBuf<i64> dsz(&C.pool, n_c + 1), dlong(&C.pool, n_c + 1), csz(&C.pool, n_c + 1),
           eoff(&C.pool, n_c + 1), toff(&C.pool, n_c + 1), nchk(&C.pool, n_c + 1),
           cbase(&C.pool, n_c + 1);
-      DBuf<i32> med(&C.pool, n_c), lng(&C.pool, n_c), ncls(&C.pool, 2);
-      HIP_TRY(hipMemsetAsync(ncls.p, 0, 2 * sizeof(i32), st()));
+      DBuf<i32> med(&C.pool, n_c), meds(&C.pool, n_c), lng(&C.pool, n_c), ncls(&C.pool, 3);
+      HIP_TRY(hipMemsetAsync(ncls.p, 0, 3 * sizeof(i32), st()));
       launch_rep_classify(lv.xadj_p, tb.p, n_c, dsz.p, dlong.p, csz.p, med.p, ncls.p, lng.p,
-                          ncls.p + 1, nchk.p, st());
+                          ncls.p + 1, nchk.p, meds.p, ncls.p + 2, st());
       launch_scan_excl(dlong.p, eoff.p, n_c, eoff.p + n_c, scratch(), st());
       launch_scan_excl(csz.p, toff.p, n_c, toff.p + n_c, scratch(), st());
       HIP_TRY(hipMemcpyAsync(C.pinned, eoff.p + n_c, sizeof(i64), hipMemcpyDeviceToHost, st()));
       HIP_TRY(hipMemcpyAsync(C.pinned + 1, toff.p + n_c, sizeof(i64), hipMemcpyDeviceToHost,
                              st()));
-      HIP_TRY(hipMemcpyAsync(C.pinned + 2, ncls.p, 2 * sizeof(i32), hipMemcpyDeviceToHost,
+      HIP_TRY(hipMemcpyAsync(C.pinned + 2, ncls.p, 3 * sizeof(i32), hipMemcpyDeviceToHost,
                              st()));
       C.sync();
       const i64 etot = C.pinned[0], ttot = C.pinned[1];
       const i64 nmed = ((const i32*)(C.pinned + 2))[0], nlng = ((const i32*)(C.pinned + 2))[1];
+      const i64 nmeds = ((const i32*)(C.pinned + 2))[2];
       RepLong RL;
       DBuf<u64> T;
       DBuf<i32> W, E, ch_row;
@@ -976,15 +977,15 @@ void Run::coarsen() {
         launch_scan_excl(chunk_cnt.p, choff.p, nch, choff.p + nch, scratch(), st());
         launch_rep_rowcnt(RL, cnt.p, st());
       }
-      launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, eoff.p, R.p, 0, cnt.p,
-                      nullptr, nullptr, nullptr, st());
+      launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, meds.p, nmeds, eoff.p, R.p,
+                      0, cnt.p, nullptr, nullptr, nullptr, st());
       launch_scan_excl(cnt.p, c->xadj.p, n_c, c->xadj.p + n_c, scratch(), st());
       m_c = read_i64(c->xadj.p + n_c);
       c->adj = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
       c->w = DBuf<i32>(&C.pool, std::max<i64>(m_c, 1));
       if (m_c > 0) {
-        launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, eoff.p, R.p, 1, nullptr,
-                        c->xadj.p, c->adj.p, c->w.p, st());
+        launch_rep_rows(g, tb.p, new_ids.p, n_c, dsz.p, med.p, nmed, meds.p, nmeds, eoff.p,
+                        R.p, 1, nullptr, c->xadj.p, c->adj.p, c->w.p, st());
         if (bucketed) launch_rl_emit(RL, RB, c->xadj.p, c->adj.p, c->w.p, st());
         else launch_rep_long(g, tb.p, new_ids.p, RL, 1, c->xadj.p, c->adj.p, c->w.p, st());
       }

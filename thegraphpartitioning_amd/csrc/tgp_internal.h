@@ -234,15 +234,17 @@ constexpr i64 REP_CHUNK_ENTRIES = 4096;
 // (their entry words R sit at the exclusive scan eoff of dlong), and for rows
 // past REP_MED entries (long rows, listed in lng) their global table capacity.
 // (nchk[e] = chunks of long row e)
+// (medium rows: med holds those of more than 256 entries, meds the others)
 void launch_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* dlong,
                          i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk,
-                         hipStream_t s);
+                         i32* meds, i32* nmeds, hipStream_t s);
 // rows of at most REP_MED entries (registers, or an LDS table for the listed
 // medium rows, whose entry words go to R at eoff): pass 0 writes cnt[c], pass
 // 1 writes row c at ox[c]
 void launch_rep_rows(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* dsz,
-                     const i32* med, i64 nmed, const i64* eoff, u64* R, int pass, i64* cnt,
-                     const i64* ox, i32* oa, i32* ow, hipStream_t s);
+                     const i32* med, i64 nmed, const i32* meds, i64 nmeds, const i64* eoff,
+                     u64* R, int pass, i64* cnt, const i64* ox, i32* oa, i32* ow,
+                     hipStream_t s);
 // Long rows: listed in lng (nl rows), cut into chunks of REP_CHUNK_ENTRIES
 // (cbase[e] = first chunk of row e, exclusive scan; ch_row/ch_beg describe
 // each chunk); per-row global tables T (64-bit slots) and W at toff with

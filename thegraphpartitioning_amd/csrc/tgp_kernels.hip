@@ -1781,9 +1781,12 @@ __device__ __forceinline__ i64 next_pow2_i64(i64 x) {
 // in global tables (dlong = d and csz = table capacity only for those).
 constexpr i64 REP_SMALL = 64;
 constexpr i64 REP_MED = 2048;
+// medium rows of at most REP_MEDW entries take one wave each (medS)
+constexpr i64 REP_MEDW = 256;
 
 __global__ void k_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* dlong,
-                               i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk) {
+                               i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk,
+                               i32* meds, i32* nmeds) {
   for (i64 c = gtid(); c < n_c; c += gstride()) {
     const i32 r = tb[2 * c], mt = tb[2 * c + 1];
     i64 d = xadj[r + 1] - xadj[r];
@@ -1797,7 +1800,8 @@ __global__ void k_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz
       lng[e] = (i32)c;
       nchk[e] = (d + REP_CHUNK_ENTRIES - 1) / REP_CHUNK_ENTRIES;
     }
-    wave_append(!is_long && d > REP_SMALL, (i32)c, med, nmed);
+    wave_append(!is_long && d > REP_MEDW, (i32)c, med, nmed);
+    wave_append(d > REP_SMALL && d <= REP_MEDW, (i32)c, meds, nmeds);
   }
 }
 
@@ -2437,6 +2441,113 @@ __global__ __launch_bounds__(256) void k_rep_med1(const i32* lst, i64 nl, const 
       }
       run += tot;
       __syncthreads();
+    }
+  }
+}
+
+// Medium rows of at most REP_MEDW entries: k_rep_med0/1 with one wave per
+// row (4 entries per lane, a 512-slot LDS table per wave, no block barriers:
+// a wave's LDS accesses are ordered by LDS-scope fences).
+constexpr int REP_MEDW_PER = (int)(REP_MEDW / WAVE);
+__device__ __forceinline__ void lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
+}
+__global__ __launch_bounds__(256) void k_rep_medw0(DGraph g, const i32* tb, const i32* new_ids,
+                                                   const i32* lst, i64 nl, const i64* dsz,
+                                                   const i64* eoff, u64* R, i64* cnt) {
+  constexpr int TC = (int)(2 * REP_MEDW);
+  __shared__ i32 Ks[256 / WAVE][TC], Ps[256 / WAVE][TC], Ws[256 / WAVE][TC];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  i32* K = Ks[wv];
+  i32* P = Ps[wv];
+  i32* W = Ws[wv];
+  for (i64 e = gwave(); e < nl; e += nwaves()) {
+    const i64 c = lst[e];
+    const i64 d = dsz[c];
+    const i32 r = tb[2 * c], mt = tb[2 * c + 1];
+    const i64 a0 = g.xadj[r], la = g.xadj[r + 1] - a0;
+    const i64 b0 = mt >= 0 ? g.xadj[mt] : 0;
+    const i32 cap = (i32)next_pow2_i64(2 * d);
+    const u32 mask = (u32)(cap - 1);
+    i32 x[REP_MEDW_PER], w[REP_MEDW_PER];
+#pragma unroll
+    for (int j = 0; j < REP_MEDW_PER; j++) {
+      const i64 p = j * WAVE + lane;
+      const i64 i = p < la ? a0 + p : b0 + (p - la);
+      x[j] = p < d ? g.adj[i] : -1;
+      w[j] = p < d ? g.w[i] : 0;
+    }
+    lds_order();  // the previous row's lookups are done
+    for (int h = lane; h < cap; h += WAVE) {
+      K[h] = -1;
+      P[h] = INT_MAX;
+      W[h] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < REP_MEDW_PER; j++) {
+      x[j] = x[j] >= 0 ? new_ids[x[j]] : -1;
+      if (x[j] == (i32)c) x[j] = -1;
+    }
+    lds_order();  // table cleared
+#pragma unroll
+    for (int j = 0; j < REP_MEDW_PER; j++) {
+      if (x[j] < 0) continue;
+      u32 h = hash32((u32)x[j]) & mask;
+      while (true) {
+        const i32 old = atomicCAS(&K[h], -1, x[j]);
+        if (old == -1 || old == x[j]) break;
+        h = (h + 1) & mask;
+      }
+      atomicMin(&P[h], j * WAVE + lane);
+      atomicAdd(&W[h], w[j]);
+    }
+    lds_order();  // every entry inserted
+    u64* Rc = R + eoff[c];
+    i32 kept = 0;
+#pragma unroll
+    for (int j = 0; j < REP_MEDW_PER; j++) {
+      const i64 p = j * WAVE + lane;
+      if (p >= d) continue;
+      u64 word = (u64)RL_NONE << 32;
+      if (x[j] >= 0) {
+        u32 h = hash32((u32)x[j]) & mask;
+        while (K[h] != x[j]) h = (h + 1) & mask;
+        if (P[h] == (i32)p) {
+          word = ((u64)(u32)x[j] << 32) | (u64)(u32)W[h];
+          kept++;
+        }
+      }
+      Rc[p] = word;
+    }
+    kept = wave_sum(kept);
+    if (lane == 0) cnt[c] = kept;
+  }
+}
+__global__ __launch_bounds__(256) void k_rep_medw1(const i32* lst, i64 nl, const i64* dsz,
+                                                   const i64* eoff, const u64* R, const i64* ox,
+                                                   i32* oa, i32* ow) {
+  const int lane = lane_id();
+  for (i64 e = gwave(); e < nl; e += nwaves()) {
+    const i64 c = lst[e];
+    const i64 d = dsz[c];
+    const u64* Rc = R + eoff[c];
+    u64 rv[REP_MEDW_PER];
+#pragma unroll
+    for (int j = 0; j < REP_MEDW_PER; j++) {
+      const i64 p = j * WAVE + lane;
+      rv[j] = p < d ? Rc[p] : ((u64)RL_NONE << 32);
+    }
+    i64 run = ox[c];
+#pragma unroll
+    for (int j = 0; j < REP_MEDW_PER; j++) {
+      const bool keep = (u32)(rv[j] >> 32) != RL_NONE;
+      const u64 bm = __ballot(keep);
+      if (keep) {
+        const i64 o = run + __popcll(bm & lanes_below(lane));
+        oa[o] = (i32)(rv[j] >> 32);
+        ow[o] = (i32)(u32)rv[j];
+      }
+      run += __popcll(bm);
     }
   }
 }
@@ -4766,9 +4877,9 @@ void launch_compact_rows(const i32* sadj, const i32* sw, const i64* ox, i64 n_c,
 
 void launch_rep_classify(const i64* xadj, const i32* tb, i64 n_c, i64* dsz, i64* dlong,
                          i64* csz, i32* med, i32* nmed, i32* lng, i32* nlng, i64* nchk,
-                         hipStream_t s) {
-  k_rep_classify<<<grid_for(n_c, 256, 1u << 16), 256, 0, s>>>(xadj, tb, n_c, dsz, dlong, csz,
-                                                              med, nmed, lng, nlng, nchk);
+                         i32* meds, i32* nmeds, hipStream_t s) {
+  k_rep_classify<<<grid_for(n_c, 256, 1u << 16), 256, 0, s>>>(
+      xadj, tb, n_c, dsz, dlong, csz, med, nmed, lng, nlng, nchk, meds, nmeds);
 }
 void launch_rep_chunks(const i32* lng, i64 nl, const i64* dsz, const i64* cbase, i32* ch_row,
                        i64* ch_beg, hipStream_t s) {
@@ -4834,17 +4945,21 @@ void launch_rep_rowcnt(const RepLong& L, i64* cnt, hipStream_t s) {
     k_rep_rowcnt<<<grid_for(L.nl, 256, 4096), 256, 0, s>>>(L.lng, L.nl, L.cbase, L.choff, cnt);
 }
 void launch_rep_rows(DGraph g, const i32* tb, const i32* new_ids, i64 n_c, const i64* dsz,
-                     const i32* med, i64 nmed, const i64* eoff, u64* R, int pass, i64* cnt,
-                     const i64* ox, i32* oa, i32* ow, hipStream_t s) {
+                     const i32* med, i64 nmed, const i32* meds, i64 nmeds, const i64* eoff,
+                     u64* R, int pass, i64* cnt, const i64* ox, i32* oa, i32* ow,
+                     hipStream_t s) {
   const unsigned g8 = grid_for(n_c, 256 / 8, 1u << 16), g64 = grid_for(n_c, 4, 1u << 16);
-  const unsigned gm = grid_for(nmed, 1, 1u << 16);
+  const unsigned gm = grid_for(nmed, 1, 1u << 16), gw = grid_for(nmeds, 4, 1u << 16);
   if (pass == 0) {
     k_rep_small<8, 0><<<g8, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 0, cnt, ox, oa, ow);
     k_rep_small<64, 0><<<g64, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 9, cnt, ox, oa, ow);
+    if (nmeds > 0)
+      k_rep_medw0<<<gw, 256, 0, s>>>(g, tb, new_ids, meds, nmeds, dsz, eoff, R, cnt);
     if (nmed > 0) k_rep_med0<<<gm, 256, 0, s>>>(g, tb, new_ids, med, nmed, dsz, eoff, R, cnt);
   } else {
     k_rep_small<8, 1><<<g8, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 0, cnt, ox, oa, ow);
     k_rep_small<64, 1><<<g64, 256, 0, s>>>(g, tb, new_ids, n_c, dsz, 9, cnt, ox, oa, ow);
+    if (nmeds > 0) k_rep_medw1<<<gw, 256, 0, s>>>(meds, nmeds, dsz, eoff, R, ox, oa, ow);
     if (nmed > 0) k_rep_med1<<<gm, 256, 0, s>>>(med, nmed, dsz, eoff, R, ox, oa, ow);
   }
 }
