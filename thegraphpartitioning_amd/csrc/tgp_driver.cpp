@@ -743,8 +743,10 @@ bool Run::check_symmetric(Level& lv, bool overlap) {
   DBuf<u64> cnt(&C.pool, 2);
   HIP_TRY(hipMemsetAsync(flags.p, 0, 6 * sizeof(i32), st()));
   HIP_TRY(hipMemsetAsync(cnt.p, 0, 2 * sizeof(u64), st()));
+  // the row of each arc chunk's first arc, for both checks
+  DBuf<i32> crow(&C.pool, std::max<i64>(sym_check_chunks(lv.m), 1));
   C.prof.begin(KID_MISC);
-  launch_rows_strict(lv.dg(), flags.p, st());
+  launch_rows_strict(lv.dg(), lv.m, crow.p, flags.p, st());
   C.prof.end(KID_MISC, 8.0 * lv.m);
   HIP_TRY(hipMemcpyAsync(C.pinned, flags.p, 6 * sizeof(i32), hipMemcpyDeviceToHost, st()));
   C.sync();
@@ -757,7 +759,7 @@ bool Run::check_symmetric(Level& lv, bool overlap) {
     sp_flags = std::move(flags);
     sp_cnt = std::move(cnt);
     sp_cls = DBuf<u8>(&C.pool, std::max<i64>(lv.n, 1));
-    sp_crow = DBuf<i32>(&C.pool, std::max<i64>(sym_check_chunks(lv.m), 1));
+    sp_crow = std::move(crow);
     HIP_TRY(hipEventRecord(C.ev_main, C.s));
     HIP_TRY(hipStreamWaitEvent(C.s2, C.ev_main, 0));
     launch_sym_check_sorted(lv.dg(), lv.m, sp_cls.p, sp_crow.p, sp_flags.p, sp_cnt.p, C.s2);
@@ -775,10 +777,8 @@ bool Run::check_symmetric(Level& lv, bool overlap) {
   DBuf<i32> wt;
   C.prof.begin(KID_MISC);
   DBuf<u8> cls;
-  DBuf<i32> crow;
   if (strict) {
     cls = DBuf<u8>(&C.pool, std::max<i64>(lv.n, 1));
-    crow = DBuf<i32>(&C.pool, std::max<i64>(sym_check_chunks(lv.m), 1));
     launch_sym_check_sorted(lv.dg(), lv.m, cls.p, crow.p, flags.p, cnt.p, st());
   } else {
     i64 cap = 1;

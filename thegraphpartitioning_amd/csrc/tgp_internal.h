@@ -209,17 +209,19 @@ void launch_csr_check(const i64* xadj, const i32* adj, i64 n, i64 m, i32* flag, 
 void launch_batch_csr_check(const BatchGraph* dg, const i32* list, i32 cnt, i32* bad,
                             hipStream_t s);
 // flags[3] |= some row is not strictly ascending; flags[5] |= some arc between
-// distinct vertices in range weighs other than 1
-void launch_rows_strict(DGraph g, i32* flags, hipStream_t s);
+// distinct vertices in range weighs other than 1.  crow (sym_check_chunks(m)
+// entries, m = xadj[n]) receives the row of each chunk's first arc, which
+// launch_sym_check_sorted then reads.
+void launch_rows_strict(DGraph g, i64 m, i32* crow, i32* flags, hipStream_t s);
 // the same checks for strictly ascending rows, by binary search of the
 // reverse arc in its (sorted) row: flags[0], flags[2] as above; of each pair
 // the arc into the row of lower (degree bit length, id) searches: cnt[1] =
 // searching arcs whose reverse was found, cnt[0] = the other arcs (symmetric
 // iff flags[0] == 0 and they agree); flags[4] |= a found pair's weight is not
-// 1; cls (n bytes) receives the degree bit lengths, crow
-// (sym_check_chunks(m) entries, m = xadj[n]) the row of each chunk's first arc
+// 1; cls (n bytes) receives the degree bit lengths; crow as launch_rows_strict
+// left it
 i64 sym_check_chunks(i64 m);
-void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, i32* crow, i32* flags, u64* cnt,
+void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, const i32* crow, i32* flags, u64* cnt,
                              hipStream_t s);
 void launch_relabel(const i32* mate, i64 n, i64* flags, i64* pos, i64* total,
                     i32* new_ids, i32* tb, ScanScratch sc, hipStream_t s);

@@ -1510,24 +1510,6 @@ __global__ __launch_bounds__(256) void k_sym_check(DGraph g, const u64* table, c
   if (__ballot(wmis) && lane == 0) atomicOr(&flags[2], 1);
 }
 
-// flags[3] |= some row is not strictly ascending (so it may hold duplicates)
-// flags[3] |= some row is not strictly ascending; flags[5] |= some arc between
-// distinct vertices in range weighs other than 1.
-__global__ __launch_bounds__(256) void k_rows_strict(DGraph g, i32* flags) {
-  bool bad = false, nonunit = false;
-  const int lane = lane_id();
-  for (i64 v = gwave(); v < g.n; v += nwaves()) {
-    const i64 s0 = g.xadj[v], s1 = g.xadj[v + 1];
-    for (i64 i = s0 + lane; i < s1; i += WAVE) {
-      const i32 u = g.adj[i];
-      if (i > s0) bad |= u <= g.adj[i - 1];
-      nonunit |= u >= 0 && u < g.n && u != v && g.w[i] != 1;
-    }
-  }
-  if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
-  if (__ballot(nonunit) && lane == 0) atomicOr(&flags[5], 1);
-}
-
 // The shape of a CSR handed over in device memory: xadj[0] = 0, offsets
 // non-decreasing, xadj[n] = m, every neighbour id in [0, n); *flag = 1 on any
 // violation.  Reads the n + 1 offsets and the m ids only (within the arrays'
@@ -1587,6 +1569,54 @@ __global__ void k_chunk_rows(DGraph g, i64 nch, i32* crow) {
     for (i64 c = (s0 + SYM_CHUNK - 1) / SYM_CHUNK; c <= (s1 - 1) / SYM_CHUNK && c < nch; c++)
       crow[c] = (i32)v;
   }
+}
+
+// flags[3] |= some row is not strictly ascending; flags[5] |= some arc between
+// distinct vertices in range weighs other than 1.  One wave per chunk of
+// SYM_CHUNK arcs (crow[c] = the row of chunk c's first arc), 64 arcs per step:
+// each lane finds its arc's row by binary search over the ends of the 64 rows
+// from the current one, so short rows share a step instead of a wave each.
+__global__ __launch_bounds__(256) void k_rows_strict(DGraph g, const i32* crow, i64 nch, i64 m,
+                                                     i32* flags) {
+  const int lane = lane_id();
+  bool bad = false, nonunit = false;
+  for (i64 c = gwave(); c < nch; c += nwaves()) {
+    i64 b = c * SYM_CHUNK;
+    const i64 bend = b + SYM_CHUNK < m ? b + SYM_CHUNK : m;
+    i64 r = crow[c];
+    i64 rs = g.xadj[r];  // start of row r
+    while (b < bend) {
+      const i64 rl = r + lane;
+      const i64 e = g.xadj[rl < g.n ? rl + 1 : g.n];  // end of row rl
+      // rows of the window that end at or before b hold none of the entries
+      const int done = __popcll(__ballot(e <= b));
+      if (done) {
+        rs = __shfl(e, done - 1);
+        r += done;
+        continue;
+      }
+      const i64 e63 = __shfl(e, WAVE - 1);
+      const i64 lim = b + WAVE < bend ? b + WAVE : bend;
+      const i64 wend = lim < e63 ? lim : e63;
+      const i64 i = b + lane;
+      int rr = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(e, rr + step - 1) <= i) rr += step;
+      // start of the row of i (the shuffle runs on every lane: a lane left
+      // out of it would read 0 where it is the source)
+      const i64 ep = __shfl(e, rr > 0 ? rr - 1 : 0);
+      const i64 start = rr ? ep : rs;
+      b = wend;
+      if (i >= wend) continue;
+      const i64 v = r + rr;
+      const i32 u = g.adj[i];
+      if (i > start) bad |= u <= g.adj[i - 1];
+      if (u >= 0 && u < g.n && u != v) nonunit |= g.w[i] != 1;
+    }
+  }
+  if (__ballot(bad) && lane == 0) atomicOr(&flags[3], 1);
+  if (__ballot(nonunit) && lane == 0) atomicOr(&flags[5], 1);
 }
 
 // Strictly ascending rows.  Of the two arcs of a pair {u, v}, the one whose
@@ -4830,8 +4860,11 @@ void launch_match_sym_round(DGraph g, i32* mate, u8* dec, i32* blkA, i32* blkC, 
   k_match_huge<<<256, 1024, 0, s>>>(g, mate, dec, blkA, blkC, prop, req, cur);
 }
 
-void launch_rows_strict(DGraph g, i32* flags, hipStream_t s) {
-  k_rows_strict<<<grid_for(g.n, 4, 1u << 16), 256, 0, s>>>(g, flags);
+void launch_rows_strict(DGraph g, i64 m, i32* crow, i32* flags, hipStream_t s) {
+  const i64 nch = sym_check_chunks(m);
+  if (nch <= 0) return;
+  k_chunk_rows<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nch, crow);
+  k_rows_strict<<<grid_for(nch, 4, 1u << 16), 256, 0, s>>>(g, crow, nch, m, flags);
 }
 void launch_csr_check(const i64* xadj, const i32* adj, i64 n, i64 m, i32* flag, hipStream_t s) {
   k_csr_check<<<grid_for(std::max(n + 1, m), 256, 8192), 256, 0, s>>>(xadj, adj, n, m, flag);
@@ -4843,12 +4876,11 @@ void launch_batch_csr_check(const BatchGraph* dg, const i32* list, i32 cnt, i32*
 }
 
 i64 sym_check_chunks(i64 m) { return (m + SYM_CHUNK - 1) / SYM_CHUNK; }
-void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, i32* crow, i32* flags, u64* cnt,
+void launch_sym_check_sorted(DGraph g, i64 m, u8* cls, const i32* crow, i32* flags, u64* cnt,
                              hipStream_t s) {
   k_deg_class<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, cls);
   const i64 nch = sym_check_chunks(m);
   if (nch <= 0) return;
-  k_chunk_rows<<<grid_for(g.n, 256, 1u << 16), 256, 0, s>>>(g, nch, crow);
   k_sym_search_sorted<<<grid_for(nch, 4, 1u << 16), 256, 0, s>>>(g, cls, crow, nch, m, flags,
                                                                  cnt);
 }
