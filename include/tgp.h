@@ -201,6 +201,10 @@ typedef struct tgp_stats {
                                       move fit its destination and the network
                                       would have sorted them all, so it could not
                                       change the selection                       */
+  int64_t level0_form;             /* what the symmetry check found on level 0:
+                                      8 = it ran, | 1 rows strictly ascending,
+                                      | 2 symmetric, | 4 every arc between
+                                      distinct vertices weighs 1; 0 = not run  */
 } tgp_stats;
 
 /* Partitions `g` into o->k parts.  Replaces main()'s pipeline

@@ -264,6 +264,33 @@ def test_nearly_symmetric_sorted_rows(oracle, kind, repair):
     assert_same(gp, op, f"nearly symmetric ({kind}) repair={repair}")
 
 
+@pytest.mark.parametrize("overlap", ["1", "0"])
+def test_level0_form_reports_the_checks(monkeypatch, overlap):
+    """tgp_stats.level0_form says what the rows check and the symmetry check
+    found on level 0 (8 = ran, 1 strictly ascending rows, 2 symmetric, 4 unit
+    weights), with the symmetry search beside the matching and before it
+    (TGP_SYM_OVERLAP).  A wrong "not ascending" costs only speed, so the
+    parity tests alone would not notice it."""
+    monkeypatch.setenv("TGP_SYM_OVERLAP", overlap)
+
+    def form(xadj, adj, w):
+        g = tgp.Graph(np.asarray(xadj, np.int64), np.asarray(adj, np.int32),
+                      np.asarray(w, np.int32))
+        return tgp.partition(g, 8, init=tgp.INIT_BLOCK, repair=True, coarsen_floor=40)[3].level0_form
+
+    g = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 3)
+    assert form(g.xadj, g.adjncy, g.adjwgt) == 8 | 1 | 2 | 4
+    gw = tgp.gen_rmat(14, 16, 0x7C0DE5EED + 3, "hash16")
+    assert form(gw.xadj, gw.adjncy, gw.adjwgt) == 8 | 1 | 2
+    # the same rows in descending order: symmetric, not ascending
+    x = np.asarray(g.xadj)
+    rev = np.concatenate([np.asarray(g.adjncy)[x[v]:x[v + 1]][::-1] for v in range(g.n)])
+    assert form(x, rev, np.ones_like(rev)) == 8 | 2
+    assert form(*_drop_or_reweight("weights", 60)) == 8 | 1 | 2
+    assert form(*_drop_or_reweight("extra", 60)) == 8 | 1
+    assert form(*_drop_or_reweight("missing", 60)) == 8 | 1
+
+
 @pytest.mark.parametrize("token,gain", [("0", "1"), ("0", "4"), ("1", "16"), ("4", "8"),
                                         ("8", "2")])
 def test_token_and_gain_lane_widths(oracle, monkeypatch, token, gain):

@@ -9,6 +9,7 @@
   preprocess/conversion.py would: symmetric, no self loops, no isolated
   vertices, ascending rows.
 """
+import ctypes as C
 import hashlib
 import os
 import re
@@ -321,3 +322,31 @@ def test_row_offsets_are_read_as_int64(tmp_path):
     assert g.xadj.tolist() == [0, 1, 4294967297]
     g = tgp.read_graph_dir(str(tmp_path), 2, 2)
     assert g.xadj.tolist() == [0, 1, 4294967297]
+
+
+def test_ctypes_mirrors_match_the_c_structs(tmp_path):
+    """The Python mirrors of include/tgp.h's structs have the C layout: a
+    program built by the host compiler prints each struct's size and every
+    field's offset, which must equal the ctypes ones."""
+    mirrors = {"tgp_graph": tgp.CGraph, "tgp_options": tgp.COptions,
+               "tgp_gpmetis_dir": tgp.CGpmetisDir, "tgp_trace": tgp.CTrace,
+               "tgp_stats": tgp.CStats, "tgp_batch_item": tgp.CBatchItem}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tgp.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            cf = fname.rstrip("_")  # "pass_" mirrors the C field "pass"
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True, timeout=60)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True,
+                         timeout=30).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

@@ -80,7 +80,8 @@ class CStats(C.Structure):
                 ("kernel_bytes", C.c_double * NKERNELS),
                 ("trace", C.POINTER(CTrace)), ("trace_cap", C.c_int64),
                 ("trace_len", C.c_int64), ("fail_level", C.c_int64),
-                ("fail_value", C.c_int64), ("sorts_skipped", C.c_int64)]
+                ("fail_value", C.c_int64), ("sorts_skipped", C.c_int64),
+                ("level0_form", C.c_int64)]
 
 
 _lib = None
@@ -350,6 +351,7 @@ class Stats:
         self.fail_level = cs.fail_level
         self.fail_value = cs.fail_value
         self.sorts_skipped = cs.sorts_skipped
+        self.level0_form = cs.level0_form
         self.kernel_ms = {KERNEL_NAMES[i]: cs.kernel_ms[i] for i in range(len(KERNEL_NAMES))}
         self.kernel_launches = {KERNEL_NAMES[i]: cs.kernel_launches[i]
                                 for i in range(len(KERNEL_NAMES))}

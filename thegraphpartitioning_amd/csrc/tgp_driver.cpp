@@ -321,6 +321,7 @@ struct Level {
   bool simple_ws = false;  // and every row is simple and each reverse has the same weight
   bool sorted = false;     // every row strictly ascending (checked on level 0 only)
   bool unitw = false;      // every arc weight is 1 (checked on level 0 only)
+  bool examined = false;   // the symmetry check ran on this level
   LevelAux aux;
   DBuf<i64> xadj;
   DBuf<i32> adj, w;
@@ -739,6 +740,7 @@ void Run::match_level(const Level& lv, DBuf<i32>& mate) {
 }
 
 bool Run::check_symmetric(Level& lv, bool overlap) {
+  lv.examined = true;
   DBuf<i32> flags(&C.pool, 6);
   DBuf<u64> cnt(&C.pool, 2);
   HIP_TRY(hipMemsetAsync(flags.p, 0, 6 * sizeof(i32), st()));
@@ -1860,6 +1862,10 @@ int Run::go(const tgp_graph* g, i32* part_out, i32* pc_out, i64* cut_out) {
     S->coarsest_npairs = npairs;
     S->match_rounds = match_rounds;
     S->sorts_skipped = sorts_skipped;
+    const Level& l0 = *L[0];
+    S->level0_form = l0.examined ? 8 | (l0.sorted ? 1 : 0) | (l0.symmetric ? 2 : 0) |
+                                       (l0.symmetric && l0.unitw ? 4 : 0)
+                                 : 0;
     S->seconds_initpart = initpart_s;
     S->seconds_total =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
